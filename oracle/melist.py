@@ -1,0 +1,244 @@
+"""Oracle: ME-list storage layout.
+
+TEST INFRASTRUCTURE ONLY (see oracle/strings.py header).
+
+Restated routines (reference file:line):
+  * condensed occupations      operators/condense_occ.f (hpvxblkseq order,
+                               include/hpvxseq.h:4 = X,P,V,H)
+  * Ms/IRREP distributions     operators/next_msgamdist2.f:2-263
+  * string lengths             operators/set_len_str2.f (resort=.false.)
+  * list layout and offsets    operators/set_op_dim2.f:146-459
+  * Ms(A) -> block index       operators/msa2idxms4op.f:24-27
+  * in-block leading dims      contract/set_op_ldim_c.f:237-308
+  * distribution id            operators/msgmdid2.f:2-34
+
+Scope: operators with one vertex (njoined=1), spaces H and P with one orbital
+subspace each, mst=0, gamt=1 (closed-shell, totally symmetric operators),
+no diagonal restriction.  Those are the operators of closed-shell CCSD.
+
+Parity status: pinned by closed-form block lengths (e.g. T2 at nocc=8,
+nvir=32 has 93 312 elements) and by the dense-tensor round trip in
+tests/test_oracle_contract.py.
+"""
+from .strings import multd2h, lenstr
+
+IHOLE, IPART, IVALE, IEXTR = 1, 2, 3, 4
+HPVXSEQ = (4, 2, 3, 1)          # include/hpvxseq.h:2-4
+
+
+def condense(occ_c, occ_a):
+    """occ_c/occ_a: dict hpvx->count.  Returns (occ_csub, hpvx_csub, occ_asub,
+    hpvx_asub) in hpvxblkseq order (operators/condense_occ.f)."""
+    oc, hc, oa, ha = [], [], [], []
+    for h in HPVXSEQ:
+        if occ_c.get(h, 0) > 0:
+            oc.append(occ_c[h])
+            hc.append(h)
+        if occ_a.get(h, 0) > 0:
+            oa.append(occ_a[h])
+            ha.append(h)
+    return oc, hc, oa, ha
+
+
+def first_msdistn(occ, mst):
+    n = len(occ)
+    if n == 0:
+        return (mst == 0), []
+    if n > 1:
+        d = list(occ[:-1]) + [mst - sum(occ[:-1])]
+    else:
+        d = [mst]
+    if abs(d[-1]) <= occ[-1]:
+        return True, d
+    ok = next_msdistn(d, mst, occ)
+    return ok, d
+
+
+def next_msdistn(d, mst, occ):
+    n = len(occ)
+    if n <= 1:
+        return False
+    while True:
+        ok = False
+        for i in range(n - 1):
+            if d[i] > -occ[i]:
+                d[i] -= 2
+                ok = True
+                break
+            else:
+                d[i] = occ[i]
+        if not ok:
+            return False
+        d[n - 1] = mst - sum(d[:n - 1])
+        if abs(d[n - 1]) <= occ[n - 1]:
+            return True
+
+
+def first_gamdistn(gamt, nsym, n):
+    if n == 0:
+        return (gamt == 1), []
+    return True, [1] * (n - 1) + [gamt]
+
+
+def next_gamdistn(g, gamt, nsym):
+    n = len(g)
+    if n <= 1:
+        return False
+    ok = False
+    ilast = 0
+    for i in range(n - 1):
+        if g[i] < nsym:
+            g[i] += 1
+            ilast = i
+            ok = True
+            break
+        else:
+            g[i] = 1
+    if ok:
+        acc = gamt
+        for i in range(ilast, n - 1):
+            acc = multd2h(acc, g[i])
+        g[n - 1] = acc
+    return ok
+
+
+def msgamdists(occ_c, occ_a, ms_c, ms_a, gam_c, gam_a, nsym):
+    """Generator over next_msgamdist2 (operators/next_msgamdist2.f:39-84):
+    innermost C IRREP, then C Ms, then A IRREP, then A Ms."""
+    ok1, mc = first_msdistn(occ_c, ms_c)
+    ok2, ma = first_msdistn(occ_a, ms_a)
+    ok3, gc = first_gamdistn(gam_c, nsym, len(occ_c))
+    ok4, ga = first_gamdistn(gam_a, nsym, len(occ_a))
+    if not (ok1 and ok2 and ok3 and ok4):
+        return
+    while True:
+        yield list(mc), list(ma), list(gc), list(ga)
+        if next_gamdistn(gc, gam_c, nsym):
+            continue
+        if next_msdistn(mc, ms_c, occ_c):
+            _, gc[:] = first_gamdistn(gam_c, nsym, len(occ_c))
+            continue
+        if next_gamdistn(ga, gam_a, nsym):
+            _, mc[:] = first_msdistn(occ_c, ms_c)
+            _, gc[:] = first_gamdistn(gam_c, nsym, len(occ_c))
+            continue
+        if next_msdistn(ma, ms_a, occ_a):
+            _, ga[:] = first_gamdistn(gam_a, nsym, len(occ_a))
+            _, mc[:] = first_msdistn(occ_c, ms_c)
+            _, gc[:] = first_gamdistn(gam_c, nsym, len(occ_c))
+            continue
+        return
+
+
+def idxms_of(occ, ms):
+    return (occ - ms) // 2 + 1
+
+
+def msa2idxms4op(msa, mst, occ_a, occ_c):
+    """operators/msa2idxms4op.f:24-27."""
+    idx = (occ_a - msa) // 2 + 1
+    off = occ_a - occ_c + mst
+    if off > 0:
+        idx -= off // 2
+    return idx
+
+
+def set_op_ldim_c(hpvx_c, hpvx_a, lens, transpose=False):
+    """contract/set_op_ldim_c.f:237-308."""
+    nc, na = len(hpvx_c), len(hpvx_a)
+    ldc = [0] * nc
+    lda = [0] * na
+    ld = 1
+    for h in HPVXSEQ:
+        if not transpose:
+            for i in range(nc):
+                if hpvx_c[i] == h:
+                    ldc[i] = ld
+                    ld *= lens[i]
+            for i in range(na):
+                if hpvx_a[i] == h:
+                    lda[i] = ld
+                    ld *= lens[nc + i]
+        else:
+            for i in range(na - 1, -1, -1):
+                if hpvx_a[i] == h:
+                    lda[i] = ld
+                    ld *= lens[nc + i]
+            for i in range(nc - 1, -1, -1):
+                if hpvx_c[i] == h:
+                    ldc[i] = ld
+                    ld *= lens[i]
+    return ldc, lda
+
+
+class Orbitals:
+    """Orbital spaces of the model: H (occupied) and P (virtual)."""
+
+    def __init__(self, h_irreps, p_irreps, nsym):
+        from .strings import Space
+        self.nsym = nsym
+        self.spaces = {IHOLE: Space(h_irreps, nsym), IPART: Space(p_irreps, nsym)}
+
+    def lenstr(self, hpvx, nel, idxms, igam):
+        return lenstr(self.spaces[hpvx], nel, idxms, igam)
+
+
+class MEList:
+    """ME-list layout of a one-vertex operator (operators/set_op_dim2.f)."""
+
+    def __init__(self, orbs, occ_c, occ_a, absym=1):
+        self.orbs = orbs
+        self.occ_c = dict(occ_c)
+        self.occ_a = dict(occ_a)
+        self.absym = absym
+        self.oc, self.hc, self.oa, self.ha = condense(self.occ_c, self.occ_a)
+        self.nc = sum(self.oc)
+        self.na = sum(self.oa)
+        nsym = orbs.nsym
+        self.blocks = {}        # (idxmsa, igama) -> list of distribution dicts
+        self.gam_ms = {}        # (igama, idxmsa) -> (offset, length)
+        self.dis_index = {}     # (idxmsa, igama, key) -> distribution dict
+        idx = 0
+        idxmsa = 0
+        for msa in range(self.na, -self.na - 1, -2):
+            msc = msa
+            if abs(msc) > self.nc:
+                continue
+            idxmsa += 1
+            for igama in range(1, nsym + 1):
+                igamc = igama
+                start = idx
+                dlist = []
+                for mc, ma, gc, ga in msgamdists(self.oc, self.oa, msc, msa,
+                                                 igamc, igama, nsym):
+                    imc = [idxms_of(o, m) for o, m in zip(self.oc, mc)]
+                    ima = [idxms_of(o, m) for o, m in zip(self.oa, ma)]
+                    lens = ([orbs.lenstr(h, o, i, g) for h, o, i, g in
+                             zip(self.hc, self.oc, imc, gc)] +
+                            [orbs.lenstr(h, o, i, g) for h, o, i, g in
+                             zip(self.ha, self.oa, ima, ga)])
+                    ln = 1
+                    for x in lens:
+                        ln *= x
+                    if ln <= 0:
+                        continue
+                    ldc, lda = set_op_ldim_c(self.hc, self.ha, lens)
+                    d = dict(off=idx, len=ln, msc=mc, msa=ma, gamc=gc, gama=ga,
+                             idxmsc=imc, idxmsa=ima, lens=lens, ldc=ldc, lda=lda,
+                             msa_tot=msa, igama=igama)
+                    dlist.append(d)
+                    key = (tuple(imc), tuple(gc), tuple(ima), tuple(ga))
+                    self.dis_index[(idxmsa, igama, key)] = d
+                    idx += ln
+                self.blocks[(idxmsa, igama)] = dlist
+                self.gam_ms[(igama, idxmsa)] = (start, idx - start)
+        self.length = idx
+
+    def find(self, msa_tot, igama, imc, gc, ima, ga):
+        idxmsa = msa2idxms4op(msa_tot, 0, self.na, self.nc)
+        key = (tuple(imc), tuple(gc), tuple(ima), tuple(ga))
+        return self.dis_index.get((idxmsa, igama, key))
+
+    def block_len(self, msa_tot, igama):
+        idxmsa = msa2idxms4op(msa_tot, 0, self.na, self.nc)
+        return self.gam_ms.get((igama, idxmsa), (0, 0))[1]
