@@ -1,0 +1,41 @@
+// Device-side work description of the grouped block-contraction launch.
+//
+// One DevBlock is one kernel call of GeCCo's block enumerator
+// (contract/contr_op1op2_wmaps_c.f:1892-1949): a GEMM
+//     R(m,n) += xfac * sum_k A(k,m) * B(k,n)
+// whose operands are read through signed index tables and whose result is
+// written through a signed index table (the string maps of
+// contract/collect_block_impr.f and contract/scatter_block.f, resolved once
+// per plan).  Depending on which side of the result is contiguous in the
+// ME-list, A/B are (Op1,Op2) gathers or (Op2,Op1) gathers.
+#pragma once
+#include <stdint.h>
+
+namespace gecco {
+
+// Signed element index: 0 = Pauli-forbidden (element is zero), otherwise
+// sign * (1 + element offset).  Same convention as GeCCo's string maps
+// (strings/set_strmap_kernel.f:100-109).
+struct DevOperand {
+    const int32_t *first;   // separable: per first index; table: full table
+    const int32_t *second;  // separable: per second index; table: nullptr
+    int64_t ld;             // table: extent of the first index
+    int32_t table;          // 1 = table mode
+    int32_t fast_first;     // gathers: 1 if consecutive k are adjacent in memory
+};
+
+struct DevBlock {
+    double xfac;
+    int64_t base_a, base_b, base_c;   // element offsets of the block starts
+    int32_t buf_a, buf_b, buf_c;      // buffer slots
+    int32_t M, N, K;
+    int32_t atomic;                   // result tiles may collide
+    int32_t pad;
+    DevOperand A, B, C;               // A: (k,m)  B: (k,n)  C: (m,n)
+};
+
+struct DevTask {
+    int32_t blk, m0, n0, k0, k1;
+};
+
+}  // namespace gecco

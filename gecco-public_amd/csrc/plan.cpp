@@ -1,0 +1,762 @@
+// Host side of the MI355X block-contraction engine: resolution of GeCCo's
+// string maps into signed index tables, task tiling of the grouped launch,
+// and the C ABI declared in include/gecco_hip.h.
+//
+// The index arithmetic restates what the reference does element by element
+// inside its kernels:
+//   * compound string index -> component strings, C before A, first
+//     component fastest (contract/collect_block_impr.f:120-290,
+//     contract/scatter_block.f:67-141),
+//   * component map lookup  map(str1 + n1*str2), entry sign*(1+index), 0 =
+//     forbidden (contract/collect_block_impr.f:255-264,381-390),
+//   * in-block address 1 + sum idx*ldim with ldim from
+//     contract/set_op_ldim_c.f:237-308,
+//   * map placement per target component from map_info
+//     (contract/set_strmapdim_c2.f:176-235),
+// but does it once per plan instead of once per call, so the kernel reads a
+// ready-made signed offset per row/column (separable case) or per element.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gecco_hip.h"
+#include "contract_types.h"
+
+extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
+                                    int ntasks, double *const *bufs, int grid,
+                                    hipStream_t stream);
+
+namespace gecco {
+
+static thread_local int g_err = 0;
+static thread_local std::string g_msg;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = code;
+    g_msg = buf;
+    return code;
+}
+
+static int ok() { return 0; }
+
+static const int64_t HPVXSEQ[4] = {4, 2, 3, 1};  // include/hpvxseq.h:2-4
+
+static void set_op_ldim_c(std::vector<int64_t> &ldc, std::vector<int64_t> &lda,
+                          const int64_t *hc, const int64_t *ha, const int64_t *nstr,
+                          int64_t nc, int64_t na, bool transpose) {
+    ldc.assign(nc, 0);
+    lda.assign(na, 0);
+    int64_t ld = 1;
+    for (int ih = 0; ih < 4; ++ih) {
+        const int64_t h = HPVXSEQ[ih];
+        if (!transpose) {
+            for (int64_t i = 0; i < nc; ++i)
+                if (hc[i] == h) { ldc[i] = ld; ld *= nstr[i]; }
+            for (int64_t i = 0; i < na; ++i)
+                if (ha[i] == h) { lda[i] = ld; ld *= nstr[nc + i]; }
+        } else {
+            for (int64_t i = na - 1; i >= 0; --i)
+                if (ha[i] == h) { lda[i] = ld; ld *= nstr[nc + i]; }
+            for (int64_t i = nc - 1; i >= 0; --i)
+                if (hc[i] == h) { ldc[i] = ld; ld *= nstr[i]; }
+        }
+    }
+}
+
+static int64_t prod(const int64_t *a, int64_t n) {
+    int64_t p = 1;
+    for (int64_t i = 0; i < n; ++i) p *= a[i];
+    return p;
+}
+
+// A target component of the operand being addressed: its string comes from
+// (at most) one component of each index space.
+struct Target {
+    int side1 = -1, comp1 = -1;   // source 1 (fastest in the map)
+    int side2 = -1, comp2 = -1;   // source 2
+    int64_t n1 = 1, n2 = 1;
+    const int64_t *map = nullptr;
+    int64_t ldim = 0;
+};
+
+struct Space {
+    std::vector<int64_t> len;  // component lengths, first fastest
+    int64_t total() const {
+        int64_t p = 1;
+        for (auto l : len) p *= l;
+        return p;
+    }
+};
+
+// Parse one map_info array (contract/set_strmapdim_c2.f:203-232) for ntgt
+// target components.  src1/src2 give the index space side and the component
+// offset of the source lists inside that space.
+static int add_targets(std::vector<Target> &out, const int64_t *mi, int64_t ntgt,
+                       const int64_t *map, const std::vector<int64_t> &ldim, int side1,
+                       int64_t off1, int64_t nsrc1, int side2, int64_t off2, int64_t nsrc2,
+                       const Space *spaces) {
+    int64_t k = 0, mapoff = 0;
+    for (int64_t t = 0; t < ntgt; ++t) {
+        Target tg;
+        int64_t n = mi[k++];
+        if (n > 1) return fail(GECCO_HIP_EINVAL, "map_info: %lld sources merged into one component (one-vertex operators only)", (long long)n);
+        if (n == 1) {
+            int64_t c = mi[k++];
+            if (c < 1 || c > nsrc1) return fail(GECCO_HIP_EINVAL, "map_info: bad source index");
+            tg.side1 = side1;
+            tg.comp1 = (int)(off1 + c - 1);
+            tg.n1 = spaces[side1].len[tg.comp1];
+        }
+        n = mi[k++];
+        if (n > 1) return fail(GECCO_HIP_EINVAL, "map_info: %lld sources merged into one component (one-vertex operators only)", (long long)n);
+        if (n == 1) {
+            int64_t c = mi[k++];
+            if (c < 1 || c > nsrc2) return fail(GECCO_HIP_EINVAL, "map_info: bad source index");
+            tg.side2 = side2;
+            tg.comp2 = (int)(off2 + c - 1);
+            tg.n2 = spaces[side2].len[tg.comp2];
+        }
+        if (tg.side1 < 0 && tg.side2 < 0) return fail(GECCO_HIP_EINVAL, "map_info: component without source");
+        tg.map = map + mapoff;
+        tg.ldim = ldim[t];
+        mapoff += tg.n1 * tg.n2;
+        out.push_back(tg);
+    }
+    return ok();
+}
+
+struct Resolved {
+    bool table = false;
+    std::vector<int32_t> first, second;  // separable vectors, or table in `first`
+    int64_t F = 0, S = 0;
+    int64_t maxaddr = -1;                // largest element offset touched
+};
+
+// Evaluate the signed offset of (f, s) for a subset of targets.
+// Returns false if the element is forbidden.
+static inline bool eval(const std::vector<Target> &tg, const int64_t *ci[2], int64_t &addr,
+                        int &sgn, int onlyside) {
+    addr = 0;
+    sgn = 1;
+    for (const auto &t : tg) {
+        if (onlyside >= 0) {
+            int side = t.side1 >= 0 ? t.side1 : t.side2;
+            if (side != onlyside) continue;
+        }
+        int64_t a = t.side1 >= 0 ? ci[t.side1][t.comp1] : 0;
+        int64_t b = t.side2 >= 0 ? ci[t.side2][t.comp2] : 0;
+        int64_t m = t.map[a + t.n1 * b];
+        if (m == 0) return false;
+        if (m < 0) { sgn = -sgn; m = -m; }
+        addr += (m - 1) * t.ldim;
+    }
+    return true;
+}
+
+static void decompose(int64_t idx, const Space &sp, int64_t *out) {
+    for (size_t c = 0; c < sp.len.size(); ++c) {
+        out[c] = idx % sp.len[c];
+        idx /= sp.len[c];
+    }
+}
+
+static int resolve(const Space &fs, const Space &ss, const std::vector<Target> &tg,
+                   int64_t oplen, Resolved &r) {
+    // targets refer to sides 0 (first space) and 1 (second space)
+    bool sep = true;
+    for (const auto &t : tg)
+        if (t.side1 >= 0 && t.side2 >= 0 && t.side1 != t.side2) sep = false;
+    r.F = fs.total();
+    r.S = ss.total();
+    r.table = !sep;
+    int64_t cf[16] = {0}, cs[16] = {0};
+    const int64_t *ci[2] = {cf, cs};
+    if (fs.len.size() > 16 || ss.len.size() > 16) return fail(GECCO_HIP_EINVAL, "too many components");
+    if (sep) {
+        r.first.resize(r.F);
+        r.second.resize(r.S);
+        int64_t mf = 0, ms = 0;
+        for (int64_t f = 0; f < r.F; ++f) {
+            decompose(f, fs, cf);
+            int64_t a;
+            int s;
+            if (!eval(tg, ci, a, s, 0)) { r.first[f] = 0; continue; }
+            if (a + 1 > INT32_MAX) return fail(GECCO_HIP_EINVAL, "block too large for 32-bit offsets");
+            r.first[f] = (int32_t)(s * (a + 1));
+            mf = std::max(mf, a);
+        }
+        for (int64_t s2 = 0; s2 < r.S; ++s2) {
+            decompose(s2, ss, cs);
+            int64_t a;
+            int s;
+            if (!eval(tg, ci, a, s, 1)) { r.second[s2] = 0; continue; }
+            if (a + 1 > INT32_MAX) return fail(GECCO_HIP_EINVAL, "block too large for 32-bit offsets");
+            r.second[s2] = (int32_t)(s * (a + 1));
+            ms = std::max(ms, a);
+        }
+        r.maxaddr = mf + ms;
+    } else {
+        if (r.F * r.S > (int64_t)1 << 31) return fail(GECCO_HIP_EINVAL, "index table too large");
+        r.first.resize(r.F * r.S);
+        int64_t mx = -1;
+        for (int64_t s2 = 0; s2 < r.S; ++s2) {
+            decompose(s2, ss, cs);
+            for (int64_t f = 0; f < r.F; ++f) {
+                decompose(f, fs, cf);
+                int64_t a;
+                int s;
+                if (!eval(tg, ci, a, s, -1)) { r.first[f + r.F * s2] = 0; continue; }
+                if (a + 1 > INT32_MAX) return fail(GECCO_HIP_EINVAL, "block too large for 32-bit offsets");
+                r.first[f + r.F * s2] = (int32_t)(s * (a + 1));
+                mx = std::max(mx, a);
+            }
+        }
+        r.maxaddr = mx;
+    }
+    if (r.maxaddr >= oplen)
+        return fail(GECCO_HIP_EINVAL, "string map addresses element %lld of a block of length %lld",
+                    (long long)r.maxaddr, (long long)oplen);
+    return ok();
+}
+
+static inline int64_t absv(int32_t v) { return v < 0 ? -(int64_t)v : (int64_t)v; }
+
+// address of first-index step f=1 relative to f=0 (separable), or of the
+// table entry (1,0) relative to (0,0); used to pick coalescing directions
+static bool first_is_contiguous(const Resolved &r) {
+    if (r.F < 2) return false;
+    if (r.table) {
+        for (int64_t s = 0; s < std::min<int64_t>(r.S, 4); ++s) {
+            int32_t a = r.first[0 + r.F * s], b = r.first[1 + r.F * s];
+            if (a != 0 && b != 0) return absv(b) - absv(a) == 1;
+        }
+        return false;
+    }
+    return r.first[0] != 0 && r.first[1] != 0 && absv(r.first[1]) - absv(r.first[0]) == 1;
+}
+
+static bool result_injective(const Resolved &r, int64_t len) {
+    std::vector<uint8_t> seen(len, 0);
+    if (r.table) {
+        for (auto v : r.first) {
+            if (!v) continue;
+            int64_t a = absv(v) - 1;
+            if (seen[a]) return false;
+            seen[a] = 1;
+        }
+        return true;
+    }
+    for (int64_t s = 0; s < r.S; ++s) {
+        if (!r.second[s]) continue;
+        for (int64_t f = 0; f < r.F; ++f) {
+            if (!r.first[f]) continue;
+            int64_t a = absv(r.first[f]) + absv(r.second[s]) - 2;
+            if (seen[a]) return false;
+            seen[a] = 1;
+        }
+    }
+    return true;
+}
+
+struct HostOperand {
+    Resolved r;
+    int fast_first = 0;
+    int64_t off_first = -1, off_second = -1;  // element offsets in the index pool
+};
+
+struct HostBlock {
+    double xfac;
+    int32_t buf1, buf2, buf12;
+    int64_t off1, off2, off12, len12;
+    int64_t M, N, K;    // kernel orientation
+    bool swap;          // A=Op2, B=Op1 when the result is contiguous along EX1
+    bool injective;
+    HostOperand A, B, C;
+};
+
+}  // namespace gecco
+
+struct gecco_hip_plan {
+    std::vector<gecco::HostBlock> blocks;
+    bool finalized = false;
+    gecco::DevBlock *d_blocks = nullptr;
+    gecco::DevTask *d_tasks = nullptr;
+    int32_t *d_index = nullptr;
+    double **d_bufs = nullptr;
+    int64_t ntasks = 0;
+    int64_t nbufs_max = 0;
+    int grid = 0;
+    double flops = 0, payload = 0, index_bytes = 0;
+    std::vector<double *> last_bufs;
+};
+
+namespace gecco {
+
+static int build_block(const gecco_hip_block_args *a, HostBlock &hb) {
+    const int64_t ncc = a->ncblk_cnt, nca = a->nablk_cnt;
+    const int64_t n1c = a->ncblk_ex1, n1a = a->nablk_ex1;
+    const int64_t n2c = a->ncblk_ex2, n2a = a->nablk_ex2;
+    for (int64_t v : {ncc, nca, n1c, n1a, n2c, n2a, a->ncblk_op1, a->nablk_op1, a->ncblk_op2,
+                      a->nablk_op2, a->ncblk_op1op2, a->nablk_op1op2})
+        if (v < 0 || v > 8) return fail(GECCO_HIP_EINVAL, "component count out of range");
+    Space cnt, ex1, ex2;
+    cnt.len.assign(a->lstr_cnt, a->lstr_cnt + ncc + nca);
+    ex1.len.assign(a->lstr_ex1, a->lstr_ex1 + n1c + n1a);
+    ex2.len.assign(a->lstr_ex2, a->lstr_ex2 + n2c + n2a);
+    for (auto *sp : {&cnt, &ex1, &ex2})
+        for (auto l : sp->len)
+            if (l <= 0) return fail(GECCO_HIP_EINVAL, "zero or negative string length");
+    const int64_t len1 = prod(a->lstrop1, a->ncblk_op1 + a->nablk_op1);
+    const int64_t len2 = prod(a->lstrop2, a->ncblk_op2 + a->nablk_op2);
+    const int64_t len12 = prod(a->lstrop1op2, a->ncblk_op1op2 + a->nablk_op1op2);
+
+    std::vector<int64_t> l1c, l1a, l2c, l2a, l12c, l12a;
+    set_op_ldim_c(l1c, l1a, a->hpvxop1c, a->hpvxop1a, a->lstrop1, a->ncblk_op1, a->nablk_op1,
+                  a->tra_op1 != 0);
+    set_op_ldim_c(l2c, l2a, a->hpvxop2c, a->hpvxop2a, a->lstrop2, a->ncblk_op2, a->nablk_op2,
+                  a->tra_op2 != 0);
+    set_op_ldim_c(l12c, l12a, a->hpvxop1op2c, a->hpvxop1op2a, a->lstrop1op2, a->ncblk_op1op2,
+                  a->nablk_op1op2, a->tra_op1op2 != 0);
+
+    // Op1(CNT, EX1): C <- (CNT C, EX1 C), A <- (CNT A, EX1 A)
+    std::vector<Target> t1, t2, t12;
+    {
+        const Space sp[2] = {cnt, ex1};
+        if (add_targets(t1, a->map_info_1c, a->ncblk_op1, a->map_ex1cntc, l1c, 0, 0, ncc, 1, 0, n1c, sp)) return g_err;
+        if (add_targets(t1, a->map_info_1a, a->nablk_op1, a->map_ex1cnta, l1a, 0, ncc, nca, 1, n1c, n1a, sp)) return g_err;
+    }
+    // Op2(CNT, EX2): C <- (CNT A, EX2 C), A <- (CNT C, EX2 A)   ("CNT^+")
+    {
+        const Space sp[2] = {cnt, ex2};
+        if (add_targets(t2, a->map_info_2c, a->ncblk_op2, a->map_ex2cntc, l2c, 0, ncc, nca, 1, 0, n2c, sp)) return g_err;
+        if (add_targets(t2, a->map_info_2a, a->nablk_op2, a->map_ex2cnta, l2a, 0, 0, ncc, 1, n2c, n2a, sp)) return g_err;
+    }
+    // Res(EX1, EX2): C <- (EX1 C, EX2 C), A <- (EX2 A, EX1 A)
+    {
+        const Space sp[2] = {ex1, ex2};
+        if (add_targets(t12, a->map_info_12c, a->ncblk_op1op2, a->map_ex1ex2c, l12c, 0, 0, n1c, 1, 0, n2c, sp)) return g_err;
+        if (add_targets(t12, a->map_info_12a, a->nablk_op1op2, a->map_ex1ex2a, l12a, 1, n2c, n2a, 0, n1c, n1a, sp)) return g_err;
+    }
+    Resolved r1, r2, r12;
+    if (resolve(cnt, ex1, t1, len1, r1)) return g_err;
+    if (resolve(cnt, ex2, t2, len2, r2)) return g_err;
+    if (resolve(ex1, ex2, t12, len12, r12)) return g_err;
+
+    hb.xfac = a->xfac;
+    hb.len12 = len12;
+    hb.injective = result_injective(r12, len12);
+    hb.swap = first_is_contiguous(r12);
+    hb.K = cnt.total();
+    if (!hb.swap) {
+        hb.M = ex1.total(); hb.N = ex2.total();
+        hb.buf1 = a->buf_op1; hb.off1 = a->off_op1;
+        hb.buf2 = a->buf_op2; hb.off2 = a->off_op2;
+        hb.A.r = std::move(r1); hb.B.r = std::move(r2); hb.C.r = std::move(r12);
+    } else {
+        hb.M = ex2.total(); hb.N = ex1.total();
+        hb.buf1 = a->buf_op2; hb.off1 = a->off_op2;
+        hb.buf2 = a->buf_op1; hb.off2 = a->off_op1;
+        hb.A.r = std::move(r2); hb.B.r = std::move(r1);
+        // result in kernel orientation (first = EX2, second = EX1)
+        Resolved &c = hb.C.r;
+        c = std::move(r12);
+        if (!c.table) {
+            std::swap(c.first, c.second);
+        } else {
+            std::vector<int32_t> tt(c.F * c.S);
+            for (int64_t s = 0; s < c.S; ++s)
+                for (int64_t f = 0; f < c.F; ++f) tt[s + c.S * f] = c.first[f + c.F * s];
+            c.first.swap(tt);
+        }
+        std::swap(c.F, c.S);
+    }
+    hb.buf12 = a->buf_op1op2;
+    hb.off12 = a->off_op1op2;
+    hb.A.fast_first = hb.A.r.table ? 1 : (first_is_contiguous(hb.A.r) ? 1 : 0);
+    hb.B.fast_first = hb.B.r.table ? 1 : (first_is_contiguous(hb.B.r) ? 1 : 0);
+    if (hb.M > INT32_MAX || hb.N > INT32_MAX || hb.K > INT32_MAX)
+        return fail(GECCO_HIP_EINVAL, "block dimension exceeds 32 bits");
+    return ok();
+}
+
+static int check(hipError_t e, const char *what) {
+    if (e != hipSuccess) return fail(GECCO_HIP_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+    return ok();
+}
+
+static void plan_free_device(gecco_hip_plan *p) {
+    if (p->d_blocks) (void)hipFree(p->d_blocks);
+    if (p->d_tasks) (void)hipFree(p->d_tasks);
+    if (p->d_index) (void)hipFree(p->d_index);
+    if (p->d_bufs) (void)hipFree(p->d_bufs);
+    p->d_blocks = nullptr;
+    p->d_tasks = nullptr;
+    p->d_index = nullptr;
+    p->d_bufs = nullptr;
+}
+
+static int plan_finalize(gecco_hip_plan *p) {
+    if (p->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    // index pool
+    std::vector<int32_t> pool;
+    auto put = [&](const std::vector<int32_t> &v) {
+        int64_t off = (int64_t)pool.size();
+        pool.insert(pool.end(), v.begin(), v.end());
+        while (pool.size() % 64) pool.push_back(0);
+        return off;
+    };
+    int64_t nb_max = 0;
+    double flops = 0, payload = 0;
+    // range overlap detection among result blocks of the plan
+    std::vector<std::pair<std::pair<int64_t, int64_t>, size_t>> ranges;
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
+            o->off_first = put(o->r.first);
+            o->off_second = o->r.table ? -1 : put(o->r.second);
+        }
+        nb_max = std::max<int64_t>(nb_max, std::max(b.buf1, std::max(b.buf2, b.buf12)) + 1);
+        flops += 2.0 * (double)b.M * (double)b.N * (double)b.K;
+        payload += 8.0 * ((double)b.A.r.maxaddr + 1 + (double)b.B.r.maxaddr + 1 + 2.0 * (double)b.len12);
+        ranges.push_back({{(int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12,
+                           (int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12 + b.len12}, i});
+    }
+    std::sort(ranges.begin(), ranges.end());
+    std::vector<uint8_t> overlap(p->blocks.size(), 0);
+    int64_t maxend = -1;
+    size_t maxend_i = 0;
+    for (size_t r = 0; r < ranges.size(); ++r) {
+        if (r > 0 && ranges[r].first.first < maxend) {
+            overlap[ranges[r].second] = 1;
+            overlap[maxend_i] = 1;
+        }
+        if (ranges[r].first.second > maxend) {
+            maxend = ranges[r].first.second;
+            maxend_i = ranges[r].second;
+        }
+    }
+    // tiles and split-K
+    const int BMh = 64, BNh = 64;
+    int64_t ntiles = 0;
+    for (auto &b : p->blocks) ntiles += ((b.M + BMh - 1) / BMh) * ((b.N + BNh - 1) / BNh);
+    const int64_t want = 4096;
+    int64_t split = ntiles > 0 ? (want + ntiles - 1) / ntiles : 1;
+    std::vector<DevTask> tasks;
+    std::vector<DevBlock> dblocks(p->blocks.size());
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / 256));
+        int64_t chunk = (b.K + ns - 1) / ns;
+        chunk = ((chunk + 15) / 16) * 16;
+        ns = (b.K + chunk - 1) / chunk;
+        DevBlock &d = dblocks[i];
+        d.xfac = b.xfac;
+        d.base_a = b.off1;
+        d.base_b = b.off2;
+        d.base_c = b.off12;
+        d.buf_a = b.buf1;
+        d.buf_b = b.buf2;
+        d.buf_c = b.buf12;
+        d.M = (int32_t)b.M;
+        d.N = (int32_t)b.N;
+        d.K = (int32_t)b.K;
+        d.atomic = (ns > 1 || !b.injective || overlap[i]) ? 1 : 0;
+        d.pad = 0;
+        for (int64_t m0 = 0; m0 < b.M; m0 += BMh)
+            for (int64_t n0 = 0; n0 < b.N; n0 += BNh)
+                for (int64_t s = 0; s < ns; ++s) {
+                    int64_t k0 = s * chunk, k1 = std::min(b.K, k0 + chunk);
+                    tasks.push_back({(int32_t)i, (int32_t)m0, (int32_t)n0, (int32_t)k0, (int32_t)k1});
+                }
+    }
+    // largest tasks first (by K extent) for load balance
+    std::stable_sort(tasks.begin(), tasks.end(), [](const DevTask &x, const DevTask &y) {
+        return (x.k1 - x.k0) > (y.k1 - y.k0);
+    });
+    if (pool.empty()) pool.push_back(0);
+    int32_t *d_index = nullptr;
+    if (check(hipMalloc(&d_index, pool.size() * sizeof(int32_t)), "hipMalloc(index)")) return g_err;
+    if (check(hipMemcpy(d_index, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy(index)")) { (void)hipFree(d_index); return g_err; }
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        DevBlock &d = dblocks[i];
+        HostOperand *hs[3] = {&b.A, &b.B, &b.C};
+        DevOperand *ds[3] = {&d.A, &d.B, &d.C};
+        for (int q = 0; q < 3; ++q) {
+            ds[q]->first = d_index + hs[q]->off_first;
+            ds[q]->second = hs[q]->r.table ? nullptr : d_index + hs[q]->off_second;
+            ds[q]->ld = hs[q]->r.F;
+            ds[q]->table = hs[q]->r.table ? 1 : 0;
+            ds[q]->fast_first = hs[q]->fast_first;
+        }
+    }
+    p->d_index = d_index;
+    if (check(hipMalloc(&p->d_blocks, std::max<size_t>(1, dblocks.size()) * sizeof(DevBlock)), "hipMalloc(blocks)")) return g_err;
+    if (!dblocks.empty() && check(hipMemcpy(p->d_blocks, dblocks.data(), dblocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice), "hipMemcpy(blocks)")) return g_err;
+    if (check(hipMalloc(&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask)), "hipMalloc(tasks)")) return g_err;
+    if (!tasks.empty() && check(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice), "hipMemcpy(tasks)")) return g_err;
+    p->nbufs_max = nb_max;
+    if (check(hipMalloc(&p->d_bufs, std::max<int64_t>(1, nb_max) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
+    p->ntasks = (int64_t)tasks.size();
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)ncu * 4);
+    p->flops = flops;
+    p->payload = payload;
+    p->index_bytes = 4.0 * (double)pool.size();
+    p->finalized = true;
+    // host copies of the index tables are no longer needed
+    for (auto &b : p->blocks)
+        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
+            std::vector<int32_t>().swap(o->r.first);
+            std::vector<int32_t>().swap(o->r.second);
+        }
+    return ok();
+}
+
+static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, hipStream_t s) {
+    if (!p->finalized) return fail(GECCO_HIP_ESTATE, "plan not finalized");
+    if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
+    for (int64_t i = 0; i < p->nbufs_max; ++i)
+        if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
+    if (p->ntasks == 0) return ok();
+    std::vector<double *> want(bufs, bufs + p->nbufs_max);
+    if (want != p->last_bufs) {
+        // slot table changes only when the caller passes different buffers;
+        // the copy is ordered on the same stream before the launch
+        if (check(hipMemcpyAsync(p->d_bufs, want.data(), want.size() * sizeof(double *), hipMemcpyHostToDevice, s), "hipMemcpyAsync(bufs)")) return g_err;
+        if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
+        p->last_bufs = want;
+    }
+    if (gecco_launch_grouped(p->d_blocks, p->d_tasks, (int)p->ntasks, p->d_bufs, p->grid, s))
+        return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return ok();
+}
+
+// ------------------------------------------------------------ staging
+static std::mutex g_stage_mu;
+struct Stage {
+    double *p = nullptr;
+    size_t n = 0;
+};
+static Stage g_stage[3];
+
+static int stage_get(int i, size_t n, double **out) {
+    if (g_stage[i].n < n) {
+        if (g_stage[i].p) (void)hipFree(g_stage[i].p);
+        g_stage[i].p = nullptr;
+        g_stage[i].n = 0;
+        if (check(hipMalloc(&g_stage[i].p, std::max<size_t>(n, 1) * sizeof(double)), "hipMalloc(stage)")) return g_err;
+        g_stage[i].n = std::max<size_t>(n, 1);
+    }
+    *out = g_stage[i].p;
+    return ok();
+}
+
+static bool is_device_ptr(const void *p) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+// One kernel call through the Fortran signature: stage host operands,
+// run the one-block plan, copy the result back.
+static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const double *xop1,
+                       const double *xop2) {
+    gecco_hip_block_args a = a0;
+    a.buf_op1 = 0;
+    a.buf_op2 = 1;
+    a.buf_op1op2 = 2;
+    a.off_op1 = a.off_op2 = a.off_op1op2 = 0;
+    const int64_t len1 = prod(a.lstrop1, a.ncblk_op1 + a.nablk_op1);
+    const int64_t len2 = prod(a.lstrop2, a.ncblk_op2 + a.nablk_op2);
+    const int64_t len12 = prod(a.lstrop1op2, a.ncblk_op1op2 + a.nablk_op1op2);
+    gecco_hip_plan plan;
+    plan.blocks.emplace_back();
+    if (build_block(&a, plan.blocks.back())) return g_err;
+    if (plan_finalize(&plan)) { plan_free_device(&plan); return g_err; }
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    double *d1 = const_cast<double *>(xop1), *d2 = const_cast<double *>(xop2), *d12 = xop1op2;
+    const bool h1 = !is_device_ptr(xop1), h2 = !is_device_ptr(xop2), h12 = !is_device_ptr(xop1op2);
+    int rc = 0;
+    if (h1 && !rc) rc = stage_get(0, len1, &d1) || check(hipMemcpy(d1, xop1, len1 * 8, hipMemcpyHostToDevice), "stage op1");
+    if (h2 && !rc) rc = stage_get(1, len2, &d2) || check(hipMemcpy(d2, xop2, len2 * 8, hipMemcpyHostToDevice), "stage op2");
+    if (h12 && !rc) rc = stage_get(2, len12, &d12) || check(hipMemcpy(d12, xop1op2, len12 * 8, hipMemcpyHostToDevice), "stage res");
+    double *bufs[3] = {d1, d2, d12};
+    if (!rc) rc = plan_execute(&plan, bufs, 3, 0);
+    if (!rc) rc = check(hipDeviceSynchronize(), "kernel");
+    if (!rc && h12) rc = check(hipMemcpy(xop1op2, d12, len12 * 8, hipMemcpyDeviceToHost), "unstage res");
+    plan_free_device(&plan);
+    return rc ? g_err : ok();
+}
+
+}  // namespace gecco
+
+using namespace gecco;
+
+extern "C" {
+
+int gecco_hip_last_error(char *msg, int64_t msglen) {
+    if (msg && msglen > 0) {
+        snprintf(msg, (size_t)msglen, "%s", g_msg.c_str());
+    }
+    return g_err;
+}
+
+#define FILL_ARGS(A)                                                                  \
+    A.xfac = *xfac;                                                                   \
+    A.tra_op1 = *tra_op1; A.tra_op2 = *tra_op2; A.tra_op1op2 = *tra_op1op2;            \
+    A.ncblk_op1 = *ncblk_op1; A.nablk_op1 = *nablk_op1;                               \
+    A.ncblk_ex1 = *ncblk_ex1; A.nablk_ex1 = *nablk_ex1;                               \
+    A.ncblk_op2 = *ncblk_op2; A.nablk_op2 = *nablk_op2;                               \
+    A.ncblk_ex2 = *ncblk_ex2; A.nablk_ex2 = *nablk_ex2;                               \
+    A.ncblk_cnt = *ncblk_cnt; A.nablk_cnt = *nablk_cnt;                               \
+    A.ncblk_op1op2 = *ncblk_op1op2; A.nablk_op1op2 = *nablk_op1op2;                   \
+    A.hpvxop1c = hpvxop1c; A.hpvxop1a = hpvxop1a; A.hpvxop2c = hpvxop2c;              \
+    A.hpvxop2a = hpvxop2a; A.hpvxop1op2c = hpvxop1op2c; A.hpvxop1op2a = hpvxop1op2a;  \
+    A.lstrop1 = lstrop1; A.lstrop2 = lstrop2; A.lstrop1op2 = lstrop1op2;              \
+    A.lstr_ex1 = lstr_ex1; A.lstr_ex2 = lstr_ex2; A.lstr_cnt = lstr_cnt;              \
+    A.map_info_12c = map_info_12c; A.map_info_12a = map_info_12a;                     \
+    A.map_info_1c = map_info_1c; A.map_info_1a = map_info_1a;                         \
+    A.map_info_2c = map_info_2c; A.map_info_2a = map_info_2a;                         \
+    A.map_ex1ex2c = map_ex1ex2c; A.map_ex1ex2a = map_ex1ex2a;                         \
+    A.map_ex1cntc = map_ex1cntc; A.map_ex1cnta = map_ex1cnta;                         \
+    A.map_ex2cntc = map_ex2cntc; A.map_ex2cnta = map_ex2cnta;
+
+void contr_blk1blk2_blocked_mm_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2, double *xscr,
+    const int64_t *lenscr, const int64_t *lenblock, const int64_t *lenblock_cnt,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1, const int64_t *ncblk_ex1,
+    const int64_t *nablk_ex1, const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2, const int64_t *ncblk_cnt,
+    const int64_t *nablk_cnt, const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a, const int64_t *hpvxop2c,
+    const int64_t *hpvxop2a, const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a, const int64_t *map_info_1c,
+    const int64_t *map_info_1a, const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a, const int64_t *map_ex1cntc,
+    const int64_t *map_ex1cnta, const int64_t *map_ex2cntc, const int64_t *map_ex2cnta) {
+    (void)xscr; (void)lenscr; (void)lenblock; (void)lenblock_cnt;
+    g_err = 0;
+    g_msg.clear();
+    gecco_hip_block_args a;
+    std::memset(&a, 0, sizeof a);
+    FILL_ARGS(a)
+    single_call(a, xop1op2, xop1, xop2);
+}
+
+void contr_blk1blk2_wmaps_c_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1, const int64_t *ncblk_ex1,
+    const int64_t *nablk_ex1, const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2, const int64_t *ncblk_cnt,
+    const int64_t *nablk_cnt, const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a, const int64_t *hpvxop2c,
+    const int64_t *hpvxop2a, const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a, const int64_t *map_info_1c,
+    const int64_t *map_info_1a, const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a, const int64_t *map_ex1cntc,
+    const int64_t *map_ex1cnta, const int64_t *map_ex2cntc, const int64_t *map_ex2cnta) {
+    g_err = 0;
+    g_msg.clear();
+    gecco_hip_block_args a;
+    std::memset(&a, 0, sizeof a);
+    FILL_ARGS(a)
+    single_call(a, xop1op2, xop1, xop2);
+}
+
+int gecco_hip_plan_create(gecco_hip_plan **plan) {
+    g_err = 0;
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan pointer");
+    *plan = new gecco_hip_plan();
+    return 0;
+}
+
+int gecco_hip_plan_destroy(gecco_hip_plan *plan) {
+    if (!plan) return 0;
+    plan_free_device(plan);
+    delete plan;
+    return 0;
+}
+
+int gecco_hip_plan_add_block(gecco_hip_plan *plan, const gecco_hip_block_args *args) {
+    g_err = 0;
+    if (!plan || !args) return fail(GECCO_HIP_EINVAL, "null argument");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (args->buf_op1 < 0 || args->buf_op2 < 0 || args->buf_op1op2 < 0 ||
+        args->off_op1 < 0 || args->off_op2 < 0 || args->off_op1op2 < 0)
+        return fail(GECCO_HIP_EINVAL, "negative buffer slot or offset");
+    HostBlock hb;
+    if (build_block(args, hb)) return g_err;
+    plan->blocks.push_back(std::move(hb));
+    return 0;
+}
+
+int gecco_hip_plan_finalize(gecco_hip_plan *plan) {
+    g_err = 0;
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    int rc = plan_finalize(plan);
+    if (rc) plan_free_device(plan);
+    return rc;
+}
+
+int gecco_hip_plan_execute(gecco_hip_plan *plan, double *const *bufs, int64_t nbufs,
+                           void *stream) {
+    g_err = 0;
+    if (!plan || (!bufs && nbufs)) return fail(GECCO_HIP_EINVAL, "null argument");
+    return plan_execute(plan, bufs, nbufs, (hipStream_t)stream);
+}
+
+int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops, int64_t *nblocks,
+                         int64_t *ntasks, double *payload_bytes, double *index_bytes) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    double f = 0;
+    for (auto &b : plan->blocks) f += 2.0 * (double)b.M * (double)b.N * (double)b.K;
+    if (flops) *flops = f;
+    if (nblocks) *nblocks = (int64_t)plan->blocks.size();
+    if (ntasks) *ntasks = plan->ntasks;
+    if (payload_bytes) *payload_bytes = plan->payload;
+    if (index_bytes) *index_bytes = plan->index_bytes;
+    return 0;
+}
+
+int gecco_hip_malloc(void **ptr, int64_t bytes) {
+    g_err = 0;
+    if (!ptr || bytes < 0) return fail(GECCO_HIP_EINVAL, "bad argument");
+    if (check(hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1)), "hipMalloc")) return g_err;
+    return 0;
+}
+int gecco_hip_free(void *ptr) { return ptr ? check(hipFree(ptr), "hipFree") : 0; }
+int gecco_hip_memcpy_htod(void *dst, const void *src, int64_t bytes) {
+    return check(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+int gecco_hip_memcpy_dtoh(void *dst, const void *src, int64_t bytes) {
+    return check(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+int gecco_hip_synchronize(void) { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+}  // extern "C"

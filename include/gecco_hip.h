@@ -1,0 +1,150 @@
+/*
+ * gecco_hip.h -- C ABI of the MI355X (gfx950) block-contraction engine.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in kernel entry points with GeCCo's own Fortran argument lists
+ *     (all arguments by reference, default INTEGER/LOGICAL = 8 bytes, symbol
+ *     = lower case + trailing underscore).  A GeCCo build links this library
+ *     instead of compiling the Fortran kernel; the block enumerator
+ *     contr_op1op2_wmaps_c calls them unchanged.  Operand buffers may be host
+ *     memory (they are staged through device memory) or device memory.
+ *
+ *  2. A plan API for the grouped launch: every kernel call of a formula step
+ *     (or of many independent problems) is recorded once with
+ *     gecco_hip_plan_add_block (the same arguments, operands given as buffer
+ *     slot + element offset), resolved into device index tables by
+ *     gecco_hip_plan_finalize, and then executed any number of times as one
+ *     persistent launch by gecco_hip_plan_execute.
+ *
+ * Errors: the Fortran-named entry points cannot return a status (the
+ * reference's subroutines stop through quit(), baserout/quit.F:36-50); they
+ * record it, and gecco_hip_last_error() returns it (0 = success).  All other
+ * functions return 0 on success and a GECCO_HIP_E* code otherwise.
+ */
+#ifndef GECCO_HIP_H
+#define GECCO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GECCO_HIP_OK 0
+#define GECCO_HIP_EINVAL 1    /* inconsistent dimensions, maps or arguments */
+#define GECCO_HIP_ENOMEM 2    /* device allocation failed                    */
+#define GECCO_HIP_EDEVICE 3   /* HIP runtime error / no device               */
+#define GECCO_HIP_ESTATE 4    /* plan used in the wrong state                */
+
+/* Last error of the calling thread (0 if none) and its message. */
+int gecco_hip_last_error(char *msg, int64_t msglen);
+
+/* Route 3 kernel: R(EX1,EX2) += xfac * sum_CNT Op1(CNT,EX1) Op2(CNT^+,EX2)
+ * with signed gathers and signed scatter through the string maps.
+ * Replaces contract/contr_blk1blk2_blocked_mm.f:2-20 (called from
+ * contract/contr_op1op2_wmaps_c.f:1927).  xscr/lenscr/lenblock/lenblock_cnt
+ * are accepted for signature compatibility; the GPU kernel needs no host
+ * scratch and its tiling does not depend on them. */
+void contr_blk1blk2_blocked_mm_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2,
+    double *xscr, const int64_t *lenscr, const int64_t *lenblock,
+    const int64_t *lenblock_cnt,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1,
+    const int64_t *ncblk_ex1, const int64_t *nablk_ex1,
+    const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2,
+    const int64_t *ncblk_cnt, const int64_t *nablk_cnt,
+    const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a,
+    const int64_t *hpvxop2c, const int64_t *hpvxop2a,
+    const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a,
+    const int64_t *map_info_1c, const int64_t *map_info_1a,
+    const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a,
+    const int64_t *map_ex1cntc, const int64_t *map_ex1cnta,
+    const int64_t *map_ex2cntc, const int64_t *map_ex2cnta);
+
+/* Route 2 kernel (map-driven scalar contraction in the reference), same
+ * result as route 3.  Replaces contract/contr_blk1blk2_wmaps_c.f:2-19
+ * (called from contract/contr_op1op2_wmaps_c.f:1899).  On the GPU both
+ * routes run the same fused gather/MFMA/scatter kernel. */
+void contr_blk1blk2_wmaps_c_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1,
+    const int64_t *ncblk_ex1, const int64_t *nablk_ex1,
+    const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2,
+    const int64_t *ncblk_cnt, const int64_t *nablk_cnt,
+    const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a,
+    const int64_t *hpvxop2c, const int64_t *hpvxop2a,
+    const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a,
+    const int64_t *map_info_1c, const int64_t *map_info_1a,
+    const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a,
+    const int64_t *map_ex1cntc, const int64_t *map_ex1cnta,
+    const int64_t *map_ex2cntc, const int64_t *map_ex2cnta);
+
+/* ------------------------------------------------------------ plan API */
+typedef struct gecco_hip_plan gecco_hip_plan;
+
+/* Description of one kernel call with the arguments of
+ * contract/contr_blk1blk2_blocked_mm.f:2-20 (pointers to int64 arrays, held
+ * by the caller only for the duration of gecco_hip_plan_add_block). */
+typedef struct gecco_hip_block_args {
+    double xfac;
+    int64_t tra_op1, tra_op2, tra_op1op2;
+    int64_t ncblk_op1, nablk_op1, ncblk_ex1, nablk_ex1;
+    int64_t ncblk_op2, nablk_op2, ncblk_ex2, nablk_ex2;
+    int64_t ncblk_cnt, nablk_cnt, ncblk_op1op2, nablk_op1op2;
+    const int64_t *hpvxop1c, *hpvxop1a, *hpvxop2c, *hpvxop2a;
+    const int64_t *hpvxop1op2c, *hpvxop1op2a;
+    const int64_t *lstrop1, *lstrop2, *lstrop1op2;
+    const int64_t *lstr_ex1, *lstr_ex2, *lstr_cnt;
+    const int64_t *map_info_12c, *map_info_12a, *map_info_1c, *map_info_1a;
+    const int64_t *map_info_2c, *map_info_2a;
+    const int64_t *map_ex1ex2c, *map_ex1ex2a, *map_ex1cntc, *map_ex1cnta;
+    const int64_t *map_ex2cntc, *map_ex2cnta;
+    /* operand placement: buffer slot (index into the array given to
+     * gecco_hip_plan_execute) and element offset of the block start */
+    int32_t buf_op1, buf_op2, buf_op1op2;
+    int64_t off_op1, off_op2, off_op1op2;
+} gecco_hip_block_args;
+
+int gecco_hip_plan_create(gecco_hip_plan **plan);
+int gecco_hip_plan_destroy(gecco_hip_plan *plan);
+/* Record one kernel call.  No device work happens here. */
+int gecco_hip_plan_add_block(gecco_hip_plan *plan, const gecco_hip_block_args *args);
+/* Resolve every block's signed gathers/scatter into device index tables,
+ * choose tiles and the task order, upload.  Must precede execute. */
+int gecco_hip_plan_finalize(gecco_hip_plan *plan);
+/* Run all blocks as one grouped launch on `stream` (hipStream_t, may be 0).
+ * bufs[i] is the device base address of slot i.  Results accumulate. */
+int gecco_hip_plan_execute(gecco_hip_plan *plan, double *const *bufs,
+                           int64_t nbufs, void *stream);
+/* Algorithmic FLOP (sum of 2*n_ex1*n_ex2*n_cnt), blocks, tasks, bytes of
+ * operand payload touched once per execute, and device index bytes. */
+int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops,
+                         int64_t *nblocks, int64_t *ntasks,
+                         double *payload_bytes, double *index_bytes);
+
+/* Device memory helpers for callers without another allocator. */
+int gecco_hip_malloc(void **ptr, int64_t bytes);
+int gecco_hip_free(void *ptr);
+int gecco_hip_memcpy_htod(void *dst, const void *src, int64_t bytes);
+int gecco_hip_memcpy_dtoh(void *dst, const void *src, int64_t bytes);
+int gecco_hip_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GECCO_HIP_H */

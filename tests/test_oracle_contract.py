@@ -1,0 +1,104 @@
+"""Oracle pins for the binary contraction (CPU): dense antisymmetrised
+einsum, route 2 == route 3, blocking invariance, spin-flip on/off."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle.contract import to_dense, from_dense, _perm_sign
+from models import orbitals, spin_symmetric, CASES, ladder, ring, t1t1
+
+
+def antisym(Y, groups):
+    out = np.zeros_like(Y)
+    for combo in itertools.product(*[list(itertools.permutations(g)) for g in groups]):
+        perm = list(range(Y.ndim))
+        s = 1
+        for g, p in zip(groups, combo):
+            s *= _perm_sign([g.index(q) for q in p])
+            for a, b in zip(g, p):
+                perm[a] = b
+        out += s * Y.transpose(perm)
+    return out
+
+
+def dense_ref(name, D1, D2):
+    if name == 'ladder':
+        return 0.5 * np.einsum('abcd,cdij->abij', D1, D2)
+    if name == 'ring':
+        return antisym(np.einsum('caki,bkcj->abji', D1, D2), [[0, 1], [2, 3]])
+    if name == 't1t1':
+        return antisym(np.einsum('ai,bj->abji', D1, D2), [[0, 1], [2, 3]])
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 4), (2, 3, 4), (4, 4, 5)])
+@pytest.mark.parametrize('name', ['ladder', 'ring', 't1t1'])
+@pytest.mark.parametrize('use_tr', [False, True])
+@pytest.mark.parametrize('route', [3, 2])
+def test_matches_dense_einsum(nsym, nocc, nvir, name, use_tr, route):
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = CASES[name](orbs)
+    x1 = spin_symmetric(a, 11)
+    x2 = spin_symmetric(b, 12)
+    ref = from_dense(r, dense_ref(name, to_dense(a, x1), to_dense(b, x2)))
+    xr = np.zeros(r.length)
+    c.run(x1, x2, xr, route=route, use_tr=use_tr)
+    assert np.abs(xr - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_dense_check_catches_a_sign_mistake():
+    """Flipping one map sign in the oracle's input must break the pin."""
+    orbs = orbitals(3, 4, 2)
+    c, a, b, r = ring(orbs)
+    x1 = spin_symmetric(a, 11)
+    x2 = spin_symmetric(b, 12)
+    ref = from_dense(r, dense_ref('ring', to_dense(a, x1), to_dense(b, x2)))
+    blocks = c.blocks(False)
+    for bl in blocks:
+        m = bl['map_ex1cntc']
+        nz = [i for i, v in enumerate(m) if v]
+        if nz:
+            m[nz[0]] = -m[nz[0]]
+    xr = np.zeros(r.length)
+    c.run(x1, x2, xr, use_tr=False, blocks=blocks)
+    assert np.abs(xr - ref).max() > 1e-6
+
+
+@pytest.mark.parametrize('name', list(CASES))
+def test_blocking_invariance(name):
+    """test/batch-*.inp style: str_block / cnt_block must not change results."""
+    orbs = orbitals(4, 6, 2)
+    c, a, b, r = CASES[name](orbs)
+    x1 = spin_symmetric(a, 3)
+    x2 = spin_symmetric(b, 4)
+    outs = []
+    for lb, lc in [(200, -1), (5, 3), (2, 1), (7, -1)]:
+        xr = np.zeros(r.length)
+        c.run(x1, x2, xr, lenblock=lb, lenblock_cnt=lc)
+        outs.append(xr)
+    for o in outs[1:]:
+        assert np.abs(o - outs[0]).max() <= 1e-13 * max(1.0, np.abs(outs[0]).max())
+
+
+@pytest.mark.parametrize('name', list(CASES))
+def test_route2_equals_route3_and_use_tr(name):
+    orbs = orbitals(4, 5, 4)
+    c, a, b, r = CASES[name](orbs)
+    x1 = spin_symmetric(a, 5)
+    x2 = spin_symmetric(b, 6)
+    outs = []
+    for route, ut in [(3, True), (2, True), (3, False), (2, False)]:
+        xr = np.zeros(r.length)
+        c.run(x1, x2, xr, route=route, use_tr=ut)
+        outs.append(xr)
+    for o in outs[1:]:
+        assert np.abs(o - outs[0]).max() <= 1e-13 * max(1.0, np.abs(outs[0]).max())
+
+
+def test_ladder_flop_count():
+    """nocc=8, nvir=32, C1 ladder: 2*(496*28*496) + 2*(1024*64*1024) with
+    the spin-flip skip (contract/contr_op1op2_wmaps_c.f:1007-1019)."""
+    c, *_ = ladder(orbitals(8, 32, 1))
+    assert c.flops(use_tr=True) == 147994624
+    assert c.flops(use_tr=False) == 2 * (2 * 496 * 28 * 496 + 1024 * 64 * 1024)
