@@ -1,0 +1,962 @@
+// Plan builder: orbital strings, string maps, ME-list layout and the block
+// enumerator of a binary contraction, producing the kernel calls that GeCCo's
+// contr_op1op2_wmaps_c would make (contract/contr_op1op2_wmaps_c.f:964-2070),
+// recorded into a gecco_hip_plan for one grouped launch.
+//
+// Formulation (own, checked bit-exactly against the reference restatement
+// in the tests):
+//  * a string is its sorted list of spin-orbitals so = 2*(orb-1)+spin
+//    (spin 0 = alpha, 1 = beta), i.e. ascending orbitals, alpha before beta
+//    (strings/iordstr2.f:68-92);
+//  * strings of one (Ms, IRREP) block are ordered by their occupation codes
+//    (0 empty < 1 alpha < 2 beta < 3 doubly) compared from the highest
+//    orbital down -- the order of the 4-slope graph index
+//    (strings/weight_4sg.f, strings/idx4sg_sb.f: arcs 0,+1,-1,+2 weighted in
+//    that order);
+//  * the merge sign of (str1, str2) is the parity of sorting str1 ++ str2
+//    (strings/iordstr2.f:61-105), 0 if a spin-orbital repeats;
+//  * the spin-flip sign is the product of the alpha-before-beta reordering
+//    parities of the string and of its flipped string
+//    (strings/set_flipmap_kernel.f, std_spsign);
+//  * the list layout is occupation -> Ms(A) descending -> IRREP(A) ->
+//    Ms/IRREP distributions in next_msgamdist2 order -> dense C x A block
+//    (operators/set_op_dim2.f:146-459, operators/next_msgamdist2.f).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gecco_hip.h"
+#include "../../include/gecco_hip_model.h"
+
+namespace gecco {
+int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *args);
+int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &&i1,
+                       std::vector<int64_t> &&i2, std::vector<int8_t> &&f, double prefac);
+int set_error(int code, const char *msg);
+
+static inline int mult(int a, int b) { return ((a - 1) ^ (b - 1)) + 1; }
+static const int HPVXBLKSEQ[4] = {4, 2, 3, 1};  // include/hpvxseq.h:4
+
+struct StrKey {
+    uint64_t w[2] = {0, 0};
+    bool operator==(const StrKey &o) const { return w[0] == o.w[0] && w[1] == o.w[1]; }
+};
+struct StrKeyHash {
+    size_t operator()(const StrKey &k) const {
+        return std::hash<uint64_t>()(k.w[0] * 0x9E3779B97F4A7C15ull ^ (k.w[1] + 0x632BE59BD9B4E019ull));
+    }
+};
+
+static StrKey make_key(const uint16_t *so, int n) {
+    StrKey k;
+    for (int i = 0; i < n; ++i) k.w[i >> 2] |= (uint64_t)(so[i] + 1) << (16 * (i & 3));
+    return k;
+}
+
+static void unpack(const StrKey &k, int n, uint16_t *so) {
+    for (int i = 0; i < n; ++i) so[i] = (uint16_t)(((k.w[i >> 2] >> (16 * (i & 3))) & 0xFFFF) - 1);
+}
+
+struct GraphBlock {
+    std::vector<StrKey> strs;
+    std::unordered_map<StrKey, int32_t, StrKeyHash> index;
+};
+
+struct Graph {
+    int space = 0, nel = 0;
+    std::vector<GraphBlock> blk;  // [(idxms-1)*nsym + (igam-1)]
+};
+
+// compare two strings of equal length by occupation codes, highest orbital first
+static bool colex_less(const uint16_t *a, const uint16_t *b, int n) {
+    // walk from the end, orbital by orbital
+    int i = n - 1, j = n - 1;
+    while (i >= 0 && j >= 0) {
+        int oa = a[i] >> 1, ob = b[j] >> 1;
+        if (oa != ob) return oa < ob;  // the string with the higher orbital is larger
+        int ca = 0, cb = 0;
+        int ii = i, jj = j;
+        while (ii >= 0 && (a[ii] >> 1) == oa) { ca |= 1 << (a[ii] & 1); --ii; }
+        while (jj >= 0 && (b[jj] >> 1) == ob) { cb |= 1 << (b[jj] & 1); --jj; }
+        // code: alpha only =1, beta only =2, both =3  (bitmask 1,2,3 coincide)
+        if (ca != cb) return ca < cb;
+        i = ii;
+        j = jj;
+    }
+    return j >= 0 && i < 0;  // the string with lower-lying leftovers is larger
+}
+
+}  // namespace gecco
+
+struct gecco_hip_orbs {
+    int nsym = 1;
+    std::vector<int> irr[5];  // hpvx 1..4: sorted IRREP per orbital
+    std::map<std::pair<int, int>, std::unique_ptr<gecco::Graph>> graphs;
+    // cached string maps, key = (space, nel1, idxms1, g1, nel2, idxms2, g2)
+    std::map<std::array<int, 7>, std::vector<int64_t>> maps;
+    std::map<std::array<int, 4>, std::vector<int64_t>> flips;
+};
+
+namespace gecco {
+
+static Graph &graph(gecco_hip_orbs *o, int space, int nel) {
+    auto key = std::make_pair(space, nel);
+    auto it = o->graphs.find(key);
+    if (it != o->graphs.end()) return *it->second;
+    auto g = std::make_unique<Graph>();
+    g->space = space;
+    g->nel = nel;
+    const int nsym = o->nsym;
+    g->blk.resize((size_t)(nel + 1) * nsym);
+    const std::vector<int> &irr = o->irr[space];
+    const int norb = (int)irr.size();
+    std::vector<std::vector<uint16_t>> tmp((size_t)(nel + 1) * nsym);
+    // alpha set of size na, beta set of size nb (combinations, ascending)
+    for (int na = 0; na <= nel; ++na) {
+        const int nb = nel - na;
+        if (na > norb || nb > norb) continue;
+        std::vector<int> A(na), B(nb);
+        for (int i = 0; i < na; ++i) A[i] = i;
+        for (;;) {
+            int ga = 1;
+            for (int x : A) ga = mult(ga, irr[x]);
+            for (int i = 0; i < nb; ++i) B[i] = i;
+            for (;;) {
+                int gg = ga;
+                for (int x : B) gg = mult(gg, irr[x]);
+                const int ms = na - nb;
+                const int idxms = (nel - ms) / 2 + 1;
+                std::vector<uint16_t> &dst = tmp[(size_t)(idxms - 1) * nsym + gg - 1];
+                // merge to sorted spin-orbital list
+                int ia = 0, ib = 0;
+                while (ia < na || ib < nb) {
+                    if (ib >= nb || (ia < na && A[ia] <= B[ib])) dst.push_back((uint16_t)(2 * A[ia++]));
+                    else dst.push_back((uint16_t)(2 * B[ib++] + 1));
+                }
+                int k = nb - 1;
+                while (k >= 0 && B[k] == norb - nb + k) --k;
+                if (k < 0) break;
+                ++B[k];
+                for (int i = k + 1; i < nb; ++i) B[i] = B[i - 1] + 1;
+            }
+            int k = na - 1;
+            while (k >= 0 && A[k] == norb - na + k) --k;
+            if (k < 0) break;
+            ++A[k];
+            for (int i = k + 1; i < na; ++i) A[i] = A[i - 1] + 1;
+        }
+    }
+    for (size_t b = 0; b < tmp.size(); ++b) {
+        const size_t ns = nel ? tmp[b].size() / nel : 0;
+        std::vector<uint32_t> order(ns);
+        for (size_t i = 0; i < ns; ++i) order[i] = (uint32_t)i;
+        const uint16_t *base = tmp[b].data();
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return colex_less(base + (size_t)x * nel, base + (size_t)y * nel, nel);
+        });
+        GraphBlock &gb = g->blk[b];
+        gb.strs.reserve(ns);
+        for (size_t i = 0; i < ns; ++i) {
+            StrKey k = make_key(base + (size_t)order[i] * nel, nel);
+            gb.index.emplace(k, (int32_t)i);
+            gb.strs.push_back(k);
+        }
+    }
+    if (nel == 0) {  // the empty string: Ms 0, IRREP 1
+        g->blk[0].strs.push_back(StrKey());
+        g->blk[0].index.emplace(StrKey(), 0);
+    }
+    Graph &ref = *g;
+    o->graphs.emplace(key, std::move(g));
+    return ref;
+}
+
+static int64_t lenstr(gecco_hip_orbs *o, int space, int nel, int idxms, int igam) {
+    if (nel < 0 || idxms < 1 || idxms > nel + 1 || igam < 1 || igam > o->nsym) return 0;
+    return (int64_t)graph(o, space, nel).blk[(size_t)(idxms - 1) * o->nsym + igam - 1].strs.size();
+}
+
+// (str1, str2) -> str12 map, str1 fastest (strings/set_strmap_kernel.f:55-118)
+static const std::vector<int64_t> &strmap(gecco_hip_orbs *o, int space, int n1, int im1, int g1,
+                                          int n2, int im2, int g2) {
+    std::array<int, 7> key{space, n1, im1, g1, n2, im2, g2};
+    auto it = o->maps.find(key);
+    if (it != o->maps.end()) return it->second;
+    const int nsym = o->nsym;
+    const Graph &G1 = graph(o, space, n1), &G2 = graph(o, space, n2);
+    const Graph &G12 = graph(o, space, n1 + n2);
+    const GraphBlock &B1 = G1.blk[(size_t)(im1 - 1) * nsym + g1 - 1];
+    const GraphBlock &B2 = G2.blk[(size_t)(im2 - 1) * nsym + g2 - 1];
+    const int ms12 = (n1 - 2 * (im1 - 1)) + (n2 - 2 * (im2 - 1));
+    const int im12 = (n1 + n2 - ms12) / 2 + 1;
+    const GraphBlock &B12 = G12.blk[(size_t)(im12 - 1) * nsym + mult(g1, g2) - 1];
+    std::vector<int64_t> m;
+    m.reserve(B1.strs.size() * B2.strs.size());
+    uint16_t s1[8], s2[8], s12[16];
+    for (const StrKey &k2 : B2.strs) {
+        unpack(k2, n2, s2);
+        for (const StrKey &k1 : B1.strs) {
+            unpack(k1, n1, s1);
+            int i = 0, j = 0, t = 0, inv = 0;
+            bool clash = false;
+            while (i < n1 || j < n2) {
+                if (j >= n2 || (i < n1 && s1[i] < s2[j])) s12[t++] = s1[i++];
+                else {
+                    if (i < n1 && s1[i] == s2[j]) { clash = true; break; }
+                    inv += n1 - i;
+                    s12[t++] = s2[j++];
+                }
+            }
+            if (clash) { m.push_back(0); continue; }
+            auto f = B12.index.find(make_key(s12, n1 + n2));
+            int64_t idx = f == B12.index.end() ? -1 : f->second;
+            if (idx < 0) { m.push_back(0); continue; }
+            m.push_back((inv & 1) ? -(idx + 1) : (idx + 1));
+        }
+    }
+    return o->maps.emplace(key, std::move(m)).first->second;
+}
+
+static int spsign(const uint16_t *s, int n) {
+    int nb = 0, trp = 0;
+    for (int i = 0; i < n; ++i) {
+        if (s[i] & 1) ++nb;
+        else trp += nb;
+    }
+    return (trp & 1) ? -1 : 1;
+}
+
+// string -> spin-flipped string, signed (strings/set_flipmap_kernel.f)
+static const std::vector<int64_t> &flipmap(gecco_hip_orbs *o, int space, int nel, int im, int g) {
+    std::array<int, 4> key{space, nel, im, g};
+    auto it = o->flips.find(key);
+    if (it != o->flips.end()) return it->second;
+    const int nsym = o->nsym;
+    const Graph &G = graph(o, space, nel);
+    const GraphBlock &B = G.blk[(size_t)(im - 1) * nsym + g - 1];
+    const GraphBlock &BF = G.blk[(size_t)(nel - (im - 1)) * nsym + g - 1];
+    std::vector<int64_t> m;
+    uint16_t s[8], f[8];
+    for (const StrKey &k : B.strs) {
+        unpack(k, nel, s);
+        for (int i = 0; i < nel; ++i) f[i] = s[i] ^ 1;
+        // restore alpha-before-beta on doubly occupied orbitals
+        for (int i = 0; i + 1 < nel; ++i)
+            if ((f[i] >> 1) == (f[i + 1] >> 1) && f[i] > f[i + 1]) std::swap(f[i], f[i + 1]);
+        const int sg = spsign(s, nel) * spsign(f, nel);
+        auto fi = BF.index.find(make_key(f, nel));
+        m.push_back(sg * (int64_t)(fi->second + 1));
+    }
+    return o->flips.emplace(key, std::move(m)).first->second;
+}
+
+// ---------------------------------------------------------- distributions
+// operators/next_msgamdist2.f: all (Ms, IRREP) distributions of a condensed
+// C/A occupation, innermost C IRREP, then C Ms, then A IRREP, then A Ms.
+struct DistSet {
+    std::vector<std::vector<int>> ms;   // per entry
+    std::vector<std::vector<int>> gam;
+};
+
+static void ms_dists(const std::vector<int> &occ, int mst, std::vector<std::vector<int>> &out) {
+    const int n = (int)occ.size();
+    out.clear();
+    if (n == 0) {
+        if (mst == 0) out.push_back({});
+        return;
+    }
+    // first index fastest, each from occ(i) downwards; last fixed by mst
+    std::vector<int> d(n);
+    for (int i = 0; i < n - 1; ++i) d[i] = occ[i];
+    for (;;) {
+        int s = 0;
+        for (int i = 0; i < n - 1; ++i) s += d[i];
+        d[n - 1] = mst - s;
+        if (std::abs(d[n - 1]) <= occ[n - 1]) out.push_back(d);
+        int i = 0;
+        for (; i < n - 1; ++i) {
+            if (d[i] > -occ[i]) { d[i] -= 2; break; }
+            d[i] = occ[i];
+        }
+        if (i == n - 1) break;
+    }
+}
+
+static void gam_dists(int n, int gamt, int nsym, std::vector<std::vector<int>> &out) {
+    out.clear();
+    if (n == 0) {
+        if (gamt == 1) out.push_back({});
+        return;
+    }
+    std::vector<int> g(n, 1);
+    for (;;) {
+        int acc = gamt;
+        for (int i = 0; i < n - 1; ++i) acc = mult(acc, g[i]);
+        g[n - 1] = acc;
+        out.push_back(g);
+        int i = 0;
+        for (; i < n - 1; ++i) {
+            if (g[i] < nsym) { ++g[i]; break; }
+            g[i] = 1;
+        }
+        if (i == n - 1) break;
+    }
+}
+
+struct Dist {
+    int64_t off = 0, len = 0;
+    std::vector<int> msc, msa, gc, ga, imc, ima;
+    std::vector<int64_t> lens, ldc, lda;
+};
+
+}  // namespace gecco
+
+struct gecco_hip_list {
+    gecco_hip_orbs *orbs = nullptr;
+    int occ_c[5] = {0}, occ_a[5] = {0};
+    int absym = 1;
+    std::vector<int> oc, hc, oa, ha;  // condensed (hpvxblkseq order)
+    int nc = 0, na = 0;
+    std::vector<gecco::Dist> dists;
+    std::map<std::vector<int>, int> find;  // (idxmsa, igama, imc, gc, ima, ga) -> dist
+    std::map<std::pair<int, int>, int64_t> blen;  // (idxmsa, igama) -> length
+    int64_t length = 0;
+};
+
+namespace gecco {
+
+static int idxms_of(int occ, int ms) { return (occ - ms) / 2 + 1; }
+
+static int msa2idxms4op(int msa, int occ_a, int occ_c) {  // operators/msa2idxms4op.f:24-27
+    int idx = (occ_a - msa) / 2 + 1;
+    int off = occ_a - occ_c;
+    if (off > 0) idx -= off / 2;
+    return idx;
+}
+
+static void set_op_ldim(const std::vector<int> &hc, const std::vector<int> &ha,
+                        const std::vector<int64_t> &lens, std::vector<int64_t> &ldc,
+                        std::vector<int64_t> &lda) {
+    const size_t nc = hc.size(), na = ha.size();
+    ldc.assign(nc, 0);
+    lda.assign(na, 0);
+    int64_t ld = 1;
+    for (int h : HPVXBLKSEQ) {
+        for (size_t i = 0; i < nc; ++i)
+            if (hc[i] == h) { ldc[i] = ld; ld *= lens[i]; }
+        for (size_t i = 0; i < na; ++i)
+            if (ha[i] == h) { lda[i] = ld; ld *= lens[nc + i]; }
+    }
+}
+
+static std::vector<int> find_key(int idxmsa, int igama, const std::vector<int> &imc,
+                                 const std::vector<int> &gc, const std::vector<int> &ima,
+                                 const std::vector<int> &ga) {
+    std::vector<int> k{idxmsa, igama};
+    k.insert(k.end(), imc.begin(), imc.end());
+    k.insert(k.end(), gc.begin(), gc.end());
+    k.insert(k.end(), ima.begin(), ima.end());
+    k.insert(k.end(), ga.begin(), ga.end());
+    return k;
+}
+
+static void condense(const int *occ_c, const int *occ_a, std::vector<int> &oc, std::vector<int> &hc,
+                     std::vector<int> &oa, std::vector<int> &ha) {
+    for (int h : HPVXBLKSEQ) {
+        if (occ_c[h] > 0) { oc.push_back(occ_c[h]); hc.push_back(h); }
+        if (occ_a[h] > 0) { oa.push_back(occ_a[h]); ha.push_back(h); }
+    }
+}
+
+static void build_list(gecco_hip_list *L) {
+    gecco_hip_orbs *o = L->orbs;
+    const int nsym = o->nsym;
+    condense(L->occ_c, L->occ_a, L->oc, L->hc, L->oa, L->ha);
+    for (int x : L->oc) L->nc += x;
+    for (int x : L->oa) L->na += x;
+    int64_t idx = 0;
+    int idxmsa = 0;
+    std::vector<std::vector<int>> mcs, mas, gcs, gas;
+    for (int msa = L->na; msa >= -L->na; msa -= 2) {
+        const int msc = msa;
+        if (std::abs(msc) > L->nc) continue;
+        ++idxmsa;
+        for (int ga_t = 1; ga_t <= nsym; ++ga_t) {
+            const int gc_t = ga_t;
+            const int64_t start = idx;
+            ms_dists(L->oa, msa, mas);
+            gam_dists((int)L->oa.size(), ga_t, nsym, gas);
+            ms_dists(L->oc, msc, mcs);
+            gam_dists((int)L->oc.size(), gc_t, nsym, gcs);
+            for (auto &ma : mas)
+                for (auto &ga : gas)
+                    for (auto &mc : mcs)
+                        for (auto &gc : gcs) {
+                            Dist d;
+                            d.msc = mc; d.msa = ma; d.gc = gc; d.ga = ga;
+                            int64_t len = 1;
+                            for (size_t i = 0; i < L->oc.size(); ++i) {
+                                d.imc.push_back(idxms_of(L->oc[i], mc[i]));
+                                d.lens.push_back(lenstr(o, L->hc[i], L->oc[i], d.imc.back(), gc[i]));
+                                len *= d.lens.back();
+                            }
+                            for (size_t i = 0; i < L->oa.size(); ++i) {
+                                d.ima.push_back(idxms_of(L->oa[i], ma[i]));
+                                d.lens.push_back(lenstr(o, L->ha[i], L->oa[i], d.ima.back(), ga[i]));
+                                len *= d.lens.back();
+                            }
+                            if (len <= 0) continue;
+                            set_op_ldim(L->hc, L->ha, d.lens, d.ldc, d.lda);
+                            d.off = idx;
+                            d.len = len;
+                            L->find[find_key(idxmsa, ga_t, d.imc, d.gc, d.ima, d.ga)] = (int)L->dists.size();
+                            L->dists.push_back(std::move(d));
+                            idx += len;
+                        }
+            L->blen[{idxmsa, ga_t}] = idx - start;
+        }
+    }
+    L->length = idx;
+}
+
+static const Dist *list_find(const gecco_hip_list *L, int msa_tot, int igama,
+                             const std::vector<int> &imc, const std::vector<int> &gc,
+                             const std::vector<int> &ima, const std::vector<int> &ga) {
+    int idxmsa = msa2idxms4op(msa_tot, L->na, L->nc);
+    auto it = L->find.find(find_key(idxmsa, igama, imc, gc, ima, ga));
+    return it == L->find.end() ? nullptr : &L->dists[it->second];
+}
+
+static int64_t block_len(const gecco_hip_list *L, int msa_tot, int igama) {
+    int idxmsa = msa2idxms4op(msa_tot, L->na, L->nc);
+    auto it = L->blen.find({idxmsa, igama});
+    return it == L->blen.end() ? 0 : it->second;
+}
+
+// --------------------------------------------------------- the enumerator
+struct Side {  // condensed occupation of one string part (EX1 C, CNT A, ...)
+    std::vector<int> occ, hpvx;
+};
+
+static Side side_of(const int *occ) {
+    Side s;
+    for (int h : HPVXBLKSEQ)
+        if (occ[h] > 0) { s.occ.push_back(occ[h]); s.hpvx.push_back(h); }
+    return s;
+}
+
+static std::vector<int64_t> map_info(const std::vector<int> &tgt, const Side &s1, const Side &s2) {
+    std::vector<int64_t> mi;
+    for (int h : tgt) {
+        std::vector<int64_t> i1, i2;
+        for (size_t i = 0; i < s1.hpvx.size(); ++i)
+            if (s1.hpvx[i] == h) i1.push_back((int64_t)i + 1);
+        for (size_t i = 0; i < s2.hpvx.size(); ++i)
+            if (s2.hpvx[i] == h) i2.push_back((int64_t)i + 1);
+        mi.push_back((int64_t)i1.size());
+        mi.insert(mi.end(), i1.begin(), i1.end());
+        mi.push_back((int64_t)i2.size());
+        mi.insert(mi.end(), i2.begin(), i2.end());
+    }
+    return mi;
+}
+
+struct DistStr {  // one distribution of a string part
+    std::vector<int> ms, gam, idxms;
+    std::vector<int64_t> len;
+};
+
+static void merge_dis(const std::vector<int64_t> &mi, size_t ntgt, const DistStr &d1,
+                      const DistStr &d2, std::vector<int> &ms, std::vector<int> &gam) {
+    ms.assign(ntgt, 0);
+    gam.assign(ntgt, 1);
+    size_t k = 0;
+    for (size_t t = 0; t < ntgt; ++t) {
+        int64_t n = mi[k++];
+        for (int64_t i = 0; i < n; ++i, ++k) {
+            ms[t] += d1.ms[mi[k] - 1];
+            gam[t] = mult(gam[t], d1.gam[mi[k] - 1]);
+        }
+        n = mi[k++];
+        for (int64_t i = 0; i < n; ++i, ++k) {
+            ms[t] += d2.ms[mi[k] - 1];
+            gam[t] = mult(gam[t], d2.gam[mi[k] - 1]);
+        }
+    }
+}
+
+// strings/get_strmap_blk_c.f:68-200 for one-vertex operators
+static void strmap_blk(gecco_hip_orbs *o, const std::vector<int> &tgt_h, const std::vector<int64_t> &mi,
+                       const Side &s1, const DistStr &d1, const Side &s2, const DistStr &d2,
+                       std::vector<int64_t> &out) {
+    out.clear();
+    size_t k = 0;
+    for (size_t t = 0; t < tgt_h.size(); ++t) {
+        int64_t n1 = mi[k++];
+        int64_t i1 = n1 ? mi[k] - 1 : -1;
+        k += n1;
+        int64_t n2 = mi[k++];
+        int64_t i2 = n2 ? mi[k] - 1 : -1;
+        k += n2;
+        if (n1 && n2) {
+            const auto &m = strmap(o, tgt_h[t], s1.occ[i1], d1.idxms[i1], d1.gam[i1], s2.occ[i2],
+                                   d2.idxms[i2], d2.gam[i2]);
+            out.insert(out.end(), m.begin(), m.end());
+        } else {
+            int64_t n = n1 ? d1.len[i1] : d2.len[i2];
+            for (int64_t i = 1; i <= n; ++i) out.push_back(i);
+        }
+    }
+}
+
+static void dist_strings(gecco_hip_orbs *o, const Side &c, const Side &a, int msc, int msa, int gc,
+                         int ga, std::vector<DistStr> &out) {
+    out.clear();
+    const int nsym = o->nsym;
+    std::vector<std::vector<int>> mcs, mas, gcs, gas;
+    ms_dists(a.occ, msa, mas);
+    gam_dists((int)a.occ.size(), ga, nsym, gas);
+    ms_dists(c.occ, msc, mcs);
+    gam_dists((int)c.occ.size(), gc, nsym, gcs);
+    for (auto &ma : mas)
+        for (auto &g_a : gas)
+            for (auto &mc : mcs)
+                for (auto &g_c : gcs) {
+                    DistStr d;
+                    bool zero = false;
+                    d.ms = mc;
+                    d.ms.insert(d.ms.end(), ma.begin(), ma.end());
+                    d.gam = g_c;
+                    d.gam.insert(d.gam.end(), g_a.begin(), g_a.end());
+                    for (size_t i = 0; i < c.occ.size(); ++i) {
+                        d.idxms.push_back(idxms_of(c.occ[i], mc[i]));
+                        d.len.push_back(lenstr(o, c.hpvx[i], c.occ[i], d.idxms.back(), g_c[i]));
+                        zero |= d.len.back() == 0;
+                    }
+                    for (size_t i = 0; i < a.occ.size(); ++i) {
+                        d.idxms.push_back(idxms_of(a.occ[i], ma[i]));
+                        d.len.push_back(lenstr(o, a.hpvx[i], a.occ[i], d.idxms.back(), g_a[i]));
+                        zero |= d.len.back() == 0;
+                    }
+                    if (!zero) out.push_back(std::move(d));
+                }
+}
+
+static DistStr part(const DistStr &d, size_t from, size_t n) {
+    DistStr r;
+    r.ms.assign(d.ms.begin() + from, d.ms.begin() + from + n);
+    r.gam.assign(d.gam.begin() + from, d.gam.begin() + from + n);
+    r.idxms.assign(d.idxms.begin() + from, d.idxms.begin() + from + n);
+    r.len.assign(d.len.begin() + from, d.len.begin() + from + n);
+    return r;
+}
+
+struct ContrCtx {
+    gecco_hip_plan *plan;
+    gecco_hip_orbs *o;
+    const gecco_hip_list *L1, *L2, *L12;
+    Side ex1c, ex1a, ex2c, ex2a, cntc, cnta;
+    std::vector<int64_t> mi12c, mi12a, mi1c, mi1a, mi2c, mi2a;
+    double fact;
+    int32_t buf1, buf2, buf12;
+    int64_t off1, off2, off12;
+    int64_t nblocks = 0;
+    double flops = 0;
+};
+
+static int emit(ContrCtx &c, double xfac, const Dist &d1, const Dist &d2, const Dist &d12,
+                const DistStr &e1, const DistStr &e2, const DistStr &cn) {
+    std::vector<int64_t> m12c, m12a, m1c, m1a, m2c, m2a;
+    const size_t n1c = c.ex1c.occ.size(), n2c = c.ex2c.occ.size(), ncc = c.cntc.occ.size();
+    const DistStr e1c = part(e1, 0, n1c), e1a = part(e1, n1c, c.ex1a.occ.size());
+    const DistStr e2c = part(e2, 0, n2c), e2a = part(e2, n2c, c.ex2a.occ.size());
+    const DistStr cc = part(cn, 0, ncc), ca = part(cn, ncc, c.cnta.occ.size());
+    strmap_blk(c.o, c.L12->hc, c.mi12c, c.ex1c, e1c, c.ex2c, e2c, m12c);
+    strmap_blk(c.o, c.L12->ha, c.mi12a, c.ex2a, e2a, c.ex1a, e1a, m12a);
+    strmap_blk(c.o, c.L1->hc, c.mi1c, c.cntc, cc, c.ex1c, e1c, m1c);
+    strmap_blk(c.o, c.L1->ha, c.mi1a, c.cnta, ca, c.ex1a, e1a, m1a);
+    strmap_blk(c.o, c.L2->hc, c.mi2c, c.cnta, ca, c.ex2c, e2c, m2c);
+    strmap_blk(c.o, c.L2->ha, c.mi2a, c.cntc, cc, c.ex2a, e2a, m2a);
+    auto v64 = [](const std::vector<int> &v) { return std::vector<int64_t>(v.begin(), v.end()); };
+    std::vector<int64_t> h1c = v64(c.L1->hc), h1a = v64(c.L1->ha), h2c = v64(c.L2->hc),
+                         h2a = v64(c.L2->ha), h12c = v64(c.L12->hc), h12a = v64(c.L12->ha);
+    std::vector<int64_t> zero{0};
+    auto P = [&](const std::vector<int64_t> &v) { return v.empty() ? zero.data() : v.data(); };
+    gecco_hip_block_args a;
+    std::memset(&a, 0, sizeof a);
+    a.xfac = xfac;
+    a.ncblk_op1 = (int64_t)c.L1->hc.size(); a.nablk_op1 = (int64_t)c.L1->ha.size();
+    a.ncblk_ex1 = (int64_t)n1c; a.nablk_ex1 = (int64_t)c.ex1a.occ.size();
+    a.ncblk_op2 = (int64_t)c.L2->hc.size(); a.nablk_op2 = (int64_t)c.L2->ha.size();
+    a.ncblk_ex2 = (int64_t)n2c; a.nablk_ex2 = (int64_t)c.ex2a.occ.size();
+    a.ncblk_cnt = (int64_t)ncc; a.nablk_cnt = (int64_t)c.cnta.occ.size();
+    a.ncblk_op1op2 = (int64_t)c.L12->hc.size(); a.nablk_op1op2 = (int64_t)c.L12->ha.size();
+    a.hpvxop1c = P(h1c); a.hpvxop1a = P(h1a); a.hpvxop2c = P(h2c); a.hpvxop2a = P(h2a);
+    a.hpvxop1op2c = P(h12c); a.hpvxop1op2a = P(h12a);
+    a.lstrop1 = P(d1.lens); a.lstrop2 = P(d2.lens); a.lstrop1op2 = P(d12.lens);
+    a.lstr_ex1 = P(e1.len); a.lstr_ex2 = P(e2.len); a.lstr_cnt = P(cn.len);
+    a.map_info_12c = P(c.mi12c); a.map_info_12a = P(c.mi12a);
+    a.map_info_1c = P(c.mi1c); a.map_info_1a = P(c.mi1a);
+    a.map_info_2c = P(c.mi2c); a.map_info_2a = P(c.mi2a);
+    a.map_ex1ex2c = P(m12c); a.map_ex1ex2a = P(m12a);
+    a.map_ex1cntc = P(m1c); a.map_ex1cnta = P(m1a);
+    a.map_ex2cntc = P(m2c); a.map_ex2cnta = P(m2a);
+    a.buf_op1 = c.buf1; a.buf_op2 = c.buf2; a.buf_op1op2 = c.buf12;
+    a.off_op1 = c.off1 + d1.off; a.off_op2 = c.off2 + d2.off; a.off_op1op2 = c.off12 + d12.off;
+    int rc = plan_add_block_internal(c.plan, &a);
+    if (rc) return rc;
+    ++c.nblocks;
+    double n = 2.0;
+    for (auto x : e1.len) n *= (double)x;
+    for (auto x : e2.len) n *= (double)x;
+    for (auto x : cn.len) n *= (double)x;
+    c.flops += n;
+    return 0;
+}
+
+static int enumerate(ContrCtx &c, bool use_tr) {
+    const gecco_hip_list *L1 = c.L1, *L2 = c.L2, *L12 = c.L12;
+    gecco_hip_orbs *o = c.o;
+    const int nsym = o->nsym;
+    const int na[3] = {L1->na, L2->na, L12->na}, nc[3] = {L1->nc, L2->nc, L12->nc};
+    int nc_cnt = 0, na_cnt = 0, nc_ex1 = 0, na_ex1 = 0, nc_ex2 = 0, na_ex2 = 0;
+    for (int x : c.cntc.occ) nc_cnt += x;
+    for (int x : c.cnta.occ) na_cnt += x;
+    for (int x : c.ex1c.occ) nc_ex1 += x;
+    for (int x : c.ex1a.occ) na_ex1 += x;
+    for (int x : c.ex2c.occ) nc_ex2 += x;
+    for (int x : c.ex2a.occ) na_ex2 += x;
+    const bool tr = use_tr && L1->absym != 0 && L2->absym != 0 && L12->absym != 0 &&
+                    L1->absym * L2->absym == L12->absym;
+    const int idx_restr = na[1] > na[0] ? 2 : 1;
+    std::vector<DistStr> D1, D2, DC;
+    std::vector<int> msr, gr, msr_a, gr_a, imr, ima, m1c, g1c, m1a, g1a, m2c, g2c, m2a, g2a;
+    for (int ms3 = na[2]; ms3 >= -na[2]; ms3 -= 2)
+        for (int ms2 = na[1]; ms2 >= -na[1]; ms2 -= 2)
+            for (int ms1 = na[0]; ms1 >= -na[0]; ms1 -= 2) {
+                const int msa[3] = {ms1, ms2, ms3};
+                if (std::abs(ms1) > nc[0] || std::abs(ms2) > nc[1] || std::abs(ms3) > nc[2]) continue;
+                const int msc_ac = ms1 + ms2 - ms3;
+                if (na_cnt + nc_cnt < std::abs(msc_ac)) continue;
+                const int mr = msa[idx_restr - 1];
+                if (tr && (ms3 < 0 || (ms3 == 0 && mr < 0))) continue;
+                double fac0 = 1.0;
+                if (tr && ms3 > 0) fac0 *= 2.0;
+                if (tr && ms3 == 0 && mr > 0) fac0 *= 2.0;
+                for (int msc_a = na_cnt; msc_a >= -na_cnt; msc_a -= 2) {
+                    const int msc_c = msc_ac - msc_a;
+                    if (std::abs(msc_c) > nc_cnt) continue;
+                    const int mex1c = ms1 - msc_c, mex1a = ms1 - msc_a;
+                    const int mex2a = ms2 - msc_c, mex2c = ms2 - msc_a;
+                    if (std::abs(mex1c) > nc_ex1 || std::abs(mex1a) > na_ex1 ||
+                        std::abs(mex2c) > nc_ex2 || std::abs(mex2a) > na_ex2) continue;
+                    if (tr && mr == 0 && ms3 == 0 && msc_c - msc_a > 0) continue;
+                    double fac = fac0;
+                    if (tr && mr == 0 && ms3 == 0 && msc_c - msc_a < 0) fac *= 2.0;
+                    for (int g3 = 1; g3 <= nsym; ++g3)
+                        for (int g2 = 1; g2 <= nsym; ++g2)
+                            for (int g1 = 1; g1 <= nsym; ++g1) {
+                                if (!block_len(L1, ms1, g1) || !block_len(L2, ms2, g2) ||
+                                    !block_len(L12, ms3, g3)) continue;
+                                const int gcac = mult(mult(g1, g2), g3);
+                                for (int gca = 1; gca <= nsym; ++gca) {
+                                    const int gcc = mult(gca, gcac);
+                                    if (c.cntc.occ.empty() && gcc > 1) continue;
+                                    if (c.cnta.occ.empty() && gca > 1) continue;
+                                    const int gx1a = mult(g1, gca), gx1c = mult(g1, gcc);
+                                    const int gx2a = mult(g2, gcc), gx2c = mult(g2, gca);
+                                    if ((c.ex1c.occ.empty() && gx1c > 1) || (c.ex1a.occ.empty() && gx1a > 1) ||
+                                        (c.ex2c.occ.empty() && gx2c > 1) || (c.ex2a.occ.empty() && gx2a > 1))
+                                        continue;
+                                    dist_strings(o, c.ex1c, c.ex1a, mex1c, mex1a, gx1c, gx1a, D1);
+                                    dist_strings(o, c.ex2c, c.ex2a, mex2c, mex2a, gx2c, gx2a, D2);
+                                    dist_strings(o, c.cntc, c.cnta, msc_c, msc_a, gcc, gca, DC);
+                                    const size_t n1c = c.ex1c.occ.size(), n2c = c.ex2c.occ.size();
+                                    const size_t ncc = c.cntc.occ.size();
+                                    for (const DistStr &e1 : D1) {
+                                        const DistStr e1c = part(e1, 0, n1c), e1a = part(e1, n1c, c.ex1a.occ.size());
+                                        for (const DistStr &e2 : D2) {
+                                            const DistStr e2c = part(e2, 0, n2c), e2a = part(e2, n2c, c.ex2a.occ.size());
+                                            merge_dis(c.mi12c, L12->hc.size(), e1c, e2c, msr, gr);
+                                            merge_dis(c.mi12a, L12->ha.size(), e2a, e1a, msr_a, gr_a);
+                                            imr.clear(); ima.clear();
+                                            for (size_t i = 0; i < L12->oc.size(); ++i) imr.push_back(idxms_of(L12->oc[i], msr[i]));
+                                            for (size_t i = 0; i < L12->oa.size(); ++i) ima.push_back(idxms_of(L12->oa[i], msr_a[i]));
+                                            const Dist *d12 = list_find(L12, ms3, g3, imr, gr, ima, gr_a);
+                                            if (!d12) continue;
+                                            for (const DistStr &cn : DC) {
+                                                const DistStr cc = part(cn, 0, ncc), ca = part(cn, ncc, c.cnta.occ.size());
+                                                merge_dis(c.mi1c, L1->hc.size(), cc, e1c, m1c, g1c);
+                                                merge_dis(c.mi1a, L1->ha.size(), ca, e1a, m1a, g1a);
+                                                merge_dis(c.mi2c, L2->hc.size(), ca, e2c, m2c, g2c);
+                                                merge_dis(c.mi2a, L2->ha.size(), cc, e2a, m2a, g2a);
+                                                std::vector<int> i1c, i1a, i2c, i2a;
+                                                for (size_t i = 0; i < L1->oc.size(); ++i) i1c.push_back(idxms_of(L1->oc[i], m1c[i]));
+                                                for (size_t i = 0; i < L1->oa.size(); ++i) i1a.push_back(idxms_of(L1->oa[i], m1a[i]));
+                                                for (size_t i = 0; i < L2->oc.size(); ++i) i2c.push_back(idxms_of(L2->oc[i], m2c[i]));
+                                                for (size_t i = 0; i < L2->oa.size(); ++i) i2a.push_back(idxms_of(L2->oa[i], m2a[i]));
+                                                const Dist *d1 = list_find(L1, ms1, g1, i1c, g1c, i1a, g1a);
+                                                const Dist *d2 = list_find(L2, ms2, g2, i2c, g2c, i2a, g2a);
+                                                if (!d1 || !d2) continue;
+                                                int rc = emit(c, c.fact * fac, *d1, *d2, *d12, e1, e2, cn);
+                                                if (rc) return rc;
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                }
+            }
+    return 0;
+}
+
+// sym_ab pairs of a list (contract/sym_ab_blk.f:214-377)
+static int sym_pairs(gecco_hip_list *L, int64_t off, std::vector<int64_t> &i1,
+                     std::vector<int64_t> &i2, std::vector<int8_t> &fs, double relfac) {
+    gecco_hip_orbs *o = L->orbs;
+    const int msc_max = L->nc, msa_max = L->na;
+    double relfac2 = relfac;
+    if (((((msc_max + msa_max) / 2 + 1) * (msc_max - msa_max) / 2) % 2) != 0) relfac2 = -relfac;
+    auto stdsign = [](const std::vector<int> &ms, const std::vector<int> &occ) {
+        if (ms.size() <= 1) return 1;
+        int nb = 0, trp = 0;
+        for (size_t b = 0; b < ms.size(); ++b) {
+            int nal = (occ[b] + ms[b]) / 2, nbe = (occ[b] - ms[b]) / 2;
+            trp += nal * nb;
+            nb += nbe;
+        }
+        return (trp & 1) ? -1 : 1;
+    };
+    for (const Dist &d : L->dists) {
+        int msa_tot = 0;
+        for (int x : d.msa) msa_tot += x;
+        if (msa_tot < 0) continue;
+        std::vector<int> mc2, ma2, imc2, ima2;
+        for (int x : d.msc) mc2.push_back(-x);
+        for (int x : d.msa) ma2.push_back(-x);
+        for (size_t i = 0; i < d.msc.size(); ++i) imc2.push_back(idxms_of(L->oc[i], mc2[i]));
+        for (size_t i = 0; i < d.msa.size(); ++i) ima2.push_back(idxms_of(L->oa[i], ma2[i]));
+        int igama = 1;
+        for (int g : d.ga) igama = mult(igama, g);
+        const Dist *d2 = list_find(L, -msa_tot, igama, imc2, d.gc, ima2, d.ga);
+        if (!d2) return set_error(GECCO_HIP_EINVAL, "sym_ab: flipped distribution missing");
+        if (msa_tot == 0 && d.off > d2->off) continue;
+        const int gsign = stdsign(d.msc, L->oc) * stdsign(d.msa, L->oa) * stdsign(mc2, L->oc) *
+                          stdsign(ma2, L->oa);
+        const size_t ncb = L->oc.size(), nab = L->oa.size();
+        std::vector<const std::vector<int64_t> *> fl;
+        for (size_t k = 0; k < ncb; ++k) fl.push_back(&flipmap(o, L->hc[k], L->oc[k], d.imc[k], d.gc[k]));
+        for (size_t k = 0; k < nab; ++k) fl.push_back(&flipmap(o, L->ha[k], L->oa[k], d.ima[k], d.ga[k]));
+        int64_t lenc = 1, lena = 1;
+        for (size_t k = 0; k < ncb; ++k) lenc *= d.lens[k];
+        for (size_t k = 0; k < nab; ++k) lena *= d.lens[ncb + k];
+        for (int64_t ia = 0; ia < lena; ++ia) {
+            int64_t st = ia, a1 = 0, a2 = 0;
+            int as = gsign;
+            for (size_t k = 0; k < nab; ++k) {
+                int64_t s = st % d.lens[ncb + k];
+                int64_t m = (*fl[ncb + k])[s];
+                a1 += s * d.lda[k];
+                a2 += ((m < 0 ? -m : m) - 1) * d.lda[k];
+                if (m < 0) as = -as;
+                st /= d.lens[ncb + k];
+            }
+            for (int64_t ic = 0; ic < lenc; ++ic) {
+                int64_t st2 = ic, c1 = 0, c2 = 0;
+                int cs = 1;
+                for (size_t k = 0; k < ncb; ++k) {
+                    int64_t s = st2 % d.lens[k];
+                    int64_t m = (*fl[k])[s];
+                    c1 += s * d.ldc[k];
+                    c2 += ((m < 0 ? -m : m) - 1) * d.ldc[k];
+                    if (m < 0) cs = -cs;
+                    st2 /= d.lens[k];
+                }
+                const int64_t x1 = d.off + a1 + c1, x2 = d2->off + a2 + c2;
+                if (x1 > x2) continue;
+                const double f = cs * as * relfac2;
+                i1.push_back(off + x1);
+                i2.push_back(off + x2);
+                fs.push_back((int8_t)(f > 0 ? 1 : -1));
+            }
+        }
+    }
+    return 0;
+}
+
+}  // namespace gecco
+
+using namespace gecco;
+
+extern "C" {
+
+int gecco_hip_orbs_create(int64_t nsym, int64_t nh, const int64_t *irrep_h, int64_t np,
+                          const int64_t *irrep_p, gecco_hip_orbs **out) {
+    if (!out || (nsym != 1 && nsym != 2 && nsym != 4 && nsym != 8) || nh < 0 || np < 0 ||
+        nh > 4000 || np > 4000)
+        return set_error(GECCO_HIP_EINVAL, "orbs: bad arguments");
+    auto *o = new gecco_hip_orbs();
+    o->nsym = (int)nsym;
+    for (int64_t i = 0; i < nh; ++i) {
+        if (irrep_h[i] < 1 || irrep_h[i] > nsym) { delete o; return set_error(GECCO_HIP_EINVAL, "orbs: bad IRREP"); }
+        o->irr[1].push_back((int)irrep_h[i]);
+    }
+    for (int64_t i = 0; i < np; ++i) {
+        if (irrep_p[i] < 1 || irrep_p[i] > nsym) { delete o; return set_error(GECCO_HIP_EINVAL, "orbs: bad IRREP"); }
+        o->irr[2].push_back((int)irrep_p[i]);
+    }
+    std::sort(o->irr[1].begin(), o->irr[1].end());
+    std::sort(o->irr[2].begin(), o->irr[2].end());
+    *out = o;
+    return 0;
+}
+
+int gecco_hip_orbs_destroy(gecco_hip_orbs *o) {
+    delete o;
+    return 0;
+}
+
+int64_t gecco_hip_orbs_lenstr(gecco_hip_orbs *o, int64_t hpvx, int64_t nel, int64_t idxms,
+                              int64_t igam) {
+    if (!o || hpvx < 1 || hpvx > 2 || nel > 8) return -1;
+    return lenstr(o, (int)hpvx, (int)nel, (int)idxms, (int)igam);
+}
+
+int gecco_hip_orbs_strmap(gecco_hip_orbs *o, int64_t hpvx, int64_t n1, int64_t idxms1, int64_t g1,
+                          int64_t n2, int64_t idxms2, int64_t g2, int64_t *out, int64_t maxlen,
+                          int64_t *len) {
+    if (!o || hpvx < 1 || hpvx > 2 || n1 + n2 > 8 || n1 < 0 || n2 < 0)
+        return set_error(GECCO_HIP_EINVAL, "strmap: bad arguments");
+    const auto &m = strmap(o, (int)hpvx, (int)n1, (int)idxms1, (int)g1, (int)n2, (int)idxms2, (int)g2);
+    *len = (int64_t)m.size();
+    if ((int64_t)m.size() > maxlen) return set_error(GECCO_HIP_EINVAL, "strmap: buffer too small");
+    std::copy(m.begin(), m.end(), out);
+    return 0;
+}
+
+int gecco_hip_orbs_flipmap(gecco_hip_orbs *o, int64_t hpvx, int64_t nel, int64_t idxms, int64_t g,
+                           int64_t *out, int64_t maxlen, int64_t *len) {
+    if (!o || hpvx < 1 || hpvx > 2 || nel > 8 || nel < 0)
+        return set_error(GECCO_HIP_EINVAL, "flipmap: bad arguments");
+    const auto &m = flipmap(o, (int)hpvx, (int)nel, (int)idxms, (int)g);
+    *len = (int64_t)m.size();
+    if ((int64_t)m.size() > maxlen) return set_error(GECCO_HIP_EINVAL, "flipmap: buffer too small");
+    std::copy(m.begin(), m.end(), out);
+    return 0;
+}
+
+int gecco_hip_list_create(gecco_hip_orbs *o, const int64_t *occ_c, const int64_t *occ_a,
+                          int64_t absym, gecco_hip_list **out) {
+    if (!o || !occ_c || !occ_a || !out) return set_error(GECCO_HIP_EINVAL, "list: null argument");
+    auto *L = new gecco_hip_list();
+    L->orbs = o;
+    L->absym = (int)absym;
+    for (int h = 1; h <= 4; ++h) {
+        L->occ_c[h] = (int)occ_c[h - 1];
+        L->occ_a[h] = (int)occ_a[h - 1];
+        if (occ_c[h - 1] < 0 || occ_a[h - 1] < 0 || occ_c[h - 1] > 8 || occ_a[h - 1] > 8 ||
+            ((h == 3 || h == 4) && (occ_c[h - 1] || occ_a[h - 1]))) {
+            delete L;
+            return set_error(GECCO_HIP_EINVAL, "list: occupations must be H/P, 0..8");
+        }
+    }
+    build_list(L);
+    *out = L;
+    return 0;
+}
+
+int gecco_hip_list_destroy(gecco_hip_list *L) {
+    delete L;
+    return 0;
+}
+
+int64_t gecco_hip_list_length(const gecco_hip_list *L) { return L ? L->length : -1; }
+
+int64_t gecco_hip_list_ndist(const gecco_hip_list *L) { return L ? (int64_t)L->dists.size() : -1; }
+
+int gecco_hip_list_dist(const gecco_hip_list *L, int64_t i, int64_t *info, int64_t ninfo) {
+    // info = [off, len, msa_tot, igama, ncomp_c, ncomp_a, then per C comp
+    //         (ms, idxms, gam, len, ldim), then per A comp (same)]
+    if (!L || i < 0 || i >= (int64_t)L->dists.size()) return set_error(GECCO_HIP_EINVAL, "list_dist: bad index");
+    const Dist &d = L->dists[i];
+    std::vector<int64_t> v;
+    int msa = 0, ga = 1;
+    for (int x : d.msa) msa += x;
+    for (int x : d.ga) ga = mult(ga, x);
+    v = {d.off, d.len, msa, ga, (int64_t)d.msc.size(), (int64_t)d.msa.size()};
+    for (size_t k = 0; k < d.msc.size(); ++k) {
+        int64_t tmp[5] = {d.msc[k], d.imc[k], d.gc[k], d.lens[k], d.ldc[k]};
+        v.insert(v.end(), tmp, tmp + 5);
+    }
+    for (size_t k = 0; k < d.msa.size(); ++k) {
+        int64_t tmp[5] = {d.msa[k], d.ima[k], d.ga[k], d.lens[d.msc.size() + k], d.lda[k]};
+        v.insert(v.end(), tmp, tmp + 5);
+    }
+    if ((int64_t)v.size() > ninfo) return set_error(GECCO_HIP_EINVAL, "list_dist: buffer too small");
+    std::copy(v.begin(), v.end(), info);
+    return 0;
+}
+
+int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *op1,
+                                   const gecco_hip_list *op2, const gecco_hip_list *res,
+                                   const int64_t *cnt_c, const int64_t *cnt_a, double fact,
+                                   int64_t use_tr, int32_t buf1, int64_t off1, int32_t buf2,
+                                   int64_t off2, int32_t buf12, int64_t off12, int64_t *nblocks,
+                                   double *flops) {
+    if (!plan || !op1 || !op2 || !res || !cnt_c || !cnt_a)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction: null argument");
+    if (op1->orbs != op2->orbs || op1->orbs != res->orbs)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction: lists of different orbital sets");
+    int cc[5] = {0}, ca[5] = {0}, e1c[5] = {0}, e1a[5] = {0}, e2c[5] = {0}, e2a[5] = {0};
+    for (int h = 1; h <= 4; ++h) {
+        cc[h] = (int)cnt_c[h - 1];
+        ca[h] = (int)cnt_a[h - 1];
+        e1c[h] = op1->occ_c[h] - cc[h];
+        e1a[h] = op1->occ_a[h] - ca[h];
+        e2c[h] = op2->occ_c[h] - ca[h];
+        e2a[h] = op2->occ_a[h] - cc[h];
+        if (cc[h] < 0 || ca[h] < 0 || e1c[h] < 0 || e1a[h] < 0 || e2c[h] < 0 || e2a[h] < 0)
+            return set_error(GECCO_HIP_EINVAL, "add_contraction: contraction exceeds an operand");
+        if (res->occ_c[h] != e1c[h] + e2c[h] || res->occ_a[h] != e1a[h] + e2a[h])
+            return set_error(GECCO_HIP_EINVAL, "add_contraction: result occupation != EX1 + EX2");
+    }
+    ContrCtx c;
+    c.plan = plan;
+    c.o = op1->orbs;
+    c.L1 = op1; c.L2 = op2; c.L12 = res;
+    c.ex1c = side_of(e1c); c.ex1a = side_of(e1a);
+    c.ex2c = side_of(e2c); c.ex2a = side_of(e2a);
+    c.cntc = side_of(cc); c.cnta = side_of(ca);
+    c.mi12c = map_info(res->hc, c.ex1c, c.ex2c);
+    c.mi12a = map_info(res->ha, c.ex2a, c.ex1a);
+    c.mi1c = map_info(op1->hc, c.cntc, c.ex1c);
+    c.mi1a = map_info(op1->ha, c.cnta, c.ex1a);
+    c.mi2c = map_info(op2->hc, c.cnta, c.ex2c);
+    c.mi2a = map_info(op2->ha, c.cntc, c.ex2a);
+    c.fact = fact;
+    c.buf1 = buf1; c.buf2 = buf2; c.buf12 = buf12;
+    c.off1 = off1; c.off2 = off2; c.off12 = off12;
+    int rc = enumerate(c, use_tr != 0);
+    if (nblocks) *nblocks = c.nblocks;
+    if (flops) *flops = c.flops;
+    return rc;
+}
+
+int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *L, int32_t buf, int64_t off,
+                              double prefac, double relfac) {
+    if (!plan || !L) return set_error(GECCO_HIP_EINVAL, "add_sym_ab: null argument");
+    std::vector<int64_t> i1, i2;
+    std::vector<int8_t> f;
+    int rc = sym_pairs(L, off, i1, i2, f, relfac);
+    if (rc) return rc;
+    return plan_add_sym_pairs(plan, buf, std::move(i1), std::move(i2), std::move(f), prefac);
+}
+
+}  // extern "C"

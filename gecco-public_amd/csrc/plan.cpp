@@ -31,6 +31,9 @@
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, double *const *bufs, int grid,
                                     hipStream_t stream);
+extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
+                                   const int8_t *fsign, int64_t npair, double prefac,
+                                   hipStream_t stream);
 
 namespace gecco {
 
@@ -287,8 +290,19 @@ struct HostBlock {
 
 }  // namespace gecco
 
+struct SymStep {
+    int32_t buf = 0;
+    double prefac = 0.5;
+    std::vector<int64_t> i1, i2;
+    std::vector<int8_t> f;
+    int64_t npair = 0;
+    int64_t *d_i1 = nullptr, *d_i2 = nullptr;
+    int8_t *d_f = nullptr;
+};
+
 struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
+    std::vector<SymStep> syms;
     bool finalized = false;
     gecco::DevBlock *d_blocks = nullptr;
     gecco::DevTask *d_tasks = nullptr;
@@ -400,6 +414,13 @@ static void plan_free_device(gecco_hip_plan *p) {
     if (p->d_tasks) (void)hipFree(p->d_tasks);
     if (p->d_index) (void)hipFree(p->d_index);
     if (p->d_bufs) (void)hipFree(p->d_bufs);
+    for (auto &y : p->syms) {
+        if (y.d_i1) (void)hipFree(y.d_i1);
+        if (y.d_i2) (void)hipFree(y.d_i2);
+        if (y.d_f) (void)hipFree(y.d_f);
+        y.d_i1 = y.d_i2 = nullptr;
+        y.d_f = nullptr;
+    }
     p->d_blocks = nullptr;
     p->d_tasks = nullptr;
     p->d_index = nullptr;
@@ -446,6 +467,7 @@ static int plan_finalize(gecco_hip_plan *p) {
             maxend_i = ranges[r].second;
         }
     }
+    for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
     // tiles and split-K
     const int BMh = 64, BNh = 64;
     int64_t ntiles = 0;
@@ -509,6 +531,22 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (check(hipMalloc(&p->d_bufs, std::max<int64_t>(1, nb_max) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
+    for (auto &y : p->syms) {
+        y.npair = (int64_t)y.i1.size();
+        if (!y.npair) continue;
+        if (check(hipMalloc(&y.d_i1, y.npair * 8), "hipMalloc(sym)") ||
+            check(hipMalloc(&y.d_i2, y.npair * 8), "hipMalloc(sym)") ||
+            check(hipMalloc(&y.d_f, y.npair), "hipMalloc(sym)") ||
+            check(hipMemcpy(y.d_i1, y.i1.data(), y.npair * 8, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+            check(hipMemcpy(y.d_i2, y.i2.data(), y.npair * 8, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+            check(hipMemcpy(y.d_f, y.f.data(), y.npair, hipMemcpyHostToDevice), "hipMemcpy(sym)"))
+            return g_err;
+        std::vector<int64_t>().swap(y.i1);
+        std::vector<int64_t>().swap(y.i2);
+        std::vector<int8_t>().swap(y.f);
+        p->payload += 16.0 * (double)y.npair;
+        p->index_bytes += 17.0 * (double)y.npair;
+    }
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t prop;
@@ -516,8 +554,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     }
     p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)ncu * 4);
     p->flops = flops;
-    p->payload = payload;
-    p->index_bytes = 4.0 * (double)pool.size();
+    p->payload += payload;
+    p->index_bytes += 4.0 * (double)pool.size();
     p->finalized = true;
     // host copies of the index tables are no longer needed
     for (auto &b : p->blocks)
@@ -533,7 +571,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
     for (int64_t i = 0; i < p->nbufs_max; ++i)
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
-    if (p->ntasks == 0) return ok();
+    if (p->ntasks == 0 && p->syms.empty()) return ok();
     std::vector<double *> want(bufs, bufs + p->nbufs_max);
     if (want != p->last_bufs) {
         // slot table changes only when the caller passes different buffers;
@@ -542,8 +580,11 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
         p->last_bufs = want;
     }
-    if (gecco_launch_grouped(p->d_blocks, p->d_tasks, (int)p->ntasks, p->d_bufs, p->grid, s))
+    if (p->ntasks && gecco_launch_grouped(p->d_blocks, p->d_tasks, (int)p->ntasks, p->d_bufs, p->grid, s))
         return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    for (auto &y : p->syms)
+        if (y.npair && gecco_launch_sym_ab(bufs[y.buf], y.d_i1, y.d_i2, y.d_f, y.npair, y.prefac, s))
+            return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
     return ok();
 }
 
@@ -607,6 +648,30 @@ static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const do
     plan_free_device(&plan);
     return rc ? g_err : ok();
 }
+
+int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *args) {
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    HostBlock hb;
+    if (build_block(args, hb)) return g_err;
+    plan->blocks.push_back(std::move(hb));
+    return ok();
+}
+
+int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &&i1,
+                       std::vector<int64_t> &&i2, std::vector<int8_t> &&f, double prefac) {
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (buf < 0) return fail(GECCO_HIP_EINVAL, "negative buffer slot");
+    SymStep y;
+    y.buf = buf;
+    y.prefac = prefac;
+    y.i1 = std::move(i1);
+    y.i2 = std::move(i2);
+    y.f = std::move(f);
+    plan->syms.push_back(std::move(y));
+    return ok();
+}
+
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 
 }  // namespace gecco
 

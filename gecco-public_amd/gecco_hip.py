@@ -203,3 +203,145 @@ class Plan:
                 self._h = ctypes.c_void_p()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------- plan builder
+MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_orbs_lenstr',
+                 'gecco_hip_orbs_strmap', 'gecco_hip_orbs_flipmap', 'gecco_hip_list_create',
+                 'gecco_hip_list_destroy', 'gecco_hip_list_length', 'gecco_hip_list_ndist',
+                 'gecco_hip_list_dist', 'gecco_hip_plan_add_contraction',
+                 'gecco_hip_plan_add_sym_ab')
+VP = ctypes.c_void_p
+_lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
+_lib.gecco_hip_orbs_create.restype = ctypes.c_int
+_lib.gecco_hip_orbs_destroy.argtypes = [VP]
+_lib.gecco_hip_orbs_lenstr.argtypes = [VP, I64, I64, I64, I64]
+_lib.gecco_hip_orbs_lenstr.restype = I64
+for _n in ('gecco_hip_orbs_strmap',):
+    getattr(_lib, _n).argtypes = [VP, I64, I64, I64, I64, I64, I64, I64, PI64, I64, PI64]
+    getattr(_lib, _n).restype = ctypes.c_int
+_lib.gecco_hip_orbs_flipmap.argtypes = [VP, I64, I64, I64, I64, PI64, I64, PI64]
+_lib.gecco_hip_orbs_flipmap.restype = ctypes.c_int
+_lib.gecco_hip_list_create.argtypes = [VP, PI64, PI64, I64, ctypes.POINTER(VP)]
+_lib.gecco_hip_list_create.restype = ctypes.c_int
+_lib.gecco_hip_list_destroy.argtypes = [VP]
+_lib.gecco_hip_list_length.argtypes = [VP]
+_lib.gecco_hip_list_length.restype = I64
+_lib.gecco_hip_list_ndist.argtypes = [VP]
+_lib.gecco_hip_list_ndist.restype = I64
+_lib.gecco_hip_list_dist.argtypes = [VP, I64, PI64, I64]
+_lib.gecco_hip_list_dist.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_contraction.argtypes = [
+    VP, VP, VP, VP, PI64, PI64, ctypes.c_double, I64, ctypes.c_int32, I64, ctypes.c_int32, I64,
+    ctypes.c_int32, I64, PI64, PD]
+_lib.gecco_hip_plan_add_contraction.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_sym_ab.argtypes = [VP, VP, ctypes.c_int32, I64, ctypes.c_double,
+                                           ctypes.c_double]
+_lib.gecco_hip_plan_add_sym_ab.restype = ctypes.c_int
+
+HOLE, PART = 1, 2
+
+
+def _occ4(d):
+    a = np.zeros(4, dtype=np.int64)
+    for h, n in d.items():
+        a[h - 1] = n
+    return a
+
+
+class Orbs:
+    """Orbital spaces (hole / particle IRREPs) of the model."""
+
+    def __init__(self, h_irreps, p_irreps, nsym):
+        h = np.asarray(list(h_irreps) or [1], dtype=np.int64)
+        p = np.asarray(list(p_irreps) or [1], dtype=np.int64)
+        self._h = VP()
+        self.nsym = nsym
+        _check(_lib.gecco_hip_orbs_create(nsym, len(h_irreps), h.ctypes.data_as(PI64),
+                                          len(p_irreps), p.ctypes.data_as(PI64),
+                                          ctypes.byref(self._h)))
+
+    def lenstr(self, hpvx, nel, idxms, igam):
+        return _lib.gecco_hip_orbs_lenstr(self._h, hpvx, nel, idxms, igam)
+
+    def _getmap(self, fn, *args):
+        n = 1 << 16
+        while True:
+            out = np.zeros(n, dtype=np.int64)
+            ln = I64()
+            rc = fn(self._h, *args, out.ctypes.data_as(PI64), n, ctypes.byref(ln))
+            if rc == 0:
+                return out[:ln.value].tolist()
+            if ln.value > n:
+                n = ln.value
+                continue
+            _check(rc)
+
+    def strmap(self, hpvx, n1, idxms1, g1, n2, idxms2, g2):
+        return self._getmap(_lib.gecco_hip_orbs_strmap, hpvx, n1, idxms1, g1, n2, idxms2, g2)
+
+    def flipmap(self, hpvx, nel, idxms, g):
+        return self._getmap(_lib.gecco_hip_orbs_flipmap, hpvx, nel, idxms, g)
+
+    def __del__(self):
+        try:
+            _lib.gecco_hip_orbs_destroy(self._h)
+        except Exception:
+            pass
+
+
+class MEList:
+    """ME-list layout of a one-vertex operator (operators/set_op_dim2.f)."""
+
+    def __init__(self, orbs, occ_c, occ_a, absym=1):
+        self.orbs = orbs
+        self.occ_c = dict(occ_c)
+        self.occ_a = dict(occ_a)
+        self.absym = absym
+        c, a = _occ4(occ_c), _occ4(occ_a)
+        self._h = VP()
+        _check(_lib.gecco_hip_list_create(orbs._h, c.ctypes.data_as(PI64), a.ctypes.data_as(PI64),
+                                          absym, ctypes.byref(self._h)))
+        self.length = _lib.gecco_hip_list_length(self._h)
+
+    def dists(self):
+        out = []
+        buf = np.zeros(64, dtype=np.int64)
+        for i in range(_lib.gecco_hip_list_ndist(self._h)):
+            _check(_lib.gecco_hip_list_dist(self._h, i, buf.ctypes.data_as(PI64), 64))
+            off, ln, msa, ga, ncc, nca = buf[:6].tolist()
+            comps = buf[6:6 + 5 * (ncc + nca)].reshape(-1, 5).tolist()
+            out.append(dict(off=off, len=ln, msa_tot=msa, igama=ga, comps_c=comps[:ncc],
+                            comps_a=comps[ncc:]))
+        return out
+
+    def __del__(self):
+        try:
+            _lib.gecco_hip_list_destroy(self._h)
+        except Exception:
+            pass
+
+
+def _plan_add_contraction(self, op1, op2, res, cnt_c, cnt_a, fact=1.0, use_tr=True,
+                          buf=(0, 1, 2), off=(0, 0, 0)):
+    """Record all kernel calls of Res += fact*Op1.Op2 (contr_op1op2)."""
+    cc, ca = _occ4(cnt_c), _occ4(cnt_a)
+    nb = I64()
+    fl = ctypes.c_double()
+    _check(_lib.gecco_hip_plan_add_contraction(
+        self._h, op1._h, op2._h, res._h, cc.ctypes.data_as(PI64), ca.ctypes.data_as(PI64),
+        float(fact), int(bool(use_tr)), buf[0], off[0], buf[1], off[1], buf[2], off[2],
+        ctypes.byref(nb), ctypes.byref(fl)))
+    self.nblocks += nb.value
+    return nb.value, fl.value
+
+
+def _plan_add_sym_ab(self, lst, buf, off=0, prefac=0.5, relfac=None):
+    """Spin-flip symmetrisation of a result list (sym_ab_blk)."""
+    if relfac is None:
+        relfac = float(lst.absym)
+    _check(_lib.gecco_hip_plan_add_sym_ab(self._h, lst._h, buf, off, prefac, relfac))
+
+
+Plan.add_contraction = _plan_add_contraction
+Plan.add_sym_ab = _plan_add_sym_ab

@@ -1,0 +1,81 @@
+/*
+ * gecco_hip_model.h -- plan builder of the MI355X block-contraction engine.
+ *
+ * Builds GeCCo's data model on the host -- orbital strings, string maps,
+ * ME-list layout -- and enumerates the kernel calls of a binary contraction
+ * as contr_op1op2_wmaps_c does (contract/contr_op1op2_wmaps_c.f:964-2070),
+ * recording them into a gecco_hip_plan (include/gecco_hip.h).  This is the
+ * "option N" boundary: a driver that holds device-resident ME-lists calls
+ * gecco_hip_plan_add_contraction once per [CONTR] formula item instead of
+ * contr_op1op2 (contract/contr_op1op2.f:2-184, reached from
+ * contract/fs_contr_drv.f:273).
+ *
+ * Scope: one-vertex operators (njoined = 1) over hole (H, hpvx 1) and
+ * particle (P, hpvx 2) spaces with one orbital subspace each, Ms(total) 0,
+ * totally symmetric (gamt 1), as in closed-shell CC.
+ */
+#ifndef GECCO_HIP_MODEL_H
+#define GECCO_HIP_MODEL_H
+
+#include <stdint.h>
+#include "gecco_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gecco_hip_orbs gecco_hip_orbs;
+typedef struct gecco_hip_list gecco_hip_list;
+
+/* Orbital spaces: IRREPs (1..nsym, D2h subgroup labels) of the hole and
+ * particle orbitals; orbitals are ordered by IRREP inside each space, as
+ * GeCCo's mostnd arrays assume (strings/lexlstr.f:60,80). */
+int gecco_hip_orbs_create(int64_t nsym, int64_t nh, const int64_t *irrep_h, int64_t np,
+                          const int64_t *irrep_p, gecco_hip_orbs **out);
+int gecco_hip_orbs_destroy(gecco_hip_orbs *orbs);
+/* Number of strings of (space hpvx, nel, idxms, igam): graph%lenstr_gm. */
+int64_t gecco_hip_orbs_lenstr(gecco_hip_orbs *orbs, int64_t hpvx, int64_t nel, int64_t idxms,
+                              int64_t igam);
+/* (str1,str2)->str12 map block, str1 fastest (strings/set_strmap_kernel.f). */
+int gecco_hip_orbs_strmap(gecco_hip_orbs *orbs, int64_t hpvx, int64_t n1, int64_t idxms1,
+                          int64_t g1, int64_t n2, int64_t idxms2, int64_t g2, int64_t *out,
+                          int64_t maxlen, int64_t *len);
+/* Spin-flip map block (strings/set_flipmap_kernel.f). */
+int gecco_hip_orbs_flipmap(gecco_hip_orbs *orbs, int64_t hpvx, int64_t nel, int64_t idxms,
+                           int64_t igam, int64_t *out, int64_t maxlen, int64_t *len);
+
+/* ME-list of a one-vertex operator; occ_c/occ_a = occupation per space type
+ * (H, P, V, X); absym = alpha/beta (spin-flip) symmetry, +1 for singlets.
+ * Layout: operators/set_op_dim2.f:146-459. */
+int gecco_hip_list_create(gecco_hip_orbs *orbs, const int64_t *occ_c, const int64_t *occ_a,
+                          int64_t absym, gecco_hip_list **out);
+int gecco_hip_list_destroy(gecco_hip_list *list);
+int64_t gecco_hip_list_length(const gecco_hip_list *list);
+int64_t gecco_hip_list_ndist(const gecco_hip_list *list);
+/* info = [off, len, Ms(A), IRREP(A), ncomp_c, ncomp_a,
+ *         per C component (ms, idxms, irrep, nstr, ldim), then per A comp]. */
+int gecco_hip_list_dist(const gecco_hip_list *list, int64_t idist, int64_t *info, int64_t ninfo);
+
+/* Record every kernel call of  Res += fact * Op1 . Op2  (contracted part of
+ * Op1's C side cnt_c[4], of Op1's A side cnt_a[4], per space type) into the
+ * plan, with spin-flip skipping and factors when use_tr (the use_tr keyword,
+ * contract/contr_op1op2_wmaps_c.f:454-458,1007-1101).  Lists live in buffer
+ * slots buf* at element offsets off*.  Returns the number of kernel calls and
+ * their algorithmic FLOP.  With use_tr the caller adds the result's
+ * spin-flip symmetrisation with gecco_hip_plan_add_sym_ab. */
+int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *op1,
+                                   const gecco_hip_list *op2, const gecco_hip_list *res,
+                                   const int64_t *cnt_c, const int64_t *cnt_a, double fact,
+                                   int64_t use_tr, int32_t buf1, int64_t off1, int32_t buf2,
+                                   int64_t off2, int32_t buf12, int64_t off12, int64_t *nblocks,
+                                   double *flops);
+/* Spin-flip symmetrisation of a result list after the plan's contractions
+ * (contract/sym_ab_blk.f, called with prefac 1/2, relfac absym at
+ * contract/contr_op1op2_wmaps_c.f:2099-2102). */
+int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *list, int32_t buf,
+                              int64_t off, double prefac, double relfac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GECCO_HIP_MODEL_H */

@@ -42,3 +42,64 @@ def test_error_path_without_device_work():
     blk['map_ex1cntc'] = [10 ** 6] * len(blk['map_ex1cntc'])   # out of the block
     with pytest.raises(gecco_hip.GeccoHipError, match='addresses element'):
         p.add(blk)
+
+
+def test_model_strings_match_oracle_bit_exact():
+    """Product string graphs / maps / flip maps == oracle restatement."""
+    import gecco_hip
+    from oracle.strings import Space, strmap, flipmap, lenstr
+    irr_h, irr_p = [1, 2, 3, 4, 1], [1, 2, 3, 4, 1, 2, 3]
+    o = gecco_hip.Orbs(irr_h, irr_p, 4)
+    for hpvx, irr in ((1, irr_h), (2, irr_p)):
+        sp = Space(irr, 4)
+        for nel in range(0, 4):
+            for im in range(1, nel + 2):
+                for g in range(1, 5):
+                    assert o.lenstr(hpvx, nel, im, g) == lenstr(sp, nel, im, g)
+                    assert o.flipmap(hpvx, nel, im, g) == flipmap(sp, nel, im, g)
+        for (n1, im1, g1) in [(1, 1, 1), (1, 2, 3), (2, 2, 1), (2, 1, 4)]:
+            for (n2, im2, g2) in [(1, 1, 2), (1, 2, 1), (2, 3, 2)]:
+                assert o.strmap(hpvx, n1, im1, g1, n2, im2, g2) == \
+                    strmap(sp, n1, im1, g1, n2, im2, g2)
+
+
+def test_model_list_layout_matches_oracle():
+    import gecco_hip
+    from oracle.melist import Orbitals, MEList as OMEList
+    irr_h, irr_p = [1, 2, 1, 2], [1, 2, 1, 2, 2]
+    o = gecco_hip.Orbs(irr_h, irr_p, 2)
+    oo = Orbitals(irr_h, irr_p, 2)
+    for occ_c, occ_a in [({2: 2}, {1: 2}), ({1: 1, 2: 1}, {1: 1, 2: 1}), ({2: 1}, {1: 1}),
+                         ({}, {}), ({2: 2}, {2: 2}), ({1: 2}, {2: 2}), ({2: 3}, {1: 3})]:
+        L = gecco_hip.MEList(o, occ_c, occ_a)
+        R = OMEList(oo, occ_c, occ_a)
+        assert L.length == R.length
+        ref = [d for dl in R.blocks.values() for d in dl]
+        got = L.dists()
+        assert len(got) == len(ref)
+        for g, r in zip(got, ref):
+            assert g['off'] == r['off'] and g['len'] == r['len']
+            assert [c[3] for c in g['comps_c'] + g['comps_a']] == r['lens']
+            assert [c[4] for c in g['comps_c']] == r['ldc']
+            assert [c[4] for c in g['comps_a']] == r['lda']
+
+
+def test_model_enumerator_counts_match_oracle():
+    """Same number of kernel calls and the same FLOP as the oracle's
+    restatement of contr_op1op2_wmaps_c (recorded on the host only)."""
+    import gecco_hip
+    from models import CASES, round_robin
+    from oracle.melist import Orbitals
+    for nsym, no, nv in [(1, 3, 4), (2, 4, 5), (4, 5, 7)]:
+        o = gecco_hip.Orbs(round_robin(no, nsym), round_robin(nv, nsym, 1), nsym)
+        oo = Orbitals(round_robin(no, nsym), round_robin(nv, nsym, 1), nsym)
+        for name, f in CASES.items():
+            c, a, b, r = f(oo)
+            for use_tr in (True, False):
+                p = gecco_hip.Plan()
+                A = gecco_hip.MEList(o, a.occ_c, a.occ_a)
+                B = gecco_hip.MEList(o, b.occ_c, b.occ_a)
+                R = gecco_hip.MEList(o, r.occ_c, r.occ_a)
+                nb, fl = p.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, use_tr)
+                assert nb == len(c.blocks(use_tr)), (name, nsym, use_tr)
+                assert fl == c.flops(use_tr)
