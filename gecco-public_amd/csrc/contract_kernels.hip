@@ -107,7 +107,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         }
 
         // signed scatter-accumulate; output element (row, col) of a 16x16
-        // MFMA tile sits in lane col + 16*(row/4), register row%4
+        // f64 MFMA tile sits in lane col + 16*(row%4), register row/4
         const DevOperand oc = b.C;
         const double xfac = b.xfac;
         const int atomic = b.atomic;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
             for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 32 + mi * 16 + lrow * 4 + r;
+                    const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
                     const int n = n0 + wn * 32 + ni * 16 + lcol;
                     if (m >= M || n >= N) continue;
                     int32_t v;

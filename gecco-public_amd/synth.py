@@ -1,0 +1,57 @@
+"""Seeded synthetic closed-shell Hamiltonians (inputs for tests and bench).
+
+Recipe: orbital energies eps_occ ~ U[occ_lo, occ_hi], eps_vir ~ U[vir_lo,
+vir_hi]; two-electron integrals from a density-fitted factorisation
+(pq|rs) = sum_P B(P,pq) B(P,rs) with B ~ N(0, sigma^2) symmetric in pq;
+orbitals and auxiliary functions assigned round-robin to nsym IRREPs of a
+D2h subgroup (XOR products), B(P,pq) = 0 unless IRREP(p)*IRREP(q) =
+IRREP(P), so (pq|rs) is totally symmetric.  The Fock operator is diagonal
+(canonical orbitals): f_pq = eps_p delta_pq.
+
+Within each space the orbitals are reordered by IRREP (stable), which is
+the orbital order GeCCo's strings assume; irrep_o/irrep_v are returned in
+that order.
+"""
+import numpy as np
+
+
+def multd2h(a, b):
+    return ((a - 1) ^ (b - 1)) + 1
+
+
+class Hamiltonian:
+    def __init__(self, eps_o, eps_v, irrep_o, irrep_v, B, nsym):
+        self.eps_o, self.eps_v = eps_o, eps_v
+        self.irrep_o, self.irrep_v = irrep_o, irrep_v
+        self.B = B            # (naux, n, n), orbitals occ then vir
+        self.nsym = nsym
+        self.nocc, self.nvir = len(eps_o), len(eps_v)
+
+    def eri(self):
+        """(pq|rs) chemists' notation, dense (n^4)."""
+        n = self.nocc + self.nvir
+        Bm = self.B.reshape(self.B.shape[0], n * n)
+        return (Bm.T @ Bm).reshape(n, n, n, n)
+
+
+def synthetic(nocc, nvir, naux, nsym=1, seed=0, sigma=0.03, occ_range=(-2.0, -0.3),
+              vir_range=(0.2, 3.0)):
+    rng = np.random.default_rng(seed)
+    eps_o = rng.uniform(*occ_range, nocc)
+    eps_v = rng.uniform(*vir_range, nvir)
+    n = nocc + nvir
+    irr = np.array([(p % nsym) + 1 for p in range(n)])
+    # order by IRREP inside each space (stable)
+    po = np.argsort(irr[:nocc], kind='stable')
+    pv = nocc + np.argsort(irr[nocc:], kind='stable')
+    perm = np.concatenate([po, pv])
+    irr = irr[perm]
+    eps = np.concatenate([eps_o, eps_v])[perm]
+    G = rng.standard_normal((naux, n, n)) * sigma
+    G = (G + G.transpose(0, 2, 1)) / np.sqrt(2.0)
+    G = G[:, perm][:, :, perm]
+    irr_aux = np.array([(P % nsym) + 1 for P in range(naux)])
+    gpq = ((irr[:, None] - 1) ^ (irr[None, :] - 1)) + 1
+    mask = gpq[None, :, :] == irr_aux[:, None, None]
+    B = np.where(mask, G, 0.0)
+    return Hamiltonian(eps[:nocc], eps[nocc:], irr[:nocc].tolist(), irr[nocc:].tolist(), B, nsym)
