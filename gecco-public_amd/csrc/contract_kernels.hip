@@ -18,25 +18,88 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
 
-__device__ __forceinline__ double gather(const double *__restrict__ base,
-                                         const int32_t *first, const int32_t *second,
-                                         int64_t ld, int table, int k, int x) {
-    int32_t v;
-    uint32_t neg;
-    int64_t addr;
-    if (table) {
-        v = first[(int64_t)k + ld * (int64_t)x];
-        if (v == 0) return 0.0;
-        neg = (uint32_t)v >> 31;
-        addr = (int64_t)(v < 0 ? -v : v) - 1;
-    } else {
-        int32_t r = first[k], c = second[x];
-        if (r == 0 || c == 0) return 0.0;
-        neg = ((uint32_t)r ^ (uint32_t)c) >> 31;
-        addr = (int64_t)(r < 0 ? -r : r) + (int64_t)(c < 0 ? -c : c) - 2;
+// Per-thread view of one gathered operand tile (BK x 64).  Thread tid loads
+// 4 elements; with consecutive-k memory order (fast_first) they share one k
+// and have 4 different m/n, otherwise they share one m/n and have 4 k.  The
+// per-m/n half of a separable address is fixed for a task and computed once;
+// affine halves need no index load at all.
+struct OpView {
+    const double *base;
+    const int32_t *first;
+    int64_t ld, f0, f1;
+    int table, faff;
+    int kk[4];
+    int xi[4];
+    int64_t cadd[4];   // separable: offset of the m/n half; table: unused
+    uint32_t cneg[4];  // sign bit of the m/n half
+    int cval[4];       // element column valid and not forbidden
+};
+
+__device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const double *base, int x0,
+                                        int X, int tid) {
+    v.base = base;
+    v.first = o.first;
+    v.ld = o.ld;
+    v.table = o.table;
+    v.faff = o.first_affine;
+    v.f0 = o.f0;
+    v.f1 = o.f1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (o.fast_first) { v.kk[r] = tid & (BK - 1); v.xi[r] = (tid >> 4) + 16 * r; }
+        else { v.kk[r] = (tid >> 6) + 4 * r; v.xi[r] = tid & 63; }
+        const int x = x0 + v.xi[r];
+        v.xi[r] = x;
+        v.cadd[r] = 0;
+        v.cneg[r] = 0;
+        v.cval[r] = x < X;
+        if (x < X && !o.table) {
+            if (o.second_affine) {
+                v.cadd[r] = o.s0 + o.s1 * (int64_t)x;
+            } else {
+                const int32_t c = o.second[x];
+                v.cval[r] = c != 0;
+                v.cneg[r] = (uint32_t)c >> 31;
+                v.cadd[r] = (int64_t)(c < 0 ? -c : c) - 1;
+            }
+        }
     }
-    double val = base[addr];
-    return neg ? -val : val;
+}
+
+// Issue the loads of one tile; the sign and the zero of forbidden or
+// out-of-range elements are applied at the LDS store (op_fix), so the loads
+// stay in flight while the MFMAs of the previous tile run.
+__device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double raw[4],
+                                        uint32_t &flags) {
+    flags = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = k0 + v.kk[r];
+        bool ok = (k < k1) && v.cval[r];
+        int64_t addr = 0;
+        uint32_t neg = 0;
+        if (v.table) {
+            const int32_t rv = ok ? v.first[(int64_t)k + v.ld * (int64_t)v.xi[r]] : 0;
+            ok = ok && rv != 0;
+            neg = (uint32_t)rv >> 31;
+            addr = (int64_t)(rv < 0 ? -rv : rv) - 1;
+        } else if (v.faff) {
+            addr = v.f0 + v.f1 * (int64_t)k + v.cadd[r];
+            neg = v.cneg[r];
+        } else {
+            const int32_t rv = ok ? v.first[k] : 0;
+            ok = ok && rv != 0;
+            neg = ((uint32_t)rv >> 31) ^ v.cneg[r];
+            addr = (int64_t)(rv < 0 ? -rv : rv) - 1 + v.cadd[r];
+        }
+        raw[r] = v.base[ok ? addr : 0];
+        flags |= ((uint32_t)ok << r) | (neg << (r + 4));
+    }
+}
+
+__device__ __forceinline__ double op_fix(double raw, uint32_t flags, int r) {
+    const double v = ((flags >> (r + 4)) & 1u) ? -raw : raw;
+    return ((flags >> r) & 1u) ? v : 0.0;
 }
 
 __global__ __launch_bounds__(NT) void grouped_contract_f64(
@@ -53,45 +116,32 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
         const DevBlock &b = blocks[task.blk];
-        const double *__restrict__ pa = bufs[b.buf_a] + b.base_a;
-        const double *__restrict__ pb = bufs[b.buf_b] + b.base_b;
-        double *pc = bufs[b.buf_c] + b.base_c;
         const int M = b.M, N = b.N;
         const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
-        const DevOperand oa = b.A, ob = b.B;
+        OpView va, vb;
+        op_init(va, b.A, bufs[b.buf_a] + b.base_a, m0, M, tid);
+        op_init(vb, b.B, bufs[b.buf_b] + b.base_b, n0, N, tid);
 
         v4d acc[2][2];
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
+        double ra[4], rb[4];
+        uint32_t fa, fb;
+        op_load(va, task.k0, k1, ra, fa);
+        op_load(vb, task.k0, k1, rb, fb);
         for (int k0 = task.k0; k0 < k1; k0 += BK) {
-            double ra[4], rb[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int e = tid + NT * r;
-                int kk, mm;
-                if (oa.fast_first) { kk = e & (BK - 1); mm = e >> 4; }
-                else { mm = e & (BM - 1); kk = e >> 6; }
-                const int k = k0 + kk, m = m0 + mm;
-                ra[r] = (k < k1 && m < M)
-                            ? gather(pa, oa.first, oa.second, oa.ld, oa.table, k, m) : 0.0;
-                int kb, nn;
-                if (ob.fast_first) { kb = e & (BK - 1); nn = e >> 4; }
-                else { nn = e & (BN - 1); kb = e >> 6; }
-                const int kq = k0 + kb, n = n0 + nn;
-                rb[r] = (kq < k1 && n < N)
-                            ? gather(pb, ob.first, ob.second, ob.ld, ob.table, kq, n) : 0.0;
-            }
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int e = tid + NT * r;
-                if (oa.fast_first) As[e & (BK - 1)][e >> 4] = ra[r];
-                else As[e >> 6][e & (BM - 1)] = ra[r];
-                if (ob.fast_first) Bs[e & (BK - 1)][e >> 4] = rb[r];
-                else Bs[e >> 6][e & (BN - 1)] = rb[r];
+                As[va.kk[r]][va.xi[r] - m0] = op_fix(ra[r], fa, r);
+                Bs[vb.kk[r]][vb.xi[r] - n0] = op_fix(rb[r], fb, r);
             }
             __syncthreads();
+            if (k0 + BK < k1) {   // next tile in flight during the MFMAs
+                op_load(va, k0 + BK, k1, ra, fa);
+                op_load(vb, k0 + BK, k1, rb, fb);
+            }
 #pragma unroll
             for (int kq = 0; kq < BK / 4; ++kq) {
                 const int kr = kq * 4 + lrow;
@@ -109,27 +159,33 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         // signed scatter-accumulate; output element (row, col) of a 16x16
         // f64 MFMA tile sits in lane col + 16*(row%4), register row/4
         const DevOperand oc = b.C;
+        double *pc = bufs[b.buf_c] + b.base_c;
         const double xfac = b.xfac;
         const int atomic = b.atomic;
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni) {
+            const int n = n0 + wn * 32 + ni * 16 + lcol;
+            const int32_t cc = (n < N && !oc.table)
+                                   ? (oc.second_affine ? (int32_t)(1 + oc.s0 + oc.s1 * (int64_t)n)
+                                                       : oc.second[n])
+                                   : 1;
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni)
+            for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
-                    const int n = n0 + wn * 32 + ni * 16 + lcol;
                     if (m >= M || n >= N) continue;
-                    int32_t v;
                     int64_t addr;
                     bool neg;
                     if (oc.table) {
-                        v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+                        const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
                         if (v == 0) continue;
                         neg = v < 0;
                         addr = (int64_t)(v < 0 ? -v : v) - 1;
                     } else {
-                        int32_t rr = oc.first[m], cc = oc.second[n];
+                        const int32_t rr = oc.first_affine
+                                               ? (int32_t)(1 + oc.f0 + oc.f1 * (int64_t)m)
+                                               : oc.first[m];
                         if (rr == 0 || cc == 0) continue;
                         neg = (rr < 0) != (cc < 0);
                         addr = (int64_t)(rr < 0 ? -rr : rr) + (int64_t)(cc < 0 ? -cc : cc) - 2;
@@ -139,7 +195,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                     if (atomic) unsafeAtomicAdd(pc + addr, val);
                     else pc[addr] += val;
                 }
-        __syncthreads();
+        }
     }
 }
 

@@ -22,6 +22,10 @@ struct DevOperand {
     int64_t ld;             // table: extent of the first index
     int32_t table;          // 1 = table mode
     int32_t fast_first;     // gathers: 1 if consecutive k are adjacent in memory
+    // separable vectors that are affine (entry = 1 + a0 + a1*i, all positive)
+    // are evaluated arithmetically instead of being loaded
+    int32_t first_affine, second_affine;
+    int64_t f0, f1, s0, s1;
 };
 
 struct DevBlock {

@@ -249,6 +249,16 @@ static bool first_is_contiguous(const Resolved &r) {
     return r.first[0] != 0 && r.first[1] != 0 && absv(r.first[1]) - absv(r.first[0]) == 1;
 }
 
+// entry i == 1 + a0 + a1*i for all i (no forbidden elements, all signs +)
+static bool affine_of(const std::vector<int32_t> &v, int64_t &a0, int64_t &a1) {
+    if (v.empty() || v[0] <= 0) return false;
+    a0 = (int64_t)v[0] - 1;
+    a1 = v.size() > 1 ? (int64_t)v[1] - (int64_t)v[0] : 0;
+    for (size_t i = 0; i < v.size(); ++i)
+        if ((int64_t)v[i] != 1 + a0 + a1 * (int64_t)i) return false;
+    return true;
+}
+
 static bool result_injective(const Resolved &r, int64_t len) {
     std::vector<uint8_t> seen(len, 0);
     if (r.table) {
@@ -521,6 +531,13 @@ static int plan_finalize(gecco_hip_plan *p) {
             ds[q]->ld = hs[q]->r.F;
             ds[q]->table = hs[q]->r.table ? 1 : 0;
             ds[q]->fast_first = hs[q]->fast_first;
+            ds[q]->first_affine = ds[q]->second_affine = 0;
+            ds[q]->f0 = ds[q]->f1 = ds[q]->s0 = ds[q]->s1 = 0;
+            if (!hs[q]->r.table) {
+                int64_t a0, a1;
+                if (affine_of(hs[q]->r.first, a0, a1)) { ds[q]->first_affine = 1; ds[q]->f0 = a0; ds[q]->f1 = a1; }
+                if (affine_of(hs[q]->r.second, a0, a1)) { ds[q]->second_affine = 1; ds[q]->s0 = a0; ds[q]->s1 = a1; }
+            }
         }
     }
     p->d_index = d_index;
@@ -552,7 +569,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
     }
-    p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)ncu * 4);
+    p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)ncu * 8);
     p->flops = flops;
     p->payload += payload;
     p->index_bytes += 4.0 * (double)pool.size();
