@@ -17,10 +17,11 @@ namespace gecco {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+constexpr int LPT = BK * BM / NT;   // elements of each operand tile per thread
 
 // Per-thread view of one gathered operand tile (BK x 64).  Thread tid loads
-// 4 elements; with consecutive-k memory order (fast_first) they share one k
-// and have 4 different m/n, otherwise they share one m/n and have 4 k.  The
+// LPT elements; with consecutive-k memory order (fast_first) they share one k
+// and have LPT different m/n, otherwise they share one m/n and have LPT k.  The
 // per-m/n half of a separable address is fixed for a task and computed once;
 // affine halves need no index load at all.
 struct OpView {
@@ -28,11 +29,11 @@ struct OpView {
     const int32_t *first;
     int64_t ld, f0, f1;
     int table, faff;
-    int kk[4];
-    int xi[4];
-    int64_t cadd[4];   // separable: offset of the m/n half; table: unused
-    uint32_t cneg[4];  // sign bit of the m/n half
-    int cval[4];       // element column valid and not forbidden
+    int kk[LPT];
+    int xi[LPT];
+    int64_t cadd[LPT];   // separable: offset of the m/n half; table: unused
+    uint32_t cneg[LPT];  // sign bit of the m/n half
+    int cval[LPT];       // element column valid and not forbidden
 };
 
 __device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const double *base, int x0,
@@ -45,9 +46,10 @@ __device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const do
     v.f0 = o.f0;
     v.f1 = o.f1;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if (o.fast_first) { v.kk[r] = tid & (BK - 1); v.xi[r] = (tid >> 4) + 16 * r; }
-        else { v.kk[r] = (tid >> 6) + 4 * r; v.xi[r] = tid & 63; }
+    for (int r = 0; r < LPT; ++r) {
+        // fast_first: BK consecutive threads walk k; else 64 threads walk m/n
+        if (o.fast_first) { v.kk[r] = tid % BK; v.xi[r] = tid / BK + (NT / BK) * r; }
+        else { v.kk[r] = tid / BM + (NT / BM) * r; v.xi[r] = tid % BM; }
         const int x = x0 + v.xi[r];
         v.xi[r] = x;
         v.cadd[r] = 0;
@@ -69,11 +71,11 @@ __device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const do
 // Issue the loads of one tile; the sign and the zero of forbidden or
 // out-of-range elements are applied at the LDS store (op_fix), so the loads
 // stay in flight while the MFMAs of the previous tile run.
-__device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double raw[4],
+__device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double raw[LPT],
                                         uint32_t &flags) {
     flags = 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < LPT; ++r) {
         const int k = k0 + v.kk[r];
         bool ok = (k < k1) && v.cval[r];
         int64_t addr = 0;
@@ -93,12 +95,12 @@ __device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double 
             addr = (int64_t)(rv < 0 ? -rv : rv) - 1 + v.cadd[r];
         }
         raw[r] = v.base[ok ? addr : 0];
-        flags |= ((uint32_t)ok << r) | (neg << (r + 4));
+        flags |= ((uint32_t)ok << r) | (neg << (r + LPT));
     }
 }
 
 __device__ __forceinline__ double op_fix(double raw, uint32_t flags, int r) {
-    const double v = ((flags >> (r + 4)) & 1u) ? -raw : raw;
+    const double v = ((flags >> (r + LPT)) & 1u) ? -raw : raw;
     return ((flags >> r) & 1u) ? v : 0.0;
 }
 
@@ -126,14 +128,14 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-        double ra[4], rb[4];
+        double ra[LPT], rb[LPT];
         uint32_t fa, fb;
         op_load(va, task.k0, k1, ra, fa);
         op_load(vb, task.k0, k1, rb, fb);
         for (int k0 = task.k0; k0 < k1; k0 += BK) {
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < LPT; ++r) {
                 As[va.kk[r]][va.xi[r] - m0] = op_fix(ra[r], fa, r);
                 Bs[vb.kk[r]][vb.xi[r] - n0] = op_fix(rb[r], fb, r);
             }
@@ -199,6 +201,127 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
     }
 }
 
+// Dense blocks whose three index maps are all affine (no forbidden
+// elements, all signs +, constant strides -- e.g. the particle-particle
+// ladder) are strided GEMMs: addresses are base + stride_k*k + stride_x*x.
+// Same tile and MFMA layout as above, K step ABK, and only a base offset, a
+// stride and a validity mask per operand and thread.
+constexpr int ABK = 32, ALPT = ABK * BM / NT;
+
+struct AffView {
+    int64_t base;     // element offset of this thread's first element at k = 0
+    int64_t sk;       // stride along k
+    int64_t rstride;  // offset between this thread's LPT elements
+    int kk0, kstep;   // k of element r: kk0 + kstep*r
+    int xi0, xstep;   // m/n (relative to the tile) of element r: xi0 + xstep*r
+    uint32_t xmask;   // element r lies inside the block along m/n
+};
+
+__device__ __forceinline__ void aff_init(AffView &v, const DevOperand &o, int x0, int X, int tid) {
+    // first = k (offset f0 + f1*k), second = m/n (offset s0 + s1*x)
+    v.sk = o.f1;
+    if (o.fast_first) {
+        v.kk0 = tid % ABK; v.kstep = 0;
+        v.xi0 = tid / ABK; v.xstep = NT / ABK;
+        v.rstride = o.s1 * (NT / ABK);
+    } else {
+        v.kk0 = tid / BM; v.kstep = NT / BM;
+        v.xi0 = tid % BM; v.xstep = 0;
+        v.rstride = o.f1 * (NT / BM);
+    }
+    v.base = o.f0 + o.s0 + o.f1 * (int64_t)v.kk0 + o.s1 * (int64_t)(x0 + v.xi0);
+    v.xmask = 0;
+#pragma unroll
+    for (int r = 0; r < ALPT; ++r) v.xmask |= (uint32_t)(x0 + v.xi0 + v.xstep * r < X) << r;
+}
+
+__device__ __forceinline__ void aff_load(const AffView &v, const double *__restrict__ p, int k0,
+                                         int k1, double out[ALPT]) {
+    const double *q = p + v.base + v.sk * (int64_t)k0;
+#pragma unroll
+    for (int r = 0; r < ALPT; ++r) {
+        const bool ok = ((v.xmask >> r) & 1u) && (k0 + v.kk0 + v.kstep * r < k1);
+        out[r] = ok ? q[v.rstride * r] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
+    const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
+    double *const *__restrict__ bufs) {
+    __shared__ double As[ABK][BM + 1];
+    __shared__ double Bs[ABK][BN + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int lrow = lane >> 4, lcol = lane & 15;
+
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const DevTask task = tasks[t];
+        const DevBlock &b = blocks[task.blk];
+        const int M = b.M, N = b.N;
+        const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
+        const double *__restrict__ pa = bufs[b.buf_a] + b.base_a;
+        const double *__restrict__ pb = bufs[b.buf_b] + b.base_b;
+        AffView va, vb;
+        aff_init(va, b.A, m0, M, tid);
+        aff_init(vb, b.B, n0, N, tid);
+
+        v4d acc[2][2];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+        double ra[ALPT], rb[ALPT];
+        aff_load(va, pa, task.k0, k1, ra);
+        aff_load(vb, pb, task.k0, k1, rb);
+        for (int k0 = task.k0; k0 < k1; k0 += ABK) {
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < ALPT; ++r) {
+                As[va.kk0 + va.kstep * r][va.xi0 + va.xstep * r] = ra[r];
+                Bs[vb.kk0 + vb.kstep * r][vb.xi0 + vb.xstep * r] = rb[r];
+            }
+            __syncthreads();
+            if (k0 + ABK < k1) {
+                aff_load(va, pa, k0 + ABK, k1, ra);
+                aff_load(vb, pb, k0 + ABK, k1, rb);
+            }
+#pragma unroll
+            for (int kq = 0; kq < ABK / 4; ++kq) {
+                const int kr = kq * 4 + lrow;
+                double a0 = As[kr][wm * 32 + lcol];
+                double a1 = As[kr][wm * 32 + 16 + lcol];
+                double b0 = Bs[kr][wn * 32 + lcol];
+                double b1 = Bs[kr][wn * 32 + 16 + lcol];
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            }
+        }
+
+        const DevOperand &oc = b.C;
+        double *pc = bufs[b.buf_c] + b.base_c + oc.f0 + oc.s0;
+        const double xfac = b.xfac;
+        const int atomic = b.atomic;
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+            const int n = n0 + wn * 32 + ni * 16 + lcol;
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
+                    if (m >= M || n >= N) continue;
+                    double *dst = pc + oc.f1 * (int64_t)m + oc.s1 * (int64_t)n;
+                    const double val = xfac * acc[mi][ni][r];
+                    if (atomic) unsafeAtomicAdd(dst, val);
+                    else *dst += val;
+                }
+        }
+    }
+}
+
 // Spin-flip symmetrisation of a result list (contract/sym_ab_blk.f:354-377):
 // for each precomputed pair (i1 <= i2, factor f = +/-relfac2):
 //   v = prefac*(x[i1] + f*x[i2]);  x[i1] = v;  x[i2] = f*v
@@ -224,6 +347,15 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::grouped_contract_f64, dim3(grid), dim3(gecco::NT), 0, stream,
                        blocks, tasks, ntasks, bufs);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
+                                           const gecco::DevTask *tasks, int ntasks,
+                                           double *const *bufs, int grid, hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    hipLaunchKernelGGL(gecco::grouped_contract_affine_f64, dim3(grid), dim3(gecco::NT), 0,
+                       stream, blocks, tasks, ntasks, bufs);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

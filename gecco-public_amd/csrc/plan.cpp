@@ -31,6 +31,9 @@
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, double *const *bufs, int grid,
                                     hipStream_t stream);
+extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
+                                           const gecco::DevTask *tasks, int ntasks,
+                                           double *const *bufs, int grid, hipStream_t stream);
 extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
                                    const int8_t *fsign, int64_t npair, double prefac,
                                    hipStream_t stream);
@@ -318,7 +321,7 @@ struct gecco_hip_plan {
     gecco::DevTask *d_tasks = nullptr;
     int32_t *d_index = nullptr;
     double **d_bufs = nullptr;
-    int64_t ntasks = 0;
+    int64_t ntasks = 0, ntasks_affine = 0;
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
@@ -482,7 +485,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     const int BMh = 64, BNh = 64;
     int64_t ntiles = 0;
     for (auto &b : p->blocks) ntiles += ((b.M + BMh - 1) / BMh) * ((b.N + BNh - 1) / BNh);
-    const int64_t want = 4096;
+    const int64_t want = 2048;
     int64_t split = ntiles > 0 ? (want + ntiles - 1) / ntiles : 1;
     std::vector<DevTask> tasks;
     std::vector<DevBlock> dblocks(p->blocks.size());
@@ -490,7 +493,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         auto &b = p->blocks[i];
         int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / 256));
         int64_t chunk = (b.K + ns - 1) / ns;
-        chunk = ((chunk + 15) / 16) * 16;
+        chunk = ((chunk + 31) / 32) * 32;
         ns = (b.K + chunk - 1) / chunk;
         DevBlock &d = dblocks[i];
         d.xfac = b.xfac;
@@ -512,10 +515,24 @@ static int plan_finalize(gecco_hip_plan *p) {
                     tasks.push_back({(int32_t)i, (int32_t)m0, (int32_t)n0, (int32_t)k0, (int32_t)k1});
                 }
     }
-    // largest tasks first (by K extent) for load balance
-    std::stable_sort(tasks.begin(), tasks.end(), [](const DevTask &x, const DevTask &y) {
+    // affine blocks (strided GEMMs) run in their own launch; within each
+    // class the largest tasks come first
+    std::vector<uint8_t> is_aff(p->blocks.size(), 0);
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        bool aff = true;
+        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
+            int64_t a0, a1;
+            aff = aff && !o->r.table && affine_of(o->r.first, a0, a1) && affine_of(o->r.second, a0, a1);
+        }
+        is_aff[i] = aff ? 1 : 0;
+    }
+    std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
+        if (is_aff[x.blk] != is_aff[y.blk]) return is_aff[x.blk] > is_aff[y.blk];
         return (x.k1 - x.k0) > (y.k1 - y.k0);
     });
+    int64_t naff = 0;
+    for (auto &t : tasks) naff += is_aff[t.blk];
     if (pool.empty()) pool.push_back(0);
     int32_t *d_index = nullptr;
     if (check(hipMalloc(&d_index, pool.size() * sizeof(int32_t)), "hipMalloc(index)")) return g_err;
@@ -548,6 +565,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (check(hipMalloc(&p->d_bufs, std::max<int64_t>(1, nb_max) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
+    p->ntasks_affine = naff;
     for (auto &y : p->syms) {
         y.npair = (int64_t)y.i1.size();
         if (!y.npair) continue;
@@ -569,7 +587,9 @@ static int plan_finalize(gecco_hip_plan *p) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
     }
-    p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)ncu * 8);
+    // one workgroup per task: the dispatcher balances uneven task sizes
+    p->grid = (int)std::min<int64_t>(p->ntasks, (int64_t)1 << 30);
+    (void)ncu;
     p->flops = flops;
     p->payload += payload;
     p->index_bytes += 4.0 * (double)pool.size();
@@ -597,7 +617,10 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
         p->last_bufs = want;
     }
-    if (p->ntasks && gecco_launch_grouped(p->d_blocks, p->d_tasks, (int)p->ntasks, p->d_bufs, p->grid, s))
+    const int64_t na = p->ntasks_affine, ng = p->ntasks - p->ntasks_affine;
+    if (na && gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na, s))
+        return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (ng && gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
         return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     for (auto &y : p->syms)
         if (y.npair && gecco_launch_sym_ab(bufs[y.buf], y.d_i1, y.d_i2, y.d_f, y.npair, y.prefac, s))
