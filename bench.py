@@ -87,7 +87,20 @@ def main():
     ap.add_argument('--nocc', type=int, default=NOCC)
     ap.add_argument('--nvir', type=int, default=NVIR)
     ap.add_argument('--nsym', type=int, default=NSYM)
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MB',
+                    help='only run a plain device copy of MB megabytes (calibrates '
+                         'FETCH_SIZE/WRITE_SIZE counters) and exit')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device='cuda:0')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({'error': 'this engine runs on one GPU; --gpus > 1 is not supported'}))
         sys.exit(2)
@@ -143,7 +156,7 @@ def main():
                    'tasks': st['ntasks']},
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
-                     'traffic': None, 'kernel': 'grouped_contract_f64',
+                     'traffic': None, 'kernel': 'grouped_contract_affine_f64',
                      'kernel_ms': round(kern_ms, 5)},
     }
     if not args.no_cpu:

@@ -204,52 +204,60 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 // Dense blocks whose three index maps are all affine (no forbidden
 // elements, all signs +, constant strides -- e.g. the particle-particle
 // ladder) are strided GEMMs: addresses are base + stride_k*k + stride_x*x.
-// Same tile and MFMA layout as above, K step ABK, and only a base offset, a
-// stride and a validity mask per operand and thread.
-constexpr int ABK = 32, ALPT = ABK * BM / NT;
-
+// Tile TBM x TBN, K step TBK, 256 threads = 2x2 waves, each wave owns
+// (TBM/2) x (TBN/2) = TM x TN MFMA 16x16 tiles.  LDS is double-buffered: one
+// barrier per K step; the global loads of step s+1 are in flight while the
+// MFMAs of step s run.
+template <int X, int TBK>
 struct AffView {
+    static constexpr int L = TBK * X / NT;  // elements per thread
     int64_t base;     // element offset of this thread's first element at k = 0
     int64_t sk;       // stride along k
-    int64_t rstride;  // offset between this thread's LPT elements
+    int64_t rstride;  // offset between this thread's elements
     int kk0, kstep;   // k of element r: kk0 + kstep*r
     int xi0, xstep;   // m/n (relative to the tile) of element r: xi0 + xstep*r
     uint32_t xmask;   // element r lies inside the block along m/n
+
+    __device__ __forceinline__ void init(const DevOperand &o, int x0, int Xn, int tid) {
+        sk = o.f1;
+        if (o.fast_first) {
+            kk0 = tid % TBK; kstep = 0;
+            xi0 = tid / TBK; xstep = NT / TBK;
+            rstride = o.s1 * (NT / TBK);
+        } else {
+            kk0 = tid / X; kstep = NT / X;
+            xi0 = tid % X; xstep = 0;
+            rstride = o.f1 * (NT / X);
+        }
+        base = o.f0 + o.s0 + o.f1 * (int64_t)kk0 + o.s1 * (int64_t)(x0 + xi0);
+        xmask = 0;
+#pragma unroll
+        for (int r = 0; r < L; ++r) xmask |= (uint32_t)(x0 + xi0 + xstep * r < Xn) << r;
+    }
+    __device__ __forceinline__ void load(const double *__restrict__ p, int k0, int k1,
+                                         double out[L]) const {
+        const double *q = p + base + sk * (int64_t)k0;
+#pragma unroll
+        for (int r = 0; r < L; ++r) {
+            const bool ok = ((xmask >> r) & 1u) && (k0 + kk0 + kstep * r < k1);
+            out[r] = ok ? q[rstride * r] : 0.0;
+        }
+    }
+    template <int LD>
+    __device__ __forceinline__ void store(double *lds, const double in[L]) const {
+#pragma unroll
+        for (int r = 0; r < L; ++r) lds[(kk0 + kstep * r) * LD + xi0 + xstep * r] = in[r];
+    }
 };
 
-__device__ __forceinline__ void aff_init(AffView &v, const DevOperand &o, int x0, int X, int tid) {
-    // first = k (offset f0 + f1*k), second = m/n (offset s0 + s1*x)
-    v.sk = o.f1;
-    if (o.fast_first) {
-        v.kk0 = tid % ABK; v.kstep = 0;
-        v.xi0 = tid / ABK; v.xstep = NT / ABK;
-        v.rstride = o.s1 * (NT / ABK);
-    } else {
-        v.kk0 = tid / BM; v.kstep = NT / BM;
-        v.xi0 = tid % BM; v.xstep = 0;
-        v.rstride = o.f1 * (NT / BM);
-    }
-    v.base = o.f0 + o.s0 + o.f1 * (int64_t)v.kk0 + o.s1 * (int64_t)(x0 + v.xi0);
-    v.xmask = 0;
-#pragma unroll
-    for (int r = 0; r < ALPT; ++r) v.xmask |= (uint32_t)(x0 + v.xi0 + v.xstep * r < X) << r;
-}
-
-__device__ __forceinline__ void aff_load(const AffView &v, const double *__restrict__ p, int k0,
-                                         int k1, double out[ALPT]) {
-    const double *q = p + v.base + v.sk * (int64_t)k0;
-#pragma unroll
-    for (int r = 0; r < ALPT; ++r) {
-        const bool ok = ((v.xmask >> r) & 1u) && (k0 + v.kk0 + v.kstep * r < k1);
-        out[r] = ok ? q[v.rstride * r] : 0.0;
-    }
-}
-
+template <int TBM, int TBN, int TBK>
 __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
     double *const *__restrict__ bufs) {
-    __shared__ double As[ABK][BM + 1];
-    __shared__ double Bs[ABK][BN + 1];
+    constexpr int LDA = TBM + 1, LDB = TBN + 1;
+    constexpr int TM = TBM / 32, TN = TBN / 32;
+    __shared__ double As[2][TBK * LDA];
+    __shared__ double Bs[2][TBK * LDB];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -263,41 +271,53 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
         const double *__restrict__ pa = bufs[b.buf_a] + b.base_a;
         const double *__restrict__ pb = bufs[b.buf_b] + b.base_b;
-        AffView va, vb;
-        aff_init(va, b.A, m0, M, tid);
-        aff_init(vb, b.B, n0, N, tid);
+        AffView<TBM, TBK> va;
+        AffView<TBN, TBK> vb;
+        va.init(b.A, m0, M, tid);
+        vb.init(b.B, n0, N, tid);
 
-        v4d acc[2][2];
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        v4d acc[TM][TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-        double ra[ALPT], rb[ALPT];
-        aff_load(va, pa, task.k0, k1, ra);
-        aff_load(vb, pb, task.k0, k1, rb);
-        for (int k0 = task.k0; k0 < k1; k0 += ABK) {
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < ALPT; ++r) {
-                As[va.kk0 + va.kstep * r][va.xi0 + va.xstep * r] = ra[r];
-                Bs[vb.kk0 + vb.kstep * r][vb.xi0 + vb.xstep * r] = rb[r];
+        double ra[AffView<TBM, TBK>::L], rb[AffView<TBN, TBK>::L];
+        __syncthreads();   // LDS of the previous task is no longer read
+        va.load(pa, task.k0, k1, ra);
+        vb.load(pb, task.k0, k1, rb);
+        va.template store<LDA>(As[0], ra);
+        vb.template store<LDB>(Bs[0], rb);
+        __syncthreads();
+        int cur = 0;
+        for (int k0 = task.k0; k0 < k1; k0 += TBK) {
+            const bool more = k0 + TBK < k1;
+            if (more) {
+                va.load(pa, k0 + TBK, k1, ra);
+                vb.load(pb, k0 + TBK, k1, rb);
             }
-            __syncthreads();
-            if (k0 + ABK < k1) {
-                aff_load(va, pa, k0 + ABK, k1, ra);
-                aff_load(vb, pb, k0 + ABK, k1, rb);
-            }
+            const double *A = As[cur];
+            const double *B = Bs[cur];
 #pragma unroll
-            for (int kq = 0; kq < ABK / 4; ++kq) {
+            for (int kq = 0; kq < TBK / 4; ++kq) {
                 const int kr = kq * 4 + lrow;
-                double a0 = As[kr][wm * 32 + lcol];
-                double a1 = As[kr][wm * 32 + 16 + lcol];
-                double b0 = Bs[kr][wn * 32 + lcol];
-                double b1 = Bs[kr][wn * 32 + 16 + lcol];
-                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                double a[TM], bb[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
             }
+            if (more) {
+                va.template store<LDA>(As[cur ^ 1], ra);
+                vb.template store<LDB>(Bs[cur ^ 1], rb);
+            }
+            __syncthreads();
+            cur ^= 1;
         }
 
         const DevOperand &oc = b.C;
@@ -305,16 +325,16 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         const double xfac = b.xfac;
         const int atomic = b.atomic;
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            const int n = n0 + wn * 32 + ni * 16 + lcol;
+        for (int j = 0; j < TN; ++j) {
+            const int n = n0 + wn * (TBN / 2) + j * 16 + lcol;
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
+                    const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
                     if (m >= M || n >= N) continue;
                     double *dst = pc + oc.f1 * (int64_t)m + oc.s1 * (int64_t)n;
-                    const double val = xfac * acc[mi][ni][r];
+                    const double val = xfac * acc[i][j][r];
                     if (atomic) unsafeAtomicAdd(dst, val);
                     else *dst += val;
                 }
@@ -350,12 +370,32 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
+// Tile variants of the affine kernel: {TBM, TBN, TBK}.
+extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
+    static const int shapes[][3] = {{64, 64, 32}, {64, 64, 16}, {64, 128, 16},
+                                    {128, 64, 16}, {128, 128, 16}, {128, 128, 8}};
+    if (v < 0 || v >= (int)(sizeof shapes / sizeof shapes[0])) return 1;
+    *bm = shapes[v][0];
+    *bn = shapes[v][1];
+    *bk = shapes[v][2];
+    return 0;
+}
+
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int grid, hipStream_t stream) {
+                                           double *const *bufs, int grid, int variant,
+                                           hipStream_t stream) {
+    using namespace gecco;
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::grouped_contract_affine_f64, dim3(grid), dim3(gecco::NT), 0,
-                       stream, blocks, tasks, ntasks, bufs);
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 32>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 1: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 2: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 3: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 4: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 5: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 8>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    default: return 1;
+    }
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -33,7 +34,9 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
                                     hipStream_t stream);
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int grid, hipStream_t stream);
+                                           double *const *bufs, int grid, int variant,
+                                           hipStream_t stream);
+extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
                                    const int8_t *fsign, int64_t npair, double prefac,
                                    hipStream_t stream);
@@ -322,6 +325,9 @@ struct gecco_hip_plan {
     int32_t *d_index = nullptr;
     double **d_bufs = nullptr;
     int64_t ntasks = 0, ntasks_affine = 0;
+    int affine_variant = 1;
+    int64_t target_tasks = 2048;
+    int64_t min_kchunk = 256;
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
@@ -481,17 +487,36 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
     }
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
-    // tiles and split-K
-    const int BMh = 64, BNh = 64;
+    // affine blocks (strided GEMMs) run in their own launch with the tile
+    // variant p->affine_variant; the others use the 64x64 gather kernel
+    std::vector<uint8_t> is_aff(p->blocks.size(), 0);
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        bool aff = true;
+        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
+            int64_t a0, a1;
+            aff = aff && !o->r.table && affine_of(o->r.first, a0, a1) && affine_of(o->r.second, a0, a1);
+        }
+        is_aff[i] = aff ? 1 : 0;
+    }
+    int abm = 64, abn = 64, abk = 32;
+    if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
+        return fail(GECCO_HIP_EINVAL, "unknown tile variant %d", p->affine_variant);
+    auto tile_m = [&](size_t i) { return is_aff[i] ? (int64_t)abm : (int64_t)64; };
+    auto tile_n = [&](size_t i) { return is_aff[i] ? (int64_t)abn : (int64_t)64; };
+    // split K so that the launch has about p->target_tasks tasks
     int64_t ntiles = 0;
-    for (auto &b : p->blocks) ntiles += ((b.M + BMh - 1) / BMh) * ((b.N + BNh - 1) / BNh);
-    const int64_t want = 2048;
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        ntiles += ((b.M + tile_m(i) - 1) / tile_m(i)) * ((b.N + tile_n(i) - 1) / tile_n(i));
+    }
+    const int64_t want = p->target_tasks;
     int64_t split = ntiles > 0 ? (want + ntiles - 1) / ntiles : 1;
     std::vector<DevTask> tasks;
     std::vector<DevBlock> dblocks(p->blocks.size());
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
-        int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / 256));
+        int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / p->min_kchunk));
         int64_t chunk = (b.K + ns - 1) / ns;
         chunk = ((chunk + 31) / 32) * 32;
         ns = (b.K + chunk - 1) / chunk;
@@ -508,24 +533,12 @@ static int plan_finalize(gecco_hip_plan *p) {
         d.K = (int32_t)b.K;
         d.atomic = (ns > 1 || !b.injective || overlap[i]) ? 1 : 0;
         d.pad = 0;
-        for (int64_t m0 = 0; m0 < b.M; m0 += BMh)
-            for (int64_t n0 = 0; n0 < b.N; n0 += BNh)
+        for (int64_t m0 = 0; m0 < b.M; m0 += tile_m(i))
+            for (int64_t n0 = 0; n0 < b.N; n0 += tile_n(i))
                 for (int64_t s = 0; s < ns; ++s) {
                     int64_t k0 = s * chunk, k1 = std::min(b.K, k0 + chunk);
                     tasks.push_back({(int32_t)i, (int32_t)m0, (int32_t)n0, (int32_t)k0, (int32_t)k1});
                 }
-    }
-    // affine blocks (strided GEMMs) run in their own launch; within each
-    // class the largest tasks come first
-    std::vector<uint8_t> is_aff(p->blocks.size(), 0);
-    for (size_t i = 0; i < p->blocks.size(); ++i) {
-        auto &b = p->blocks[i];
-        bool aff = true;
-        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
-            int64_t a0, a1;
-            aff = aff && !o->r.table && affine_of(o->r.first, a0, a1) && affine_of(o->r.second, a0, a1);
-        }
-        is_aff[i] = aff ? 1 : 0;
     }
     std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
         if (is_aff[x.blk] != is_aff[y.blk]) return is_aff[x.blk] > is_aff[y.blk];
@@ -618,7 +631,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         p->last_bufs = want;
     }
     const int64_t na = p->ntasks_affine, ng = p->ntasks - p->ntasks_affine;
-    if (na && gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na, s))
+    if (na && gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
+                                          p->affine_variant, s))
         return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if (ng && gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
         return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -798,6 +812,9 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     g_err = 0;
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan pointer");
     *plan = new gecco_hip_plan();
+    if (const char *e = getenv("GECCO_HIP_AFFINE_TILE")) (*plan)->affine_variant = atoi(e);
+    if (const char *e = getenv("GECCO_HIP_TARGET_TASKS")) (*plan)->target_tasks = std::max(1L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
     return 0;
 }
 

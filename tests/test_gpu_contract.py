@@ -145,3 +145,30 @@ def test_product_enumerator_matches_oracle(nsym, nocc, nvir, name, use_tr):
     plan.execute([d1, d2, dr])
     torch.cuda.synchronize()
     assert _relerr(dr.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize('name', ['ladder', 'hh_ladder', 'energy', 'ring'])
+def test_affine_tile_variants(variant, name, monkeypatch):
+    """Every tile variant of the strided-GEMM kernel gives the oracle result."""
+    torch = _torch()
+    import gecco_hip
+    monkeypatch.setenv('GECCO_HIP_AFFINE_TILE', str(variant))
+    monkeypatch.setenv('GECCO_HIP_MIN_KCHUNK', '32')
+    print(f'variant {variant} case {name}', flush=True)
+    orbs = orbitals(6, 21, 2)
+    c, a, b, r = CASES[name](orbs)
+    x1 = spin_symmetric(a, 71)
+    x2 = spin_symmetric(b, 72)
+    blocks = c.blocks(True)
+    ref = _oracle(c, x1, x2, r.length, blocks)
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+            torch.zeros(r.length, dtype=torch.float64, device=dev)]
+    plan = gecco_hip.Plan()
+    for bl in blocks:
+        plan.add(bl)
+    plan.finalize()
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL
