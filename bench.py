@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 NOCC, NVIR, NSYM = 20, 100, 4
 PEAK_FP64 = 78.6        # TFLOP/s, MI355X dense FP64 (vector and MFMA)
 PEAK_HBM = 8000.0       # GB/s
+# HBM bytes per launch of the ladder kernel at nocc=20 nvir=100 nsym=4, from
+# rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (copy-calibrated factors);
+# see profiles/grouped_contract_affine_f64_ccsd_ladder_o20_v100_s4_pmc.md
+MEASURED_TRAFFIC = {(20, 100, 4): 565.6e6}
 
 
 def round_robin(n, nsym, start=0):
@@ -156,7 +160,8 @@ def main():
                    'tasks': st['ntasks']},
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
-                     'traffic': None, 'kernel': 'grouped_contract_affine_f64',
+                     'traffic': MEASURED_TRAFFIC.get((args.nocc, args.nvir, args.nsym)),
+                     'kernel': 'grouped_contract_affine_f64',
                      'kernel_ms': round(kern_ms, 5)},
     }
     if not args.no_cpu:

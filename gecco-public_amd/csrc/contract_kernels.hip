@@ -127,6 +127,10 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         v4d acc[2][2];
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        // bit 0: second 16-row half of the wave tile overlaps the block,
+        // bit 1: second 16-column half does (the first halves always do or
+        // the whole wave tile is padding, whose results are discarded)
+        const uint32_t live = (uint32_t)(m0 + wm * 32 + 16 < M) | ((uint32_t)(n0 + wn * 32 + 16 < N) << 1);
 
         double ra[LPT], rb[LPT];
         uint32_t fa, fb;
@@ -152,9 +156,9 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                 double b0 = Bs[kr][wn * 32 + lcol];
                 double b1 = Bs[kr][wn * 32 + 16 + lcol];
                 acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                if (live & 2u) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                if (live & 1u) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                if (live == 3u) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
             }
         }
 
@@ -281,6 +285,13 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        // 16-row / 16-column MFMA sub-tiles of this wave that overlap the
+        // block (wave-uniform); the others are padding and are skipped
+        uint32_t live_m = 0, live_n = 0;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) live_m |= (uint32_t)(m0 + wm * (TBM / 2) + i * 16 < M) << i;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) live_n |= (uint32_t)(n0 + wn * (TBN / 2) + j * 16 < N) << j;
 
         double ra[AffView<TBM, TBK>::L], rb[AffView<TBN, TBK>::L];
         __syncthreads();   // LDS of the previous task is no longer read
@@ -310,7 +321,8 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int j = 0; j < TN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+                        if (((live_m >> i) & (live_n >> j)) & 1u)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
             }
             if (more) {
                 va.template store<LDA>(As[cur ^ 1], ra);
