@@ -904,6 +904,49 @@ int gecco_hip_list_dist(const gecco_hip_list *L, int64_t i, int64_t *info, int64
     return 0;
 }
 
+int gecco_hip_list_spinorbs(const gecco_hip_list *L, int16_t *out, int64_t nout) {
+    // per element: spin-orbital index so = 2*(orb-1)+spin of every electron,
+    // C components then A components (component order of the list), each
+    // component's string in ascending order
+    if (!L || !out) return set_error(GECCO_HIP_EINVAL, "list_spinorbs: null argument");
+    const int nel = L->nc + L->na;
+    if (nout < L->length * (int64_t)nel) return set_error(GECCO_HIP_EINVAL, "list_spinorbs: buffer too small");
+    gecco_hip_orbs *o = L->orbs;
+    const int nsym = o->nsym;
+    const size_t ncb = L->oc.size();
+    for (const Dist &d : L->dists) {
+        std::vector<const GraphBlock *> gb;
+        std::vector<int> occ, hp;
+        for (size_t k = 0; k < ncb; ++k) {
+            gb.push_back(&graph(o, L->hc[k], L->oc[k]).blk[(size_t)(d.imc[k] - 1) * nsym + d.gc[k] - 1]);
+            occ.push_back(L->oc[k]);
+        }
+        for (size_t k = 0; k < L->oa.size(); ++k) {
+            gb.push_back(&graph(o, L->ha[k], L->oa[k]).blk[(size_t)(d.ima[k] - 1) * nsym + d.ga[k] - 1]);
+            occ.push_back(L->oa[k]);
+        }
+        std::vector<int64_t> ld(d.ldc);
+        ld.insert(ld.end(), d.lda.begin(), d.lda.end());
+        const size_t nc = gb.size();
+        std::vector<int64_t> idx(nc, 0);
+        for (int64_t e = 0; e < d.len; ++e) {
+            int64_t off = d.off;
+            for (size_t c = 0; c < nc; ++c) off += idx[c] * ld[c];
+            int16_t *dst = out + off * nel;
+            uint16_t so[8];
+            for (size_t c = 0; c < nc; ++c) {
+                unpack(gb[c]->strs[idx[c]], occ[c], so);
+                for (int q = 0; q < occ[c]; ++q) *dst++ = (int16_t)so[q];
+            }
+            for (size_t c = 0; c < nc; ++c) {   // odometer, first component fastest
+                if (++idx[c] < (int64_t)gb[c]->strs.size()) break;
+                idx[c] = 0;
+            }
+        }
+    }
+    return 0;
+}
+
 int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *op1,
                                    const gecco_hip_list *op2, const gecco_hip_list *res,
                                    const int64_t *cnt_c, const int64_t *cnt_a, double fact,

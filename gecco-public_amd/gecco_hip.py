@@ -210,7 +210,7 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_orbs_strmap', 'gecco_hip_orbs_flipmap', 'gecco_hip_list_create',
                  'gecco_hip_list_destroy', 'gecco_hip_list_length', 'gecco_hip_list_ndist',
                  'gecco_hip_list_dist', 'gecco_hip_plan_add_contraction',
-                 'gecco_hip_plan_add_sym_ab')
+                 'gecco_hip_plan_add_sym_ab', 'gecco_hip_list_spinorbs')
 VP = ctypes.c_void_p
 _lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
 _lib.gecco_hip_orbs_create.restype = ctypes.c_int
@@ -238,6 +238,8 @@ _lib.gecco_hip_plan_add_contraction.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_sym_ab.argtypes = [VP, VP, ctypes.c_int32, I64, ctypes.c_double,
                                            ctypes.c_double]
 _lib.gecco_hip_plan_add_sym_ab.restype = ctypes.c_int
+_lib.gecco_hip_list_spinorbs.argtypes = [VP, ctypes.POINTER(ctypes.c_int16), I64]
+_lib.gecco_hip_list_spinorbs.restype = ctypes.c_int
 
 HOLE, PART = 1, 2
 
@@ -314,6 +316,26 @@ class MEList:
             out.append(dict(off=off, len=ln, msa_tot=msa, igama=ga, comps_c=comps[:ncc],
                             comps_a=comps[ncc:]))
         return out
+
+    @property
+    def nel(self):
+        return sum(self.occ_c.values()) + sum(self.occ_a.values())
+
+    def spinorbs(self):
+        """(length, nel) int16 array of the spin-orbitals of every element."""
+        out = np.zeros((self.length, max(1, self.nel)), dtype=np.int16)
+        _check(_lib.gecco_hip_list_spinorbs(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+                                            out.size))
+        return out[:, :self.nel]
+
+    def comp_spaces(self):
+        """Space (1=H, 2=P) of each electron column of spinorbs()."""
+        from itertools import chain
+        cols = []
+        for side in (self.occ_c, self.occ_a):
+            for h in (4, 2, 3, 1):
+                cols += [h] * side.get(h, 0)
+        return cols
 
     def __del__(self):
         try:

@@ -56,6 +56,12 @@ int64_t gecco_hip_list_ndist(const gecco_hip_list *list);
  *         per C component (ms, idxms, irrep, nstr, ldim), then per A comp]. */
 int gecco_hip_list_dist(const gecco_hip_list *list, int64_t idist, int64_t *info, int64_t ninfo);
 
+/* Spin-orbitals of every element (integral import, denominators):
+ * out[elem*(nc+na) + q] = 2*(orb-1)+spin (spin 0 alpha, 1 beta) of electron q,
+ * C components then A components, each string ascending; orbitals numbered
+ * inside their space.  nout >= length*(nc+na). */
+int gecco_hip_list_spinorbs(const gecco_hip_list *list, int16_t *out, int64_t nout);
+
 /* Record every kernel call of  Res += fact * Op1 . Op2  (contracted part of
  * Op1's C side cnt_c[4], of Op1's A side cnt_a[4], per space type) into the
  * plan, with spin-flip skipping and factors when use_tr (the use_tr keyword,
