@@ -210,10 +210,12 @@ class CCSD:
         buf['T2'].zero_()
         Dcat = torch.cat([D['T1'], D['T2']])
         hist_x, hist_e = [], []
+        bmat = np.zeros((ndiis, ndiis))     # DIIS overlaps <e_i|e_j>, updated by rows
         energy = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         it = 0
+        rnorm = float('inf')
         for it in range(1, maxit + 1):
             self.evaluate(torch.cuda.current_stream().cuda_stream)
             t = torch.cat([buf['T1'], buf['T2']])
@@ -228,22 +230,28 @@ class CCSD:
             if rnorm < tol:
                 break
             step = omega / Dcat
-            hist_x.append(t + step)
-            hist_e.append(step)
-            if len(hist_x) > ndiis:
+            if len(hist_x) == ndiis:
                 hist_x.pop(0)
                 hist_e.pop(0)
+                bmat[:-1, :-1] = bmat[1:, 1:].copy()
+            hist_x.append(t + step)
+            hist_e.append(step)
             m = len(hist_x)
-            E = torch.stack(hist_e)
+            # new row of the overlap matrix: elementwise products and
+            # reductions (a library GEMM with K = vector length and M,N <= 8
+            # runs without K-splitting and takes hundreds of ms)
+            row = torch.stack([(e * step).sum() for e in hist_e]).cpu().numpy()
+            bmat[m - 1, :m] = row
+            bmat[:m, m - 1] = row
             Bm = np.zeros((m + 1, m + 1))
-            Bm[:m, :m] = (E @ E.T).cpu().numpy()
+            Bm[:m, :m] = bmat[:m, :m]
             Bm[m, :m] = Bm[:m, m] = -1.0
             rhs_d = np.zeros(m + 1)
             rhs_d[m] = -1.0
             c = np.linalg.solve(Bm, rhs_d)[:m]
-            new = torch.zeros_like(t)
-            for ci, xi in zip(c, hist_x):
-                new += float(ci) * xi
+            new = hist_x[-1] * float(c[-1])
+            for ci, xi in zip(c[:-1], hist_x[:-1]):
+                new.add_(xi, alpha=float(ci))
             buf['T1'].copy_(new[:n1])
             buf['T2'].copy_(new[n1:])
         torch.cuda.synchronize()

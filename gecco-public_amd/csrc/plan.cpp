@@ -328,6 +328,7 @@ struct gecco_hip_plan {
     int affine_variant = 1;
     int64_t target_tasks = 2048;
     int64_t min_kchunk = 256;
+    int64_t max_kchunk = 4096;
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
@@ -516,7 +517,10 @@ static int plan_finalize(gecco_hip_plan *p) {
     std::vector<DevBlock> dblocks(p->blocks.size());
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
+        // parallel split for the whole plan, and a bound on the serial K
+        // extent of one task (rank-reducing blocks, e.g. full contractions)
         int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / p->min_kchunk));
+        ns = std::max<int64_t>(ns, (b.K + p->max_kchunk - 1) / p->max_kchunk);
         int64_t chunk = (b.K + ns - 1) / ns;
         chunk = ((chunk + 31) / 32) * 32;
         ns = (b.K + chunk - 1) / chunk;
@@ -815,6 +819,7 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     if (const char *e = getenv("GECCO_HIP_AFFINE_TILE")) (*plan)->affine_variant = atoi(e);
     if (const char *e = getenv("GECCO_HIP_TARGET_TASKS")) (*plan)->target_tasks = std::max(1L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_MAX_KCHUNK")) (*plan)->max_kchunk = std::max(64L, atol(e));
     return 0;
 }
 
@@ -863,6 +868,25 @@ int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops, int64_t *nbl
     if (ntasks) *ntasks = plan->ntasks;
     if (payload_bytes) *payload_bytes = plan->payload;
     if (index_bytes) *index_bytes = plan->index_bytes;
+    return 0;
+}
+
+int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *info, int64_t ninfo) {
+    // info = [M, N, K, swap, injective, A.table, B.table, C.table,
+    //         A.fast_first, B.fast_first, A affine (first,second), B affine, C affine]
+    if (!plan || i < 0 || i >= (int64_t)plan->blocks.size() || ninfo < 16)
+        return fail(GECCO_HIP_EINVAL, "plan_block_info: bad arguments");
+    const HostBlock &b = plan->blocks[i];
+    info[0] = b.M; info[1] = b.N; info[2] = b.K; info[3] = b.swap; info[4] = b.injective;
+    info[5] = b.A.r.table; info[6] = b.B.r.table; info[7] = b.C.r.table;
+    info[8] = b.A.fast_first; info[9] = b.B.fast_first;
+    int64_t a0, a1;
+    info[10] = !b.A.r.table && affine_of(b.A.r.first, a0, a1);
+    info[11] = !b.A.r.table && affine_of(b.A.r.second, a0, a1);
+    info[12] = !b.B.r.table && affine_of(b.B.r.first, a0, a1);
+    info[13] = !b.B.r.table && affine_of(b.B.r.second, a0, a1);
+    info[14] = !b.C.r.table && affine_of(b.C.r.first, a0, a1);
+    info[15] = !b.C.r.table && affine_of(b.C.r.second, a0, a1);
     return 0;
 }
 
