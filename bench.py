@@ -1,15 +1,21 @@
 """Benchmark of the MI355X block-contraction engine.
 
-Workload: the particle-particle ladder of closed-shell CCSD,
-    R(ab,ij) += sum_{c<d} V(ab,cd) T(cd,ij),
-on the CCSD Hamiltonian of nocc=20, nvir=100, 4 IRREPs assigned
-round-robin, GeCCo Ms/IRREP-blocked ME-lists, spin-flip symmetry on
-(use_tr), FP64.  One step = one grouped launch of every kernel call of the
-contraction (8 blocks, 5.58 GFLOP) followed by the spin-flip
-symmetrisation of R, operands resident in HBM.
+Default workload (--workload ccsd): closed-shell CCSD amplitude iterations on
+the synthetic Hamiltonian nocc=20, nvir=100, naux=300, 4 IRREPs
+(round-robin), eps_occ ~ U[-2,-0.3], eps_vir ~ U[0.2,3.0], B ~ N(0,0.03^2),
+GeCCo Ms/IRREP-blocked ME-lists, spin-flip symmetry on, FP64.  One step =
+one iteration: the whole CCSD formula (33 binary contractions, 4632 kernel
+calls of GeCCo's enumerator, 24.9 GFLOP, three grouped launches plus
+spin-flip symmetrisations) and the preconditioned DIIS update, operands
+resident in HBM.  After the timed steps a fresh solve runs to
+||Omega|| < 1e-8 and its energy is compared with the oracle's FP64 energy
+(tests/golden/ccsd_o20_v100_s4.json) when that fixture matches the recipe.
 
-Prints one JSON line (see --help).  --gpus N > 1 is not supported: the
-engine runs on one GPU here.
+--workload ladder: the particle-particle ladder contraction alone (8 kernel
+calls, 5.58 GFLOP) at the same sizes.
+
+Prints one JSON line.  --gpus N > 1 is not supported: the engine runs on one
+GPU here.
 """
 import argparse
 import json
@@ -22,20 +28,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'gecco-public_amd'))
 sys.path.insert(0, ROOT)
 
-NOCC, NVIR, NSYM = 20, 100, 4
+NOCC, NVIR, NAUX, NSYM, SEED, SIGMA = 20, 100, 300, 4, 0, 0.03
 PEAK_FP64 = 78.6        # TFLOP/s, MI355X dense FP64 (vector and MFMA)
 PEAK_HBM = 8000.0       # GB/s
+METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
 # HBM bytes per launch of the ladder kernel at nocc=20 nvir=100 nsym=4, from
 # rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (copy-calibrated factors);
 # see profiles/grouped_contract_affine_f64_ccsd_ladder_o20_v100_s4_pmc.md
-MEASURED_TRAFFIC = {(20, 100, 4): 565.6e6}
+MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 565.6e6}
+GOLDEN = os.path.join(ROOT, 'tests', 'golden', 'ccsd_o20_v100_s4.json')
 
 
 def round_robin(n, nsym, start=0):
     return [((start + k) % nsym) + 1 for k in range(n)]
 
 
-def build(nocc, nvir, nsym):
+# ------------------------------------------------------------------- ladder
+def build_ladder(nocc, nvir, nsym):
     import gecco_hip
     o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
     V = gecco_hip.MEList(o, {2: 2}, {2: 2})
@@ -47,7 +56,6 @@ def build(nocc, nvir, nsym):
     ps = gecco_hip.Plan()
     ps.add_sym_ab(R, 2)
     ps.finalize()
-    # spin-flip symmetrisation plans for the synthetic inputs
     pv = gecco_hip.Plan()
     pv.add_sym_ab(V, 0)
     pv.finalize()
@@ -57,9 +65,9 @@ def build(nocc, nvir, nsym):
     return dict(o=o, V=V, T=T, R=R, pc=pc, ps=ps, pv=pv, pt=pt, nblocks=nb, flops=fl)
 
 
-def cpu_baseline(nocc, nvir, nsym):
-    """Oracle route 3 (collect, dgemm as a plain loop, scatter; single core)
-    on the same contraction; returns (TFLOP/s, seconds, sample description)."""
+def cpu_ladder(nocc, nvir, nsym):
+    """Oracle route 3 (collect, dgemm as a plain loop, scatter; one core) on
+    one full ladder step."""
     import numpy as np
     from oracle.melist import Orbitals, MEList
     from oracle.contract import Contraction
@@ -73,44 +81,18 @@ def cpu_baseline(nocc, nvir, nsym):
     xv = rng.standard_normal(V.length)
     xt = rng.standard_normal(T.length)
     xr = np.zeros(R.length)
-    sample = blocks  # one full step: every kernel call of the contraction
-    fl = sum(2.0 * np.prod(b['lstr_ex1']) * np.prod(b['lstr_ex2']) * np.prod(b['lstr_cnt']) for b in sample)
+    fl = sum(2.0 * np.prod(b['lstr_ex1']) * np.prod(b['lstr_ex2']) * np.prod(b['lstr_cnt']) for b in blocks)
     t0 = time.perf_counter()
-    c.run(xv, xt, xr, route=3, use_tr=False, screen=False, blocks=sample)
+    c.run(xv, xt, xr, route=3, use_tr=False, screen=False, blocks=blocks)
     dt = time.perf_counter() - t0
-    return fl / dt / 1e12, dt, (f'one full step: {len(sample)} of {len(blocks)} kernel calls '
-                                f'({fl / 1e9:.2f} GFLOP), oracle route 3 with a plain-loop dgemm')
+    return fl / dt / 1e12, dt, (f'one full step: {len(blocks)} kernel calls ({fl / 1e9:.2f} GFLOP), '
+                                'oracle route 3 with a plain-loop dgemm')
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=50)
-    ap.add_argument('--warmup', type=int, default=5)
-    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--nocc', type=int, default=NOCC)
-    ap.add_argument('--nvir', type=int, default=NVIR)
-    ap.add_argument('--nsym', type=int, default=NSYM)
-    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MB',
-                    help='only run a plain device copy of MB megabytes (calibrates '
-                         'FETCH_SIZE/WRITE_SIZE counters) and exit')
-    args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device='cuda:0')
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
-        return
-    if args.gpus != 1:
-        print(json.dumps({'error': 'this engine runs on one GPU; --gpus > 1 is not supported'}))
-        sys.exit(2)
+def run_ladder(args):
     import torch
     dev = torch.device('cuda:0')
-    w = build(args.nocc, args.nvir, args.nsym)
+    w = build_ladder(args.nocc, args.nvir, args.nsym)
     g = torch.Generator(device=dev)
     g.manual_seed(0)
     xv = torch.randn(w['V'].length, dtype=torch.float64, device=dev, generator=g) * 0.05
@@ -125,50 +107,180 @@ def main():
         w['pc'].execute(bufs, sh)
         w['ps'].execute(bufs, sh)
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
+    w['pc'].set_timing(True)
+    w['pc'].timing()
     t0 = time.perf_counter()
-    for e0, e1 in ev:
-        e0.record(stream)
+    for _ in range(args.steps):
         w['pc'].execute(bufs, sh)
-        e1.record(stream)
         w['ps'].execute(bufs, sh)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev)
-    ms_step = wall * 1e3 / args.steps
+    tm = w['pc'].timing()
+    kern_ms = tm['ms'][0] / max(1, tm['launches'][0])
     flops = w['flops']
-    st = w['pc'].stats()
-    achieved = flops / (kern_ms * 1e-3) / 1e12
-    value = flops * args.steps / wall / 1e12
+    achieved = tm['flops'][0] / (kern_ms * 1e-3) / 1e12
     out = {
-        'metric': 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh',
-        'value': round(value, 4),
-        'unit': 'TFLOP/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': round(ms_step, 5),
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f64',
+        'metric': METRIC, 'value': round(flops * args.steps / wall / 1e12, 4), 'unit': 'TFLOP/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / args.steps, 5), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic (random spin-flip-symmetric V and T in GeCCo ME-list layout)',
-        'config': {'workload': 'ccsd_pp_ladder_contraction', 'nocc': args.nocc,
-                   'nvir': args.nvir, 'nsym': args.nsym, 'use_tr': True,
-                   'kernel_calls': w['nblocks'], 'gflop_per_step': round(flops / 1e9, 4),
-                   'tasks': st['ntasks']},
+        'config': {'workload': 'ccsd_pp_ladder_contraction', 'nocc': args.nocc, 'nvir': args.nvir,
+                   'nsym': args.nsym, 'use_tr': True, 'kernel_calls': w['nblocks'],
+                   'gflop_per_step': round(flops / 1e9, 4), 'tasks': w['pc'].stats()['ntasks']},
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
-                     'traffic': MEASURED_TRAFFIC.get((args.nocc, args.nvir, args.nsym)),
-                     'kernel': 'grouped_contract_affine_f64',
-                     'kernel_ms': round(kern_ms, 5)},
+                     'traffic': MEASURED_TRAFFIC.get(('ladder', args.nocc, args.nvir, args.nsym)),
+                     'kernel': 'grouped_contract_affine_f64', 'kernel_ms': round(kern_ms, 5)},
     }
     if not args.no_cpu:
-        v, dt, sample = cpu_baseline(args.nocc, args.nvir, args.nsym)
-        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'TFLOP/s', 'cores': 1,
-                               'kind': 'port', 'sample': sample, 'seconds': round(dt, 2),
+        v, dt, sample = cpu_ladder(args.nocc, args.nvir, args.nsym)
+        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'TFLOP/s', 'cores': 1, 'kind': 'port',
+                               'sample': sample, 'seconds': round(dt, 2),
                                'cpu': platform.processor() or platform.machine()}
+    return out
+
+
+# --------------------------------------------------------------------- ccsd
+def cpu_ccsd_sample(ham, flops_iter):
+    """Oracle route 3 (one core) on the last level of the CCSD formula (the
+    residual contractions, about half of an iteration's FLOP); returns the
+    per-iteration rate scaled by FLOP, the seconds and a description."""
+    import numpy as np
+    import ccsd
+    from oracle.melist import Orbitals, MEList
+    from oracle.contract import Contraction
+    orbs = Orbitals(ham.irrep_o, ham.irrep_v, ham.nsym)
+    L = {k: MEList(orbs, c, a) for k, (c, a) in ccsd.LISTS.items()}
+    rng = np.random.default_rng(2)
+    X = {k: 0.01 * rng.standard_normal(L[k].length) for k in L}
+    terms = []
+    fl = 0.0
+    for res, a, b, cc, ca, fact in ccsd.LEVELS[-1]:
+        c = Contraction(L[a], L[b], L[res], cc, ca, fact)
+        blocks = c.blocks(True)
+        fl += sum(2.0 * np.prod(x['lstr_ex1']) * np.prod(x['lstr_ex2']) * np.prod(x['lstr_cnt'])
+                  for x in blocks)
+        terms.append((res, a, b, c, blocks))
+    t0 = time.perf_counter()
+    for res, a, b, c, blocks in terms:
+        c.run(X[a], X[b], X[res], route=3, use_tr=False, screen=False, blocks=blocks)
+    dt = time.perf_counter() - t0
+    per_iter = dt * flops_iter / fl
+    return 1.0 / per_iter, dt, (f'the {len(terms)} residual contractions of the last formula level '
+                                f'({fl / 1e9:.2f} of {flops_iter / 1e9:.2f} GFLOP per iteration), oracle '
+                                'route 3 with a plain-loop dgemm, scaled to one iteration by FLOP')
+
+
+def run_ccsd(args):
+    import torch
+    import ccsd
+    from synth import synthetic
+    ham = synthetic(args.nocc, args.nvir, args.naux, nsym=args.nsym, seed=args.seed, sigma=args.sigma)
+    t0 = time.perf_counter()
+    solver = ccsd.CCSD(ham)
+    setup_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    # warmup iterations from T = 0, then the timed steps continue the solve
+    solver.solve(tol=0.0, maxit=args.warmup)
+    for p in solver.plans:
+        p.set_timing(True)
+        p.timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solve(tol=0.0, maxit=args.steps, restart=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [0.0, 0.0, 0.0]
+    nl = [0, 0, 0]
+    fl = [0.0, 0.0]
+    for p in solver.plans:
+        tm = p.timing()
+        p.set_timing(False)
+        for i in range(3):
+            ms[i] += tm['ms'][i]
+            nl[i] += tm['launches'][i]
+        fl[0] += tm['flops'][0]
+        fl[1] += tm['flops'][1]
+    # dominant contraction kernel by time
+    dom = 0 if ms[0] >= ms[1] else 1
+    names = ['grouped_contract_affine_f64', 'grouped_contract_f64']
+    dom_ms = ms[dom] / max(1, nl[dom])
+    achieved = fl[dom] / (ms[dom] / args.steps * 1e-3) / 1e12 if ms[dom] > 0 else 0.0
+    # time to convergence from T = 0
+    torch.cuda.synchronize()
+    res = solver.solve(tol=args.tol, maxit=200)
+    ref = None
+    if os.path.exists(GOLDEN):
+        g = json.load(open(GOLDEN))
+        rec = g['recipe']
+        if (rec['nocc'], rec['nvir'], rec['naux'], rec['nsym'], rec['seed'], rec['sigma']) == \
+                (args.nocc, args.nvir, args.naux, args.nsym, args.seed, args.sigma):
+            ref = g['energy']
+    out = {
+        'metric': METRIC, 'value': round(args.steps / wall, 3), 'unit': 'iters/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / args.steps, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': f'synthetic DF Hamiltonian (seed {args.seed}, naux {args.naux}, sigma {args.sigma})',
+        'config': {'workload': 'ccsd_iteration', 'nocc': args.nocc, 'nvir': args.nvir,
+                   'naux': args.naux, 'nsym': args.nsym, 'use_tr': True,
+                   'kernel_calls_per_iter': solver.nblocks,
+                   'gflop_per_iter': round(solver.flops / 1e9, 3)},
+        'contraction_tflops': round(solver.flops * args.steps / wall / 1e12, 3),
+        'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
+                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
+                                'sym_ab_pairs_f64': round(ms[2] / args.steps, 4)},
+        'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
+                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4), 'traffic': None,
+                     'kernel': names[dom], 'kernel_ms': round(dom_ms, 5)},
+        'convergence': {'tol': args.tol, 'iterations': res['iterations'],
+                        'seconds': round(res['seconds'], 4), 'energy': res['energy'],
+                        'residual': res['residual'],
+                        'oracle_energy': ref,
+                        'abs_dE': None if ref is None else abs(res['energy'] - ref)},
+        'setup_seconds': round(setup_s, 2),
+    }
+    if not args.no_cpu:
+        v, dt, sample = cpu_ccsd_sample(ham, solver.flops)
+        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
+                               'sample': sample, 'seconds': round(dt, 2),
+                               'cpu': platform.processor() or platform.machine()}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=30)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', choices=['ccsd', 'ladder'], default='ccsd')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--nocc', type=int, default=NOCC)
+    ap.add_argument('--nvir', type=int, default=NVIR)
+    ap.add_argument('--naux', type=int, default=NAUX)
+    ap.add_argument('--nsym', type=int, default=NSYM)
+    ap.add_argument('--seed', type=int, default=SEED)
+    ap.add_argument('--sigma', type=float, default=SIGMA)
+    ap.add_argument('--tol', type=float, default=1e-8)
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MB',
+                    help='only run a plain device copy of MB megabytes (calibrates '
+                         'FETCH_SIZE/WRITE_SIZE counters) and exit')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this engine runs on one GPU; --gpus > 1 is not supported'}))
+        sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device='cuda:0')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
+        return
+    out = run_ladder(args) if args.workload == 'ladder' else run_ccsd(args)
     print(json.dumps(out), flush=True)
 
 

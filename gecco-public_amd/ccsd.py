@@ -200,17 +200,20 @@ class CCSD:
         for p in self.plans:
             p.execute(self.bufs, sh)
 
-    def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None):
-        """Iterate to ||Omega|| < tol (Omega = rhs - D*t over T1 and T2).
-        Returns a dict with energy, iterations, residual norm and timings."""
+    def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True):
+        """Iterate to ||Omega|| < tol (Omega = rhs - D*t over T1 and T2), at
+        most maxit iterations.  restart: start from T = 0 with an empty DIIS
+        space; otherwise continue the previous solve.  Returns a dict with
+        energy, iterations, residual norm and timings."""
         torch = self.torch
         buf, D = self.buf, self.D
         n1 = self.L['T1'].length
-        buf['T1'].zero_()
-        buf['T2'].zero_()
         Dcat = torch.cat([D['T1'], D['T2']])
-        hist_x, hist_e = [], []
-        bmat = np.zeros((ndiis, ndiis))     # DIIS overlaps <e_i|e_j>, updated by rows
+        if restart or not hasattr(self, '_diis'):
+            buf['T1'].zero_()
+            buf['T2'].zero_()
+            self._diis = ([], [], np.zeros((ndiis, ndiis)))
+        hist_x, hist_e, bmat = self._diis   # DIIS overlaps <e_i|e_j>, updated by rows
         energy = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()

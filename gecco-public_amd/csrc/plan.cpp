@@ -333,6 +333,13 @@ struct gecco_hip_plan {
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
     std::vector<double *> last_bufs;
+    // optional HIP-event timing of the launches of execute (per launch class:
+    // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation)
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double ms[3] = {0, 0, 0};
+    int64_t launches[3] = {0, 0, 0};
+    double flops_affine = 0, flops_general = 0;
 };
 
 namespace gecco {
@@ -550,6 +557,11 @@ static int plan_finalize(gecco_hip_plan *p) {
     });
     int64_t naff = 0;
     for (auto &t : tasks) naff += is_aff[t.blk];
+    p->flops_affine = p->flops_general = 0;
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        const double f = 2.0 * (double)p->blocks[i].M * (double)p->blocks[i].N * (double)p->blocks[i].K;
+        (is_aff[i] ? p->flops_affine : p->flops_general) += f;
+    }
     if (pool.empty()) pool.push_back(0);
     int32_t *d_index = nullptr;
     if (check(hipMalloc(&d_index, pool.size() * sizeof(int32_t)), "hipMalloc(index)")) return g_err;
@@ -635,14 +647,34 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         p->last_bufs = want;
     }
     const int64_t na = p->ntasks_affine, ng = p->ntasks - p->ntasks_affine;
-    if (na && gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
-                                          p->affine_variant, s))
-        return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-    if (ng && gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
-        return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    auto mark = [&](int cls, bool begin) -> int {
+        if (!p->timing) return 0;
+        hipEvent_t e;
+        if (check(hipEventCreate(&e), "hipEventCreate") || check(hipEventRecord(e, s), "hipEventRecord")) return g_err;
+        if (begin) p->pending.push_back({cls, {e, nullptr}});
+        else p->pending.back().second.second = e;
+        return 0;
+    };
+    if (na) {
+        if (mark(0, true)) return g_err;
+        if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
+                                        p->affine_variant, s))
+            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (mark(0, false)) return g_err;
+    }
+    if (ng) {
+        if (mark(1, true)) return g_err;
+        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
+            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (mark(1, false)) return g_err;
+    }
+    bool anysym = false;
+    for (auto &y : p->syms) anysym |= y.npair > 0;
+    if (anysym && mark(2, true)) return g_err;
     for (auto &y : p->syms)
         if (y.npair && gecco_launch_sym_ab(bufs[y.buf], y.d_i1, y.d_i2, y.d_f, y.npair, y.prefac, s))
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
+    if (anysym && mark(2, false)) return g_err;
     return ok();
 }
 
@@ -887,6 +919,38 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *in
     info[13] = !b.B.r.table && affine_of(b.B.r.second, a0, a1);
     info[14] = !b.C.r.table && affine_of(b.C.r.first, a0, a1);
     info[15] = !b.C.r.table && affine_of(b.C.r.second, a0, a1);
+    return 0;
+}
+
+int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    plan->timing = on != 0;
+    return 0;
+}
+
+int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops) {
+    // ms[3], launches[3]: accumulated times and counts of the strided-GEMM
+    // launches, the gather launches and the symmetrisation launches since the
+    // last reset (waits for the recorded events); flops[2]: per-execute FLOP
+    // of the two contraction kernels.  Resets the accumulators.
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    for (auto &pe : plan->pending) {
+        float t = 0.f;
+        if (check(hipEventSynchronize(pe.second.second), "hipEventSynchronize")) return g_err;
+        if (check(hipEventElapsedTime(&t, pe.second.first, pe.second.second), "hipEventElapsedTime")) return g_err;
+        plan->ms[pe.first] += t;
+        plan->launches[pe.first] += 1;
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    plan->pending.clear();
+    for (int i = 0; i < 3; ++i) {
+        if (ms) ms[i] = plan->ms[i];
+        if (launches) launches[i] = plan->launches[i];
+        plan->ms[i] = 0;
+        plan->launches[i] = 0;
+    }
+    if (flops) { flops[0] = plan->flops_affine; flops[1] = plan->flops_general; }
     return 0;
 }
 

@@ -65,7 +65,8 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_add_block', 'gecco_hip_plan_finalize',
              'gecco_hip_plan_execute', 'gecco_hip_plan_stats', 'gecco_hip_malloc',
              'gecco_hip_free', 'gecco_hip_memcpy_htod', 'gecco_hip_memcpy_dtoh',
-             'gecco_hip_synchronize')
+             'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
+             'gecco_hip_plan_block_info')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -78,6 +79,9 @@ _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
                                         ctypes.c_void_p]
 _lib.gecco_hip_plan_stats.argtypes = [ctypes.c_void_p, PD, PI64, PI64, PD, PD]
+_lib.gecco_hip_plan_set_timing.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_timing.argtypes = [ctypes.c_void_p, PD, PI64, PD]
+_lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
 
 
 def lib():
@@ -195,6 +199,23 @@ class Plan:
                                          ctypes.byref(nt), ctypes.byref(pb), ctypes.byref(ib)))
         return dict(flops=f.value, nblocks=nb.value, ntasks=nt.value,
                     payload_bytes=pb.value, index_bytes=ib.value)
+
+    def set_timing(self, on=True):
+        _check(_lib.gecco_hip_plan_set_timing(self._h, int(bool(on))))
+
+    def timing(self):
+        """{'ms': [gemm, gather, sym], 'launches': [...], 'flops': [gemm, gather]}
+        accumulated since the previous call."""
+        ms = (ctypes.c_double * 3)()
+        n = (I64 * 3)()
+        fl = (ctypes.c_double * 2)()
+        _check(_lib.gecco_hip_plan_timing(self._h, ms, n, fl))
+        return dict(ms=list(ms), launches=list(n), flops=list(fl))
+
+    def block_info(self, i):
+        info = (I64 * 16)()
+        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 16))
+        return list(info)
 
     def __del__(self):
         try:

@@ -137,6 +137,18 @@ int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops,
                          int64_t *nblocks, int64_t *ntasks,
                          double *payload_bytes, double *index_bytes);
 
+/* Per-launch HIP-event timing of execute (off by default).  plan_timing
+ * returns the accumulated ms and launch counts of the three launch classes
+ * (strided-GEMM contraction kernel, gather contraction kernel, spin-flip
+ * symmetrisation) since the previous call, and the FLOP per execute of the
+ * two contraction kernels; it waits for the recorded events. */
+int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on);
+int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops);
+/* Kernel-call details for diagnostics: [M, N, K, swap, injective, A/B/C
+ * table, A/B fast_first, affine flags of A/B/C first/second vectors]. */
+int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t iblock, int64_t *info,
+                              int64_t ninfo);
+
 /* Device memory helpers for callers without another allocator. */
 int gecco_hip_malloc(void **ptr, int64_t bytes);
 int gecco_hip_free(void *ptr);
