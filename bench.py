@@ -191,13 +191,15 @@ def run_ccsd(args):
     solver.solve(tol=0.0, maxit=args.steps, restart=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = [0.0, 0.0, 0.0]
-    nl = [0, 0, 0]
+    ms = [0.0, 0.0, 0.0, 0.0]
+    nl = [0, 0, 0, 0]
     fl = [0.0, 0.0]
+    gbytes = 0.0
     for p in solver.plans:
         tm = p.timing()
         p.set_timing(False)
-        for i in range(3):
+        gbytes += p.workspace()['gather_bytes']
+        for i in range(4):
             ms[i] += tm['ms'][i]
             nl[i] += tm['launches'][i]
         fl[0] += tm['flops'][0]
@@ -230,7 +232,9 @@ def run_ccsd(args):
         'contraction_tflops': round(solver.flops * args.steps / wall / 1e12, 3),
         'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
                                 'grouped_contract_f64': round(ms[1] / args.steps, 4),
-                                'sym_ab_pairs_f64': round(ms[2] / args.steps, 4)},
+                                'sym_ab_pairs_f64': round(ms[2] / args.steps, 4),
+                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'gather_operands_gbps': round(gbytes / (ms[3] / args.steps * 1e-3) / 1e9, 1) if ms[3] > 0 else None,
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4), 'traffic': None,
                      'kernel': names[dom], 'kernel_ms': round(dom_ms, 5)},

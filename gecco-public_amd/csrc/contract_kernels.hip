@@ -167,7 +167,8 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         const DevOperand oc = b.C;
         double *pc = bufs[b.buf_c] + b.base_c;
         const double xfac = b.xfac;
-        const int atomic = b.atomic;
+        int64_t addr[2][2][4];   // -1: padding or forbidden
+        double val[2][2][4];
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
             const int n = n0 + wn * 32 + ni * 16 + lcol;
@@ -180,27 +181,53 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
-                    if (m >= M || n >= N) continue;
-                    int64_t addr;
-                    bool neg;
-                    if (oc.table) {
-                        const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
-                        if (v == 0) continue;
-                        neg = v < 0;
-                        addr = (int64_t)(v < 0 ? -v : v) - 1;
-                    } else {
-                        const int32_t rr = oc.first_affine
-                                               ? (int32_t)(1 + oc.f0 + oc.f1 * (int64_t)m)
-                                               : oc.first[m];
-                        if (rr == 0 || cc == 0) continue;
-                        neg = (rr < 0) != (cc < 0);
-                        addr = (int64_t)(rr < 0 ? -rr : rr) + (int64_t)(cc < 0 ? -cc : cc) - 2;
+                    int64_t ad = -1;
+                    bool neg = false;
+                    if (m < M && n < N) {
+                        if (oc.table) {
+                            const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+                            neg = v < 0;
+                            ad = (int64_t)(v < 0 ? -v : v) - 1;
+                        } else {
+                            const int32_t rr = oc.first_affine
+                                                   ? (int32_t)(1 + oc.f0 + oc.f1 * (int64_t)m)
+                                                   : oc.first[m];
+                            neg = (rr < 0) != (cc < 0);
+                            ad = (rr == 0 || cc == 0)
+                                     ? -1
+                                     : (int64_t)(rr < 0 ? -rr : rr) + (int64_t)(cc < 0 ? -cc : cc) - 2;
+                        }
                     }
-                    double val = xfac * acc[mi][ni][r];
-                    if (neg) val = -val;
-                    if (atomic) unsafeAtomicAdd(pc + addr, val);
-                    else pc[addr] += val;
+                    const double v = xfac * acc[mi][ni][r];
+                    addr[mi][ni][r] = ad;
+                    val[mi][ni][r] = neg ? -v : v;
                 }
+        }
+        if (b.atomic) {
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (addr[mi][ni][r] >= 0) unsafeAtomicAdd(pc + addr[mi][ni][r], val[mi][ni][r]);
+        } else {
+            // the task owns its result elements: loads first, then stores
+            double old[2][2][4];
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        old[mi][ni][r] = addr[mi][ni][r] >= 0 ? pc[addr[mi][ni][r]] : 0.0;
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (addr[mi][ni][r] >= 0) pc[addr[mi][ni][r]] = old[mi][ni][r] + val[mi][ni][r];
         }
     }
 }
@@ -333,24 +360,131 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         }
 
         const DevOperand &oc = b.C;
-        double *pc = bufs[b.buf_c] + b.base_c + oc.f0 + oc.s0;
+        double *pc = bufs[b.buf_c] + b.base_c;
         const double xfac = b.xfac;
-        const int atomic = b.atomic;
+        const bool caff = !oc.table && oc.first_affine && oc.second_affine;
+        // addresses (-1: padding or forbidden) and signed values of this
+        // lane's TM*TN*4 result elements
+        int64_t addr[TM][TN][4];
+        double val[TM][TN][4];
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             const int n = n0 + wn * (TBN / 2) + j * 16 + lcol;
+            int64_t cadd = 0;
+            bool cneg = false, cok = n < N;
+            if (cok && !oc.table) {
+                if (oc.second_affine) {
+                    cadd = oc.s0 + oc.s1 * (int64_t)n;
+                } else {
+                    const int32_t cv = oc.second[n];
+                    cok = cv != 0;
+                    cneg = cv < 0;
+                    cadd = (int64_t)(cv < 0 ? -cv : cv) - 1;
+                }
+            }
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
-                    if (m >= M || n >= N) continue;
-                    double *dst = pc + oc.f1 * (int64_t)m + oc.s1 * (int64_t)n;
-                    const double val = xfac * acc[i][j][r];
-                    if (atomic) unsafeAtomicAdd(dst, val);
-                    else *dst += val;
+                    int64_t a = -1;
+                    bool neg = cneg;
+                    if (m < M && cok) {
+                        if (caff) {
+                            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
+                        } else if (oc.table) {
+                            const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+                            neg = v < 0;
+                            a = (int64_t)(v < 0 ? -v : v) - 1;
+                        } else if (oc.first_affine) {
+                            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
+                        } else {
+                            const int32_t rv = oc.first[m];
+                            neg = neg != (rv < 0);
+                            a = rv == 0 ? -1 : (int64_t)(rv < 0 ? -rv : rv) - 1 + cadd;
+                        }
+                    }
+                    const double v = xfac * acc[i][j][r];
+                    addr[i][j][r] = a;
+                    val[i][j][r] = neg ? -v : v;
                 }
         }
+        if (b.atomic) {
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (addr[i][j][r] >= 0) unsafeAtomicAdd(pc + addr[i][j][r], val[i][j][r]);
+        } else {
+            // this task owns its result elements (injective map, one K
+            // chunk, no overlapping block): all loads are issued before the
+            // stores, so the read-modify-writes overlap in flight
+            double old[TM][TN][4];
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        old[i][j][r] = addr[i][j][r] >= 0 ? pc[addr[i][j][r]] : 0.0;
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (addr[i][j][r] >= 0) pc[addr[i][j][r]] = old[i][j][r] + val[i][j][r];
+        }
+    }
+}
+
+// Signed gather of operands into scratch (one launch for all jobs of a
+// plan); HBM-bound: one read of each allowed source element, one write.
+__global__ __launch_bounds__(256) void gather_operands_f64(
+    const DevGather *__restrict__ jobs, const DevGatherTask *__restrict__ tasks, int ntasks,
+    double *const *__restrict__ bufs, double *__restrict__ scratch) {
+    const DevGatherTask t = tasks[blockIdx.x];
+    const DevGather &g = jobs[t.job];
+    const double *__restrict__ src = bufs[g.src_buf] + g.src_base;
+    double *__restrict__ dst = scratch + g.dst_base;
+    const DevOperand &o = g.src;
+    const uint32_t inner = g.kfast ? (uint32_t)g.K : (uint32_t)g.X;
+    for (uint32_t e = (uint32_t)t.e0 + threadIdx.x; e < (uint32_t)t.e1; e += blockDim.x) {
+        const uint32_t q = e / inner, rmd = e - q * inner;
+        const uint32_t k = g.kfast ? rmd : q, x = g.kfast ? q : rmd;
+        int64_t addr;
+        bool neg, ok = true;
+        if (o.table) {
+            const int32_t v = o.first[(int64_t)k + o.ld * (int64_t)x];
+            ok = v != 0;
+            neg = v < 0;
+            addr = (int64_t)(v < 0 ? -v : v) - 1;
+        } else {
+            int64_t a1, a2;
+            bool n1 = false, n2 = false;
+            if (o.first_affine) {
+                a1 = o.f0 + o.f1 * (int64_t)k;
+            } else {
+                const int32_t v = o.first[k];
+                ok = v != 0; n1 = v < 0; a1 = (int64_t)(v < 0 ? -v : v) - 1;
+            }
+            if (o.second_affine) {
+                a2 = o.s0 + o.s1 * (int64_t)x;
+            } else {
+                const int32_t v = o.second[x];
+                ok = ok && v != 0; n2 = v < 0; a2 = (int64_t)(v < 0 ? -v : v) - 1;
+            }
+            neg = n1 != n2;
+            addr = a1 + a2;
+        }
+        double val = 0.0;
+        if (ok) {
+            const double xv = src[addr];
+            val = neg ? -xv : xv;
+        }
+        dst[e] = val;
     }
 }
 
@@ -408,6 +542,15 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
     case 5: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 8>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
     default: return 1;
     }
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
+                                   int ntasks, double *const *bufs, double *scratch,
+                                   hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    hipLaunchKernelGGL(gecco::gather_operands_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
+                       ntasks, bufs, scratch);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -42,4 +42,23 @@ struct DevTask {
     int32_t blk, m0, n0, k0, k1;
 };
 
+// Materialisation of a gathered operand (the reference's collect step,
+// contract/collect_block_impr.f): dense[e] = sign * src[addr(k,x)], or 0 for a
+// forbidden element, with e = k + K*x (kfast) or x + X*k, written into
+// plan-owned scratch before the strided-GEMM launch.  K*X < 2^31.
+struct DevGather {
+    DevOperand src;        // index data of the source operand (k, x)
+    int64_t src_base;      // element offset of the source block
+    int64_t dst_base;      // element offset in the scratch buffer
+    int32_t src_buf;       // buffer slot of the source list
+    int32_t K, X;
+    int32_t kfast;
+};
+
+struct DevGatherTask {
+    int32_t job;
+    int32_t e0, e1;        // element range [e0, e1) of the job's dense array
+    int32_t pad;
+};
+
 }  // namespace gecco

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gecco_hip.h"
@@ -39,6 +40,9 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
                                    const int8_t *fsign, int64_t npair, double prefac,
+                                   hipStream_t stream);
+extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
+                                   int ntasks, double *const *bufs, double *scratch,
                                    hipStream_t stream);
 
 namespace gecco {
@@ -301,6 +305,8 @@ struct HostBlock {
     int64_t M, N, K;    // kernel orientation
     bool swap;          // A=Op2, B=Op1 when the result is contiguous along EX1
     bool injective;
+    bool strided = false;             // runs on the strided-GEMM kernel
+    bool mat_a = false, mat_b = false;  // A / B materialised into scratch
     HostOperand A, B, C;
 };
 
@@ -325,6 +331,13 @@ struct gecco_hip_plan {
     int32_t *d_index = nullptr;
     double **d_bufs = nullptr;
     int64_t ntasks = 0, ntasks_affine = 0;
+    // operand materialisation (signed gather into plan-owned scratch)
+    gecco::DevGather *d_gjobs = nullptr;
+    gecco::DevGatherTask *d_gtasks = nullptr;
+    double *d_scratch = nullptr;
+    int64_t ngjobs = 0, ngtasks = 0, scratch_elems = 0;
+    int64_t materialize_min = 32;   // min(M, N) from which gathers are materialised
+    double gather_bytes = 0;
     int affine_variant = 1;
     int64_t target_tasks = 2048;
     int64_t min_kchunk = 256;
@@ -334,11 +347,12 @@ struct gecco_hip_plan {
     double flops = 0, payload = 0, index_bytes = 0;
     std::vector<double *> last_bufs;
     // optional HIP-event timing of the launches of execute (per launch class:
-    // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation)
+    // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation,
+    // 3 operand materialisation)
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    double ms[3] = {0, 0, 0};
-    int64_t launches[3] = {0, 0, 0};
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
     double flops_affine = 0, flops_general = 0;
 };
 
@@ -441,6 +455,12 @@ static void plan_free_device(gecco_hip_plan *p) {
     if (p->d_tasks) (void)hipFree(p->d_tasks);
     if (p->d_index) (void)hipFree(p->d_index);
     if (p->d_bufs) (void)hipFree(p->d_bufs);
+    if (p->d_gjobs) (void)hipFree(p->d_gjobs);
+    if (p->d_gtasks) (void)hipFree(p->d_gtasks);
+    if (p->d_scratch) (void)hipFree(p->d_scratch);
+    p->d_gjobs = nullptr;
+    p->d_gtasks = nullptr;
+    p->d_scratch = nullptr;
     for (auto &y : p->syms) {
         if (y.d_i1) (void)hipFree(y.d_i1);
         if (y.d_i2) (void)hipFree(y.d_i2);
@@ -495,17 +515,28 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
     }
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
-    // affine blocks (strided GEMMs) run in their own launch with the tile
-    // variant p->affine_variant; the others use the 64x64 gather kernel
+    // Blocks whose operands A and B are strided views (affine index maps)
+    // run on the strided-GEMM kernel with the tile variant p->affine_variant;
+    // its epilogue scatters through any result map.  A block with gathered
+    // operands joins them when min(M, N) >= p->materialize_min: each such
+    // operand is first written densely into scratch by the gather launch
+    // (one read and one write of the operand, instead of one gathered read
+    // per result tile).  The remaining blocks use the 64x64 gather kernel.
     std::vector<uint8_t> is_aff(p->blocks.size(), 0);
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
-        bool aff = true;
-        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
-            int64_t a0, a1;
-            aff = aff && !o->r.table && affine_of(o->r.first, a0, a1) && affine_of(o->r.second, a0, a1);
-        }
-        is_aff[i] = aff ? 1 : 0;
+        int64_t a0, a1;
+        auto strided = [&](const HostOperand &o) {
+            return !o.r.table && affine_of(o.r.first, a0, a1) && affine_of(o.r.second, a0, a1);
+        };
+        const bool sa = strided(b.A), sb = strided(b.B);
+        const bool big = p->materialize_min > 0 && std::min(b.M, b.N) >= p->materialize_min;
+        const bool fa = b.K * b.M < ((int64_t)1 << 31), fb = b.K * b.N < ((int64_t)1 << 31);
+        const bool ca = sa || (big && fa), cb = sb || (big && fb);
+        b.strided = ca && cb;
+        b.mat_a = b.strided && !sa;
+        b.mat_b = b.strided && !sb;
+        is_aff[i] = b.strided ? 1 : 0;
     }
     int abm = 64, abn = 64, abk = 32;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
@@ -587,12 +618,99 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
     }
     p->d_index = d_index;
+    // materialisation jobs; identical gathers of the same list share one job
+    {
+        std::vector<DevGather> jobs;
+        std::vector<const HostOperand *> job_src;
+        std::vector<int32_t> job_buf;
+        std::vector<int64_t> job_base;
+        std::unordered_map<uint64_t, std::vector<size_t>> seen;
+        const int32_t slot = (int32_t)nb_max;
+        int64_t next = 0;
+        for (size_t i = 0; i < p->blocks.size(); ++i) {
+            auto &b = p->blocks[i];
+            DevBlock &d = dblocks[i];
+            for (int q = 0; q < 2; ++q) {
+                if (!(q == 0 ? b.mat_a : b.mat_b)) continue;
+                const HostOperand &ho = q == 0 ? b.A : b.B;
+                DevOperand &dop = q == 0 ? d.A : d.B;
+                const int32_t buf = q == 0 ? b.buf1 : b.buf2;
+                const int64_t base = q == 0 ? b.off1 : b.off2;
+                const int64_t K = b.K, X = q == 0 ? b.M : b.N;
+                const int kfast = ho.fast_first;
+                uint64_t h = 1469598103934665603ull;
+                auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+                mix((uint64_t)buf); mix((uint64_t)base); mix((uint64_t)K); mix((uint64_t)X);
+                mix((uint64_t)ho.r.table); mix((uint64_t)kfast);
+                for (auto v : ho.r.first) mix((uint64_t)(uint32_t)v);
+                for (auto v : ho.r.second) mix((uint64_t)(uint32_t)v);
+                size_t j = jobs.size();
+                for (size_t c : seen[h]) {
+                    const HostOperand *o2 = job_src[c];
+                    if (job_buf[c] == buf && job_base[c] == base && jobs[c].K == K &&
+                        jobs[c].X == X && jobs[c].kfast == kfast && o2->r.table == ho.r.table &&
+                        o2->r.first == ho.r.first && o2->r.second == ho.r.second) {
+                        j = c;
+                        break;
+                    }
+                }
+                if (j == jobs.size()) {
+                    DevGather g;
+                    std::memset(&g, 0, sizeof g);
+                    g.src = dop;
+                    g.src_base = base;
+                    g.src_buf = buf;
+                    g.dst_base = next;
+                    g.K = (int32_t)K;
+                    g.X = (int32_t)X;
+                    g.kfast = kfast;
+                    next += ((K * X + 63) / 64) * 64;
+                    jobs.push_back(g);
+                    job_src.push_back(&ho);
+                    job_buf.push_back(buf);
+                    job_base.push_back(base);
+                    seen[h].push_back(j);
+                }
+                DevOperand m;
+                std::memset(&m, 0, sizeof m);
+                m.ld = K;
+                m.fast_first = kfast;
+                m.first_affine = m.second_affine = 1;
+                m.f1 = kfast ? 1 : X;
+                m.s1 = kfast ? K : 1;
+                dop = m;
+                (q == 0 ? d.buf_a : d.buf_b) = slot;
+                (q == 0 ? d.base_a : d.base_b) = jobs[j].dst_base;
+            }
+        }
+        std::vector<DevGatherTask> gt;
+        const int64_t chunk = 8192;
+        double gbytes = 0;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const int64_t n = (int64_t)jobs[j].K * jobs[j].X;
+            for (int64_t e0 = 0; e0 < n; e0 += chunk)
+                gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk), 0});
+            gbytes += 16.0 * (double)n;
+        }
+        p->ngjobs = (int64_t)jobs.size();
+        p->ngtasks = (int64_t)gt.size();
+        p->scratch_elems = next;
+        p->gather_bytes = gbytes;
+        if (!jobs.empty()) {
+            if (check(hipMalloc(&p->d_scratch, (size_t)next * sizeof(double)), "hipMalloc(scratch)") ||
+                check(hipMalloc(&p->d_gjobs, jobs.size() * sizeof(DevGather)), "hipMalloc(gather jobs)") ||
+                check(hipMemcpy(p->d_gjobs, jobs.data(), jobs.size() * sizeof(DevGather), hipMemcpyHostToDevice), "hipMemcpy(gather jobs)") ||
+                check(hipMalloc(&p->d_gtasks, gt.size() * sizeof(DevGatherTask)), "hipMalloc(gather tasks)") ||
+                check(hipMemcpy(p->d_gtasks, gt.data(), gt.size() * sizeof(DevGatherTask), hipMemcpyHostToDevice), "hipMemcpy(gather tasks)"))
+                return g_err;
+        }
+    }
     if (check(hipMalloc(&p->d_blocks, std::max<size_t>(1, dblocks.size()) * sizeof(DevBlock)), "hipMalloc(blocks)")) return g_err;
     if (!dblocks.empty() && check(hipMemcpy(p->d_blocks, dblocks.data(), dblocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice), "hipMemcpy(blocks)")) return g_err;
     if (check(hipMalloc(&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask)), "hipMalloc(tasks)")) return g_err;
     if (!tasks.empty() && check(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice), "hipMemcpy(tasks)")) return g_err;
     p->nbufs_max = nb_max;
-    if (check(hipMalloc(&p->d_bufs, std::max<int64_t>(1, nb_max) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
+    if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
     p->ntasks_affine = naff;
     for (auto &y : p->syms) {
@@ -639,6 +757,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
     if (p->ntasks == 0 && p->syms.empty()) return ok();
     std::vector<double *> want(bufs, bufs + p->nbufs_max);
+    if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
     if (want != p->last_bufs) {
         // slot table changes only when the caller passes different buffers;
         // the copy is ordered on the same stream before the launch
@@ -655,6 +774,12 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         else p->pending.back().second.second = e;
         return 0;
     };
+    if (p->ngtasks) {
+        if (mark(3, true)) return g_err;
+        if (gecco_launch_gather(p->d_gjobs, p->d_gtasks, (int)p->ngtasks, p->d_bufs, p->d_scratch, s))
+            return fail(GECCO_HIP_EDEVICE, "gather launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (mark(3, false)) return g_err;
+    }
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
@@ -852,6 +977,7 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     if (const char *e = getenv("GECCO_HIP_TARGET_TASKS")) (*plan)->target_tasks = std::max(1L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MAX_KCHUNK")) (*plan)->max_kchunk = std::max(64L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_MATERIALIZE_MIN")) (*plan)->materialize_min = std::max(0L, atol(e));
     return 0;
 }
 
@@ -919,6 +1045,24 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *in
     info[13] = !b.B.r.table && affine_of(b.B.r.second, a0, a1);
     info[14] = !b.C.r.table && affine_of(b.C.r.first, a0, a1);
     info[15] = !b.C.r.table && affine_of(b.C.r.second, a0, a1);
+    if (ninfo >= 19) { info[16] = b.strided; info[17] = b.mat_a; info[18] = b.mat_b; }
+    return 0;
+}
+
+int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (min_dim < 0) return fail(GECCO_HIP_EINVAL, "negative threshold");
+    plan->materialize_min = min_dim;
+    return 0;
+}
+
+int gecco_hip_plan_workspace(const gecco_hip_plan *plan, int64_t *scratch_bytes,
+                             int64_t *gather_jobs, double *gather_bytes) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (scratch_bytes) *scratch_bytes = 8 * plan->scratch_elems;
+    if (gather_jobs) *gather_jobs = plan->ngjobs;
+    if (gather_bytes) *gather_bytes = plan->gather_bytes;
     return 0;
 }
 
@@ -929,10 +1073,11 @@ int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on) {
 }
 
 int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops) {
-    // ms[3], launches[3]: accumulated times and counts of the strided-GEMM
-    // launches, the gather launches and the symmetrisation launches since the
-    // last reset (waits for the recorded events); flops[2]: per-execute FLOP
-    // of the two contraction kernels.  Resets the accumulators.
+    // ms[4], launches[4]: accumulated times and counts of the strided-GEMM
+    // launches, the gather-kernel launches, the symmetrisation launches and
+    // the operand-materialisation launches since the last reset (waits for
+    // the recorded events); flops[2]: per-execute FLOP of the two contraction
+    // kernels.  Resets the accumulators.
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
     for (auto &pe : plan->pending) {
         float t = 0.f;
@@ -944,7 +1089,7 @@ int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, d
         (void)hipEventDestroy(pe.second.second);
     }
     plan->pending.clear();
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         if (ms) ms[i] = plan->ms[i];
         if (launches) launches[i] = plan->launches[i];
         plan->ms[i] = 0;

@@ -66,7 +66,8 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_execute', 'gecco_hip_plan_stats', 'gecco_hip_malloc',
              'gecco_hip_free', 'gecco_hip_memcpy_htod', 'gecco_hip_memcpy_dtoh',
              'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
-             'gecco_hip_plan_block_info')
+             'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
+             'gecco_hip_plan_workspace')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -82,6 +83,8 @@ _lib.gecco_hip_plan_stats.argtypes = [ctypes.c_void_p, PD, PI64, PI64, PD, PD]
 _lib.gecco_hip_plan_set_timing.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_timing.argtypes = [ctypes.c_void_p, PD, PI64, PD]
 _lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
+_lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD]
 
 
 def lib():
@@ -204,18 +207,31 @@ class Plan:
         _check(_lib.gecco_hip_plan_set_timing(self._h, int(bool(on))))
 
     def timing(self):
-        """{'ms': [gemm, gather, sym], 'launches': [...], 'flops': [gemm, gather]}
-        accumulated since the previous call."""
-        ms = (ctypes.c_double * 3)()
-        n = (I64 * 3)()
+        """{'ms': [gemm, gather, sym, materialise], 'launches': [...],
+        'flops': [gemm, gather]} accumulated since the previous call."""
+        ms = (ctypes.c_double * 4)()
+        n = (I64 * 4)()
         fl = (ctypes.c_double * 2)()
         _check(_lib.gecco_hip_plan_timing(self._h, ms, n, fl))
         return dict(ms=list(ms), launches=list(n), flops=list(fl))
 
     def block_info(self, i):
-        info = (I64 * 16)()
-        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 16))
+        info = (I64 * 19)()
+        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 19))
         return list(info)
+
+    def set_materialize(self, min_dim):
+        """Materialise gathered operands of kernel calls with min(M, N) >=
+        min_dim into scratch (0 disables); before finalize."""
+        _check(_lib.gecco_hip_plan_set_materialize(self._h, int(min_dim)))
+
+    def workspace(self):
+        sb = I64()
+        nj = I64()
+        gb = ctypes.c_double()
+        _check(_lib.gecco_hip_plan_workspace(self._h, ctypes.byref(sb), ctypes.byref(nj),
+                                             ctypes.byref(gb)))
+        return dict(scratch_bytes=sb.value, gather_jobs=nj.value, gather_bytes=gb.value)
 
     def __del__(self):
         try:

@@ -399,8 +399,10 @@ def call_kernel(L, b, x1, x2, xr, route, lenblock=200, lenblock_cnt=-1, lenscr=N
 
 
 # ------------------------------------------------------------------ sym_ab
-def sym_ab(mel, x, prefac, relfac):
-    """contract/sym_ab_blk.f:214-405 for a one-vertex list, in place."""
+def sym_ab_pairs(mel, relfac):
+    """contract/sym_ab_blk.f:214-405 for a one-vertex list, as the list of
+    element pairs (i1 <= i2) and factors f = csign*asign*relfac2 that the
+    reference updates as v = prefac*(x1 + f*x2); x1 = v; x2 = f*v."""
     orbs = mel.orbs
     nsym = orbs.nsym
     msc_max, msa_max = mel.nc, mel.na
@@ -408,6 +410,7 @@ def sym_ab(mel, x, prefac, relfac):
         relfac2 = -relfac
     else:
         relfac2 = relfac
+    I1, I2, F = [], [], []
     for msa in range(msa_max, -1, -2):
         msc = msa
         if abs(msc) > msc_max:
@@ -428,48 +431,47 @@ def sym_ab(mel, x, prefac, relfac):
                 if d1 is None:
                     continue
                 d2 = mel.find(-msa, igama, imc2, gc, ima2, ga)
-                lens = d1['lens']
-                ncb = len(mel.oc)
                 if msa == 0 and d1['off'] > d2['off']:
                     continue
+                lens = d1['lens']
+                ncb = len(mel.oc)
                 fl = []
                 for k in range(ncb):
-                    fl.append(flipmap(orbs.spaces[mel.hc[k]], mel.oc[k], imc[k], gc[k]))
+                    fl.append(np.asarray(flipmap(orbs.spaces[mel.hc[k]], mel.oc[k], imc[k], gc[k])))
                 for k in range(len(mel.oa)):
-                    fl.append(flipmap(orbs.spaces[mel.ha[k]], mel.oa[k], ima[k], ga[k]))
-                ldc, lda = d1['ldc'], d1['lda']
-                lena = int(np.prod(lens[ncb:])) if len(lens) > ncb else 1
-                lenc = int(np.prod(lens[:ncb])) if ncb else 1
-                for ia in range(lena):
-                    st = ia
-                    asign = gsign
-                    ia_s, ia_f = [], []
-                    for k in range(len(mel.oa)):
-                        s = st % lens[ncb + k]
-                        m = fl[ncb + k][s]
-                        ia_s.append(s); ia_f.append(abs(m) - 1)
-                        asign *= (1 if m > 0 else -1)
-                        st //= lens[ncb + k]
-                    for ic in range(lenc):
-                        st = ic
-                        csign = 1
-                        ic_s, ic_f = [], []
-                        for k in range(ncb):
-                            s = st % lens[k]
-                            m = fl[k][s]
-                            ic_s.append(s); ic_f.append(abs(m) - 1)
-                            csign *= (1 if m > 0 else -1)
-                            st //= lens[k]
-                        i1 = d1['off'] + sum(a * b for a, b in zip(ic_s, ldc)) + \
-                            sum(a * b for a, b in zip(ia_s, lda))
-                        i2 = d2['off'] + sum(a * b for a, b in zip(ic_f, ldc)) + \
-                            sum(a * b for a, b in zip(ia_f, lda))
-                        if i1 > i2:
-                            continue
-                        f = csign * asign * relfac2
-                        v = prefac * (x[i1] + f * x[i2])
-                        x[i1] = v
-                        x[i2] = f * v
+                    fl.append(np.asarray(flipmap(orbs.spaces[mel.ha[k]], mel.oa[k], ima[k], ga[k])))
+                ld = d1['ldc'] + d1['lda']
+                # all elements of the block: component string indices, first fastest
+                if lens:
+                    grids = np.meshgrid(*[np.arange(n) for n in lens], indexing='ij')
+                    i1 = d1['off'] + sum(g * l for g, l in zip(grids, ld))
+                    i2 = d2['off'] + sum((np.abs(f[g]) - 1) * l for f, g, l in zip(fl, grids, ld))
+                    sg = gsign * np.prod([np.sign(f[g]) for f, g in zip(fl, grids)], axis=0)
+                else:   # operator without strings (a scalar)
+                    i1 = np.array([d1['off']])
+                    i2 = np.array([d2['off']])
+                    sg = np.array([gsign])
+                i1, i2, sg = i1.ravel(), i2.ravel(), np.asarray(sg).ravel()
+                keep = i1 <= i2
+                I1.append(i1[keep])
+                I2.append(i2[keep])
+                F.append(sg[keep] * relfac2)
+    if not I1:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+    return np.concatenate(I1), np.concatenate(I2), np.concatenate(F).astype(float)
+
+
+def apply_sym_pairs(x, pairs, prefac):
+    """The reference's pair update, vectorised (pairs are disjoint)."""
+    i1, i2, f = pairs
+    v = prefac * (x[i1] + f * x[i2])
+    x[i1] = v
+    x[i2] = f * v
+
+
+def sym_ab(mel, x, prefac, relfac):
+    """contract/sym_ab_blk.f:214-405 applied to list x in place."""
+    apply_sym_pairs(x, sym_ab_pairs(mel, relfac), prefac)
 
 
 # ------------------------------------------------------------ dense maps

@@ -443,7 +443,17 @@ def std_spsign_msdis(msdis, occ):
     return std_spsign(scr)
 
 
+@lru_cache(maxsize=None)
+def _flipmap_cached(space_key, nel, idxms, igam):
+    return tuple(_flipmap(_SPACES[space_key], nel, idxms, igam))
+
+
 def flipmap(space, nel, idxms, igam):
+    """strings/set_flipmap_kernel.f (cached)."""
+    return list(_flipmap_cached(_key(space), nel, idxms, igam))
+
+
+def _flipmap(space, nel, idxms, igam):
     """strings/set_flipmap_kernel.f: string -> spin-flipped string, signed."""
     ms = nel - 2 * (idxms - 1)
     out = []

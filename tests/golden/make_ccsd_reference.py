@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, 'gecco-public_amd'))
 import ccsd  # noqa: E402
 from synth import synthetic  # noqa: E402
 from oracle.melist import Orbitals, MEList  # noqa: E402
-from oracle.contract import Contraction, sym_ab  # noqa: E402
+from oracle.contract import Contraction, sym_ab_pairs, apply_sym_pairs  # noqa: E402
 from oracle.strings import strings_of, to_spinorbs  # noqa: E402
 
 H, P = 1, 2
@@ -91,6 +91,11 @@ def main():
             c = Contraction(L[a], L[b], L[res], cc, ca, fact)
             lv.append((res, a, b, c, c.blocks(True)))
         contractions.append(lv)
+    pairs = {}
+    for lv in contractions:
+        for res, *_ in lv:
+            if res not in pairs:
+                pairs[res] = sym_ab_pairs(L[res], 1.0)
     print(f'setup {time.time() - t0:.1f} s', flush=True)
     X['T1'] = np.zeros(L['T1'].length)
     X['T2'] = np.zeros(L['T2'].length)
@@ -112,7 +117,7 @@ def main():
                 if res not in written:
                     written.append(res)
             for res in written:
-                sym_ab(L[res], X[res], 0.5, 1.0)
+                apply_sym_pairs(X[res], pairs[res], 0.5)
         t = np.concatenate([X['T1'], X['T2']])
         omega = np.concatenate([X['R1'], X['R2']]) - Dcat * t
         rn = float(np.linalg.norm(omega))

@@ -32,10 +32,13 @@ def _relerr(got, ref):
 
 @pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 5, 9), (1, 8, 32)])
 @pytest.mark.parametrize('name', list(CASES))
-def test_plan_matches_oracle(nsym, nocc, nvir, name):
+@pytest.mark.parametrize('materialize', [0, 1, 32])
+def test_plan_matches_oracle(nsym, nocc, nvir, name, materialize):
+    """materialize = min(M, N) from which gathered operands are written into
+    scratch and the call runs on the strided-GEMM kernel (0: never)."""
     torch = _torch()
     import gecco_hip
-    print(f'case {name} nsym={nsym} nocc={nocc} nvir={nvir}', flush=True)
+    print(f'case {name} nsym={nsym} nocc={nocc} nvir={nvir} materialize={materialize}', flush=True)
     orbs = orbitals(nocc, nvir, nsym)
     c, a, b, r = CASES[name](orbs)
     x1 = spin_symmetric(a, 21)
@@ -48,6 +51,7 @@ def test_plan_matches_oracle(nsym, nocc, nvir, name):
         d2 = torch.from_numpy(x2).to(dev)
         dr = torch.zeros(r.length, dtype=torch.float64, device=dev)
         plan = gecco_hip.Plan()
+        plan.set_materialize(materialize)
         for bl in blocks:
             plan.add(bl)
         plan.finalize()
@@ -172,3 +176,40 @@ def test_affine_tile_variants(variant, name, monkeypatch):
     plan.execute(bufs)
     torch.cuda.synchronize()
     assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL
+
+
+def test_materialised_gathers_share_jobs():
+    """The ring contraction (all three index maps gathered) recorded twice
+    into one plan: the second copy reuses the first copy's materialised
+    operands, and both accumulate into the same result."""
+    torch = _torch()
+    import gecco_hip
+    orbs = orbitals(5, 12, 2)
+    c, a, b, r = CASES['ring'](orbs)
+    x1 = spin_symmetric(a, 81)
+    x2 = spin_symmetric(b, 82)
+    blocks = c.blocks(True)
+    ref = _oracle(c, x1, x2, r.length, blocks)
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+            torch.zeros(r.length, dtype=torch.float64, device=dev)]
+    one = gecco_hip.Plan()
+    one.set_materialize(1)
+    two = gecco_hip.Plan()
+    two.set_materialize(1)
+    for bl in blocks:
+        one.add(bl)
+        two.add(bl)
+        two.add(bl)
+    one.finalize()
+    two.finalize()
+    infos = [one.block_info(i) for i in range(len(blocks))]
+    assert any(x[17] or x[18] for x in infos), 'ring gathers are expected to be materialised'
+    assert all(x[16] for x in infos)
+    w1, w2 = one.workspace(), two.workspace()
+    assert w1['gather_jobs'] > 0
+    assert w2['gather_jobs'] == w1['gather_jobs']
+    assert w2['scratch_bytes'] == w1['scratch_bytes']
+    two.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[2].cpu().numpy(), 2 * ref) <= TOL
