@@ -287,8 +287,10 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
     double *const *__restrict__ bufs) {
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int TM = TBM / 32, TN = TBN / 32;
-    __shared__ double As[2][TBK * LDA];
-    __shared__ double Bs[2][TBK * LDB];
+    constexpr int SA = TBK * LDA, SB = TBK * LDB;
+    // double-buffered A and B tiles; the whole array stages the result
+    // range of a task with span > 0 after the K loop
+    __shared__ double lds[2 * SA + 2 * SB];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -324,8 +326,8 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         __syncthreads();   // LDS of the previous task is no longer read
         va.load(pa, task.k0, k1, ra);
         vb.load(pb, task.k0, k1, rb);
-        va.template store<LDA>(As[0], ra);
-        vb.template store<LDB>(Bs[0], rb);
+        va.template store<LDA>(lds, ra);
+        vb.template store<LDB>(lds + 2 * SA, rb);
         __syncthreads();
         int cur = 0;
         for (int k0 = task.k0; k0 < k1; k0 += TBK) {
@@ -334,8 +336,8 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
                 va.load(pa, k0 + TBK, k1, ra);
                 vb.load(pb, k0 + TBK, k1, rb);
             }
-            const double *A = As[cur];
-            const double *B = Bs[cur];
+            const double *A = lds + cur * SA;
+            const double *B = lds + 2 * SA + cur * SB;
 #pragma unroll
             for (int kq = 0; kq < TBK / 4; ++kq) {
                 const int kr = kq * 4 + lrow;
@@ -352,8 +354,8 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
             }
             if (more) {
-                va.template store<LDA>(As[cur ^ 1], ra);
-                vb.template store<LDB>(Bs[cur ^ 1], rb);
+                va.template store<LDA>(lds + (cur ^ 1) * SA, ra);
+                vb.template store<LDB>(lds + 2 * SA + (cur ^ 1) * SB, rb);
             }
             __syncthreads();
             cur ^= 1;
@@ -363,72 +365,110 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         double *pc = bufs[b.buf_c] + b.base_c;
         const double xfac = b.xfac;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
-        // addresses (-1: padding or forbidden) and signed values of this
-        // lane's TM*TN*4 result elements
-        int64_t addr[TM][TN][4];
-        double val[TM][TN][4];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
+        // element offset (-1: padding or forbidden) and signed value of
+        // result element (i, j, r) of this lane
+        auto elem = [&](int i, int j, int r, int64_t &a, double &v) {
             const int n = n0 + wn * (TBN / 2) + j * 16 + lcol;
-            int64_t cadd = 0;
-            bool cneg = false, cok = n < N;
-            if (cok && !oc.table) {
-                if (oc.second_affine) {
-                    cadd = oc.s0 + oc.s1 * (int64_t)n;
+            const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
+            a = -1;
+            bool neg = false;
+            if (m < M && n < N) {
+                if (caff) {
+                    a = oc.f0 + oc.f1 * (int64_t)m + oc.s0 + oc.s1 * (int64_t)n;
+                } else if (oc.table) {
+                    const int32_t t = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+                    neg = t < 0;
+                    a = (int64_t)(t < 0 ? -t : t) - 1;
                 } else {
-                    const int32_t cv = oc.second[n];
-                    cok = cv != 0;
-                    cneg = cv < 0;
-                    cadd = (int64_t)(cv < 0 ? -cv : cv) - 1;
+                    int64_t ar, ac;
+                    bool nr = false, nc = false, ok = true;
+                    if (oc.first_affine) {
+                        ar = oc.f0 + oc.f1 * (int64_t)m;
+                    } else {
+                        const int32_t t = oc.first[m];
+                        ok = t != 0; nr = t < 0; ar = (int64_t)(t < 0 ? -t : t) - 1;
+                    }
+                    if (oc.second_affine) {
+                        ac = oc.s0 + oc.s1 * (int64_t)n;
+                    } else {
+                        const int32_t t = oc.second[n];
+                        ok = ok && t != 0; nc = t < 0; ac = (int64_t)(t < 0 ? -t : t) - 1;
+                    }
+                    neg = nr != nc;
+                    a = ok ? ar + ac : -1;
                 }
             }
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
-                    int64_t a = -1;
-                    bool neg = cneg;
-                    if (m < M && cok) {
-                        if (caff) {
-                            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
-                        } else if (oc.table) {
-                            const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
-                            neg = v < 0;
-                            a = (int64_t)(v < 0 ? -v : v) - 1;
-                        } else if (oc.first_affine) {
-                            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
-                        } else {
-                            const int32_t rv = oc.first[m];
-                            neg = neg != (rv < 0);
-                            a = rv == 0 ? -1 : (int64_t)(rv < 0 ? -rv : rv) - 1 + cadd;
-                        }
-                    }
-                    const double v = xfac * acc[i][j][r];
-                    addr[i][j][r] = a;
-                    val[i][j][r] = neg ? -v : v;
-                }
-        }
-        if (b.atomic) {
+            const double x = xfac * acc[i][j][r];
+            v = neg ? -x : x;
+        };
+        if (task.span > 0) {
+            // LDS-staged result range: merges the contributions of this
+            // tile to the same element (e.g. the two orderings of an
+            // antisymmetrised string pair), then one contiguous write-back.
+            // The K loop ended with a barrier, so the tiles are dead.
+            const int span = task.span;
+            for (int i = tid; i < span; i += NT) lds[i] = 0.0;
+            __syncthreads();
 #pragma unroll
             for (int j = 0; j < TN; ++j)
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (addr[i][j][r] >= 0) unsafeAtomicAdd(pc + addr[i][j][r], val[i][j][r]);
+                    for (int r = 0; r < 4; ++r) {
+                        int64_t a;
+                        double v;
+                        elem(i, j, r, a, v);
+                        if (a >= 0) atomicAdd(lds + (a - task.lo), v);
+                    }
+            __syncthreads();
+            double *pr = pc + task.lo;
+            if (task.owned) {
+                for (int i0 = 0; i0 < span; i0 += 4 * NT) {
+                    double o[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * NT + tid;
+                        o[u] = i < span ? pr[i] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * NT + tid;
+                        if (i < span) pr[i] = o[u] + lds[i];
+                    }
+                }
+            } else {
+                for (int i = tid; i < span; i += NT) {
+                    const double v = lds[i];
+                    if (v != 0.0) unsafeAtomicAdd(pr + i, v);
+                }
+            }
+        } else if (b.atomic) {
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        int64_t a;
+                        double v;
+                        elem(i, j, r, a, v);
+                        if (a >= 0) unsafeAtomicAdd(pc + a, v);
+                    }
         } else {
             // this task owns its result elements (injective map, one K
             // chunk, no overlapping block): all loads are issued before the
             // stores, so the read-modify-writes overlap in flight
-            double old[TM][TN][4];
+            int64_t addr[TM][TN][4];
+            double val[TM][TN][4], old[TM][TN][4];
 #pragma unroll
             for (int j = 0; j < TN; ++j)
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
+                    for (int r = 0; r < 4; ++r) {
+                        elem(i, j, r, addr[i][j][r], val[i][j][r]);
                         old[i][j][r] = addr[i][j][r] >= 0 ? pc[addr[i][j][r]] : 0.0;
+                    }
 #pragma unroll
             for (int j = 0; j < TN; ++j)
 #pragma unroll
@@ -436,6 +476,132 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                         if (addr[i][j][r] >= 0) pc[addr[i][j][r]] = old[i][j][r] + val[i][j][r];
+        }
+    }
+}
+
+// Short contractions (K <= KMAX) with strided A and B, e.g. the rank-1 to
+// rank-25 updates T1 x integral of CCSD: MFMA tiles would idle on padding
+// and on per-task latency, so each thread computes whole dot products on the
+// vector ALU.  A task covers dm rows x dn columns (dn in {64, 128, 256},
+// dm = 8 * NT / dn): thread
+// (nl, ml) owns column n0 + nl and rows m0 + ml + (NT/dn) * i, i < 8.  The A
+// tile (K x dm) is staged in LDS, the thread's B column in registers.  The
+// result map is read n-fastest (table = 2 stores the table transposed).
+constexpr int DIRECT_A = 32 * 32;        // LDS doubles for the A tile
+constexpr int DIRECT_STAGE = 4096;       // LDS doubles for a staged result range
+
+template <int KMAX>
+__global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
+    const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
+    double *const *__restrict__ bufs) {
+    __shared__ double lds[DIRECT_A + DIRECT_STAGE];
+    double *const As = lds;
+    double *const Cs = lds + DIRECT_A;
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const DevTask task = tasks[t];
+        const DevBlock &b = blocks[task.blk];
+        const int M = b.M, N = b.N, K = b.K;
+        const int dn = task.pad, dm = 8 * NT / dn, ng = NT / dn;
+        const int nl = tid % dn, ml = tid / dn;
+        const int m0 = task.m0, n = task.n0 + nl;
+        const DevOperand &oa = b.A, &ob = b.B, &oc = b.C;
+        const double *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
+        const double *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
+        double *pc = bufs[b.buf_c] + b.base_c;
+        __syncthreads();   // LDS of the previous task is no longer read
+        for (int e = tid; e < K * dm; e += NT) {
+            const int k = e / dm, r = e - k * dm;
+            const int m = m0 + r;
+            As[e] = m < M ? pa[oa.f1 * (int64_t)k + oa.s1 * (int64_t)m] : 0.0;
+        }
+        const int span = task.span;
+        for (int i = tid; i < span; i += NT) Cs[i] = 0.0;
+        double bk[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            bk[k] = (k < K && n < N) ? pb[ob.f1 * (int64_t)k + ob.s1 * (int64_t)n] : 0.0;
+        __syncthreads();
+        // result column data (separable / affine maps)
+        int64_t cadd = 0;
+        bool cneg = false, cok = n < N;
+        if (cok && oc.table == 0) {
+            if (oc.second_affine) {
+                cadd = oc.s0 + oc.s1 * (int64_t)n;
+            } else {
+                const int32_t v = oc.second[n];
+                cok = v != 0;
+                cneg = v < 0;
+                cadd = (int64_t)(v < 0 ? -v : v) - 1;
+            }
+        }
+        int64_t addr[8];
+        double val[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = ml + ng * i;
+            const int m = m0 + r;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                if (k < K) acc = fma(As[k * dm + r], bk[k], acc);
+            int64_t a = -1;
+            bool neg = cneg;
+            if (m < M && cok) {
+                if (oc.table) {
+                    const int32_t v = oc.first[(int64_t)n + oc.ld * (int64_t)m];
+                    neg = v < 0;
+                    a = (int64_t)(v < 0 ? -v : v) - 1;
+                } else if (oc.first_affine) {
+                    a = oc.f0 + oc.f1 * (int64_t)m + cadd;
+                } else {
+                    const int32_t v = oc.first[m];
+                    neg = neg != (v < 0);
+                    a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
+                }
+            }
+            const double x = b.xfac * acc;
+            addr[i] = a;
+            val[i] = neg ? -x : x;
+        }
+        if (span > 0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (addr[i] >= 0) atomicAdd(Cs + (addr[i] - task.lo), val[i]);
+            __syncthreads();
+            double *pr = pc + task.lo;
+            if (task.owned) {
+                for (int i0 = 0; i0 < span; i0 += 4 * NT) {
+                    double o[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * NT + tid;
+                        o[u] = i < span ? pr[i] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * NT + tid;
+                        if (i < span) pr[i] = o[u] + Cs[i];
+                    }
+                }
+            } else {
+                for (int i = tid; i < span; i += NT) {
+                    const double v = Cs[i];
+                    if (v != 0.0) unsafeAtomicAdd(pr + i, v);
+                }
+            }
+        } else if (b.atomic) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (addr[i] >= 0) unsafeAtomicAdd(pc + addr[i], val[i]);
+        } else {
+            double old[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) old[i] = addr[i] >= 0 ? pc[addr[i]] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (addr[i] >= 0) pc[addr[i]] = old[i] + val[i];
         }
     }
 }
@@ -544,6 +710,23 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
+
+extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
+                                           const gecco::DevTask *tasks, int ntasks,
+                                           double *const *bufs, int kmax, hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    if (kmax <= 8)
+        hipLaunchKernelGGL(gecco::grouped_contract_direct_f64<8>, dim3(ntasks), dim3(gecco::NT), 0,
+                           stream, blocks, tasks, ntasks, bufs);
+    else if (kmax <= 32)
+        hipLaunchKernelGGL(gecco::grouped_contract_direct_f64<32>, dim3(ntasks), dim3(gecco::NT), 0,
+                           stream, blocks, tasks, ntasks, bufs);
+    else
+        return 1;
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_direct_stage_cap(void) { return gecco::DIRECT_STAGE; }
 
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, double *const *bufs, double *scratch,

@@ -40,6 +40,12 @@ struct DevBlock {
 
 struct DevTask {
     int32_t blk, m0, n0, k0, k1;
+    // span > 0 (strided-GEMM kernel only): the tile's results are first
+    // reduced in LDS over the block-relative element range [lo, lo + span)
+    // and then written back as one contiguous range -- read-modify-write if
+    // the task owns the range (owned = 1), else atomically
+    int32_t span, owned, pad;
+    int64_t lo;
 };
 
 // Materialisation of a gathered operand (the reference's collect step,

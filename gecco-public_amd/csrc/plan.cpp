@@ -41,6 +41,10 @@ extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
                                    const int8_t *fsign, int64_t npair, double prefac,
                                    hipStream_t stream);
+extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
+                                           const gecco::DevTask *tasks, int ntasks,
+                                           double *const *bufs, int kmax, hipStream_t stream);
+extern "C" int gecco_direct_stage_cap(void);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, double *const *bufs, double *scratch,
                                    hipStream_t stream);
@@ -245,6 +249,66 @@ static int resolve(const Space &fs, const Space &ss, const std::vector<Target> &
 
 static inline int64_t absv(int32_t v) { return v < 0 ? -(int64_t)v : (int64_t)v; }
 
+// A table whose entries factor as entry(f, s) = first[f] (+) second[s] --
+// zero pattern an outer product of a row and a column pattern, sign a
+// product of row and column signs, offset a sum of row and column offsets --
+// is separable even when its targets mix both index spaces (e.g. the string
+// pair of an alpha and a beta electron: no Pauli exclusion, offset
+// idx(a) + n_a * idx(b)).  Converts such a table in place.
+static void try_separate(Resolved &r) {
+    if (!r.table || r.F == 0 || r.S == 0) return;
+    const int64_t F = r.F, S = r.S;
+    const std::vector<int32_t> &t = r.first;
+    int64_t f0 = -1, s0 = -1;
+    for (int64_t s = 0; s < S && f0 < 0; ++s)
+        for (int64_t f = 0; f < F; ++f)
+            if (t[f + F * s]) { f0 = f; s0 = s; break; }
+    std::vector<int32_t> fv(F, 0), sv(S, 0);
+    if (f0 < 0) {   // nothing allowed
+        r.first.swap(fv);
+        r.second.swap(sv);
+        r.table = false;
+        return;
+    }
+    const int32_t ref = t[f0 + F * s0];
+    const int64_t aref = absv(ref) - 1;
+    // column offsets relative to the smallest one in row f0
+    int64_t cmin = INT64_MAX;
+    for (int64_t s = 0; s < S; ++s)
+        if (t[f0 + F * s]) cmin = std::min(cmin, absv(t[f0 + F * s]) - 1);
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t v = t[f0 + F * s];
+        if (!v) continue;
+        const int64_t a = absv(v) - 1 - cmin;
+        if (a + 1 > INT32_MAX) return;
+        sv[s] = (int32_t)((v < 0) == (ref < 0) ? a + 1 : -(a + 1));
+    }
+    for (int64_t f = 0; f < F; ++f) {
+        const int32_t v = t[f + F * s0];
+        if (!v) continue;
+        const int64_t a = absv(v) - 1 - aref + cmin;
+        if (a < 0 || a + 1 > INT32_MAX) return;
+        fv[f] = (int32_t)(v < 0 ? -(a + 1) : a + 1);
+    }
+    // row signs were taken relative to column s0 whose own sign is
+    // sign(sv[s0]) * sign(ref) = +; verify every entry
+    for (int64_t s = 0; s < S; ++s)
+        for (int64_t f = 0; f < F; ++f) {
+            const int32_t v = t[f + F * s];
+            const int32_t a = fv[f], b = sv[s];
+            if (!a || !b) {
+                if (v) return;
+                continue;
+            }
+            if (!v) return;
+            const bool neg = (a < 0) != (b < 0);
+            if ((v < 0) != neg || absv(v) != absv(a) + absv(b) - 1) return;
+        }
+    r.first.swap(fv);
+    r.second.swap(sv);
+    r.table = false;
+}
+
 // address of first-index step f=1 relative to f=0 (separable), or of the
 // table entry (1,0) relative to (0,0); used to pick coalescing directions
 static bool first_is_contiguous(const Resolved &r) {
@@ -267,6 +331,42 @@ static bool affine_of(const std::vector<int32_t> &v, int64_t &a0, int64_t &a1) {
     for (size_t i = 0; i < v.size(); ++i)
         if ((int64_t)v[i] != 1 + a0 + a1 * (int64_t)i) return false;
     return true;
+}
+
+// Range [amin, amax] of the element offsets a tile (rows [m0, m1), columns
+// [n0, n1) in kernel orientation) writes, and how many elements it writes.
+static int64_t tile_range(const Resolved &c, int64_t m0, int64_t m1, int64_t n0, int64_t n1,
+                          int64_t &amin, int64_t &amax) {
+    amin = INT64_MAX;
+    amax = -1;
+    int64_t count = 0;
+    if (c.table) {
+        for (int64_t n = n0; n < n1; ++n)
+            for (int64_t m = m0; m < m1; ++m) {
+                const int32_t v = c.first[m + c.F * n];
+                if (!v) continue;
+                const int64_t a = absv(v) - 1;
+                amin = std::min(amin, a);
+                amax = std::max(amax, a);
+                ++count;
+            }
+        return count;
+    }
+    int64_t fmin = INT64_MAX, fmax = -1, fc = 0, smin = INT64_MAX, smax = -1, sc = 0;
+    for (int64_t m = m0; m < m1; ++m)
+        if (c.first[m]) {
+            const int64_t a = absv(c.first[m]) - 1;
+            fmin = std::min(fmin, a); fmax = std::max(fmax, a); ++fc;
+        }
+    for (int64_t n = n0; n < n1; ++n)
+        if (c.second[n]) {
+            const int64_t a = absv(c.second[n]) - 1;
+            smin = std::min(smin, a); smax = std::max(smax, a); ++sc;
+        }
+    if (!fc || !sc) return 0;
+    amin = fmin + smin;
+    amax = fmax + smax;
+    return fc * sc;
 }
 
 static bool result_injective(const Resolved &r, int64_t len) {
@@ -296,6 +396,7 @@ struct HostOperand {
     Resolved r;
     int fast_first = 0;
     int64_t off_first = -1, off_second = -1;  // element offsets in the index pool
+    int64_t off_first_t = -1;                  // transposed table (result of direct blocks)
 };
 
 struct HostBlock {
@@ -305,7 +406,8 @@ struct HostBlock {
     int64_t M, N, K;    // kernel orientation
     bool swap;          // A=Op2, B=Op1 when the result is contiguous along EX1
     bool injective;
-    bool strided = false;             // runs on the strided-GEMM kernel
+    bool strided = false;             // A and B strided (natively or materialised)
+    bool direct = false;              // runs on the short-K vector-ALU kernel
     bool mat_a = false, mat_b = false;  // A / B materialised into scratch
     HostOperand A, B, C;
 };
@@ -330,13 +432,16 @@ struct gecco_hip_plan {
     gecco::DevTask *d_tasks = nullptr;
     int32_t *d_index = nullptr;
     double **d_bufs = nullptr;
-    int64_t ntasks = 0, ntasks_affine = 0;
+    // task classes, in this order in d_tasks: strided-GEMM kernel, direct
+    // kernel with K <= 8, direct kernel with K <= 32, gather kernel
+    int64_t ntasks = 0, ntasks_affine = 0, ntasks_direct8 = 0, ntasks_direct32 = 0;
     // operand materialisation (signed gather into plan-owned scratch)
     gecco::DevGather *d_gjobs = nullptr;
     gecco::DevGatherTask *d_gtasks = nullptr;
     double *d_scratch = nullptr;
     int64_t ngjobs = 0, ngtasks = 0, scratch_elems = 0;
-    int64_t materialize_min = 32;   // min(M, N) from which gathers are materialised
+    int64_t materialize_min = 1;    // min(M, N) from which gathers are materialised
+    int64_t nstaged = 0;            // tasks with an LDS-staged result range
     double gather_bytes = 0;
     int affine_variant = 1;
     int64_t target_tasks = 2048;
@@ -348,12 +453,13 @@ struct gecco_hip_plan {
     std::vector<double *> last_bufs;
     // optional HIP-event timing of the launches of execute (per launch class:
     // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation,
-    // 3 operand materialisation)
+    // 3 operand materialisation, 4 direct short-K kernel)
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    double ms[4] = {0, 0, 0, 0};
-    int64_t launches[4] = {0, 0, 0, 0};
-    double flops_affine = 0, flops_general = 0;
+    double ms[5] = {0, 0, 0, 0, 0};
+    int64_t launches[5] = {0, 0, 0, 0, 0};
+    double flops_affine = 0, flops_general = 0, flops_direct = 0;
+    int64_t direct_kmax = 32;   // K bound of the direct kernel (0 disables it)
 };
 
 namespace gecco {
@@ -407,6 +513,9 @@ static int build_block(const gecco_hip_block_args *a, HostBlock &hb) {
     if (resolve(cnt, ex1, t1, len1, r1)) return g_err;
     if (resolve(cnt, ex2, t2, len2, r2)) return g_err;
     if (resolve(ex1, ex2, t12, len12, r12)) return g_err;
+    try_separate(r1);
+    try_separate(r2);
+    try_separate(r12);
 
     hb.xfac = a->xfac;
     hb.len12 = len12;
@@ -536,13 +645,38 @@ static int plan_finalize(gecco_hip_plan *p) {
         b.strided = ca && cb;
         b.mat_a = b.strided && !sa;
         b.mat_b = b.strided && !sb;
-        is_aff[i] = b.strided ? 1 : 0;
+        // short contractions with a thin result side (or a very short
+        // contraction) run on the direct kernel; see
+        // grouped_contract_direct_f64
+        b.direct = b.strided && b.K <= p->direct_kmax &&
+                   (std::min(b.M, b.N) <= 32 || b.K <= 8);
+        is_aff[i] = b.strided && !b.direct ? 1 : 0;
     }
     int abm = 64, abn = 64, abk = 32;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
         return fail(GECCO_HIP_EINVAL, "unknown tile variant %d", p->affine_variant);
-    auto tile_m = [&](size_t i) { return is_aff[i] ? (int64_t)abm : (int64_t)64; };
-    auto tile_n = [&](size_t i) { return is_aff[i] ? (int64_t)abn : (int64_t)64; };
+    // class of a task's block: 0 strided GEMM, 1 direct K <= 8, 2 direct
+    // K <= 32, 3 gather kernel; tile shape per block
+    std::vector<int> cls(p->blocks.size());
+    std::vector<int64_t> tm(p->blocks.size()), tn(p->blocks.size());
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        if (b.direct) {
+            cls[i] = b.K <= 8 ? 1 : 2;
+            // columns per task: 64, 128 or 256 (a divisor of the 256
+            // threads); 64 (with 32 rows) when the result map is not
+            // injective, so that colliding elements tend to share a task
+            // whose result range fits the LDS stage
+            tn[i] = (b.N <= 64 || !b.injective) ? 64 : b.N <= 128 ? 128 : 256;
+            tm[i] = 8 * 256 / tn[i];
+        } else if (is_aff[i]) {
+            cls[i] = 0; tm[i] = abm; tn[i] = abn;
+        } else {
+            cls[i] = 3; tm[i] = 64; tn[i] = 64;
+        }
+    }
+    auto tile_m = [&](size_t i) { return tm[i]; };
+    auto tile_n = [&](size_t i) { return tn[i]; };
     // split K so that the launch has about p->target_tasks tasks
     int64_t ntiles = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
@@ -556,12 +690,14 @@ static int plan_finalize(gecco_hip_plan *p) {
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
         // parallel split for the whole plan, and a bound on the serial K
-        // extent of one task (rank-reducing blocks, e.g. full contractions)
+        // extent of one task (rank-reducing blocks, e.g. full contractions);
+        // direct blocks are never split
         int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / p->min_kchunk));
         ns = std::max<int64_t>(ns, (b.K + p->max_kchunk - 1) / p->max_kchunk);
         int64_t chunk = (b.K + ns - 1) / ns;
         chunk = ((chunk + 31) / 32) * 32;
         ns = (b.K + chunk - 1) / chunk;
+        if (b.direct) { ns = 1; chunk = b.K; }
         DevBlock &d = dblocks[i];
         d.xfac = b.xfac;
         d.base_a = b.off1;
@@ -582,16 +718,69 @@ static int plan_finalize(gecco_hip_plan *p) {
                     tasks.push_back({(int32_t)i, (int32_t)m0, (int32_t)n0, (int32_t)k0, (int32_t)k1});
                 }
     }
+    // Result range of every task.  Tasks of the strided-GEMM kernel whose
+    // result map is a table or not injective stage their tile in LDS when
+    // the range fits (lds_cap doubles) and is at least 1/4 dense; a task
+    // owns its range when no other task's range intersects it.
+    {
+        const int64_t lds_cap_gemm = 2 * (int64_t)abk * (abm + 1) + 2 * (int64_t)abk * (abn + 1);
+        const int64_t lds_cap_direct = gecco_direct_stage_cap();
+        struct Iv { int64_t lo, hi; size_t t; };
+        std::vector<Iv> ivs;
+        ivs.reserve(tasks.size());
+        for (size_t t = 0; t < tasks.size(); ++t) {
+            DevTask &task = tasks[t];
+            const size_t i = (size_t)task.blk;
+            const HostBlock &b = p->blocks[i];
+            task.span = 0;
+            task.owned = 0;
+            task.pad = b.direct ? (int32_t)tn[i] : 0;   // columns per direct task
+            task.lo = 0;
+            const int64_t lds_cap = b.direct ? lds_cap_direct : lds_cap_gemm;
+            const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
+            const int64_t n1 = std::min<int64_t>(b.N, task.n0 + tile_n(i));
+            int64_t amin, amax;
+            const int64_t cnt = tile_range(b.C.r, task.m0, m1, task.n0, n1, amin, amax);
+            if (!cnt) continue;
+            const int64_t key = (int64_t)b.buf12 << 44;
+            ivs.push_back({key + b.off12 + amin, key + b.off12 + amax, t});
+            const int64_t span = amax - amin + 1;
+            if (b.strided && (b.C.r.table || !b.injective) && span <= lds_cap && 4 * cnt >= span) {
+                task.span = (int32_t)span;
+                task.lo = amin;
+            }
+        }
+        std::sort(ivs.begin(), ivs.end(), [](const Iv &x, const Iv &y) { return x.lo < y.lo; });
+        for (size_t c0 = 0; c0 < ivs.size();) {
+            size_t c1 = c0 + 1;
+            int64_t hi = ivs[c0].hi;
+            while (c1 < ivs.size() && ivs[c1].lo <= hi) hi = std::max(hi, ivs[c1++].hi);
+            if (c1 == c0 + 1) tasks[ivs[c0].t].owned = 1;
+            c0 = c1;
+        }
+        p->nstaged = 0;
+        for (auto &t : tasks) p->nstaged += t.span > 0;
+    }
     std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
-        if (is_aff[x.blk] != is_aff[y.blk]) return is_aff[x.blk] > is_aff[y.blk];
+        if (cls[x.blk] != cls[y.blk]) return cls[x.blk] < cls[y.blk];
         return (x.k1 - x.k0) > (y.k1 - y.k0);
     });
-    int64_t naff = 0;
-    for (auto &t : tasks) naff += is_aff[t.blk];
-    p->flops_affine = p->flops_general = 0;
+    int64_t ncls[4] = {0, 0, 0, 0};
+    for (auto &t : tasks) ncls[cls[t.blk]]++;
+    p->flops_affine = p->flops_general = p->flops_direct = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         const double f = 2.0 * (double)p->blocks[i].M * (double)p->blocks[i].N * (double)p->blocks[i].K;
-        (is_aff[i] ? p->flops_affine : p->flops_general) += f;
+        (cls[i] == 0 ? p->flops_affine : cls[i] == 3 ? p->flops_general : p->flops_direct) += f;
+    }
+    // direct blocks read a tabled result map n-fastest
+    for (size_t i = 0; i < p->blocks.size(); ++i) {
+        auto &b = p->blocks[i];
+        if (!b.direct || !b.C.r.table) continue;
+        const Resolved &c = b.C.r;
+        std::vector<int32_t> tt((size_t)(c.F * c.S));
+        for (int64_t n = 0; n < c.S; ++n)
+            for (int64_t m = 0; m < c.F; ++m) tt[n + c.S * m] = c.first[m + c.F * n];
+        b.C.off_first_t = put(tt);
     }
     if (pool.empty()) pool.push_back(0);
     int32_t *d_index = nullptr;
@@ -604,9 +793,11 @@ static int plan_finalize(gecco_hip_plan *p) {
         DevOperand *ds[3] = {&d.A, &d.B, &d.C};
         for (int q = 0; q < 3; ++q) {
             ds[q]->first = d_index + hs[q]->off_first;
+            if (q == 2 && b.direct && hs[q]->r.table) ds[q]->first = d_index + hs[q]->off_first_t;
             ds[q]->second = hs[q]->r.table ? nullptr : d_index + hs[q]->off_second;
             ds[q]->ld = hs[q]->r.F;
             ds[q]->table = hs[q]->r.table ? 1 : 0;
+            if (q == 2 && b.direct && hs[q]->r.table) { ds[q]->ld = hs[q]->r.S; ds[q]->table = 2; }
             ds[q]->fast_first = hs[q]->fast_first;
             ds[q]->first_affine = ds[q]->second_affine = 0;
             ds[q]->f0 = ds[q]->f1 = ds[q]->s0 = ds[q]->s1 = 0;
@@ -712,7 +903,9 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
-    p->ntasks_affine = naff;
+    p->ntasks_affine = ncls[0];
+    p->ntasks_direct8 = ncls[1];
+    p->ntasks_direct32 = ncls[2];
     for (auto &y : p->syms) {
         y.npair = (int64_t)y.i1.size();
         if (!y.npair) continue;
@@ -765,7 +958,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
         p->last_bufs = want;
     }
-    const int64_t na = p->ntasks_affine, ng = p->ntasks - p->ntasks_affine;
+    const int64_t na = p->ntasks_affine, nd8 = p->ntasks_direct8, nd32 = p->ntasks_direct32;
+    const int64_t ng = p->ntasks - na - nd8 - nd32;
     auto mark = [&](int cls, bool begin) -> int {
         if (!p->timing) return 0;
         hipEvent_t e;
@@ -787,9 +981,17 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
+    if (nd8 + nd32) {
+        if (mark(4, true)) return g_err;
+        if (nd8 && gecco_launch_grouped_direct(p->d_blocks, p->d_tasks + na, (int)nd8, p->d_bufs, 8, s))
+            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (nd32 && gecco_launch_grouped_direct(p->d_blocks, p->d_tasks + na + nd8, (int)nd32, p->d_bufs, 32, s))
+            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (mark(4, false)) return g_err;
+    }
     if (ng) {
         if (mark(1, true)) return g_err;
-        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
+        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na + nd8 + nd32, (int)ng, p->d_bufs, (int)ng, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
@@ -978,6 +1180,7 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MAX_KCHUNK")) (*plan)->max_kchunk = std::max(64L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MATERIALIZE_MIN")) (*plan)->materialize_min = std::max(0L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_DIRECT_KMAX")) (*plan)->direct_kmax = std::min(32L, std::max(0L, atol(e)));
     return 0;
 }
 
@@ -1046,6 +1249,21 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *in
     info[14] = !b.C.r.table && affine_of(b.C.r.first, a0, a1);
     info[15] = !b.C.r.table && affine_of(b.C.r.second, a0, a1);
     if (ninfo >= 19) { info[16] = b.strided; info[17] = b.mat_a; info[18] = b.mat_b; }
+    if (ninfo >= 20) info[19] = b.direct;
+    return 0;
+}
+
+int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t i, int64_t which,
+                               int32_t *first, int64_t nfirst, int32_t *second, int64_t nsecond) {
+    if (!plan || i < 0 || i >= (int64_t)plan->blocks.size() || which < 0 || which > 2)
+        return fail(GECCO_HIP_EINVAL, "plan_block_index: bad arguments");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "index tables are released by finalize");
+    const HostBlock &b = plan->blocks[i];
+    const Resolved &r = which == 0 ? b.A.r : which == 1 ? b.B.r : b.C.r;
+    if ((int64_t)r.first.size() > nfirst || (int64_t)r.second.size() > nsecond)
+        return fail(GECCO_HIP_EINVAL, "plan_block_index: output too small");
+    if (first) std::copy(r.first.begin(), r.first.end(), first);
+    if (second) std::copy(r.second.begin(), r.second.end(), second);
     return 0;
 }
 
@@ -1058,8 +1276,9 @@ int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim) {
 }
 
 int gecco_hip_plan_workspace(const gecco_hip_plan *plan, int64_t *scratch_bytes,
-                             int64_t *gather_jobs, double *gather_bytes) {
+                             int64_t *gather_jobs, double *gather_bytes, int64_t *staged_tasks) {
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (staged_tasks) *staged_tasks = plan->nstaged;
     if (scratch_bytes) *scratch_bytes = 8 * plan->scratch_elems;
     if (gather_jobs) *gather_jobs = plan->ngjobs;
     if (gather_bytes) *gather_bytes = plan->gather_bytes;
@@ -1073,11 +1292,12 @@ int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on) {
 }
 
 int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops) {
-    // ms[4], launches[4]: accumulated times and counts of the strided-GEMM
-    // launches, the gather-kernel launches, the symmetrisation launches and
-    // the operand-materialisation launches since the last reset (waits for
-    // the recorded events); flops[2]: per-execute FLOP of the two contraction
-    // kernels.  Resets the accumulators.
+    // ms[5], launches[5]: accumulated times and counts of the strided-GEMM
+    // launches, the gather-kernel launches, the symmetrisation launches, the
+    // operand-materialisation launches and the direct-kernel launches since
+    // the last reset (waits for the recorded events); flops[3]: per-execute
+    // FLOP of the strided-GEMM, gather and direct kernels.  Resets the
+    // accumulators.
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
     for (auto &pe : plan->pending) {
         float t = 0.f;
@@ -1089,13 +1309,13 @@ int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, d
         (void)hipEventDestroy(pe.second.second);
     }
     plan->pending.clear();
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
         if (ms) ms[i] = plan->ms[i];
         if (launches) launches[i] = plan->launches[i];
         plan->ms[i] = 0;
         plan->launches[i] = 0;
     }
-    if (flops) { flops[0] = plan->flops_affine; flops[1] = plan->flops_general; }
+    if (flops) { flops[0] = plan->flops_affine; flops[1] = plan->flops_general; flops[2] = plan->flops_direct; }
     return 0;
 }
 

@@ -67,7 +67,7 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_free', 'gecco_hip_memcpy_htod', 'gecco_hip_memcpy_dtoh',
              'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
-             'gecco_hip_plan_workspace')
+             'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -84,7 +84,9 @@ _lib.gecco_hip_plan_set_timing.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_timing.argtypes = [ctypes.c_void_p, PD, PI64, PD]
 _lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
 _lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
-_lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD]
+_lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD, PI64]
+_lib.gecco_hip_plan_block_index.argtypes = [ctypes.c_void_p, I64, I64, ctypes.POINTER(ctypes.c_int32),
+                                            I64, ctypes.POINTER(ctypes.c_int32), I64]
 
 
 def lib():
@@ -207,18 +209,34 @@ class Plan:
         _check(_lib.gecco_hip_plan_set_timing(self._h, int(bool(on))))
 
     def timing(self):
-        """{'ms': [gemm, gather, sym, materialise], 'launches': [...],
-        'flops': [gemm, gather]} accumulated since the previous call."""
-        ms = (ctypes.c_double * 4)()
-        n = (I64 * 4)()
-        fl = (ctypes.c_double * 2)()
+        """{'ms': [gemm, gather, sym, materialise, direct], 'launches': [...],
+        'flops': [gemm, gather, direct]} accumulated since the previous call."""
+        ms = (ctypes.c_double * 5)()
+        n = (I64 * 5)()
+        fl = (ctypes.c_double * 3)()
         _check(_lib.gecco_hip_plan_timing(self._h, ms, n, fl))
         return dict(ms=list(ms), launches=list(n), flops=list(fl))
 
     def block_info(self, i):
-        info = (I64 * 19)()
-        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 19))
+        info = (I64 * 20)()
+        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 20))
         return list(info)
+
+    def block_index(self, i, which):
+        """Resolved signed index data (first, second) of operand which (0 A,
+        1 B, 2 C) of kernel call i, kernel orientation; before finalize."""
+        info = self.block_info(i)
+        M, N, K = info[0], info[1], info[2]
+        F, S = (K, M) if which == 0 else (K, N) if which == 1 else (M, N)
+        table = info[5 + which]
+        first = np.zeros(F * S if table else F, dtype=np.int32)
+        second = np.zeros(1 if table else S, dtype=np.int32)
+        _check(_lib.gecco_hip_plan_block_index(
+            self._h, i, which, first.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), first.size,
+            second.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), second.size))
+        if table:
+            return first.reshape(S, F).T, None
+        return first, second
 
     def set_materialize(self, min_dim):
         """Materialise gathered operands of kernel calls with min(M, N) >=
@@ -229,9 +247,11 @@ class Plan:
         sb = I64()
         nj = I64()
         gb = ctypes.c_double()
+        st = I64()
         _check(_lib.gecco_hip_plan_workspace(self._h, ctypes.byref(sb), ctypes.byref(nj),
-                                             ctypes.byref(gb)))
-        return dict(scratch_bytes=sb.value, gather_jobs=nj.value, gather_bytes=gb.value)
+                                             ctypes.byref(gb), ctypes.byref(st)))
+        return dict(scratch_bytes=sb.value, gather_jobs=nj.value, gather_bytes=gb.value,
+                    staged_tasks=st.value)
 
     def __del__(self):
         try:

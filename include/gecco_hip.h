@@ -138,30 +138,42 @@ int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops,
                          double *payload_bytes, double *index_bytes);
 
 /* Per-launch HIP-event timing of execute (off by default).  plan_timing
- * returns the accumulated ms and launch counts of the four launch classes
- * (ms[4], launches[4]: strided-GEMM contraction kernel, gather contraction
- * kernel, spin-flip symmetrisation, operand materialisation) since the
- * previous call, and the FLOP per execute of the two contraction kernels
- * (flops[2]); it waits for the recorded events. */
+ * returns the accumulated ms and launch counts of the five launch classes
+ * (ms[5], launches[5]: strided-GEMM contraction kernel, gather contraction
+ * kernel, spin-flip symmetrisation, operand materialisation, direct
+ * short-contraction kernel) since the previous call, and the FLOP per
+ * execute of the three contraction kernels (flops[3]: strided GEMM, gather,
+ * direct); it waits for the recorded events. */
 int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on);
 int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops);
 /* Kernel-call details for diagnostics: [M, N, K, swap, injective, A/B/C
  * table, A/B fast_first, affine flags of A/B/C first/second vectors] and,
- * with ninfo >= 19, [strided-GEMM kernel, A materialised, B materialised].
+ * with ninfo >= 19, [A and B strided, A materialised, B materialised] and,
+ * with ninfo >= 20, [direct short-contraction kernel].
  * Index flags are valid before finalize, the last three after it. */
 int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t iblock, int64_t *info,
                               int64_t ninfo);
+/* Resolved signed index data of operand `which` (0 = A, 1 = B, 2 = result C,
+ * kernel orientation) of a recorded kernel call, before finalize: entries
+ * sign*(1 + element offset), 0 = forbidden.  Separable: first[F] (K for A/B,
+ * M for C) and second[S]; table (block_info flags): first[F*S], F fastest,
+ * second unused. */
+int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t iblock, int64_t which,
+                               int32_t *first, int64_t nfirst, int32_t *second, int64_t nsecond);
 /* Operand materialisation: a kernel call whose Op1/Op2 gathers are not
  * strided views and whose result tile counts satisfy min(n_ex1, n_ex2) >=
  * min_dim has those operands written densely (signed, zeros for forbidden
  * strings) into plan-owned scratch once per execute -- the reference's
  * collect step, contract/collect_block_impr.f -- and then runs on the
- * strided-GEMM kernel.  0 disables; default 32.  Before finalize only. */
+ * strided-GEMM or direct kernel.  0 disables; default 1.  Before finalize
+ * only. */
 int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim);
-/* Scratch bytes held by a finalized plan, number of gather jobs and the HBM
- * bytes they move per execute (one read and one write per element). */
+/* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
+ * bytes they move per execute (one read and one write per element), and the
+ * number of tasks whose result tile is reduced in LDS and written back as
+ * one contiguous range (result maps that are tables or not injective). */
 int gecco_hip_plan_workspace(const gecco_hip_plan *plan, int64_t *scratch_bytes,
-                             int64_t *gather_jobs, double *gather_bytes);
+                             int64_t *gather_jobs, double *gather_bytes, int64_t *staged_tasks);
 
 /* Device memory helpers for callers without another allocator. */
 int gecco_hip_malloc(void **ptr, int64_t bytes);

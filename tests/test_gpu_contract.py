@@ -213,3 +213,47 @@ def test_materialised_gathers_share_jobs():
     two.execute(bufs)
     torch.cuda.synchronize()
     assert _relerr(bufs[2].cpu().numpy(), 2 * ref) <= TOL
+
+
+@pytest.mark.parametrize('materialize,variant', [(0, 1), (1, 1), (1, 0), (1, 4)])
+def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
+    """Every binary contraction of the CCSD formula table, as kernel calls of
+    the product enumerator, against the oracle's contr_op1op2 restatement.
+    Covers table-indexed and non-injective results (antisymmetrised outer
+    products) on the LDS-staged write-back of the strided-GEMM kernel."""
+    torch = _torch()
+    import gecco_hip
+    import ccsd
+    from oracle.melist import MEList as OMEList
+    from models import round_robin
+    monkeypatch.setenv('GECCO_HIP_AFFINE_TILE', str(variant))
+    nocc, nvir, nsym = 4, 9, 2
+    orbs = orbitals(nocc, nvir, nsym)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    dev = torch.device('cuda:0')
+    staged = 0
+    for li, level in enumerate(ccsd.LEVELS):
+        for ti, (res, a, b, cc, ca, fact) in enumerate(level):
+            print(f'term L{li}.{ti} {res}<-{a}x{b} materialize={materialize} variant={variant}', flush=True)
+            La, Lb, Lr = (OMEList(orbs, *ccsd.LISTS[n]) for n in (a, b, res))
+            from oracle.contract import Contraction
+            c = Contraction(La, Lb, Lr, cc, ca, fact)
+            x1 = spin_symmetric(La, 100 + ti)
+            x2 = spin_symmetric(Lb, 200 + ti)
+            x0 = spin_symmetric(Lr, 300 + ti)
+            ref = x0.copy()
+            c.run(x1, x2, ref, route=3, use_tr=True, screen=False)
+            A, B, R = (gecco_hip.MEList(o, *ccsd.LISTS[n]) for n in (a, b, res))
+            plan = gecco_hip.Plan()
+            plan.set_materialize(materialize)
+            plan.add_contraction(A, B, R, cc, ca, fact, True)
+            plan.add_sym_ab(R, 2)
+            plan.finalize()
+            staged += plan.workspace()['staged_tasks']
+            bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+                    torch.from_numpy(x0).to(dev)]
+            plan.execute(bufs)
+            torch.cuda.synchronize()
+            assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL, (res, a, b)
+    if materialize:
+        assert staged > 0
