@@ -109,8 +109,19 @@ class CCSD:
         self.orbs = gecco_hip.Orbs(ham.irrep_o, ham.irrep_v, ham.nsym)
         self.L = {n: gecco_hip.MEList(self.orbs, c, a) for n, (c, a) in LISTS.items()}
         self.slot = {n: i for i, n in enumerate(LISTS)}
-        self.buf = {n: torch.zeros(max(1, self.L[n].length), dtype=torch.float64, device=self.dev)
-                    for n in LISTS}
+        # T1|T2 and R1|R2 are contiguous: the amplitude vector and the
+        # residual (without its diagonal part) of the solver
+        self.n1, self.n2 = self.L['T1'].length, self.L['T2'].length
+        self.amp = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
+        self.res = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
+        self.buf = {}
+        for n in LISTS:
+            if n in ('T1', 'T2', 'R1', 'R2'):
+                v = self.amp if n[0] == 'T' else self.res
+                self.buf[n] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+            else:
+                self.buf[n] = torch.zeros(max(1, self.L[n].length), dtype=torch.float64,
+                                          device=self.dev)
         t0 = time.perf_counter()
         self._integrals()
         self._denominators()
@@ -119,8 +130,16 @@ class CCSD:
         self.plans = []
         self.flops = 0.0
         self.nblocks = 0
-        for level in LEVELS:
+        for li, level in enumerate(LEVELS):
             p = gecco_hip.Plan()
+            if li == 0:
+                # start of an iteration: intermediates from their sources or zero
+                for n in LISTS:
+                    if n in INIT:
+                        src, f = INIT[n]
+                        p.add_init(self.slot[n], self.L[n].length, buf_src=self.slot[src], factor=f)
+                    elif n not in INTEGRALS and n not in ('T1', 'T2'):
+                        p.add_init(self.slot[n], self.L[n].length)
             written = []
             for res, a, b, cc, ca, fact in level:
                 nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], cc, ca, fact, use_tr,
@@ -165,98 +184,48 @@ class CCSD:
         torch.cuda.empty_cache()
 
     def _denominators(self):
-        torch = self.torch
+        """Diagonal preconditioner of T1|T2 (dia4op_ev): eps of the C strings
+        minus eps of the A strings, built by the library on the host."""
         ham = self.ham
-        eps = torch.from_numpy(np.concatenate([ham.eps_o, ham.eps_v])).to(self.dev)
-        self.D = {}
-        for name in ('T1', 'T2'):
-            L = self.L[name]
-            so = torch.from_numpy(L.spinorbs().astype(np.int64)).to(self.dev)
-            sp = L.comp_spaces()
-            nc = sum(L.occ_c.values())
-            d = torch.zeros(L.length, dtype=torch.float64, device=self.dev)
-            for q in range(L.nel):
-                orb = so[:, q] >> 1
-                e = eps[orb if sp[q] == H else orb + ham.nocc]
-                d += -e if q < nc else e
-            self.D[name] = d
+        d = np.concatenate([self.L[n].diagonal(ham.eps_o, ham.eps_v) for n in ('T1', 'T2')])
+        self.diag = self.torch.from_numpy(d).to(self.dev)
 
     # ---------------------------------------------------------- iteration
     def evaluate(self, stream=None):
         """Energy and residual lists for the amplitudes in T1/T2 (one pass of
-        the formula)."""
-        buf = self.buf
-        torch = self.torch
-        for n in LISTS:
-            if n in INIT:
-                src, f = INIT[n]
-                if f == 1.0:
-                    buf[n].copy_(buf[src])
-                else:
-                    torch.mul(buf[src], f, out=buf[n])
-            elif n not in INTEGRALS and n not in ('T1', 'T2'):
-                buf[n].zero_()
-        sh = None if stream is None else stream
+        the formula: list initialisation, three grouped contraction levels,
+        spin-flip symmetrisation -- all native launches)."""
         for p in self.plans:
-            p.execute(self.bufs, sh)
+            p.execute(self.bufs, stream)
 
     def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True):
-        """Iterate to ||Omega|| < tol (Omega = rhs - D*t over T1 and T2), at
-        most maxit iterations.  restart: start from T = 0 with an empty DIIS
-        space; otherwise continue the previous solve.  Returns a dict with
-        energy, iterations, residual norm and timings."""
+        """Iterate to ||g|| < tol, g = D*t + R over T1|T2 (R: the formula's
+        residual lists, D: the diagonal preconditioner), at most maxit
+        iterations, with the native DIIS step (gecco_hip.DIIS).  restart:
+        start from T = 0 with an empty subspace; otherwise continue.  Returns
+        a dict with energy, iterations, residual norm and timings."""
+        import gecco_hip
         torch = self.torch
-        buf, D = self.buf, self.D
-        n1 = self.L['T1'].length
-        Dcat = torch.cat([D['T1'], D['T2']])
-        if restart or not hasattr(self, '_diis'):
-            buf['T1'].zero_()
-            buf['T2'].zero_()
-            self._diis = ([], [], np.zeros((ndiis, ndiis)))
-        hist_x, hist_e, bmat = self._diis   # DIIS overlaps <e_i|e_j>, updated by rows
+        if restart or getattr(self, '_diis', None) is None or self._diis.maxsub != ndiis:
+            self.amp.zero_()
+            self._diis = gecco_hip.DIIS(self.amp.numel(), ndiis)
+        diis = self._diis
+        stream = torch.cuda.current_stream().cuda_stream
         energy = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         it = 0
         rnorm = float('inf')
         for it in range(1, maxit + 1):
-            self.evaluate(torch.cuda.current_stream().cuda_stream)
-            t = torch.cat([buf['T1'], buf['T2']])
-            rhs = torch.cat([buf['R1'], buf['R2']])
-            omega = rhs - Dcat * t
-            rnorm = float(torch.linalg.vector_norm(omega))
-            energy = float(buf['E'][0])
+            self.evaluate(stream)
+            rnorm, energy, conv = diis.step(self.amp, self.res, self.diag, energy=self.buf['E'],
+                                            tol=tol, stream=stream)
             if on_iter:
                 on_iter(it, energy, rnorm)
             if verbose:
                 print(f'  it {it:3d}  E = {energy:.12f}  |R| = {rnorm:.3e}', flush=True)
-            if rnorm < tol:
+            if conv:
                 break
-            step = omega / Dcat
-            if len(hist_x) == ndiis:
-                hist_x.pop(0)
-                hist_e.pop(0)
-                bmat[:-1, :-1] = bmat[1:, 1:].copy()
-            hist_x.append(t + step)
-            hist_e.append(step)
-            m = len(hist_x)
-            # new row of the overlap matrix: elementwise products and
-            # reductions (a library GEMM with K = vector length and M,N <= 8
-            # runs without K-splitting and takes hundreds of ms)
-            row = torch.stack([(e * step).sum() for e in hist_e]).cpu().numpy()
-            bmat[m - 1, :m] = row
-            bmat[:m, m - 1] = row
-            Bm = np.zeros((m + 1, m + 1))
-            Bm[:m, :m] = bmat[:m, :m]
-            Bm[m, :m] = Bm[:m, m] = -1.0
-            rhs_d = np.zeros(m + 1)
-            rhs_d[m] = -1.0
-            c = np.linalg.solve(Bm, rhs_d)[:m]
-            new = hist_x[-1] * float(c[-1])
-            for ci, xi in zip(c[:-1], hist_x[:-1]):
-                new.add_(xi, alpha=float(ci))
-            buf['T1'].copy_(new[:n1])
-            buf['T2'].copy_(new[n1:])
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         return dict(energy=energy, iterations=it, residual=rnorm, seconds=wall,

@@ -654,6 +654,22 @@ __global__ __launch_bounds__(256) void gather_operands_f64(
     }
 }
 
+// List initialisation (copy with factor, or zero) ahead of the contractions.
+__global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict__ jobs,
+                                                      const DevInitTask *__restrict__ tasks,
+                                                      double *const *__restrict__ bufs) {
+    const DevInitTask t = tasks[blockIdx.x];
+    const DevInit &j = jobs[t.job];
+    double *__restrict__ dst = bufs[j.dst] + j.off_dst;
+    if (j.src >= 0) {
+        const double *__restrict__ src = bufs[j.src] + j.off_src;
+        const double f = j.factor;
+        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) dst[e] = f * src[e];
+    } else {
+        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) dst[e] = 0.0;
+    }
+}
+
 // Spin-flip symmetrisation of a result list (contract/sym_ab_blk.f:354-377):
 // for each precomputed pair (i1 <= i2, factor f = +/-relfac2):
 //   v = prefac*(x[i1] + f*x[i2]);  x[i1] = v;  x[i2] = f*v
@@ -734,6 +750,13 @@ extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::De
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::gather_operands_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
                        ntasks, bufs, scratch);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
+                                 int ntasks, double *const *bufs, hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    hipLaunchKernelGGL(gecco::init_lists_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks, bufs);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -61,6 +61,19 @@ struct DevGather {
     int32_t kfast;
 };
 
+// List initialisation before a plan's contractions: dst[i] = factor *
+// src[i] (src < 0: 0) for i < len.
+struct DevInit {
+    int64_t off_dst, off_src, len;
+    double factor;
+    int32_t dst, src;
+};
+
+struct DevInitTask {
+    int32_t job, pad;
+    int64_t e0, e1;
+};
+
 struct DevGatherTask {
     int32_t job;
     int32_t e0, e1;        // element range [e0, e1) of the job's dense array

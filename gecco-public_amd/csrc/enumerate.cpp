@@ -947,6 +947,64 @@ int gecco_hip_list_spinorbs(const gecco_hip_list *L, int16_t *out, int64_t nout)
     return 0;
 }
 
+int gecco_hip_list_diagonal(const gecco_hip_list *L, const double *eps_h, const double *eps_p,
+                            double *out, int64_t nout) {
+    // out[e] = sum of eps over the C strings - sum over the A strings
+    // (contract/dia4op_ev.f:298-365 with fac = +1 for C, -1 for A)
+    if (!L || !eps_h || !eps_p || !out) return set_error(GECCO_HIP_EINVAL, "list_diagonal: null argument");
+    if (nout < L->length) return set_error(GECCO_HIP_EINVAL, "list_diagonal: buffer too small");
+    gecco_hip_orbs *o = L->orbs;
+    const int nsym = o->nsym;
+    for (const Dist &d : L->dists) {
+        std::vector<const GraphBlock *> gb;
+        std::vector<int> occ;
+        std::vector<const double *> eps;
+        std::vector<double> fac;
+        for (size_t k = 0; k < L->oc.size(); ++k) {
+            gb.push_back(&graph(o, L->hc[k], L->oc[k]).blk[(size_t)(d.imc[k] - 1) * nsym + d.gc[k] - 1]);
+            occ.push_back(L->oc[k]);
+            eps.push_back(L->hc[k] == 1 ? eps_h : eps_p);
+            fac.push_back(1.0);
+        }
+        for (size_t k = 0; k < L->oa.size(); ++k) {
+            gb.push_back(&graph(o, L->ha[k], L->oa[k]).blk[(size_t)(d.ima[k] - 1) * nsym + d.ga[k] - 1]);
+            occ.push_back(L->oa[k]);
+            eps.push_back(L->ha[k] == 1 ? eps_h : eps_p);
+            fac.push_back(-1.0);
+        }
+        std::vector<int64_t> ld(d.ldc);
+        ld.insert(ld.end(), d.lda.begin(), d.lda.end());
+        const size_t nc = gb.size();
+        // per component: the orbital-energy sum of each string
+        std::vector<std::vector<double>> sums(nc);
+        for (size_t c = 0; c < nc; ++c) {
+            sums[c].resize(gb[c]->strs.size());
+            uint16_t so[8];
+            for (size_t i = 0; i < gb[c]->strs.size(); ++i) {
+                unpack(gb[c]->strs[i], occ[c], so);
+                double x = 0.0;
+                for (int q = 0; q < occ[c]; ++q) x += eps[c][so[q] >> 1];
+                sums[c][i] = fac[c] * x;
+            }
+        }
+        std::vector<int64_t> idx(nc, 0);
+        for (int64_t e = 0; e < d.len; ++e) {
+            int64_t off = d.off;
+            double x = 0.0;
+            for (size_t c = 0; c < nc; ++c) {
+                off += idx[c] * ld[c];
+                x += sums[c][idx[c]];
+            }
+            out[off] = x;
+            for (size_t c = 0; c < nc; ++c) {   // odometer, first component fastest
+                if (++idx[c] < (int64_t)gb[c]->strs.size()) break;
+                idx[c] = 0;
+            }
+        }
+    }
+    return 0;
+}
+
 int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *op1,
                                    const gecco_hip_list *op2, const gecco_hip_list *res,
                                    const int64_t *cnt_c, const int64_t *cnt_a, double fact,

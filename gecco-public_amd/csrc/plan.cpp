@@ -45,6 +45,8 @@ extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            double *const *bufs, int kmax, hipStream_t stream);
 extern "C" int gecco_direct_stage_cap(void);
+extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
+                                 int ntasks, double *const *bufs, hipStream_t stream);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, double *const *bufs, double *scratch,
                                    hipStream_t stream);
@@ -426,6 +428,10 @@ struct SymStep {
 
 struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
+    std::vector<gecco::DevInit> inits;            // list initialisation, first launch
+    gecco::DevInit *d_inits = nullptr;
+    gecco::DevInitTask *d_itasks = nullptr;
+    int64_t nitasks = 0;
     std::vector<SymStep> syms;
     bool finalized = false;
     gecco::DevBlock *d_blocks = nullptr;
@@ -564,6 +570,10 @@ static void plan_free_device(gecco_hip_plan *p) {
     if (p->d_tasks) (void)hipFree(p->d_tasks);
     if (p->d_index) (void)hipFree(p->d_index);
     if (p->d_bufs) (void)hipFree(p->d_bufs);
+    if (p->d_inits) (void)hipFree(p->d_inits);
+    if (p->d_itasks) (void)hipFree(p->d_itasks);
+    p->d_inits = nullptr;
+    p->d_itasks = nullptr;
     if (p->d_gjobs) (void)hipFree(p->d_gjobs);
     if (p->d_gtasks) (void)hipFree(p->d_gtasks);
     if (p->d_scratch) (void)hipFree(p->d_scratch);
@@ -624,6 +634,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
     }
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
+    for (auto &j : p->inits) nb_max = std::max<int64_t>(nb_max, std::max(j.dst, j.src) + 1);
     // Blocks whose operands A and B are strided views (affine index maps)
     // run on the strided-GEMM kernel with the tile variant p->affine_variant;
     // its epilogue scatters through any result map.  A block with gathered
@@ -903,6 +914,20 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
+    if (!p->inits.empty()) {
+        std::vector<DevInitTask> it;
+        const int64_t chunk = 16384;
+        for (size_t j = 0; j < p->inits.size(); ++j)
+            for (int64_t e0 = 0; e0 < p->inits[j].len; e0 += chunk)
+                it.push_back({(int32_t)j, 0, e0, std::min(p->inits[j].len, e0 + chunk)});
+        p->nitasks = (int64_t)it.size();
+        if (p->nitasks &&
+            (check(hipMalloc(&p->d_inits, p->inits.size() * sizeof(DevInit)), "hipMalloc(inits)") ||
+             check(hipMemcpy(p->d_inits, p->inits.data(), p->inits.size() * sizeof(DevInit), hipMemcpyHostToDevice), "hipMemcpy(inits)") ||
+             check(hipMalloc(&p->d_itasks, it.size() * sizeof(DevInitTask)), "hipMalloc(init tasks)") ||
+             check(hipMemcpy(p->d_itasks, it.data(), it.size() * sizeof(DevInitTask), hipMemcpyHostToDevice), "hipMemcpy(init tasks)")))
+            return g_err;
+    }
     p->ntasks_affine = ncls[0];
     p->ntasks_direct8 = ncls[1];
     p->ntasks_direct32 = ncls[2];
@@ -948,7 +973,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
     for (int64_t i = 0; i < p->nbufs_max; ++i)
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
-    if (p->ntasks == 0 && p->syms.empty()) return ok();
+    if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0) return ok();
     std::vector<double *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
     if (want != p->last_bufs) {
@@ -968,6 +993,9 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         else p->pending.back().second.second = e;
         return 0;
     };
+    if (p->nitasks &&
+        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, s))
+        return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
     if (p->ngtasks) {
         if (mark(3, true)) return g_err;
         if (gecco_launch_gather(p->d_gjobs, p->d_gtasks, (int)p->ngtasks, p->d_bufs, p->d_scratch, s))
@@ -1089,6 +1117,7 @@ int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &
 }
 
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+void clear_error() { g_err = 0; g_msg.clear(); }
 
 }  // namespace gecco
 
@@ -1264,6 +1293,28 @@ int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t i, int64_t wh
         return fail(GECCO_HIP_EINVAL, "plan_block_index: output too small");
     if (first) std::copy(r.first.begin(), r.first.end(), first);
     if (second) std::copy(r.second.begin(), r.second.end(), second);
+    return 0;
+}
+
+int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
+                            int32_t buf_src, int64_t off_src, double factor) {
+    g_err = 0;
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (buf_dst < 0 || off_dst < 0 || len < 0 || (buf_src >= 0 && off_src < 0))
+        return fail(GECCO_HIP_EINVAL, "add_init: bad slot, offset or length");
+    if (buf_src == buf_dst && buf_src >= 0 && off_src < off_dst + len && off_dst < off_src + len &&
+        off_src != off_dst)
+        return fail(GECCO_HIP_EINVAL, "add_init: overlapping source and destination");
+    DevInit j;
+    std::memset(&j, 0, sizeof j);
+    j.dst = buf_dst;
+    j.src = buf_src < 0 ? -1 : buf_src;
+    j.off_dst = off_dst;
+    j.off_src = buf_src < 0 ? 0 : off_src;
+    j.len = len;
+    j.factor = factor;
+    plan->inits.push_back(j);
     return 0;
 }
 

@@ -1,0 +1,313 @@
+// Amplitude solver of the MI355X engine: diagonal preconditioning and Pulay
+// DIIS on device-resident amplitude vectors, and the C ABI declared in
+// include/gecco_hip_solver.h.
+//
+// One step restates, for a vector function g(t) = diag*t + rest:
+//   r = g / diag, |diag| <= 1e-10 -> 1e-10      (baserout/diavc.f:31-35)
+//   subspace += (r, v = t - r)                  (solve/optc_diis_sbsp_add.f:150-205)
+//   B(i, j) = <r_i|r_j>, oldest dropped when full (solve/optc_update_redsp1.f)
+//   [B -1; -1 0] (w, l) = (0, -1)               (solve/optc_diis_extr.f:66-105)
+//   t = sum_i w_i v_i                           (solve/optc_expand_vec.f)
+// Device work per step: one fused pass (g, r, v, ||g||^2 and the new row of
+// B), a one-workgroup reduction in fixed order, one expansion pass; the
+// host solves the (m+1) x (m+1) system after a single synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gecco_hip_solver.h"
+
+namespace gecco {
+// error state shared with the contraction engine (plan.cpp)
+int set_error(int code, const char *msg);
+void clear_error();
+}  // namespace gecco
+
+namespace {
+
+constexpr int MAXSUB = 16;       // largest DIIS subspace
+constexpr int NTH = 256;
+constexpr int NGROUP = 1024;     // workgroups of the fused pass
+constexpr double DIA_THRESH = 1e-10;
+
+struct Slots {
+    int m;                        // vectors in the subspace after the step
+    int s[MAXSUB];                // slot of the logical vector j (j = m-1 is the new one)
+};
+
+struct Weights {
+    int m;
+    int s[MAXSUB];
+    double w[MAXSUB];
+};
+
+// g = diag*t + rest; r = g/diag; v = t - r into slot s[m-1]; per-workgroup
+// partial sums of <r_j|r> (j < m) and <g|g>, in part[blockIdx.x*(MAXSUB+1)+.]
+__global__ __launch_bounds__(NTH) void diis_prepare_f64(
+    const double *__restrict__ t, const double *__restrict__ rest,
+    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
+    int64_t n, Slots sl, double *__restrict__ part) {
+    double acc[MAXSUB + 1];
+#pragma unroll
+    for (int j = 0; j <= MAXSUB; ++j) acc[j] = 0.0;
+    const int m = sl.m;
+    double *__restrict__ rn = rsub + (int64_t)sl.s[m - 1] * n;
+    double *__restrict__ vn = vsub + (int64_t)sl.s[m - 1] * n;
+    for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
+        const double ti = t[i], d = diag[i];
+        const double g = d * ti + rest[i];
+        const double dv = fabs(d) <= DIA_THRESH ? DIA_THRESH : d;
+        const double r = g / dv;
+        rn[i] = r;
+        vn[i] = ti - r;
+        acc[MAXSUB] += g * g;
+#pragma unroll
+        for (int j = 0; j < MAXSUB; ++j) {
+            if (j < m - 1) acc[j] += r * rsub[(int64_t)sl.s[j] * n + i];
+            else if (j == m - 1) acc[j] += r * r;
+        }
+    }
+    __shared__ double red[NTH / 64][MAXSUB + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j <= MAXSUB; ++j) {
+        double v = acc[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= MAXSUB) {
+        double v = 0.0;
+        for (int w = 0; w < NTH / 64; ++w) v += red[w][threadIdx.x];
+        part[(int64_t)blockIdx.x * (MAXSUB + 1) + threadIdx.x] = v;
+    }
+}
+
+// out[j] = sum over workgroups of part[g][j] in workgroup order; out[MAXSUB+1]
+// = energy scalar (or 0)
+__global__ __launch_bounds__(NTH) void diis_reduce_f64(const double *__restrict__ part, int ngroup,
+                                                       const double *__restrict__ energy,
+                                                       double *__restrict__ out) {
+    __shared__ double red[NTH / 64][MAXSUB + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j <= MAXSUB; ++j) {
+        double v = 0.0;
+        for (int g = threadIdx.x; g < ngroup; g += NTH) v += part[(int64_t)g * (MAXSUB + 1) + j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= MAXSUB) {
+        double v = 0.0;
+        for (int w = 0; w < NTH / 64; ++w) v += red[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+    if (threadIdx.x == 0) out[MAXSUB + 1] = energy ? energy[0] : 0.0;
+}
+
+// t = sum_j w_j v_j
+__global__ __launch_bounds__(NTH) void diis_expand_f64(double *__restrict__ t,
+                                                       const double *__restrict__ vsub, int64_t n,
+                                                       Weights wt) {
+    for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
+        double x = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXSUB; ++j)
+            if (j < wt.m) x += wt.w[j] * vsub[(int64_t)wt.s[j] * n + i];
+        t[i] = x;
+    }
+}
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return gecco::set_error(code, buf);
+}
+
+int check(hipError_t e, const char *what) {
+    if (e != hipSuccess) return fail(GECCO_HIP_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+    return 0;
+}
+
+// Solve a x = b (dense, row-major, size k) by Gaussian elimination with
+// partial pivoting; returns false if a pivot vanishes.  The reference
+// factorises the packed symmetric indefinite matrix with LINPACK dspco/dspsl
+// (math/linpack.f); both give the unique solution of the augmented system.
+bool solve_dense(std::vector<double> a, std::vector<double> &b, int k) {
+    for (int c = 0; c < k; ++c) {
+        int p = c;
+        for (int r = c + 1; r < k; ++r)
+            if (std::fabs(a[r * k + c]) > std::fabs(a[p * k + c])) p = r;
+        if (a[p * k + c] == 0.0) return false;
+        if (p != c) {
+            for (int j = 0; j < k; ++j) std::swap(a[p * k + j], a[c * k + j]);
+            std::swap(b[p], b[c]);
+        }
+        for (int r = c + 1; r < k; ++r) {
+            const double f = a[r * k + c] / a[c * k + c];
+            if (f == 0.0) continue;
+            for (int j = c; j < k; ++j) a[r * k + j] -= f * a[c * k + j];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = k - 1; r >= 0; --r) {
+        double s = b[r];
+        for (int j = r + 1; j < k; ++j) s -= a[r * k + j] * b[j];
+        b[r] = s / a[r * k + r];
+    }
+    return true;
+}
+
+}  // namespace
+
+struct gecco_hip_diis {
+    int64_t n = 0;
+    int maxsub = 0;
+    double *rsub = nullptr, *vsub = nullptr;   // maxsub slots of n doubles each
+    double *part = nullptr;                    // NGROUP x (MAXSUB+1) partial sums
+    double *red = nullptr;                     // MAXSUB+2 reduced values (device)
+    double *host = nullptr;                    // pinned copy of red
+    std::vector<int> order;                    // slot of logical vector j, oldest first
+    std::vector<double> bmat;                  // <r_i|r_j>, logical order, maxsub x maxsub
+    std::vector<double> w;                     // weights of the last step
+};
+
+extern "C" {
+
+int gecco_hip_diis_create(int64_t n, int64_t maxsub, gecco_hip_diis **out) {
+    gecco::clear_error();
+    if (!out || n <= 0 || maxsub < 1 || maxsub > MAXSUB)
+        return fail(GECCO_HIP_EINVAL, "diis_create: need n > 0 and 1 <= maxsub <= %d", MAXSUB);
+    auto *d = new gecco_hip_diis();
+    d->n = n;
+    d->maxsub = (int)maxsub;
+    d->bmat.assign((size_t)maxsub * maxsub, 0.0);
+    if (check(hipMalloc(&d->rsub, (size_t)maxsub * n * sizeof(double)), "hipMalloc(diis r)") ||
+        check(hipMalloc(&d->vsub, (size_t)maxsub * n * sizeof(double)), "hipMalloc(diis v)") ||
+        check(hipMalloc(&d->part, (size_t)NGROUP * (MAXSUB + 1) * sizeof(double)), "hipMalloc(diis partials)") ||
+        check(hipMalloc(&d->red, (MAXSUB + 2) * sizeof(double)), "hipMalloc(diis reduction)") ||
+        check(hipHostMalloc(&d->host, (MAXSUB + 2) * sizeof(double)), "hipHostMalloc(diis)")) {
+        gecco_hip_diis_destroy(d);
+        return GECCO_HIP_ENOMEM;
+    }
+    *out = d;
+    return 0;
+}
+
+int gecco_hip_diis_destroy(gecco_hip_diis *d) {
+    if (!d) return 0;
+    if (d->rsub) (void)hipFree(d->rsub);
+    if (d->vsub) (void)hipFree(d->vsub);
+    if (d->part) (void)hipFree(d->part);
+    if (d->red) (void)hipFree(d->red);
+    if (d->host) (void)hipHostFree(d->host);
+    delete d;
+    return 0;
+}
+
+int gecco_hip_diis_reset(gecco_hip_diis *d) {
+    gecco::clear_error();
+    if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    d->order.clear();
+    std::fill(d->bmat.begin(), d->bmat.end(), 0.0);
+    d->w.clear();
+    return 0;
+}
+
+int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const double *diag,
+                        const double *energy_dev, double tol, double *gnorm, double *energy,
+                        int64_t *converged, void *stream) {
+    gecco::clear_error();
+    if (!d || !t || !rest || !diag) return fail(GECCO_HIP_EINVAL, "diis_step: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int ms = d->maxsub;
+    // slot of the new vector: a free one, or the oldest one's when full
+    const bool full = (int)d->order.size() == ms;
+    int slot;
+    if (full) {
+        slot = d->order.front();
+        d->order.erase(d->order.begin());
+        // drop the oldest row/column of B (optc_update_redsp1 'shift')
+        for (int i = 0; i < ms - 1; ++i)
+            for (int j = 0; j < ms - 1; ++j) d->bmat[i * ms + j] = d->bmat[(i + 1) * ms + j + 1];
+    } else {
+        slot = (int)d->order.size();
+    }
+    d->order.push_back(slot);
+    const int m = (int)d->order.size();
+    Slots sl;
+    std::memset(&sl, 0, sizeof sl);
+    sl.m = m;
+    for (int j = 0; j < m; ++j) sl.s[j] = d->order[j];
+    const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
+    hipLaunchKernelGGL(diis_prepare_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub, d->vsub,
+                       d->n, sl, d->part);
+    hipLaunchKernelGGL(diis_reduce_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->red);
+    if (check(hipGetLastError(), "diis kernels") ||
+        check(hipMemcpyAsync(d->host, d->red, (MAXSUB + 2) * sizeof(double), hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(diis)") ||
+        check(hipStreamSynchronize(s), "hipStreamSynchronize(diis)"))
+        return GECCO_HIP_EDEVICE;
+    const double gn = std::sqrt(d->host[MAXSUB]);
+    if (gnorm) *gnorm = gn;
+    if (energy) *energy = d->host[MAXSUB + 1];
+    for (int j = 0; j < m; ++j) {
+        d->bmat[(m - 1) * ms + j] = d->host[j];
+        d->bmat[j * ms + m - 1] = d->host[j];
+    }
+    if (converged) *converged = gn < tol ? 1 : 0;
+    if (gn < tol) {
+        d->w.assign(m, 0.0);
+        return 0;
+    }
+    // augmented system [B -1; -1 0] (w, l) = (0, -1)
+    const int k = m + 1;
+    std::vector<double> a((size_t)k * k, 0.0), b(k, 0.0);
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) a[i * k + j] = d->bmat[i * ms + j];
+        a[i * k + m] = -1.0;
+        a[m * k + i] = -1.0;
+    }
+    b[m] = -1.0;
+    if (!solve_dense(a, b, k))
+        return fail(GECCO_HIP_EINVAL, "DIIS matrix is singular (subspace of %d vectors)", m);
+    Weights wt;
+    std::memset(&wt, 0, sizeof wt);
+    wt.m = m;
+    d->w.assign(b.begin(), b.begin() + m);
+    for (int j = 0; j < m; ++j) {
+        wt.s[j] = d->order[j];
+        wt.w[j] = b[j];
+    }
+    hipLaunchKernelGGL(diis_expand_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, wt);
+    return check(hipGetLastError(), "diis expand");
+}
+
+int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64_t *m) {
+    if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    if (m) *m = (int64_t)d->w.size();
+    if (w)
+        for (int64_t j = 0; j < std::min<int64_t>(nw, (int64_t)d->w.size()); ++j) w[j] = d->w[j];
+    return 0;
+}
+
+int gecco_hip_diis_matrix(const gecco_hip_diis *d, double *b, int64_t nb, int64_t *m) {
+    if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    const int64_t k = (int64_t)d->order.size();
+    if (m) *m = k;
+    if (b && nb >= k * k)
+        for (int64_t i = 0; i < k; ++i)
+            for (int64_t j = 0; j < k; ++j) b[i * k + j] = d->bmat[i * d->maxsub + j];
+    return 0;
+}
+
+}  // extern "C"

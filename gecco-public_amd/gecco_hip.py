@@ -267,7 +267,10 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_orbs_strmap', 'gecco_hip_orbs_flipmap', 'gecco_hip_list_create',
                  'gecco_hip_list_destroy', 'gecco_hip_list_length', 'gecco_hip_list_ndist',
                  'gecco_hip_list_dist', 'gecco_hip_plan_add_contraction',
-                 'gecco_hip_plan_add_sym_ab', 'gecco_hip_list_spinorbs')
+                 'gecco_hip_plan_add_sym_ab', 'gecco_hip_list_spinorbs',
+                 'gecco_hip_list_diagonal')
+SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
+                  'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix')
 VP = ctypes.c_void_p
 _lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
 _lib.gecco_hip_orbs_create.restype = ctypes.c_int
@@ -297,6 +300,19 @@ _lib.gecco_hip_plan_add_sym_ab.argtypes = [VP, VP, ctypes.c_int32, I64, ctypes.c
 _lib.gecco_hip_plan_add_sym_ab.restype = ctypes.c_int
 _lib.gecco_hip_list_spinorbs.argtypes = [VP, ctypes.POINTER(ctypes.c_int16), I64]
 _lib.gecco_hip_list_spinorbs.restype = ctypes.c_int
+_lib.gecco_hip_list_diagonal.argtypes = [VP, PD, PD, PD, I64]
+_lib.gecco_hip_list_diagonal.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_init.argtypes = [VP, ctypes.c_int32, I64, I64, ctypes.c_int32, I64,
+                                         ctypes.c_double]
+_lib.gecco_hip_plan_add_init.restype = ctypes.c_int
+_lib.gecco_hip_diis_create.argtypes = [I64, I64, ctypes.POINTER(VP)]
+_lib.gecco_hip_diis_destroy.argtypes = [VP]
+_lib.gecco_hip_diis_reset.argtypes = [VP]
+_lib.gecco_hip_diis_step.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, PD, PD, PI64, VP]
+_lib.gecco_hip_diis_weights.argtypes = [VP, PD, I64, PI64]
+_lib.gecco_hip_diis_matrix.argtypes = [VP, PD, I64, PI64]
+for _n in SOLVER_SYMBOLS:
+    getattr(_lib, _n).restype = ctypes.c_int
 
 HOLE, PART = 1, 2
 
@@ -385,6 +401,16 @@ class MEList:
                                             out.size))
         return out[:, :self.nel]
 
+    def diagonal(self, eps_h, eps_p):
+        """Diagonal preconditioner (dia4op_ev): per element, the orbital
+        energies of the C strings minus those of the A strings."""
+        eh = np.ascontiguousarray(eps_h, dtype=np.float64)
+        ep = np.ascontiguousarray(eps_p, dtype=np.float64)
+        out = np.zeros(max(1, self.length))
+        _check(_lib.gecco_hip_list_diagonal(self._h, eh.ctypes.data_as(PD), ep.ctypes.data_as(PD),
+                                            out.ctypes.data_as(PD), out.size))
+        return out[:self.length]
+
     def comp_spaces(self):
         """Space (1=H, 2=P) of each electron column of spinorbs()."""
         from itertools import chain
@@ -422,5 +448,62 @@ def _plan_add_sym_ab(self, lst, buf, off=0, prefac=0.5, relfac=None):
     _check(_lib.gecco_hip_plan_add_sym_ab(self._h, lst._h, buf, off, prefac, relfac))
 
 
+def _plan_add_init(self, buf_dst, length, buf_src=-1, factor=1.0, off_dst=0, off_src=0):
+    """Initialise slot buf_dst[off_dst:off_dst+length] to factor * slot
+    buf_src (or to zero when buf_src < 0) at the start of every execute."""
+    _check(_lib.gecco_hip_plan_add_init(self._h, int(buf_dst), int(off_dst), int(length),
+                                        int(buf_src), int(off_src), float(factor)))
+
+
 Plan.add_contraction = _plan_add_contraction
 Plan.add_sym_ab = _plan_add_sym_ab
+Plan.add_init = _plan_add_init
+
+
+class DIIS:
+    """Diagonal-preconditioned Pulay DIIS on a device-resident amplitude
+    vector of n doubles (gecco_hip_diis_*)."""
+
+    def __init__(self, n, maxsub=8):
+        self._h = VP()
+        _check(_lib.gecco_hip_diis_create(int(n), int(maxsub), ctypes.byref(self._h)))
+        self.n = int(n)
+        self.maxsub = int(maxsub)
+
+    def reset(self):
+        _check(_lib.gecco_hip_diis_reset(self._h))
+
+    def step(self, t, rest, diag, energy=None, tol=0.0, stream=None):
+        """One iteration for g(t) = diag*t + rest; t is updated in place
+        unless ||g|| < tol.  Returns (||g||, energy or None, converged)."""
+        for x in (t, rest, diag):
+            if x.numel() != self.n:
+                raise GeccoHipError('diis.step: vector length mismatch')
+        gn = ctypes.c_double()
+        en = ctypes.c_double()
+        cv = I64()
+        s = VP(0 if stream is None else int(stream))
+        _check(_lib.gecco_hip_diis_step(self._h, _ptr(t), _ptr(rest), _ptr(diag),
+                                        None if energy is None else _ptr(energy), float(tol),
+                                        ctypes.byref(gn), ctypes.byref(en), ctypes.byref(cv), s))
+        return gn.value, (en.value if energy is not None else None), bool(cv.value)
+
+    def weights(self):
+        w = (ctypes.c_double * 16)()
+        m = I64()
+        _check(_lib.gecco_hip_diis_weights(self._h, w, 16, ctypes.byref(m)))
+        return np.array(w[:m.value])
+
+    def matrix(self):
+        b = (ctypes.c_double * 256)()
+        m = I64()
+        _check(_lib.gecco_hip_diis_matrix(self._h, b, 256, ctypes.byref(m)))
+        return np.array(b[:m.value * m.value]).reshape(m.value, m.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.gecco_hip_diis_destroy(self._h)
+                self._h = VP()
+        except Exception:
+            pass

@@ -160,6 +160,14 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t iblock, int64_
  * second unused. */
 int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t iblock, int64_t which,
                                int32_t *first, int64_t nfirst, int32_t *second, int64_t nsecond);
+/* List initialisation run as the first launch of every execute, before
+ * the contractions: dst[off_dst + i] = factor * src[off_src + i] for i < len,
+ * or 0 when buf_src < 0 -- the zeroing of a result that is not updated
+ * (contract/contr_op1op2_wmaps_c.f:644,680) and the copies that start an
+ * intermediate from another list (contract/frm_sched2.f, [ADD]/copy).
+ * Before finalize only. */
+int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
+                            int32_t buf_src, int64_t off_src, double factor);
 /* Operand materialisation: a kernel call whose Op1/Op2 gathers are not
  * strided views and whose result tile counts satisfy min(n_ex1, n_ex2) >=
  * min_dim has those operands written densely (signed, zeros for forbidden

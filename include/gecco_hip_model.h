@@ -62,6 +62,14 @@ int gecco_hip_list_dist(const gecco_hip_list *list, int64_t idist, int64_t *info
  * inside their space.  nout >= length*(nc+na). */
 int gecco_hip_list_spinorbs(const gecco_hip_list *list, int16_t *out, int64_t nout);
 
+/* Diagonal preconditioner of a list from orbital energies (Fock diagonal),
+ * contract/dia4op_ev.f:298-365 (one-electron part): out[e] = sum of eps
+ * over the electrons of the C strings - sum over the A strings; eps_h[nh]
+ * and eps_p[np] indexed by the orbital inside its space.  Host arrays,
+ * nout >= length. */
+int gecco_hip_list_diagonal(const gecco_hip_list *list, const double *eps_h, const double *eps_p,
+                            double *out, int64_t nout);
+
 /* Record every kernel call of  Res += fact * Op1 . Op2  (contracted part of
  * Op1's C side cnt_c[4], of Op1's A side cnt_a[4], per space type) into the
  * plan, with spin-flip skipping and factors when use_tr (the use_tr keyword,

@@ -1,0 +1,61 @@
+/*
+ * gecco_hip_solver.h -- amplitude solver of the MI355X engine: diagonal
+ * preconditioner and Pulay DIIS on device-resident amplitude vectors.
+ *
+ * Replaces the in-core route of GeCCo's DIIS optimiser for one amplitude
+ * vector (solve/optc_macit.f:229,392,398):
+ *   - the preconditioned step  r = g / D with |D| <= 1e-10 -> 1e-10
+ *     (baserout/diavc.f:1-38, called from solve/optc_diis_sbsp_add.f:155),
+ *   - the subspace vectors r and v = t - r
+ *     (solve/optc_diis_sbsp_add.f:150-205),
+ *   - the subspace matrix B(i,j) = <r_i|r_j>, oldest vector dropped when the
+ *     subspace is full (solve/optc_update_redsp1.f),
+ *   - the weights from [B -1; -1 0] (w, l) = (0, -1)
+ *     (solve/optc_diis_extr.f:66-105),
+ *   - the new amplitudes t = sum_i w_i v_i (solve/optc_expand_vec.f),
+ *   - the gradient norm ||g|| (type_xret = 1 accumulation,
+ *     contract/contr_op1op2_wmaps_c.f:2060-2061, solve/optc_macit.f).
+ * The diagonal D of a list (contract/dia4op_ev.f:300-365, one-electron
+ * part) is gecco_hip_list_diagonal in gecco_hip_model.h.
+ *
+ * All vectors are device pointers of n doubles; errors as in gecco_hip.h.
+ */
+#ifndef GECCO_HIP_SOLVER_H
+#define GECCO_HIP_SOLVER_H
+
+#include <stdint.h>
+#include "gecco_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gecco_hip_diis gecco_hip_diis;
+
+/* Subspace of at most maxsub (1..16) vectors of length n (device memory for
+ * 2*maxsub vectors is allocated here). */
+int gecco_hip_diis_create(int64_t n, int64_t maxsub, gecco_hip_diis **out);
+int gecco_hip_diis_destroy(gecco_hip_diis *d);
+/* Empty the subspace. */
+int gecco_hip_diis_reset(gecco_hip_diis *d);
+
+/* One iteration for the vector function g(t) = diag*t + rest (rest: g
+ * without its diagonal part, as the formula computes it).  Writes ||g|| to
+ * *gnorm and, if energy_dev is given, the device scalar energy_dev[0] to
+ * *energy (read in the same synchronisation).  If ||g|| < tol, *converged =
+ * 1 and t is unchanged; otherwise t is replaced by the DIIS extrapolation.
+ * Work is queued on `stream` (hipStream_t, may be 0); returns after the
+ * stream has drained the reduction. */
+int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const double *diag,
+                        const double *energy_dev, double tol, double *gnorm, double *energy,
+                        int64_t *converged, void *stream);
+
+/* Weights of the last step (w[0..m-1], oldest vector first) and the
+ * current subspace matrix (b[m*m], row-major), for checks. */
+int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64_t *m);
+int gecco_hip_diis_matrix(const gecco_hip_diis *d, double *b, int64_t nb, int64_t *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GECCO_HIP_SOLVER_H */
