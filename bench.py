@@ -35,7 +35,12 @@ METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
 # HBM bytes per launch of the ladder kernel at nocc=20 nvir=100 nsym=4, from
 # rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (copy-calibrated factors);
 # see profiles/grouped_contract_affine_f64_ccsd_ladder_o20_v100_s4_pmc.md
-MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 565.6e6}
+# HBM bytes per dispatch of the dominant kernel from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes scaled by a plain-copy calibration (summaries in
+# profiles/); for the CCSD iteration it is the average over the three
+# formula-level launches of the strided-GEMM kernel
+MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 565.6e6,
+                    ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine_f64'): 1079.5e6}
 GOLDEN = os.path.join(ROOT, 'tests', 'golden', 'ccsd_o20_v100_s4.json')
 
 
@@ -241,7 +246,9 @@ def run_ccsd(args):
                                 'gather_operands_f64': round(ms[3] / args.steps, 4)},
         'gather_operands_gbps': round(gbytes / (ms[3] / args.steps * 1e-3) / 1e9, 1) if ms[3] > 0 else None,
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
-                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4), 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
+                     'traffic': MEASURED_TRAFFIC.get(('ccsd', args.nocc, args.nvir, args.naux, args.nsym,
+                                                      names[dom])),
                      'kernel': names[dom], 'kernel_ms': round(dom_ms, 5)},
         'convergence': {'tol': args.tol, 'iterations': res['iterations'],
                         'seconds': round(res['seconds'], 4), 'energy': res['energy'],
