@@ -10,6 +10,8 @@
 // Workgroup: 256 threads = 4 wave64, tile 64x64, K step 16; each wave owns a
 // 32x32 sub-tile = 2x2 MFMA tiles.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include "contract_types.h"
 
 namespace gecco {
@@ -670,6 +672,26 @@ __global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict_
     }
 }
 
+// Partial trace rows: dst[row] += fact * sum_k sign_k * src[col_k] over the
+// row's entries (CSR, one thread per row, fixed order: no atomics).
+__global__ __launch_bounds__(256) void trace_rows_f64(const double *__restrict__ src,
+                                                      double *__restrict__ dst,
+                                                      const int64_t *__restrict__ rowptr,
+                                                      const int64_t *__restrict__ rowdst,
+                                                      const int64_t *__restrict__ col,
+                                                      const int8_t *__restrict__ sgn, int64_t nrows,
+                                                      double fact) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const double v = src[col[k]];
+            acc += sgn[k] < 0 ? -v : v;
+        }
+        dst[rowdst[r]] += fact * acc;
+    }
+}
+
 // Spin-flip symmetrisation of a result list (contract/sym_ab_blk.f:354-377):
 // for each precomputed pair (i1 <= i2, factor f = +/-relfac2):
 //   v = prefac*(x[i1] + f*x[i2]);  x[i1] = v;  x[i2] = f*v
@@ -757,6 +779,16 @@ extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevIni
                                  int ntasks, double *const *bufs, hipStream_t stream) {
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::init_lists_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks, bufs);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
+                                  const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
+                                  int64_t nrows, double fact, hipStream_t stream) {
+    if (nrows <= 0) return 0;
+    const int64_t grid = std::min<int64_t>((nrows + 255) / 256, 65536);
+    hipLaunchKernelGGL(gecco::trace_rows_f64, dim3((unsigned)grid), dim3(256), 0, stream, src, dst,
+                       rowptr, rowdst, col, sgn, nrows, fact);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

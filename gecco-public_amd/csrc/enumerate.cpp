@@ -40,6 +40,9 @@ int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *ar
 int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &&i1,
                        std::vector<int64_t> &&i2, std::vector<int8_t> &&f, double prefac);
 int set_error(int code, const char *msg);
+int plan_add_trace_rows(gecco_hip_plan *plan, int32_t buf_src, int64_t off_src, int32_t buf_dst,
+                        int64_t off_dst, double fact, std::vector<int64_t> &&dst,
+                        std::vector<int64_t> &&src, std::vector<int8_t> &&sg);
 
 static inline int mult(int a, int b) { return ((a - 1) ^ (b - 1)) + 1; }
 static const int HPVXBLKSEQ[4] = {4, 2, 3, 1};  // include/hpvxseq.h:4
@@ -717,6 +720,165 @@ static int enumerate(ContrCtx &c, bool use_tr) {
     return 0;
 }
 
+// ------------------------------------------------------ the partial trace
+// Sources (J component, OPtr component; -1 = none) of every target
+// component of a map_info array (contract/set_strmapdim_c2.f layout).
+static std::vector<std::pair<int, int>> map_sources(const std::vector<int64_t> &mi, size_t ntgt) {
+    std::vector<std::pair<int, int>> out;
+    size_t k = 0;
+    for (size_t t = 0; t < ntgt; ++t) {
+        int64_t n1 = mi[k++];
+        int a = n1 ? (int)mi[k] - 1 : -1;
+        k += n1;
+        int64_t n2 = mi[k++];
+        int b = n2 ? (int)mi[k] - 1 : -1;
+        k += n2;
+        out.push_back({a, b});
+    }
+    return out;
+}
+
+static void split_index(int64_t idx, const std::vector<int64_t> &lens, std::vector<int64_t> &out) {
+    out.resize(lens.size());
+    for (size_t i = 0; i < lens.size(); ++i) {
+        out[i] = idx % lens[i];
+        idx /= lens[i];
+    }
+}
+
+// (OPtr element, OP element, sign) of OPtr(IC,IA) += sum_J OP(J.IC, J.IA),
+// J the same string on the C and A sides: the loops of
+// contract/trace_op.f:460-700 (Ms and IRREP pairs of (OP(A), OPtr(A)),
+// distributions of OPtr, distributions of J with IRREP innermost as
+// operators/next_msgamdist_diag.f) and of contract/trace_op_blk.f:80-150.
+static int trace_triples(const gecco_hip_list *Lop, const gecco_hip_list *Ltr, const int *cnt5,
+                         std::vector<int64_t> &dst, std::vector<int64_t> &src,
+                         std::vector<int8_t> &sg) {
+    gecco_hip_orbs *o = Lop->orbs;
+    const int nsym = o->nsym;
+    const Side cnt = side_of(cnt5);
+    Side trc, tra;
+    trc.occ = Ltr->oc; trc.hpvx = Ltr->hc;
+    tra.occ = Ltr->oa; tra.hpvx = Ltr->ha;
+    const std::vector<int64_t> mic = map_info(Lop->hc, cnt, trc), mia = map_info(Lop->ha, cnt, tra);
+    const auto srcc = map_sources(mic, Lop->hc.size()), srca = map_sources(mia, Lop->ha.size());
+    int na_cnt = 0;
+    for (int x : cnt.occ) na_cnt += x;
+    const size_t ntc = Ltr->oc.size(), nta = Ltr->oa.size();
+    std::vector<std::vector<int>> mcs, gcs;
+    std::vector<int> mc, gc, ma, ga, imc, ima;
+    std::vector<int64_t> mapc, mapa, sa, sc, sj;
+    for (int ms_tr = Ltr->na; ms_tr >= -Ltr->na; ms_tr -= 2)
+        for (int ms_op = Lop->na; ms_op >= -Lop->na; ms_op -= 2) {
+            if (std::abs(ms_op) > Lop->nc || std::abs(ms_tr) > Ltr->nc) continue;
+            const int msc = ms_op - ms_tr;
+            if (std::abs(msc) > na_cnt) continue;
+            for (int g_tr = 1; g_tr <= nsym; ++g_tr)
+                for (int g_op = 1; g_op <= nsym; ++g_op) {
+                    const int gcnt = mult(g_op, g_tr);
+                    if (!block_len(Lop, ms_op, g_op)) continue;
+                    ms_dists(cnt.occ, msc, mcs);
+                    gam_dists((int)cnt.occ.size(), gcnt, nsym, gcs);
+                    for (const Dist &dt : Ltr->dists) {
+                        int mtot = 0, gtot = 1;
+                        for (int x : dt.msa) mtot += x;
+                        for (int x : dt.ga) gtot = mult(gtot, x);
+                        if (mtot != ms_tr || gtot != g_tr) continue;
+                        DistStr tC, tA;
+                        tC.ms = dt.msc; tC.gam = dt.gc; tC.idxms = dt.imc;
+                        tC.len.assign(dt.lens.begin(), dt.lens.begin() + ntc);
+                        tA.ms = dt.msa; tA.gam = dt.ga; tA.idxms = dt.ima;
+                        tA.len.assign(dt.lens.begin() + ntc, dt.lens.end());
+                        for (const auto &mj : mcs)
+                            for (const auto &gj : gcs) {
+                                DistStr cn;
+                                cn.ms = mj;
+                                cn.gam = gj;
+                                bool zero = false;
+                                for (size_t i = 0; i < cnt.occ.size(); ++i) {
+                                    cn.idxms.push_back(idxms_of(cnt.occ[i], mj[i]));
+                                    cn.len.push_back(lenstr(o, cnt.hpvx[i], cnt.occ[i], cn.idxms.back(), gj[i]));
+                                    zero |= cn.len.back() == 0;
+                                }
+                                if (zero) continue;
+                                merge_dis(mic, Lop->hc.size(), cn, tC, mc, gc);
+                                merge_dis(mia, Lop->ha.size(), cn, tA, ma, ga);
+                                imc.clear(); ima.clear();
+                                for (size_t i = 0; i < Lop->oc.size(); ++i) imc.push_back(idxms_of(Lop->oc[i], mc[i]));
+                                for (size_t i = 0; i < Lop->oa.size(); ++i) ima.push_back(idxms_of(Lop->oa[i], ma[i]));
+                                const Dist *dop = list_find(Lop, ms_op, g_op, imc, gc, ima, ga);
+                                if (!dop) continue;
+                                strmap_blk(o, Lop->hc, mic, cnt, cn, trc, tC, mapc);
+                                strmap_blk(o, Lop->ha, mia, cnt, cn, tra, tA, mapa);
+                                // per OP component: map block offset and source lengths
+                                std::vector<int64_t> offc, offa, n1c, n1a;
+                                int64_t acc = 0;
+                                for (auto &pr : srcc) {
+                                    offc.push_back(acc);
+                                    const int64_t n1 = pr.first >= 0 ? cn.len[pr.first] : 1;
+                                    const int64_t n2 = pr.second >= 0 ? tC.len[pr.second] : 1;
+                                    n1c.push_back(n1);
+                                    acc += n1 * n2;
+                                }
+                                acc = 0;
+                                for (auto &pr : srca) {
+                                    offa.push_back(acc);
+                                    const int64_t n1 = pr.first >= 0 ? cn.len[pr.first] : 1;
+                                    const int64_t n2 = pr.second >= 0 ? tA.len[pr.second] : 1;
+                                    n1a.push_back(n1);
+                                    acc += n1 * n2;
+                                }
+                                int64_t n_tra = 1, n_trc = 1, n_cnt = 1;
+                                for (auto x : tA.len) n_tra *= x;
+                                for (auto x : tC.len) n_trc *= x;
+                                for (auto x : cn.len) n_cnt *= x;
+                                for (int64_t ita = 0; ita < n_tra; ++ita) {
+                                    split_index(ita, tA.len, sa);
+                                    for (int64_t icn = 0; icn < n_cnt; ++icn) {
+                                        split_index(icn, cn.len, sj);
+                                        int sgn_a = 1;
+                                        int64_t off_a = 0;
+                                        bool ok = true;
+                                        for (size_t t = 0; t < srca.size() && ok; ++t) {
+                                            const int64_t i1 = srca[t].first >= 0 ? sj[srca[t].first] : 0;
+                                            const int64_t i2 = srca[t].second >= 0 ? sa[srca[t].second] : 0;
+                                            const int64_t e = mapa[offa[t] + i1 + n1a[t] * i2];
+                                            if (!e) { ok = false; break; }
+                                            if (e < 0) sgn_a = -sgn_a;
+                                            off_a += (std::abs(e) - 1) * dop->lda[t];
+                                        }
+                                        if (!ok) continue;
+                                        int64_t tr_a = dt.off;
+                                        for (size_t i = 0; i < nta; ++i) tr_a += sa[i] * dt.lda[i];
+                                        for (int64_t itc = 0; itc < n_trc; ++itc) {
+                                            split_index(itc, tC.len, sc);
+                                            int sgn = sgn_a;
+                                            int64_t off = dop->off + off_a;
+                                            bool okc = true;
+                                            for (size_t t = 0; t < srcc.size(); ++t) {
+                                                const int64_t i1 = srcc[t].first >= 0 ? sj[srcc[t].first] : 0;
+                                                const int64_t i2 = srcc[t].second >= 0 ? sc[srcc[t].second] : 0;
+                                                const int64_t e = mapc[offc[t] + i1 + n1c[t] * i2];
+                                                if (!e) { okc = false; break; }
+                                                if (e < 0) sgn = -sgn;
+                                                off += (std::abs(e) - 1) * dop->ldc[t];
+                                            }
+                                            if (!okc) continue;
+                                            int64_t tr = tr_a;
+                                            for (size_t i = 0; i < ntc; ++i) tr += sc[i] * dt.ldc[i];
+                                            dst.push_back(tr);
+                                            src.push_back(off);
+                                            sg.push_back((int8_t)sgn);
+                                        }
+                                    }
+                                }
+                            }
+                    }
+                }
+        }
+    return 0;
+}
+
 // sym_ab pairs of a list (contract/sym_ab_blk.f:214-377)
 static int sym_pairs(gecco_hip_list *L, int64_t off, std::vector<int64_t> &i1,
                      std::vector<int64_t> &i2, std::vector<int8_t> &fs, double relfac) {
@@ -1048,6 +1210,27 @@ int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *o
     if (nblocks) *nblocks = c.nblocks;
     if (flops) *flops = c.flops;
     return rc;
+}
+
+int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
+                             const gecco_hip_list *trop, const int64_t *cnt, double fact,
+                             int32_t buf_op, int64_t off_op, int32_t buf_trop, int64_t off_trop,
+                             int64_t *nterms) {
+    if (!plan || !op || !trop || !cnt) return set_error(GECCO_HIP_EINVAL, "add_trace: null argument");
+    if (op->orbs != trop->orbs) return set_error(GECCO_HIP_EINVAL, "add_trace: lists of different orbital sets");
+    int c5[5] = {0};
+    for (int h = 1; h <= 4; ++h) {
+        c5[h] = (int)cnt[h - 1];
+        if (c5[h] < 0 || op->occ_c[h] - c5[h] != trop->occ_c[h] || op->occ_a[h] - c5[h] != trop->occ_a[h])
+            return set_error(GECCO_HIP_EINVAL, "add_trace: OP occupation minus J does not match OPtr");
+    }
+    std::vector<int64_t> dst, src;
+    std::vector<int8_t> sg;
+    int rc = trace_triples(op, trop, c5, dst, src, sg);
+    if (rc) return rc;
+    if (nterms) *nterms = (int64_t)dst.size();
+    return plan_add_trace_rows(plan, buf_op, off_op, buf_trop, off_trop, fact, std::move(dst),
+                               std::move(src), std::move(sg));
 }
 
 int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *L, int32_t buf, int64_t off,

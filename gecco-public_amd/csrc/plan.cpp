@@ -45,6 +45,9 @@ extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            double *const *bufs, int kmax, hipStream_t stream);
 extern "C" int gecco_direct_stage_cap(void);
+extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
+                                  const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
+                                  int64_t nrows, double fact, hipStream_t stream);
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
                                  int ntasks, double *const *bufs, hipStream_t stream);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
@@ -426,8 +429,21 @@ struct SymStep {
     int8_t *d_f = nullptr;
 };
 
+// Partial trace of a list into another (CSR rows of signed sources).
+struct TraceStep {
+    int32_t buf_src = 0, buf_dst = 0;
+    int64_t off_src = 0, off_dst = 0;
+    double fact = 1.0;
+    std::vector<int64_t> rowptr, rowdst, col;
+    std::vector<int8_t> sgn;
+    int64_t nrows = 0, nnz = 0;
+    int64_t *d_rowptr = nullptr, *d_rowdst = nullptr, *d_col = nullptr;
+    int8_t *d_sgn = nullptr;
+};
+
 struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
+    std::vector<TraceStep> traces;                // after the contractions
     std::vector<gecco::DevInit> inits;            // list initialisation, first launch
     gecco::DevInit *d_inits = nullptr;
     gecco::DevInitTask *d_itasks = nullptr;
@@ -570,6 +586,14 @@ static void plan_free_device(gecco_hip_plan *p) {
     if (p->d_tasks) (void)hipFree(p->d_tasks);
     if (p->d_index) (void)hipFree(p->d_index);
     if (p->d_bufs) (void)hipFree(p->d_bufs);
+    for (auto &tr : p->traces) {
+        if (tr.d_rowptr) (void)hipFree(tr.d_rowptr);
+        if (tr.d_rowdst) (void)hipFree(tr.d_rowdst);
+        if (tr.d_col) (void)hipFree(tr.d_col);
+        if (tr.d_sgn) (void)hipFree(tr.d_sgn);
+        tr.d_rowptr = tr.d_rowdst = tr.d_col = nullptr;
+        tr.d_sgn = nullptr;
+    }
     if (p->d_inits) (void)hipFree(p->d_inits);
     if (p->d_itasks) (void)hipFree(p->d_itasks);
     p->d_inits = nullptr;
@@ -635,6 +659,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     }
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
     for (auto &j : p->inits) nb_max = std::max<int64_t>(nb_max, std::max(j.dst, j.src) + 1);
+    for (auto &tr : p->traces) nb_max = std::max<int64_t>(nb_max, std::max(tr.buf_src, tr.buf_dst) + 1);
     // Blocks whose operands A and B are strided views (affine index maps)
     // run on the strided-GEMM kernel with the tile variant p->affine_variant;
     // its epilogue scatters through any result map.  A block with gathered
@@ -914,6 +939,22 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
+    for (auto &tr : p->traces) {
+        if (!tr.nrows) continue;
+        if (check(hipMalloc(&tr.d_rowptr, (tr.nrows + 1) * 8), "hipMalloc(trace)") ||
+            check(hipMalloc(&tr.d_rowdst, tr.nrows * 8), "hipMalloc(trace)") ||
+            check(hipMalloc(&tr.d_col, tr.nnz * 8), "hipMalloc(trace)") ||
+            check(hipMalloc(&tr.d_sgn, tr.nnz), "hipMalloc(trace)") ||
+            check(hipMemcpy(tr.d_rowptr, tr.rowptr.data(), (tr.nrows + 1) * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
+            check(hipMemcpy(tr.d_rowdst, tr.rowdst.data(), tr.nrows * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
+            check(hipMemcpy(tr.d_col, tr.col.data(), tr.nnz * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
+            check(hipMemcpy(tr.d_sgn, tr.sgn.data(), tr.nnz, hipMemcpyHostToDevice), "hipMemcpy(trace)"))
+            return g_err;
+        std::vector<int64_t>().swap(tr.rowptr);
+        std::vector<int64_t>().swap(tr.rowdst);
+        std::vector<int64_t>().swap(tr.col);
+        std::vector<int8_t>().swap(tr.sgn);
+    }
     if (!p->inits.empty()) {
         std::vector<DevInitTask> it;
         const int64_t chunk = 16384;
@@ -973,7 +1014,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
     for (int64_t i = 0; i < p->nbufs_max; ++i)
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
-    if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0) return ok();
+    if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty()) return ok();
     std::vector<double *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
     if (want != p->last_bufs) {
@@ -1023,6 +1064,11 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
+    for (auto &tr : p->traces)
+        if (tr.nrows && gecco_launch_trace(bufs[tr.buf_src] + tr.off_src, bufs[tr.buf_dst] + tr.off_dst,
+                                           tr.d_rowptr, tr.d_rowdst, tr.d_col, tr.d_sgn, tr.nrows,
+                                           tr.fact, s))
+            return fail(GECCO_HIP_EDEVICE, "trace launch failed");
     bool anysym = false;
     for (auto &y : p->syms) anysym |= y.npair > 0;
     if (anysym && mark(2, true)) return g_err;
@@ -1113,6 +1159,39 @@ int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &
     y.i2 = std::move(i2);
     y.f = std::move(f);
     plan->syms.push_back(std::move(y));
+    return ok();
+}
+
+int plan_add_trace_rows(gecco_hip_plan *plan, int32_t buf_src, int64_t off_src, int32_t buf_dst,
+                        int64_t off_dst, double fact, std::vector<int64_t> &&dst,
+                        std::vector<int64_t> &&src, std::vector<int8_t> &&sg) {
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (buf_src < 0 || buf_dst < 0 || off_src < 0 || off_dst < 0)
+        return fail(GECCO_HIP_EINVAL, "negative buffer slot or offset");
+    TraceStep tr;
+    tr.buf_src = buf_src;
+    tr.buf_dst = buf_dst;
+    tr.off_src = off_src;
+    tr.off_dst = off_dst;
+    tr.fact = fact;
+    // CSR by destination; entries of one row keep the enumeration order
+    std::vector<size_t> ord(dst.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return dst[a] < dst[b]; });
+    tr.rowptr.push_back(0);
+    for (size_t j = 0; j < ord.size(); ++j) {
+        const size_t i = ord[j];
+        if (tr.rowdst.empty() || tr.rowdst.back() != dst[i]) {
+            if (!tr.rowdst.empty()) tr.rowptr.push_back((int64_t)j);
+            tr.rowdst.push_back(dst[i]);
+        }
+        tr.col.push_back(src[i]);
+        tr.sgn.push_back(sg[i]);
+    }
+    if (!tr.rowdst.empty()) tr.rowptr.push_back((int64_t)ord.size());
+    tr.nrows = (int64_t)tr.rowdst.size();
+    tr.nnz = (int64_t)tr.col.size();
+    plan->traces.push_back(std::move(tr));
     return ok();
 }
 

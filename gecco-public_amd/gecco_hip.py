@@ -268,7 +268,7 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_list_destroy', 'gecco_hip_list_length', 'gecco_hip_list_ndist',
                  'gecco_hip_list_dist', 'gecco_hip_plan_add_contraction',
                  'gecco_hip_plan_add_sym_ab', 'gecco_hip_list_spinorbs',
-                 'gecco_hip_list_diagonal')
+                 'gecco_hip_list_diagonal', 'gecco_hip_plan_add_trace')
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
                   'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix')
 VP = ctypes.c_void_p
@@ -301,6 +301,9 @@ _lib.gecco_hip_plan_add_sym_ab.restype = ctypes.c_int
 _lib.gecco_hip_list_spinorbs.argtypes = [VP, ctypes.POINTER(ctypes.c_int16), I64]
 _lib.gecco_hip_list_spinorbs.restype = ctypes.c_int
 _lib.gecco_hip_list_diagonal.argtypes = [VP, PD, PD, PD, I64]
+_lib.gecco_hip_plan_add_trace.argtypes = [VP, VP, VP, PI64, ctypes.c_double, ctypes.c_int32, I64,
+                                          ctypes.c_int32, I64, PI64]
+_lib.gecco_hip_plan_add_trace.restype = ctypes.c_int
 _lib.gecco_hip_list_diagonal.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_init.argtypes = [VP, ctypes.c_int32, I64, I64, ctypes.c_int32, I64,
                                          ctypes.c_double]
@@ -455,7 +458,19 @@ def _plan_add_init(self, buf_dst, length, buf_src=-1, factor=1.0, off_dst=0, off
                                         int(buf_src), int(off_src), float(factor)))
 
 
+def _plan_add_trace(self, op, trop, cnt, fact=1.0, buf=(0, 1), off=(0, 0)):
+    """Record OPtr += fact * sum_J OP(J.IC, J.IA) (trace_op); buf/off =
+    slots and offsets of (OP, OPtr).  Returns the number of element terms."""
+    c = _occ4(cnt)
+    n = I64()
+    _check(_lib.gecco_hip_plan_add_trace(self._h, op._h, trop._h, c.ctypes.data_as(PI64),
+                                         float(fact), buf[0], off[0], buf[1], off[1],
+                                         ctypes.byref(n)))
+    return n.value
+
+
 Plan.add_contraction = _plan_add_contraction
+Plan.add_trace = _plan_add_trace
 Plan.add_sym_ab = _plan_add_sym_ab
 Plan.add_init = _plan_add_init
 

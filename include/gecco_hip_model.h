@@ -83,6 +83,18 @@ int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *o
                                    int64_t use_tr, int32_t buf1, int64_t off1, int32_t buf2,
                                    int64_t off2, int32_t buf12, int64_t off12, int64_t *nblocks,
                                    double *flops);
+/* Partial trace (self-contraction) of a list, run after the plan's
+ * contractions:  OPtr(IC, IA) += fact * sum_J s * OP(J.IC, J.IA), with the
+ * same string J (occupation cnt[4] per space type H, P, V, X) removed from
+ * the C and the A side of OP.  Replaces contract/trace_op.f:2-830 with its
+ * kernel contract/trace_op_blk.f (dispatched from contract/contr_op1op2.f:162
+ * for self-contractions); trop may be the scalar list (a full trace, the
+ * xret case).  Returns the number of element terms. */
+int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
+                             const gecco_hip_list *trop, const int64_t *cnt, double fact,
+                             int32_t buf_op, int64_t off_op, int32_t buf_trop, int64_t off_trop,
+                             int64_t *nterms);
+
 /* Spin-flip symmetrisation of a result list after the plan's contractions
  * (contract/sym_ab_blk.f, called with prefac 1/2, relfac absym at
  * contract/contr_op1op2_wmaps_c.f:2099-2102). */

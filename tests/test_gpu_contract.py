@@ -257,3 +257,39 @@ def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
             assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL, (res, a, b)
     if materialize:
         assert staged > 0
+
+
+@pytest.mark.parametrize('nsym', [1, 2, 4])
+@pytest.mark.parametrize('name', ['pp_over_p', 'pp_full', 'hp_over_h', 'hp_full', 'ph_over_p',
+                                  'p_full'])
+def test_trace_matches_oracle(name, nsym):
+    """Partial trace (trace_op) on the GPU against the oracle restatement,
+    accumulating onto existing data, in a plan after a contraction."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    from oracle.melist import MEList as OMEList
+    from oracle.trace import Trace
+    from test_oracle_trace import CASES as TCASES
+    print(f'trace {name} nsym={nsym}', flush=True)
+    (occ_c, occ_a), cnt = TCASES[name]
+    orbs = orbitals(4, 7, nsym)
+    op = OMEList(orbs, occ_c, occ_a)
+    tr_c = {h: n - cnt.get(h, 0) for h, n in occ_c.items() if n - cnt.get(h, 0)}
+    tr_a = {h: n - cnt.get(h, 0) for h, n in occ_a.items() if n - cnt.get(h, 0)}
+    trop = OMEList(orbs, tr_c, tr_a)
+    x = spin_symmetric(op, 11)
+    x0 = spin_symmetric(trop, 12)
+    ref = Trace(op, trop, cnt, fact=-0.5).run(x, x0.copy())
+    o = gecco_hip.Orbs(round_robin(4, nsym), round_robin(7, nsym, 1), nsym)
+    A = gecco_hip.MEList(o, occ_c, occ_a)
+    T = gecco_hip.MEList(o, tr_c, tr_a)
+    p = gecco_hip.Plan()
+    n = p.add_trace(A, T, cnt, -0.5, buf=(0, 1))
+    assert n > 0
+    p.finalize()
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x).to(dev), torch.from_numpy(x0).to(dev)]
+    p.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[1].cpu().numpy(), ref) <= TOL
