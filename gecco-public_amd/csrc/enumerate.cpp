@@ -27,6 +27,7 @@
 #include <array>
 #include <cstring>
 #include <map>
+#include <array>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -323,9 +324,12 @@ struct Dist {
 
 struct gecco_hip_list {
     gecco_hip_orbs *orbs = nullptr;
-    int occ_c[5] = {0}, occ_a[5] = {0};
+    int occ_c[5] = {0}, occ_a[5] = {0};     // summed over the vertices
+    int njoined = 1;
+    std::vector<std::array<int, 10>> vocc;  // per vertex: C[1..4] at 1..4, A[1..4] at 6..9
     int absym = 1;
-    std::vector<int> oc, hc, oa, ha;  // condensed (hpvxblkseq order)
+    std::vector<int> oc, hc, oa, ha;  // condensed (hpvxblkseq order, vertex inner)
+    std::vector<int> vc, va;          // vertex of every condensed component
     int nc = 0, na = 0;
     std::vector<gecco::Dist> dists;
     std::map<std::vector<int>, int> find;  // (idxmsa, igama, imc, gc, ima, ga) -> dist
@@ -370,18 +374,21 @@ static std::vector<int> find_key(int idxmsa, int igama, const std::vector<int> &
     return k;
 }
 
-static void condense(const int *occ_c, const int *occ_a, std::vector<int> &oc, std::vector<int> &hc,
-                     std::vector<int> &oa, std::vector<int> &ha) {
-    for (int h : HPVXBLKSEQ) {
-        if (occ_c[h] > 0) { oc.push_back(occ_c[h]); hc.push_back(h); }
-        if (occ_a[h] > 0) { oa.push_back(occ_a[h]); ha.push_back(h); }
-    }
+// operators/condense_occ.f: the vertex loop inside the hpvxblkseq loop
+static void condense_n(const std::vector<std::array<int, 10>> &vocc, std::vector<int> &oc,
+                       std::vector<int> &hc, std::vector<int> &vc, std::vector<int> &oa,
+                       std::vector<int> &ha, std::vector<int> &va) {
+    for (int h : HPVXBLKSEQ)
+        for (size_t v = 0; v < vocc.size(); ++v) {
+            if (vocc[v][h] > 0) { oc.push_back(vocc[v][h]); hc.push_back(h); vc.push_back((int)v); }
+            if (vocc[v][5 + h] > 0) { oa.push_back(vocc[v][5 + h]); ha.push_back(h); va.push_back((int)v); }
+        }
 }
 
 static void build_list(gecco_hip_list *L) {
     gecco_hip_orbs *o = L->orbs;
     const int nsym = o->nsym;
-    condense(L->occ_c, L->occ_a, L->oc, L->hc, L->oa, L->ha);
+    condense_n(L->vocc, L->oc, L->hc, L->vc, L->oa, L->ha, L->va);
     for (int x : L->oc) L->nc += x;
     for (int x : L->oa) L->na += x;
     int64_t idx = 0;
@@ -879,6 +886,224 @@ static int trace_triples(const gecco_hip_list *Lop, const gecco_hip_list *Ltr, c
     return 0;
 }
 
+// ------------------------------------------------------------ the reorder
+// map_info of one side of a reorder: per target component (vertex v,
+// space h) the I0 component with the same (v, h) and, on vertex kv, the K
+// component of space h; k_first puts K as source 1 (A side:
+// contract/reo_blk_wmaps_c.f:517-548 builds the A maps from (K, I0)).
+static std::vector<int64_t> reo_map_info(const std::vector<int> &th, const std::vector<int> &tv,
+                                         const std::vector<int> &i0h, const std::vector<int> &i0v,
+                                         const std::vector<int> &kh, int kv, bool k_first) {
+    std::vector<int64_t> mi;
+    for (size_t t = 0; t < th.size(); ++t) {
+        std::vector<int64_t> i0, k;
+        for (size_t j = 0; j < i0h.size(); ++j)
+            if (i0h[j] == th[t] && i0v[j] == tv[t]) i0.push_back((int64_t)j + 1);
+        if (tv[t] == kv)
+            for (size_t j = 0; j < kh.size(); ++j)
+                if (kh[j] == th[t]) k.push_back((int64_t)j + 1);
+        const auto &s1 = k_first ? k : i0;
+        const auto &s2 = k_first ? i0 : k;
+        mi.push_back((int64_t)s1.size());
+        mi.insert(mi.end(), s1.begin(), s1.end());
+        mi.push_back((int64_t)s2.size());
+        mi.insert(mi.end(), s2.begin(), s2.end());
+    }
+    return mi;
+}
+
+// contract/split_msgmdis.f for one side: the Ms/IRREP of the unknown (I0)
+// components from the target distribution and the known (K) ones
+static bool split_dis(const std::vector<int64_t> &mi, size_t ntgt, const std::vector<int> &ms_t,
+                      const std::vector<int> &g_t, const DistStr &known, bool known_first,
+                      size_t n_unknown, std::vector<int> &ms, std::vector<int> &gam) {
+    ms.assign(n_unknown, INT32_MIN);
+    gam.assign(n_unknown, 0);
+    size_t k = 0;
+    for (size_t t = 0; t < ntgt; ++t) {
+        std::vector<int64_t> s[2];
+        for (int q = 0; q < 2; ++q) {
+            const int64_t n = mi[k++];
+            for (int64_t i = 0; i < n; ++i) s[q].push_back(mi[k++] - 1);
+        }
+        const auto &kn = known_first ? s[0] : s[1];
+        const auto &un = known_first ? s[1] : s[0];
+        int m = ms_t[t], g = g_t[t];
+        for (auto j : kn) { m -= known.ms[j]; g = mult(g, known.gam[j]); }
+        if (un.empty()) {
+            if (m != 0 || g != 1) return false;
+            continue;
+        }
+        const int64_t j = un[0];
+        if (ms[j] != INT32_MIN && (ms[j] != m || gam[j] != g)) return false;
+        ms[j] = m;
+        gam[j] = g;
+    }
+    for (auto x : ms)
+        if (x == INT32_MIN) return false;
+    return true;
+}
+
+// (REO element, ORI element, sign) of REO(I') += sign_reo s_ori s_reo ORI(I)
+// for the shift string K moving from vertex frm of ORI to vertex to of REO:
+// the loops of contract/reo_op_wmaps_c.f:380-560 (ORI distributions) and
+// contract/reo_blk_wmaps_c.f:210-840 (Ms and IRREP of K, K distributions,
+// split/merge, string loops K(A), I0(A), K(C), I0(C)).
+static int reorder_triples(const gecco_hip_list *Lo, const gecco_hip_list *Lr, int frm, int to,
+                           const int *kc5, const int *ka5, int sign_reo,
+                           std::vector<int64_t> &dst, std::vector<int64_t> &src,
+                           std::vector<int8_t> &sg) {
+    gecco_hip_orbs *o = Lo->orbs;
+    const int nsym = o->nsym;
+    // I0 = ORI without K on vertex frm; REO must be I0 with K on vertex to
+    std::vector<std::array<int, 10>> i0 = Lo->vocc, chk;
+    for (int h = 1; h <= 4; ++h) {
+        i0[frm][h] -= kc5[h];
+        i0[frm][5 + h] -= ka5[h];
+        if (i0[frm][h] < 0 || i0[frm][5 + h] < 0)
+            return set_error(GECCO_HIP_EINVAL, "add_reorder: K is not part of the from-vertex");
+    }
+    chk = i0;
+    if ((int)chk.size() < Lr->njoined) chk.resize(Lr->njoined, std::array<int, 10>{});
+    for (int h = 1; h <= 4; ++h) { chk[to][h] += kc5[h]; chk[to][5 + h] += ka5[h]; }
+    if ((int)chk.size() != Lr->njoined) return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex counts differ");
+    for (int v = 0; v < Lr->njoined; ++v)
+        for (int x = 0; x < 10; ++x)
+            if (chk[v][x] != Lr->vocc[v][x])
+                return set_error(GECCO_HIP_EINVAL, "add_reorder: REO is not I0 with K on the to-vertex");
+    std::vector<int> i0c_o, i0c_h, i0c_v, i0a_o, i0a_h, i0a_v;
+    condense_n(i0, i0c_o, i0c_h, i0c_v, i0a_o, i0a_h, i0a_v);
+    Side kc = side_of(kc5), ka = side_of(ka5);
+    Side i0c, i0a;
+    i0c.occ = i0c_o; i0c.hpvx = i0c_h;
+    i0a.occ = i0a_o; i0a.hpvx = i0a_h;
+    const auto mi_oc = reo_map_info(Lo->hc, Lo->vc, i0c_h, i0c_v, kc.hpvx, frm, false);
+    const auto mi_oa = reo_map_info(Lo->ha, Lo->va, i0a_h, i0a_v, ka.hpvx, frm, true);
+    const auto mi_rc = reo_map_info(Lr->hc, Lr->vc, i0c_h, i0c_v, kc.hpvx, to, false);
+    const auto mi_ra = reo_map_info(Lr->ha, Lr->va, i0a_h, i0a_v, ka.hpvx, to, true);
+    const auto so_c = map_sources(mi_oc, Lo->hc.size()), so_a = map_sources(mi_oa, Lo->ha.size());
+    const auto sr_c = map_sources(mi_rc, Lr->hc.size()), sr_a = map_sources(mi_ra, Lr->ha.size());
+    int kc_max = 0, ka_max = 0, i0c_max = 0, i0a_max = 0;
+    for (int x : kc.occ) kc_max += x;
+    for (int x : ka.occ) ka_max += x;
+    for (int x : i0c_o) i0c_max += x;
+    for (int x : i0a_o) i0a_max += x;
+    std::vector<DistStr> DK;
+    std::vector<int> mc, gc, ma, ga, rmc, rgc, rma, rga, imc, ima;
+    std::vector<int64_t> m_oc, m_oa, m_rc, m_ra, ska, sia, skc, sic;
+    // per target component of a map: (source-1 comp, source-2 comp, n1, offset)
+    struct Tg { int a, b; int64_t n1, off; };
+    auto targets = [](const std::vector<std::pair<int, int>> &srcs, const DistStr &d1,
+                      const DistStr &d2) {
+        std::vector<Tg> t;
+        int64_t acc = 0;
+        for (auto &pr : srcs) {
+            const int64_t n1 = pr.first >= 0 ? d1.len[pr.first] : 1;
+            const int64_t n2 = pr.second >= 0 ? d2.len[pr.second] : 1;
+            t.push_back({pr.first, pr.second, n1, acc});
+            acc += n1 * n2;
+        }
+        return t;
+    };
+    auto addr = [](const std::vector<Tg> &tg, const std::vector<int64_t> &map,
+                   const std::vector<int64_t> &ld, const std::vector<int64_t> &i1,
+                   const std::vector<int64_t> &i2, int &sgn, int64_t &off) {
+        for (size_t t = 0; t < tg.size(); ++t) {
+            const int64_t x1 = tg[t].a >= 0 ? i1[tg[t].a] : 0;
+            const int64_t x2 = tg[t].b >= 0 ? i2[tg[t].b] : 0;
+            const int64_t e = map[tg[t].off + x1 + tg[t].n1 * x2];
+            if (!e) return false;
+            if (e < 0) sgn = -sgn;
+            off += (std::abs(e) - 1) * ld[t];
+        }
+        return true;
+    };
+    for (const Dist &d : Lo->dists) {
+        int ms_op_c = 0, ms_op_a = 0, g_op = 1;
+        for (int x : d.msc) ms_op_c += x;
+        for (int x : d.msa) ms_op_a += x;
+        for (int x : d.ga) g_op = mult(g_op, x);
+        for (int ms_ka = ka_max; ms_ka >= -ka_max; ms_ka -= 2) {
+            if (std::abs(ms_op_a - ms_ka) > i0a_max) continue;
+            for (int ms_kc = kc_max; ms_kc >= -kc_max; ms_kc -= 2) {
+                if (std::abs(ms_op_c - ms_kc) > i0c_max) continue;
+                for (int g_ka = 1; g_ka <= nsym; ++g_ka)
+                    for (int g_kc = 1; g_kc <= nsym; ++g_kc) {
+                        dist_strings(o, kc, ka, ms_kc, ms_ka, g_kc, g_ka, DK);
+                        for (const DistStr &dk : DK) {
+                            const DistStr dkc = part(dk, 0, kc.occ.size()), dka = part(dk, kc.occ.size(), ka.occ.size());
+                            if (!split_dis(mi_oc, Lo->hc.size(), d.msc, d.gc, dkc, false, i0c_o.size(), mc, gc) ||
+                                !split_dis(mi_oa, Lo->ha.size(), d.msa, d.ga, dka, true, i0a_o.size(), ma, ga))
+                                continue;
+                            DistStr di0c, di0a;
+                            bool bad = false;
+                            for (size_t i = 0; i < i0c_o.size() && !bad; ++i) {
+                                if (std::abs(mc[i]) > i0c_o[i] || (i0c_o[i] - mc[i]) % 2) { bad = true; break; }
+                                di0c.ms.push_back(mc[i]); di0c.gam.push_back(gc[i]);
+                                di0c.idxms.push_back(idxms_of(i0c_o[i], mc[i]));
+                                di0c.len.push_back(lenstr(o, i0c_h[i], i0c_o[i], di0c.idxms.back(), gc[i]));
+                                bad = di0c.len.back() == 0;
+                            }
+                            for (size_t i = 0; i < i0a_o.size() && !bad; ++i) {
+                                if (std::abs(ma[i]) > i0a_o[i] || (i0a_o[i] - ma[i]) % 2) { bad = true; break; }
+                                di0a.ms.push_back(ma[i]); di0a.gam.push_back(ga[i]);
+                                di0a.idxms.push_back(idxms_of(i0a_o[i], ma[i]));
+                                di0a.len.push_back(lenstr(o, i0a_h[i], i0a_o[i], di0a.idxms.back(), ga[i]));
+                                bad = di0a.len.back() == 0;
+                            }
+                            if (bad) continue;
+                            merge_dis(mi_rc, Lr->hc.size(), di0c, dkc, rmc, rgc);
+                            merge_dis(mi_ra, Lr->ha.size(), dka, di0a, rma, rga);
+                            imc.clear(); ima.clear();
+                            int ms_r = 0;
+                            for (size_t i = 0; i < Lr->oc.size(); ++i) imc.push_back(idxms_of(Lr->oc[i], rmc[i]));
+                            for (size_t i = 0; i < Lr->oa.size(); ++i) { ima.push_back(idxms_of(Lr->oa[i], rma[i])); ms_r += rma[i]; }
+                            const Dist *dr = list_find(Lr, ms_r, g_op, imc, rgc, ima, rga);
+                            if (!dr) continue;
+                            strmap_blk(o, Lo->hc, mi_oc, i0c, di0c, kc, dkc, m_oc);
+                            strmap_blk(o, Lo->ha, mi_oa, ka, dka, i0a, di0a, m_oa);
+                            strmap_blk(o, Lr->hc, mi_rc, i0c, di0c, kc, dkc, m_rc);
+                            strmap_blk(o, Lr->ha, mi_ra, ka, dka, i0a, di0a, m_ra);
+                            const auto t_oc = targets(so_c, di0c, dkc), t_oa = targets(so_a, dka, di0a);
+                            const auto t_rc = targets(sr_c, di0c, dkc), t_ra = targets(sr_a, dka, di0a);
+                            int64_t nka = 1, nia = 1, nkc = 1, nic = 1;
+                            for (auto x : dka.len) nka *= x;
+                            for (auto x : di0a.len) nia *= x;
+                            for (auto x : dkc.len) nkc *= x;
+                            for (auto x : di0c.len) nic *= x;
+                            for (int64_t xka = 0; xka < nka; ++xka) {
+                                split_index(xka, dka.len, ska);
+                                for (int64_t xia = 0; xia < nia; ++xia) {
+                                    split_index(xia, di0a.len, sia);
+                                    int sa = sign_reo;
+                                    int64_t oa = d.off, ra = dr->off;
+                                    if (!addr(t_oa, m_oa, d.lda, ska, sia, sa, oa) ||
+                                        !addr(t_ra, m_ra, dr->lda, ska, sia, sa, ra))
+                                        continue;
+                                    for (int64_t xkc = 0; xkc < nkc; ++xkc) {
+                                        split_index(xkc, dkc.len, skc);
+                                        for (int64_t xic = 0; xic < nic; ++xic) {
+                                            split_index(xic, di0c.len, sic);
+                                            int s2 = sa;
+                                            int64_t oo = oa, rr = ra;
+                                            if (!addr(t_oc, m_oc, d.ldc, sic, skc, s2, oo) ||
+                                                !addr(t_rc, m_rc, dr->ldc, sic, skc, s2, rr))
+                                                continue;
+                                            dst.push_back(rr);
+                                            src.push_back(oo);
+                                            sg.push_back((int8_t)s2);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+            }
+        }
+    }
+    return 0;
+}
+
 // sym_ab pairs of a list (contract/sym_ab_blk.f:214-377)
 static int sym_pairs(gecco_hip_list *L, int64_t off, std::vector<int64_t> &i1,
                      std::vector<int64_t> &i2, std::vector<int8_t> &fs, double relfac) {
@@ -1014,24 +1239,48 @@ int gecco_hip_orbs_flipmap(gecco_hip_orbs *o, int64_t hpvx, int64_t nel, int64_t
     return 0;
 }
 
-int gecco_hip_list_create(gecco_hip_orbs *o, const int64_t *occ_c, const int64_t *occ_a,
-                          int64_t absym, gecco_hip_list **out) {
-    if (!o || !occ_c || !occ_a || !out) return set_error(GECCO_HIP_EINVAL, "list: null argument");
+int gecco_hip_list_create_n(gecco_hip_orbs *o, int64_t njoined, const int64_t *occ, int64_t absym,
+                            gecco_hip_list **out) {
+    if (!o || !occ || !out || njoined < 1 || njoined > 8)
+        return set_error(GECCO_HIP_EINVAL, "list: null argument or bad vertex count");
     auto *L = new gecco_hip_list();
     L->orbs = o;
     L->absym = (int)absym;
-    for (int h = 1; h <= 4; ++h) {
-        L->occ_c[h] = (int)occ_c[h - 1];
-        L->occ_a[h] = (int)occ_a[h - 1];
-        if (occ_c[h - 1] < 0 || occ_a[h - 1] < 0 || occ_c[h - 1] > 8 || occ_a[h - 1] > 8 ||
-            ((h == 3 || h == 4) && (occ_c[h - 1] || occ_a[h - 1]))) {
+    L->njoined = (int)njoined;
+    L->vocc.assign(njoined, std::array<int, 10>{});
+    int nc = 0, na = 0;
+    for (int64_t v = 0; v < njoined; ++v)
+        for (int ca = 0; ca < 2; ++ca)
+            for (int h = 1; h <= 4; ++h) {
+                const int64_t n = occ[v * 8 + ca * 4 + h - 1];
+                if (n < 0 || n > 8 || ((h == 3 || h == 4) && n)) {
+                    delete L;
+                    return set_error(GECCO_HIP_EINVAL, "list: occupations must be H/P, 0..8");
+                }
+                L->vocc[v][ca * 5 + h] = (int)n;
+                (ca ? L->occ_a : L->occ_c)[h] += (int)n;
+                (ca ? na : nc) += (int)n;
+            }
+    for (int h = 1; h <= 4; ++h)
+        if (L->occ_c[h] > 8 || L->occ_a[h] > 8) {
             delete L;
-            return set_error(GECCO_HIP_EINVAL, "list: occupations must be H/P, 0..8");
+            return set_error(GECCO_HIP_EINVAL, "list: more than 8 electrons of one space");
         }
+    if ((nc - na) % 2) {
+        delete L;
+        return set_error(GECCO_HIP_EINVAL, "list: Ms(total) = 0 needs creator and annihilator counts of equal parity");
     }
     build_list(L);
     *out = L;
     return 0;
+}
+
+int gecco_hip_list_create(gecco_hip_orbs *o, const int64_t *occ_c, const int64_t *occ_a,
+                          int64_t absym, gecco_hip_list **out) {
+    if (!o || !occ_c || !occ_a || !out) return set_error(GECCO_HIP_EINVAL, "list: null argument");
+    int64_t occ[8];
+    for (int h = 0; h < 4; ++h) { occ[h] = occ_c[h]; occ[4 + h] = occ_a[h]; }
+    return gecco_hip_list_create_n(o, 1, occ, absym, out);
 }
 
 int gecco_hip_list_destroy(gecco_hip_list *L) {
@@ -1177,6 +1426,8 @@ int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *o
         return set_error(GECCO_HIP_EINVAL, "add_contraction: null argument");
     if (op1->orbs != op2->orbs || op1->orbs != res->orbs)
         return set_error(GECCO_HIP_EINVAL, "add_contraction: lists of different orbital sets");
+    if (op1->njoined > 1 || op2->njoined > 1 || res->njoined > 1)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction: one-vertex lists only");
     int cc[5] = {0}, ca[5] = {0}, e1c[5] = {0}, e1a[5] = {0}, e2c[5] = {0}, e2a[5] = {0};
     for (int h = 1; h <= 4; ++h) {
         cc[h] = (int)cnt_c[h - 1];
@@ -1218,6 +1469,7 @@ int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
                              int64_t *nterms) {
     if (!plan || !op || !trop || !cnt) return set_error(GECCO_HIP_EINVAL, "add_trace: null argument");
     if (op->orbs != trop->orbs) return set_error(GECCO_HIP_EINVAL, "add_trace: lists of different orbital sets");
+    if (op->njoined > 1 || trop->njoined > 1) return set_error(GECCO_HIP_EINVAL, "add_trace: one-vertex lists only");
     int c5[5] = {0};
     for (int h = 1; h <= 4; ++h) {
         c5[h] = (int)cnt[h - 1];
@@ -1230,6 +1482,32 @@ int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
     if (rc) return rc;
     if (nterms) *nterms = (int64_t)dst.size();
     return plan_add_trace_rows(plan, buf_op, off_op, buf_trop, off_trop, fact, std::move(dst),
+                               std::move(src), std::move(sg));
+}
+
+int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
+                               const gecco_hip_list *reo, int64_t from_vtx, int64_t to_vtx,
+                               const int64_t *shift_c, const int64_t *shift_a, double fact,
+                               int64_t sign_reo, int32_t buf_ori, int64_t off_ori, int32_t buf_reo,
+                               int64_t off_reo, int64_t *nterms) {
+    if (!plan || !ori || !reo || !shift_c || !shift_a)
+        return set_error(GECCO_HIP_EINVAL, "add_reorder: null argument");
+    if (ori->orbs != reo->orbs) return set_error(GECCO_HIP_EINVAL, "add_reorder: lists of different orbital sets");
+    if (from_vtx < 0 || from_vtx >= ori->njoined || to_vtx < 0 || to_vtx >= reo->njoined)
+        return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex out of range");
+    if (sign_reo != 1 && sign_reo != -1) return set_error(GECCO_HIP_EINVAL, "add_reorder: sign must be +1 or -1");
+    int kc[5] = {0}, ka[5] = {0};
+    for (int h = 1; h <= 4; ++h) {
+        kc[h] = (int)shift_c[h - 1];
+        ka[h] = (int)shift_a[h - 1];
+        if (kc[h] < 0 || ka[h] < 0) return set_error(GECCO_HIP_EINVAL, "add_reorder: negative shift occupation");
+    }
+    std::vector<int64_t> dst, src;
+    std::vector<int8_t> sg;
+    int rc = reorder_triples(ori, reo, (int)from_vtx, (int)to_vtx, kc, ka, (int)sign_reo, dst, src, sg);
+    if (rc) return rc;
+    if (nterms) *nterms = (int64_t)dst.size();
+    return plan_add_trace_rows(plan, buf_ori, off_ori, buf_reo, off_reo, fact, std::move(dst),
                                std::move(src), std::move(sg));
 }
 

@@ -268,7 +268,8 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_list_destroy', 'gecco_hip_list_length', 'gecco_hip_list_ndist',
                  'gecco_hip_list_dist', 'gecco_hip_plan_add_contraction',
                  'gecco_hip_plan_add_sym_ab', 'gecco_hip_list_spinorbs',
-                 'gecco_hip_list_diagonal', 'gecco_hip_plan_add_trace')
+                 'gecco_hip_list_diagonal', 'gecco_hip_plan_add_trace', 'gecco_hip_list_create_n',
+                 'gecco_hip_plan_add_reorder')
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
                   'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix')
 VP = ctypes.c_void_p
@@ -304,6 +305,11 @@ _lib.gecco_hip_list_diagonal.argtypes = [VP, PD, PD, PD, I64]
 _lib.gecco_hip_plan_add_trace.argtypes = [VP, VP, VP, PI64, ctypes.c_double, ctypes.c_int32, I64,
                                           ctypes.c_int32, I64, PI64]
 _lib.gecco_hip_plan_add_trace.restype = ctypes.c_int
+_lib.gecco_hip_list_create_n.argtypes = [VP, I64, PI64, I64, ctypes.POINTER(VP)]
+_lib.gecco_hip_list_create_n.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_reorder.argtypes = [VP, VP, VP, I64, I64, PI64, PI64, ctypes.c_double, I64,
+                                            ctypes.c_int32, I64, ctypes.c_int32, I64, PI64]
+_lib.gecco_hip_plan_add_reorder.restype = ctypes.c_int
 _lib.gecco_hip_list_diagonal.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_init.argtypes = [VP, ctypes.c_int32, I64, I64, ctypes.c_int32, I64,
                                          ctypes.c_double]
@@ -369,17 +375,30 @@ class Orbs:
 
 
 class MEList:
-    """ME-list layout of a one-vertex operator (operators/set_op_dim2.f)."""
+    """ME-list layout of an operator (operators/set_op_dim2.f).  occ_c/occ_a:
+    dict hpvx->count for one vertex, or lists of such dicts (one per
+    vertex)."""
 
     def __init__(self, orbs, occ_c, occ_a, absym=1):
         self.orbs = orbs
-        self.occ_c = dict(occ_c)
-        self.occ_a = dict(occ_a)
+        if isinstance(occ_c, dict):
+            occ_c, occ_a = [occ_c], [occ_a]
+        self.vocc_c = [dict(x) for x in occ_c]
+        self.vocc_a = [dict(x) for x in occ_a]
+        self.njoined = len(self.vocc_c)
+        self.occ_c, self.occ_a = {}, {}
+        for x in self.vocc_c:
+            for h, n in x.items():
+                self.occ_c[h] = self.occ_c.get(h, 0) + n
+        for x in self.vocc_a:
+            for h, n in x.items():
+                self.occ_a[h] = self.occ_a.get(h, 0) + n
         self.absym = absym
-        c, a = _occ4(occ_c), _occ4(occ_a)
+        occ = np.concatenate([np.concatenate([_occ4(c), _occ4(a)])
+                              for c, a in zip(self.vocc_c, self.vocc_a)]).astype(np.int64)
         self._h = VP()
-        _check(_lib.gecco_hip_list_create(orbs._h, c.ctypes.data_as(PI64), a.ctypes.data_as(PI64),
-                                          absym, ctypes.byref(self._h)))
+        _check(_lib.gecco_hip_list_create_n(orbs._h, self.njoined, occ.ctypes.data_as(PI64), absym,
+                                            ctypes.byref(self._h)))
         self.length = _lib.gecco_hip_list_length(self._h)
 
     def dists(self):
@@ -469,7 +488,22 @@ def _plan_add_trace(self, op, trop, cnt, fact=1.0, buf=(0, 1), off=(0, 0)):
     return n.value
 
 
+def _plan_add_reorder(self, ori, reo, frm, to, shift_c, shift_a, fact=1.0, sign_reo=1,
+                      buf=(0, 1), off=(0, 0)):
+    """Record REO += fact*sign_reo * ORI with the shift string K (shift_c,
+    shift_a) moved from vertex frm of ORI to vertex to of REO (reo_op_wmaps_c).
+    Returns the number of element terms."""
+    kc, ka = _occ4(shift_c), _occ4(shift_a)
+    n = I64()
+    _check(_lib.gecco_hip_plan_add_reorder(self._h, ori._h, reo._h, int(frm), int(to),
+                                           kc.ctypes.data_as(PI64), ka.ctypes.data_as(PI64),
+                                           float(fact), int(sign_reo), buf[0], off[0], buf[1],
+                                           off[1], ctypes.byref(n)))
+    return n.value
+
+
 Plan.add_contraction = _plan_add_contraction
+Plan.add_reorder = _plan_add_reorder
 Plan.add_trace = _plan_add_trace
 Plan.add_sym_ab = _plan_add_sym_ab
 Plan.add_init = _plan_add_init

@@ -49,6 +49,14 @@ int gecco_hip_orbs_flipmap(gecco_hip_orbs *orbs, int64_t hpvx, int64_t nel, int6
  * Layout: operators/set_op_dim2.f:146-459. */
 int gecco_hip_list_create(gecco_hip_orbs *orbs, const int64_t *occ_c, const int64_t *occ_a,
                           int64_t absym, gecco_hip_list **out);
+/* ME-list of an operator with njoined vertices (intermediates with several
+ * vertices): occ[v*8 + 0..3] = C occupation of vertex v (H, P, V, X),
+ * occ[v*8 + 4..7] = its A occupation.  Components are condensed as
+ * operators/condense_occ.f does (X,P,V,H outer, vertex inner); layout as
+ * gecco_hip_list_create.  Contractions and traces take one-vertex lists;
+ * reorders (gecco_hip_plan_add_reorder) take any. */
+int gecco_hip_list_create_n(gecco_hip_orbs *orbs, int64_t njoined, const int64_t *occ,
+                            int64_t absym, gecco_hip_list **out);
 int gecco_hip_list_destroy(gecco_hip_list *list);
 int64_t gecco_hip_list_length(const gecco_hip_list *list);
 int64_t gecco_hip_list_ndist(const gecco_hip_list *list);
@@ -94,6 +102,23 @@ int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
                              const gecco_hip_list *trop, const int64_t *cnt, double fact,
                              int32_t buf_op, int64_t off_op, int32_t buf_trop, int64_t off_trop,
                              int64_t *nterms);
+
+/* Reorder: the shift string K (occupations shift_c[4], shift_a[4] per space
+ * H, P, V, X) moves from vertex from_vtx of ori to vertex to_vtx of reo,
+ *   REO(I') += fact * sign_reo * s_ori(I0,K) * s_reo(I0,K) * ORI(I),
+ * I0 = ORI without K; C strings merge (I0, K), A strings (K, I0).  Replaces
+ * contract/reo_op_wmaps_c.f:2-575 with its block kernel
+ * contract/reo_blk_wmaps_c.f:2-854 (called from contract/fs_reo_drv.f:166
+ * and for reordered contraction results, contract/contr_op1op2_wmaps_c.f:1987).
+ * sign_reo (+1/-1) is the sign the formula layer assigns to moving K past
+ * the operators of intervening vertices (reorder_info%sign_reo).  Runs after
+ * the plan's contractions, with the partial traces.  Returns the number of
+ * element terms. */
+int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
+                               const gecco_hip_list *reo, int64_t from_vtx, int64_t to_vtx,
+                               const int64_t *shift_c, const int64_t *shift_a, double fact,
+                               int64_t sign_reo, int32_t buf_ori, int64_t off_ori, int32_t buf_reo,
+                               int64_t off_reo, int64_t *nterms);
 
 /* Spin-flip symmetrisation of a result list after the plan's contractions
  * (contract/sym_ab_blk.f, called with prefac 1/2, relfac absym at

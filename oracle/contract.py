@@ -93,6 +93,8 @@ class Contraction:
     """
 
     def __init__(self, op1, op2, res, cnt_c, cnt_a, fact=1.0):
+        if max(op1.njoined, op2.njoined, res.njoined) > 1:
+            raise ValueError('Contraction: one-vertex operators only')
         self.op1, self.op2, self.res = op1, op2, res
         self.orbs = op1.orbs
         self.fact = fact

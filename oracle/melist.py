@@ -12,9 +12,10 @@ Restated routines (reference file:line):
   * in-block leading dims      contract/set_op_ldim_c.f:237-308
   * distribution id            operators/msgmdid2.f:2-34
 
-Scope: operators with one vertex (njoined=1), spaces H and P with one orbital
-subspace each, mst=0, gamt=1 (closed-shell, totally symmetric operators),
-no diagonal restriction.  Those are the operators of closed-shell CCSD.
+Scope: operators with one or more vertices (njoined >= 1; the condensed
+components of all vertices, X,P,V,H outer and vertex inner), spaces H and P
+with one orbital subspace each, mst=0, gamt=1 (closed-shell, totally
+symmetric operators), no diagonal restriction.
 
 Parity status: pinned by closed-form block lengths (e.g. T2 at nocc=8,
 nvir=32 has 93 312 elements) and by the dense-tensor round trip in
@@ -38,6 +39,20 @@ def condense(occ_c, occ_a):
             oa.append(occ_a[h])
             ha.append(h)
     return oc, hc, oa, ha
+
+
+def condense_n(vocc_c, vocc_a):
+    """Several vertices (lists of dicts hpvx->count): operators/condense_occ.f
+    with the vertex loop inside the hpvxblkseq loop.  Returns the condensed
+    lists plus the vertex (0-based) of every C and A component."""
+    oc, hc, vc, oa, ha, va = [], [], [], [], [], []
+    for h in HPVXSEQ:
+        for v in range(len(vocc_c)):
+            if vocc_c[v].get(h, 0) > 0:
+                oc.append(vocc_c[v][h]); hc.append(h); vc.append(v)
+            if vocc_a[v].get(h, 0) > 0:
+                oa.append(vocc_a[v][h]); ha.append(h); va.append(v)
+    return oc, hc, vc, oa, ha, va
 
 
 def first_msdistn(occ, mst):
@@ -187,11 +202,25 @@ class MEList:
     """ME-list layout of a one-vertex operator (operators/set_op_dim2.f)."""
 
     def __init__(self, orbs, occ_c, occ_a, absym=1):
+        """occ_c/occ_a: dict hpvx->count (one vertex) or a list of such
+        dicts, one per vertex."""
         self.orbs = orbs
-        self.occ_c = dict(occ_c)
-        self.occ_a = dict(occ_a)
+        if isinstance(occ_c, dict):
+            occ_c, occ_a = [occ_c], [occ_a]
+        self.vocc_c = [dict(x) for x in occ_c]
+        self.vocc_a = [dict(x) for x in occ_a]
+        self.njoined = len(self.vocc_c)
+        self.occ_c, self.occ_a = {}, {}
+        for x in self.vocc_c:
+            for h, n in x.items():
+                self.occ_c[h] = self.occ_c.get(h, 0) + n
+        for x in self.vocc_a:
+            for h, n in x.items():
+                self.occ_a[h] = self.occ_a.get(h, 0) + n
         self.absym = absym
-        self.oc, self.hc, self.oa, self.ha = condense(self.occ_c, self.occ_a)
+        self.oc, self.hc, self.vc, self.oa, self.ha, self.va = condense_n(self.vocc_c, self.vocc_a)
+        if (sum(self.oc) - sum(self.oa)) % 2:
+            raise ValueError('MEList: Ms(total) = 0 needs creator and annihilator counts of equal parity')
         self.nc = sum(self.oc)
         self.na = sum(self.oa)
         nsym = orbs.nsym

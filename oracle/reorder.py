@@ -1,0 +1,301 @@
+"""Oracle (TEST INFRASTRUCTURE ONLY): CPU restatement of GeCCo's reorder of
+a multi-vertex operator, a shift string K moving from vertex `frm` of ORI to
+vertex `to` of REO:
+
+    REO(I') += fact * sign_reo * s_ori(I0, K) * s_reo(I0, K) * ORI(I),
+
+I = the ORI string built from I0 and K, I' = the REO string built from the
+same I0 and K.  Only tests may import it; the product path is
+gecco_hip.Plan.add_reorder (C++ in gecco-public_amd/csrc/enumerate.cpp, HIP
+kernel trace_rows_f64, the signed gather-reduce shared with partial traces).
+
+Follows
+  contract/reo_op_wmaps_c.f:380-560   loops over Ms(A), IRREP(A) and the
+                                      distributions of ORI (next_msgamdist2)
+  contract/reo_blk_wmaps_c.f:210-560  loops over Ms and IRREP of K (A, then C)
+                                      and K's distributions; I0 from
+                                      split_msgmdis, REO from merge_msgmdis
+  contract/reo_blk_wmaps_c.f:560-840  string loops K(A), I0(A), K(C), I0(C);
+                                      C maps (I0 fastest, K), A maps (K
+                                      fastest, I0); the accumulate at :835
+sign_reo is the global sign the formula layer derives for moving K past the
+operators of intervening vertices (reorder_info%sign_reo); it is an input.
+
+Parity status: pinned -- tests/test_oracle_reorder.py checks the dense form:
+the reorder is an axis permutation of the antisymmetric tensors, REO[i0',k]
+= fact*sign_reo*ORI[i0,k] with K's axes moved from the `frm` vertex's
+components to the `to` vertex's, and a move there and back is the identity.
+"""
+import numpy as np
+
+from .melist import condense, condense_n, msgamdists, idxms_of, HPVXSEQ
+from .strings import strmap, multd2h
+
+
+def _sources(comp_h, comp_v, i0_h, i0_v, k_h, kvertex, k_first):
+    """map_info of one side: per target component (vertex v, space h) the
+    I0 component with the same (v, h) and, on vertex kvertex, the K
+    component of space h; k_first puts K as source 1 (A side)."""
+    out = []
+    for h, v in zip(comp_h, comp_v):
+        i0 = [j for j, (hh, vv) in enumerate(zip(i0_h, i0_v)) if hh == h and vv == v]
+        kk = [j for j, hh in enumerate(k_h) if hh == h] if v == kvertex else []
+        out.append((kk, i0) if k_first else (i0, kk))
+    return out
+
+
+def _merge(srcs, d1, d2):
+    """contract/merge_msgmdis.f: Ms adds, IRREP multiplies."""
+    ms, gm = [], []
+    for s1, s2 in srcs:
+        m, g = 0, 1
+        for j in s1:
+            m += d1[0][j]; g = multd2h(g, d1[1][j])
+        for j in s2:
+            m += d2[0][j]; g = multd2h(g, d2[1][j])
+        ms.append(m); gm.append(g)
+    return ms, gm
+
+
+def _split(srcs, ms_t, gm_t, known, n_unknown, known_is_first):
+    """contract/split_msgmdis.f for one side: the unknown (I0) parts from
+    the target distribution and the known (K) parts; None if impossible."""
+    ms = [None] * n_unknown
+    gm = [None] * n_unknown
+    for t, (s1, s2) in enumerate(srcs):
+        kn, un = (s1, s2) if known_is_first else (s2, s1)
+        m, g = ms_t[t], gm_t[t]
+        for j in kn:
+            m -= known[0][j]
+            g = multd2h(g, known[1][j])
+        if not un:
+            if m != 0 or g != 1:
+                return None
+            continue
+        j = un[0]
+        if ms[j] is not None and (ms[j] != m or gm[j] != g):
+            return None
+        ms[j], gm[j] = m, g
+    if any(x is None for x in ms):
+        return None
+    return ms, gm
+
+
+class Reorder:
+    def __init__(self, ori, reo, frm, to, shift_c, shift_a, fact=1.0, sign_reo=1):
+        self.ori, self.reo = ori, reo
+        self.orbs = ori.orbs
+        self.frm, self.to = frm, to
+        self.fact, self.sign = fact, sign_reo
+        i0c = [dict(x) for x in ori.vocc_c]
+        i0a = [dict(x) for x in ori.vocc_a]
+        for h, n in shift_c.items():
+            i0c[frm][h] = i0c[frm].get(h, 0) - n
+        for h, n in shift_a.items():
+            i0a[frm][h] = i0a[frm].get(h, 0) - n
+        if any(v < 0 for x in i0c + i0a for v in x.values()):
+            raise ValueError('reorder: K is not part of the from-vertex')
+        chk_c = [dict(x) for x in i0c]
+        chk_a = [dict(x) for x in i0a]
+        for h, n in shift_c.items():
+            chk_c[to][h] = chk_c[to].get(h, 0) + n
+        for h, n in shift_a.items():
+            chk_a[to][h] = chk_a[to].get(h, 0) + n
+        norm = lambda L: [{h: n for h, n in x.items() if n} for x in L]
+        if norm(chk_c) != norm(reo.vocc_c) or norm(chk_a) != norm(reo.vocc_a):
+            raise ValueError('reorder: REO is not I0 with K on the to-vertex')
+        (self.i0c_o, self.i0c_h, self.i0c_v,
+         self.i0a_o, self.i0a_h, self.i0a_v) = condense_n(i0c, i0a)
+        self.kc_o, self.kc_h, self.ka_o, self.ka_h = condense(shift_c, shift_a)
+        self.ori_c = _sources(ori.hc, ori.vc, self.i0c_h, self.i0c_v, self.kc_h, frm, False)
+        self.ori_a = _sources(ori.ha, ori.va, self.i0a_h, self.i0a_v, self.ka_h, frm, True)
+        self.reo_c = _sources(reo.hc, reo.vc, self.i0c_h, self.i0c_v, self.kc_h, to, False)
+        self.reo_a = _sources(reo.ha, reo.va, self.i0a_h, self.i0a_v, self.ka_h, to, True)
+
+    def _lens(self, occs, hs, ms, gm):
+        return [self.orbs.lenstr(h, o, idxms_of(o, m), g) for o, h, m, g in zip(occs, hs, ms, gm)]
+
+    def _maps(self, tgt_h, srcs, p1, p2):
+        """strings/get_strmap_blk_c.f: per target component (src1 comp,
+        src2 comp, n1, map); p = (occs, hpvxs, ms, gams, lens) of a source."""
+        out = []
+        for h, (s1, s2) in zip(tgt_h, srcs):
+            if s1 and s2:
+                a, b = s1[0], s2[0]
+                m = strmap(self.orbs.spaces[h], p1[0][a], idxms_of(p1[0][a], p1[2][a]), p1[3][a],
+                           p2[0][b], idxms_of(p2[0][b], p2[2][b]), p2[3][b])
+                out.append((a, b, p1[4][a], m))
+            elif s1:
+                a = s1[0]
+                out.append((a, None, p1[4][a], list(range(1, p1[4][a] + 1))))
+            else:
+                b = s2[0]
+                out.append((None, b, 1, list(range(1, p2[4][b] + 1))))
+        return out
+
+    @staticmethod
+    def _split_idx(idx, lens):
+        out = []
+        for n in lens:
+            out.append(idx % n)
+            idx //= n
+        return out
+
+    @staticmethod
+    def _addr(maps, lds, i1s, i2s):
+        sgn, off = 1, 0
+        for (a, b, n1, m), ld in zip(maps, lds):
+            e = m[(i1s[a] if a is not None else 0) + n1 * (i2s[b] if b is not None else 0)]
+            if e == 0:
+                return None
+            if e < 0:
+                sgn = -sgn
+            off += (abs(e) - 1) * ld
+        return sgn, off
+
+    def triples(self):
+        """(REO element, ORI element, sign) of every element move."""
+        ori, reo, orbs = self.ori, self.reo, self.orbs
+        nsym = orbs.nsym
+        out = []
+        kc_max, ka_max = sum(self.kc_o), sum(self.ka_o)
+        i0c_max, i0a_max = sum(self.i0c_o), sum(self.i0a_o)
+        for blk in ori.blocks.values():
+            for d in blk:
+                ms_op_c, ms_op_a = sum(d['msc']), sum(d['msa'])
+                gm_op_c = gm_op_a = d['igama']
+                for ms_k_a in range(ka_max, -ka_max - 1, -2):
+                    if abs(ms_op_a - ms_k_a) > i0a_max:
+                        continue
+                    for ms_k_c in range(kc_max, -kc_max - 1, -2):
+                        if abs(ms_op_c - ms_k_c) > i0c_max:
+                            continue
+                        for gm_k_a in range(1, nsym + 1):
+                            for gm_k_c in range(1, nsym + 1):
+                                for kmc, kma, kgc, kga in msgamdists(self.kc_o, self.ka_o, ms_k_c, ms_k_a,
+                                                                     gm_k_c, gm_k_a, nsym):
+                                    out += self._dist(d, kmc, kma, kgc, kga)
+        return out
+
+    def _dist(self, d, kmc, kma, kgc, kga):
+        ori, reo = self.ori, self.reo
+        sc = _split(self.ori_c, d['msc'], d['gamc'], (kmc, kgc), len(self.i0c_o), False)
+        sa = _split(self.ori_a, d['msa'], d['gama'], (kma, kga), len(self.i0a_o), True)
+        if sc is None or sa is None:
+            return []
+        (i0mc, i0gc), (i0ma, i0ga) = sc, sa
+        for o, m in zip(self.i0c_o + self.i0a_o, i0mc + i0ma):
+            if abs(m) > o or (o - m) % 2:
+                return []
+        rmc, rgc = _merge(self.reo_c, (i0mc, i0gc), (kmc, kgc))
+        rma, rga = _merge(self.reo_a, (kma, kga), (i0ma, i0ga))
+        lk_c = self._lens(self.kc_o, self.kc_h, kmc, kgc)
+        lk_a = self._lens(self.ka_o, self.ka_h, kma, kga)
+        li_c = self._lens(self.i0c_o, self.i0c_h, i0mc, i0gc)
+        li_a = self._lens(self.i0a_o, self.i0a_h, i0ma, i0ga)
+        if any(x == 0 for x in lk_c + lk_a + li_c + li_a):
+            return []
+        imc = [idxms_of(o, m) for o, m in zip(reo.oc, rmc)]
+        ima = [idxms_of(o, m) for o, m in zip(reo.oa, rma)]
+        dr = reo.find(sum(rma), d['igama'], imc, rgc, ima, rga)
+        if dr is None:
+            return []
+        pkc = (self.kc_o, self.kc_h, kmc, kgc, lk_c)
+        pka = (self.ka_o, self.ka_h, kma, kga, lk_a)
+        pic = (self.i0c_o, self.i0c_h, i0mc, i0gc, li_c)
+        pia = (self.i0a_o, self.i0a_h, i0ma, i0ga, li_a)
+        m_oc = self._maps(ori.hc, self.ori_c, pic, pkc)
+        m_oa = self._maps(ori.ha, self.ori_a, pka, pia)
+        m_rc = self._maps(reo.hc, self.reo_c, pic, pkc)
+        m_ra = self._maps(reo.ha, self.reo_a, pka, pia)
+        out = []
+        nka, nia = int(np.prod(lk_a)), int(np.prod(li_a))
+        nkc, nic = int(np.prod(lk_c)), int(np.prod(li_c))
+        for ka in range(nka):
+            ska = self._split_idx(ka, lk_a)
+            for ia in range(nia):
+                sia = self._split_idx(ia, li_a)
+                ao = self._addr(m_oa, d['lda'], ska, sia)
+                ar = self._addr(m_ra, dr['lda'], ska, sia)
+                if ao is None or ar is None:
+                    continue
+                for kc in range(nkc):
+                    skc = self._split_idx(kc, lk_c)
+                    for ic in range(nic):
+                        sic = self._split_idx(ic, li_c)
+                        co = self._addr(m_oc, d['ldc'], sic, skc)
+                        cr = self._addr(m_rc, dr['ldc'], sic, skc)
+                        if co is None or cr is None:
+                            continue
+                        s = self.sign * ao[0] * ar[0] * co[0] * cr[0]
+                        out.append((dr['off'] + ar[1] + cr[1], d['off'] + ao[1] + co[1], s))
+        return out
+
+    def run(self, x_ori, x_reo):
+        for ir, io, s in self.triples():
+            x_reo[ir] += self.fact * s * x_ori[io]
+        return x_reo
+
+
+def dense_reorder(D_ori, ori, reo, ro):
+    """Dense form: the axes of K move from the frm-vertex components of ORI
+    to the to-vertex components of REO; C components hold (I0, K) axes, A
+    components (K, I0)."""
+    letters = iter('abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ')
+    lab_i0c = {j: [next(letters) for _ in range(o)] for j, o in enumerate(ro.i0c_o)}
+    lab_i0a = {j: [next(letters) for _ in range(o)] for j, o in enumerate(ro.i0a_o)}
+    lab_kc = {j: [next(letters) for _ in range(o)] for j, o in enumerate(ro.kc_o)}
+    lab_ka = {j: [next(letters) for _ in range(o)] for j, o in enumerate(ro.ka_o)}
+
+    def axes(srcs_c, srcs_a):
+        s = ''
+        for s1, s2 in srcs_c:            # (I0, K)
+            for j in s1:
+                s += ''.join(lab_i0c[j])
+            for j in s2:
+                s += ''.join(lab_kc[j])
+        for s1, s2 in srcs_a:            # (K, I0)
+            for j in s1:
+                s += ''.join(lab_ka[j])
+            for j in s2:
+                s += ''.join(lab_i0a[j])
+        return s
+    T = np.einsum(axes(ro.ori_c, ro.ori_a) + '->' + axes(ro.reo_c, ro.reo_a), D_ori)
+    # a REO component built from an I0 part (p axes) and a K part (q axes)
+    # receives every split of its string: antisymmetrise over its axes with
+    # weight 1/(p! q!)
+    pos = 0
+    comps = [(o, s1, s2, True) for o, (s1, s2) in zip(reo.oc, ro.reo_c)] + \
+            [(o, s1, s2, False) for o, (s1, s2) in zip(reo.oa, ro.reo_a)]
+    for o, s1, s2, is_c in comps:
+        if s1 and s2:
+            T = _antisym(T, list(range(pos, pos + o)))
+            p = (ro.i0c_o if is_c else ro.ka_o)[s1[0]]
+            q = o - p
+            T = T / (_fact(p) * _fact(q))
+        pos += o
+    return ro.fact * ro.sign * T
+
+
+def _fact(k):
+    r = 1
+    for i in range(2, k + 1):
+        r *= i
+    return r
+
+
+def _antisym(T, axes):
+    import itertools
+    out = np.zeros_like(T)
+    for perm in itertools.permutations(range(len(axes))):
+        sgn = 1
+        p = list(perm)
+        for i in range(len(p)):
+            for j in range(i + 1, len(p)):
+                if p[i] > p[j]:
+                    sgn = -sgn
+        order = list(range(T.ndim))
+        for i, a in enumerate(axes):
+            order[a] = axes[p[i]]
+        out += sgn * np.transpose(T, order)
+    return out

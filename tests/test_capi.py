@@ -79,7 +79,11 @@ def test_model_list_layout_matches_oracle():
     o = gecco_hip.Orbs(irr_h, irr_p, 2)
     oo = Orbitals(irr_h, irr_p, 2)
     for occ_c, occ_a in [({2: 2}, {1: 2}), ({1: 1, 2: 1}, {1: 1, 2: 1}), ({2: 1}, {1: 1}),
-                         ({}, {}), ({2: 2}, {2: 2}), ({1: 2}, {2: 2}), ({2: 3}, {1: 3})]:
+                         ({}, {}), ({2: 2}, {2: 2}), ({1: 2}, {2: 2}), ({2: 3}, {1: 3}),
+                         # several vertices (intermediates): condense_occ order
+                         ([{2: 1}, {2: 1}], [{1: 1}, {1: 1}]),
+                         ([{1: 1, 2: 1}, {2: 1}], [{2: 1, 1: 1}, {1: 1}]),
+                         ([{2: 2}, {}, {1: 1}], [{1: 1}, {1: 1}, {2: 1}])]:
         L = gecco_hip.MEList(o, occ_c, occ_a)
         R = OMEList(oo, occ_c, occ_a)
         assert L.length == R.length

@@ -293,3 +293,36 @@ def test_trace_matches_oracle(name, nsym):
     p.execute(bufs)
     torch.cuda.synchronize()
     assert _relerr(bufs[1].cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize('nsym', [1, 2, 4])
+@pytest.mark.parametrize('name', ['p_to_v1', 'h_to_v0', 'ph_merge', 'mixed', 'split', 'p_to_empty'])
+def test_reorder_matches_oracle(name, nsym):
+    """Reorder of a two-vertex list (reo_op_wmaps_c) on the GPU against the
+    oracle restatement, accumulating onto existing data."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    from oracle.melist import MEList as OMEList
+    from oracle.reorder import Reorder
+    from test_oracle_reorder import CASES as RCASES, _reo_occ
+    print(f'reorder {name} nsym={nsym}', flush=True)
+    oc, oa, frm, to, kc, ka = RCASES[name]
+    orbs = orbitals(3, 6, nsym)
+    ori = OMEList(orbs, oc, oa)
+    rc, ra = _reo_occ(oc, oa, frm, to, kc, ka)
+    reo = OMEList(orbs, rc, ra)
+    x = np.random.default_rng(13).standard_normal(ori.length)
+    y0 = np.random.default_rng(14).standard_normal(reo.length)
+    ref = Reorder(ori, reo, frm, to, kc, ka, fact=0.25, sign_reo=-1).run(x, y0.copy())
+    o = gecco_hip.Orbs(round_robin(3, nsym), round_robin(6, nsym, 1), nsym)
+    A = gecco_hip.MEList(o, oc, oa)
+    B = gecco_hip.MEList(o, rc, ra)
+    p = gecco_hip.Plan()
+    assert p.add_reorder(A, B, frm, to, kc, ka, 0.25, -1, buf=(0, 1)) > 0
+    p.finalize()
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x).to(dev), torch.from_numpy(y0).to(dev)]
+    p.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[1].cpu().numpy(), ref) <= TOL
