@@ -242,7 +242,7 @@ def run_ccsd(args):
         'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
                                 'grouped_contract_f64': round(ms[1] / args.steps, 4),
                                 'grouped_contract_direct_f64': round(ms[4] / args.steps, 4),
-                                'sym_ab_pairs_f64': round(ms[2] / args.steps, 4),
+                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
                                 'gather_operands_f64': round(ms[3] / args.steps, 4)},
         'gather_operands_gbps': round(gbytes / (ms[3] / args.steps * 1e-3) / 1e9, 1) if ms[3] > 0 else None,
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,

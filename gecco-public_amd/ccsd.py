@@ -132,6 +132,8 @@ class CCSD:
         self.nblocks = 0
         for li, level in enumerate(LEVELS):
             p = gecco_hip.Plan()
+            for n in INTEGRALS:           # fixed during the solve
+                p.set_constant(self.slot[n])
             if li == 0:
                 # start of an iteration: intermediates from their sources or zero
                 for n in LISTS:
@@ -162,9 +164,12 @@ class CCSD:
         torch = self.torch
         ham = self.ham
         n = ham.nocc + ham.nvir
-        B = torch.from_numpy(ham.B.reshape(ham.B.shape[0], n * n)).to(self.dev)
-        eri = (B.T @ B).reshape(n, n, n, n)      # (pq|rs)
-        del B
+        if getattr(ham, 'eri_full', None) is not None:   # dense (pq|rs) given
+            eri = torch.from_numpy(ham.eri_full).to(self.dev)
+        else:                                             # density fitting
+            B = torch.from_numpy(ham.B.reshape(ham.B.shape[0], n * n)).to(self.dev)
+            eri = (B.T @ B).reshape(n, n, n, n)      # (pq|rs)
+            del B
         for name in INTEGRALS:
             L = self.L[name]
             so = torch.from_numpy(L.spinorbs().astype(np.int64)).to(self.dev)

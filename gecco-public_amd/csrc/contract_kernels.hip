@@ -692,15 +692,20 @@ __global__ __launch_bounds__(256) void trace_rows_f64(const double *__restrict__
     }
 }
 
-// Spin-flip symmetrisation of a result list (contract/sym_ab_blk.f:354-377):
-// for each precomputed pair (i1 <= i2, factor f = +/-relfac2):
-//   v = prefac*(x[i1] + f*x[i2]);  x[i1] = v;  x[i2] = f*v
-__global__ void sym_ab_pairs_f64(double *__restrict__ x, const int64_t *__restrict__ i1,
-                                 const int64_t *__restrict__ i2,
-                                 const int8_t *__restrict__ fsign, int64_t npair,
-                                 double prefac) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npair;
-         p += (int64_t)gridDim.x * blockDim.x) {
+// All spin-flip symmetrisation steps of a plan in one launch (lists of one
+// plan are distinct buffers or disjoint ranges; pairs of one list are
+// disjoint because the flip is an involution).
+__global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__restrict__ steps,
+                                                          const DevSymTask *__restrict__ tasks,
+                                                          double *const *__restrict__ bufs,
+                                                          const int32_t *__restrict__ i1,
+                                                          const int32_t *__restrict__ i2,
+                                                          const int8_t *__restrict__ fsign) {
+    const DevSymTask t = tasks[blockIdx.x];
+    const DevSymStep &st = steps[t.step];
+    double *__restrict__ x = bufs[st.buf] + st.base;
+    const double prefac = st.prefac;
+    for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += blockDim.x) {
         const int64_t a = i1[p], c = i2[p];
         const double f = (double)fsign[p];
         const double v = prefac * (x[a] + f * x[c]);
@@ -792,13 +797,13 @@ extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t 
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
-extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
-                                   const int8_t *fsign, int64_t npair, double prefac,
-                                   hipStream_t stream) {
-    if (npair <= 0) return 0;
-    int64_t g = (npair + 255) / 256;
-    if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(gecco::sym_ab_pairs_f64, dim3((unsigned)g), dim3(256), 0, stream, x, i1,
-                       i2, fsign, npair, prefac);
+extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
+                                        const gecco::DevSymTask *tasks, int ntasks,
+                                        double *const *bufs, const int32_t *i1, const int32_t *i2,
+                                        const int8_t *fsign, hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    hipLaunchKernelGGL(gecco::sym_ab_batched_f64, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
+                       bufs, i1, i2, fsign);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
+

@@ -74,6 +74,19 @@ struct DevInitTask {
     int64_t e0, e1;
 };
 
+// Spin-flip symmetrisation steps of a plan, run as one launch: pairs
+// (base + i1[k], base + i2[k], f[k]) of step s occupy [pair0, pair0 + n).
+struct DevSymStep {
+    int64_t base, pair0;
+    double prefac;
+    int32_t buf, pad;
+};
+
+struct DevSymTask {
+    int32_t step, pad;
+    int64_t p0, p1;        // pair range (global pair indices)
+};
+
 struct DevGatherTask {
     int32_t job;
     int32_t e0, e1;        // element range [e0, e1) of the job's dense array

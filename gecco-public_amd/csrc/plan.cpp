@@ -38,13 +38,14 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            double *const *bufs, int grid, int variant,
                                            hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
-extern "C" int gecco_launch_sym_ab(double *x, const int64_t *i1, const int64_t *i2,
-                                   const int8_t *fsign, int64_t npair, double prefac,
-                                   hipStream_t stream);
 extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            double *const *bufs, int kmax, hipStream_t stream);
 extern "C" int gecco_direct_stage_cap(void);
+extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
+                                        const gecco::DevSymTask *tasks, int ntasks,
+                                        double *const *bufs, const int32_t *i1, const int32_t *i2,
+                                        const int8_t *fsign, hipStream_t stream);
 extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
                                   const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
                                   int64_t nrows, double fact, hipStream_t stream);
@@ -425,8 +426,6 @@ struct SymStep {
     std::vector<int64_t> i1, i2;
     std::vector<int8_t> f;
     int64_t npair = 0;
-    int64_t *d_i1 = nullptr, *d_i2 = nullptr;
-    int8_t *d_f = nullptr;
 };
 
 // Partial trace of a list into another (CSR rows of signed sources).
@@ -449,6 +448,18 @@ struct gecco_hip_plan {
     gecco::DevInitTask *d_itasks = nullptr;
     int64_t nitasks = 0;
     std::vector<SymStep> syms;
+    // all symmetrisation pairs of the plan, one launch (int32 offsets
+    // relative to each step's base element)
+    gecco::DevSymStep *d_symsteps = nullptr;
+    gecco::DevSymTask *d_symtasks = nullptr;
+    int32_t *d_sym_i1 = nullptr, *d_sym_i2 = nullptr;
+    int8_t *d_sym_f = nullptr;
+    int64_t nsymtasks = 0;
+    // buffer slots whose contents stay fixed between executes (integrals):
+    // their gathers run on the first execute and after a slot changes
+    std::vector<uint8_t> constant_slot;
+    int64_t ngtasks_static = 0;
+    bool static_done = false;
     bool finalized = false;
     gecco::DevBlock *d_blocks = nullptr;
     gecco::DevTask *d_tasks = nullptr;
@@ -604,13 +615,15 @@ static void plan_free_device(gecco_hip_plan *p) {
     p->d_gjobs = nullptr;
     p->d_gtasks = nullptr;
     p->d_scratch = nullptr;
-    for (auto &y : p->syms) {
-        if (y.d_i1) (void)hipFree(y.d_i1);
-        if (y.d_i2) (void)hipFree(y.d_i2);
-        if (y.d_f) (void)hipFree(y.d_f);
-        y.d_i1 = y.d_i2 = nullptr;
-        y.d_f = nullptr;
-    }
+    if (p->d_symsteps) (void)hipFree(p->d_symsteps);
+    if (p->d_symtasks) (void)hipFree(p->d_symtasks);
+    if (p->d_sym_i1) (void)hipFree(p->d_sym_i1);
+    if (p->d_sym_i2) (void)hipFree(p->d_sym_i2);
+    if (p->d_sym_f) (void)hipFree(p->d_sym_f);
+    p->d_symsteps = nullptr;
+    p->d_symtasks = nullptr;
+    p->d_sym_i1 = p->d_sym_i2 = nullptr;
+    p->d_sym_f = nullptr;
     p->d_blocks = nullptr;
     p->d_tasks = nullptr;
     p->d_index = nullptr;
@@ -913,12 +926,19 @@ static int plan_finalize(gecco_hip_plan *p) {
         std::vector<DevGatherTask> gt;
         const int64_t chunk = 8192;
         double gbytes = 0;
-        for (size_t j = 0; j < jobs.size(); ++j) {
-            const int64_t n = (int64_t)jobs[j].K * jobs[j].X;
-            for (int64_t e0 = 0; e0 < n; e0 += chunk)
-                gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk), 0});
-            gbytes += 16.0 * (double)n;
-        }
+        auto is_const = [&](int32_t slot) {
+            return slot >= 0 && slot < (int32_t)p->constant_slot.size() && p->constant_slot[slot];
+        };
+        // gathers of constant slots first: they run only when needed
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t j = 0; j < jobs.size(); ++j) {
+                if (is_const(jobs[j].src_buf) != (pass == 0)) continue;
+                const int64_t n = (int64_t)jobs[j].K * jobs[j].X;
+                for (int64_t e0 = 0; e0 < n; e0 += chunk)
+                    gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk), 0});
+                if (pass == 1) gbytes += 16.0 * (double)n;
+                if (pass == 0) p->ngtasks_static = (int64_t)gt.size();
+            }
         p->ngjobs = (int64_t)jobs.size();
         p->ngtasks = (int64_t)gt.size();
         p->scratch_elems = next;
@@ -972,21 +992,55 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->ntasks_affine = ncls[0];
     p->ntasks_direct8 = ncls[1];
     p->ntasks_direct32 = ncls[2];
-    for (auto &y : p->syms) {
-        y.npair = (int64_t)y.i1.size();
-        if (!y.npair) continue;
-        if (check(hipMalloc(&y.d_i1, y.npair * 8), "hipMalloc(sym)") ||
-            check(hipMalloc(&y.d_i2, y.npair * 8), "hipMalloc(sym)") ||
-            check(hipMalloc(&y.d_f, y.npair), "hipMalloc(sym)") ||
-            check(hipMemcpy(y.d_i1, y.i1.data(), y.npair * 8, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
-            check(hipMemcpy(y.d_i2, y.i2.data(), y.npair * 8, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
-            check(hipMemcpy(y.d_f, y.f.data(), y.npair, hipMemcpyHostToDevice), "hipMemcpy(sym)"))
+    {
+        std::vector<DevSymStep> steps;
+        std::vector<DevSymTask> stasks;
+        std::vector<int32_t> a1, a2;
+        std::vector<int8_t> ff;
+        const int64_t chunk = 8192;
+        for (auto &y : p->syms) {
+            y.npair = (int64_t)y.i1.size();
+            if (!y.npair) continue;
+            int64_t base = INT64_MAX, top = 0;
+            for (int64_t k = 0; k < y.npair; ++k) {
+                base = std::min(base, std::min(y.i1[k], y.i2[k]));
+                top = std::max(top, std::max(y.i1[k], y.i2[k]));
+            }
+            if (top - base >= ((int64_t)1 << 31))
+                return fail(GECCO_HIP_EINVAL, "symmetrised list spans more than 2^31 elements");
+            DevSymStep st;
+            std::memset(&st, 0, sizeof st);
+            st.base = base;
+            st.pair0 = (int64_t)a1.size();
+            st.prefac = y.prefac;
+            st.buf = y.buf;
+            for (int64_t k = 0; k < y.npair; ++k) {
+                a1.push_back((int32_t)(y.i1[k] - base));
+                a2.push_back((int32_t)(y.i2[k] - base));
+                ff.push_back(y.f[k]);
+            }
+            for (int64_t p0 = 0; p0 < y.npair; p0 += chunk)
+                stasks.push_back({(int32_t)steps.size(), 0, st.pair0 + p0, st.pair0 + std::min(y.npair, p0 + chunk)});
+            steps.push_back(st);
+            std::vector<int64_t>().swap(y.i1);
+            std::vector<int64_t>().swap(y.i2);
+            std::vector<int8_t>().swap(y.f);
+            p->payload += 16.0 * (double)y.npair;
+            p->index_bytes += 9.0 * (double)y.npair;
+        }
+        p->nsymtasks = (int64_t)stasks.size();
+        if (p->nsymtasks &&
+            (check(hipMalloc(&p->d_symsteps, steps.size() * sizeof(DevSymStep)), "hipMalloc(sym)") ||
+             check(hipMemcpy(p->d_symsteps, steps.data(), steps.size() * sizeof(DevSymStep), hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+             check(hipMalloc(&p->d_symtasks, stasks.size() * sizeof(DevSymTask)), "hipMalloc(sym)") ||
+             check(hipMemcpy(p->d_symtasks, stasks.data(), stasks.size() * sizeof(DevSymTask), hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+             check(hipMalloc(&p->d_sym_i1, a1.size() * 4), "hipMalloc(sym)") ||
+             check(hipMalloc(&p->d_sym_i2, a2.size() * 4), "hipMalloc(sym)") ||
+             check(hipMalloc(&p->d_sym_f, ff.size()), "hipMalloc(sym)") ||
+             check(hipMemcpy(p->d_sym_i1, a1.data(), a1.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+             check(hipMemcpy(p->d_sym_i2, a2.data(), a2.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
+             check(hipMemcpy(p->d_sym_f, ff.data(), ff.size(), hipMemcpyHostToDevice), "hipMemcpy(sym)")))
             return g_err;
-        std::vector<int64_t>().swap(y.i1);
-        std::vector<int64_t>().swap(y.i2);
-        std::vector<int8_t>().swap(y.f);
-        p->payload += 16.0 * (double)y.npair;
-        p->index_bytes += 17.0 * (double)y.npair;
     }
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -1023,6 +1077,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (check(hipMemcpyAsync(p->d_bufs, want.data(), want.size() * sizeof(double *), hipMemcpyHostToDevice, s), "hipMemcpyAsync(bufs)")) return g_err;
         if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
         p->last_bufs = want;
+        p->static_done = false;   // constant slots may hold other data now
     }
     const int64_t na = p->ntasks_affine, nd8 = p->ntasks_direct8, nd32 = p->ntasks_direct32;
     const int64_t ng = p->ntasks - na - nd8 - nd32;
@@ -1037,12 +1092,15 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (p->nitasks &&
         gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, s))
         return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
-    if (p->ngtasks) {
+    const int64_t g0 = p->static_done ? p->ngtasks_static : 0;
+    if (p->ngtasks > g0) {
         if (mark(3, true)) return g_err;
-        if (gecco_launch_gather(p->d_gjobs, p->d_gtasks, (int)p->ngtasks, p->d_bufs, p->d_scratch, s))
+        if (gecco_launch_gather(p->d_gjobs, p->d_gtasks + g0, (int)(p->ngtasks - g0), p->d_bufs,
+                                p->d_scratch, s))
             return fail(GECCO_HIP_EDEVICE, "gather launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(3, false)) return g_err;
     }
+    p->static_done = true;
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
@@ -1069,13 +1127,13 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
                                            tr.d_rowptr, tr.d_rowdst, tr.d_col, tr.d_sgn, tr.nrows,
                                            tr.fact, s))
             return fail(GECCO_HIP_EDEVICE, "trace launch failed");
-    bool anysym = false;
-    for (auto &y : p->syms) anysym |= y.npair > 0;
-    if (anysym && mark(2, true)) return g_err;
-    for (auto &y : p->syms)
-        if (y.npair && gecco_launch_sym_ab(bufs[y.buf], y.d_i1, y.d_i2, y.d_f, y.npair, y.prefac, s))
+    if (p->nsymtasks) {
+        if (mark(2, true)) return g_err;
+        if (gecco_launch_sym_batched(p->d_symsteps, p->d_symtasks, (int)p->nsymtasks, p->d_bufs,
+                                     p->d_sym_i1, p->d_sym_i2, p->d_sym_f, s))
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
-    if (anysym && mark(2, false)) return g_err;
+        if (mark(2, false)) return g_err;
+    }
     return ok();
 }
 
@@ -1394,6 +1452,15 @@ int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_d
     j.len = len;
     j.factor = factor;
     plan->inits.push_back(j);
+    return 0;
+}
+
+int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (slot < 0) return fail(GECCO_HIP_EINVAL, "negative slot");
+    if ((int32_t)plan->constant_slot.size() <= slot) plan->constant_slot.resize(slot + 1, 0);
+    plan->constant_slot[slot] = 1;
     return 0;
 }
 

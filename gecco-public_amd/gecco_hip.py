@@ -67,7 +67,8 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_free', 'gecco_hip_memcpy_htod', 'gecco_hip_memcpy_dtoh',
              'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
-             'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index')
+             'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
+             'gecco_hip_plan_set_constant')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -84,6 +85,7 @@ _lib.gecco_hip_plan_set_timing.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_timing.argtypes = [ctypes.c_void_p, PD, PI64, PD]
 _lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
 _lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_set_constant.argtypes = [ctypes.c_void_p, ctypes.c_int32]
 _lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD, PI64]
 _lib.gecco_hip_plan_block_index.argtypes = [ctypes.c_void_p, I64, I64, ctypes.POINTER(ctypes.c_int32),
                                             I64, ctypes.POINTER(ctypes.c_int32), I64]
@@ -237,6 +239,11 @@ class Plan:
         if table:
             return first.reshape(S, F).T, None
         return first, second
+
+    def set_constant(self, slot):
+        """Slot `slot` keeps its data between executes (materialised gathers
+        of it are reused); before finalize."""
+        _check(_lib.gecco_hip_plan_set_constant(self._h, int(slot)))
 
     def set_materialize(self, min_dim):
         """Materialise gathered operands of kernel calls with min(M, N) >=

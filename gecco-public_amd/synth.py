@@ -34,6 +34,32 @@ class Hamiltonian:
         return (Bm.T @ Bm).reshape(n, n, n, n)
 
 
+class DenseHamiltonian:
+    """Canonical closed-shell Hamiltonian given by orbital energies and the
+    dense (pq|rs) over the correlated orbitals (occupied then virtual, each
+    ordered by IRREP), e.g. imported from a quantum-chemistry program."""
+
+    def __init__(self, eps_o, eps_v, irrep_o, irrep_v, eri, nsym, e_ref=0.0):
+        self.eps_o, self.eps_v = np.asarray(eps_o, float), np.asarray(eps_v, float)
+        self.irrep_o = [int(x) for x in irrep_o]
+        self.irrep_v = [int(x) for x in irrep_v]
+        self.eri_full = np.ascontiguousarray(eri, dtype=np.float64)
+        self.nsym = int(nsym)
+        self.nocc, self.nvir = len(self.eps_o), len(self.eps_v)
+        self.e_ref = float(e_ref)       # reference (SCF) energy
+
+    def eri(self):
+        return self.eri_full
+
+
+def load_hamiltonian(path):
+    """DenseHamiltonian from an .npz with eps_o, eps_v, irrep_o, irrep_v,
+    nsym, eri and e_scf (the format tests/golden/make_ch2_dalton.py writes)."""
+    z = np.load(path, allow_pickle=False)
+    return DenseHamiltonian(z['eps_o'], z['eps_v'], z['irrep_o'], z['irrep_v'], z['eri'],
+                            int(z['nsym']), float(z['e_scf']))
+
+
 def synthetic(nocc, nvir, naux, nsym=1, seed=0, sigma=0.03, occ_range=(-2.0, -0.3),
               vir_range=(0.2, 3.0)):
     rng = np.random.default_rng(seed)

@@ -168,6 +168,12 @@ int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t iblock, int64
  * Before finalize only. */
 int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
                             int32_t buf_src, int64_t off_src, double factor);
+/* Declare that buffer slot `slot` holds the same data at every execute
+ * (e.g. integral lists): the materialised gathers of that slot are built on
+ * the first execute and again only when the slot's pointer changes.  The
+ * caller must not modify the slot's contents between executes.  Before
+ * finalize only. */
+int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot);
 /* Operand materialisation: a kernel call whose Op1/Op2 gathers are not
  * strided views and whose result tile counts satisfy min(n_ex1, n_ex2) >=
  * min_dim has those operands written densely (signed, zeros for forbidden

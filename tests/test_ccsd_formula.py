@@ -74,3 +74,17 @@ def test_dense_ccsd_is_exact_for_two_electrons():
         ham = synthetic(1, 6, 12, nsym=nsym, seed=3, sigma=0.15)
         e, *_ = DenseCCSD(ham).solve(1e-12)
         assert abs(e - fci_two_electron(ham)) < 1e-12
+
+
+def test_dense_ccsd_reproduces_reference_ch2_energy():
+    """Known answer of the reference's own test: CH2/DZ (DALTON files of
+    test/molecules/CH2-DZ-DALTON, frozen 1s), CCSD total energy
+    -38.907618798245 (test/CCPT-energy.e.ref:690, printed to 1e-12)."""
+    import os
+    from synth import load_hamiltonian
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'ch2_dz_dalton.npz')
+    ham = load_hamiltonian(path)
+    assert (ham.nocc, ham.nvir, ham.nsym) == (3, 9, 4)
+    e, *_ = DenseCCSD(ham).solve(1e-11)
+    ref = float(np.load(path)['e_ccsd_ref'])
+    assert abs(ham.e_ref + e - ref) < 5e-12
