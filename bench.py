@@ -4,10 +4,10 @@ Default workload (--workload ccsd): closed-shell CCSD amplitude iterations on
 the synthetic Hamiltonian nocc=20, nvir=100, naux=300, 4 IRREPs
 (round-robin), eps_occ ~ U[-2,-0.3], eps_vir ~ U[0.2,3.0], B ~ N(0,0.03^2),
 GeCCo Ms/IRREP-blocked ME-lists, spin-flip symmetry on, FP64.  One step =
-one iteration: the whole CCSD formula (33 binary contractions, 4632 kernel
-calls of GeCCo's enumerator, 24.9 GFLOP, three grouped launches plus
-spin-flip symmetrisations) and the preconditioned DIIS update, operands
-resident in HBM.  After the timed steps a fresh solve runs to
+one iteration: the whole CCSD formula (38 binary contractions, 4672 kernel
+calls of GeCCo's enumerator, 21.2 GFLOP, three plan executes with list
+initialisation, grouped contraction launches and spin-flip
+symmetrisation) and the native DIIS step, operands resident in HBM.  After the timed steps a fresh solve runs to
 ||Omega|| < 1e-8 and its energy is compared with the oracle's FP64 energy
 (tests/golden/ccsd_o20_v100_s4.json) when that fixture matches the recipe.
 

@@ -2,7 +2,13 @@
 
 The amplitude equations are the spin-orbital CCSD equations with the
 intermediates of Stanton, Gauss, Watts and Bartlett (J. Chem. Phys. 94, 4334
-(1991)), written as a list of GeCCo binary contractions over ME-lists in
+(1991)), refactored so that no v^4 intermediate is built:
+  1/2 tau_ij^ef W_abef = 1/2 tau_ij^ef <ab||ef> - P(ab) t_m^b Zamij
+                         + 1/8 tau_mn^ab tau_ij^ef <mn||ef>,
+  Zamij = 1/2 tau_ij^ef <am||ef>,
+and the last term is folded into W_mnij, whose tau.<mn||ef> term then
+carries 1/2 instead of 1/4.  The equations are
+written as a list of GeCCo binary contractions over ME-lists in
 GeCCo's Ms/IRREP-blocked layout -- the role the factorised formula file
 plays for GeCCo's scheduler (contract/frm_sched2.f:117-365): every item is a
 [CONTR] of two lists into a third, evaluated by the engine's plan
@@ -10,15 +16,16 @@ plays for GeCCo's scheduler (contract/frm_sched2.f:117-365): every item is a
 depend on each other and run as one grouped launch.
 
 Solver (solve/solve_nleq.f:377-787 with solve/optcont.f): diagonal
-preconditioner D = sum(eps of A strings) - sum(eps of C strings)
-(contract/dia4op_ev.f:300-365, step x = R/D as baserout/diavc.f), Pulay
+preconditioner D = sum(eps of C strings) - sum(eps of A strings)
+(contract/dia4op_ev.f:300-365, step r = g/D as baserout/diavc.f), Pulay
 DIIS on the preconditioned step (solve/optc_diis_extr.f:43-108), energy
-and residual norm in FP64.
+and residual norm in FP64 -- the library's native DIIS step
+(gecco_hip.DIIS).
 
 List conventions (dense reading of a list; C components then A components,
 P before H inside each side, as the list layout orders them):
   T1[a;i] = t_i^a, T2[ab;ij] = t_ij^ab, H<C><A>[c..;a..] = <c..||a..>,
-  Fae[a;e], Fmi[m;i], Fme[m;e], Wmnij[mn;ij], Wabef[ab;ef], Wmbej[b m;e j].
+  Fae[a;e], Fmi[m;i], Fme[m;e], Wmnij[mn;ij], Zamij[a m;ij], Wmbej[b m;e j].
 """
 import time
 
@@ -34,7 +41,7 @@ LISTS = {
     'Fae': ({P: 1}, {P: 1}), 'F2ae': ({P: 1}, {P: 1}),
     'Fmi': ({H: 1}, {H: 1}), 'F2mi': ({H: 1}, {H: 1}),
     'Fme': ({H: 1}, {P: 1}),
-    'Wmnij': ({H: 2}, {H: 2}), 'Wabef': ({P: 2}, {P: 2}),
+    'Wmnij': ({H: 2}, {H: 2}), 'Zamij': ({P: 1, H: 1}, {H: 2}),
     'Wmbej': ({P: 1, H: 1}, {P: 1, H: 1}),
     'Z': ({H: 2}, {P: 1, H: 1}), 'Y': ({P: 2}, {P: 1, H: 1}),
     'E': ({}, {}),
@@ -48,7 +55,7 @@ INTEGRALS = [n for n in LISTS if n.startswith('H')]
 # lists initialised as factor * source at the start of each iteration
 # (others: zero).  Hovvo reads <bm||ej> = -<mb||ej>, hence -1 for Wmbej.
 INIT = {'tau': ('T2', 1.0), 'taut': ('T2', 1.0), 'Wmnij': ('Hoooo', 1.0),
-        'Wabef': ('Hvvvv', 1.0), 'Wmbej': ('Hovvo', -1.0), 'R2': ('Hvvoo', 1.0)}
+        'Wmbej': ('Hovvo', -1.0), 'R2': ('Hvvoo', 1.0)}
 
 # (result, op1, op2, CNT on op1's C side, CNT on op1's A side, factor)
 LEVELS = [
@@ -61,7 +68,6 @@ LEVELS = [
      ('Fmi', 'T1', 'Hooov', {P: 1}, {H: 1}, 1.0),
      ('F2mi', 'T1', 'Hooov', {P: 1}, {H: 1}, 1.0),
      ('Wmnij', 'Hooov', 'T1', {}, {P: 1}, 1.0),
-     ('Wabef', 'Hovvv', 'T1', {H: 1}, {}, -1.0),
      ('Wmbej', 'Hovvv', 'T1', {}, {P: 1}, 1.0),
      ('Wmbej', 'T1', 'Hooov', {}, {H: 1}, 1.0),
      ('Wmbej', 'T2', 'Hoovv', {P: 1}, {H: 1}, 0.5),
@@ -77,8 +83,9 @@ LEVELS = [
      ('F2mi', 'Hoovv', 'taut', {H: 1}, {P: 2}, 1.0),
      ('F2ae', 'T1', 'Fme', {}, {H: 1}, -0.5),
      ('F2mi', 'Fme', 'T1', {}, {P: 1}, 0.5),
-     ('Wmnij', 'Hoovv', 'tau', {}, {P: 2}, 0.5),
-     ('Wabef', 'tau', 'Hoovv', {}, {H: 2}, 0.5),
+     ('Wmnij', 'Hoovv', 'tau', {}, {P: 2}, 1.0),
+     ('R2', 'Hvvvv', 'tau', {}, {P: 2}, 1.0),
+     ('Zamij', 'Hovvv', 'tau', {}, {P: 2}, 1.0),
      ('Wmbej', 'T1', 'Z', {}, {H: 1}, 1.0),
      ('R1', 'T2', 'Fme', {P: 1}, {H: 1}, 1.0),
      ('R2', 'Y', 'T1', {}, {P: 1}, -1.0),
@@ -88,7 +95,7 @@ LEVELS = [
      ('R2', 'F2ae', 'T2', {}, {P: 1}, 1.0),
      ('R2', 'T2', 'F2mi', {}, {H: 1}, -1.0),
      ('R2', 'tau', 'Wmnij', {}, {H: 2}, 1.0),
-     ('R2', 'Wabef', 'tau', {}, {P: 2}, 1.0),
+     ('R2', 'Zamij', 'T1', {H: 1}, {}, -1.0),
      ('R2', 'T2', 'Wmbej', {P: 1}, {H: 1}, -1.0)],
 ]
 

@@ -458,26 +458,26 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
                     }
         } else {
             // this task owns its result elements (injective map, one K
-            // chunk, no overlapping block): all loads are issued before the
-            // stores, so the read-modify-writes overlap in flight
-            int64_t addr[TM][TN][4];
-            double val[TM][TN][4], old[TM][TN][4];
+            // chunk, no overlapping block): per column sub-tile, all loads
+            // are issued before the stores, so the read-modify-writes
+            // overlap in flight while the live registers stay bounded
 #pragma unroll
-            for (int j = 0; j < TN; ++j)
+            for (int j = 0; j < TN; ++j) {
+                int64_t addr[TM][4];
+                double val[TM][4], old[TM][4];
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        elem(i, j, r, addr[i][j][r], val[i][j][r]);
-                        old[i][j][r] = addr[i][j][r] >= 0 ? pc[addr[i][j][r]] : 0.0;
+                        elem(i, j, r, addr[i][r], val[i][r]);
+                        old[i][r] = addr[i][r] >= 0 ? pc[addr[i][r]] : 0.0;
                     }
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (addr[i][j][r] >= 0) pc[addr[i][j][r]] = old[i][j][r] + val[i][j][r];
+                        if (addr[i][r] >= 0) pc[addr[i][r]] = old[i][r] + val[i][r];
+            }
         }
     }
 }
