@@ -283,6 +283,118 @@ struct AffView {
     }
 };
 
+// Element offset (-1: forbidden) and sign of result element (m, n) through
+// the result map oc (caff: both halves affine).
+__device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff, int m, int n,
+                                                 bool &neg) {
+    neg = false;
+    if (caff) return oc.f0 + oc.f1 * (int64_t)m + oc.s0 + oc.s1 * (int64_t)n;
+    if (oc.table) {
+        const int32_t t = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+        neg = t < 0;
+        return (int64_t)(t < 0 ? -t : t) - 1;
+    }
+    int64_t ar, ac;
+    bool nr = false, nc = false, ok = true;
+    if (oc.first_affine) {
+        ar = oc.f0 + oc.f1 * (int64_t)m;
+    } else {
+        const int32_t t = oc.first[m];
+        ok = t != 0; nr = t < 0; ar = (int64_t)(t < 0 ? -t : t) - 1;
+    }
+    if (oc.second_affine) {
+        ac = oc.s0 + oc.s1 * (int64_t)n;
+    } else {
+        const int32_t t = oc.second[n];
+        ok = ok && t != 0; nc = t < 0; ac = (int64_t)(t < 0 ? -t : t) - 1;
+    }
+    neg = nr != nc;
+    return ok ? ar + ac : -1;
+}
+
+// Tasks of the strided-GEMM launch whose tile is at most 16 x 16 (results
+// with a short side of at most 16 and a long contraction: energies, the
+// one-index intermediates and singles terms of coupled-cluster equations).
+// Each wave runs MFMA 16x16x4 over its own part of the contraction range,
+// with the operands loaded from memory straight into the MFMA lane layout
+// (lane: k = lane / 16, m or n = lane % 16) and the next step's loads in
+// flight during the current step's MFMAs; no LDS traffic and no barrier per
+// step.  The four partial tiles are summed in LDS (1024 doubles) at the end.
+// A step covers 16 * SKINNY_U consecutive k: 4 * SKINNY_U per wave.
+constexpr int SKINNY_U = 8;
+
+__device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock *__restrict__ blocks,
+                                            double *const *__restrict__ bufs, double *lds, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int km = lane >> 4, xl = lane & 15;
+    const DevBlock &b = blocks[task.blk];
+    const int M = b.M, N = b.N;
+    const int m = task.m0 + xl, n = task.n0 + xl;
+    const bool mok = m < M, nok = n < N;
+    const int koff = 4 * SKINNY_U * wave + km;   // k of MFMA u: step start + koff + 4u
+    v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+    // range [task.k0, task.k1) from the start of block task.blk, along its chain
+    int blk = task.blk, kb = 0, kl = task.k0;
+    for (;;) {
+        const DevBlock &cb = blocks[blk];
+        const int kh = min(cb.K, task.k1 - kb);
+        const DevOperand &oa = cb.A, &ob = cb.B;
+        const double *__restrict__ qa =
+            bufs[cb.buf_a] + cb.base_a + oa.f0 + oa.s0 + (mok ? oa.s1 * (int64_t)m : 0);
+        const double *__restrict__ qb =
+            bufs[cb.buf_b] + cb.base_b + ob.f0 + ob.s0 + (nok ? ob.s1 * (int64_t)n : 0);
+        const int64_t fa = oa.f1, fb = ob.f1;
+        double a[SKINNY_U], bv[SKINNY_U];
+        auto load = [&](int k0, double *ra, double *rb) {
+#pragma unroll
+            for (int u = 0; u < SKINNY_U; ++u) {
+                const int k = k0 + koff + 4 * u;
+                const bool ok = k < kh;
+                ra[u] = (ok && mok) ? qa[fa * (int64_t)k] : 0.0;
+                rb[u] = (ok && nok) ? qb[fb * (int64_t)k] : 0.0;
+            }
+        };
+        load(kl, a, bv);
+        for (int k0 = kl; k0 < kh; k0 += 16 * SKINNY_U) {
+            double an[SKINNY_U], bn[SKINNY_U];
+            const bool more = k0 + 16 * SKINNY_U < kh;
+            if (more) load(k0 + 16 * SKINNY_U, an, bn);
+#pragma unroll
+            for (int u = 0; u < SKINNY_U; ++u)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], acc, 0, 0, 0);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < SKINNY_U; ++u) { a[u] = an[u]; bv[u] = bn[u]; }
+            }
+        }
+        kb += cb.K;
+        blk = cb.next;
+        if (blk < 0 || kb >= task.k1) break;
+        kl = 0;
+    }
+    __syncthreads();   // LDS of the previous task is no longer read
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[wave * 256 + r * 64 + lane] = acc[r];
+    __syncthreads();
+    // thread tid owns tile element (r, lane) = (tid / 64, tid % 64):
+    // m = m0 + 4 r + lane / 16, n = n0 + lane % 16
+    const double sum = lds[tid] + lds[256 + tid] + lds[512 + tid] + lds[768 + tid];
+    const int mm = task.m0 + (tid >> 6) * 4 + (lane >> 4), nn = task.n0 + (lane & 15);
+    if (mm < M && nn < N) {
+        const DevOperand &oc = b.C;
+        const bool caff = !oc.table && oc.first_affine && oc.second_affine;
+        bool neg;
+        const int64_t a = result_offset(oc, caff, mm, nn, neg);
+        if (a >= 0) {
+            double *pc = bufs[b.buf_c] + b.base_c;
+            const double x = b.xfac * sum;
+            const double v = neg ? -x : x;
+            if (b.atomic) unsafeAtomicAdd(pc + a, v);
+            else pc[a] += v;
+        }
+    }
+}
+
 template <int TBM, int TBN, int TBK>
 __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
@@ -293,6 +405,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
     // double-buffered A and B tiles; the whole array stages the result
     // range of a task with span > 0 after the K loop
     __shared__ double lds[2 * SA + 2 * SB];
+    static_assert(2 * SA + 2 * SB >= 1024, "LDS too small for the skinny-task reduction");
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -301,9 +414,20 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
+        if (task.pad) {   // tile of at most 16 x 16
+            skinny_task(task, blocks, bufs, lds, tid);
+            continue;
+        }
         const DevBlock &b = blocks[task.blk];
         const int M = b.M, N = b.N;
-        const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
+        const int m0 = task.m0, n0 = task.n0;
+        // The K steps walk the task's range [task.k0, task.k1), counted from
+        // the start of block task.blk and continued along its chain: blocks
+        // of one chain share M, N and the result map, and differ in their
+        // operands.  (blk, kb): current block and the chain position of its
+        // start; [kl, kh): block-local range of the current step's block.
+        int blk = task.blk, kb = 0, kcur = b.K;
+        int kl = task.k0, kh = min(kcur, task.k1);
         const double *__restrict__ pa = bufs[b.buf_a] + b.base_a;
         const double *__restrict__ pb = bufs[b.buf_b] + b.base_b;
         AffView<TBM, TBK> va;
@@ -326,17 +450,34 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
 
         double ra[AffView<TBM, TBK>::L], rb[AffView<TBN, TBK>::L];
         __syncthreads();   // LDS of the previous task is no longer read
-        va.load(pa, task.k0, k1, ra);
-        vb.load(pb, task.k0, k1, rb);
+        va.load(pa, kl, kh, ra);
+        vb.load(pb, kl, kh, rb);
         va.template store<LDA>(lds, ra);
         vb.template store<LDB>(lds + 2 * SA, rb);
         __syncthreads();
         int cur = 0;
-        for (int k0 = task.k0; k0 < k1; k0 += TBK) {
-            const bool more = k0 + TBK < k1;
+        for (;;) {
+            // next step: the rest of this block, else the next chain block
+            bool more = true;
+            kl += TBK;
+            if (kl >= kh) {
+                kb += kcur;
+                blk = blocks[blk].next;
+                more = blk >= 0 && kb < task.k1;
+                if (more) {
+                    const DevBlock &nb = blocks[blk];
+                    kcur = nb.K;
+                    kl = 0;
+                    kh = min(kcur, task.k1 - kb);
+                    pa = bufs[nb.buf_a] + nb.base_a;
+                    pb = bufs[nb.buf_b] + nb.base_b;
+                    va.init(nb.A, m0, M, tid);
+                    vb.init(nb.B, n0, N, tid);
+                }
+            }
             if (more) {
-                va.load(pa, k0 + TBK, k1, ra);
-                vb.load(pb, k0 + TBK, k1, rb);
+                va.load(pa, kl, kh, ra);
+                vb.load(pb, kl, kh, rb);
             }
             const double *A = lds + cur * SA;
             const double *B = lds + 2 * SA + cur * SB;
@@ -361,6 +502,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
             }
             __syncthreads();
             cur ^= 1;
+            if (!more) break;
         }
 
         const DevOperand &oc = b.C;
@@ -374,32 +516,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
             const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
             a = -1;
             bool neg = false;
-            if (m < M && n < N) {
-                if (caff) {
-                    a = oc.f0 + oc.f1 * (int64_t)m + oc.s0 + oc.s1 * (int64_t)n;
-                } else if (oc.table) {
-                    const int32_t t = oc.first[(int64_t)m + oc.ld * (int64_t)n];
-                    neg = t < 0;
-                    a = (int64_t)(t < 0 ? -t : t) - 1;
-                } else {
-                    int64_t ar, ac;
-                    bool nr = false, nc = false, ok = true;
-                    if (oc.first_affine) {
-                        ar = oc.f0 + oc.f1 * (int64_t)m;
-                    } else {
-                        const int32_t t = oc.first[m];
-                        ok = t != 0; nr = t < 0; ar = (int64_t)(t < 0 ? -t : t) - 1;
-                    }
-                    if (oc.second_affine) {
-                        ac = oc.s0 + oc.s1 * (int64_t)n;
-                    } else {
-                        const int32_t t = oc.second[n];
-                        ok = ok && t != 0; nc = t < 0; ac = (int64_t)(t < 0 ? -t : t) - 1;
-                    }
-                    neg = nr != nc;
-                    a = ok ? ar + ac : -1;
-                }
-            }
+            if (m < M && n < N) a = result_offset(oc, caff, m, n, neg);
             const double x = xfac * acc[i][j][r];
             v = neg ? -x : x;
         };

@@ -34,11 +34,17 @@ struct DevBlock {
     int32_t buf_a, buf_b, buf_c;      // buffer slots
     int32_t M, N, K;
     int32_t atomic;                   // result tiles may collide
-    int32_t pad;
+    // next block of the chain (-1: none): blocks of the strided-GEMM kernel
+    // that add into the same result block through the same result map; a
+    // task's contraction range continues into the next block of its chain
+    int32_t next;
     DevOperand A, B, C;               // A: (k,m)  B: (k,n)  C: (m,n)
 };
 
 struct DevTask {
+    // contraction range [k0, k1) of the tile (m0, n0): counted from the start
+    // of block blk, continuing along its chain (strided-GEMM kernel) when k1
+    // exceeds blk's K
     int32_t blk, m0, n0, k0, k1;
     // span > 0 (strided-GEMM kernel only): the tile's results are first
     // reduced in LDS over the block-relative element range [lo, lo + span)

@@ -414,7 +414,9 @@ struct HostBlock {
     bool injective;
     bool strided = false;             // A and B strided (natively or materialised)
     bool direct = false;              // runs on the short-K vector-ALU kernel
+    bool skinny = false;              // strided-GEMM launch, 16 x 16 wave-split-K tiles
     bool mat_a = false, mat_b = false;  // A / B materialised into scratch
+    int32_t next = -1;                  // next block of its chain (finalize)
     HostOperand A, B, C;
 };
 
@@ -474,6 +476,10 @@ struct gecco_hip_plan {
     double *d_scratch = nullptr;
     int64_t ngjobs = 0, ngtasks = 0, scratch_elems = 0;
     int64_t materialize_min = 1;    // min(M, N) from which gathers are materialised
+    bool chain = true;              // link strided-GEMM blocks with a common result tile
+    int64_t skinny_max = 16;        // min(M, N) up to which 16 x 16 wave-split-K tiles are used
+    int64_t skinny_kchunk = 2048;   // contraction length of one such task
+    int64_t nchained = 0;           // blocks linked behind a chain head
     int64_t nstaged = 0;            // tasks with an LDS-staged result range
     double gather_bytes = 0;
     int affine_variant = 1;
@@ -656,20 +662,6 @@ static int plan_finalize(gecco_hip_plan *p) {
         ranges.push_back({{(int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12,
                            (int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12 + b.len12}, i});
     }
-    std::sort(ranges.begin(), ranges.end());
-    std::vector<uint8_t> overlap(p->blocks.size(), 0);
-    int64_t maxend = -1;
-    size_t maxend_i = 0;
-    for (size_t r = 0; r < ranges.size(); ++r) {
-        if (r > 0 && ranges[r].first.first < maxend) {
-            overlap[ranges[r].second] = 1;
-            overlap[maxend_i] = 1;
-        }
-        if (ranges[r].first.second > maxend) {
-            maxend = ranges[r].first.second;
-            maxend_i = ranges[r].second;
-        }
-    }
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
     for (auto &j : p->inits) nb_max = std::max<int64_t>(nb_max, std::max(j.dst, j.src) + 1);
     for (auto &tr : p->traces) nb_max = std::max<int64_t>(nb_max, std::max(tr.buf_src, tr.buf_dst) + 1);
@@ -700,6 +692,76 @@ static int plan_finalize(gecco_hip_plan *p) {
         b.direct = b.strided && b.K <= p->direct_kmax &&
                    (std::min(b.M, b.N) <= 32 || b.K <= 8);
         is_aff[i] = b.strided && !b.direct ? 1 : 0;
+        // results with a short side of at most skinny_max: 16 x 16 tiles
+        // whose contraction range is split between the waves (skinny_task)
+        b.skinny = is_aff[i] && std::min(b.M, b.N) <= p->skinny_max;
+    }
+    // Chains: strided-GEMM blocks that add into the same result block
+    // through the same result map -- e.g. the IRREP/Ms distributions of the
+    // contraction indices of a ring term, each a kernel call of the
+    // reference with its own operand blocks -- are linked (DevBlock::next).
+    // One task runs the GEMM over their concatenated contraction ranges and
+    // writes its result tile once, instead of one read-modify-write (or
+    // atomic) pass per block.
+    std::vector<int32_t> chain_next(p->blocks.size(), -1);
+    std::vector<size_t> chain_head(p->blocks.size());
+    {
+        std::unordered_map<uint64_t, std::vector<size_t>> heads;
+        std::vector<size_t> tail(p->blocks.size());
+        auto same_result = [&](const HostBlock &x, const HostBlock &y) {
+            return x.buf12 == y.buf12 && x.off12 == y.off12 && x.M == y.M && x.N == y.N &&
+                   x.xfac == y.xfac && x.injective == y.injective && x.C.r.table == y.C.r.table &&
+                   x.C.r.first == y.C.r.first && x.C.r.second == y.C.r.second;
+        };
+        p->nchained = 0;
+        for (size_t i = 0; i < p->blocks.size(); ++i) {
+            chain_head[i] = i;
+            tail[i] = i;
+            if (!is_aff[i] || !p->chain) continue;
+            const HostBlock &b = p->blocks[i];
+            uint64_t h = 1469598103934665603ull;
+            auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+            mix((uint64_t)b.buf12); mix((uint64_t)b.off12); mix((uint64_t)b.M); mix((uint64_t)b.N);
+            uint64_t xb;
+            std::memcpy(&xb, &b.xfac, 8);
+            mix(xb);
+            mix((uint64_t)b.C.r.table);
+            for (auto v : b.C.r.first) mix((uint64_t)(uint32_t)v);
+            for (auto v : b.C.r.second) mix((uint64_t)(uint32_t)v);
+            auto &cand = heads[h];
+            bool linked = false;
+            for (size_t c : cand)
+                if (same_result(p->blocks[c], b)) {
+                    chain_head[i] = c;
+                    chain_next[tail[c]] = (int32_t)i;
+                    tail[c] = i;
+                    linked = true;
+                    ++p->nchained;
+                    break;
+                }
+            if (!linked) cand.push_back(i);
+        }
+    }
+    // range overlap among result blocks: a chain counts once
+    std::vector<uint8_t> overlap(p->blocks.size(), 0);
+    {
+        std::vector<std::pair<std::pair<int64_t, int64_t>, size_t>> hr;
+        for (auto &r : ranges)
+            if (chain_head[r.second] == r.second) hr.push_back(r);
+        std::sort(hr.begin(), hr.end());
+        int64_t maxend = -1;
+        size_t maxend_i = 0;
+        for (size_t r = 0; r < hr.size(); ++r) {
+            if (r > 0 && hr[r].first.first < maxend) {
+                overlap[hr[r].second] = 1;
+                overlap[maxend_i] = 1;
+            }
+            if (hr[r].first.second > maxend) {
+                maxend = hr[r].first.second;
+                maxend_i = hr[r].second;
+            }
+        }
+        for (size_t i = 0; i < p->blocks.size(); ++i) overlap[i] = overlap[chain_head[i]];
     }
     int abm = 64, abn = 64, abk = 32;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
@@ -718,6 +780,8 @@ static int plan_finalize(gecco_hip_plan *p) {
             // whose result range fits the LDS stage
             tn[i] = (b.N <= 64 || !b.injective) ? 64 : b.N <= 128 ? 128 : 256;
             tm[i] = 8 * 256 / tn[i];
+        } else if (b.skinny) {
+            cls[i] = 0; tm[i] = 16; tn[i] = 16;
         } else if (is_aff[i]) {
             cls[i] = 0; tm[i] = abm; tn[i] = abn;
         } else {
@@ -730,23 +794,39 @@ static int plan_finalize(gecco_hip_plan *p) {
     int64_t ntiles = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
+        if (chain_head[i] != i) continue;
         ntiles += ((b.M + tile_m(i) - 1) / tile_m(i)) * ((b.N + tile_n(i) - 1) / tile_n(i));
     }
     const int64_t want = p->target_tasks;
     int64_t split = ntiles > 0 ? (want + ntiles - 1) / ntiles : 1;
     std::vector<DevTask> tasks;
     std::vector<DevBlock> dblocks(p->blocks.size());
+    std::vector<int64_t> nsplit(p->blocks.size(), 1);
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
+        const bool head = chain_head[i] == i;
+        // contraction length of the chain
+        int64_t Kc = 0;
+        for (int64_t j = (int64_t)i; head && j >= 0; j = chain_next[(size_t)j]) Kc += p->blocks[(size_t)j].K;
         // parallel split for the whole plan, and a bound on the serial K
         // extent of one task (rank-reducing blocks, e.g. full contractions);
         // direct blocks are never split
-        int64_t ns = std::min<int64_t>(split, std::max<int64_t>(1, b.K / p->min_kchunk));
-        ns = std::max<int64_t>(ns, (b.K + p->max_kchunk - 1) / p->max_kchunk);
-        int64_t chunk = (b.K + ns - 1) / ns;
-        chunk = ((chunk + 31) / 32) * 32;
-        ns = (b.K + chunk - 1) / chunk;
-        if (b.direct) { ns = 1; chunk = b.K; }
+        int64_t ns = 1, chunk = Kc;
+        if (!head) {
+            ns = nsplit[chain_head[i]];
+        } else if (Kc == 0) {
+            ns = 0;                       // empty contraction range: no tasks
+        } else if (b.skinny) {
+            ns = (Kc + p->skinny_kchunk - 1) / p->skinny_kchunk;
+            chunk = (Kc + ns - 1) / ns;
+        } else if (!b.direct) {
+            ns = std::min<int64_t>(split, std::max<int64_t>(1, Kc / p->min_kchunk));
+            ns = std::max<int64_t>(ns, (Kc + p->max_kchunk - 1) / p->max_kchunk);
+            chunk = (Kc + ns - 1) / ns;
+            chunk = ((chunk + 31) / 32) * 32;
+            ns = (Kc + chunk - 1) / chunk;
+        }
+        nsplit[i] = ns;
         DevBlock &d = dblocks[i];
         d.xfac = b.xfac;
         d.base_a = b.off1;
@@ -759,12 +839,21 @@ static int plan_finalize(gecco_hip_plan *p) {
         d.N = (int32_t)b.N;
         d.K = (int32_t)b.K;
         d.atomic = (ns > 1 || !b.injective || overlap[i]) ? 1 : 0;
-        d.pad = 0;
+        d.next = b.next = chain_next[i];
+        if (!head) continue;
+        // a task's range [k0, k1) counts from the start of the chain block
+        // it begins in
         for (int64_t m0 = 0; m0 < b.M; m0 += tile_m(i))
             for (int64_t n0 = 0; n0 < b.N; n0 += tile_n(i))
                 for (int64_t s = 0; s < ns; ++s) {
-                    int64_t k0 = s * chunk, k1 = std::min(b.K, k0 + chunk);
-                    tasks.push_back({(int32_t)i, (int32_t)m0, (int32_t)n0, (int32_t)k0, (int32_t)k1});
+                    const int64_t c0 = s * chunk, c1 = std::min(Kc, c0 + chunk);
+                    int64_t j = (int64_t)i, start = 0;
+                    while (start + p->blocks[(size_t)j].K <= c0) {
+                        start += p->blocks[(size_t)j].K;
+                        j = chain_next[(size_t)j];
+                    }
+                    tasks.push_back({(int32_t)j, (int32_t)m0, (int32_t)n0, (int32_t)(c0 - start),
+                                     (int32_t)(c1 - start)});
                 }
     }
     // Result range of every task.  Tasks of the strided-GEMM kernel whose
@@ -783,7 +872,9 @@ static int plan_finalize(gecco_hip_plan *p) {
             const HostBlock &b = p->blocks[i];
             task.span = 0;
             task.owned = 0;
-            task.pad = b.direct ? (int32_t)tn[i] : 0;   // columns per direct task
+            // direct tasks: columns per task; strided-GEMM launch: 1 for a
+            // 16 x 16 wave-split-K task
+            task.pad = b.direct ? (int32_t)tn[i] : b.skinny ? 1 : 0;
             task.lo = 0;
             const int64_t lds_cap = b.direct ? lds_cap_direct : lds_cap_gemm;
             const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
@@ -794,7 +885,8 @@ static int plan_finalize(gecco_hip_plan *p) {
             const int64_t key = (int64_t)b.buf12 << 44;
             ivs.push_back({key + b.off12 + amin, key + b.off12 + amax, t});
             const int64_t span = amax - amin + 1;
-            if (b.strided && (b.C.r.table || !b.injective) && span <= lds_cap && 4 * cnt >= span) {
+            if (b.strided && !b.skinny && (b.C.r.table || !b.injective) && span <= lds_cap &&
+                4 * cnt >= span) {
                 task.span = (int32_t)span;
                 task.lo = amin;
             }
@@ -810,9 +902,14 @@ static int plan_finalize(gecco_hip_plan *p) {
         p->nstaged = 0;
         for (auto &t : tasks) p->nstaged += t.span > 0;
     }
+    // by class, then longest first (a wave-split-K task covers 8 times the
+    // contraction length of a tile task in about the same time)
+    auto cost = [&](const DevTask &x) {
+        return (int64_t)(x.k1 - x.k0) / (p->blocks[(size_t)x.blk].skinny ? 8 : 1);
+    };
     std::stable_sort(tasks.begin(), tasks.end(), [&](const DevTask &x, const DevTask &y) {
         if (cls[x.blk] != cls[y.blk]) return cls[x.blk] < cls[y.blk];
-        return (x.k1 - x.k0) > (y.k1 - y.k0);
+        return cost(x) > cost(y);
     });
     int64_t ncls[4] = {0, 0, 0, 0};
     for (auto &t : tasks) ncls[cls[t.blk]]++;
@@ -1346,6 +1443,9 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MAX_KCHUNK")) (*plan)->max_kchunk = std::max(64L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MATERIALIZE_MIN")) (*plan)->materialize_min = std::max(0L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_CHAIN")) (*plan)->chain = atoi(e) != 0;
+    if (const char *e = getenv("GECCO_HIP_SKINNY_MAX")) (*plan)->skinny_max = std::max(0L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_SKINNY_KCHUNK")) (*plan)->skinny_kchunk = std::max(128L, atol(e));
     if (const char *e = getenv("GECCO_HIP_DIRECT_KMAX")) (*plan)->direct_kmax = std::min(32L, std::max(0L, atol(e)));
     return 0;
 }
@@ -1416,6 +1516,9 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *in
     info[15] = !b.C.r.table && affine_of(b.C.r.second, a0, a1);
     if (ninfo >= 19) { info[16] = b.strided; info[17] = b.mat_a; info[18] = b.mat_b; }
     if (ninfo >= 20) info[19] = b.direct;
+    if (ninfo >= 23) { info[20] = b.buf12; info[21] = b.off12; info[22] = b.len12; }
+    if (ninfo >= 24) info[23] = b.next;
+    if (ninfo >= 25) info[24] = b.skinny;
     return 0;
 }
 
@@ -1461,6 +1564,13 @@ int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot) {
     if (slot < 0) return fail(GECCO_HIP_EINVAL, "negative slot");
     if ((int32_t)plan->constant_slot.size() <= slot) plan->constant_slot.resize(slot + 1, 0);
     plan->constant_slot[slot] = 1;
+    return 0;
+}
+
+int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    plan->chain = on != 0;
     return 0;
 }
 

@@ -68,7 +68,7 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
-             'gecco_hip_plan_set_constant')
+             'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -86,6 +86,7 @@ _lib.gecco_hip_plan_timing.argtypes = [ctypes.c_void_p, PD, PI64, PD]
 _lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
 _lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_set_constant.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+_lib.gecco_hip_plan_set_chain.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD, PI64]
 _lib.gecco_hip_plan_block_index.argtypes = [ctypes.c_void_p, I64, I64, ctypes.POINTER(ctypes.c_int32),
                                             I64, ctypes.POINTER(ctypes.c_int32), I64]
@@ -220,8 +221,8 @@ class Plan:
         return dict(ms=list(ms), launches=list(n), flops=list(fl))
 
     def block_info(self, i):
-        info = (I64 * 20)()
-        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 20))
+        info = (I64 * 25)()
+        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 25))
         return list(info)
 
     def block_index(self, i, which):
@@ -244,6 +245,12 @@ class Plan:
         """Slot `slot` keeps its data between executes (materialised gathers
         of it are reused); before finalize."""
         _check(_lib.gecco_hip_plan_set_constant(self._h, int(slot)))
+
+    def set_chain(self, on):
+        """Run strided-GEMM kernel calls that add into the same result block
+        through the same result map as one GEMM over their concatenated
+        contraction ranges (default on); before finalize."""
+        _check(_lib.gecco_hip_plan_set_chain(self._h, 1 if on else 0))
 
     def set_materialize(self, min_dim):
         """Materialise gathered operands of kernel calls with min(M, N) >=

@@ -149,8 +149,12 @@ int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, d
 /* Kernel-call details for diagnostics: [M, N, K, swap, injective, A/B/C
  * table, A/B fast_first, affine flags of A/B/C first/second vectors] and,
  * with ninfo >= 19, [A and B strided, A materialised, B materialised] and,
- * with ninfo >= 20, [direct short-contraction kernel].
- * Index flags are valid before finalize, the last three after it. */
+ * with ninfo >= 20, [direct short-contraction kernel] and, with
+ * ninfo >= 23, [result buffer, result block offset, result block length]
+ * and, with ninfo >= 24, [next kernel call of its chain, -1 if none; see
+ * gecco_hip_plan_set_chain] and, with ninfo >= 25, [16 x 16 wave-split-K
+ * tiles: a result with a short side of at most 16].  Index flags are valid
+ * before finalize, the kernel-class and chain entries after it. */
 int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t iblock, int64_t *info,
                               int64_t ninfo);
 /* Resolved signed index data of operand `which` (0 = A, 1 = B, 2 = result C,
@@ -182,6 +186,13 @@ int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot);
  * strided-GEMM or direct kernel.  0 disables; default 1.  Before finalize
  * only. */
 int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim);
+/* Chains (default on): kernel calls of the strided-GEMM kernel that add
+ * into the same result block through the same result map -- in the
+ * reference, the calls of contract/contr_op1op2_wmaps_c.f for the different
+ * Ms/IRREP distributions of the contracted strings that land in one result
+ * block -- are run as one GEMM over their concatenated contraction ranges,
+ * so each result tile is written once.  Before finalize only. */
+int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
 /* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
  * bytes they move per execute (one read and one write per element), and the
  * number of tasks whose result tile is reduced in LDS and written back as

@@ -215,12 +215,15 @@ def test_materialised_gathers_share_jobs():
     assert _relerr(bufs[2].cpu().numpy(), 2 * ref) <= TOL
 
 
-@pytest.mark.parametrize('materialize,variant', [(0, 1), (1, 1), (1, 0), (1, 4)])
-def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
+@pytest.mark.parametrize('materialize,variant,chain', [(0, 1, 1), (1, 1, 1), (1, 0, 1), (1, 4, 1),
+                                                    (1, 1, 0)])
+def test_ccsd_terms_match_oracle(materialize, variant, chain, monkeypatch):
     """Every binary contraction of the CCSD formula table, as kernel calls of
     the product enumerator, against the oracle's contr_op1op2 restatement.
     Covers table-indexed and non-injective results (antisymmetrised outer
-    products) on the LDS-staged write-back of the strided-GEMM kernel."""
+    products) on the LDS-staged write-back of the strided-GEMM kernel, and
+    chained kernel calls (one GEMM over the contraction ranges of the calls
+    that add into one result block)."""
     torch = _torch()
     import gecco_hip
     import ccsd
@@ -231,10 +234,11 @@ def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
     orbs = orbitals(nocc, nvir, nsym)
     o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
     dev = torch.device('cuda:0')
-    staged = 0
+    staged = linked = skinny = 0
     for li, level in enumerate(ccsd.LEVELS):
         for ti, (res, a, b, cc, ca, fact) in enumerate(level):
-            print(f'term L{li}.{ti} {res}<-{a}x{b} materialize={materialize} variant={variant}', flush=True)
+            print(f'term L{li}.{ti} {res}<-{a}x{b} materialize={materialize} variant={variant} '
+                  f'chain={chain}', flush=True)
             La, Lb, Lr = (OMEList(orbs, *ccsd.LISTS[n]) for n in (a, b, res))
             from oracle.contract import Contraction
             c = Contraction(La, Lb, Lr, cc, ca, fact)
@@ -246,10 +250,14 @@ def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
             A, B, R = (gecco_hip.MEList(o, *ccsd.LISTS[n]) for n in (a, b, res))
             plan = gecco_hip.Plan()
             plan.set_materialize(materialize)
-            plan.add_contraction(A, B, R, cc, ca, fact, True)
+            plan.set_chain(chain)
+            nb, _ = plan.add_contraction(A, B, R, cc, ca, fact, True)
             plan.add_sym_ab(R, 2)
             plan.finalize()
             staged += plan.workspace()['staged_tasks']
+            infos = [plan.block_info(i) for i in range(nb)]
+            linked += sum(x[23] >= 0 for x in infos)
+            skinny += sum(x[24] for x in infos)
             bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
                     torch.from_numpy(x0).to(dev)]
             plan.execute(bufs)
@@ -257,6 +265,10 @@ def test_ccsd_terms_match_oracle(materialize, variant, monkeypatch):
             assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL, (res, a, b)
     if materialize:
         assert staged > 0
+    # chains link strided-GEMM calls only (gathered operands are on that
+    # kernel only when materialised)
+    assert (linked > 0) == bool(chain and materialize)
+    assert skinny > 0 or not materialize     # 16 x 16 wave-split-K tiles
 
 
 @pytest.mark.parametrize('nsym', [1, 2, 4])
