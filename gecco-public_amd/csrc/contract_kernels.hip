@@ -602,9 +602,11 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
 // Short contractions (K <= KMAX) with strided A and B, e.g. the rank-1 to
 // rank-25 updates T1 x integral of CCSD: MFMA tiles would idle on padding
 // and on per-task latency, so each thread computes whole dot products on the
-// vector ALU.  A task covers dm rows x dn columns (dn in {64, 128, 256},
-// dm = 8 * NT / dn): thread
-// (nl, ml) owns column n0 + nl and rows m0 + ml + (NT/dn) * i, i < 8.  The A
+// vector ALU.  A task covers dm rows x dn columns, task.pad = dn | ng << 9 |
+// rpt << 18: the first ng * dn <= NT threads form ng row groups of dn
+// columns (the others only help to load), dm = rpt * ng with
+// K * dm <= DIRECT_A.  Thread
+// (nl, ml) owns column n0 + nl and rows m0 + ml + ng * i, i < rpt.  The A
 // tile (K x dm) is staged in LDS, the thread's B column in registers.  The
 // result map is read n-fastest (table = 2 stores the table transposed).
 constexpr int DIRECT_A = 32 * 32;        // LDS doubles for the A tile
@@ -622,8 +624,10 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
         const DevTask task = tasks[t];
         const DevBlock &b = blocks[task.blk];
         const int M = b.M, N = b.N, K = b.K;
-        const int dn = task.pad, dm = 8 * NT / dn, ng = NT / dn;
+        const int dn = task.pad & 511, ng = (task.pad >> 9) & 511, rpt = task.pad >> 18;
+        const int dm = rpt * ng;
         const int nl = tid % dn, ml = tid / dn;
+        const bool active = ml < ng;
         const int m0 = task.m0, n = task.n0 + nl;
         const DevOperand &oa = b.A, &ob = b.B, &oc = b.C;
         const double *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
@@ -644,7 +648,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
         __syncthreads();
         // result column data (separable / affine maps)
         int64_t cadd = 0;
-        bool cneg = false, cok = n < N;
+        bool cneg = false, cok = active && n < N;
         if (cok && oc.table == 0) {
             if (oc.second_affine) {
                 cadd = oc.s0 + oc.s1 * (int64_t)n;
@@ -661,13 +665,16 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
         for (int i = 0; i < 8; ++i) {
             const int r = ml + ng * i;
             const int m = m0 + r;
+            const bool live = i < rpt && cok;
             double acc = 0.0;
+            if (live) {
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k)
-                if (k < K) acc = fma(As[k * dm + r], bk[k], acc);
+                for (int k = 0; k < KMAX; ++k)
+                    if (k < K) acc = fma(As[k * dm + r], bk[k], acc);
+            }
             int64_t a = -1;
             bool neg = cneg;
-            if (m < M && cok) {
+            if (live && m < M) {
                 if (oc.table) {
                     const int32_t v = oc.first[(int64_t)n + oc.ld * (int64_t)m];
                     neg = v < 0;

@@ -770,16 +770,23 @@ static int plan_finalize(gecco_hip_plan *p) {
     // K <= 32, 3 gather kernel; tile shape per block
     std::vector<int> cls(p->blocks.size());
     std::vector<int64_t> tm(p->blocks.size()), tn(p->blocks.size());
+    // direct tasks: columns dn | row groups ng << 9 | rows per thread << 18
+    std::vector<int32_t> dpack(p->blocks.size(), 0);
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
         if (b.direct) {
             cls[i] = b.K <= 8 ? 1 : 2;
-            // columns per task: 64, 128 or 256 (a divisor of the 256
-            // threads); 64 (with 32 rows) when the result map is not
-            // injective, so that colliding elements tend to share a task
-            // whose result range fits the LDS stage
-            tn[i] = (b.N <= 64 || !b.injective) ? 64 : b.N <= 128 ? 128 : 256;
-            tm[i] = 8 * 256 / tn[i];
+            // columns per task: all of N up to 256 threads, at most 64 when
+            // the result map is not injective (colliding elements then tend
+            // to share a task whose result range fits the LDS stage); as
+            // many row groups as fit the 256 threads and the K x rows A tile
+            // of 1024 doubles, up to 8 rows per thread
+            const int64_t dn = std::min<int64_t>(b.N, b.injective ? 256 : 64);
+            const int64_t ng = std::min<int64_t>(256 / dn, std::max<int64_t>(1, 1024 / b.K));
+            const int64_t rpt = std::min<int64_t>(8, 1024 / (b.K * ng));
+            tn[i] = dn;
+            tm[i] = rpt * ng;
+            dpack[i] = (int32_t)(dn | (ng << 9) | (rpt << 18));
         } else if (b.skinny) {
             cls[i] = 0; tm[i] = 16; tn[i] = 16;
         } else if (is_aff[i]) {
@@ -874,7 +881,7 @@ static int plan_finalize(gecco_hip_plan *p) {
             task.owned = 0;
             // direct tasks: columns per task; strided-GEMM launch: 1 for a
             // 16 x 16 wave-split-K task
-            task.pad = b.direct ? (int32_t)tn[i] : b.skinny ? 1 : 0;
+            task.pad = b.direct ? dpack[i] : b.skinny ? 1 : 0;
             task.lo = 0;
             const int64_t lds_cap = b.direct ? lds_cap_direct : lds_cap_gemm;
             const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
