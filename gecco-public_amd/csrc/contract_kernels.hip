@@ -321,7 +321,7 @@ __device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff
 // flight during the current step's MFMAs; no LDS traffic and no barrier per
 // step.  The four partial tiles are summed in LDS (1024 doubles) at the end.
 // A step covers 16 * SKINNY_U consecutive k: 4 * SKINNY_U per wave.
-constexpr int SKINNY_U = 8;
+constexpr int SKINNY_U = 4;
 
 __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock *__restrict__ blocks,
                                             double *const *__restrict__ bufs, double *lds, int tid) {
@@ -395,17 +395,20 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
     }
 }
 
+// 128 x 128 tiles hold 128 accumulator registers per lane: the launch bound
+// asks for 2 waves per SIMD (at most 256 registers per lane in total).
 template <int TBM, int TBN, int TBK>
-__global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
+__global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine_f64(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
     double *const *__restrict__ bufs) {
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int TM = TBM / 32, TN = TBN / 32;
     constexpr int SA = TBK * LDA, SB = TBK * LDB;
-    // double-buffered A and B tiles; the whole array stages the result
-    // range of a task with span > 0 after the K loop
-    __shared__ double lds[2 * SA + 2 * SB];
-    static_assert(2 * SA + 2 * SB >= 1024, "LDS too small for the skinny-task reduction");
+    // double-buffered A and B tiles; after the K loop the array holds half
+    // of the result tile (epilogue), or the partial tiles of a skinny task
+    constexpr int LDS_N = 2 * SA + 2 * SB > (TBM / 2) * (TBN + 1) ? 2 * SA + 2 * SB : (TBM / 2) * (TBN + 1);
+    __shared__ double lds[LDS_N];
+    static_assert(LDS_N >= 1024, "LDS too small for the skinny-task reduction");
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -509,91 +512,40 @@ __global__ __launch_bounds__(NT) void grouped_contract_affine_f64(
         double *pc = bufs[b.buf_c] + b.base_c;
         const double xfac = b.xfac;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
-        // element offset (-1: padding or forbidden) and signed value of
-        // result element (i, j, r) of this lane
-        auto elem = [&](int i, int j, int r, int64_t &a, double &v) {
-            const int n = n0 + wn * (TBN / 2) + j * 16 + lcol;
-            const int m = m0 + wm * (TBM / 2) + i * 16 + r * 4 + lrow;
-            a = -1;
-            bool neg = false;
-            if (m < M && n < N) a = result_offset(oc, caff, m, n, neg);
-            const double x = xfac * acc[i][j][r];
-            v = neg ? -x : x;
-        };
-        if (task.span > 0) {
-            // LDS-staged result range: merges the contributions of this
-            // tile to the same element (e.g. the two orderings of an
-            // antisymmetrised string pair), then one contiguous write-back.
-            // The K loop ended with a barrier, so the tiles are dead.
-            const int span = task.span;
-            for (int i = tid; i < span; i += NT) lds[i] = 0.0;
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
+        // The tile goes through LDS as a row-major [row][col] array (row
+        // stride TBN + 1 against bank conflicts), one half of the rows --
+        // those of the waves with wm = h -- at a time; then each thread adds
+        // elements n-fastest, so the result-map reads are coalesced.  Every
+        // element is a non-returning atomic add, also when the task owns its
+        // result elements (injective map, one K chunk, no overlapping
+        // block): with a single writer the L2 performs the same IEEE
+        // addition as a read-modify-write, and the wave does not wait for
+        // old values.  Colliding elements of a non-injective map (the two
+        // orderings of an antisymmetrised string pair) add atomically.
+        constexpr int HR = TBM / 2, LDE = TBN + 1;
+        for (int h = 0; h < 2; ++h) {
+            if (h) __syncthreads();   // the previous half has been read
+            if (wm == h) {
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        int64_t a;
-                        double v;
-                        elem(i, j, r, a, v);
-                        if (a >= 0) atomicAdd(lds + (a - task.lo), v);
-                    }
-            __syncthreads();
-            double *pr = pc + task.lo;
-            if (task.owned) {
-                for (int i0 = 0; i0 < span; i0 += 4 * NT) {
-                    double o[4];
+                    for (int j = 0; j < TN; ++j)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * NT + tid;
-                        o[u] = i < span ? pr[i] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * NT + tid;
-                        if (i < span) pr[i] = o[u] + lds[i];
-                    }
-                }
-            } else {
-                for (int i = tid; i < span; i += NT) {
-                    const double v = lds[i];
-                    if (v != 0.0) unsafeAtomicAdd(pr + i, v);
-                }
+                        for (int r = 0; r < 4; ++r)
+                            lds[(i * 16 + r * 4 + lrow) * LDE + wn * (TBN / 2) + j * 16 + lcol] =
+                                acc[i][j][r];
             }
-        } else if (b.atomic) {
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        int64_t a;
-                        double v;
-                        elem(i, j, r, a, v);
-                        if (a >= 0) unsafeAtomicAdd(pc + a, v);
-                    }
-        } else {
-            // this task owns its result elements (injective map, one K
-            // chunk, no overlapping block): per column sub-tile, all loads
-            // are issued before the stores, so the read-modify-writes
-            // overlap in flight while the live registers stay bounded
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                int64_t addr[TM][4];
-                double val[TM][4], old[TM][4];
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        elem(i, j, r, addr[i][r], val[i][r]);
-                        old[i][r] = addr[i][r] >= 0 ? pc[addr[i][r]] : 0.0;
-                    }
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (addr[i][r] >= 0) pc[addr[i][r]] = old[i][r] + val[i][r];
+            __syncthreads();   // (h = 0: the K loop ended with a barrier)
+            const int mh = m0 + h * HR;
+            for (int e = tid; e < HR * TBN; e += NT) {
+                const int row = e / TBN, col = e - row * TBN;
+                const int m = mh + row, n = n0 + col;
+                if (m >= M || n >= N) continue;
+                bool neg;
+                const int64_t a = result_offset(oc, caff, m, n, neg);
+                if (a < 0) continue;
+                const double x = xfac * lds[row * LDE + col];
+                unsafeAtomicAdd(pc + a, neg ? -x : x);
             }
         }
     }

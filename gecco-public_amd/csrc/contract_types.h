@@ -46,8 +46,8 @@ struct DevTask {
     // of block blk, continuing along its chain (strided-GEMM kernel) when k1
     // exceeds blk's K
     int32_t blk, m0, n0, k0, k1;
-    // span > 0 (strided-GEMM kernel only): the tile's results are first
-    // reduced in LDS over the block-relative element range [lo, lo + span)
+    // span > 0 (direct short-contraction kernel only): the tile's results are
+    // first reduced in LDS over the block-relative element range [lo, lo + span)
     // and then written back as one contiguous range -- read-modify-write if
     // the task owns the range (owned = 1), else atomically
     int32_t span, owned, pad;

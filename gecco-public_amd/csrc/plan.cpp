@@ -863,12 +863,11 @@ static int plan_finalize(gecco_hip_plan *p) {
                                      (int32_t)(c1 - start)});
                 }
     }
-    // Result range of every task.  Tasks of the strided-GEMM kernel whose
-    // result map is a table or not injective stage their tile in LDS when
-    // the range fits (lds_cap doubles) and is at least 1/4 dense; a task
-    // owns its range when no other task's range intersects it.
+    // Result range of every task.  Tasks of the direct kernel whose result
+    // map is a table or not injective stage their tile in LDS when the range
+    // fits (lds_cap doubles) and is at least 1/4 dense; a task owns its range
+    // when no other task's range intersects it.
     {
-        const int64_t lds_cap_gemm = 2 * (int64_t)abk * (abm + 1) + 2 * (int64_t)abk * (abn + 1);
         const int64_t lds_cap_direct = gecco_direct_stage_cap();
         struct Iv { int64_t lo, hi; size_t t; };
         std::vector<Iv> ivs;
@@ -883,7 +882,6 @@ static int plan_finalize(gecco_hip_plan *p) {
             // 16 x 16 wave-split-K task
             task.pad = b.direct ? dpack[i] : b.skinny ? 1 : 0;
             task.lo = 0;
-            const int64_t lds_cap = b.direct ? lds_cap_direct : lds_cap_gemm;
             const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
             const int64_t n1 = std::min<int64_t>(b.N, task.n0 + tile_n(i));
             int64_t amin, amax;
@@ -892,7 +890,7 @@ static int plan_finalize(gecco_hip_plan *p) {
             const int64_t key = (int64_t)b.buf12 << 44;
             ivs.push_back({key + b.off12 + amin, key + b.off12 + amax, t});
             const int64_t span = amax - amin + 1;
-            if (b.strided && !b.skinny && (b.C.r.table || !b.injective) && span <= lds_cap &&
+            if (b.direct && (b.C.r.table || !b.injective) && span <= lds_cap_direct &&
                 4 * cnt >= span) {
                 task.span = (int32_t)span;
                 task.lo = amin;
