@@ -18,6 +18,21 @@ namespace gecco {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// Global-memory loads, stores and atomics through global-address-space
+// pointers.  Operand, result and index pointers are read from device tables,
+// so the compiler would otherwise emit flat instructions, which also count
+// against the LDS wait counter: every wait for an LDS read would then wait
+// for all outstanding prefetch loads and atomics as well.
+typedef const double __attribute__((address_space(1))) *gcdptr;
+typedef double __attribute__((address_space(1))) *gdptr;
+typedef const int32_t __attribute__((address_space(1))) *gciptr;
+__device__ __forceinline__ double ldg(const double *p) { return *(gcdptr)p; }
+__device__ __forceinline__ int32_t ldg(const int32_t *p) { return *(gciptr)p; }
+__device__ __forceinline__ void stg(double *p, double v) { *(gdptr)p = v; }
+__device__ __forceinline__ void atomic_add_g(double *p, double v) {
+    (void)__builtin_amdgcn_global_atomic_fadd_f64((gdptr)p, v);
+}
+
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
 constexpr int LPT = BK * BM / NT;   // elements of each operand tile per thread
 
@@ -61,7 +76,7 @@ __device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const do
             if (o.second_affine) {
                 v.cadd[r] = o.s0 + o.s1 * (int64_t)x;
             } else {
-                const int32_t c = o.second[x];
+                const int32_t c = ldg(o.second + x);
                 v.cval[r] = c != 0;
                 v.cneg[r] = (uint32_t)c >> 31;
                 v.cadd[r] = (int64_t)(c < 0 ? -c : c) - 1;
@@ -83,7 +98,7 @@ __device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double 
         int64_t addr = 0;
         uint32_t neg = 0;
         if (v.table) {
-            const int32_t rv = ok ? v.first[(int64_t)k + v.ld * (int64_t)v.xi[r]] : 0;
+            const int32_t rv = ok ? ldg(v.first + ((int64_t)k + v.ld * (int64_t)v.xi[r])) : 0;
             ok = ok && rv != 0;
             neg = (uint32_t)rv >> 31;
             addr = (int64_t)(rv < 0 ? -rv : rv) - 1;
@@ -91,12 +106,12 @@ __device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double 
             addr = v.f0 + v.f1 * (int64_t)k + v.cadd[r];
             neg = v.cneg[r];
         } else {
-            const int32_t rv = ok ? v.first[k] : 0;
+            const int32_t rv = ok ? ldg(v.first + k) : 0;
             ok = ok && rv != 0;
             neg = ((uint32_t)rv >> 31) ^ v.cneg[r];
             addr = (int64_t)(rv < 0 ? -rv : rv) - 1 + v.cadd[r];
         }
-        raw[r] = v.base[ok ? addr : 0];
+        raw[r] = ldg(v.base + (ok ? addr : 0));
         flags |= ((uint32_t)ok << r) | (neg << (r + LPT));
     }
 }
@@ -176,7 +191,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
             const int n = n0 + wn * 32 + ni * 16 + lcol;
             const int32_t cc = (n < N && !oc.table)
                                    ? (oc.second_affine ? (int32_t)(1 + oc.s0 + oc.s1 * (int64_t)n)
-                                                       : oc.second[n])
+                                                       : ldg(oc.second + n))
                                    : 1;
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
@@ -187,13 +202,13 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                     bool neg = false;
                     if (m < M && n < N) {
                         if (oc.table) {
-                            const int32_t v = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+                            const int32_t v = ldg(oc.first + ((int64_t)m + oc.ld * (int64_t)n));
                             neg = v < 0;
                             ad = (int64_t)(v < 0 ? -v : v) - 1;
                         } else {
                             const int32_t rr = oc.first_affine
                                                    ? (int32_t)(1 + oc.f0 + oc.f1 * (int64_t)m)
-                                                   : oc.first[m];
+                                                   : ldg(oc.first + m);
                             neg = (rr < 0) != (cc < 0);
                             ad = (rr == 0 || cc == 0)
                                      ? -1
@@ -212,7 +227,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                 for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (addr[mi][ni][r] >= 0) unsafeAtomicAdd(pc + addr[mi][ni][r], val[mi][ni][r]);
+                        if (addr[mi][ni][r] >= 0) atomic_add_g(pc + addr[mi][ni][r], val[mi][ni][r]);
         } else {
             // the task owns its result elements: loads first, then stores
             double old[2][2][4];
@@ -222,14 +237,14 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                 for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        old[mi][ni][r] = addr[mi][ni][r] >= 0 ? pc[addr[mi][ni][r]] : 0.0;
+                        old[mi][ni][r] = addr[mi][ni][r] >= 0 ? ldg(pc + addr[mi][ni][r]) : 0.0;
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
                 for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (addr[mi][ni][r] >= 0) pc[addr[mi][ni][r]] = old[mi][ni][r] + val[mi][ni][r];
+                        if (addr[mi][ni][r] >= 0) stg(pc + addr[mi][ni][r], old[mi][ni][r] + val[mi][ni][r]);
         }
     }
 }
@@ -241,39 +256,68 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 // (TBM/2) x (TBN/2) = TM x TN MFMA 16x16 tiles.  LDS is double-buffered: one
 // barrier per K step; the global loads of step s+1 are in flight while the
 // MFMAs of step s run.
+//
+// Operand loads are raw buffer loads: a descriptor (scalar registers) spans
+// exactly the operand block, each element has a 32-bit byte offset that
+// advances by a constant per K step, and the hardware returns 0 for any
+// offset at or beyond the block's extent.  An element outside the block's
+// rows/columns starts at offset 0x80000000; the planner keeps operand
+// extents at or below 2^30 bytes, so with all its K steps added the offset
+// stays in [2^31, 2^32) and out of range.  Elements beyond the block's K (the
+// last step of a block) use offset 0x80000000 directly.  Masking thus costs
+// no branch, and a full step one add per element.
+constexpr uint32_t BUF_OOB = 0x80000000u;
+constexpr int BUF_FLAGS = 0x00020000;   // raw buffer, gfx9-family descriptor word 3
+
+__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 template <int X, int TBK>
 struct AffView {
     static constexpr int L = TBK * X / NT;  // elements per thread
-    int64_t base;     // element offset of this thread's first element at k = 0
-    int64_t sk;       // stride along k
-    int64_t rstride;  // offset between this thread's elements
-    int kk0, kstep;   // k of element r: kk0 + kstep*r
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t off[L];  // byte offset of element r at the current K step
+    uint32_t step;    // bytes per K step
+    int kk0, kstep;   // k of element r relative to the step start: kk0 + kstep*r
     int xi0, xstep;   // m/n (relative to the tile) of element r: xi0 + xstep*r
-    uint32_t xmask;   // element r lies inside the block along m/n
 
-    __device__ __forceinline__ void init(const DevOperand &o, int x0, int Xn, int tid) {
-        sk = o.f1;
+    // operand block at blk (element (k, x) at o.f0 + o.s0 + o.f1*k + o.s1*x,
+    // k < K, x < Xn); the first step starts at k = kstart
+    __device__ __forceinline__ void init(const double *blk, const DevOperand &o, int K, int x0,
+                                         int Xn, int kstart, int tid) {
         if (o.fast_first) {
             kk0 = tid % TBK; kstep = 0;
             xi0 = tid / TBK; xstep = NT / TBK;
-            rstride = o.s1 * (NT / TBK);
         } else {
             kk0 = tid / X; kstep = NT / X;
             xi0 = tid % X; xstep = 0;
-            rstride = o.f1 * (NT / X);
         }
-        base = o.f0 + o.s0 + o.f1 * (int64_t)kk0 + o.s1 * (int64_t)(x0 + xi0);
-        xmask = 0;
-#pragma unroll
-        for (int r = 0; r < L; ++r) xmask |= (uint32_t)(x0 + xi0 + xstep * r < Xn) << r;
-    }
-    __device__ __forceinline__ void load(const double *__restrict__ p, int k0, int k1,
-                                         double out[L]) const {
-        const double *q = p + base + sk * (int64_t)k0;
+        const int64_t extent = (o.f1 * (int64_t)(K - 1) + o.s1 * (int64_t)(Xn - 1) + 1) * 8;
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)(blk + o.f0 + o.s0), (short)0, (int)extent,
+                                                 BUF_FLAGS);
+        step = (uint32_t)(o.f1 * TBK * 8);
 #pragma unroll
         for (int r = 0; r < L; ++r) {
-            const bool ok = ((xmask >> r) & 1u) && (k0 + kk0 + kstep * r < k1);
-            out[r] = ok ? q[rstride * r] : 0.0;
+            const int x = x0 + xi0 + xstep * r, k = kstart + kk0 + kstep * r;
+            off[r] = x < Xn ? (uint32_t)((o.f1 * (int64_t)k + o.s1 * (int64_t)x) * 8) : BUF_OOB;
+        }
+    }
+    // loads the step starting at block-local k0 (the range ends at k1) and
+    // advances to the next step
+    __device__ __forceinline__ void load(int k0, int k1, double out[L]) {
+        if (k0 + TBK <= k1) {
+#pragma unroll
+            for (int r = 0; r < L; ++r) {
+                out[r] = buf_load_f64(rsrc, off[r]);
+                off[r] += step;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < L; ++r) {
+                out[r] = buf_load_f64(rsrc, k0 + kk0 + kstep * r < k1 ? off[r] : BUF_OOB);
+                off[r] += step;
+            }
         }
     }
     template <int LD>
@@ -290,7 +334,7 @@ __device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff
     neg = false;
     if (caff) return oc.f0 + oc.f1 * (int64_t)m + oc.s0 + oc.s1 * (int64_t)n;
     if (oc.table) {
-        const int32_t t = oc.first[(int64_t)m + oc.ld * (int64_t)n];
+        const int32_t t = ldg(oc.first + ((int64_t)m + oc.ld * (int64_t)n));
         neg = t < 0;
         return (int64_t)(t < 0 ? -t : t) - 1;
     }
@@ -299,13 +343,13 @@ __device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff
     if (oc.first_affine) {
         ar = oc.f0 + oc.f1 * (int64_t)m;
     } else {
-        const int32_t t = oc.first[m];
+        const int32_t t = ldg(oc.first + m);
         ok = t != 0; nr = t < 0; ar = (int64_t)(t < 0 ? -t : t) - 1;
     }
     if (oc.second_affine) {
         ac = oc.s0 + oc.s1 * (int64_t)n;
     } else {
-        const int32_t t = oc.second[n];
+        const int32_t t = ldg(oc.second + n);
         ok = ok && t != 0; nc = t < 0; ac = (int64_t)(t < 0 ? -t : t) - 1;
     }
     neg = nr != nc;
@@ -350,8 +394,8 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
             for (int u = 0; u < SKINNY_U; ++u) {
                 const int k = k0 + koff + 4 * u;
                 const bool ok = k < kh;
-                ra[u] = (ok && mok) ? qa[fa * (int64_t)k] : 0.0;
-                rb[u] = (ok && nok) ? qb[fb * (int64_t)k] : 0.0;
+                ra[u] = (ok && mok) ? ldg(qa + fa * (int64_t)k) : 0.0;
+                rb[u] = (ok && nok) ? ldg(qb + fb * (int64_t)k) : 0.0;
             }
         };
         load(kl, a, bv);
@@ -389,8 +433,8 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
             double *pc = bufs[b.buf_c] + b.base_c;
             const double x = b.xfac * sum;
             const double v = neg ? -x : x;
-            if (b.atomic) unsafeAtomicAdd(pc + a, v);
-            else pc[a] += v;
+            if (b.atomic) atomic_add_g(pc + a, v);
+            else stg(pc + a, ldg(pc + a) + v);
         }
     }
 }
@@ -431,30 +475,26 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         // start; [kl, kh): block-local range of the current step's block.
         int blk = task.blk, kb = 0, kcur = b.K;
         int kl = task.k0, kh = min(kcur, task.k1);
-        const double *__restrict__ pa = bufs[b.buf_a] + b.base_a;
-        const double *__restrict__ pb = bufs[b.buf_b] + b.base_b;
         AffView<TBM, TBK> va;
         AffView<TBN, TBK> vb;
-        va.init(b.A, m0, M, tid);
-        vb.init(b.B, n0, N, tid);
+        va.init(bufs[b.buf_a] + b.base_a, b.A, kcur, m0, M, kl, tid);
+        vb.init(bufs[b.buf_b] + b.base_b, b.B, kcur, n0, N, kl, tid);
 
         v4d acc[TM][TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-        // 16-row / 16-column MFMA sub-tiles of this wave that overlap the
-        // block (wave-uniform); the others are padding and are skipped
-        uint32_t live_m = 0, live_n = 0;
-#pragma unroll
-        for (int i = 0; i < TM; ++i) live_m |= (uint32_t)(m0 + wm * (TBM / 2) + i * 16 < M) << i;
-#pragma unroll
-        for (int j = 0; j < TN; ++j) live_n |= (uint32_t)(n0 + wn * (TBN / 2) + j * 16 < N) << j;
-
+        // a wave whose 16-row or 16-column MFMA sub-tiles all lie outside
+        // the block skips its MFMAs (scalar branch); otherwise every
+        // sub-tile is computed, padding rows and columns from the zeros that
+        // the masked loads put into LDS
+        const bool wave_live = __builtin_amdgcn_readfirstlane(
+            (m0 + wm * (TBM / 2) < M && n0 + wn * (TBN / 2) < N) ? 1 : 0);
         double ra[AffView<TBM, TBK>::L], rb[AffView<TBN, TBK>::L];
         __syncthreads();   // LDS of the previous task is no longer read
-        va.load(pa, kl, kh, ra);
-        vb.load(pb, kl, kh, rb);
+        va.load(kl, kh, ra);
+        vb.load(kl, kh, rb);
         va.template store<LDA>(lds, ra);
         vb.template store<LDB>(lds + 2 * SA, rb);
         __syncthreads();
@@ -472,32 +512,31 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     kcur = nb.K;
                     kl = 0;
                     kh = min(kcur, task.k1 - kb);
-                    pa = bufs[nb.buf_a] + nb.base_a;
-                    pb = bufs[nb.buf_b] + nb.base_b;
-                    va.init(nb.A, m0, M, tid);
-                    vb.init(nb.B, n0, N, tid);
+                    va.init(bufs[nb.buf_a] + nb.base_a, nb.A, kcur, m0, M, 0, tid);
+                    vb.init(bufs[nb.buf_b] + nb.base_b, nb.B, kcur, n0, N, 0, tid);
                 }
             }
             if (more) {
-                va.load(pa, kl, kh, ra);
-                vb.load(pb, kl, kh, rb);
+                va.load(kl, kh, ra);
+                vb.load(kl, kh, rb);
             }
             const double *A = lds + cur * SA;
             const double *B = lds + 2 * SA + cur * SB;
+            if (wave_live) {
 #pragma unroll
-            for (int kq = 0; kq < TBK / 4; ++kq) {
-                const int kr = kq * 4 + lrow;
-                double a[TM], bb[TN];
+                for (int kq = 0; kq < TBK / 4; ++kq) {
+                    const int kr = kq * 4 + lrow;
+                    double a[TM], bb[TN];
 #pragma unroll
-                for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
+                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
 #pragma unroll
-                for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
+                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
 #pragma unroll
-                for (int i = 0; i < TM; ++i)
+                    for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        if (((live_m >> i) & (live_n >> j)) & 1u)
+                        for (int j = 0; j < TN; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+                }
             }
             if (more) {
                 va.template store<LDA>(lds + (cur ^ 1) * SA, ra);
@@ -545,7 +584,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                 const int64_t a = result_offset(oc, caff, m, n, neg);
                 if (a < 0) continue;
                 const double x = xfac * lds[row * LDE + col];
-                unsafeAtomicAdd(pc + a, neg ? -x : x);
+                atomic_add_g(pc + a, neg ? -x : x);
             }
         }
     }
@@ -589,14 +628,14 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
         for (int e = tid; e < K * dm; e += NT) {
             const int k = e / dm, r = e - k * dm;
             const int m = m0 + r;
-            As[e] = m < M ? pa[oa.f1 * (int64_t)k + oa.s1 * (int64_t)m] : 0.0;
+            As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : 0.0;
         }
         const int span = task.span;
         for (int i = tid; i < span; i += NT) Cs[i] = 0.0;
         double bk[KMAX];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
-            bk[k] = (k < K && n < N) ? pb[ob.f1 * (int64_t)k + ob.s1 * (int64_t)n] : 0.0;
+            bk[k] = (k < K && n < N) ? ldg(pb + (ob.f1 * (int64_t)k + ob.s1 * (int64_t)n)) : 0.0;
         __syncthreads();
         // result column data (separable / affine maps)
         int64_t cadd = 0;
@@ -605,7 +644,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
             if (oc.second_affine) {
                 cadd = oc.s0 + oc.s1 * (int64_t)n;
             } else {
-                const int32_t v = oc.second[n];
+                const int32_t v = ldg(oc.second + n);
                 cok = v != 0;
                 cneg = v < 0;
                 cadd = (int64_t)(v < 0 ? -v : v) - 1;
@@ -628,13 +667,13 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
             bool neg = cneg;
             if (live && m < M) {
                 if (oc.table) {
-                    const int32_t v = oc.first[(int64_t)n + oc.ld * (int64_t)m];
+                    const int32_t v = ldg(oc.first + ((int64_t)n + oc.ld * (int64_t)m));
                     neg = v < 0;
                     a = (int64_t)(v < 0 ? -v : v) - 1;
                 } else if (oc.first_affine) {
                     a = oc.f0 + oc.f1 * (int64_t)m + cadd;
                 } else {
-                    const int32_t v = oc.first[m];
+                    const int32_t v = ldg(oc.first + m);
                     neg = neg != (v < 0);
                     a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
                 }
@@ -655,31 +694,31 @@ __global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int i = i0 + u * NT + tid;
-                        o[u] = i < span ? pr[i] : 0.0;
+                        o[u] = i < span ? ldg(pr + i) : 0.0;
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int i = i0 + u * NT + tid;
-                        if (i < span) pr[i] = o[u] + Cs[i];
+                        if (i < span) stg(pr + i, o[u] + Cs[i]);
                     }
                 }
             } else {
                 for (int i = tid; i < span; i += NT) {
                     const double v = Cs[i];
-                    if (v != 0.0) unsafeAtomicAdd(pr + i, v);
+                    if (v != 0.0) atomic_add_g(pr + i, v);
                 }
             }
         } else if (b.atomic) {
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (addr[i] >= 0) unsafeAtomicAdd(pc + addr[i], val[i]);
+                if (addr[i] >= 0) atomic_add_g(pc + addr[i], val[i]);
         } else {
             double old[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) old[i] = addr[i] >= 0 ? pc[addr[i]] : 0.0;
+            for (int i = 0; i < 8; ++i) old[i] = addr[i] >= 0 ? ldg(pc + addr[i]) : 0.0;
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                if (addr[i] >= 0) pc[addr[i]] = old[i] + val[i];
+                if (addr[i] >= 0) stg(pc + addr[i], old[i] + val[i]);
         }
     }
 }

@@ -675,13 +675,17 @@ static int plan_finalize(gecco_hip_plan *p) {
     std::vector<uint8_t> is_aff(p->blocks.size(), 0);
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
-        int64_t a0, a1;
-        auto strided = [&](const HostOperand &o) {
-            return !o.r.table && affine_of(o.r.first, a0, a1) && affine_of(o.r.second, a0, a1);
+        // the strided-GEMM kernel addresses an operand block with unsigned
+        // 32-bit byte offsets: non-negative strides, extents up to 2^30 bytes
+        const int64_t max_extent = (int64_t)1 << 30;
+        auto strided = [&](const HostOperand &o, int64_t X) {
+            int64_t f0, f1, s0, s1;
+            if (o.r.table || !affine_of(o.r.first, f0, f1) || !affine_of(o.r.second, s0, s1)) return false;
+            return f1 >= 0 && s1 >= 0 && (f1 * (b.K - 1) + s1 * (X - 1) + 1) * 8 <= max_extent;
         };
-        const bool sa = strided(b.A), sb = strided(b.B);
+        const bool sa = strided(b.A, b.M), sb = strided(b.B, b.N);
         const bool big = p->materialize_min > 0 && std::min(b.M, b.N) >= p->materialize_min;
-        const bool fa = b.K * b.M < ((int64_t)1 << 31), fb = b.K * b.N < ((int64_t)1 << 31);
+        const bool fa = b.K * b.M * 8 <= max_extent, fb = b.K * b.N * 8 <= max_extent;
         const bool ca = sa || (big && fa), cb = sb || (big && fb);
         b.strided = ca && cb;
         b.mat_a = b.strided && !sa;
