@@ -485,7 +485,7 @@ struct gecco_hip_plan {
     int affine_variant = 1;
     int64_t target_tasks = 2048;
     int64_t min_kchunk = 256;
-    int64_t max_kchunk = 4096;
+    int64_t max_kchunk = 1024;      // longest contraction range of one strided-GEMM task
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
