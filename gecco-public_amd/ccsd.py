@@ -7,7 +7,11 @@ intermediates of Stanton, Gauss, Watts and Bartlett (J. Chem. Phys. 94, 4334
                          + 1/8 tau_mn^ab tau_ij^ef <mn||ef>,
   Zamij = 1/2 tau_ij^ef <am||ef>,
 and the last term is folded into W_mnij, whose tau.<mn||ef> term then
-carries 1/2 instead of 1/4.  The equations are
+carries 1/2 instead of 1/4.  The doubles term -P(ij)P(ab) t_i^e t_m^a
+<mb||ej> contracts the virtual index first, t_i^e <mb||ej> (an o^3 v
+intermediate instead of the o v^3 one of contracting m first); it has
+Zamij's index structure and reaches R2 through the same t_m Zamij
+contraction, so it accumulates into Zamij.  The equations are
 written as a list of GeCCo binary contractions over ME-lists in
 GeCCo's Ms/IRREP-blocked layout -- the role the factorised formula file
 plays for GeCCo's scheduler (contract/frm_sched2.f:117-365): every item is a
@@ -43,7 +47,7 @@ LISTS = {
     'Fme': ({H: 1}, {P: 1}),
     'Wmnij': ({H: 2}, {H: 2}), 'Zamij': ({P: 1, H: 1}, {H: 2}),
     'Wmbej': ({P: 1, H: 1}, {P: 1, H: 1}),
-    'Z': ({H: 2}, {P: 1, H: 1}), 'Y': ({P: 2}, {P: 1, H: 1}),
+    'Z': ({H: 2}, {P: 1, H: 1}),
     'E': ({}, {}),
     'Hoovv': ({H: 2}, {P: 2}), 'Hvvoo': ({P: 2}, {H: 2}),
     'Hoooo': ({H: 2}, {H: 2}), 'Hvvvv': ({P: 2}, {P: 2}),
@@ -71,7 +75,7 @@ LEVELS = [
      ('Wmbej', 'Hovvv', 'T1', {}, {P: 1}, 1.0),
      ('Wmbej', 'T1', 'Hooov', {}, {H: 1}, 1.0),
      ('Wmbej', 'T2', 'Hoovv', {P: 1}, {H: 1}, 0.5),
-     ('Y', 'T1', 'Hovvo', {}, {H: 1}, -1.0),
+     ('Zamij', 'Hovvo', 'T1', {}, {P: 1}, 1.0),
      ('R1', 'T1', 'Hovvo', {P: 1}, {H: 1}, -1.0),
      ('R1', 'Hovvv', 'T2', {H: 1}, {P: 2}, -1.0),
      ('R1', 'T2', 'Hooov', {P: 1}, {H: 2}, -1.0),
@@ -88,7 +92,6 @@ LEVELS = [
      ('Zamij', 'Hovvv', 'tau', {}, {P: 2}, 1.0),
      ('Wmbej', 'T1', 'Z', {}, {H: 1}, 1.0),
      ('R1', 'T2', 'Fme', {P: 1}, {H: 1}, 1.0),
-     ('R2', 'Y', 'T1', {}, {P: 1}, -1.0),
      ('E', 'Hoovv', 'tau', {H: 2}, {P: 2}, 1.0)],
     [('R1', 'Fae', 'T1', {}, {P: 1}, 1.0),
      ('R1', 'T1', 'Fmi', {}, {H: 1}, -1.0),
