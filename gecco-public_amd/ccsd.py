@@ -103,6 +103,32 @@ LEVELS = [
 ]
 
 
+
+def sym_schedule(levels):
+    """Lists to spin-flip symmetrise at the end of each level.  The
+    symmetrisation (contract/sym_ab_blk.f) is a linear projector and the
+    initial contents of every list (integrals, zeros) are fixed points of
+    it, so contributions of several levels can be projected together: a list
+    is projected at the end of level L when it is written there and read in
+    a later level no later than its next write, or never written again (the
+    residuals).  Items of one level read only lists of earlier levels."""
+    writes = [{t[0] for t in lv} for lv in levels]
+    reads = [{t[1] for t in lv} | {t[2] for t in lv} for lv in levels]
+    out = []
+    for li, lv in enumerate(levels):
+        todo = []
+        for res in dict.fromkeys(t[0] for t in lv):
+            nxt = next((l for l in range(li + 1, len(levels)) if res in writes[l]), None)
+            last = len(levels) - 1 if nxt is None else nxt
+            if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
+                todo.append(res)
+        out.append(todo)
+    return out
+
+
+SYM_AFTER = sym_schedule(LEVELS)
+
+
 class CCSD:
     """Device-resident closed-shell CCSD driver.
 
@@ -152,16 +178,13 @@ class CCSD:
                         p.add_init(self.slot[n], self.L[n].length, buf_src=self.slot[src], factor=f)
                     elif n not in INTEGRALS and n not in ('T1', 'T2'):
                         p.add_init(self.slot[n], self.L[n].length)
-            written = []
             for res, a, b, cc, ca, fact in level:
                 nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], cc, ca, fact, use_tr,
                                            buf=(self.slot[a], self.slot[b], self.slot[res]))
                 self.flops += fl
                 self.nblocks += nb
-                if res not in written:
-                    written.append(res)
             if use_tr:
-                for res in written:
+                for res in SYM_AFTER[li]:
                     p.add_sym_ab(self.L[res], self.slot[res])
             p.finalize()
             self.plans.append(p)

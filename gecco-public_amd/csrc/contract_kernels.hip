@@ -734,40 +734,50 @@ __global__ __launch_bounds__(256) void gather_operands_f64(
     double *__restrict__ dst = scratch + g.dst_base;
     const DevOperand &o = g.src;
     const uint32_t inner = g.kfast ? (uint32_t)g.K : (uint32_t)g.X;
-    for (uint32_t e = (uint32_t)t.e0 + threadIdx.x; e < (uint32_t)t.e1; e += blockDim.x) {
+    // signed source offset of element e (-1: forbidden); U elements per
+    // thread and round, so their index and data loads are in flight together
+    auto source = [&](uint32_t e, bool &neg) -> int64_t {
         const uint32_t q = e / inner, rmd = e - q * inner;
         const uint32_t k = g.kfast ? rmd : q, x = g.kfast ? q : rmd;
-        int64_t addr;
-        bool neg, ok = true;
         if (o.table) {
-            const int32_t v = o.first[(int64_t)k + o.ld * (int64_t)x];
-            ok = v != 0;
+            const int32_t v = ldg(o.first + ((int64_t)k + o.ld * (int64_t)x));
             neg = v < 0;
-            addr = (int64_t)(v < 0 ? -v : v) - 1;
+            return v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1;
+        }
+        int64_t a1, a2;
+        bool n1 = false, n2 = false, ok = true;
+        if (o.first_affine) {
+            a1 = o.f0 + o.f1 * (int64_t)k;
         } else {
-            int64_t a1, a2;
-            bool n1 = false, n2 = false;
-            if (o.first_affine) {
-                a1 = o.f0 + o.f1 * (int64_t)k;
-            } else {
-                const int32_t v = o.first[k];
-                ok = v != 0; n1 = v < 0; a1 = (int64_t)(v < 0 ? -v : v) - 1;
-            }
-            if (o.second_affine) {
-                a2 = o.s0 + o.s1 * (int64_t)x;
-            } else {
-                const int32_t v = o.second[x];
-                ok = ok && v != 0; n2 = v < 0; a2 = (int64_t)(v < 0 ? -v : v) - 1;
-            }
-            neg = n1 != n2;
-            addr = a1 + a2;
+            const int32_t v = ldg(o.first + k);
+            ok = v != 0; n1 = v < 0; a1 = (int64_t)(v < 0 ? -v : v) - 1;
         }
-        double val = 0.0;
-        if (ok) {
-            const double xv = src[addr];
-            val = neg ? -xv : xv;
+        if (o.second_affine) {
+            a2 = o.s0 + o.s1 * (int64_t)x;
+        } else {
+            const int32_t v = ldg(o.second + x);
+            ok = ok && v != 0; n2 = v < 0; a2 = (int64_t)(v < 0 ? -v : v) - 1;
         }
-        dst[e] = val;
+        neg = n1 != n2;
+        return ok ? a1 + a2 : -1;
+    };
+    constexpr int U = 4;
+    for (uint32_t e0 = (uint32_t)t.e0 + threadIdx.x; e0 < (uint32_t)t.e1; e0 += U * blockDim.x) {
+        int64_t addr[U];
+        bool neg[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = e0 + u * blockDim.x;
+            addr[u] = e < (uint32_t)t.e1 ? source(e, neg[u]) : -1;
+        }
+        double val[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) val[u] = addr[u] >= 0 ? ldg(src + addr[u]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = e0 + u * blockDim.x;
+            if (e < (uint32_t)t.e1) stg(dst + e, neg[u] ? -val[u] : val[u]);
+        }
     }
 }
 
@@ -821,11 +831,11 @@ __global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__re
     double *__restrict__ x = bufs[st.buf] + st.base;
     const double prefac = st.prefac;
     for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += blockDim.x) {
-        const int64_t a = i1[p], c = i2[p];
+        const int64_t a = ldg(i1 + p), c = ldg(i2 + p);
         const double f = (double)fsign[p];
-        const double v = prefac * (x[a] + f * x[c]);
-        x[a] = v;
-        if (c != a) x[c] = f * v;
+        const double v = prefac * (ldg(x + a) + f * ldg(x + c));
+        stg(x + a, v);
+        if (c != a) stg(x + c, f * v);
     }
 }
 
