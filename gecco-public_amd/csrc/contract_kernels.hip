@@ -439,6 +439,123 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
     }
 }
 
+// Tasks of the strided-GEMM launch with a short contraction (K <= 32) and
+// strided A and B, e.g. the rank-1 to rank-25 updates T1 x integral of
+// CCSD: MFMA tiles would idle on padding and on per-task latency, so each
+// thread computes whole dot products on the vector ALU.  They share the
+// launch with the GEMM tiles, whose tail they fill.  A task covers dm rows x dn columns, task.pad = dn | ng << 9 |
+// rpt << 18: the first ng * dn <= NT threads form ng row groups of dn
+// columns (the others only help to load), dm = rpt * ng with
+// K * dm <= DIRECT_A.  Thread
+// (nl, ml) owns column n0 + nl and rows m0 + ml + ng * i, i < rpt.  The A
+// tile (K x dm) is staged in LDS, the thread's B column in registers.  The
+// result map is read n-fastest (table = 2 stores the table transposed).
+constexpr int DIRECT_A = 32 * 32;        // LDS doubles for the A tile
+constexpr int DIRECT_STAGE = 4096;       // LDS doubles for a staged result range
+
+__device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock *__restrict__ blocks,
+                                            double *const *__restrict__ bufs, double *lds, int tid) {
+    double *const As = lds;
+    double *const Cs = lds + DIRECT_A;
+    const DevBlock &b = blocks[task.blk];
+    const int M = b.M, N = b.N, K = b.K;
+    const int dn = task.pad & 511, ng = (task.pad >> 9) & 511, rpt = task.pad >> 18;
+    const int dm = rpt * ng;
+    const int nl = tid % dn, ml = tid / dn;
+    const bool active = ml < ng;
+    const int m0 = task.m0, n = task.n0 + nl;
+    const DevOperand &oa = b.A, &ob = b.B, &oc = b.C;
+    const double *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
+    const double *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
+    double *pc = bufs[b.buf_c] + b.base_c;
+    __syncthreads();   // LDS of the previous task is no longer read
+    for (int e = tid; e < K * dm; e += NT) {
+        const int k = e / dm, r = e - k * dm;
+        const int m = m0 + r;
+        As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : 0.0;
+    }
+    const int span = task.span;
+    for (int i = tid; i < span; i += NT) Cs[i] = 0.0;
+    __syncthreads();
+    // result column data (separable / affine maps)
+    int64_t cadd = 0;
+    bool cneg = false, cok = active && n < N;
+    if (cok && oc.table == 0) {
+        if (oc.second_affine) {
+            cadd = oc.s0 + oc.s1 * (int64_t)n;
+        } else {
+            const int32_t v = ldg(oc.second + n);
+            cok = v != 0;
+            cneg = v < 0;
+            cadd = (int64_t)(v < 0 ? -v : v) - 1;
+        }
+    }
+    // K outer, rpt rows inner: one B value live at a time
+    double acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+    if (cok) {
+        const double *__restrict__ pbn = pb + ob.s1 * (int64_t)n;
+#pragma unroll 4
+        for (int k = 0; k < K; ++k) {
+            const double bk = ldg(pbn + ob.f1 * (int64_t)k);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i < rpt) acc[i] = fma(As[k * dm + ml + ng * i], bk, acc[i]);
+        }
+    }
+    // results: into the LDS stage (span > 0), else non-returning atomic adds
+    // (with a single writer the L2 performs the same IEEE addition as a
+    // read-modify-write)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int m = m0 + ml + ng * i;
+        if (!(i < rpt && cok && m < M)) continue;
+        int64_t a;
+        bool neg = cneg;
+        if (oc.table) {
+            const int32_t v = ldg(oc.first + ((int64_t)n + oc.ld * (int64_t)m));
+            neg = v < 0;
+            a = (int64_t)(v < 0 ? -v : v) - 1;
+        } else if (oc.first_affine) {
+            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
+        } else {
+            const int32_t v = ldg(oc.first + m);
+            neg = neg != (v < 0);
+            a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
+        }
+        if (a < 0) continue;
+        const double x = b.xfac * acc[i];
+        const double v = neg ? -x : x;
+        if (span > 0) atomicAdd(Cs + (a - task.lo), v);
+        else atomic_add_g(pc + a, v);
+    }
+    if (span > 0) {
+        __syncthreads();
+        double *pr = pc + task.lo;
+        if (task.owned) {
+            for (int i0 = 0; i0 < span; i0 += 4 * NT) {
+                double o[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = i0 + u * NT + tid;
+                    o[u] = i < span ? ldg(pr + i) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = i0 + u * NT + tid;
+                    if (i < span) stg(pr + i, o[u] + Cs[i]);
+                }
+            }
+        } else {
+            for (int i = tid; i < span; i += NT) {
+                const double v = Cs[i];
+                if (v != 0.0) atomic_add_g(pr + i, v);
+            }
+        }
+    }
+}
+
 // 128 x 128 tiles hold 128 accumulator registers per lane: the launch bound
 // asks for 2 waves per SIMD (at most 256 registers per lane in total).
 template <int TBM, int TBN, int TBK>
@@ -449,8 +566,10 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
     constexpr int TM = TBM / 32, TN = TBN / 32;
     constexpr int SA = TBK * LDA, SB = TBK * LDB;
     // double-buffered A and B tiles; after the K loop the array holds half
-    // of the result tile (epilogue), or the partial tiles of a skinny task
-    constexpr int LDS_N = 2 * SA + 2 * SB > (TBM / 2) * (TBN + 1) ? 2 * SA + 2 * SB : (TBM / 2) * (TBN + 1);
+    // of the result tile (epilogue), or the partial tiles of a skinny task;
+    // a direct task uses DIRECT_A + DIRECT_STAGE doubles
+    constexpr int LDS_T = 2 * SA + 2 * SB > (TBM / 2) * (TBN + 1) ? 2 * SA + 2 * SB : (TBM / 2) * (TBN + 1);
+    constexpr int LDS_N = LDS_T > DIRECT_A + DIRECT_STAGE ? LDS_T : DIRECT_A + DIRECT_STAGE;
     __shared__ double lds[LDS_N];
     static_assert(LDS_N >= 1024, "LDS too small for the skinny-task reduction");
     const int tid = threadIdx.x;
@@ -461,8 +580,12 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
-        if (task.pad) {   // tile of at most 16 x 16
+        if (task.pad == 1) {   // tile of at most 16 x 16
             skinny_task(task, blocks, bufs, lds, tid);
+            continue;
+        }
+        if (task.pad > 1) {    // short contraction on the vector ALU
+            direct_task(task, blocks, bufs, lds, tid);
             continue;
         }
         const DevBlock &b = blocks[task.blk];
@@ -586,139 +709,6 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                 const double x = xfac * lds[row * LDE + col];
                 atomic_add_g(pc + a, neg ? -x : x);
             }
-        }
-    }
-}
-
-// Short contractions (K <= KMAX) with strided A and B, e.g. the rank-1 to
-// rank-25 updates T1 x integral of CCSD: MFMA tiles would idle on padding
-// and on per-task latency, so each thread computes whole dot products on the
-// vector ALU.  A task covers dm rows x dn columns, task.pad = dn | ng << 9 |
-// rpt << 18: the first ng * dn <= NT threads form ng row groups of dn
-// columns (the others only help to load), dm = rpt * ng with
-// K * dm <= DIRECT_A.  Thread
-// (nl, ml) owns column n0 + nl and rows m0 + ml + ng * i, i < rpt.  The A
-// tile (K x dm) is staged in LDS, the thread's B column in registers.  The
-// result map is read n-fastest (table = 2 stores the table transposed).
-constexpr int DIRECT_A = 32 * 32;        // LDS doubles for the A tile
-constexpr int DIRECT_STAGE = 4096;       // LDS doubles for a staged result range
-
-template <int KMAX>
-__global__ __launch_bounds__(NT) void grouped_contract_direct_f64(
-    const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs) {
-    __shared__ double lds[DIRECT_A + DIRECT_STAGE];
-    double *const As = lds;
-    double *const Cs = lds + DIRECT_A;
-    const int tid = threadIdx.x;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
-        const DevTask task = tasks[t];
-        const DevBlock &b = blocks[task.blk];
-        const int M = b.M, N = b.N, K = b.K;
-        const int dn = task.pad & 511, ng = (task.pad >> 9) & 511, rpt = task.pad >> 18;
-        const int dm = rpt * ng;
-        const int nl = tid % dn, ml = tid / dn;
-        const bool active = ml < ng;
-        const int m0 = task.m0, n = task.n0 + nl;
-        const DevOperand &oa = b.A, &ob = b.B, &oc = b.C;
-        const double *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
-        const double *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
-        double *pc = bufs[b.buf_c] + b.base_c;
-        __syncthreads();   // LDS of the previous task is no longer read
-        for (int e = tid; e < K * dm; e += NT) {
-            const int k = e / dm, r = e - k * dm;
-            const int m = m0 + r;
-            As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : 0.0;
-        }
-        const int span = task.span;
-        for (int i = tid; i < span; i += NT) Cs[i] = 0.0;
-        double bk[KMAX];
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            bk[k] = (k < K && n < N) ? ldg(pb + (ob.f1 * (int64_t)k + ob.s1 * (int64_t)n)) : 0.0;
-        __syncthreads();
-        // result column data (separable / affine maps)
-        int64_t cadd = 0;
-        bool cneg = false, cok = active && n < N;
-        if (cok && oc.table == 0) {
-            if (oc.second_affine) {
-                cadd = oc.s0 + oc.s1 * (int64_t)n;
-            } else {
-                const int32_t v = ldg(oc.second + n);
-                cok = v != 0;
-                cneg = v < 0;
-                cadd = (int64_t)(v < 0 ? -v : v) - 1;
-            }
-        }
-        int64_t addr[8];
-        double val[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int r = ml + ng * i;
-            const int m = m0 + r;
-            const bool live = i < rpt && cok;
-            double acc = 0.0;
-            if (live) {
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (k < K) acc = fma(As[k * dm + r], bk[k], acc);
-            }
-            int64_t a = -1;
-            bool neg = cneg;
-            if (live && m < M) {
-                if (oc.table) {
-                    const int32_t v = ldg(oc.first + ((int64_t)n + oc.ld * (int64_t)m));
-                    neg = v < 0;
-                    a = (int64_t)(v < 0 ? -v : v) - 1;
-                } else if (oc.first_affine) {
-                    a = oc.f0 + oc.f1 * (int64_t)m + cadd;
-                } else {
-                    const int32_t v = ldg(oc.first + m);
-                    neg = neg != (v < 0);
-                    a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
-                }
-            }
-            const double x = b.xfac * acc;
-            addr[i] = a;
-            val[i] = neg ? -x : x;
-        }
-        if (span > 0) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (addr[i] >= 0) atomicAdd(Cs + (addr[i] - task.lo), val[i]);
-            __syncthreads();
-            double *pr = pc + task.lo;
-            if (task.owned) {
-                for (int i0 = 0; i0 < span; i0 += 4 * NT) {
-                    double o[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * NT + tid;
-                        o[u] = i < span ? ldg(pr + i) : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * NT + tid;
-                        if (i < span) stg(pr + i, o[u] + Cs[i]);
-                    }
-                }
-            } else {
-                for (int i = tid; i < span; i += NT) {
-                    const double v = Cs[i];
-                    if (v != 0.0) atomic_add_g(pr + i, v);
-                }
-            }
-        } else if (b.atomic) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (addr[i] >= 0) atomic_add_g(pc + addr[i], val[i]);
-        } else {
-            double old[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) old[i] = addr[i] >= 0 ? ldg(pc + addr[i]) : 0.0;
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (addr[i] >= 0) stg(pc + addr[i], old[i] + val[i]);
         }
     }
 }
@@ -879,20 +869,6 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
-extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
-                                           const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int kmax, hipStream_t stream) {
-    if (ntasks <= 0) return 0;
-    if (kmax <= 8)
-        hipLaunchKernelGGL(gecco::grouped_contract_direct_f64<8>, dim3(ntasks), dim3(gecco::NT), 0,
-                           stream, blocks, tasks, ntasks, bufs);
-    else if (kmax <= 32)
-        hipLaunchKernelGGL(gecco::grouped_contract_direct_f64<32>, dim3(ntasks), dim3(gecco::NT), 0,
-                           stream, blocks, tasks, ntasks, bufs);
-    else
-        return 1;
-    return hipGetLastError() == hipSuccess ? 0 : 3;
-}
 
 extern "C" int gecco_direct_stage_cap(void) { return gecco::DIRECT_STAGE; }
 

@@ -38,9 +38,6 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            double *const *bufs, int grid, int variant,
                                            hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
-extern "C" int gecco_launch_grouped_direct(const gecco::DevBlock *blocks,
-                                           const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int kmax, hipStream_t stream);
 extern "C" int gecco_direct_stage_cap(void);
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
@@ -467,9 +464,10 @@ struct gecco_hip_plan {
     gecco::DevTask *d_tasks = nullptr;
     int32_t *d_index = nullptr;
     double **d_bufs = nullptr;
-    // task classes, in this order in d_tasks: strided-GEMM kernel, direct
-    // kernel with K <= 8, direct kernel with K <= 32, gather kernel
-    int64_t ntasks = 0, ntasks_affine = 0, ntasks_direct8 = 0, ntasks_direct32 = 0;
+    // task classes, in this order in d_tasks: strided-GEMM launch (GEMM
+    // tiles, 16 x 16 wave-split-K tiles and direct short-contraction tasks),
+    // gather kernel
+    int64_t ntasks = 0, ntasks_affine = 0;
     // operand materialisation (signed gather into plan-owned scratch)
     gecco::DevGather *d_gjobs = nullptr;
     gecco::DevGatherTask *d_gtasks = nullptr;
@@ -492,7 +490,8 @@ struct gecco_hip_plan {
     std::vector<double *> last_bufs;
     // optional HIP-event timing of the launches of execute (per launch class:
     // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation,
-    // 3 operand materialisation, 4 direct short-K kernel)
+    // 3 operand materialisation, 4 unused: the direct short-contraction tasks
+    // run in the strided-GEMM launch)
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     double ms[5] = {0, 0, 0, 0, 0};
@@ -770,8 +769,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     int abm = 64, abn = 64, abk = 32;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
         return fail(GECCO_HIP_EINVAL, "unknown tile variant %d", p->affine_variant);
-    // class of a task's block: 0 strided GEMM, 1 direct K <= 8, 2 direct
-    // K <= 32, 3 gather kernel; tile shape per block
+    // class of a task's block: 0 strided-GEMM launch, 3 gather kernel; tile
+    // shape per block
     std::vector<int> cls(p->blocks.size());
     std::vector<int64_t> tm(p->blocks.size()), tn(p->blocks.size());
     // direct tasks: columns dn | row groups ng << 9 | rows per thread << 18
@@ -779,7 +778,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
         if (b.direct) {
-            cls[i] = b.K <= 8 ? 1 : 2;
+            cls[i] = 0;
             // columns per task: all of N up to 256 threads, at most 64 when
             // the result map is not injective (colliding elements then tend
             // to share a task whose result range fits the LDS stage); as
@@ -925,7 +924,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->flops_affine = p->flops_general = p->flops_direct = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         const double f = 2.0 * (double)p->blocks[i].M * (double)p->blocks[i].N * (double)p->blocks[i].K;
-        (cls[i] == 0 ? p->flops_affine : cls[i] == 3 ? p->flops_general : p->flops_direct) += f;
+        (p->blocks[i].direct ? p->flops_direct : cls[i] == 3 ? p->flops_general : p->flops_affine) += f;
     }
     // direct blocks read a tabled result map n-fastest
     for (size_t i = 0; i < p->blocks.size(); ++i) {
@@ -1096,8 +1095,6 @@ static int plan_finalize(gecco_hip_plan *p) {
             return g_err;
     }
     p->ntasks_affine = ncls[0];
-    p->ntasks_direct8 = ncls[1];
-    p->ntasks_direct32 = ncls[2];
     {
         std::vector<DevSymStep> steps;
         std::vector<DevSymTask> stasks;
@@ -1185,8 +1182,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         p->last_bufs = want;
         p->static_done = false;   // constant slots may hold other data now
     }
-    const int64_t na = p->ntasks_affine, nd8 = p->ntasks_direct8, nd32 = p->ntasks_direct32;
-    const int64_t ng = p->ntasks - na - nd8 - nd32;
+    const int64_t na = p->ntasks_affine;
+    const int64_t ng = p->ntasks - na;
     auto mark = [&](int cls, bool begin) -> int {
         if (!p->timing) return 0;
         hipEvent_t e;
@@ -1214,17 +1211,9 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
-    if (nd8 + nd32) {
-        if (mark(4, true)) return g_err;
-        if (nd8 && gecco_launch_grouped_direct(p->d_blocks, p->d_tasks + na, (int)nd8, p->d_bufs, 8, s))
-            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-        if (nd32 && gecco_launch_grouped_direct(p->d_blocks, p->d_tasks + na + nd8, (int)nd32, p->d_bufs, 32, s))
-            return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-        if (mark(4, false)) return g_err;
-    }
     if (ng) {
         if (mark(1, true)) return g_err;
-        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na + nd8 + nd32, (int)ng, p->d_bufs, (int)ng, s))
+        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }

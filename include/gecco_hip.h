@@ -139,11 +139,12 @@ int gecco_hip_plan_stats(const gecco_hip_plan *plan, double *flops,
 
 /* Per-launch HIP-event timing of execute (off by default).  plan_timing
  * returns the accumulated ms and launch counts of the five launch classes
- * (ms[5], launches[5]: strided-GEMM contraction kernel, gather contraction
- * kernel, spin-flip symmetrisation, operand materialisation, direct
- * short-contraction kernel) since the previous call, and the FLOP per
- * execute of the three contraction kernels (flops[3]: strided GEMM, gather,
- * direct); it waits for the recorded events. */
+ * (ms[5], launches[5]: strided-GEMM contraction launch, gather contraction
+ * kernel, spin-flip symmetrisation, operand materialisation, and a fifth
+ * class that stays 0 -- the direct short-contraction tasks run in the
+ * strided-GEMM launch) since the previous call, and the FLOP per execute of
+ * the contraction classes (flops[3]: strided-GEMM tiles, gather kernel,
+ * direct tasks); it waits for the recorded events. */
 int gecco_hip_plan_set_timing(gecco_hip_plan *plan, int64_t on);
 int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, double *flops);
 /* Kernel-call details for diagnostics: [M, N, K, swap, injective, A/B/C

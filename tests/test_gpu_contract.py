@@ -234,7 +234,7 @@ def test_ccsd_terms_match_oracle(materialize, variant, chain, monkeypatch):
     orbs = orbitals(nocc, nvir, nsym)
     o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
     dev = torch.device('cuda:0')
-    staged = linked = skinny = 0
+    staged = linked = skinny = direct = 0
     for li, level in enumerate(ccsd.LEVELS):
         for ti, (res, a, b, cc, ca, fact) in enumerate(level):
             print(f'term L{li}.{ti} {res}<-{a}x{b} materialize={materialize} variant={variant} '
@@ -258,6 +258,7 @@ def test_ccsd_terms_match_oracle(materialize, variant, chain, monkeypatch):
             infos = [plan.block_info(i) for i in range(nb)]
             linked += sum(x[23] >= 0 for x in infos)
             skinny += sum(x[24] for x in infos)
+            direct += sum(x[19] for x in infos)
             bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
                     torch.from_numpy(x0).to(dev)]
             plan.execute(bufs)
@@ -269,6 +270,7 @@ def test_ccsd_terms_match_oracle(materialize, variant, chain, monkeypatch):
     # kernel only when materialised)
     assert (linked > 0) == bool(chain and materialize)
     assert skinny > 0 or not materialize     # 16 x 16 wave-split-K tiles
+    assert direct > 0                        # short contractions on the vector ALU
 
 
 @pytest.mark.parametrize('nsym', [1, 2, 4])
