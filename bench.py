@@ -186,16 +186,22 @@ def run_ccsd(args):
     solver = ccsd.CCSD(ham)
     setup_s = time.perf_counter() - t0
     torch.cuda.synchronize()
-    # warmup iterations from T = 0, then the timed steps continue the solve
+    # warmup iterations from T = 0 (the first one captures the iteration as
+    # a HIP graph), then the timed steps continue the solve with replays
     solver.solve(tol=0.0, maxit=args.warmup)
-    for p in solver.plans:
-        p.set_timing(True)
-        p.timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solve(tol=0.0, maxit=args.steps, restart=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # kernel durations (roofline): as many further iterations launched one by
+    # one (the same kernels as the replayed graph), timed with HIP events on
+    # the launch stream
+    for p in solver.plans:
+        p.set_timing(True)
+        p.timing()
+    solver.solve(tol=0.0, maxit=args.steps, restart=False, mode='async')
+    torch.cuda.synchronize()
     # launch classes: 0 strided-GEMM launch (with the direct short-contraction
     # tasks), 1 gather kernel, 2 spin-flip symmetrisation, 3 operand
     # materialisation, 4 unused

@@ -236,17 +236,34 @@ class CCSD:
         for p in self.plans:
             p.execute(self.bufs, stream)
 
-    def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True):
+    def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True,
+              mode='graph'):
         """Iterate to ||g|| < tol, g = D*t + R over T1|T2 (R: the formula's
         residual lists, D: the diagonal preconditioner), at most maxit
         iterations, with the native DIIS step (gecco_hip.DIIS).  restart:
-        start from T = 0 with an empty subspace; otherwise continue.  Returns
-        a dict with energy, iterations, residual norm and timings."""
+        start from T = 0 with an empty subspace; otherwise continue.
+
+        mode 'sync': the DIIS step solves its small system on the host, one
+        synchronisation per iteration.  'async': the whole step stays on the
+        device; the host reads iteration i's norms (from pinned memory) while
+        iteration i+1 runs, so the GPU queue never drains and one iteration
+        after the converged one is also run (its DIIS step meets the
+        tolerance again and leaves T unchanged).  'graph': as 'async', with
+        the iteration (three plan executes and the DIIS step) captured once
+        as a HIP graph and replayed.  Returns a dict with energy, iterations,
+        residual norm and timings."""
         import gecco_hip
         torch = self.torch
-        if restart or getattr(self, '_diis', None) is None or self._diis.maxsub != ndiis:
+        if mode not in ('sync', 'async', 'graph'):
+            raise ValueError(f'unknown solve mode {mode!r}')
+        want_async = mode != 'sync'
+        d = getattr(self, '_diis', None)
+        if restart or d is None or d.maxsub != ndiis or getattr(self, '_diis_async', None) != want_async:
             self.amp.zero_()
             self._diis = gecco_hip.DIIS(self.amp.numel(), ndiis)
+            self._diis_async = want_async
+            self._diis_steps = 0
+            self._graph = None
         diis = self._diis
         stream = torch.cuda.current_stream().cuda_stream
         energy = None
@@ -254,16 +271,63 @@ class CCSD:
         t0 = time.perf_counter()
         it = 0
         rnorm = float('inf')
-        for it in range(1, maxit + 1):
-            self.evaluate(stream)
-            rnorm, energy, conv = diis.step(self.amp, self.res, self.diag, energy=self.buf['E'],
-                                            tol=tol, stream=stream)
-            if on_iter:
-                on_iter(it, energy, rnorm)
-            if verbose:
-                print(f'  it {it:3d}  E = {energy:.12f}  |R| = {rnorm:.3e}', flush=True)
-            if conv:
-                break
+        if mode == 'sync':
+            for it in range(1, maxit + 1):
+                self.evaluate(stream)
+                rnorm, energy, conv = diis.step(self.amp, self.res, self.diag, energy=self.buf['E'],
+                                                tol=tol, stream=stream)
+                if on_iter:
+                    on_iter(it, energy, rnorm)
+                if verbose:
+                    print(f'  it {it:3d}  E = {energy:.12f}  |R| = {rnorm:.3e}', flush=True)
+                if conv:
+                    break
+        else:
+            def iteration(s):
+                self.evaluate(s)
+                diis.step_async(self.amp, self.res, self.diag, energy=self.buf['E'], tol=tol, stream=s)
+            graph = None
+            if mode == 'graph':
+                key = (tol, stream)
+                if self._graph is not None and self._graph[0] == key:
+                    graph = self._graph[1]
+            events = []
+            base = self._diis_steps
+            done = 0
+            for it in range(1, maxit + 1):
+                if graph is not None:
+                    graph.replay()
+                else:
+                    iteration(stream)
+                    if mode == 'graph':
+                        # captured after one eager iteration (a plan's first
+                        # execute uploads its buffer table and constant
+                        # gathers); replayed from the next iteration on
+                        torch.cuda.current_stream().synchronize()
+                        graph = torch.cuda.CUDAGraph()
+                        with torch.cuda.graph(graph):
+                            iteration(torch.cuda.current_stream().cuda_stream)
+                        self._graph = ((tol, stream), graph)
+                ev = torch.cuda.Event()
+                ev.record()
+                events.append(ev)
+                self._diis_steps += 1
+                # the previous iteration's record, while this one runs
+                conv = False
+                while done < it - 1 or (it == maxit and done < it):
+                    events[done].synchronize()
+                    rnorm, energy, code = diis.status(base + done)
+                    done += 1
+                    if on_iter:
+                        on_iter(done, energy, rnorm)
+                    if verbose:
+                        print(f'  it {done:3d}  E = {energy:.12f}  |R| = {rnorm:.3e}', flush=True)
+                    if code != 0:
+                        conv = True
+                        break
+                if conv:
+                    it = done
+                    break
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         return dict(energy=energy, iterations=it, residual=rnorm, seconds=wall,

@@ -9,8 +9,13 @@
 //   [B -1; -1 0] (w, l) = (0, -1)               (solve/optc_diis_extr.f:66-105)
 //   t = sum_i w_i v_i                           (solve/optc_expand_vec.f)
 // Device work per step: one fused pass (g, r, v, ||g||^2 and the new row of
-// B), a one-workgroup reduction in fixed order, one expansion pass; the
-// host solves the (m+1) x (m+1) system after a single synchronisation.
+// B), a one-workgroup reduction in fixed order, one expansion pass.  The
+// synchronous step solves the (m+1) x (m+1) system on the host after a
+// single synchronisation; the asynchronous step keeps the subspace
+// bookkeeping and the solve on the device (no synchronisation, the same
+// launches with the same arguments every iteration, so an iteration can be
+// captured and replayed as a graph) and reports norms through a ring of
+// records in pinned host memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,10 +54,11 @@ struct Weights {
 
 // g = diag*t + rest; r = g/diag; v = t - r into slot s[m-1]; per-workgroup
 // partial sums of <r_j|r> (j < m) and <g|g>, in part[blockIdx.x*(MAXSUB+1)+.]
-__global__ __launch_bounds__(NTH) void diis_prepare_f64(
-    const double *__restrict__ t, const double *__restrict__ rest,
-    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
-    int64_t n, Slots sl, double *__restrict__ part) {
+__device__ __forceinline__ void prepare_body(const double *__restrict__ t,
+                                             const double *__restrict__ rest,
+                                             const double *__restrict__ diag,
+                                             double *__restrict__ rsub, double *__restrict__ vsub,
+                                             int64_t n, const Slots &sl, double *__restrict__ part) {
     double acc[MAXSUB + 1];
 #pragma unroll
     for (int j = 0; j <= MAXSUB; ++j) acc[j] = 0.0;
@@ -89,6 +95,13 @@ __global__ __launch_bounds__(NTH) void diis_prepare_f64(
     }
 }
 
+__global__ __launch_bounds__(NTH) void diis_prepare_f64(
+    const double *__restrict__ t, const double *__restrict__ rest,
+    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
+    int64_t n, Slots sl, double *__restrict__ part) {
+    prepare_body(t, rest, diag, rsub, vsub, n, sl, part);
+}
+
 // out[j] = sum over workgroups of part[g][j] in workgroup order; out[MAXSUB+1]
 // = energy scalar (or 0)
 __global__ __launch_bounds__(NTH) void diis_reduce_f64(const double *__restrict__ part, int ngroup,
@@ -120,6 +133,153 @@ __global__ __launch_bounds__(NTH) void diis_expand_f64(double *__restrict__ t,
 #pragma unroll
         for (int j = 0; j < MAXSUB; ++j)
             if (j < wt.m) x += wt.w[j] * vsub[(int64_t)wt.s[j] * n + i];
+        t[i] = x;
+    }
+}
+
+// ---------------------------------------------------------------- async
+// Device-resident bookkeeping of the asynchronous step.
+constexpr int RING = 64;   // status records kept in pinned host memory
+
+struct DevState {
+    int m;                        // vectors in the subspace
+    int order[MAXSUB];            // slot of logical vector j, oldest first
+    int skip;                     // 1: the last step leaves t unchanged
+    int pad;
+    int64_t steps;                // asynchronous steps taken
+    double b[MAXSUB * MAXSUB];    // <r_i|r_j>, logical order, stride MAXSUB
+    double w[MAXSUB];             // weights of the last step
+};
+
+struct StatusRec {                // one per step, written by the device
+    double gnorm, energy;
+    int64_t step;                 // index of the step (-1: none yet)
+    int64_t converged;            // 1 converged, 0 extrapolated, -1 singular system
+};
+
+// slot of the new vector: a free one, or the oldest one's when full; drops
+// the oldest row/column of B (optc_update_redsp1 'shift')
+__global__ void diis_begin_f64(DevState *st, int maxsub) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int m = st->m, slot;
+    if (m == maxsub) {
+        slot = st->order[0];
+        for (int j = 0; j + 1 < m; ++j) st->order[j] = st->order[j + 1];
+        for (int i = 0; i + 1 < m; ++i)
+            for (int j = 0; j + 1 < m; ++j) st->b[i * MAXSUB + j] = st->b[(i + 1) * MAXSUB + j + 1];
+        m -= 1;
+    } else {
+        slot = m;
+    }
+    st->order[m] = slot;
+    st->m = m + 1;
+}
+
+__global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
+    const double *__restrict__ t, const double *__restrict__ rest,
+    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
+    int64_t n, const DevState *__restrict__ st, double *__restrict__ part) {
+    Slots sl;
+    sl.m = st->m;
+    for (int j = 0; j < MAXSUB; ++j) sl.s[j] = j < sl.m ? st->order[j] : 0;
+    prepare_body(t, rest, diag, rsub, vsub, n, sl, part);
+}
+
+// Reduction in workgroup order (as diis_reduce_f64), the new row of B, the
+// augmented system [B -1; -1 0] (w, l) = (0, -1) by Gaussian elimination
+// with partial pivoting (the host step's algorithm), and the status record.
+__global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
+                                                       const double *__restrict__ energy,
+                                                       DevState *__restrict__ st, double tol,
+                                                       StatusRec *ring) {
+    __shared__ double red[NTH / 64][MAXSUB + 1];
+    __shared__ double tot[MAXSUB + 1];
+    __shared__ double a[(MAXSUB + 1) * (MAXSUB + 1)], rhs[MAXSUB + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j <= MAXSUB; ++j) {
+        double v = 0.0;
+        for (int g = threadIdx.x; g < ngroup; g += NTH) v += part[(int64_t)g * (MAXSUB + 1) + j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= MAXSUB) {
+        double v = 0.0;
+        for (int w = 0; w < NTH / 64; ++w) v += red[w][threadIdx.x];
+        tot[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int m = st->m;
+    for (int j = 0; j < m; ++j) {
+        st->b[(m - 1) * MAXSUB + j] = tot[j];
+        st->b[j * MAXSUB + m - 1] = tot[j];
+    }
+    const double gn = sqrt(tot[MAXSUB]);
+    int64_t code = 0;
+    if (gn < tol) {
+        code = 1;
+        for (int j = 0; j < m; ++j) st->w[j] = 0.0;
+    } else {
+        const int k = m + 1;
+        for (int i = 0; i < k; ++i) {
+            for (int j = 0; j < k; ++j)
+                a[i * k + j] = (i < m && j < m) ? st->b[i * MAXSUB + j] : (i == j ? 0.0 : -1.0);
+            rhs[i] = i == m ? -1.0 : 0.0;
+        }
+        for (int c = 0; c < k && code == 0; ++c) {
+            int p = c;
+            for (int r = c + 1; r < k; ++r)
+                if (fabs(a[r * k + c]) > fabs(a[p * k + c])) p = r;
+            if (a[p * k + c] == 0.0) {
+                code = -1;
+                break;
+            }
+            if (p != c) {
+                for (int j = 0; j < k; ++j) {
+                    const double x = a[p * k + j];
+                    a[p * k + j] = a[c * k + j];
+                    a[c * k + j] = x;
+                }
+                const double x = rhs[p];
+                rhs[p] = rhs[c];
+                rhs[c] = x;
+            }
+            for (int r = c + 1; r < k; ++r) {
+                const double f = a[r * k + c] / a[c * k + c];
+                if (f == 0.0) continue;
+                for (int j = c; j < k; ++j) a[r * k + j] -= f * a[c * k + j];
+                rhs[r] -= f * rhs[c];
+            }
+        }
+        if (code == 0) {
+            for (int r = k - 1; r >= 0; --r) {
+                double x = rhs[r];
+                for (int j = r + 1; j < k; ++j) x -= a[r * k + j] * rhs[j];
+                rhs[r] = x / a[r * k + r];
+            }
+            for (int j = 0; j < m; ++j) st->w[j] = rhs[j];
+        }
+    }
+    st->skip = code != 0 ? 1 : 0;
+    StatusRec &rec = ring[st->steps % RING];
+    rec.gnorm = gn;
+    rec.energy = energy ? energy[0] : 0.0;
+    rec.converged = code;
+    __threadfence_system();
+    rec.step = st->steps;
+    st->steps += 1;
+}
+
+__global__ __launch_bounds__(NTH) void diis_expand_async_f64(double *__restrict__ t,
+                                                             const double *__restrict__ vsub,
+                                                             int64_t n,
+                                                             const DevState *__restrict__ st) {
+    if (st->skip) return;
+    const int m = st->m;
+    for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
+        double x = 0.0;
+        for (int j = 0; j < m; ++j) x += st->w[j] * vsub[(int64_t)st->order[j] * n + i];
         t[i] = x;
     }
 }
@@ -179,6 +339,9 @@ struct gecco_hip_diis {
     std::vector<int> order;                    // slot of logical vector j, oldest first
     std::vector<double> bmat;                  // <r_i|r_j>, logical order, maxsub x maxsub
     std::vector<double> w;                     // weights of the last step
+    DevState *dstate = nullptr;                // bookkeeping of the asynchronous step
+    StatusRec *ring = nullptr;                 // pinned status records (device-written)
+    bool async = false;                        // steps so far were asynchronous
 };
 
 extern "C" {
@@ -195,9 +358,16 @@ int gecco_hip_diis_create(int64_t n, int64_t maxsub, gecco_hip_diis **out) {
         check(hipMalloc(&d->vsub, (size_t)maxsub * n * sizeof(double)), "hipMalloc(diis v)") ||
         check(hipMalloc(&d->part, (size_t)NGROUP * (MAXSUB + 1) * sizeof(double)), "hipMalloc(diis partials)") ||
         check(hipMalloc(&d->red, (MAXSUB + 2) * sizeof(double)), "hipMalloc(diis reduction)") ||
-        check(hipHostMalloc(&d->host, (MAXSUB + 2) * sizeof(double)), "hipHostMalloc(diis)")) {
+        check(hipHostMalloc(&d->host, (MAXSUB + 2) * sizeof(double)), "hipHostMalloc(diis)") ||
+        check(hipMalloc(&d->dstate, sizeof(DevState)), "hipMalloc(diis state)") ||
+        check(hipHostMalloc(&d->ring, RING * sizeof(StatusRec), hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc(diis status)")) {
         gecco_hip_diis_destroy(d);
         return GECCO_HIP_ENOMEM;
+    }
+    if (gecco_hip_diis_reset(d)) {
+        gecco_hip_diis_destroy(d);
+        return GECCO_HIP_EDEVICE;
     }
     *out = d;
     return 0;
@@ -210,6 +380,8 @@ int gecco_hip_diis_destroy(gecco_hip_diis *d) {
     if (d->part) (void)hipFree(d->part);
     if (d->red) (void)hipFree(d->red);
     if (d->host) (void)hipHostFree(d->host);
+    if (d->dstate) (void)hipFree(d->dstate);
+    if (d->ring) (void)hipHostFree(d->ring);
     delete d;
     return 0;
 }
@@ -220,6 +392,17 @@ int gecco_hip_diis_reset(gecco_hip_diis *d) {
     d->order.clear();
     std::fill(d->bmat.begin(), d->bmat.end(), 0.0);
     d->w.clear();
+    d->async = false;
+    // device state from a zeroed host copy (synchronous, no memset)
+    DevState zero;
+    std::memset(&zero, 0, sizeof zero);
+    if (check(hipMemcpy(d->dstate, &zero, sizeof zero, hipMemcpyHostToDevice), "hipMemcpy(diis state)"))
+        return GECCO_HIP_EDEVICE;
+    for (int r = 0; r < RING; ++r) {
+        d->ring[r].step = -1;
+        d->ring[r].gnorm = d->ring[r].energy = 0.0;
+        d->ring[r].converged = 0;
+    }
     return 0;
 }
 
@@ -228,6 +411,7 @@ int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const 
                         int64_t *converged, void *stream) {
     gecco::clear_error();
     if (!d || !t || !rest || !diag) return fail(GECCO_HIP_EINVAL, "diis_step: null argument");
+    if (d->async) return fail(GECCO_HIP_ESTATE, "diis_step: subspace holds asynchronous steps (reset first)");
     hipStream_t s = (hipStream_t)stream;
     const int ms = d->maxsub;
     // slot of the new vector: a free one, or the oldest one's when full
@@ -292,8 +476,48 @@ int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const 
     return check(hipGetLastError(), "diis expand");
 }
 
+int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, const double *diag,
+                              const double *energy_dev, double tol, void *stream) {
+    gecco::clear_error();
+    if (!d || !t || !rest || !diag) return fail(GECCO_HIP_EINVAL, "diis_step_async: null argument");
+    if (!d->order.empty()) return fail(GECCO_HIP_ESTATE, "diis_step_async: subspace holds synchronous steps (reset first)");
+    hipStream_t s = (hipStream_t)stream;
+    d->async = true;
+    const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
+    hipLaunchKernelGGL(diis_begin_f64, dim3(1), dim3(64), 0, s, d->dstate, d->maxsub);
+    hipLaunchKernelGGL(diis_prepare_async_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub,
+                       d->vsub, d->n, d->dstate, d->part);
+    hipLaunchKernelGGL(diis_finish_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->dstate,
+                       tol, d->ring);
+    hipLaunchKernelGGL(diis_expand_async_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, d->dstate);
+    return check(hipGetLastError(), "diis async kernels");
+}
+
+int gecco_hip_diis_status(const gecco_hip_diis *d, int64_t step, double *gnorm, double *energy,
+                          int64_t *converged) {
+    gecco::clear_error();
+    if (!d || step < 0) return fail(GECCO_HIP_EINVAL, "diis_status: bad arguments");
+    const volatile StatusRec &rec = d->ring[step % RING];
+    if (rec.step != step)
+        return fail(GECCO_HIP_ESTATE, "diis_status: step %lld not completed or no longer kept",
+                    (long long)step);
+    if (gnorm) *gnorm = rec.gnorm;
+    if (energy) *energy = rec.energy;
+    if (converged) *converged = rec.converged;
+    return 0;
+}
+
 int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64_t *m) {
     if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    if (d->async) {
+        DevState st;
+        if (check(hipMemcpy(&st, d->dstate, sizeof st, hipMemcpyDeviceToHost), "hipMemcpy(diis state)"))
+            return GECCO_HIP_EDEVICE;
+        if (m) *m = st.m;
+        if (w)
+            for (int64_t j = 0; j < std::min<int64_t>(nw, st.m); ++j) w[j] = st.w[j];
+        return 0;
+    }
     if (m) *m = (int64_t)d->w.size();
     if (w)
         for (int64_t j = 0; j < std::min<int64_t>(nw, (int64_t)d->w.size()); ++j) w[j] = d->w[j];
@@ -302,6 +526,16 @@ int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64
 
 int gecco_hip_diis_matrix(const gecco_hip_diis *d, double *b, int64_t nb, int64_t *m) {
     if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    if (d->async) {
+        DevState st;
+        if (check(hipMemcpy(&st, d->dstate, sizeof st, hipMemcpyDeviceToHost), "hipMemcpy(diis state)"))
+            return GECCO_HIP_EDEVICE;
+        if (m) *m = st.m;
+        if (b && nb >= (int64_t)st.m * st.m)
+            for (int i = 0; i < st.m; ++i)
+                for (int j = 0; j < st.m; ++j) b[i * st.m + j] = st.b[i * MAXSUB + j];
+        return 0;
+    }
     const int64_t k = (int64_t)d->order.size();
     if (m) *m = k;
     if (b && nb >= k * k)

@@ -285,7 +285,8 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_list_diagonal', 'gecco_hip_plan_add_trace', 'gecco_hip_list_create_n',
                  'gecco_hip_plan_add_reorder')
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
-                  'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix')
+                  'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix',
+                  'gecco_hip_diis_step_async', 'gecco_hip_diis_status')
 VP = ctypes.c_void_p
 _lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
 _lib.gecco_hip_orbs_create.restype = ctypes.c_int
@@ -332,6 +333,8 @@ _lib.gecco_hip_diis_create.argtypes = [I64, I64, ctypes.POINTER(VP)]
 _lib.gecco_hip_diis_destroy.argtypes = [VP]
 _lib.gecco_hip_diis_reset.argtypes = [VP]
 _lib.gecco_hip_diis_step.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, PD, PD, PI64, VP]
+_lib.gecco_hip_diis_step_async.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, VP]
+_lib.gecco_hip_diis_status.argtypes = [VP, I64, PD, PD, PI64]
 _lib.gecco_hip_diis_weights.argtypes = [VP, PD, I64, PI64]
 _lib.gecco_hip_diis_matrix.argtypes = [VP, PD, I64, PI64]
 for _n in SOLVER_SYMBOLS:
@@ -550,6 +553,28 @@ class DIIS:
                                         None if energy is None else _ptr(energy), float(tol),
                                         ctypes.byref(gn), ctypes.byref(en), ctypes.byref(cv), s))
         return gn.value, (en.value if energy is not None else None), bool(cv.value)
+
+    def step_async(self, t, rest, diag, energy=None, tol=0.0, stream=None):
+        """The same step with the subspace bookkeeping and the small solve on
+        the device: nothing is synchronised, and every call queues the same
+        launches (an iteration can be captured as a graph).  Its record is
+        read with status(i) for the i-th asynchronous step since reset."""
+        for x in (t, rest, diag):
+            if x.numel() != self.n:
+                raise GeccoHipError('diis.step_async: vector length mismatch')
+        s = VP(0 if stream is None else int(stream))
+        _check(_lib.gecco_hip_diis_step_async(self._h, _ptr(t), _ptr(rest), _ptr(diag),
+                                              None if energy is None else _ptr(energy), float(tol), s))
+
+    def status(self, step):
+        """(||g||, energy, code) of asynchronous step `step` once the stream
+        has passed it; code 1 converged, 0 extrapolated, -1 singular."""
+        gn = ctypes.c_double()
+        en = ctypes.c_double()
+        cv = I64()
+        _check(_lib.gecco_hip_diis_status(self._h, int(step), ctypes.byref(gn), ctypes.byref(en),
+                                          ctypes.byref(cv)))
+        return gn.value, en.value, int(cv.value)
 
     def weights(self):
         w = (ctypes.c_double * 16)()

@@ -50,6 +50,26 @@ int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const 
                         const double *energy_dev, double tol, double *gnorm, double *energy,
                         int64_t *converged, void *stream);
 
+/* The same step without any synchronisation: the subspace bookkeeping (slot
+ * rotation, the new row of B, the augmented solve) runs on the device, so
+ * every step queues the same four launches with the same arguments and an
+ * iteration can be captured and replayed as a graph.  Step i (0-based since
+ * create/reset) writes its record into a ring of 64 in pinned host memory;
+ * read it with gecco_hip_diis_status once the stream has passed the step.
+ * A converged step (||g|| < tol) or a singular system leaves t unchanged.
+ * A subspace is either synchronous or asynchronous until reset.  No
+ * counterpart in the reference, whose solver is host code between
+ * contractions (solve/optc_macit.f); it serves callers that keep the whole
+ * iteration on the GPU. */
+int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, const double *diag,
+                              const double *energy_dev, double tol, void *stream);
+/* Record of asynchronous step `step`: ||g||, energy_dev[0] at that step, and
+ * converged = 1 (met tol), 0 (extrapolated) or -1 (singular system).  Fails
+ * with GECCO_HIP_ESTATE if the step has not completed or its record was
+ * overwritten (more than 64 steps ago). */
+int gecco_hip_diis_status(const gecco_hip_diis *d, int64_t step, double *gnorm, double *energy,
+                          int64_t *converged);
+
 /* Weights of the last step (w[0..m-1], oldest vector first) and the
  * current subspace matrix (b[m*m], row-major), for checks. */
 int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64_t *m);

@@ -50,3 +50,45 @@ def test_ch2_dz_matches_reference_energy():
     res = ccsd.CCSD(ham).solve(tol=1e-10, maxit=100)
     ref = float(np.load(path)['e_ccsd_ref'])
     assert abs(ham.e_ref + res['energy'] - ref) < 1e-10, (ham.e_ref + res['energy'], ref)
+
+
+@pytest.mark.parametrize('mode', ['sync', 'async'])
+def test_solve_modes_reach_the_same_energy(mode):
+    """The host-solve DIIS step, the device-resident asynchronous step and
+    the graph-captured iteration (the default) converge to the same energy
+    in the same number of iterations."""
+    _torch()
+    import ccsd
+    print(f'ccsd solve mode={mode} vs graph', flush=True)
+    ham = synthetic(4, 9, 27, nsym=4, seed=11, sigma=0.1)
+    a = ccsd.CCSD(ham).solve(tol=1e-10, maxit=60, mode=mode)
+    b = ccsd.CCSD(ham).solve(tol=1e-10, maxit=60, mode='graph')
+    assert a['residual'] < 1e-10 and b['residual'] < 1e-10
+    assert abs(a['energy'] - b['energy']) < 1e-11
+    assert a['iterations'] == b['iterations']
+
+
+def test_graph_replays_match_eager_iterations():
+    """Replays of the captured iteration, with eager work and
+    synchronisations between them, give the same per-iteration norms and
+    energies as the same iterations launched eagerly."""
+    torch = _torch()
+    import ccsd
+    ham = synthetic(3, 8, 24, nsym=2, seed=13, sigma=0.1)
+    k = 9
+    eager = []
+    ccsd.CCSD(ham).solve(tol=0.0, maxit=k, mode='async', on_iter=lambda i, e, r: eager.append((e, r)))
+    c = ccsd.CCSD(ham)
+    replayed = []
+    c.solve(tol=0.0, maxit=1, mode='graph', on_iter=lambda i, e, r: replayed.append((e, r)))
+    for _ in range(k - 1):
+        x = torch.ones(4096, dtype=torch.float64, device='cuda:0').cumsum(0)   # eager work
+        torch.cuda.synchronize()
+        assert float(x[-1]) == 4096.0
+        c.solve(tol=0.0, maxit=1, mode='graph', restart=False,
+                on_iter=lambda i, e, r: replayed.append((e, r)))
+    assert c._graph is not None
+    assert len(replayed) == len(eager) == k
+    for (e1, r1), (e2, r2) in zip(eager, replayed):
+        assert abs(e1 - e2) <= 1e-12 * max(1.0, abs(e1))
+        assert abs(r1 - r2) <= 1e-10 * max(1e-12, r1)

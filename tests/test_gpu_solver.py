@@ -112,3 +112,59 @@ def test_plan_list_initialisation():
     b2[50:250] = 0.0
     np.testing.assert_array_equal(bufs[1].cpu().numpy(), b1)
     np.testing.assert_array_equal(bufs[2].cpu().numpy(), b2)
+
+
+@pytest.mark.parametrize('n,maxsub,steps', [(1000, 8, 12), (300001, 4, 9)])
+def test_async_step_matches_host_step(n, maxsub, steps):
+    """The device-resident step (bookkeeping and the augmented solve on the
+    GPU, records in pinned memory) against the host-solve step on the same
+    vector function."""
+    torch = _torch()
+    import gecco_hip
+    print(f'diis async n={n} maxsub={maxsub}', flush=True)
+    rng = np.random.default_rng(n + 1)
+    d = rng.uniform(0.5, 3.0, n)
+    b = 0.1 * rng.standard_normal(n)
+    c = 0.05 * rng.standard_normal(n)
+    dev = torch.device('cuda:0')
+    d_dev = torch.from_numpy(d).to(dev)
+    e_dev = torch.tensor([-2.5], dtype=torch.float64, device=dev)
+    sync, asy = gecco_hip.DIIS(n, maxsub), gecco_hip.DIIS(n, maxsub)
+    t1 = torch.zeros(n, dtype=torch.float64, device=dev)
+    t2 = torch.zeros(n, dtype=torch.float64, device=dev)
+    for it in range(steps):
+        tt = t1.cpu().numpy()
+        rest = torch.from_numpy(b + c * tt * tt + 0.01 * np.roll(tt, 1)).to(dev)
+        gn, en, conv = sync.step(t1, rest, d_dev, energy=e_dev)
+        asy.step_async(t2, rest, d_dev, energy=e_dev)
+        torch.cuda.synchronize()
+        gn2, en2, code = asy.status(it)
+        assert code == 0 and en2 == en == -2.5
+        assert abs(gn2 - gn) <= 1e-12 * gn
+        scale = float(t1.abs().max())
+        assert float((t1 - t2).abs().max()) <= 1e-11 * scale, it
+        np.testing.assert_allclose(asy.matrix(), sync.matrix(), rtol=0, atol=1e-13 * np.abs(sync.matrix()).max())
+        t2.copy_(t1)                   # both continue from the same amplitudes
+    with pytest.raises(gecco_hip.GeccoHipError):
+        asy.status(steps)              # not taken yet
+    with pytest.raises(gecco_hip.GeccoHipError):
+        sync.step_async(t1, rest, d_dev)   # subspace holds synchronous steps
+
+
+def test_async_step_converged_leaves_t():
+    torch = _torch()
+    import gecco_hip
+    n = 500
+    dev = torch.device('cuda:0')
+    d = torch.full((n,), 2.0, dtype=torch.float64, device=dev)
+    t = torch.full((n,), 0.5, dtype=torch.float64, device=dev)
+    rest = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+    diis = gecco_hip.DIIS(n, 4)
+    diis.step_async(t, rest, d, tol=1e-12)
+    torch.cuda.synchronize()
+    gn, _, code = diis.status(0)
+    assert code == 1 and gn == 0.0
+    assert torch.equal(t, torch.full((n,), 0.5, dtype=torch.float64, device=dev))
+    diis.reset()
+    with pytest.raises(gecco_hip.GeccoHipError):
+        diis.status(0)
