@@ -138,16 +138,20 @@ __global__ __launch_bounds__(NTH) void diis_expand_f64(double *__restrict__ t,
 }
 
 // ---------------------------------------------------------------- async
-// Device-resident bookkeeping of the asynchronous step.
+// Device-resident bookkeeping of the asynchronous step.  The state always
+// describes the coming step: m counts the vectors including the one the
+// coming step adds, whose slot is order[m-1] (reset: m = 1, order[0] = 0).
 constexpr int RING = 64;   // status records kept in pinned host memory
 
 struct DevState {
-    int m;                        // vectors in the subspace
+    int m;                        // vectors after the coming step
     int order[MAXSUB];            // slot of logical vector j, oldest first
     int skip;                     // 1: the last step leaves t unchanged
-    int pad;
+    int mw;                       // vectors of the last step's extrapolation
+    int wslot[MAXSUB];            // their slots
     int64_t steps;                // asynchronous steps taken
     double b[MAXSUB * MAXSUB];    // <r_i|r_j>, logical order, stride MAXSUB
+    double bw[MAXSUB * MAXSUB];   // B of the last step's extrapolation
     double w[MAXSUB];             // weights of the last step
 };
 
@@ -156,24 +160,6 @@ struct StatusRec {                // one per step, written by the device
     int64_t step;                 // index of the step (-1: none yet)
     int64_t converged;            // 1 converged, 0 extrapolated, -1 singular system
 };
-
-// slot of the new vector: a free one, or the oldest one's when full; drops
-// the oldest row/column of B (optc_update_redsp1 'shift')
-__global__ void diis_begin_f64(DevState *st, int maxsub) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int m = st->m, slot;
-    if (m == maxsub) {
-        slot = st->order[0];
-        for (int j = 0; j + 1 < m; ++j) st->order[j] = st->order[j + 1];
-        for (int i = 0; i + 1 < m; ++i)
-            for (int j = 0; j + 1 < m; ++j) st->b[i * MAXSUB + j] = st->b[(i + 1) * MAXSUB + j + 1];
-        m -= 1;
-    } else {
-        slot = m;
-    }
-    st->order[m] = slot;
-    st->m = m + 1;
-}
 
 __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
     const double *__restrict__ t, const double *__restrict__ rest,
@@ -187,80 +173,117 @@ __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
 
 // Reduction in workgroup order (as diis_reduce_f64), the new row of B, the
 // augmented system [B -1; -1 0] (w, l) = (0, -1) by Gaussian elimination
-// with partial pivoting (the host step's algorithm), and the status record.
+// with partial pivoting (the host step's algorithm; each pivot column's
+// search, row swap, factors and row updates in parallel over the
+// workgroup), the status record, and the slot of the next step's vector:
+// a free one, or the oldest one's when full, whose row and column of B are
+// dropped (optc_update_redsp1 'shift').
 __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
                                                        const double *__restrict__ energy,
-                                                       DevState *__restrict__ st, double tol,
-                                                       StatusRec *ring) {
+                                                       DevState *__restrict__ st, int maxsub,
+                                                       double tol, StatusRec *ring) {
+    constexpr int KM = MAXSUB + 1;
     __shared__ double red[NTH / 64][MAXSUB + 1];
     __shared__ double tot[MAXSUB + 1];
-    __shared__ double a[(MAXSUB + 1) * (MAXSUB + 1)], rhs[MAXSUB + 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ double a[KM * KM], rhs[KM], fac[KM];
+    __shared__ int piv, code_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // partial sums of this thread, all loads in flight together
+    constexpr int PER = (NGROUP + NTH - 1) / NTH;
+    double v[MAXSUB + 1];
+#pragma unroll
+    for (int j = 0; j <= MAXSUB; ++j) v[j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = tid + q * NTH;
+        if (g < ngroup) {
+#pragma unroll
+            for (int j = 0; j <= MAXSUB; ++j) v[j] += part[(int64_t)g * (MAXSUB + 1) + j];
+        }
+    }
+#pragma unroll
     for (int j = 0; j <= MAXSUB; ++j) {
-        double v = 0.0;
-        for (int g = threadIdx.x; g < ngroup; g += NTH) v += part[(int64_t)g * (MAXSUB + 1) + j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if (lane == 0) red[wave][j] = v;
+        double x = v[j];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) red[wave][j] = x;
     }
     __syncthreads();
-    if (threadIdx.x <= MAXSUB) {
-        double v = 0.0;
-        for (int w = 0; w < NTH / 64; ++w) v += red[w][threadIdx.x];
-        tot[threadIdx.x] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     const int m = st->m;
-    for (int j = 0; j < m; ++j) {
-        st->b[(m - 1) * MAXSUB + j] = tot[j];
-        st->b[j * MAXSUB + m - 1] = tot[j];
+    const int k = m + 1;
+    if (tid <= MAXSUB) {
+        double x = 0.0;
+        for (int w = 0; w < NTH / 64; ++w) x += red[w][tid];
+        tot[tid] = x;
+    }
+    __syncthreads();
+    if (tid < m) {
+        st->b[(m - 1) * MAXSUB + tid] = tot[tid];
+        st->b[tid * MAXSUB + m - 1] = tot[tid];
     }
     const double gn = sqrt(tot[MAXSUB]);
-    int64_t code = 0;
-    if (gn < tol) {
-        code = 1;
-        for (int j = 0; j < m; ++j) st->w[j] = 0.0;
-    } else {
-        const int k = m + 1;
-        for (int i = 0; i < k; ++i) {
-            for (int j = 0; j < k; ++j)
-                a[i * k + j] = (i < m && j < m) ? st->b[i * MAXSUB + j] : (i == j ? 0.0 : -1.0);
-            rhs[i] = i == m ? -1.0 : 0.0;
-        }
-        for (int c = 0; c < k && code == 0; ++c) {
-            int p = c;
-            for (int r = c + 1; r < k; ++r)
-                if (fabs(a[r * k + c]) > fabs(a[p * k + c])) p = r;
-            if (a[p * k + c] == 0.0) {
-                code = -1;
-                break;
+    const bool conv = gn < tol;
+    if (tid == 0) code_s = conv ? 1 : 0;
+    // augmented matrix (row-major, k x k) and right-hand side
+    for (int e = tid; e < k * k; e += NTH) {
+        const int i = e / k, j = e - i * k;
+        a[e] = (i < m && j < m) ? (i == m - 1 ? tot[j] : j == m - 1 ? tot[i] : st->b[i * MAXSUB + j])
+                                : (i == j ? 0.0 : -1.0);
+        if (i < m && j < m) st->bw[i * MAXSUB + j] = a[e];
+    }
+    if (tid < k) rhs[tid] = tid == m ? -1.0 : 0.0;
+    __syncthreads();
+    if (!conv) {
+        for (int c = 0; c < k; ++c) {
+            if (tid == 0) {
+                int p = c;
+                for (int r = c + 1; r < k; ++r)
+                    if (fabs(a[r * k + c]) > fabs(a[p * k + c])) p = r;
+                piv = p;
+                if (a[p * k + c] == 0.0) code_s = -1;
             }
-            if (p != c) {
-                for (int j = 0; j < k; ++j) {
-                    const double x = a[p * k + j];
-                    a[p * k + j] = a[c * k + j];
-                    a[c * k + j] = x;
+            __syncthreads();
+            if (code_s != 0) break;
+            const int p = piv;
+            if (p != c && tid <= k) {
+                if (tid < k) {
+                    const double x = a[p * k + tid];
+                    a[p * k + tid] = a[c * k + tid];
+                    a[c * k + tid] = x;
+                } else {
+                    const double x = rhs[p];
+                    rhs[p] = rhs[c];
+                    rhs[c] = x;
                 }
-                const double x = rhs[p];
-                rhs[p] = rhs[c];
-                rhs[c] = x;
             }
-            for (int r = c + 1; r < k; ++r) {
-                const double f = a[r * k + c] / a[c * k + c];
+            __syncthreads();
+            if (tid > c && tid < k) fac[tid] = a[tid * k + c] / a[c * k + c];
+            __syncthreads();
+            // rows r > c, columns j >= c, and the right-hand side (j = k)
+            const int nc = k - c + 1;
+            for (int e = tid; e < (k - c - 1) * nc; e += NTH) {
+                const int r = c + 1 + e / nc, j = c + e % nc;
+                const double f = fac[r];
                 if (f == 0.0) continue;
-                for (int j = c; j < k; ++j) a[r * k + j] -= f * a[c * k + j];
-                rhs[r] -= f * rhs[c];
+                if (j < k) a[r * k + j] -= f * a[c * k + j];
+                else rhs[r] -= f * rhs[c];
             }
-        }
-        if (code == 0) {
-            for (int r = k - 1; r >= 0; --r) {
-                double x = rhs[r];
-                for (int j = r + 1; j < k; ++j) x -= a[r * k + j] * rhs[j];
-                rhs[r] = x / a[r * k + r];
-            }
-            for (int j = 0; j < m; ++j) st->w[j] = rhs[j];
+            __syncthreads();
         }
     }
+    if (tid != 0) return;
+    int64_t code = code_s;
+    if (code == 0) {
+        for (int r = k - 1; r >= 0; --r) {
+            double x = rhs[r];
+            for (int j = r + 1; j < k; ++j) x -= a[r * k + j] * rhs[j];
+            rhs[r] = x / a[r * k + r];
+        }
+    }
+    for (int j = 0; j < m; ++j) {
+        st->w[j] = code == 0 ? rhs[j] : 0.0;
+        st->wslot[j] = st->order[j];
+    }
+    st->mw = m;
     st->skip = code != 0 ? 1 : 0;
     StatusRec &rec = ring[st->steps % RING];
     rec.gnorm = gn;
@@ -269,6 +292,19 @@ __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict_
     __threadfence_system();
     rec.step = st->steps;
     st->steps += 1;
+    // the next step's vector
+    int mm = m, slot;
+    if (mm == maxsub) {
+        slot = st->order[0];
+        for (int j = 0; j + 1 < mm; ++j) st->order[j] = st->order[j + 1];
+        for (int i = 0; i + 1 < mm; ++i)
+            for (int j = 0; j + 1 < mm; ++j) st->b[i * MAXSUB + j] = st->b[(i + 1) * MAXSUB + j + 1];
+        mm -= 1;
+    } else {
+        slot = mm;
+    }
+    st->order[mm] = slot;
+    st->m = mm + 1;
 }
 
 __global__ __launch_bounds__(NTH) void diis_expand_async_f64(double *__restrict__ t,
@@ -276,10 +312,10 @@ __global__ __launch_bounds__(NTH) void diis_expand_async_f64(double *__restrict_
                                                              int64_t n,
                                                              const DevState *__restrict__ st) {
     if (st->skip) return;
-    const int m = st->m;
+    const int m = st->mw;
     for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
         double x = 0.0;
-        for (int j = 0; j < m; ++j) x += st->w[j] * vsub[(int64_t)st->order[j] * n + i];
+        for (int j = 0; j < m; ++j) x += st->w[j] * vsub[(int64_t)st->wslot[j] * n + i];
         t[i] = x;
     }
 }
@@ -396,6 +432,7 @@ int gecco_hip_diis_reset(gecco_hip_diis *d) {
     // device state from a zeroed host copy (synchronous, no memset)
     DevState zero;
     std::memset(&zero, 0, sizeof zero);
+    zero.m = 1;              // the first step adds the vector in slot 0
     if (check(hipMemcpy(d->dstate, &zero, sizeof zero, hipMemcpyHostToDevice), "hipMemcpy(diis state)"))
         return GECCO_HIP_EDEVICE;
     for (int r = 0; r < RING; ++r) {
@@ -484,11 +521,10 @@ int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, 
     hipStream_t s = (hipStream_t)stream;
     d->async = true;
     const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
-    hipLaunchKernelGGL(diis_begin_f64, dim3(1), dim3(64), 0, s, d->dstate, d->maxsub);
     hipLaunchKernelGGL(diis_prepare_async_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub,
                        d->vsub, d->n, d->dstate, d->part);
     hipLaunchKernelGGL(diis_finish_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->dstate,
-                       tol, d->ring);
+                       d->maxsub, tol, d->ring);
     hipLaunchKernelGGL(diis_expand_async_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, d->dstate);
     return check(hipGetLastError(), "diis async kernels");
 }
@@ -513,9 +549,9 @@ int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64
         DevState st;
         if (check(hipMemcpy(&st, d->dstate, sizeof st, hipMemcpyDeviceToHost), "hipMemcpy(diis state)"))
             return GECCO_HIP_EDEVICE;
-        if (m) *m = st.m;
+        if (m) *m = st.mw;
         if (w)
-            for (int64_t j = 0; j < std::min<int64_t>(nw, st.m); ++j) w[j] = st.w[j];
+            for (int64_t j = 0; j < std::min<int64_t>(nw, st.mw); ++j) w[j] = st.w[j];
         return 0;
     }
     if (m) *m = (int64_t)d->w.size();
@@ -530,10 +566,13 @@ int gecco_hip_diis_matrix(const gecco_hip_diis *d, double *b, int64_t nb, int64_
         DevState st;
         if (check(hipMemcpy(&st, d->dstate, sizeof st, hipMemcpyDeviceToHost), "hipMemcpy(diis state)"))
             return GECCO_HIP_EDEVICE;
-        if (m) *m = st.m;
-        if (b && nb >= (int64_t)st.m * st.m)
-            for (int i = 0; i < st.m; ++i)
-                for (int j = 0; j < st.m; ++j) b[i * st.m + j] = st.b[i * MAXSUB + j];
+        // the matrix of the last step's extrapolation (the state already
+        // describes the next step)
+        const int held = st.mw;
+        if (m) *m = held;
+        if (b && nb >= (int64_t)held * held)
+            for (int i = 0; i < held; ++i)
+                for (int j = 0; j < held; ++j) b[i * held + j] = st.bw[i * MAXSUB + j];
         return 0;
     }
     const int64_t k = (int64_t)d->order.size();
