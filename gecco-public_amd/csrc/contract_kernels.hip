@@ -778,12 +778,26 @@ __global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict_
     const DevInitTask t = tasks[blockIdx.x];
     const DevInit &j = jobs[t.job];
     double *__restrict__ dst = bufs[j.dst] + j.off_dst;
+    // U elements per thread and round, their loads in flight together
+    constexpr int U = 4;
     if (j.src >= 0) {
         const double *__restrict__ src = bufs[j.src] + j.off_src;
         const double f = j.factor;
-        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) dst[e] = f * src[e];
+        for (int64_t e0 = t.e0 + threadIdx.x; e0 < t.e1; e0 += U * blockDim.x) {
+            double x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t e = e0 + u * blockDim.x;
+                x[u] = e < t.e1 ? ldg(src + e) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t e = e0 + u * blockDim.x;
+                if (e < t.e1) stg(dst + e, f * x[u]);
+            }
+        }
     } else {
-        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) dst[e] = 0.0;
+        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) stg(dst + e, 0.0);
     }
 }
 

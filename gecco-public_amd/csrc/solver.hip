@@ -54,17 +54,19 @@ struct Weights {
 
 // g = diag*t + rest; r = g/diag; v = t - r into slot s[m-1]; per-workgroup
 // partial sums of <r_j|r> (j < m) and <g|g>, in part[blockIdx.x*(MAXSUB+1)+.]
-__device__ __forceinline__ void prepare_body(const double *__restrict__ t,
-                                             const double *__restrict__ rest,
-                                             const double *__restrict__ diag,
-                                             double *__restrict__ rsub, double *__restrict__ vsub,
-                                             int64_t n, const Slots &sl, double *__restrict__ part) {
-    double acc[MAXSUB + 1];
+// M = sl.m vectors after the step (compile-time: the per-element loop over
+// the old vectors has no branches)
+template <int M>
+__device__ __forceinline__ void prepare_m(const double *__restrict__ t,
+                                          const double *__restrict__ rest,
+                                          const double *__restrict__ diag,
+                                          double *__restrict__ rsub, double *__restrict__ vsub,
+                                          int64_t n, const Slots &sl, double acc[MAXSUB + 1]) {
+    double *__restrict__ rn = rsub + (int64_t)sl.s[M - 1] * n;
+    double *__restrict__ vn = vsub + (int64_t)sl.s[M - 1] * n;
+    const double *__restrict__ old[M > 1 ? M - 1 : 1];
 #pragma unroll
-    for (int j = 0; j <= MAXSUB; ++j) acc[j] = 0.0;
-    const int m = sl.m;
-    double *__restrict__ rn = rsub + (int64_t)sl.s[m - 1] * n;
-    double *__restrict__ vn = vsub + (int64_t)sl.s[m - 1] * n;
+    for (int j = 0; j < M - 1; ++j) old[j] = rsub + (int64_t)sl.s[j] * n;
     for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
         const double ti = t[i], d = diag[i];
         const double g = d * ti + rest[i];
@@ -74,10 +76,26 @@ __device__ __forceinline__ void prepare_body(const double *__restrict__ t,
         vn[i] = ti - r;
         acc[MAXSUB] += g * g;
 #pragma unroll
-        for (int j = 0; j < MAXSUB; ++j) {
-            if (j < m - 1) acc[j] += r * rsub[(int64_t)sl.s[j] * n + i];
-            else if (j == m - 1) acc[j] += r * r;
-        }
+        for (int j = 0; j < M - 1; ++j) acc[j] += r * old[j][i];
+        acc[M - 1] += r * r;
+    }
+}
+
+__device__ __forceinline__ void prepare_body(const double *__restrict__ t,
+                                             const double *__restrict__ rest,
+                                             const double *__restrict__ diag,
+                                             double *__restrict__ rsub, double *__restrict__ vsub,
+                                             int64_t n, const Slots &sl, double *__restrict__ part) {
+    double acc[MAXSUB + 1];
+#pragma unroll
+    for (int j = 0; j <= MAXSUB; ++j) acc[j] = 0.0;
+    switch (sl.m) {
+#define GECCO_PREP(M) case M: prepare_m<M>(t, rest, diag, rsub, vsub, n, sl, acc); break;
+        GECCO_PREP(1) GECCO_PREP(2) GECCO_PREP(3) GECCO_PREP(4) GECCO_PREP(5) GECCO_PREP(6)
+        GECCO_PREP(7) GECCO_PREP(8) GECCO_PREP(9) GECCO_PREP(10) GECCO_PREP(11) GECCO_PREP(12)
+        GECCO_PREP(13) GECCO_PREP(14) GECCO_PREP(15) GECCO_PREP(16)
+#undef GECCO_PREP
+        default: break;
     }
     __shared__ double red[NTH / 64][MAXSUB + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -124,17 +142,37 @@ __global__ __launch_bounds__(NTH) void diis_reduce_f64(const double *__restrict_
     if (threadIdx.x == 0) out[MAXSUB + 1] = energy ? energy[0] : 0.0;
 }
 
-// t = sum_j w_j v_j
-__global__ __launch_bounds__(NTH) void diis_expand_f64(double *__restrict__ t,
-                                                       const double *__restrict__ vsub, int64_t n,
-                                                       Weights wt) {
+// t = sum_j w_j v_j (M = wt.m compile-time)
+template <int M>
+__device__ __forceinline__ void expand_m(double *__restrict__ t, const double *__restrict__ vsub,
+                                         int64_t n, const Weights &wt) {
+    const double *__restrict__ v[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) v[j] = vsub + (int64_t)wt.s[j] * n;
     for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
         double x = 0.0;
 #pragma unroll
-        for (int j = 0; j < MAXSUB; ++j)
-            if (j < wt.m) x += wt.w[j] * vsub[(int64_t)wt.s[j] * n + i];
+        for (int j = 0; j < M; ++j) x += wt.w[j] * v[j][i];
         t[i] = x;
     }
+}
+
+__device__ __forceinline__ void expand_body(double *__restrict__ t, const double *__restrict__ vsub,
+                                            int64_t n, const Weights &wt) {
+    switch (wt.m) {
+#define GECCO_EXP(M) case M: expand_m<M>(t, vsub, n, wt); break;
+        GECCO_EXP(1) GECCO_EXP(2) GECCO_EXP(3) GECCO_EXP(4) GECCO_EXP(5) GECCO_EXP(6)
+        GECCO_EXP(7) GECCO_EXP(8) GECCO_EXP(9) GECCO_EXP(10) GECCO_EXP(11) GECCO_EXP(12)
+        GECCO_EXP(13) GECCO_EXP(14) GECCO_EXP(15) GECCO_EXP(16)
+#undef GECCO_EXP
+        default: break;
+    }
+}
+
+__global__ __launch_bounds__(NTH) void diis_expand_f64(double *__restrict__ t,
+                                                       const double *__restrict__ vsub, int64_t n,
+                                                       Weights wt) {
+    expand_body(t, vsub, n, wt);
 }
 
 // ---------------------------------------------------------------- async
@@ -312,12 +350,13 @@ __global__ __launch_bounds__(NTH) void diis_expand_async_f64(double *__restrict_
                                                              int64_t n,
                                                              const DevState *__restrict__ st) {
     if (st->skip) return;
-    const int m = st->mw;
-    for (int64_t i = (int64_t)blockIdx.x * NTH + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTH) {
-        double x = 0.0;
-        for (int j = 0; j < m; ++j) x += st->w[j] * vsub[(int64_t)st->wslot[j] * n + i];
-        t[i] = x;
+    Weights wt;
+    wt.m = st->mw;
+    for (int j = 0; j < MAXSUB; ++j) {
+        wt.s[j] = j < wt.m ? st->wslot[j] : 0;
+        wt.w[j] = j < wt.m ? st->w[j] : 0.0;
     }
+    expand_body(t, vsub, n, wt);
 }
 
 int fail(int code, const char *fmt, ...) {
