@@ -1029,7 +1029,7 @@ static int plan_finalize(gecco_hip_plan *p) {
             }
         }
         std::vector<DevGatherTask> gt;
-        const int64_t chunk = 4096;   // elements per workgroup (16 per thread)
+        const int64_t chunk = 2048;   // elements per workgroup (8 per thread)
         double gbytes = 0;
         auto is_const = [&](int32_t slot) {
             return slot >= 0 && slot < (int32_t)p->constant_slot.size() && p->constant_slot[slot];
@@ -1082,7 +1082,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     }
     if (!p->inits.empty()) {
         std::vector<DevInitTask> it;
-        const int64_t chunk = 4096;   // elements per workgroup (16 per thread)
+        const int64_t chunk = 2048;   // elements per workgroup (8 per thread)
         for (size_t j = 0; j < p->inits.size(); ++j)
             for (int64_t e0 = 0; e0 < p->inits[j].len; e0 += chunk)
                 it.push_back({(int32_t)j, 0, e0, std::min(p->inits[j].len, e0 + chunk)});
