@@ -35,6 +35,8 @@ import time
 
 import numpy as np
 
+import formula as _formula
+
 H, P = 1, 2
 
 # name: (C occupation, A occupation)
@@ -61,8 +63,11 @@ INTEGRALS = [n for n in LISTS if n.startswith('H')]
 INIT = {'tau': ('T2', 1.0), 'taut': ('T2', 1.0), 'Wmnij': ('Hoooo', 1.0),
         'Wmbej': ('Hovvo', -1.0), 'R2': ('Hvvoo', 1.0)}
 
-# (result, op1, op2, CNT on op1's C side, CNT on op1's A side, factor)
-LEVELS = [
+# The formula as a flat list in GeCCo's order (formula.py): list
+# initialisations, then the binary contractions (result, op1, op2, CNT on
+# op1's C side, CNT on op1's A side, factor).  The levels below are derived
+# from it by formula.schedule; the nesting here shows them.
+_TERM_LEVELS = [
     [('tau', 'T1', 'T1', {}, {}, -0.5),
      ('taut', 'T1', 'T1', {}, {}, -0.25),
      ('Fme', 'Hoovv', 'T1', {H: 1}, {P: 1}, 1.0),
@@ -104,54 +109,64 @@ LEVELS = [
 
 
 
-def sym_schedule(levels):
-    """Lists to spin-flip symmetrise at the end of each level.  The
-    symmetrisation (contract/sym_ab_blk.f) is a linear projector and the
-    initial contents of every list (integrals, zeros) are fixed points of
-    it, so contributions of several levels can be projected together: a list
-    is projected at the end of level L when it is written there and read in
-    a later level no later than its next write, or never written again (the
-    residuals).  Items of one level read only lists of earlier levels."""
-    writes = [{t[0] for t in lv} for lv in levels]
-    reads = [{t[1] for t in lv} | {t[2] for t in lv} for lv in levels]
-    out = []
-    for li, lv in enumerate(levels):
-        todo = []
-        for res in dict.fromkeys(t[0] for t in lv):
-            nxt = next((l for l in range(li + 1, len(levels)) if res in writes[l]), None)
-            last = len(levels) - 1 if nxt is None else nxt
-            if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
-                todo.append(res)
-        out.append(todo)
-    return out
+TERMS = [t for level in _TERM_LEVELS for t in level]
 
 
-SYM_AFTER = sym_schedule(LEVELS)
+def _formula_items():
+    items = []
+    for n in LISTS:
+        if n in INIT:
+            src, f = INIT[n]
+            items.append(_formula.init(n, src, f))
+        elif n not in INTEGRALS and n not in ('T1', 'T2'):
+            items.append(_formula.init(n))
+    items += [_formula.contr(*t) for t in TERMS]
+    return items
+
+
+# The CCSD formula ({"lists", "constant", "items"}, see formula.py), its
+# schedule, and the contraction tuples and projections per level.
+FORMULA = {'lists': LISTS, 'constant': INTEGRALS, 'items': _formula_items()}
+SCHEDULE = _formula.schedule(FORMULA['items'])
+LEVELS = [[(c['target'], c['op1'], c['op2'], c['cnt_c'], c['cnt_a'], c['factor']) for c in lv['contr']]
+          for lv in SCHEDULE]
+SYM_AFTER = [lv['sym'] for lv in SCHEDULE]
 
 
 class CCSD:
-    """Device-resident closed-shell CCSD driver.
+    """Device-resident closed-shell coupled-cluster driver.
 
-    ham: synthetic Hamiltonian (synth.Hamiltonian); device: torch device.
+    ham: Hamiltonian (synth.Hamiltonian / synth.DenseHamiltonian); device:
+    torch device; formula: a formula ({"lists", "constant", "items"}, see
+    formula.py), by default the CCSD formula above.  The formula's lists
+    must include the amplitudes T1, T2, the residuals R1, R2 and the energy
+    E; its constant lists are integral lists <c..||a..> named
+    H<spaces of C><spaces of A> as above (filled from ham).
     """
 
-    def __init__(self, ham, device='cuda:0', use_tr=True):
+    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None):
         import torch
         import gecco_hip
         self.torch = torch
         self.ham = ham
         self.dev = torch.device(device)
         self.use_tr = use_tr
+        self.formula = FORMULA if formula is None else formula
+        lists = self.formula['lists']
+        for n in ('T1', 'T2', 'R1', 'R2', 'E'):
+            if n not in lists:
+                raise ValueError(f'formula lacks the list {n}')
+        self.constant = list(self.formula['constant'])
         self.orbs = gecco_hip.Orbs(ham.irrep_o, ham.irrep_v, ham.nsym)
-        self.L = {n: gecco_hip.MEList(self.orbs, c, a) for n, (c, a) in LISTS.items()}
-        self.slot = {n: i for i, n in enumerate(LISTS)}
+        self.L = {n: gecco_hip.MEList(self.orbs, c, a) for n, (c, a) in lists.items()}
+        self.slot = {n: i for i, n in enumerate(lists)}
         # T1|T2 and R1|R2 are contiguous: the amplitude vector and the
         # residual (without its diagonal part) of the solver
         self.n1, self.n2 = self.L['T1'].length, self.L['T2'].length
         self.amp = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
         self.res = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
         self.buf = {}
-        for n in LISTS:
+        for n in lists:
             if n in ('T1', 'T2', 'R1', 'R2'):
                 v = self.amp if n[0] == 'T' else self.res
                 self.buf[n] = v[:self.n1] if n[1] == '1' else v[self.n1:]
@@ -166,30 +181,33 @@ class CCSD:
         self.plans = []
         self.flops = 0.0
         self.nblocks = 0
-        for li, level in enumerate(LEVELS):
+        sched = SCHEDULE if formula is None else _formula.schedule(self.formula['items'])
+        for level in sched:
             p = gecco_hip.Plan()
-            for n in INTEGRALS:           # fixed during the solve
+            for n in self.constant:       # fixed during the solve
                 p.set_constant(self.slot[n])
-            if li == 0:
-                # start of an iteration: intermediates from their sources or zero
-                for n in LISTS:
-                    if n in INIT:
-                        src, f = INIT[n]
-                        p.add_init(self.slot[n], self.L[n].length, buf_src=self.slot[src], factor=f)
-                    elif n not in INTEGRALS and n not in ('T1', 'T2'):
-                        p.add_init(self.slot[n], self.L[n].length)
-            for res, a, b, cc, ca, fact in level:
-                nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], cc, ca, fact, use_tr,
+            # list initialisations of the level: from a source or zero
+            for it in level['init']:
+                n = it['target']
+                if 'source' in it:
+                    p.add_init(self.slot[n], self.L[n].length, buf_src=self.slot[it['source']],
+                               factor=it['factor'])
+                else:
+                    p.add_init(self.slot[n], self.L[n].length)
+            for c in level['contr']:
+                res, a, b = c['target'], c['op1'], c['op2']
+                nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], c['cnt_c'], c['cnt_a'],
+                                           c['factor'], use_tr,
                                            buf=(self.slot[a], self.slot[b], self.slot[res]))
                 self.flops += fl
                 self.nblocks += nb
             if use_tr:
-                for res in SYM_AFTER[li]:
+                for res in level['sym']:
                     p.add_sym_ab(self.L[res], self.slot[res])
             p.finalize()
             self.plans.append(p)
         self.t_setup_plans = time.perf_counter() - t0
-        self.bufs = [self.buf[n] for n in LISTS]
+        self.bufs = [self.buf[n] for n in lists]
 
     # ------------------------------------------------------------ setup
     def _integrals(self):
@@ -203,7 +221,7 @@ class CCSD:
             B = torch.from_numpy(ham.B.reshape(ham.B.shape[0], n * n)).to(self.dev)
             eri = (B.T @ B).reshape(n, n, n, n)      # (pq|rs)
             del B
-        for name in INTEGRALS:
+        for name in self.constant:
             L = self.L[name]
             so = torch.from_numpy(L.spinorbs().astype(np.int64)).to(self.dev)
             sp = L.comp_spaces()

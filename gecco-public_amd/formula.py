@@ -1,0 +1,152 @@
+"""Formula lists for the engine: a serialised form of GeCCo's formula items
+and the schedule that turns them into grouped plan launches.
+
+The reference keeps a factorised formula as a linked list of items
+(include/def_formula_item.h:1-28) with a command code each
+(include/def_formula_item.h:28-46) and executes it item by item in
+contract/frm_sched2.f:117-365.  The items kept here, with their reference
+commands:
+
+  {"cmd": "INIT", "target": L}                  command_set_target_init (0):
+                                                L = 0
+  {"cmd": "INIT", "target": L, "source": S,     command_cp_intm (14) with a
+   "factor": f}                                 factor: L = f * S
+  {"cmd": "CONTR", "target": L, "op1": A,       command_add_bc (8): the binary
+   "op2": B, "cnt_c": {...}, "cnt_a": {...},    contraction of A and B over the
+   "factor": f}                                 strings of occupation cnt_c (on
+                                                A's creator side) and cnt_a (on
+                                                A's annihilator side), added to
+                                                L (contr_op1op2)
+  {"cmd": "SYMMETRISE", "target": L}            command_symmetrise (5): the
+                                                spin-flip projection of L
+                                                (contract/sym_ab_blk.f)
+
+Occupations are dicts over the spaces {"H": n, "P": n}.  A formula file is
+JSON: {"lists": {name: {"C": occ, "A": occ}}, "constant": [names],
+"items": [...]}.  Intermediates are created with the lists and kept for the
+whole formula (the engine's lists are device-resident; command_new_intermediate
+and command_del_intermediate have no work left to do).
+
+Schedule: items run in levels; the items of a level are independent and
+run as one grouped launch.  A CONTR item goes into the first level after
+every earlier (in list order) write of its operands -- all contributions to
+a list commute, so several writers share a level -- and not before an
+earlier item that reads its target.  An INIT goes after the earlier reads
+and writes of its target and before the later ones.  The spin-flip
+projection is linear and idempotent and the initial contents of every list
+are fixed points of it, so contributions of several levels are projected
+together: a list is projected at the end of the last level that writes it
+before a later level reads it, and residual lists (never read) after their
+last write; explicit SYMMETRISE items are thereby implied.
+"""
+import json
+
+H, P = 1, 2
+_SPACE = {'H': H, 'P': P}
+_NAME = {H: 'H', P: 'P'}
+
+
+def _occ_in(o):
+    return {_SPACE[k]: int(v) for k, v in o.items() if int(v)}
+
+
+def _occ_out(o):
+    return {_NAME[k]: int(v) for k, v in o.items() if int(v)}
+
+
+def contr(target, op1, op2, cnt_c, cnt_a, factor):
+    """A CONTR item (occupations keyed by the engine's space codes)."""
+    return {'cmd': 'CONTR', 'target': target, 'op1': op1, 'op2': op2,
+            'cnt_c': dict(cnt_c), 'cnt_a': dict(cnt_a), 'factor': float(factor)}
+
+
+def init(target, source=None, factor=1.0):
+    it = {'cmd': 'INIT', 'target': target}
+    if source is not None:
+        it.update(source=source, factor=float(factor))
+    return it
+
+
+def dump(formula, path):
+    """Write a formula ({"lists", "constant", "items"}) as JSON."""
+    out = {'lists': {n: {'C': _occ_out(c), 'A': _occ_out(a)} for n, (c, a) in formula['lists'].items()},
+           'constant': list(formula.get('constant', [])), 'items': []}
+    for it in formula['items']:
+        it = dict(it)
+        if it['cmd'] == 'CONTR':
+            it['cnt_c'] = _occ_out(it['cnt_c'])
+            it['cnt_a'] = _occ_out(it['cnt_a'])
+        out['items'].append(it)
+    with open(path, 'w') as f:
+        json.dump(out, f, indent=1)
+
+
+def load(path):
+    """Read a formula written by dump (or by hand in the same form)."""
+    with open(path) as f:
+        d = json.load(f)
+    lists = {n: (_occ_in(v['C']), _occ_in(v['A'])) for n, v in d['lists'].items()}
+    items = []
+    for it in d['items']:
+        if it['cmd'] not in ('INIT', 'CONTR', 'SYMMETRISE'):
+            raise ValueError(f'unknown formula command {it["cmd"]!r}')
+        if it['cmd'] == 'CONTR':
+            it = contr(it['target'], it['op1'], it['op2'], _occ_in(it['cnt_c']), _occ_in(it['cnt_a']),
+                       it['factor'])
+        items.append(it)
+    for it in items:
+        for key in ('target', 'op1', 'op2', 'source'):
+            if key in it and it[key] not in lists:
+                raise ValueError(f'formula item refers to undefined list {it[key]!r}')
+    return {'lists': lists, 'constant': list(d.get('constant', [])), 'items': items}
+
+
+def schedule(items):
+    """Levels of a formula: a list of dicts {"init": [INIT items],
+    "contr": [CONTR items], "sym": [list names]} in execution order (the
+    INITs of a level run before its contractions, the projections after
+    them).
+
+    A contraction reads its operands as they are at the start of its level,
+    so with items taken in list order:
+      read after write   an operand's earlier writes are in earlier levels;
+      write after read   the target's earlier readers are in the same or
+                         earlier levels;
+      after an INIT      the target's last INIT is in the same or an earlier
+                         level.
+    Contributions to one list commute and need no order among themselves.
+    An INIT follows the earlier reads and writes of its target and the
+    writes of its source."""
+    last_write, last_read, last_init = {}, {}, {}     # list -> level
+    inits, contrs = {}, {}
+    for it in items:
+        t = it['target']
+        if it['cmd'] == 'SYMMETRISE':
+            continue
+        if it['cmd'] == 'INIT':
+            src = it.get('source')
+            lv = max(last_read.get(t, -1) + 1, last_write.get(t, -1) + 1,
+                     last_write.get(src, -1) + 1 if src else 0)
+            inits.setdefault(lv, []).append(it)
+            last_write[t] = max(last_write.get(t, -1), lv)
+            last_init[t] = lv
+            if src:
+                last_read[src] = max(last_read.get(src, -1), lv)
+            continue
+        lv = max(last_write.get(it['op1'], -1) + 1, last_write.get(it['op2'], -1) + 1,
+                 last_read.get(t, 0), last_init.get(t, 0))
+        contrs.setdefault(lv, []).append(it)
+        last_write[t] = max(last_write.get(t, -1), lv)
+        for op in (it['op1'], it['op2']):
+            last_read[op] = max(last_read.get(op, -1), lv)
+    nlev = max(list(contrs) + list(inits) + [-1]) + 1
+    levels = [{'init': inits.get(l, []), 'contr': contrs.get(l, []), 'sym': []} for l in range(nlev)]
+    writes = [{c['target'] for c in lv['contr']} for lv in levels]
+    reads = [{c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']} for lv in levels]
+    for li, lv in enumerate(levels):
+        for res in dict.fromkeys(c['target'] for c in lv['contr']):
+            nxt = next((l for l in range(li + 1, nlev) if res in writes[l]), None)
+            last = nlev - 1 if nxt is None else nxt
+            if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
+                lv['sym'].append(res)
+    return levels

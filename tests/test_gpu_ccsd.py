@@ -118,3 +118,25 @@ def test_hf_fcidump_mp2_and_ccsd():
     e_ref, *_ = Dense(ham).solve(1e-11)
     assert res['residual'] < 1e-10
     assert abs(res['energy'] - e_ref) < 1e-10, (res['energy'], e_ref)
+
+
+def test_mp2_formula_file_gives_reference_mp2():
+    """The MP2 formula file (gecco-public_amd/formulas/mp2.json: R2 = <ab||ij>,
+    E = 1/4 <ij||ab> t_ij^ab) on the driver: one Jacobi step gives the MP2
+    amplitudes, the second iteration the reference's known MP2 energy of
+    HF/cc-pVDZ (test/python-interface.s.ref:629) and a zero residual."""
+    import json
+    import os
+    _torch()
+    import ccsd
+    import formula
+    from fcidump import hamiltonian_from_fcidump
+    here = os.path.dirname(__file__)
+    meta = json.load(open(os.path.join(here, 'golden', 'hf_hf_molpro.json')))
+    ham = hamiltonian_from_fcidump(os.path.join(here, 'golden', 'hf_hf_molpro_FCIDUMP.gz'),
+                                   meta['closed'], meta['frozen'])
+    f = formula.load(os.path.join(here, '..', 'gecco-public_amd', 'formulas', 'mp2.json'))
+    print('mp2 formula file', flush=True)
+    res = ccsd.CCSD(ham, formula=f).solve(tol=1e-12, maxit=5, mode='sync')
+    assert res['iterations'] == 2
+    assert abs(res['energy'] - meta['e_mp2_ref']) < 1e-11, res['energy']
