@@ -52,14 +52,6 @@ class DenseHamiltonian:
         return self.eri_full
 
 
-def load_hamiltonian(path):
-    """DenseHamiltonian from an .npz with eps_o, eps_v, irrep_o, irrep_v,
-    nsym, eri and e_scf (the format tests/golden/make_ch2_dalton.py writes)."""
-    z = np.load(path, allow_pickle=False)
-    return DenseHamiltonian(z['eps_o'], z['eps_v'], z['irrep_o'], z['irrep_v'], z['eri'],
-                            int(z['nsym']), float(z['e_scf']))
-
-
 def synthetic(nocc, nvir, naux, nsym=1, seed=0, sigma=0.03, occ_range=(-2.0, -0.3),
               vir_range=(0.2, 3.0)):
     rng = np.random.default_rng(seed)

@@ -80,11 +80,14 @@ def test_dense_ccsd_reproduces_reference_ch2_energy():
     """Known answer of the reference's own test: CH2/DZ (DALTON files of
     test/molecules/CH2-DZ-DALTON, frozen 1s), CCSD total energy
     -38.907618798245 (test/CCPT-energy.e.ref:690, printed to 1e-12)."""
+    import json
     import os
-    from synth import load_hamiltonian
-    path = os.path.join(os.path.dirname(__file__), 'golden', 'ch2_dz_dalton.npz')
-    ham = load_hamiltonian(path)
+    from dalton import hamiltonian_from_dalton
+    gold = os.path.join(os.path.dirname(__file__), 'golden')
+    ham = hamiltonian_from_dalton(os.path.join(gold, 'ch2_dz_SIRIFC.gz'),
+                                  os.path.join(gold, 'ch2_dz_MOTWOINT.gz'))
     assert (ham.nocc, ham.nvir, ham.nsym) == (3, 9, 4)
+    assert abs(ham.e_ref - (-38.821908746679)) < 1e-11      # E(SCF) of SIRIFC
     e, *_ = DenseCCSD(ham).solve(1e-11)
-    ref = float(np.load(path)['e_ccsd_ref'])
+    ref = json.load(open(os.path.join(gold, 'ch2_dz.json')))['e_ccsd_ref']
     assert abs(ham.e_ref + e - ref) < 5e-12

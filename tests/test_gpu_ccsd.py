@@ -43,12 +43,14 @@ def test_ch2_dz_matches_reference_energy():
     import os
     torch = _torch()
     import ccsd
-    from synth import load_hamiltonian
-    path = os.path.join(os.path.dirname(__file__), 'golden', 'ch2_dz_dalton.npz')
-    ham = load_hamiltonian(path)
+    import json
+    from dalton import hamiltonian_from_dalton
+    gold = os.path.join(os.path.dirname(__file__), 'golden')
+    ham = hamiltonian_from_dalton(os.path.join(gold, 'ch2_dz_SIRIFC.gz'),
+                                  os.path.join(gold, 'ch2_dz_MOTWOINT.gz'))
     print('ch2/dz ccsd', flush=True)
     res = ccsd.CCSD(ham).solve(tol=1e-10, maxit=100)
-    ref = float(np.load(path)['e_ccsd_ref'])
+    ref = json.load(open(os.path.join(gold, 'ch2_dz.json')))['e_ccsd_ref']
     assert abs(ham.e_ref + res['energy'] - ref) < 1e-10, (ham.e_ref + res['energy'], ref)
 
 
