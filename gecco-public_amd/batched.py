@@ -1,0 +1,113 @@
+"""Many independent coupled-cluster instances (different molecules or
+orbital spaces, uneven sizes) iterated together: the kernel calls of every
+instance's formula level run in one grouped launch, and the whole batched
+iteration -- the level launches and every instance's asynchronous DIIS step
+-- is captured once as a HIP graph and replayed.
+
+Each instance keeps its own ME-lists, integrals, preconditioner and DIIS
+subspace (ccsd.CCSD with plans=False); instance k's lists take the buffer
+slots k * nlists + list index of the shared plans.  An instance that has
+converged keeps its amplitudes (its DIIS step meets the tolerance again);
+the batch iterates until every instance has converged or maxit is reached.
+"""
+import time
+
+import ccsd
+
+
+class BatchedCCSD:
+    def __init__(self, hams, device='cuda:0', use_tr=True, formula=None):
+        import gecco_hip
+        import torch
+        self.torch = torch
+        self.inst = [ccsd.CCSD(h, device, use_tr, formula, plans=False) for h in hams]
+        nl = len(self.inst[0].formula['lists'])
+        self.nlists = nl
+        t0 = time.perf_counter()
+        self.plans = []
+        for li in range(len(self.inst[0].schedule)):
+            p = gecco_hip.Plan()
+            for k, c in enumerate(self.inst):
+                c.record_level(p, c.schedule[li], k * nl)
+            p.finalize()
+            self.plans.append(p)
+        self.t_setup_plans = time.perf_counter() - t0
+        self.bufs = [b for c in self.inst for b in c.bufs]
+        self.flops = sum(c.flops for c in self.inst)
+        self.nblocks = sum(c.nblocks for c in self.inst)
+        self._graph = None
+        self._diis = None
+
+    def evaluate(self, stream=None):
+        for p in self.plans:
+            p.execute(self.bufs, stream)
+
+    def solve(self, tol=1e-8, maxit=100, ndiis=8, mode='graph', restart=True):
+        """Iterate every instance to ||g|| < tol (at most maxit iterations);
+        mode 'graph' replays the captured batched iteration, 'async' launches
+        it eagerly.  Returns per-instance energies, iterations and residuals,
+        and the batch's wall time."""
+        import gecco_hip
+        torch = self.torch
+        if mode not in ('async', 'graph'):
+            raise ValueError(f'unknown solve mode {mode!r}')
+        if restart or self._diis is None:
+            for c in self.inst:
+                c.amp.zero_()
+            self._diis = [gecco_hip.DIIS(c.amp.numel(), ndiis) for c in self.inst]
+            self._steps = 0
+            self._graph = None
+        stream = torch.cuda.current_stream().cuda_stream
+
+        def iteration(s):
+            self.evaluate(s)
+            for c, d in zip(self.inst, self._diis):
+                d.step_async(c.amp, c.res, c.diag, energy=c.buf['E'], tol=tol, stream=s)
+
+        n = len(self.inst)
+        conv_at = [None] * n
+        energy = [None] * n
+        resid = [None] * n
+        events = []
+        base = self._steps
+        done = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph = self._graph[1] if (self._graph is not None and self._graph[0] == tol) else None
+        it = 0
+        for it in range(1, maxit + 1):
+            if graph is not None:
+                graph.replay()
+            else:
+                iteration(stream)
+                if mode == 'graph':
+                    torch.cuda.current_stream().synchronize()
+                    graph = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(graph):
+                        iteration(torch.cuda.current_stream().cuda_stream)
+                    self._graph = (tol, graph)
+            ev = torch.cuda.Event()
+            ev.record()
+            events.append(ev)
+            self._steps += 1
+            finished = False
+            while done < it - 1 or (it == maxit and done < it):
+                events[done].synchronize()
+                for k, d in enumerate(self._diis):
+                    if conv_at[k] is not None:
+                        continue
+                    g, e, code = d.status(base + done)
+                    energy[k], resid[k] = e, g
+                    if code != 0:
+                        conv_at[k] = done + 1
+                done += 1
+                if all(x is not None for x in conv_at):
+                    finished = True
+                    break
+            if finished:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return dict(energies=energy, iterations=[x if x is not None else it for x in conv_at],
+                    residuals=resid, converged=[x is not None for x in conv_at], seconds=wall,
+                    batch_iterations=done, flops_per_iter=self.flops, kernel_calls=self.nblocks)

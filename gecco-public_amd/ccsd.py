@@ -141,10 +141,12 @@ class CCSD:
     formula.py), by default the CCSD formula above.  The formula's lists
     must include the amplitudes T1, T2, the residuals R1, R2 and the energy
     E; its constant lists are integral lists <c..||a..> named
-    H<spaces of C><spaces of A> as above (filled from ham).
+    H<spaces of C><spaces of A> as above (filled from ham).  plans=False
+    sets up the lists, integrals and preconditioner only (for a batched
+    driver that builds the plans of several instances together).
     """
 
-    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None):
+    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True):
         import torch
         import gecco_hip
         self.torch = torch
@@ -181,33 +183,40 @@ class CCSD:
         self.plans = []
         self.flops = 0.0
         self.nblocks = 0
-        sched = SCHEDULE if formula is None else _formula.schedule(self.formula['items'])
-        for level in sched:
-            p = gecco_hip.Plan()
-            for n in self.constant:       # fixed during the solve
-                p.set_constant(self.slot[n])
-            # list initialisations of the level: from a source or zero
-            for it in level['init']:
-                n = it['target']
-                if 'source' in it:
-                    p.add_init(self.slot[n], self.L[n].length, buf_src=self.slot[it['source']],
-                               factor=it['factor'])
-                else:
-                    p.add_init(self.slot[n], self.L[n].length)
-            for c in level['contr']:
-                res, a, b = c['target'], c['op1'], c['op2']
-                nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], c['cnt_c'], c['cnt_a'],
-                                           c['factor'], use_tr,
-                                           buf=(self.slot[a], self.slot[b], self.slot[res]))
-                self.flops += fl
-                self.nblocks += nb
-            if use_tr:
-                for res in level['sym']:
-                    p.add_sym_ab(self.L[res], self.slot[res])
-            p.finalize()
-            self.plans.append(p)
+        self.schedule = SCHEDULE if formula is None else _formula.schedule(self.formula['items'])
+        if plans:
+            for level in self.schedule:
+                p = gecco_hip.Plan()
+                self.record_level(p, level, 0)
+                p.finalize()
+                self.plans.append(p)
         self.t_setup_plans = time.perf_counter() - t0
         self.bufs = [self.buf[n] for n in lists]
+
+    def record_level(self, p, level, base):
+        """Record one scheduled level of this instance into plan p, its
+        lists in buffer slots base + list index: constant slots, list
+        initialisations, contractions and spin-flip projections."""
+        for n in self.constant:       # fixed during the solve
+            p.set_constant(base + self.slot[n])
+        # list initialisations of the level: from a source or zero
+        for it in level['init']:
+            n = it['target']
+            if 'source' in it:
+                p.add_init(base + self.slot[n], self.L[n].length, buf_src=base + self.slot[it['source']],
+                           factor=it['factor'])
+            else:
+                p.add_init(base + self.slot[n], self.L[n].length)
+        for c in level['contr']:
+            res, a, b = c['target'], c['op1'], c['op2']
+            nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], c['cnt_c'], c['cnt_a'],
+                                       c['factor'], self.use_tr,
+                                       buf=(base + self.slot[a], base + self.slot[b], base + self.slot[res]))
+            self.flops += fl
+            self.nblocks += nb
+        if self.use_tr:
+            for res in level['sym']:
+                p.add_sym_ab(self.L[res], base + self.slot[res])
 
     # ------------------------------------------------------------ setup
     def _integrals(self):

@@ -142,3 +142,28 @@ def test_mp2_formula_file_gives_reference_mp2():
     res = ccsd.CCSD(ham, formula=f).solve(tol=1e-12, maxit=5, mode='sync')
     assert res['iterations'] == 2
     assert abs(res['energy'] - meta['e_mp2_ref']) < 1e-11, res['energy']
+
+
+def test_batched_instances_match_separate_solves():
+    """Four instances of different sizes and IRREP counts in one grouped
+    launch per level (and one captured graph per iteration) converge to the
+    energies of separate solves, each in its own number of iterations."""
+    _torch()
+    import ccsd
+    from batched import BatchedCCSD
+    cases = [(2, 5, 1, 3), (3, 7, 2, 4), (4, 9, 4, 5), (3, 8, 2, 6)]
+    hams = [synthetic(o, v, 3 * v, nsym=s, seed=seed, sigma=0.1) for o, v, s, seed in cases]
+    print('batched ccsd', cases, flush=True)
+    sep = [ccsd.CCSD(h).solve(tol=1e-10, maxit=80) for h in hams]
+    b = BatchedCCSD(hams)
+    res = b.solve(tol=1e-10, maxit=80)
+    assert all(res['converged'])
+    for k, r in enumerate(sep):
+        assert abs(res['energies'][k] - r['energy']) < 1e-11, (k, res['energies'][k], r['energy'])
+        assert res['iterations'][k] == r['iterations'], (k, res['iterations'], r['iterations'])
+    assert res['batch_iterations'] == max(r['iterations'] for r in sep) + 1 or \
+        res['batch_iterations'] == max(r['iterations'] for r in sep)
+    # a second solve reuses the captured graph
+    res2 = b.solve(tol=1e-10, maxit=80)
+    assert res2['energies'] == res['energies'] or \
+        max(abs(x - y) for x, y in zip(res2['energies'], res['energies'])) < 1e-12
