@@ -14,6 +14,12 @@ symmetrisation) and the native DIIS step, operands resident in HBM.  After the t
 --workload ladder: the particle-particle ladder contraction alone (8 kernel
 calls, 5.58 GFLOP) at the same sizes.
 
+--workload batched: 64 independent CCSD instances of uneven size (nocc ~
+U[8,24], nvir ~ U[40,160], naux = 3 nvir, 4 IRREPs, seeds 0..63) in one
+grouped launch per formula level and one replayed graph per batched
+iteration; also their time to convergence against solving them one after
+another.
+
 Prints one JSON line.  --gpus N > 1 is not supported: the engine runs on one
 GPU here.
 """
@@ -147,6 +153,42 @@ def run_ladder(args):
 
 
 # --------------------------------------------------------------------- ccsd
+def kernel_times(plans, run, steps):
+    """Kernel durations for the roofline: run() launches `steps` iterations
+    one by one (the same kernels as the replayed graph) with HIP-event timing
+    on the launch stream.  Launch classes: 0 strided-GEMM launch (with the
+    direct short-contraction tasks), 1 gather kernel, 2 spin-flip
+    symmetrisation, 3 operand materialisation, 4 unused.  The dominant
+    contraction kernel by time is the strided-GEMM launch (its GEMM tiles and
+    the direct tasks that share it: FLOP classes 0 and 2) or the gather
+    kernel (FLOP class 1)."""
+    import torch
+    for p in plans:
+        p.set_timing(True)
+        p.timing()
+    run()
+    torch.cuda.synchronize()
+    ms = [0.0] * 5
+    nl = [0] * 5
+    fl = [0.0] * 3
+    gbytes = 0.0
+    for p in plans:
+        tm = p.timing()
+        p.set_timing(False)
+        gbytes += p.workspace()['gather_bytes']
+        for i in range(5):
+            ms[i] += tm['ms'][i]
+            nl[i] += tm['launches'][i]
+        for i in range(3):
+            fl[i] += tm['flops'][i]
+    cand = [(ms[0], 0, fl[0] + fl[2]), (ms[1], 1, fl[1])]
+    _, dom, dom_fl = max(cand)
+    dom_ms = ms[dom] / max(1, nl[dom])
+    achieved = dom_fl / (ms[dom] / steps * 1e-3) / 1e12 if ms[dom] > 0 else 0.0
+    return dict(ms=ms, launches=nl, names=['grouped_contract_affine_f64', 'grouped_contract_f64'],
+                dom=dom, dom_ms=dom_ms, achieved=achieved, gather_bytes=gbytes)
+
+
 def cpu_ccsd_sample(ham, flops_iter):
     """Oracle route 3 (one core) on the last level of the CCSD formula (the
     residual contractions, about half of an iteration's FLOP); returns the
@@ -197,36 +239,10 @@ def run_ccsd(args):
     # kernel durations (roofline): as many further iterations launched one by
     # one (the same kernels as the replayed graph), timed with HIP events on
     # the launch stream
-    for p in solver.plans:
-        p.set_timing(True)
-        p.timing()
-    solver.solve(tol=0.0, maxit=args.steps, restart=False, mode='async')
-    torch.cuda.synchronize()
-    # launch classes: 0 strided-GEMM launch (with the direct short-contraction
-    # tasks), 1 gather kernel, 2 spin-flip symmetrisation, 3 operand
-    # materialisation, 4 unused
-    ms = [0.0] * 5
-    nl = [0] * 5
-    fl = [0.0] * 3
-    gbytes = 0.0
-    for p in solver.plans:
-        tm = p.timing()
-        p.set_timing(False)
-        gbytes += p.workspace()['gather_bytes']
-        for i in range(5):
-            ms[i] += tm['ms'][i]
-            nl[i] += tm['launches'][i]
-        for i in range(3):
-            fl[i] += tm['flops'][i]
-    # dominant contraction kernel by time: the strided-GEMM launch (its GEMM
-    # tiles and the direct short-contraction tasks that share it: FLOP
-    # classes 0 and 2) or the gather kernel (FLOP class 1)
-    cand = [(ms[0], 0, fl[0] + fl[2]), (ms[1], 1, fl[1])]
-    _, dom_ms_idx, dom_fl = max(cand)
-    names = ['grouped_contract_affine_f64', 'grouped_contract_f64']
-    dom = dom_ms_idx
-    dom_ms = ms[dom_ms_idx] / max(1, nl[dom_ms_idx])
-    achieved = dom_fl / (ms[dom_ms_idx] / args.steps * 1e-3) / 1e12 if ms[dom_ms_idx] > 0 else 0.0
+    kt = kernel_times(solver.plans, lambda: solver.solve(tol=0.0, maxit=args.steps, restart=False,
+                                                        mode='async'), args.steps)
+    ms, names, dom, dom_ms, achieved, gbytes = (kt[k] for k in ('ms', 'names', 'dom', 'dom_ms',
+                                                                'achieved', 'gather_bytes'))
     # time to convergence from T = 0
     torch.cuda.synchronize()
     res = solver.solve(tol=args.tol, maxit=200)
@@ -273,12 +289,117 @@ def run_ccsd(args):
     return out
 
 
+# ------------------------------------------------------------------ batched
+def run_batched(args):
+    """--workload batched: args.instances independent CCSD instances of
+    uneven size (synth.batch_recipe: nocc ~ U[8,24], nvir ~ U[40,160],
+    naux = 3 nvir, 4 IRREPs, seeds 0..count-1) iterated together, one grouped
+    launch per formula level for all instances and one replayed graph per
+    batched iteration.  One step = one batched iteration (every instance
+    advances by one).  The batch is then solved to convergence from T = 0,
+    and the same instances are solved one after another (each with its own
+    plans and graph) for the batching's gain."""
+    import torch
+    from batched import BatchedCCSD
+    from synth import synthetic, batch_recipe
+    recipe = batch_recipe(args.instances, nsym=args.nsym, sigma=args.sigma)
+    t0 = time.perf_counter()
+    hams = [synthetic(**r) for r in recipe]
+
+    def progress(k, c):
+        if k % 8 == 7 or k == len(hams) - 1:
+            print(f'# instance {k + 1}/{len(hams)} set up ({time.perf_counter() - t0:.1f} s)',
+                  file=sys.stderr, flush=True)
+    b = BatchedCCSD(hams, on_instance=progress)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f'# batched setup: {len(hams)} instances, {b.flops / 1e9:.1f} GFLOP and {b.nblocks} kernel '
+          f'calls per batched iteration, {setup_s:.1f} s', file=sys.stderr, flush=True)
+    b.solve(tol=0.0, maxit=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.solve(tol=0.0, maxit=args.steps, restart=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = kernel_times(b.plans, lambda: b.solve(tol=0.0, maxit=args.steps, restart=False, mode='async'),
+                      args.steps)
+    res = b.solve(tol=args.tol, maxit=200, timing=True)
+    print(f'# batched solve: {res["batch_iterations"]} batched iterations, {res["seconds"]:.3f} s',
+          file=sys.stderr, flush=True)
+    # the same instances one after another (per-instance plans and graphs)
+    seq_s, seq_iters, seq_dE = 0.0, 0, 0.0
+    seq_ms = []             # per-iteration milliseconds of each instance alone
+    for k, c in enumerate(b.inst):
+        if k % 16 == 15:
+            print(f'# sequential solve {k + 1}/{len(b.inst)}', file=sys.stderr, flush=True)
+        c.build_plans()
+        r = c.solve(tol=args.tol, maxit=200)
+        seq_s += r['seconds']
+        seq_iters += r['iterations']
+        seq_ms.append(1e3 * r['seconds'] / max(1, r['iterations']))
+        seq_dE = max(seq_dE, abs(r['energy'] - res['energies'][k]))
+        c.plans = []
+        c._graph = None
+    print(f'# sequential solves: {seq_iters} iterations, {seq_s:.3f} s', file=sys.stderr, flush=True)
+    # batched iteration i against the same instances' iterations alone
+    # (instance k is active in batched iterations 1..iterations[k])
+    tail = []
+    for i, ms_i in enumerate(res['iteration_ms']):
+        act = [k for k, n in enumerate(res['iterations']) if n > i]
+        tail.append((i + 1, len(act), round(ms_i, 3), round(sum(seq_ms[k] for k in act), 3)))
+    for row in tail[::8] + ([tail[-1]] if (len(tail) - 1) % 8 else []):
+        print('# batched iteration %d: %d active, %.3f ms (alone: %.3f ms)' % row, file=sys.stderr,
+              flush=True)
+    ms, names, dom = kt['ms'], kt['names'], kt['dom']
+    sizes = [(r['nocc'], r['nvir']) for r in recipe]
+    out = {
+        'metric': METRIC, 'value': round(args.steps / wall, 3), 'unit': 'batched iters/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / args.steps, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': f'synthetic DF Hamiltonians (seeds 0..{args.instances - 1}, naux = 3 nvir, '
+                f'sigma {args.sigma})',
+        'config': {'workload': 'ccsd_batched_instances', 'instances': args.instances,
+                   'nocc_range': [8, 24], 'nvir_range': [40, 160], 'nsym': args.nsym,
+                   'use_tr': True, 'sizes': sizes, 'kernel_calls_per_iter': b.nblocks,
+                   'gflop_per_iter': round(b.flops / 1e9, 3)},
+        'instance_iters_per_s': round(args.instances * args.steps / wall, 2),
+        'contraction_tflops': round(b.flops * args.steps / wall / 1e12, 3),
+        'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
+                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
+                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
+                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'roofline': {'bound': 'mfma', 'achieved': round(kt['achieved'], 4), 'peak': PEAK_FP64,
+                     'unit': 'TFLOP/s', 'frac': round(kt['achieved'] / PEAK_FP64, 4),
+                     'traffic': MEASURED_TRAFFIC.get(('batched', args.instances, args.nsym, names[dom])),
+                     'kernel': names[dom], 'kernel_ms': round(kt['dom_ms'], 5)},
+        'convergence': {'tol': args.tol, 'batch_iterations': res['batch_iterations'],
+                        'seconds': round(res['seconds'], 4), 'all_converged': all(res['converged']),
+                        'iterations': res['iterations'],
+                        'sequential_seconds': round(seq_s, 4), 'sequential_iterations': seq_iters,
+                        'speedup_vs_sequential': round(seq_s / res['seconds'], 3),
+                        'batched_gpu_ms': round(sum(res['iteration_ms']), 2),
+                        'max_abs_dE_vs_sequential': seq_dE},
+        'setup_seconds': round(setup_s, 2),
+    }
+    if not args.no_cpu:
+        mid = sorted(range(len(hams)), key=lambda k: b.inst[k].flops)[len(hams) // 2]
+        v, dt, sample = cpu_ccsd_sample(hams[mid], b.flops)
+        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'batched iters/s', 'cores': 1,
+                               'kind': 'port',
+                               'sample': f'instance {mid} (nocc {sizes[mid][0]}, nvir {sizes[mid][1]}): '
+                                         + sample.replace('one iteration', 'one batched iteration'),
+                               'seconds': round(dt, 2), 'cpu': platform.processor() or platform.machine()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=30)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', choices=['ccsd', 'ladder'], default='ccsd')
+    ap.add_argument('--workload', choices=['ccsd', 'ladder', 'batched'], default='ccsd')
+    ap.add_argument('--instances', type=int, default=64, help='batched workload: instance count')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--nocc', type=int, default=NOCC)
     ap.add_argument('--nvir', type=int, default=NVIR)
@@ -304,7 +425,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
         return
-    out = run_ladder(args) if args.workload == 'ladder' else run_ccsd(args)
+    out = {'ladder': run_ladder, 'ccsd': run_ccsd, 'batched': run_batched}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -6,9 +6,13 @@ iteration -- the level launches and every instance's asynchronous DIIS step
 
 Each instance keeps its own ME-lists, integrals, preconditioner and DIIS
 subspace (ccsd.CCSD with plans=False); instance k's lists take the buffer
-slots k * nlists + list index of the shared plans.  An instance that has
-converged keeps its amplitudes (its DIIS step meets the tolerance again);
-the batch iterates until every instance has converged or maxit is reached.
+slots k * nlists + list index of the shared plans and it is instance group
+k of every plan.  Each instance's asynchronous DIIS step sets its entry of a
+device flag array when it meets the tolerance (leaving the amplitudes
+unchanged), and every launch skips the work of the flagged instances, so a
+converged instance costs nothing further while the replayed graph stays the
+same; the batch iterates until every instance has converged or maxit is
+reached.
 """
 import time
 
@@ -16,20 +20,34 @@ import ccsd
 
 
 class BatchedCCSD:
-    def __init__(self, hams, device='cuda:0', use_tr=True, formula=None):
+    def __init__(self, hams, device='cuda:0', use_tr=True, formula=None, on_instance=None,
+                 skip_converged=True):
+        """on_instance(k, instance), if given, is called after instance k's
+        lists and integrals are set up.  skip_converged=False keeps
+        evaluating converged instances."""
         import gecco_hip
         import torch
         self.torch = torch
-        self.inst = [ccsd.CCSD(h, device, use_tr, formula, plans=False) for h in hams]
+        self.inst = []
+        for k, h in enumerate(hams):
+            self.inst.append(ccsd.CCSD(h, device, use_tr, formula, plans=False))
+            if on_instance is not None:
+                on_instance(k, self.inst[-1])
         nl = len(self.inst[0].formula['lists'])
         self.nlists = nl
         t0 = time.perf_counter()
+        # converged flags, one per instance (set by the DIIS steps)
+        self.done = torch.zeros(len(self.inst), dtype=torch.int32, device=device)
+        self.skip_converged = skip_converged
         self.plans = []
         for li in range(len(self.inst[0].schedule)):
             p = gecco_hip.Plan()
             for k, c in enumerate(self.inst):
+                p.set_group(k)
                 c.record_level(p, c.schedule[li], k * nl)
             p.finalize()
+            if skip_converged:
+                p.set_skip(self.done)
             self.plans.append(p)
         self.t_setup_plans = time.perf_counter() - t0
         self.bufs = [b for c in self.inst for b in c.bufs]
@@ -42,11 +60,12 @@ class BatchedCCSD:
         for p in self.plans:
             p.execute(self.bufs, stream)
 
-    def solve(self, tol=1e-8, maxit=100, ndiis=8, mode='graph', restart=True):
+    def solve(self, tol=1e-8, maxit=100, ndiis=8, mode='graph', restart=True, timing=False):
         """Iterate every instance to ||g|| < tol (at most maxit iterations);
         mode 'graph' replays the captured batched iteration, 'async' launches
         it eagerly.  Returns per-instance energies, iterations and residuals,
-        and the batch's wall time."""
+        and the batch's wall time (timing=True: also the GPU milliseconds of
+        every batched iteration, 'iteration_ms')."""
         import gecco_hip
         torch = self.torch
         if mode not in ('async', 'graph'):
@@ -55,6 +74,9 @@ class BatchedCCSD:
             for c in self.inst:
                 c.amp.zero_()
             self._diis = [gecco_hip.DIIS(c.amp.numel(), ndiis) for c in self.inst]
+            self.done.zero_()
+            for k, d in enumerate(self._diis):
+                d.set_flag(self.done[k:k + 1])
             self._steps = 0
             self._graph = None
         stream = torch.cuda.current_stream().cuda_stream
@@ -73,7 +95,10 @@ class BatchedCCSD:
         done = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        graph = self._graph[1] if (self._graph is not None and self._graph[0] == tol) else None
+        ev0 = torch.cuda.Event(enable_timing=timing)
+        ev0.record()
+        graph = self._graph[1] if (mode == 'graph' and self._graph is not None and
+                                   self._graph[0] == tol) else None
         it = 0
         for it in range(1, maxit + 1):
             if graph is not None:
@@ -86,7 +111,7 @@ class BatchedCCSD:
                     with torch.cuda.graph(graph):
                         iteration(torch.cuda.current_stream().cuda_stream)
                     self._graph = (tol, graph)
-            ev = torch.cuda.Event()
+            ev = torch.cuda.Event(enable_timing=timing)
             ev.record()
             events.append(ev)
             self._steps += 1
@@ -108,6 +133,9 @@ class BatchedCCSD:
                 break
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        return dict(energies=energy, iterations=[x if x is not None else it for x in conv_at],
-                    residuals=resid, converged=[x is not None for x in conv_at], seconds=wall,
-                    batch_iterations=done, flops_per_iter=self.flops, kernel_calls=self.nblocks)
+        out = dict(energies=energy, iterations=[x if x is not None else it for x in conv_at],
+                   residuals=resid, converged=[x is not None for x in conv_at], seconds=wall,
+                   batch_iterations=done, flops_per_iter=self.flops, kernel_calls=self.nblocks)
+        if timing:
+            out['iteration_ms'] = [a.elapsed_time(b) for a, b in zip([ev0] + events[:-1], events)]
+        return out

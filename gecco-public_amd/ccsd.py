@@ -185,13 +185,23 @@ class CCSD:
         self.nblocks = 0
         self.schedule = SCHEDULE if formula is None else _formula.schedule(self.formula['items'])
         if plans:
-            for level in self.schedule:
-                p = gecco_hip.Plan()
-                self.record_level(p, level, 0)
-                p.finalize()
-                self.plans.append(p)
+            self.build_plans()
         self.t_setup_plans = time.perf_counter() - t0
         self.bufs = [self.buf[n] for n in lists]
+
+    def build_plans(self):
+        """One plan per scheduled level of this instance alone (buffer slots
+        = list indices)."""
+        import gecco_hip
+        self.plans = []
+        self.flops = 0.0
+        self.nblocks = 0
+        for level in self.schedule:
+            p = gecco_hip.Plan()
+            self.record_level(p, level, 0)
+            p.finalize()
+            self.plans.append(p)
+        self._graph = None
 
     def record_level(self, p, level, base):
         """Record one scheduled level of this instance into plan p, its

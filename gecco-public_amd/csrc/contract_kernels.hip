@@ -121,9 +121,14 @@ __device__ __forceinline__ double op_fix(double raw, uint32_t flags, int r) {
     return ((flags >> r) & 1u) ? v : 0.0;
 }
 
+// Instance groups: a task of group g >= 0 does nothing while skip[g] != 0.
+__device__ __forceinline__ bool skipped(const int32_t *__restrict__ skip, int grp) {
+    return skip != nullptr && grp >= 0 && ldg(skip + grp) != 0;
+}
+
 __global__ __launch_bounds__(NT) void grouped_contract_f64(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs) {
+    double *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
     __shared__ double As[BK][BM + 1];
     __shared__ double Bs[BK][BN + 1];
     const int tid = threadIdx.x;
@@ -134,6 +139,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
+        if (skipped(skip, task.grp)) continue;
         const DevBlock &b = blocks[task.blk];
         const int M = b.M, N = b.N;
         const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
@@ -561,7 +567,7 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 template <int TBM, int TBN, int TBK>
 __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine_f64(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs) {
+    double *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int TM = TBM / 32, TN = TBN / 32;
     constexpr int SA = TBK * LDA, SB = TBK * LDB;
@@ -580,6 +586,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
+        if (skipped(skip, task.grp)) continue;
         if (task.pad == 1) {   // tile of at most 16 x 16
             skinny_task(task, blocks, bufs, lds, tid);
             continue;
@@ -717,8 +724,10 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
 // plan); HBM-bound: one read of each allowed source element, one write.
 __global__ __launch_bounds__(256) void gather_operands_f64(
     const DevGather *__restrict__ jobs, const DevGatherTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs, double *__restrict__ scratch) {
+    double *const *__restrict__ bufs, double *__restrict__ scratch,
+    const int32_t *__restrict__ skip) {
     const DevGatherTask t = tasks[blockIdx.x];
+    if (skipped(skip, t.grp)) return;
     const DevGather &g = jobs[t.job];
     const double *__restrict__ src = bufs[g.src_buf] + g.src_base;
     double *__restrict__ dst = scratch + g.dst_base;
@@ -774,8 +783,10 @@ __global__ __launch_bounds__(256) void gather_operands_f64(
 // List initialisation (copy with factor, or zero) ahead of the contractions.
 __global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict__ jobs,
                                                       const DevInitTask *__restrict__ tasks,
-                                                      double *const *__restrict__ bufs) {
+                                                      double *const *__restrict__ bufs,
+                                                      const int32_t *__restrict__ skip) {
     const DevInitTask t = tasks[blockIdx.x];
+    if (skipped(skip, t.grp)) return;
     const DevInit &j = jobs[t.job];
     double *__restrict__ dst = bufs[j.dst] + j.off_dst;
     // U elements per thread and round, their loads in flight together
@@ -829,8 +840,10 @@ __global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__re
                                                           double *const *__restrict__ bufs,
                                                           const int32_t *__restrict__ i1,
                                                           const int32_t *__restrict__ i2,
-                                                          const int8_t *__restrict__ fsign) {
+                                                          const int8_t *__restrict__ fsign,
+                                                          const int32_t *__restrict__ skip) {
     const DevSymTask t = tasks[blockIdx.x];
+    if (skipped(skip, t.grp)) return;
     const DevSymStep &st = steps[t.step];
     double *__restrict__ x = bufs[st.buf] + st.base;
     const double prefac = st.prefac;
@@ -847,10 +860,10 @@ __global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__re
 
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, double *const *bufs, int grid,
-                                    hipStream_t stream) {
+                                    const int32_t *skip, hipStream_t stream) {
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::grouped_contract_f64, dim3(grid), dim3(gecco::NT), 0, stream,
-                       blocks, tasks, ntasks, bufs);
+                       blocks, tasks, ntasks, bufs, skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -868,16 +881,16 @@ extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            double *const *bufs, int grid, int variant,
-                                           hipStream_t stream) {
+                                           const int32_t *skip, hipStream_t stream) {
     using namespace gecco;
     if (ntasks <= 0) return 0;
     switch (variant) {
-    case 0: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 32>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
-    case 1: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
-    case 2: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
-    case 3: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
-    case 4: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
-    case 5: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 8>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs); break;
+    case 0: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 32>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 1: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 2: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 3: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 4: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 5: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 8>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     default: return 1;
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;
@@ -888,17 +901,19 @@ extern "C" int gecco_direct_stage_cap(void) { return gecco::DIRECT_STAGE; }
 
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, double *const *bufs, double *scratch,
-                                   hipStream_t stream) {
+                                   const int32_t *skip, hipStream_t stream) {
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::gather_operands_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                       ntasks, bufs, scratch);
+                       ntasks, bufs, scratch, skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
-                                 int ntasks, double *const *bufs, hipStream_t stream) {
+                                 int ntasks, double *const *bufs, const int32_t *skip,
+                                 hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::init_lists_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks, bufs);
+    hipLaunchKernelGGL(gecco::init_lists_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks, bufs,
+                       skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -915,10 +930,11 @@ extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t 
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
                                         double *const *bufs, const int32_t *i1, const int32_t *i2,
-                                        const int8_t *fsign, hipStream_t stream) {
+                                        const int8_t *fsign, const int32_t *skip,
+                                        hipStream_t stream) {
     if (ntasks <= 0) return 0;
     hipLaunchKernelGGL(gecco::sym_ab_batched_f64, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
-                       bufs, i1, i2, fsign);
+                       bufs, i1, i2, fsign, skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 

@@ -51,6 +51,9 @@ struct DevTask {
     // and then written back as one contiguous range -- read-modify-write if
     // the task owns the range (owned = 1), else atomically
     int32_t span, owned, pad;
+    // instance group: the task is skipped while skip[grp] != 0 (skip flags
+    // given to the plan); grp < 0 never skips
+    int32_t grp, pad2;
     int64_t lo;
 };
 
@@ -76,7 +79,7 @@ struct DevInit {
 };
 
 struct DevInitTask {
-    int32_t job, pad;
+    int32_t job, grp;      // grp: instance group (see DevTask)
     int64_t e0, e1;
 };
 
@@ -89,14 +92,14 @@ struct DevSymStep {
 };
 
 struct DevSymTask {
-    int32_t step, pad;
+    int32_t step, grp;     // grp: instance group (see DevTask)
     int64_t p0, p1;        // pair range (global pair indices)
 };
 
 struct DevGatherTask {
     int32_t job;
     int32_t e0, e1;        // element range [e0, e1) of the job's dense array
-    int32_t pad;
+    int32_t grp;           // instance group (see DevTask)
 };
 
 }  // namespace gecco

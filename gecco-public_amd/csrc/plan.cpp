@@ -32,25 +32,27 @@
 
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, double *const *bufs, int grid,
-                                    hipStream_t stream);
+                                    const int32_t *skip, hipStream_t stream);
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            double *const *bufs, int grid, int variant,
-                                           hipStream_t stream);
+                                           const int32_t *skip, hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_direct_stage_cap(void);
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
                                         double *const *bufs, const int32_t *i1, const int32_t *i2,
-                                        const int8_t *fsign, hipStream_t stream);
+                                        const int8_t *fsign, const int32_t *skip,
+                                        hipStream_t stream);
 extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
                                   const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
                                   int64_t nrows, double fact, hipStream_t stream);
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
-                                 int ntasks, double *const *bufs, hipStream_t stream);
+                                 int ntasks, double *const *bufs, const int32_t *skip,
+                                 hipStream_t stream);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, double *const *bufs, double *scratch,
-                                   hipStream_t stream);
+                                   const int32_t *skip, hipStream_t stream);
 
 namespace gecco {
 
@@ -414,6 +416,7 @@ struct HostBlock {
     bool skinny = false;              // strided-GEMM launch, 16 x 16 wave-split-K tiles
     bool mat_a = false, mat_b = false;  // A / B materialised into scratch
     int32_t next = -1;                  // next block of its chain (finalize)
+    int32_t group = 0;                  // instance group (skip flags)
     HostOperand A, B, C;
 };
 
@@ -421,6 +424,7 @@ struct HostBlock {
 
 struct SymStep {
     int32_t buf = 0;
+    int32_t group = 0;      // instance group (skip flags)
     double prefac = 0.5;
     std::vector<int64_t> i1, i2;
     std::vector<int8_t> f;
@@ -443,6 +447,7 @@ struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
     std::vector<TraceStep> traces;                // after the contractions
     std::vector<gecco::DevInit> inits;            // list initialisation, first launch
+    std::vector<int32_t> init_group;              // instance group of each init
     gecco::DevInit *d_inits = nullptr;
     gecco::DevInitTask *d_itasks = nullptr;
     int64_t nitasks = 0;
@@ -498,6 +503,12 @@ struct gecco_hip_plan {
     int64_t launches[5] = {0, 0, 0, 0, 0};
     double flops_affine = 0, flops_general = 0, flops_direct = 0;
     int64_t direct_kmax = 32;   // K bound of the direct kernel (0 disables it)
+    // instance groups: items recorded after set_group(g) belong to group g;
+    // with skip flags set, every launch skips the tasks of group g while
+    // skip[g] != 0 (device memory, read when each task starts)
+    int32_t cur_group = 0, max_group = 0;
+    const int32_t *d_skip = nullptr;
+    int64_t nskip = 0;
 };
 
 namespace gecco {
@@ -713,6 +724,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         std::vector<size_t> tail(p->blocks.size());
         auto same_result = [&](const HostBlock &x, const HostBlock &y) {
             return x.buf12 == y.buf12 && x.off12 == y.off12 && x.M == y.M && x.N == y.N &&
+                   x.group == y.group &&
                    x.xfac == y.xfac && x.injective == y.injective && x.C.r.table == y.C.r.table &&
                    x.C.r.first == y.C.r.first && x.C.r.second == y.C.r.second;
         };
@@ -884,6 +896,8 @@ static int plan_finalize(gecco_hip_plan *p) {
             // direct tasks: columns per task; strided-GEMM launch: 1 for a
             // 16 x 16 wave-split-K task
             task.pad = b.direct ? dpack[i] : b.skinny ? 1 : 0;
+            task.grp = b.group;
+            task.pad2 = 0;
             task.lo = 0;
             const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
             const int64_t n1 = std::min<int64_t>(b.N, task.n0 + tile_n(i));
@@ -967,7 +981,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     {
         std::vector<DevGather> jobs;
         std::vector<const HostOperand *> job_src;
-        std::vector<int32_t> job_buf;
+        std::vector<int32_t> job_buf, job_grp;
         std::vector<int64_t> job_base;
         std::unordered_map<uint64_t, std::vector<size_t>> seen;
         const int32_t slot = (int32_t)nb_max;
@@ -992,7 +1006,7 @@ static int plan_finalize(gecco_hip_plan *p) {
                 size_t j = jobs.size();
                 for (size_t c : seen[h]) {
                     const HostOperand *o2 = job_src[c];
-                    if (job_buf[c] == buf && job_base[c] == base && jobs[c].K == K &&
+                    if (job_buf[c] == buf && job_base[c] == base && job_grp[c] == b.group && jobs[c].K == K &&
                         jobs[c].X == X && jobs[c].kfast == kfast && o2->r.table == ho.r.table &&
                         o2->r.first == ho.r.first && o2->r.second == ho.r.second) {
                         j = c;
@@ -1013,6 +1027,7 @@ static int plan_finalize(gecco_hip_plan *p) {
                     jobs.push_back(g);
                     job_src.push_back(&ho);
                     job_buf.push_back(buf);
+                    job_grp.push_back(b.group);
                     job_base.push_back(base);
                     seen[h].push_back(j);
                 }
@@ -1034,13 +1049,15 @@ static int plan_finalize(gecco_hip_plan *p) {
         auto is_const = [&](int32_t slot) {
             return slot >= 0 && slot < (int32_t)p->constant_slot.size() && p->constant_slot[slot];
         };
-        // gathers of constant slots first: they run only when needed
+        // gathers of constant slots first: they run only when needed (and
+        // are never skipped: group -1)
         for (int pass = 0; pass < 2; ++pass)
             for (size_t j = 0; j < jobs.size(); ++j) {
                 if (is_const(jobs[j].src_buf) != (pass == 0)) continue;
                 const int64_t n = (int64_t)jobs[j].K * jobs[j].X;
                 for (int64_t e0 = 0; e0 < n; e0 += chunk)
-                    gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk), 0});
+                    gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk),
+                                  pass == 0 ? -1 : job_grp[j]});
                 if (pass == 1) gbytes += 16.0 * (double)n;
                 if (pass == 0) p->ngtasks_static = (int64_t)gt.size();
             }
@@ -1085,7 +1102,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         const int64_t chunk = 2048;   // elements per workgroup (8 per thread)
         for (size_t j = 0; j < p->inits.size(); ++j)
             for (int64_t e0 = 0; e0 < p->inits[j].len; e0 += chunk)
-                it.push_back({(int32_t)j, 0, e0, std::min(p->inits[j].len, e0 + chunk)});
+                it.push_back({(int32_t)j, p->init_group[j], e0, std::min(p->inits[j].len, e0 + chunk)});
         p->nitasks = (int64_t)it.size();
         if (p->nitasks &&
             (check(hipMalloc(&p->d_inits, p->inits.size() * sizeof(DevInit)), "hipMalloc(inits)") ||
@@ -1123,7 +1140,7 @@ static int plan_finalize(gecco_hip_plan *p) {
                 ff.push_back(y.f[k]);
             }
             for (int64_t p0 = 0; p0 < y.npair; p0 += chunk)
-                stasks.push_back({(int32_t)steps.size(), 0, st.pair0 + p0, st.pair0 + std::min(y.npair, p0 + chunk)});
+                stasks.push_back({(int32_t)steps.size(), y.group, st.pair0 + p0, st.pair0 + std::min(y.npair, p0 + chunk)});
             steps.push_back(st);
             std::vector<int64_t>().swap(y.i1);
             std::vector<int64_t>().swap(y.i2);
@@ -1171,6 +1188,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
     for (int64_t i = 0; i < p->nbufs_max; ++i)
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
+    if (p->d_skip && !p->traces.empty())
+        return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace steps");
     if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty()) return ok();
     std::vector<double *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
@@ -1193,13 +1212,13 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         return 0;
     };
     if (p->nitasks &&
-        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, s))
+        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, p->d_skip, s))
         return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
     const int64_t g0 = p->static_done ? p->ngtasks_static : 0;
     if (p->ngtasks > g0) {
         if (mark(3, true)) return g_err;
         if (gecco_launch_gather(p->d_gjobs, p->d_gtasks + g0, (int)(p->ngtasks - g0), p->d_bufs,
-                                p->d_scratch, s))
+                                p->d_scratch, p->d_skip, s))
             return fail(GECCO_HIP_EDEVICE, "gather launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(3, false)) return g_err;
     }
@@ -1207,13 +1226,13 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
-                                        p->affine_variant, s))
+                                        p->affine_variant, p->d_skip, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
     if (ng) {
         if (mark(1, true)) return g_err;
-        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, s))
+        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, p->d_skip, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
@@ -1225,7 +1244,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (p->nsymtasks) {
         if (mark(2, true)) return g_err;
         if (gecco_launch_sym_batched(p->d_symsteps, p->d_symtasks, (int)p->nsymtasks, p->d_bufs,
-                                     p->d_sym_i1, p->d_sym_i2, p->d_sym_f, s))
+                                     p->d_sym_i1, p->d_sym_i2, p->d_sym_f, p->d_skip, s))
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
         if (mark(2, false)) return g_err;
     }
@@ -1297,6 +1316,7 @@ int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *ar
     if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
     HostBlock hb;
     if (build_block(args, hb)) return g_err;
+    hb.group = plan->cur_group;
     plan->blocks.push_back(std::move(hb));
     return ok();
 }
@@ -1311,6 +1331,7 @@ int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &
     y.i1 = std::move(i1);
     y.i2 = std::move(i2);
     y.f = std::move(f);
+    y.group = plan->cur_group;
     plan->syms.push_back(std::move(y));
     return ok();
 }
@@ -1464,6 +1485,7 @@ int gecco_hip_plan_add_block(gecco_hip_plan *plan, const gecco_hip_block_args *a
         return fail(GECCO_HIP_EINVAL, "negative buffer slot or offset");
     HostBlock hb;
     if (build_block(args, hb)) return g_err;
+    hb.group = plan->cur_group;
     plan->blocks.push_back(std::move(hb));
     return 0;
 }
@@ -1553,6 +1575,7 @@ int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_d
     j.len = len;
     j.factor = factor;
     plan->inits.push_back(j);
+    plan->init_group.push_back(plan->cur_group);
     return 0;
 }
 
@@ -1562,6 +1585,27 @@ int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot) {
     if (slot < 0) return fail(GECCO_HIP_EINVAL, "negative slot");
     if ((int32_t)plan->constant_slot.size() <= slot) plan->constant_slot.resize(slot + 1, 0);
     plan->constant_slot[slot] = 1;
+    return 0;
+}
+
+int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (group < 0) return fail(GECCO_HIP_EINVAL, "negative instance group");
+    plan->cur_group = group;
+    plan->max_group = std::max(plan->max_group, group);
+    return 0;
+}
+
+int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (flags && nflags <= plan->max_group)
+        return fail(GECCO_HIP_EINVAL, "skip flags: %lld given, the plan records groups up to %d",
+                    (long long)nflags, plan->max_group);
+    if (flags && !is_device_ptr(flags))
+        return fail(GECCO_HIP_EINVAL, "skip flags must be device memory");
+    plan->d_skip = flags;
+    plan->nskip = flags ? nflags : 0;
     return 0;
 }
 

@@ -188,6 +188,7 @@ struct DevState {
     int mw;                       // vectors of the last step's extrapolation
     int wslot[MAXSUB];            // their slots
     int64_t steps;                // asynchronous steps taken
+    double gn_last;               // ||g|| of the last step that evaluated it
     double b[MAXSUB * MAXSUB];    // <r_i|r_j>, logical order, stride MAXSUB
     double bw[MAXSUB * MAXSUB];   // B of the last step's extrapolation
     double w[MAXSUB];             // weights of the last step
@@ -202,7 +203,9 @@ struct StatusRec {                // one per step, written by the device
 __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
     const double *__restrict__ t, const double *__restrict__ rest,
     const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
-    int64_t n, const DevState *__restrict__ st, double *__restrict__ part) {
+    int64_t n, const DevState *__restrict__ st, double *__restrict__ part,
+    const int32_t *__restrict__ flag) {
+    if (flag && *flag) return;    // converged before this step: nothing to add
     Slots sl;
     sl.m = st->m;
     for (int j = 0; j < MAXSUB; ++j) sl.s[j] = j < sl.m ? st->order[j] : 0;
@@ -219,7 +222,24 @@ __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
 __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
                                                        const double *__restrict__ energy,
                                                        DevState *__restrict__ st, int maxsub,
-                                                       double tol, StatusRec *ring) {
+                                                       double tol, StatusRec *ring,
+                                                       int32_t *__restrict__ flag) {
+    // a flag set before this step (the instance converged and its residual
+    // is no longer re-evaluated): repeat the converged record, leave the
+    // subspace and t as they are
+    if (flag && *flag) {
+        if (threadIdx.x == 0) {
+            StatusRec &rec = ring[st->steps % RING];
+            rec.gnorm = st->gn_last;
+            rec.energy = energy ? energy[0] : 0.0;
+            rec.converged = 1;
+            __threadfence_system();
+            rec.step = st->steps;
+            st->steps += 1;
+            st->skip = 1;
+        }
+        return;
+    }
     constexpr int KM = MAXSUB + 1;
     __shared__ double red[NTH / 64][MAXSUB + 1];
     __shared__ double tot[MAXSUB + 1];
@@ -325,8 +345,10 @@ __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict_
     st->skip = code != 0 ? 1 : 0;
     StatusRec &rec = ring[st->steps % RING];
     rec.gnorm = gn;
+    st->gn_last = gn;
     rec.energy = energy ? energy[0] : 0.0;
     rec.converged = code;
+    if (flag) *flag = code == 1 ? 1 : 0;
     __threadfence_system();
     rec.step = st->steps;
     st->steps += 1;
@@ -416,6 +438,7 @@ struct gecco_hip_diis {
     std::vector<double> w;                     // weights of the last step
     DevState *dstate = nullptr;                // bookkeeping of the asynchronous step
     StatusRec *ring = nullptr;                 // pinned status records (device-written)
+    int32_t *flag = nullptr;                   // optional convergence flag (device)
     bool async = false;                        // steps so far were asynchronous
 };
 
@@ -561,11 +584,25 @@ int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, 
     d->async = true;
     const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
     hipLaunchKernelGGL(diis_prepare_async_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub,
-                       d->vsub, d->n, d->dstate, d->part);
+                       d->vsub, d->n, d->dstate, d->part, d->flag);
     hipLaunchKernelGGL(diis_finish_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->dstate,
-                       d->maxsub, tol, d->ring);
+                       d->maxsub, tol, d->ring, d->flag);
     hipLaunchKernelGGL(diis_expand_async_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, d->dstate);
     return check(hipGetLastError(), "diis async kernels");
+}
+
+int gecco_hip_diis_set_flag(gecco_hip_diis *d, int32_t *flag) {
+    gecco::clear_error();
+    if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    if (flag) {
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, flag) != hipSuccess || at.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            return fail(GECCO_HIP_EINVAL, "diis_set_flag: flag must be device memory");
+        }
+    }
+    d->flag = flag;
+    return 0;
 }
 
 int gecco_hip_diis_status(const gecco_hip_diis *d, int64_t step, double *gnorm, double *energy,

@@ -68,7 +68,8 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_synchronize', 'gecco_hip_plan_set_timing', 'gecco_hip_plan_timing',
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
-             'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain')
+             'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
+             'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -78,6 +79,8 @@ _lib.gecco_hip_plan_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
 _lib.gecco_hip_plan_destroy.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_add_block.argtypes = [ctypes.c_void_p, ctypes.POINTER(BlockArgs)]
 _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
+_lib.gecco_hip_plan_set_group.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+_lib.gecco_hip_plan_set_skip.argtypes = [ctypes.c_void_p, ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
                                         ctypes.c_void_p]
 _lib.gecco_hip_plan_stats.argtypes = [ctypes.c_void_p, PD, PI64, PI64, PD, PD]
@@ -252,6 +255,24 @@ class Plan:
         contraction ranges (default on); before finalize."""
         _check(_lib.gecco_hip_plan_set_chain(self._h, 1 if on else 0))
 
+    def set_group(self, group):
+        """Record the following kernel calls, list initialisations and
+        symmetrisations as instance group `group` (default 0); before
+        finalize."""
+        _check(_lib.gecco_hip_plan_set_group(self._h, int(group)))
+
+    def set_skip(self, flags):
+        """Device int32 tensor of one flag per instance group (None: off);
+        every execute then skips the work of group g while flags[g] != 0."""
+        if flags is None:
+            _check(_lib.gecco_hip_plan_set_skip(self._h, None, 0))
+            self._skip = None
+            return
+        if str(flags.dtype) != 'torch.int32' or not flags.is_contiguous():
+            raise GeccoHipError('set_skip: flags must be a contiguous int32 tensor')
+        _check(_lib.gecco_hip_plan_set_skip(self._h, _ptr(flags), flags.numel()))
+        self._skip = flags            # keeps the memory alive
+
     def set_materialize(self, min_dim):
         """Materialise gathered operands of kernel calls with min(M, N) >=
         min_dim into scratch (0 disables); before finalize."""
@@ -286,7 +307,7 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_plan_add_reorder')
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
                   'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix',
-                  'gecco_hip_diis_step_async', 'gecco_hip_diis_status')
+                  'gecco_hip_diis_step_async', 'gecco_hip_diis_status', 'gecco_hip_diis_set_flag')
 VP = ctypes.c_void_p
 _lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
 _lib.gecco_hip_orbs_create.restype = ctypes.c_int
@@ -335,6 +356,7 @@ _lib.gecco_hip_diis_reset.argtypes = [VP]
 _lib.gecco_hip_diis_step.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, PD, PD, PI64, VP]
 _lib.gecco_hip_diis_step_async.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, VP]
 _lib.gecco_hip_diis_status.argtypes = [VP, I64, PD, PD, PI64]
+_lib.gecco_hip_diis_set_flag.argtypes = [VP, VP]
 _lib.gecco_hip_diis_weights.argtypes = [VP, PD, I64, PI64]
 _lib.gecco_hip_diis_matrix.argtypes = [VP, PD, I64, PI64]
 for _n in SOLVER_SYMBOLS:
@@ -565,6 +587,16 @@ class DIIS:
         s = VP(0 if stream is None else int(stream))
         _check(_lib.gecco_hip_diis_step_async(self._h, _ptr(t), _ptr(rest), _ptr(diag),
                                               None if energy is None else _ptr(energy), float(tol), s))
+
+    def set_flag(self, flag):
+        """One-element device int32 tensor (or None) that an asynchronous
+        step sets to 1 when it meets tol; while it is set, later asynchronous
+        steps leave t and the subspace unchanged and repeat the converged
+        record.  The caller clears it on restart."""
+        if flag is not None and (str(flag.dtype) != 'torch.int32' or flag.numel() != 1):
+            raise GeccoHipError('set_flag: one-element int32 tensor expected')
+        _check(_lib.gecco_hip_diis_set_flag(self._h, None if flag is None else _ptr(flag)))
+        self._flag = flag
 
     def status(self, step):
         """(||g||, energy, code) of asynchronous step `step` once the stream

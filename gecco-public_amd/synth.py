@@ -73,3 +73,18 @@ def synthetic(nocc, nvir, naux, nsym=1, seed=0, sigma=0.03, occ_range=(-2.0, -0.
     mask = gpq[None, :, :] == irr_aux[:, None, None]
     B = np.where(mask, G, 0.0)
     return Hamiltonian(eps[:nocc], eps[nocc:], irr[:nocc].tolist(), irr[nocc:].tolist(), B, nsym)
+
+
+def batch_recipe(count=64, nocc_range=(8, 24), nvir_range=(40, 160), aux_per_vir=3, nsym=4,
+                 sigma=0.03):
+    """Arguments of synthetic() for a batch of independent instances of
+    uneven size: instance k has seed k, nocc and nvir uniform integers in the
+    (inclusive) ranges drawn from default_rng([k, 1]), naux = aux_per_vir *
+    nvir."""
+    out = []
+    for k in range(count):
+        r = np.random.default_rng([k, 1])
+        o = int(r.integers(nocc_range[0], nocc_range[1] + 1))
+        v = int(r.integers(nvir_range[0], nvir_range[1] + 1))
+        out.append(dict(nocc=o, nvir=v, naux=aux_per_vir * v, nsym=nsym, seed=k, sigma=sigma))
+    return out

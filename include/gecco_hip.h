@@ -194,6 +194,19 @@ int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim);
  * block -- are run as one GEMM over their concatenated contraction ranges,
  * so each result tile is written once.  Before finalize only. */
 int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
+/* Instance groups, for many independent problems (e.g. CC instances of
+ * uneven size) recorded into one plan: every kernel call, list
+ * initialisation and symmetrisation recorded after set_group(g) belongs to
+ * group g (default 0; before finalize only).  set_skip gives device memory
+ * flags[nflags] (nflags above the largest group; NULL turns skipping off):
+ * every launch of execute then skips the work of group g while flags[g] != 0,
+ * read on the device when each task starts -- so a captured graph of the
+ * plan skips the instances that have converged since (see
+ * gecco_hip_diis_set_flag).  Gathers of constant slots are never skipped;
+ * plans with trace steps refuse skip flags.  No counterpart in the
+ * reference, which runs one problem per process. */
+int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
+int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags);
 /* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
  * bytes they move per execute (one read and one write per element), and the
  * number of tasks whose result tile is reduced in LDS and written back as

@@ -69,6 +69,15 @@ int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, 
  * overwritten (more than 64 steps ago). */
 int gecco_hip_diis_status(const gecco_hip_diis *d, int64_t step, double *gnorm, double *energy,
                           int64_t *converged);
+/* Device int32 that an asynchronous step sets to 1 when it meets tol
+ * (converged) and leaves 0 otherwise (NULL: none; the caller keeps the
+ * memory alive and clears it together with reset).  While the flag is set,
+ * later asynchronous steps do no subspace work and leave t unchanged; each
+ * repeats the converged record (||g|| of the converging step).  With plan
+ * skip flags (gecco_hip_plan_set_skip) on the same memory, an instance's
+ * contractions and DIIS steps stop once it has converged.  No counterpart
+ * in the reference. */
+int gecco_hip_diis_set_flag(gecco_hip_diis *d, int32_t *flag);
 
 /* Weights of the last step (w[0..m-1], oldest vector first) and the
  * current subspace matrix (b[m*m], row-major), for checks. */

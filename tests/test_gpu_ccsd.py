@@ -163,7 +163,13 @@ def test_batched_instances_match_separate_solves():
         assert res['iterations'][k] == r['iterations'], (k, res['iterations'], r['iterations'])
     assert res['batch_iterations'] == max(r['iterations'] for r in sep) + 1 or \
         res['batch_iterations'] == max(r['iterations'] for r in sep)
+    assert b.done.cpu().tolist() == [1] * len(cases)    # every instance flagged converged
     # a second solve reuses the captured graph
     res2 = b.solve(tol=1e-10, maxit=80)
-    assert res2['energies'] == res['energies'] or \
-        max(abs(x - y) for x, y in zip(res2['energies'], res['energies'])) < 1e-12
+    assert max(abs(x - y) for x, y in zip(res2['energies'], res['energies'])) < 1e-12
+    assert res2['iterations'] == res['iterations']
+    # without skipping converged instances: the same energies and counts
+    b2 = BatchedCCSD(hams, skip_converged=False)
+    res3 = b2.solve(tol=1e-10, maxit=80)
+    assert max(abs(x - y) for x, y in zip(res3['energies'], res['energies'])) < 1e-12
+    assert res3['iterations'] == res['iterations']

@@ -116,6 +116,55 @@ def test_many_contractions_one_launch():
         assert _relerr(bufs[3 * k + 2].cpu().numpy(), ref) <= TOL
 
 
+def test_instance_groups_skip_flags():
+    """Contractions recorded as instance groups of one plan: with skip flags
+    set, the flagged group's list initialisation, gathers and kernel calls do
+    nothing (its result keeps its prior contents exactly) and the others
+    match the oracle; cleared flags or no flags run everything."""
+    torch = _torch()
+    import gecco_hip
+    orbs = orbitals(5, 8, 4)
+    dev = torch.device('cuda:0')
+    plan = gecco_hip.Plan()
+    bufs, refs, prior = [], [], []
+    rng = np.random.default_rng(7)
+    for k, name in enumerate(['ladder', 'ring', 't1t1']):
+        plan.set_group(k)
+        c, a, b, r = CASES[name](orbs)
+        x1 = spin_symmetric(a, 60 + k)
+        x2 = spin_symmetric(b, 70 + k)
+        blocks = c.blocks(True)
+        refs.append(_oracle(c, x1, x2, r.length, blocks))
+        prior.append(rng.standard_normal(r.length))
+        base = len(bufs)
+        bufs += [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+                 torch.from_numpy(prior[-1]).to(dev)]
+        plan.add_init(base + 2, r.length)           # result = 0, then the contraction
+        for bl in blocks:
+            plan.add(bl, buf=(base, base + 1, base + 2))
+    plan.finalize()
+    flags = torch.tensor([0, 1, 0], dtype=torch.int32, device=dev)
+    with pytest.raises(gecco_hip.GeccoHipError):
+        plan.set_skip(flags[:2])                     # groups 0..2 need 3 flags
+    plan.set_skip(flags)
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    for k in (0, 2):
+        assert _relerr(bufs[3 * k + 2].cpu().numpy(), refs[k]) <= TOL
+    assert np.array_equal(bufs[5].cpu().numpy(), prior[1])
+    flags.zero_()
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert _relerr(bufs[3 * k + 2].cpu().numpy(), refs[k]) <= TOL
+    flags.fill_(1)
+    bufs[2].fill_(3.0)
+    plan.set_skip(None)                              # no flags: everything runs
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[2].cpu().numpy(), refs[0]) <= TOL
+
+
 @pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 6, 10)])
 @pytest.mark.parametrize('name', list(CASES))
 @pytest.mark.parametrize('use_tr', [True, False])

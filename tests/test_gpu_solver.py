@@ -168,3 +168,51 @@ def test_async_step_converged_leaves_t():
     diis.reset()
     with pytest.raises(gecco_hip.GeccoHipError):
         diis.status(0)
+
+
+def test_async_flag_freezes_converged_steps():
+    """With a convergence flag, the converging asynchronous step sets it;
+    later steps leave t and the subspace unchanged whatever residual they
+    are given and repeat the converged record until the flag is cleared."""
+    torch = _torch()
+    import gecco_hip
+    n = 700
+    dev = torch.device('cuda:0')
+    g = torch.Generator(device='cpu').manual_seed(3)
+    v = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+    d = (1.0 + torch.rand(n, generator=g, dtype=torch.float64)).to(dev)
+    t = torch.zeros(n, dtype=torch.float64, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    diis = gecco_hip.DIIS(n, 4)
+    diis.set_flag(flag)
+    # with t = 0 before each step, g = D t + rest = c v and ||g|| = c ||v||
+    tol = 1e-3 * float(torch.linalg.norm(v))
+    codes = []
+    for k, c in enumerate([1.0, 0.3, 0.05, 1e-4]):
+        t.zero_()
+        diis.step_async(t, c * v, d, tol=tol)
+        torch.cuda.synchronize()
+        gn, _, code = diis.status(k)
+        codes.append(code)
+        assert int(flag.item()) == (1 if code == 1 else 0)
+    assert codes == [0, 0, 0, 1]
+    gn_conv = diis.status(3)[0]
+    w = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+    t.copy_(w)
+    t_conv = t.clone()
+    b_conv = diis.matrix()
+    diis.step_async(t, 5.0 * v, d, tol=tol)      # a large gradient: ignored while flagged
+    torch.cuda.synchronize()
+    gn, _, code = diis.status(4)
+    assert code == 1 and gn == gn_conv
+    assert torch.equal(t, t_conv)
+    assert np.array_equal(diis.matrix(), b_conv)
+    # cleared flag and reset: steps run again
+    flag.zero_()
+    diis.reset()
+    diis.step_async(t, 5.0 * v, d, tol=tol)
+    torch.cuda.synchronize()
+    assert diis.status(0)[2] == 0 and int(flag.item()) == 0
+    assert not torch.equal(t, t_conv)
+    with pytest.raises(gecco_hip.GeccoHipError):
+        diis.set_flag(torch.zeros(2, dtype=torch.int32, device=dev))
