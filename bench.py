@@ -14,6 +14,12 @@ symmetrisation) and the native DIIS step, operands resident in HBM.  After the t
 --workload ladder: the particle-particle ladder contraction alone (8 kernel
 calls, 5.58 GFLOP) at the same sizes.
 
+--workload triples: the CCSDT triples contractions T3 <- V.T3 (pp ladder,
+hh ladder, ring) and T3 <- V.T2 at nocc=12, nvir=60, naux=180, 4 IRREPs
+(gecco-public_amd/formulas/ccsdt_triples.json): one step = one evaluation of
+R3 (initialisation, one grouped launch for the five terms, spin-flip
+symmetrisation) with random spin-symmetric T2 and T3.
+
 --workload batched: 64 independent CCSD instances of uneven size (nocc ~
 U[8,24], nvir ~ U[40,160], naux = 3 nvir, 4 IRREPs, seeds 0..63) in one
 grouped launch per formula level and one replayed graph per batched
@@ -35,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, 'gecco-public_amd'))
 sys.path.insert(0, ROOT)
 
 NOCC, NVIR, NAUX, NSYM, SEED, SIGMA = 20, 100, 300, 4, 0, 0.03
+T3_NOCC, T3_NVIR, T3_NAUX = 12, 60, 180
 PEAK_FP64 = 78.6        # TFLOP/s, MI355X dense FP64 (vector and MFMA)
 PEAK_HBM = 8000.0       # GB/s
 METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
@@ -289,6 +296,108 @@ def run_ccsd(args):
     return out
 
 
+# ------------------------------------------------------------------ triples
+def cpu_triples_sample(ham, flops_eval):
+    """Oracle route 3 (one core) on the T3 <- V(ov,oo).T2 term; returns the
+    per-evaluation rate scaled by FLOP, the seconds and a description."""
+    import numpy as np
+    from oracle.melist import Orbitals, MEList
+    from oracle.contract import Contraction
+    H, P = 1, 2
+    orbs = Orbitals(ham.irrep_o, ham.irrep_v, ham.nsym)
+    V = MEList(orbs, {P: 1, H: 1}, {H: 2})
+    T2 = MEList(orbs, {P: 2}, {H: 2})
+    R3 = MEList(orbs, {P: 3}, {H: 3})
+    c = Contraction(V, T2, R3, {H: 1}, {})
+    blocks = c.blocks(True)
+    fl = sum(2.0 * np.prod(x['lstr_ex1']) * np.prod(x['lstr_ex2']) * np.prod(x['lstr_cnt'])
+             for x in blocks)
+    rng = np.random.default_rng(3)
+    x1, x2 = rng.standard_normal(V.length), rng.standard_normal(T2.length)
+    xr = np.zeros(R3.length)
+    t0 = time.perf_counter()
+    c.run(x1, x2, xr, route=3, use_tr=False, screen=False, blocks=blocks)
+    dt = time.perf_counter() - t0
+    per_eval = dt * flops_eval / fl
+    return 1.0 / per_eval, dt, (f'the T3 <- V(ov,oo).T2 term ({fl / 1e9:.2f} of {flops_eval / 1e9:.1f} '
+                                'GFLOP per evaluation), oracle route 3 with a plain-loop dgemm, '
+                                'scaled to one evaluation by FLOP')
+
+
+def run_triples(args):
+    import torch
+    import ccsd
+    import formula
+    import gecco_hip
+    from synth import synthetic
+    ham = synthetic(args.nocc, args.nvir, args.naux, nsym=args.nsym, seed=args.seed, sigma=args.sigma)
+    fpath = os.path.join(ROOT, 'gecco-public_amd', 'formulas', 'ccsdt_triples.json')
+    t0 = time.perf_counter()
+    solver = ccsd.CCSD(ham, formula=formula.load(fpath))
+    setup_s = time.perf_counter() - t0
+    print(f'# triples setup {setup_s:.1f} s, {solver.flops / 1e9:.1f} GFLOP, {solver.nblocks} kernel '
+          'calls per evaluation', file=sys.stderr, flush=True)
+    # random spin-symmetric amplitudes (projected by the library)
+    g = torch.Generator(device='cuda:0')
+    g.manual_seed(5)
+    for n, scale in (('T2', 0.05), ('T3', 0.01)):
+        solver.buf[n].copy_(scale * torch.randn(solver.buf[n].numel(), dtype=torch.float64,
+                                                device='cuda:0', generator=g))
+    proj = gecco_hip.Plan()
+    for n in ('T2', 'T3'):
+        proj.add_sym_ab(solver.L[n], solver.slot[n])
+    proj.finalize()
+    proj.execute(solver.bufs)
+    for _ in range(args.warmup):
+        solver.evaluate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.evaluate()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = kernel_times(solver.plans, lambda: [solver.evaluate() for _ in range(args.steps)], args.steps)
+    # size-independent check: every term is linear in (T2, T3)
+    r1 = solver.buf['R3'].clone()
+    for n in ('T2', 'T3'):
+        solver.buf[n].mul_(2.0)
+    solver.evaluate()
+    torch.cuda.synchronize()
+    lin = float((solver.buf['R3'] - 2.0 * r1).abs().max() / (2.0 * r1.abs().max()))
+    ms, names, dom = kt['ms'], kt['names'], kt['dom']
+    out = {
+        'metric': METRIC, 'value': round(solver.flops * args.steps / wall / 1e12, 4), 'unit': 'TFLOP/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / args.steps, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': f'synthetic DF Hamiltonian (seed {args.seed}, naux {args.naux}, sigma {args.sigma}), '
+                'random spin-symmetric T2/T3',
+        'config': {'workload': 'ccsdt_triples_contractions', 'nocc': args.nocc, 'nvir': args.nvir,
+                   'naux': args.naux, 'nsym': args.nsym, 'use_tr': True,
+                   'terms': ['pp ladder V(vv,vv).T3', 'hh ladder V(oo,oo).T3', 'ring T3.V(ov,vo)',
+                             'V(vv,vo).T2', 'V(ov,oo).T2'],
+                   'kernel_calls': solver.nblocks, 'gflop_per_step': round(solver.flops / 1e9, 3),
+                   't3_elements': solver.L['T3'].length},
+        'kernels_ms_per_step': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
+                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
+                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
+                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'roofline': {'bound': 'mfma', 'achieved': round(kt['achieved'], 4), 'peak': PEAK_FP64,
+                     'unit': 'TFLOP/s', 'frac': round(kt['achieved'] / PEAK_FP64, 4),
+                     'traffic': MEASURED_TRAFFIC.get(('triples', args.nocc, args.nvir, args.nsym,
+                                                      names[dom])),
+                     'kernel': names[dom], 'kernel_ms': round(kt['dom_ms'], 5)},
+        'linearity_rel_err': lin,
+        'setup_seconds': round(setup_s, 2),
+    }
+    if not args.no_cpu:
+        v, dt, sample = cpu_triples_sample(ham, solver.flops)
+        out['cpu_baseline'] = {'value': round(v * solver.flops / 1e12, 6), 'unit': 'TFLOP/s', 'cores': 1,
+                               'kind': 'port', 'sample': sample, 'seconds': round(dt, 2),
+                               'cpu': platform.processor() or platform.machine()}
+    return out
+
+
 # ------------------------------------------------------------------ batched
 def run_batched(args):
     """--workload batched: args.instances independent CCSD instances of
@@ -398,12 +507,12 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=30)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', choices=['ccsd', 'ladder', 'batched'], default='ccsd')
+    ap.add_argument('--workload', choices=['ccsd', 'ladder', 'batched', 'triples'], default='ccsd')
     ap.add_argument('--instances', type=int, default=64, help='batched workload: instance count')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--nocc', type=int, default=NOCC)
-    ap.add_argument('--nvir', type=int, default=NVIR)
-    ap.add_argument('--naux', type=int, default=NAUX)
+    ap.add_argument('--nocc', type=int, default=None, help=f'default {NOCC} (triples: {T3_NOCC})')
+    ap.add_argument('--nvir', type=int, default=None, help=f'default {NVIR} (triples: {T3_NVIR})')
+    ap.add_argument('--naux', type=int, default=None, help=f'default {NAUX} (triples: {T3_NAUX})')
     ap.add_argument('--nsym', type=int, default=NSYM)
     ap.add_argument('--seed', type=int, default=SEED)
     ap.add_argument('--sigma', type=float, default=SIGMA)
@@ -412,6 +521,10 @@ def main():
                     help='only run a plain device copy of MB megabytes (calibrates '
                          'FETCH_SIZE/WRITE_SIZE counters) and exit')
     args = ap.parse_args()
+    t3 = args.workload == 'triples'
+    args.nocc = args.nocc if args.nocc is not None else (T3_NOCC if t3 else NOCC)
+    args.nvir = args.nvir if args.nvir is not None else (T3_NVIR if t3 else NVIR)
+    args.naux = args.naux if args.naux is not None else (T3_NAUX if t3 else NAUX)
     if args.gpus != 1:
         print(json.dumps({'error': 'this engine runs on one GPU; --gpus > 1 is not supported'}))
         sys.exit(2)
@@ -425,7 +538,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
         return
-    out = {'ladder': run_ladder, 'ccsd': run_ccsd, 'batched': run_batched}[args.workload](args)
+    out = {'ladder': run_ladder, 'ccsd': run_ccsd, 'batched': run_batched,
+           'triples': run_triples}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

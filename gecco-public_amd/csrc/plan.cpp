@@ -328,13 +328,15 @@ static bool first_is_contiguous(const Resolved &r) {
     return r.first[0] != 0 && r.first[1] != 0 && absv(r.first[1]) - absv(r.first[0]) == 1;
 }
 
-// entry i == 1 + a0 + a1*i for all i (no forbidden elements, all signs +)
+// entry i == 1 + a0 + a1*i for all i, and every entry positive: no forbidden
+// element (0) and no sign flip (a decreasing run such as [1, 0] or [2, 1, 0,
+// -1] fits the formula but holds forbidden or negative entries)
 static bool affine_of(const std::vector<int32_t> &v, int64_t &a0, int64_t &a1) {
     if (v.empty() || v[0] <= 0) return false;
     a0 = (int64_t)v[0] - 1;
     a1 = v.size() > 1 ? (int64_t)v[1] - (int64_t)v[0] : 0;
     for (size_t i = 0; i < v.size(); ++i)
-        if ((int64_t)v[i] != 1 + a0 + a1 * (int64_t)i) return false;
+        if (v[i] <= 0 || (int64_t)v[i] != 1 + a0 + a1 * (int64_t)i) return false;
     return true;
 }
 
@@ -1539,6 +1541,15 @@ int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t i, int64_t *in
     if (ninfo >= 23) { info[20] = b.buf12; info[21] = b.off12; info[22] = b.len12; }
     if (ninfo >= 24) info[23] = b.next;
     if (ninfo >= 25) info[24] = b.skinny;
+    if (ninfo >= 30) {
+        // operands in kernel orientation (buf1/off1 hold A: op2 when
+        // swapped) and the factor's bit pattern
+        info[25] = b.buf1;
+        info[26] = b.off1;
+        info[27] = b.buf2;
+        info[28] = b.off2;
+        std::memcpy(&info[29], &b.xfac, sizeof(double));
+    }
     return 0;
 }
 

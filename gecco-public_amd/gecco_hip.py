@@ -13,6 +13,7 @@ library is missing, importing this module raises.
 """
 import ctypes
 import os
+import struct
 
 import numpy as np
 
@@ -224,9 +225,16 @@ class Plan:
         return dict(ms=list(ms), launches=list(n), flops=list(fl))
 
     def block_info(self, i):
-        info = (I64 * 25)()
-        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 25))
+        info = (I64 * 30)()
+        _check(_lib.gecco_hip_plan_block_info(self._h, i, info, 30))
         return list(info)
+
+    def block_operands(self, i):
+        """(buffer slot, element offset) of A and B in kernel orientation
+        and the factor of kernel call i."""
+        info = self.block_info(i)
+        xfac = struct.unpack('<d', struct.pack('<q', info[29]))[0]
+        return (info[25], info[26]), (info[27], info[28]), xfac
 
     def block_index(self, i, which):
         """Resolved signed index data (first, second) of operand which (0 A,

@@ -154,8 +154,10 @@ int gecco_hip_plan_timing(gecco_hip_plan *plan, double *ms, int64_t *launches, d
  * ninfo >= 23, [result buffer, result block offset, result block length]
  * and, with ninfo >= 24, [next kernel call of its chain, -1 if none; see
  * gecco_hip_plan_set_chain] and, with ninfo >= 25, [16 x 16 wave-split-K
- * tiles: a result with a short side of at most 16].  Index flags are valid
- * before finalize, the kernel-class and chain entries after it. */
+ * tiles: a result with a short side of at most 16] and, with ninfo >= 30,
+ * [A buffer slot, A block offset, B buffer slot, B block offset (kernel
+ * orientation), bit pattern of the factor xfac as a double].  Index flags are
+ * valid before finalize, the kernel-class and chain entries after it. */
 int gecco_hip_plan_block_info(const gecco_hip_plan *plan, int64_t iblock, int64_t *info,
                               int64_t ninfo);
 /* Resolved signed index data of operand `which` (0 = A, 1 = B, 2 = result C,

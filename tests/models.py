@@ -71,3 +71,49 @@ def energy_like(orbs):
 
 CASES = {'ladder': ladder, 'ring': ring, 't1t1': t1t1, 'hh_ladder': hh_ladder,
          'fock_pp': fock_pp, 'energy': energy_like}
+
+
+# Triples (T3) terms of CCSDT: R3(abc,ijk) from V.T3 and V.T2.
+def _t3_lists(orbs):
+    return MEList(orbs, {P: 3}, {H: 3}), MEList(orbs, {P: 3}, {H: 3})
+
+
+def t3_pp_ladder(orbs):
+    """R3(abc,ijk) += sum_{d<e} V(ab,de) T3(dec,ijk), antisymmetrised in abc."""
+    T3, R3 = _t3_lists(orbs)
+    V = MEList(orbs, {P: 2}, {P: 2})
+    return Contraction(V, T3, R3, {}, {P: 2}), V, T3, R3
+
+
+def t3_hh_ladder(orbs):
+    """R3(abc,ijk) += sum_{l<m} V(lm,ij) T3(abc,lmk)."""
+    T3, R3 = _t3_lists(orbs)
+    V = MEList(orbs, {H: 2}, {H: 2})
+    return Contraction(V, T3, R3, {H: 2}, {}), V, T3, R3
+
+
+def t3_ring(orbs):
+    """R3(abc,ijk) += sum_{dl} T3(dab,lij) W(cl,dk), antisymmetrised."""
+    T3, R3 = _t3_lists(orbs)
+    W = MEList(orbs, {P: 1, H: 1}, {P: 1, H: 1})
+    return Contraction(T3, W, R3, {P: 1}, {H: 1}), T3, W, R3
+
+
+def t3_vvvo_t2(orbs):
+    """R3(abc,ijk) += sum_e V(bc,ek) T2(ea,ij), antisymmetrised."""
+    _, R3 = _t3_lists(orbs)
+    V = MEList(orbs, {P: 2}, {P: 1, H: 1})
+    T2 = MEList(orbs, {P: 2}, {H: 2})
+    return Contraction(V, T2, R3, {}, {P: 1}), V, T2, R3
+
+
+def t3_ovoo_t2(orbs):
+    """R3(abc,ijk) += sum_m V(cm,jk) T2(ab,mi), antisymmetrised."""
+    _, R3 = _t3_lists(orbs)
+    V = MEList(orbs, {P: 1, H: 1}, {H: 2})
+    T2 = MEList(orbs, {P: 2}, {H: 2})
+    return Contraction(V, T2, R3, {H: 1}, {}), V, T2, R3
+
+
+T3_CASES = {'t3_pp_ladder': t3_pp_ladder, 't3_hh_ladder': t3_hh_ladder, 't3_ring': t3_ring,
+            't3_vvvo_t2': t3_vvvo_t2, 't3_ovoo_t2': t3_ovoo_t2}

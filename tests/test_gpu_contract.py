@@ -7,7 +7,7 @@ atomics): max |gpu - oracle| <= 1e-12 * max |oracle|.
 import numpy as np
 import pytest
 
-from models import orbitals, spin_symmetric, CASES
+from models import orbitals, spin_symmetric, CASES, T3_CASES
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -198,6 +198,73 @@ def test_product_enumerator_matches_oracle(nsym, nocc, nvir, name, use_tr):
     plan.execute([d1, d2, dr])
     torch.cuda.synchronize()
     assert _relerr(dr.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 3, 4), (2, 4, 6), (4, 6, 10)])
+@pytest.mark.parametrize('name', list(T3_CASES))
+@pytest.mark.parametrize('use_tr', [True, False])
+def test_triples_match_oracle(nsym, nocc, nvir, name, use_tr):
+    """CCSDT triples terms (three-body ME-lists) through the product plan
+    builder against the oracle's contr_op1op2 restatement."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    print(f'triples {name} nsym={nsym} nocc={nocc} nvir={nvir} use_tr={use_tr}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = T3_CASES[name](orbs)
+    x1 = spin_symmetric(a, 71)
+    x2 = spin_symmetric(b, 72)
+    x0 = spin_symmetric(r, 73)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=use_tr, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    A = gecco_hip.MEList(o, a.occ_c, a.occ_a)
+    B = gecco_hip.MEList(o, b.occ_c, b.occ_a)
+    R = gecco_hip.MEList(o, r.occ_c, r.occ_a)
+    plan = gecco_hip.Plan()
+    plan.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, use_tr)
+    if use_tr:
+        plan.add_sym_ab(R, 2)
+    plan.finalize()
+    dev = torch.device('cuda:0')
+    dr = torch.from_numpy(x0).to(dev)
+    plan.execute([torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev), dr])
+    torch.cuda.synchronize()
+    assert _relerr(dr.cpu().numpy(), ref) <= TOL
+
+
+def test_all_triples_terms_one_launch():
+    """The five triples terms into one R3 list in one grouped launch (the
+    triples bench's plan) against the sum of the oracle's terms."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    nsym, nocc, nvir = 4, 5, 9
+    orbs = orbitals(nocc, nvir, nsym)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    dev = torch.device('cuda:0')
+    plan = gecco_hip.Plan()
+    bufs = [None]
+    ref = None
+    for k, name in enumerate(T3_CASES):
+        c, a, b, r = T3_CASES[name](orbs)
+        if ref is None:
+            ref = np.zeros(r.length)
+            R = gecco_hip.MEList(o, r.occ_c, r.occ_a)
+            bufs[0] = torch.zeros(r.length, dtype=torch.float64, device=dev)
+        x1 = spin_symmetric(a, 80 + k)
+        x2 = spin_symmetric(b, 90 + k)
+        c.run(x1, x2, ref, route=3, use_tr=True, screen=False)
+        A = gecco_hip.MEList(o, a.occ_c, a.occ_a)
+        B = gecco_hip.MEList(o, b.occ_c, b.occ_a)
+        s = len(bufs)
+        bufs += [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev)]
+        plan.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, True, buf=(s, s + 1, 0))
+    plan.add_sym_ab(R, 0)
+    plan.finalize()
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[0].cpu().numpy(), ref) <= TOL
 
 
 @pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5])
