@@ -20,6 +20,13 @@ hh ladder, ring) and T3 <- V.T2 at nocc=12, nvir=60, naux=180, 4 IRREPs
 R3 (initialisation, one grouped launch for the five terms, spin-flip
 symmetrisation) with random spin-symmetric T2 and T3.
 
+--workload learned: the learned per-amplitude preconditioner (trained by
+gecco-public_amd/train_precond.py, parameters in --precond) against the
+diagonal one on closed-shell CCSD at nocc=30, nvir=150, naux=450, 4 IRREPs:
+iterations and time to ||Omega|| < 1e-8 with each, the energy difference,
+and the network's inference kernel.  One step = one learned-preconditioner
+iteration (graph mode is not used: the iteration index is an argument).
+
 --workload batched: 64 independent CCSD instances of uneven size (nocc ~
 U[8,24], nvir ~ U[40,160], naux = 3 nvir, 4 IRREPs, seeds 0..63) in one
 grouped launch per formula level and one replayed graph per batched
@@ -42,6 +49,8 @@ sys.path.insert(0, ROOT)
 
 NOCC, NVIR, NAUX, NSYM, SEED, SIGMA = 20, 100, 300, 4, 0, 0.03
 T3_NOCC, T3_NVIR, T3_NAUX = 12, 60, 180
+L_NOCC, L_NVIR, L_NAUX = 30, 150, 450
+PRECOND = os.path.join(ROOT, 'gecco-public_amd', 'precond', 'ccsd_mlp.npz')
 PEAK_FP64 = 78.6        # TFLOP/s, MI355X dense FP64 (vector and MFMA)
 PEAK_HBM = 8000.0       # GB/s
 METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
@@ -398,6 +407,92 @@ def run_triples(args):
     return out
 
 
+# ------------------------------------------------------------------ learned
+def run_learned(args):
+    import torch
+    import ccsd
+    import gecco_hip
+    import learned
+    from synth import synthetic
+    if not os.path.exists(args.precond):
+        raise SystemExit(f'trained preconditioner {args.precond} not found '
+                         '(gecco-public_amd/train_precond.py writes it)')
+    pre = learned.LearnedPreconditioner.load(args.precond)
+    ham = synthetic(args.nocc, args.nvir, args.naux, nsym=args.nsym, seed=args.seed, sigma=args.sigma)
+    t0 = time.perf_counter()
+    solver = ccsd.CCSD(ham)
+    setup_s = time.perf_counter() - t0
+    print(f'# learned-preconditioner setup {setup_s:.1f} s, {solver.flops / 1e9:.1f} GFLOP per iteration, '
+          f'{solver.amp.numel()} amplitudes', file=sys.stderr, flush=True)
+    solver.solve(tol=0.0, maxit=2, mode='async')                  # warm-up
+    plain = solver.solve(tol=args.tol, maxit=200, mode='async')
+    print(f"# diagonal: {plain['iterations']} iterations {plain['seconds']:.3f} s", file=sys.stderr,
+          flush=True)
+    lrn = solver.solve(tol=args.tol, maxit=200, mode='async', precond=pre)
+    print(f"# learned: {lrn['iterations']} iterations {lrn['seconds']:.3f} s, fallback at "
+          f"{lrn['fallback_at']}", file=sys.stderr, flush=True)
+    plain_g = solver.solve(tol=args.tol, maxit=200, mode='graph')
+    # the converged energies at a tighter residual
+    tight_p = solver.solve(tol=1e-10, maxit=300, mode='graph')
+    tight_l = solver.solve(tol=1e-10, maxit=300, mode='async', precond=pre)
+    # throughput of learned-preconditioner iterations (no convergence test)
+    solver.solve(tol=0.0, maxit=args.warmup, mode='async', precond=pre)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solve(tol=0.0, maxit=args.steps, mode='async', precond=pre)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the inference kernel alone, HIP events on the launch stream
+    n = solver.amp.numel()
+    scale = torch.empty_like(solver.amp)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        pre.scale(solver.diag, solver.res, solver.amp, 5, scale, stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        pre.scale(solver.diag, solver.res, solver.amp, 5, scale, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    mac = 128 * 4 + 128 * 128 + 128                  # per amplitude
+    inf_tflops = 2.0 * mac * n / (inf_ms * 1e-3) / 1e12
+    out = {
+        'metric': METRIC, 'value': round(args.steps / wall, 3), 'unit': 'iters/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / args.steps, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': f'synthetic DF Hamiltonian (seed {args.seed}, naux {args.naux}, sigma {args.sigma})',
+        'config': {'workload': 'ccsd_learned_preconditioner', 'nocc': args.nocc, 'nvir': args.nvir,
+                   'naux': args.naux, 'nsym': args.nsym, 'use_tr': True, 'amplitudes': n,
+                   'gflop_per_iter': round(solver.flops / 1e9, 3),
+                   'network': '4-128-128-1 SiLU, bf16 weights, fp32 accumulation'},
+        'convergence': {'tol': args.tol,
+                        'diagonal': {'iterations': plain['iterations'], 'seconds': round(plain['seconds'], 4),
+                                     'energy': plain['energy'], 'residual': plain['residual']},
+                        'diagonal_graph': {'iterations': plain_g['iterations'],
+                                           'seconds': round(plain_g['seconds'], 4)},
+                        'learned': {'iterations': lrn['iterations'], 'seconds': round(lrn['seconds'], 4),
+                                    'energy': lrn['energy'], 'residual': lrn['residual'],
+                                    'fallback_at': lrn['fallback_at']},
+                        'abs_dE_learned_vs_diagonal': abs(lrn['energy'] - plain['energy']),
+                        'tol_1e-10': {'diagonal_iterations': tight_p['iterations'],
+                                      'learned_iterations': tight_l['iterations'],
+                                      'learned_fallback_at': tight_l['fallback_at'],
+                                      'abs_dE': abs(tight_l['energy'] - tight_p['energy'])}},
+        'roofline': {'bound': 'mfma', 'achieved': round(inf_tflops, 4), 'peak': 2500.0,
+                     'unit': 'TFLOP/s', 'frac': round(inf_tflops / 2500.0, 6), 'traffic': None,
+                     'kernel': 'mlp_infer_kernel<0> (+ feature norms)', 'kernel_ms': round(inf_ms, 5)},
+        'setup_seconds': round(setup_s, 2),
+    }
+    summary = os.path.splitext(args.precond)[0] + '.json'
+    if os.path.exists(summary):
+        out['training'] = json.load(open(summary))
+        out['training'].pop('hamiltonians', None)
+    return out
+
+
 # ------------------------------------------------------------------ batched
 def run_batched(args):
     """--workload batched: args.instances independent CCSD instances of
@@ -507,7 +602,9 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=30)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', choices=['ccsd', 'ladder', 'batched', 'triples'], default='ccsd')
+    ap.add_argument('--workload', choices=['ccsd', 'ladder', 'batched', 'triples', 'learned'],
+                    default='ccsd')
+    ap.add_argument('--precond', default=PRECOND, help='learned workload: trained parameters (.npz)')
     ap.add_argument('--instances', type=int, default=64, help='batched workload: instance count')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--nocc', type=int, default=None, help=f'default {NOCC} (triples: {T3_NOCC})')
@@ -521,10 +618,11 @@ def main():
                     help='only run a plain device copy of MB megabytes (calibrates '
                          'FETCH_SIZE/WRITE_SIZE counters) and exit')
     args = ap.parse_args()
-    t3 = args.workload == 'triples'
-    args.nocc = args.nocc if args.nocc is not None else (T3_NOCC if t3 else NOCC)
-    args.nvir = args.nvir if args.nvir is not None else (T3_NVIR if t3 else NVIR)
-    args.naux = args.naux if args.naux is not None else (T3_NAUX if t3 else NAUX)
+    dflt = {'triples': (T3_NOCC, T3_NVIR, T3_NAUX), 'learned': (L_NOCC, L_NVIR, L_NAUX)}.get(
+        args.workload, (NOCC, NVIR, NAUX))
+    args.nocc = args.nocc if args.nocc is not None else dflt[0]
+    args.nvir = args.nvir if args.nvir is not None else dflt[1]
+    args.naux = args.naux if args.naux is not None else dflt[2]
     if args.gpus != 1:
         print(json.dumps({'error': 'this engine runs on one GPU; --gpus > 1 is not supported'}))
         sys.exit(2)
@@ -539,7 +637,7 @@ def main():
         print(json.dumps({'copy_bytes_read': 8 * n, 'copy_bytes_written': 8 * n, 'launches': 5}))
         return
     out = {'ladder': run_ladder, 'ccsd': run_ccsd, 'batched': run_batched,
-           'triples': run_triples}[args.workload](args)
+           'triples': run_triples, 'learned': run_learned}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

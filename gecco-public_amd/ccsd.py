@@ -31,6 +31,7 @@ P before H inside each side, as the list layout orders them):
   T1[a;i] = t_i^a, T2[ab;ij] = t_ij^ab, H<C><A>[c..;a..] = <c..||a..>,
   Fae[a;e], Fmi[m;i], Fme[m;e], Wmnij[mn;ij], Zamij[a m;ij], Wmbej[b m;e j].
 """
+import math
 import time
 
 import numpy as np
@@ -274,7 +275,7 @@ class CCSD:
             p.execute(self.bufs, stream)
 
     def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True,
-              mode='graph'):
+              mode='graph', precond=None, fallback=2.0):
         """Iterate to ||g|| < tol, g = D*t + R over T1|T2 (R: the formula's
         residual lists, D: the diagonal preconditioner), at most maxit
         iterations, with the native DIIS step (gecco_hip.DIIS).  restart:
@@ -288,11 +289,21 @@ class CCSD:
         tolerance again and leaves T unchanged).  'graph': as 'async', with
         the iteration (three plan executes and the DIIS step) captured once
         as a HIP graph and replayed.  Returns a dict with energy, iterations,
-        residual norm and timings."""
+        residual norm and timings.
+
+        precond: a learned.LearnedPreconditioner; each step then uses
+        r = s * g / D with s from the network (mode 'graph' runs as
+        'async': the iteration index is a kernel argument).  Fallback to the
+        plain preconditioner for the rest of the solve once ||g|| exceeds
+        `fallback` times its smallest value so far (from the fourth
+        iteration on) or is not finite; the result reports the iteration
+        ('fallback_at', None if the network stayed on)."""
         import gecco_hip
         torch = self.torch
         if mode not in ('sync', 'async', 'graph'):
             raise ValueError(f'unknown solve mode {mode!r}')
+        if precond is not None and mode == 'graph':
+            mode = 'async'
         want_async = mode != 'sync'
         d = getattr(self, '_diis', None)
         if restart or d is None or d.maxsub != ndiis or getattr(self, '_diis_async', None) != want_async:
@@ -303,6 +314,28 @@ class CCSD:
             self._graph = None
         diis = self._diis
         stream = torch.cuda.current_stream().cuda_stream
+        # learned preconditioner: per-amplitude factors s, refilled before
+        # every step
+        learn = {'on': precond is not None, 'best': float('inf'), 'at': None}
+        if precond is not None:
+            if getattr(self, '_scale', None) is None:
+                self._scale = torch.empty_like(self.amp)
+            diis.set_scale(self._scale)
+        else:
+            diis.set_scale(None)
+
+        def scale_step(k, s):
+            if learn['on']:
+                precond.scale(self.diag, self.res, self.amp, k, self._scale, s)
+
+        def watch(k, rn):
+            if not learn['on']:
+                return
+            if not math.isfinite(rn) or (k >= 4 and rn > fallback * learn['best']):
+                learn['on'] = False
+                learn['at'] = k
+                diis.set_scale(None)
+            learn['best'] = min(learn['best'], rn)
         energy = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -311,8 +344,10 @@ class CCSD:
         if mode == 'sync':
             for it in range(1, maxit + 1):
                 self.evaluate(stream)
+                scale_step(it - 1, stream)
                 rnorm, energy, conv = diis.step(self.amp, self.res, self.diag, energy=self.buf['E'],
                                                 tol=tol, stream=stream)
+                watch(it, rnorm)
                 if on_iter:
                     on_iter(it, energy, rnorm)
                 if verbose:
@@ -320,8 +355,9 @@ class CCSD:
                 if conv:
                     break
         else:
-            def iteration(s):
+            def iteration(s, k=0):
                 self.evaluate(s)
+                scale_step(k, s)
                 diis.step_async(self.amp, self.res, self.diag, energy=self.buf['E'], tol=tol, stream=s)
             graph = None
             if mode == 'graph':
@@ -335,7 +371,7 @@ class CCSD:
                 if graph is not None:
                     graph.replay()
                 else:
-                    iteration(stream)
+                    iteration(stream, it - 1)
                     if mode == 'graph':
                         # captured after one eager iteration (a plan's first
                         # execute uploads its buffer table and constant
@@ -355,6 +391,7 @@ class CCSD:
                     events[done].synchronize()
                     rnorm, energy, code = diis.status(base + done)
                     done += 1
+                    watch(done, rnorm)
                     if on_iter:
                         on_iter(done, energy, rnorm)
                     if verbose:
@@ -367,5 +404,8 @@ class CCSD:
                     break
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        diis.set_scale(None)
         return dict(energy=energy, iterations=it, residual=rnorm, seconds=wall,
-                    iters_per_s=it / wall, flops_per_iter=self.flops, kernel_calls=self.nblocks)
+                    iters_per_s=it / wall, flops_per_iter=self.flops, kernel_calls=self.nblocks,
+                    preconditioner='learned' if precond is not None else 'diagonal',
+                    fallback_at=learn['at'])

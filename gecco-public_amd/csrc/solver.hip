@@ -60,6 +60,7 @@ template <int M>
 __device__ __forceinline__ void prepare_m(const double *__restrict__ t,
                                           const double *__restrict__ rest,
                                           const double *__restrict__ diag,
+                                          const double *__restrict__ scale,
                                           double *__restrict__ rsub, double *__restrict__ vsub,
                                           int64_t n, const Slots &sl, double acc[MAXSUB + 1]) {
     double *__restrict__ rn = rsub + (int64_t)sl.s[M - 1] * n;
@@ -71,7 +72,8 @@ __device__ __forceinline__ void prepare_m(const double *__restrict__ t,
         const double ti = t[i], d = diag[i];
         const double g = d * ti + rest[i];
         const double dv = fabs(d) <= DIA_THRESH ? DIA_THRESH : d;
-        const double r = g / dv;
+        double r = g / dv;
+        if (scale) r *= scale[i];     // learned per-amplitude factor
         rn[i] = r;
         vn[i] = ti - r;
         acc[MAXSUB] += g * g;
@@ -84,13 +86,14 @@ __device__ __forceinline__ void prepare_m(const double *__restrict__ t,
 __device__ __forceinline__ void prepare_body(const double *__restrict__ t,
                                              const double *__restrict__ rest,
                                              const double *__restrict__ diag,
+                                             const double *__restrict__ scale,
                                              double *__restrict__ rsub, double *__restrict__ vsub,
                                              int64_t n, const Slots &sl, double *__restrict__ part) {
     double acc[MAXSUB + 1];
 #pragma unroll
     for (int j = 0; j <= MAXSUB; ++j) acc[j] = 0.0;
     switch (sl.m) {
-#define GECCO_PREP(M) case M: prepare_m<M>(t, rest, diag, rsub, vsub, n, sl, acc); break;
+#define GECCO_PREP(M) case M: prepare_m<M>(t, rest, diag, scale, rsub, vsub, n, sl, acc); break;
         GECCO_PREP(1) GECCO_PREP(2) GECCO_PREP(3) GECCO_PREP(4) GECCO_PREP(5) GECCO_PREP(6)
         GECCO_PREP(7) GECCO_PREP(8) GECCO_PREP(9) GECCO_PREP(10) GECCO_PREP(11) GECCO_PREP(12)
         GECCO_PREP(13) GECCO_PREP(14) GECCO_PREP(15) GECCO_PREP(16)
@@ -115,9 +118,10 @@ __device__ __forceinline__ void prepare_body(const double *__restrict__ t,
 
 __global__ __launch_bounds__(NTH) void diis_prepare_f64(
     const double *__restrict__ t, const double *__restrict__ rest,
-    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
+    const double *__restrict__ diag, const double *__restrict__ scale,
+    double *__restrict__ rsub, double *__restrict__ vsub,
     int64_t n, Slots sl, double *__restrict__ part) {
-    prepare_body(t, rest, diag, rsub, vsub, n, sl, part);
+    prepare_body(t, rest, diag, scale, rsub, vsub, n, sl, part);
 }
 
 // out[j] = sum over workgroups of part[g][j] in workgroup order; out[MAXSUB+1]
@@ -202,14 +206,15 @@ struct StatusRec {                // one per step, written by the device
 
 __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
     const double *__restrict__ t, const double *__restrict__ rest,
-    const double *__restrict__ diag, double *__restrict__ rsub, double *__restrict__ vsub,
+    const double *__restrict__ diag, const double *__restrict__ scale,
+    double *__restrict__ rsub, double *__restrict__ vsub,
     int64_t n, const DevState *__restrict__ st, double *__restrict__ part,
     const int32_t *__restrict__ flag) {
     if (flag && *flag) return;    // converged before this step: nothing to add
     Slots sl;
     sl.m = st->m;
     for (int j = 0; j < MAXSUB; ++j) sl.s[j] = j < sl.m ? st->order[j] : 0;
-    prepare_body(t, rest, diag, rsub, vsub, n, sl, part);
+    prepare_body(t, rest, diag, scale, rsub, vsub, n, sl, part);
 }
 
 // Reduction in workgroup order (as diis_reduce_f64), the new row of B, the
@@ -439,6 +444,7 @@ struct gecco_hip_diis {
     DevState *dstate = nullptr;                // bookkeeping of the asynchronous step
     StatusRec *ring = nullptr;                 // pinned status records (device-written)
     int32_t *flag = nullptr;                   // optional convergence flag (device)
+    const double *scale = nullptr;             // optional per-amplitude step factor (device)
     bool async = false;                        // steps so far were asynchronous
 };
 
@@ -532,8 +538,8 @@ int gecco_hip_diis_step(gecco_hip_diis *d, double *t, const double *rest, const 
     sl.m = m;
     for (int j = 0; j < m; ++j) sl.s[j] = d->order[j];
     const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
-    hipLaunchKernelGGL(diis_prepare_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub, d->vsub,
-                       d->n, sl, d->part);
+    hipLaunchKernelGGL(diis_prepare_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->scale, d->rsub,
+                       d->vsub, d->n, sl, d->part);
     hipLaunchKernelGGL(diis_reduce_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->red);
     if (check(hipGetLastError(), "diis kernels") ||
         check(hipMemcpyAsync(d->host, d->red, (MAXSUB + 2) * sizeof(double), hipMemcpyDeviceToHost, s),
@@ -583,12 +589,26 @@ int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, 
     hipStream_t s = (hipStream_t)stream;
     d->async = true;
     const int ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (d->n + NTH - 1) / NTH));
-    hipLaunchKernelGGL(diis_prepare_async_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->rsub,
-                       d->vsub, d->n, d->dstate, d->part, d->flag);
+    hipLaunchKernelGGL(diis_prepare_async_f64, dim3(ng), dim3(NTH), 0, s, t, rest, diag, d->scale,
+                       d->rsub, d->vsub, d->n, d->dstate, d->part, d->flag);
     hipLaunchKernelGGL(diis_finish_f64, dim3(1), dim3(NTH), 0, s, d->part, ng, energy_dev, d->dstate,
                        d->maxsub, tol, d->ring, d->flag);
     hipLaunchKernelGGL(diis_expand_async_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, d->dstate);
     return check(hipGetLastError(), "diis async kernels");
+}
+
+int gecco_hip_diis_set_scale(gecco_hip_diis *d, const double *scale) {
+    gecco::clear_error();
+    if (!d) return fail(GECCO_HIP_EINVAL, "null diis");
+    if (scale) {
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, scale) != hipSuccess || at.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            return fail(GECCO_HIP_EINVAL, "diis_set_scale: scale must be device memory");
+        }
+    }
+    d->scale = scale;
+    return 0;
 }
 
 int gecco_hip_diis_set_flag(gecco_hip_diis *d, int32_t *flag) {

@@ -315,7 +315,13 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
                  'gecco_hip_plan_add_reorder')
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
                   'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix',
-                  'gecco_hip_diis_step_async', 'gecco_hip_diis_status', 'gecco_hip_diis_set_flag')
+                  'gecco_hip_diis_step_async', 'gecco_hip_diis_status', 'gecco_hip_diis_set_flag',
+                  'gecco_hip_diis_set_scale')
+MLP_SYMBOLS = ('gecco_hip_mlp_infer', 'gecco_hip_mlp_scale', 'gecco_hip_mlp_grad', 'gecco_hip_mlp_adam')
+MLP_P = 17281
+MLP_SCALE_WORK = 2 + 2 * 512
+MLP_GRAD_GROUPS = 256
+MLP_GRAD_WORK = MLP_GRAD_GROUPS * MLP_P + 2 * MLP_GRAD_GROUPS
 VP = ctypes.c_void_p
 _lib.gecco_hip_orbs_create.argtypes = [I64, I64, PI64, I64, PI64, ctypes.POINTER(VP)]
 _lib.gecco_hip_orbs_create.restype = ctypes.c_int
@@ -365,6 +371,13 @@ _lib.gecco_hip_diis_step.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, PD, PD
 _lib.gecco_hip_diis_step_async.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, VP]
 _lib.gecco_hip_diis_status.argtypes = [VP, I64, PD, PD, PI64]
 _lib.gecco_hip_diis_set_flag.argtypes = [VP, VP]
+_lib.gecco_hip_diis_set_scale.argtypes = [VP, VP]
+_lib.gecco_hip_mlp_infer.argtypes = [VP, VP, I64, VP, VP]
+_lib.gecco_hip_mlp_scale.argtypes = [VP, VP, VP, VP, I64, ctypes.c_double, VP, VP, VP]
+_lib.gecco_hip_mlp_grad.argtypes = [VP, VP, VP, VP, I64, VP, VP, VP, VP]
+_lib.gecco_hip_mlp_adam.argtypes = [VP, VP, VP, VP, I64, ctypes.c_double, VP]
+for _n in MLP_SYMBOLS:
+    getattr(_lib, _n).restype = ctypes.c_int
 _lib.gecco_hip_diis_weights.argtypes = [VP, PD, I64, PI64]
 _lib.gecco_hip_diis_matrix.argtypes = [VP, PD, I64, PI64]
 for _n in SOLVER_SYMBOLS:
@@ -596,6 +609,14 @@ class DIIS:
         _check(_lib.gecco_hip_diis_step_async(self._h, _ptr(t), _ptr(rest), _ptr(diag),
                                               None if energy is None else _ptr(energy), float(tol), s))
 
+    def set_scale(self, scale):
+        """Device float64 vector of length n (or None): later steps use
+        r = scale * g / D (learned per-amplitude factor)."""
+        if scale is not None and (str(scale.dtype) != 'torch.float64' or scale.numel() != self.n):
+            raise GeccoHipError('set_scale: float64 tensor of the amplitude length expected')
+        _check(_lib.gecco_hip_diis_set_scale(self._h, None if scale is None else _ptr(scale)))
+        self._scale = scale
+
     def set_flag(self, flag):
         """One-element device int32 tensor (or None) that an asynchronous
         step sets to 1 when it meets tol; while it is set, later asynchronous
@@ -635,3 +656,54 @@ class DIIS:
                 self._h = VP()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ learned preconditioner
+def _mlp_check(t, dtype, n, what):
+    if str(t.dtype) != dtype or t.numel() != n or not t.is_contiguous() or not t.is_cuda:
+        raise GeccoHipError(f'{what}: contiguous {dtype} device tensor of {n} elements expected')
+
+
+def mlp_infer(params, x, stream=None):
+    """y for feature rows x (n x 4, float32 device tensor)."""
+    import torch
+    n = x.shape[0]
+    _mlp_check(params, 'torch.float32', MLP_P, 'mlp_infer params')
+    _mlp_check(x, 'torch.float32', 4 * n, 'mlp_infer x')
+    y = torch.empty(n, dtype=torch.float32, device=x.device)
+    _check(_lib.gecco_hip_mlp_infer(_ptr(params), _ptr(x), n, _ptr(y), VP(0 if stream is None else int(stream))))
+    return y
+
+
+def mlp_scale(params, diag, rest, t, it, scale, work, stream=None):
+    """scale = 1 + y(features of every amplitude); work: float64 device
+    tensor of MLP_SCALE_WORK elements."""
+    n = diag.numel()
+    _mlp_check(params, 'torch.float32', MLP_P, 'mlp_scale params')
+    for v, nm in ((diag, 'diag'), (rest, 'rest'), (t, 't'), (scale, 'scale')):
+        _mlp_check(v, 'torch.float64', n, f'mlp_scale {nm}')
+    _mlp_check(work, 'torch.float64', MLP_SCALE_WORK, 'mlp_scale work')
+    _check(_lib.gecco_hip_mlp_scale(_ptr(params), _ptr(diag), _ptr(rest), _ptr(t), n, float(it),
+                                    _ptr(scale), _ptr(work), VP(0 if stream is None else int(stream))))
+
+
+def mlp_grad(params, x, ut, dt, grad, loss, work, stream=None):
+    """Loss gradient over the pairs (x, ut, dt) into grad (float32,
+    MLP_P) and the loss into loss[0] (float64 device tensor or None)."""
+    n = ut.numel()
+    _mlp_check(params, 'torch.float32', MLP_P, 'mlp_grad params')
+    _mlp_check(x, 'torch.float32', 4 * n, 'mlp_grad x')
+    _mlp_check(ut, 'torch.float32', n, 'mlp_grad ut')
+    _mlp_check(dt, 'torch.float32', n, 'mlp_grad dt')
+    _mlp_check(grad, 'torch.float32', MLP_P, 'mlp_grad grad')
+    _mlp_check(work, 'torch.float32', MLP_GRAD_WORK, 'mlp_grad work')
+    _check(_lib.gecco_hip_mlp_grad(_ptr(params), _ptr(x), _ptr(ut), _ptr(dt), n, _ptr(grad),
+                                   None if loss is None else _ptr(loss), _ptr(work),
+                                   VP(0 if stream is None else int(stream))))
+
+
+def mlp_adam(params, m, v, grad, step, lr, stream=None):
+    for t, nm in ((params, 'params'), (m, 'm'), (v, 'v'), (grad, 'grad')):
+        _mlp_check(t, 'torch.float32', MLP_P, f'mlp_adam {nm}')
+    _check(_lib.gecco_hip_mlp_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(grad), int(step), float(lr),
+                                   VP(0 if stream is None else int(stream))))

@@ -78,6 +78,14 @@ int gecco_hip_diis_status(const gecco_hip_diis *d, int64_t step, double *gnorm, 
  * contractions and DIIS steps stop once it has converged.  No counterpart
  * in the reference. */
 int gecco_hip_diis_set_flag(gecco_hip_diis *d, int32_t *flag);
+/* Device vector scale[n] (NULL: none, the default): later steps use
+ * r = scale * g / D in place of r = g / D -- a per-amplitude factor on the
+ * preconditioned step, e.g. the learned preconditioner's s = 1 + y
+ * (gecco_hip_mlp_scale, gecco_hip_mlp.h).  ||g|| and the convergence test
+ * are unchanged.  The caller keeps the memory alive and fills it before
+ * each step.  No counterpart in the reference (baserout/diavc.f divides by
+ * D only). */
+int gecco_hip_diis_set_scale(gecco_hip_diis *d, const double *scale);
 
 /* Weights of the last step (w[0..m-1], oldest vector first) and the
  * current subspace matrix (b[m*m], row-major), for checks. */
