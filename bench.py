@@ -244,6 +244,13 @@ def run_ccsd(args):
     solver = ccsd.CCSD(ham)
     setup_s = time.perf_counter() - t0
     torch.cuda.synchronize()
+    tune_report, tune_s = None, 0.0
+    if args.tune:
+        t0 = time.perf_counter()
+        solver.solve(tol=0.0, maxit=3, mode='async')        # amplitudes of a few iterations
+        tune_report = solver.tune()
+        tune_s = time.perf_counter() - t0
+        print(f'# tuned in {tune_s:.1f} s: {tune_report}', file=sys.stderr, flush=True)
     # warmup iterations from T = 0 (the first one captures the iteration as
     # a HIP graph), then the timed steps continue the solve with replays
     solver.solve(tol=0.0, maxit=args.warmup)
@@ -297,6 +304,8 @@ def run_ccsd(args):
                         'abs_dE': None if ref is None else abs(res['energy'] - ref)},
         'setup_seconds': round(setup_s, 2),
     }
+    if tune_report is not None:
+        out['tuning'] = {'seconds': round(tune_s, 1), 'levels': tune_report}
     if not args.no_cpu:
         v, dt, sample = cpu_ccsd_sample(ham, solver.flops)
         out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
@@ -357,6 +366,12 @@ def run_triples(args):
         proj.add_sym_ab(solver.L[n], solver.slot[n])
     proj.finalize()
     proj.execute(solver.bufs)
+    tune_report, tune_s = None, 0.0
+    if args.tune:
+        t1 = time.perf_counter()
+        tune_report = solver.tune()
+        tune_s = time.perf_counter() - t1
+        print(f'# tuned in {tune_s:.1f} s: {tune_report}', file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         solver.evaluate()
     torch.cuda.synchronize()
@@ -399,6 +414,8 @@ def run_triples(args):
         'linearity_rel_err': lin,
         'setup_seconds': round(setup_s, 2),
     }
+    if tune_report is not None:
+        out['tuning'] = {'seconds': round(tune_s, 1), 'levels': tune_report}
     if not args.no_cpu:
         v, dt, sample = cpu_triples_sample(ham, solver.flops)
         out['cpu_baseline'] = {'value': round(v * solver.flops / 1e12, 6), 'unit': 'TFLOP/s', 'cores': 1,
@@ -607,6 +624,8 @@ def main():
     ap.add_argument('--precond', default=PRECOND, help='learned workload: trained parameters (.npz)')
     ap.add_argument('--instances', type=int, default=64, help='batched workload: instance count')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--tune', action='store_true',
+                    help='ccsd / triples workloads: autotune the launch settings of every formula level first')
     ap.add_argument('--nocc', type=int, default=None, help=f'default {NOCC} (triples: {T3_NOCC})')
     ap.add_argument('--nvir', type=int, default=None, help=f'default {NVIR} (triples: {T3_NVIR})')
     ap.add_argument('--naux', type=int, default=None, help=f'default {NAUX} (triples: {T3_NAUX})')

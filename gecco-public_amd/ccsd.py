@@ -190,19 +190,53 @@ class CCSD:
         self.t_setup_plans = time.perf_counter() - t0
         self.bufs = [self.buf[n] for n in lists]
 
-    def build_plans(self):
+    def build_plans(self, tuning=None):
         """One plan per scheduled level of this instance alone (buffer slots
-        = list indices)."""
+        = list indices); tuning: optional launch settings per level (dicts
+        of gecco_hip.Plan.set_tuning arguments, e.g. from tune())."""
         import gecco_hip
         self.plans = []
         self.flops = 0.0
         self.nblocks = 0
-        for level in self.schedule:
+        for li, level in enumerate(self.schedule):
             p = gecco_hip.Plan()
+            if tuning is not None and tuning[li]:
+                p.set_tuning(**tuning[li])
             self.record_level(p, level, 0)
             p.finalize()
             self.plans.append(p)
+        self.tuning = tuning
         self._graph = None
+
+    def tune(self, reps=5, axes=None):
+        """Pick each level's launch settings (tile variant, task split,
+        contraction chunk) by timing on the live buffers
+        (autotune.tune); returns the per-level table.  The amplitudes are
+        not changed (a level writes intermediates and residuals only)."""
+        import gecco_hip
+        import autotune
+        flops, nblocks = self.flops, self.nblocks
+        tuning, report = [], []
+        for li, level in enumerate(self.schedule):
+            def build(opts, level=level):
+                p = gecco_hip.Plan()
+                p.set_tuning(**opts)
+                self.record_level(p, level, 0)
+                p.finalize()
+                return p
+            # levels before this one leave their outputs in the buffers
+            for q in self.plans[:li]:
+                q.execute(self.bufs)
+            plan, opts, ms, table = autotune.tune(build, self.bufs, axes=axes or autotune.AXES,
+                                                  reps=reps)
+            self.plans[li] = plan
+            tuning.append(opts)
+            report.append({'level': li, 'chosen': opts, 'ms': round(ms, 4),
+                           'default_ms': round(table[0][1], 4), 'candidates': len(table)})
+        self.flops, self.nblocks = flops, nblocks
+        self.tuning = tuning
+        self._graph = None
+        return report
 
     def record_level(self, p, level, base):
         """Record one scheduled level of this instance into plan p, its

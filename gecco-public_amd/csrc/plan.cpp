@@ -1599,6 +1599,20 @@ int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot) {
     return 0;
 }
 
+int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
+                              int64_t min_kchunk, int64_t max_kchunk) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    int bm, bn, bk;
+    if (variant >= 0 && gecco_affine_variant_shape((int)variant, &bm, &bn, &bk))
+        return fail(GECCO_HIP_EINVAL, "unknown tile variant %lld", (long long)variant);
+    if (variant >= 0) plan->affine_variant = (int)variant;
+    if (target_tasks > 0) plan->target_tasks = target_tasks;
+    if (min_kchunk > 0) plan->min_kchunk = std::max<int64_t>(32, min_kchunk);
+    if (max_kchunk > 0) plan->max_kchunk = std::max<int64_t>(64, max_kchunk);
+    return 0;
+}
+
 int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group) {
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
     if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");

@@ -70,7 +70,7 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
              'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
-             'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip')
+             'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -81,6 +81,7 @@ _lib.gecco_hip_plan_destroy.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_add_block.argtypes = [ctypes.c_void_p, ctypes.POINTER(BlockArgs)]
 _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_set_group.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+_lib.gecco_hip_plan_set_tuning.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
 _lib.gecco_hip_plan_set_skip.argtypes = [ctypes.c_void_p, ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
                                         ctypes.c_void_p]
@@ -262,6 +263,13 @@ class Plan:
         through the same result map as one GEMM over their concatenated
         contraction ranges (default on); before finalize."""
         _check(_lib.gecco_hip_plan_set_chain(self._h, 1 if on else 0))
+
+    def set_tuning(self, variant=None, target_tasks=None, min_kchunk=None, max_kchunk=None):
+        """Strided-GEMM tile variant, task-split target and contraction
+        chunk bounds (None keeps the current value); before finalize."""
+        v = lambda x: -1 if x is None else int(x)
+        _check(_lib.gecco_hip_plan_set_tuning(self._h, v(variant), v(target_tasks), v(min_kchunk),
+                                              v(max_kchunk)))
 
     def set_group(self, group):
         """Record the following kernel calls, list initialisations and

@@ -208,6 +208,17 @@ int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
  * plans with trace steps refuse skip flags.  No counterpart in the
  * reference, which runs one problem per process. */
 int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
+/* Launch shape of the strided-GEMM kernel (before finalize; a negative or
+ * zero argument keeps the current value): tile variant (see
+ * gecco_hip_plan_block_info's tile shapes: 0 64x64x32, 1 64x64x16 (default),
+ * 2 64x128x16, 3 128x64x16, 4 128x128x16, 5 128x128x8), the number of tasks
+ * the contraction ranges are split towards (default 2048), and the shortest
+ * and longest contraction range of one task (defaults 256, 1024).  The
+ * autotuner (gecco-public_amd/autotune.py) picks them per plan by timing.
+ * No counterpart in the reference (its batching constants str_block and
+ * cnt_block are input keywords, contract/contr_blk1blk2_blocked_mm.f:184-227). */
+int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
+                              int64_t min_kchunk, int64_t max_kchunk);
 int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags);
 /* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
  * bytes they move per execute (one read and one write per element), and the

@@ -173,3 +173,23 @@ def test_batched_instances_match_separate_solves():
     res3 = b2.solve(tol=1e-10, maxit=80)
     assert max(abs(x - y) for x, y in zip(res3['energies'], res['energies'])) < 1e-12
     assert res3['iterations'] == res['iterations']
+
+
+def test_autotuned_plans_give_the_same_solve():
+    """Launch settings picked by the autotuner change only the schedule: the
+    tuned solve reaches the same energy in the same number of iterations."""
+    _torch()
+    import ccsd
+    h = synthetic(4, 12, 36, nsym=2, seed=5, sigma=0.1)
+    c = ccsd.CCSD(h)
+    ref = c.solve(tol=1e-10, maxit=80)
+    print('autotune small ccsd', flush=True)
+    report = c.tune(reps=2)
+    assert len(report) == len(c.plans) and all(r['candidates'] >= 5 for r in report)
+    res = c.solve(tol=1e-10, maxit=80)
+    assert res['iterations'] == ref['iterations']
+    assert abs(res['energy'] - ref['energy']) < 1e-11
+    # the same settings applied without timing
+    c.build_plans(c.tuning)
+    res2 = c.solve(tol=1e-10, maxit=80)
+    assert abs(res2['energy'] - ref['energy']) < 1e-11
