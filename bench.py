@@ -248,7 +248,7 @@ def run_ccsd(args):
     if args.tune:
         t0 = time.perf_counter()
         solver.solve(tol=0.0, maxit=3, mode='async')        # amplitudes of a few iterations
-        tune_report = solver.tune()
+        tune_report = solver.tune(log=lambda m: print('# tune ' + m, file=sys.stderr, flush=True))
         tune_s = time.perf_counter() - t0
         print(f'# tuned in {tune_s:.1f} s: {tune_report}', file=sys.stderr, flush=True)
     # warmup iterations from T = 0 (the first one captures the iteration as
@@ -369,7 +369,7 @@ def run_triples(args):
     tune_report, tune_s = None, 0.0
     if args.tune:
         t1 = time.perf_counter()
-        tune_report = solver.tune()
+        tune_report = solver.tune(log=lambda m: print('# tune ' + m, file=sys.stderr, flush=True))
         tune_s = time.perf_counter() - t1
         print(f'# tuned in {tune_s:.1f} s: {tune_report}', file=sys.stderr, flush=True)
     for _ in range(args.warmup):

@@ -32,6 +32,32 @@ def time_plan(plan, bufs, reps=5, warmup=2):
     return e0.elapsed_time(e1) / reps
 
 
+def tune_inplace(plan, bufs, axes=AXES, start=None, reps=5, margin=0.01, log=None):
+    """The same search on one plan that keeps its host data
+    (Plan.keep_host): every candidate is a Plan.retune, and the plan ends
+    with the best settings.  Returns (options, ms, table); log(text), if
+    given, is called after every candidate."""
+    best_opts = dict(DEFAULT if start is None else start)
+    plan.retune(**best_opts)
+    best_ms = time_plan(plan, bufs, reps)
+    table = [(dict(best_opts), best_ms)]
+    for name, values in axes:
+        for v in values:
+            if best_opts.get(name) == v:
+                continue
+            opts = dict(best_opts)
+            opts[name] = v
+            plan.retune(**opts)
+            ms = time_plan(plan, bufs, reps)
+            table.append((opts, ms))
+            if log:
+                log(f'{opts}: {ms:.4f} ms (best {best_ms:.4f})')
+            if ms < (1.0 - margin) * best_ms:
+                best_opts, best_ms = opts, ms
+    plan.retune(**best_opts)
+    return best_opts, best_ms, table
+
+
 def tune(build, bufs, axes=AXES, start=None, reps=5, margin=0.01):
     """build(options) -> finalized plan.  Returns (plan, options, ms, table)
     with table = [(options, ms), ...] of every candidate timed."""

@@ -208,27 +208,27 @@ class CCSD:
         self.tuning = tuning
         self._graph = None
 
-    def tune(self, reps=5, axes=None):
+    def tune(self, reps=5, axes=None, log=None):
         """Pick each level's launch settings (tile variant, task split,
         contraction chunk) by timing on the live buffers
-        (autotune.tune); returns the per-level table.  The amplitudes are
-        not changed (a level writes intermediates and residuals only)."""
+        (autotune.tune_inplace: the level is recorded once and finalized
+        again per candidate); returns the per-level table.  The amplitudes
+        are not changed (a level writes intermediates and residuals only)."""
         import gecco_hip
         import autotune
         flops, nblocks = self.flops, self.nblocks
         tuning, report = [], []
         for li, level in enumerate(self.schedule):
-            def build(opts, level=level):
-                p = gecco_hip.Plan()
-                p.set_tuning(**opts)
-                self.record_level(p, level, 0)
-                p.finalize()
-                return p
+            plan = gecco_hip.Plan()
+            plan.keep_host(True)
+            self.record_level(plan, level, 0)
+            plan.finalize()
             # levels before this one leave their outputs in the buffers
             for q in self.plans[:li]:
                 q.execute(self.bufs)
-            plan, opts, ms, table = autotune.tune(build, self.bufs, axes=axes or autotune.AXES,
-                                                  reps=reps)
+            opts, ms, table = autotune.tune_inplace(plan, self.bufs, axes=axes or autotune.AXES,
+                                                    reps=reps, log=log)
+            plan.keep_host(False)
             self.plans[li] = plan
             tuning.append(opts)
             report.append({'level': li, 'chosen': opts, 'ms': round(ms, 4),

@@ -509,6 +509,8 @@ struct gecco_hip_plan {
     // with skip flags set, every launch skips the tasks of group g while
     // skip[g] != 0 (device memory, read when each task starts)
     int32_t cur_group = 0, max_group = 0;
+    // keep the host index data after finalize (gecco_hip_plan_retune)
+    bool keep_host = false;
     const int32_t *d_skip = nullptr;
     int64_t nskip = 0;
 };
@@ -646,6 +648,26 @@ static void plan_free_device(gecco_hip_plan *p) {
     p->d_tasks = nullptr;
     p->d_index = nullptr;
     p->d_bufs = nullptr;
+}
+
+static void plan_release_host(gecco_hip_plan *p) {
+    for (auto &b : p->blocks)
+        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
+            std::vector<int32_t>().swap(o->r.first);
+            std::vector<int32_t>().swap(o->r.second);
+        }
+    for (auto &y : p->syms) {
+        std::vector<int64_t>().swap(y.i1);
+        std::vector<int64_t>().swap(y.i2);
+        std::vector<int8_t>().swap(y.f);
+    }
+    for (auto &tr : p->traces) {
+        std::vector<int64_t>().swap(tr.rowptr);
+        std::vector<int64_t>().swap(tr.rowdst);
+        std::vector<int64_t>().swap(tr.col);
+        std::vector<int8_t>().swap(tr.sgn);
+    }
+    p->keep_host = false;
 }
 
 static int plan_finalize(gecco_hip_plan *p) {
@@ -1094,10 +1116,12 @@ static int plan_finalize(gecco_hip_plan *p) {
             check(hipMemcpy(tr.d_col, tr.col.data(), tr.nnz * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
             check(hipMemcpy(tr.d_sgn, tr.sgn.data(), tr.nnz, hipMemcpyHostToDevice), "hipMemcpy(trace)"))
             return g_err;
-        std::vector<int64_t>().swap(tr.rowptr);
-        std::vector<int64_t>().swap(tr.rowdst);
-        std::vector<int64_t>().swap(tr.col);
-        std::vector<int8_t>().swap(tr.sgn);
+        if (!p->keep_host) {
+            std::vector<int64_t>().swap(tr.rowptr);
+            std::vector<int64_t>().swap(tr.rowdst);
+            std::vector<int64_t>().swap(tr.col);
+            std::vector<int8_t>().swap(tr.sgn);
+        }
     }
     if (!p->inits.empty()) {
         std::vector<DevInitTask> it;
@@ -1144,9 +1168,11 @@ static int plan_finalize(gecco_hip_plan *p) {
             for (int64_t p0 = 0; p0 < y.npair; p0 += chunk)
                 stasks.push_back({(int32_t)steps.size(), y.group, st.pair0 + p0, st.pair0 + std::min(y.npair, p0 + chunk)});
             steps.push_back(st);
-            std::vector<int64_t>().swap(y.i1);
-            std::vector<int64_t>().swap(y.i2);
-            std::vector<int8_t>().swap(y.f);
+            if (!p->keep_host) {
+                std::vector<int64_t>().swap(y.i1);
+                std::vector<int64_t>().swap(y.i2);
+                std::vector<int8_t>().swap(y.f);
+            }
             p->payload += 16.0 * (double)y.npair;
             p->index_bytes += 9.0 * (double)y.npair;
         }
@@ -1177,11 +1203,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->index_bytes += 4.0 * (double)pool.size();
     p->finalized = true;
     // host copies of the index tables are no longer needed
-    for (auto &b : p->blocks)
-        for (HostOperand *o : {&b.A, &b.B, &b.C}) {
-            std::vector<int32_t>().swap(o->r.first);
-            std::vector<int32_t>().swap(o->r.second);
-        }
+    if (!p->keep_host) plan_release_host(p);
     return ok();
 }
 
@@ -1610,6 +1632,36 @@ int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t tar
     if (target_tasks > 0) plan->target_tasks = target_tasks;
     if (min_kchunk > 0) plan->min_kchunk = std::max<int64_t>(32, min_kchunk);
     if (max_kchunk > 0) plan->max_kchunk = std::max<int64_t>(64, max_kchunk);
+    return 0;
+}
+
+int gecco_hip_plan_keep_host(gecco_hip_plan *plan, int64_t on) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (!plan->finalized) { plan->keep_host = on != 0; return 0; }
+    if (on) return fail(GECCO_HIP_ESTATE, "keep_host after finalize: the host data is released");
+    if (plan->keep_host) plan_release_host(plan);
+    return 0;
+}
+
+int gecco_hip_plan_retune(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks, int64_t min_kchunk,
+                          int64_t max_kchunk) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (!plan->finalized || !plan->keep_host)
+        return fail(GECCO_HIP_ESTATE, "retune needs a finalized plan that keeps its host data");
+    int bm, bn, bk;
+    if (variant >= 0 && gecco_affine_variant_shape((int)variant, &bm, &bn, &bk))
+        return fail(GECCO_HIP_EINVAL, "unknown tile variant %lld", (long long)variant);
+    plan_free_device(plan);
+    plan->finalized = false;
+    plan->static_done = false;
+    plan->last_bufs.clear();
+    plan->payload = plan->index_bytes = 0;
+    plan->nsymtasks = plan->nitasks = plan->ngtasks = plan->ngtasks_static = plan->ngjobs = 0;
+    if (variant >= 0) plan->affine_variant = (int)variant;
+    if (target_tasks > 0) plan->target_tasks = target_tasks;
+    if (min_kchunk > 0) plan->min_kchunk = std::max<int64_t>(32, min_kchunk);
+    if (max_kchunk > 0) plan->max_kchunk = std::max<int64_t>(64, max_kchunk);
+    if (plan_finalize(plan)) { plan_free_device(plan); return g_err; }
     return 0;
 }
 

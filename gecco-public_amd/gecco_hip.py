@@ -70,7 +70,8 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_block_info', 'gecco_hip_plan_set_materialize',
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
              'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
-             'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning')
+             'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning',
+             'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -82,6 +83,8 @@ _lib.gecco_hip_plan_add_block.argtypes = [ctypes.c_void_p, ctypes.POINTER(BlockA
 _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_set_group.argtypes = [ctypes.c_void_p, ctypes.c_int32]
 _lib.gecco_hip_plan_set_tuning.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
+_lib.gecco_hip_plan_keep_host.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_retune.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
 _lib.gecco_hip_plan_set_skip.argtypes = [ctypes.c_void_p, ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
                                         ctypes.c_void_p]
@@ -270,6 +273,18 @@ class Plan:
         v = lambda x: -1 if x is None else int(x)
         _check(_lib.gecco_hip_plan_set_tuning(self._h, v(variant), v(target_tasks), v(min_kchunk),
                                               v(max_kchunk)))
+
+    def keep_host(self, on=True):
+        """Before finalize: keep the host index data (for retune); after
+        finalize, keep_host(False) releases it."""
+        _check(_lib.gecco_hip_plan_keep_host(self._h, 1 if on else 0))
+
+    def retune(self, variant=None, target_tasks=None, min_kchunk=None, max_kchunk=None):
+        """Finalize again with other launch settings (plan kept its host
+        data)."""
+        v = lambda x: -1 if x is None else int(x)
+        _check(_lib.gecco_hip_plan_retune(self._h, v(variant), v(target_tasks), v(min_kchunk),
+                                          v(max_kchunk)))
 
     def set_group(self, group):
         """Record the following kernel calls, list initialisations and

@@ -219,6 +219,14 @@ int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
  * cnt_block are input keywords, contract/contr_blk1blk2_blocked_mm.f:184-227). */
 int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
                               int64_t min_kchunk, int64_t max_kchunk);
+/* keep_host(1) before finalize: the plan keeps its host index data, so that
+ * gecco_hip_plan_retune can finalize it again with other launch settings
+ * (arguments as set_tuning) without recording it again; keep_host(0) after
+ * finalize releases that data.  retune frees the plan's device data and
+ * finalizes anew (the next execute uploads the buffer table again). */
+int gecco_hip_plan_keep_host(gecco_hip_plan *plan, int64_t on);
+int gecco_hip_plan_retune(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
+                          int64_t min_kchunk, int64_t max_kchunk);
 int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags);
 /* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
  * bytes they move per execute (one read and one write per element), and the
