@@ -269,6 +269,18 @@ def run_ccsd(args):
     # time to convergence from T = 0
     torch.cuda.synchronize()
     res = solver.solve(tol=args.tol, maxit=200)
+    # mixed precision: float32 lists until ||g|| < 1e-6, then float64
+    # refinement; and the float32 iteration rate (graph replays)
+    t0 = time.perf_counter()
+    solver.build_f32()
+    f32_setup = time.perf_counter() - t0
+    mixed = solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200)
+    solver.solve(tol=0.0, maxit=args.warmup, precision='f32')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
+    torch.cuda.synchronize()
+    wall32 = time.perf_counter() - t0
     ref = None
     if os.path.exists(GOLDEN):
         g = json.load(open(GOLDEN))
@@ -302,6 +314,14 @@ def run_ccsd(args):
                         'residual': res['residual'],
                         'oracle_energy': ref,
                         'abs_dE': None if ref is None else abs(res['energy'] - ref)},
+        'mixed_precision': {'switch': 1e-6, 'f32_iterations': mixed['f32_iterations'],
+                            'f64_iterations': mixed['f64_iterations'],
+                            'seconds': round(mixed['seconds'], 4), 'energy': mixed['energy'],
+                            'residual': mixed['residual'],
+                            'abs_dE_vs_f64': abs(mixed['energy'] - res['energy']),
+                            'abs_dE_vs_oracle': None if ref is None else abs(mixed['energy'] - ref),
+                            'f32_iters_per_s': round(args.steps / wall32, 3),
+                            'f32_setup_seconds': round(f32_setup, 2)},
         'setup_seconds': round(setup_s, 2),
     }
     if tune_report is not None:
@@ -381,6 +401,28 @@ def run_triples(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kt = kernel_times(solver.plans, lambda: [solver.evaluate() for _ in range(args.steps)], args.steps)
+    # float32 lists (f32 MFMA): rate and error against the float64 R3
+    solver.build_f32()
+    for n in ('T2', 'T3'):
+        solver.buf32[n].copy_(solver.buf[n])
+    for _ in range(args.warmup):
+        for p in solver.plans32:
+            p.execute(solver.bufs32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in solver.plans32:
+            p.execute(solver.bufs32)
+    torch.cuda.synchronize()
+    wall32 = time.perf_counter() - t0
+    r64 = solver.buf['R3']
+    r32 = solver.buf32['R3'].double()
+    f32_err = {'max_abs_rel': float((r32 - r64).abs().max() / r64.abs().max()),
+               'norm_rel': float((r32 - r64).norm() / r64.norm())}
+    # error per kernel-call block of R3 is bounded by the list-wide maximum;
+    # the worst single element relative to its own magnitude above 1e-3 max
+    big = r64.abs() > 1e-3 * r64.abs().max()
+    f32_err['worst_element_rel'] = float(((r32 - r64).abs()[big] / r64.abs()[big]).max())
     # size-independent check: every term is linear in (T2, T3)
     r1 = solver.buf['R3'].clone()
     for n in ('T2', 'T3'):
@@ -412,6 +454,8 @@ def run_triples(args):
                                                       names[dom])),
                      'kernel': names[dom], 'kernel_ms': round(kt['dom_ms'], 5)},
         'linearity_rel_err': lin,
+        'f32': {'tflops': round(solver.flops * args.steps / wall32 / 1e12, 4),
+                'ms_per_step': round(wall32 * 1e3 / args.steps, 4), 'error_vs_f64': f32_err},
         'setup_seconds': round(setup_s, 2),
     }
     if tune_report is not None:

@@ -263,6 +263,29 @@ class CCSD:
             for res in level['sym']:
                 p.add_sym_ab(self.L[res], base + self.slot[res])
 
+    def solve_mixed(self, tol=1e-8, switch=1e-6, maxit=100, ndiis=8, mode='graph', verbose=False,
+                    stall=6):
+        """Mixed-precision solve: float32 residuals (solve(precision='f32'))
+        until ||g|| < switch (or the float32 residual stalls: `stall`
+        iterations without a 10% gain), then float64 iterations from those
+        amplitudes with a fresh DIIS subspace until ||g|| < tol -- iterative
+        refinement of the amplitude residual, whose converged point is the
+        float64 one.  At nocc=20, nvir=100 the float32 residual levels off
+        near 1e-6; a fresh subspace then needs fewer float64 iterations than
+        continuing the float32 one.
+        Returns the float64 phase's result with 'f32_iterations',
+        'f32_seconds' and the total 'seconds' / 'iterations'."""
+        r32 = self.solve(tol=switch, maxit=maxit, ndiis=ndiis, mode=mode, precision='f32',
+                         verbose=verbose, stall=stall)
+        r64 = self.solve(tol=tol, maxit=maxit, ndiis=ndiis, mode=mode, precision='f64',
+                         keep_amplitudes=True, verbose=verbose)
+        out = dict(r64)
+        out.update(f32_iterations=r32['iterations'], f32_seconds=r32['seconds'],
+                   f64_iterations=r64['iterations'], f64_seconds=r64['seconds'],
+                   iterations=r32['iterations'] + r64['iterations'],
+                   seconds=r32['seconds'] + r64['seconds'], precision='mixed', switch=switch)
+        return out
+
     # ------------------------------------------------------------ setup
     def _integrals(self):
         """<pq||rs> in every integral list from the DF factors, on the device."""
@@ -308,8 +331,47 @@ class CCSD:
         for p in self.plans:
             p.execute(self.bufs, stream)
 
+    # ------------------------------------------------------ float32 lists
+    def build_f32(self):
+        """Float32 copies of every list (integrals rounded once) and plans
+        that run the formula on them: the bulk of a mixed-precision solve."""
+        import gecco_hip
+        torch = self.torch
+        lists = self.formula['lists']
+        self.amp32 = torch.zeros(self.n1 + self.n2, dtype=torch.float32, device=self.dev)
+        self.res32 = torch.zeros_like(self.amp32)
+        self.buf32 = {}
+        for n in lists:
+            if n in ('T1', 'T2', 'R1', 'R2'):
+                v = self.amp32 if n[0] == 'T' else self.res32
+                self.buf32[n] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+            else:
+                self.buf32[n] = self.buf[n].to(torch.float32)
+        self.bufs32 = [self.buf32[n] for n in lists]
+        flops, nblocks = self.flops, self.nblocks
+        self.plans32 = []
+        for li, level in enumerate(self.schedule):
+            p = gecco_hip.Plan()
+            p.set_precision('f32')
+            self.record_level(p, level, 0)
+            p.finalize()
+            self.plans32.append(p)
+        self.flops, self.nblocks = flops, nblocks
+
+    def evaluate32(self, stream=None):
+        """evaluate() on the float32 lists: T1|T2 rounded to float32, the
+        residual and energy returned in float64."""
+        if getattr(self, 'plans32', None) is None:
+            self.build_f32()
+        self.amp32.copy_(self.amp)
+        for p in self.plans32:
+            p.execute(self.bufs32, stream)
+        self.res.copy_(self.res32)
+        self.buf['E'].copy_(self.buf32['E'])
+
     def solve(self, tol=1e-8, maxit=100, ndiis=8, verbose=False, on_iter=None, restart=True,
-              mode='graph', precond=None, fallback=2.0):
+              mode='graph', precond=None, fallback=2.0, precision='f64', keep_amplitudes=False,
+              stall=None):
         """Iterate to ||g|| < tol, g = D*t + R over T1|T2 (R: the formula's
         residual lists, D: the diagonal preconditioner), at most maxit
         iterations, with the native DIIS step (gecco_hip.DIIS).  restart:
@@ -331,7 +393,14 @@ class CCSD:
         plain preconditioner for the rest of the solve once ||g|| exceeds
         `fallback` times its smallest value so far (from the fourth
         iteration on) or is not finite; the result reports the iteration
-        ('fallback_at', None if the network stayed on)."""
+        ('fallback_at', None if the network stayed on).
+
+        precision 'f32': the residual comes from the float32 lists
+        (evaluate32); the amplitudes, the DIIS step and the convergence test
+        stay float64.  keep_amplitudes: with restart, start the new subspace
+        from the current amplitudes instead of T = 0 (see solve_mixed).
+        stall: also stop once ||g|| has not fallen below 0.9 times its best
+        value for `stall` iterations (the float32 residual's floor)."""
         import gecco_hip
         torch = self.torch
         if mode not in ('sync', 'async', 'graph'):
@@ -340,8 +409,14 @@ class CCSD:
             mode = 'async'
         want_async = mode != 'sync'
         d = getattr(self, '_diis', None)
+        if precision not in ('f64', 'f32'):
+            raise ValueError(f'unknown precision {precision!r}')
+        evaluate = self.evaluate32 if precision == 'f32' else self.evaluate
+        if precision == 'f32' and getattr(self, 'plans32', None) is None:
+            self.build_f32()
         if restart or d is None or d.maxsub != ndiis or getattr(self, '_diis_async', None) != want_async:
-            self.amp.zero_()
+            if not keep_amplitudes:
+                self.amp.zero_()
             self._diis = gecco_hip.DIIS(self.amp.numel(), ndiis)
             self._diis_async = want_async
             self._diis_steps = 0
@@ -362,6 +437,14 @@ class CCSD:
             if learn['on']:
                 precond.scale(self.diag, self.res, self.amp, k, self._scale, s)
 
+        prog = {'best': float('inf'), 'at': 0, 'stalled': False}
+
+        def stalled(k, rn):
+            if rn < 0.9 * prog['best']:
+                prog['best'], prog['at'] = rn, k
+            prog['stalled'] = stall is not None and k - prog['at'] >= stall
+            return prog['stalled']
+
         def watch(k, rn):
             if not learn['on']:
                 return
@@ -377,11 +460,13 @@ class CCSD:
         rnorm = float('inf')
         if mode == 'sync':
             for it in range(1, maxit + 1):
-                self.evaluate(stream)
+                evaluate(stream)
                 scale_step(it - 1, stream)
                 rnorm, energy, conv = diis.step(self.amp, self.res, self.diag, energy=self.buf['E'],
                                                 tol=tol, stream=stream)
                 watch(it, rnorm)
+                if stalled(it, rnorm):
+                    break
                 if on_iter:
                     on_iter(it, energy, rnorm)
                 if verbose:
@@ -390,12 +475,12 @@ class CCSD:
                     break
         else:
             def iteration(s, k=0):
-                self.evaluate(s)
+                evaluate(s)
                 scale_step(k, s)
                 diis.step_async(self.amp, self.res, self.diag, energy=self.buf['E'], tol=tol, stream=s)
             graph = None
             if mode == 'graph':
-                key = (tol, stream)
+                key = (tol, stream, precision)
                 if self._graph is not None and self._graph[0] == key:
                     graph = self._graph[1]
             events = []
@@ -414,7 +499,7 @@ class CCSD:
                         graph = torch.cuda.CUDAGraph()
                         with torch.cuda.graph(graph):
                             iteration(torch.cuda.current_stream().cuda_stream)
-                        self._graph = ((tol, stream), graph)
+                        self._graph = ((tol, stream, precision), graph)
                 ev = torch.cuda.Event()
                 ev.record()
                 events.append(ev)
@@ -426,6 +511,9 @@ class CCSD:
                     rnorm, energy, code = diis.status(base + done)
                     done += 1
                     watch(done, rnorm)
+                    if stalled(done, rnorm):
+                        conv = True
+                        break
                     if on_iter:
                         on_iter(done, energy, rnorm)
                     if verbose:
@@ -439,7 +527,8 @@ class CCSD:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         diis.set_scale(None)
-        return dict(energy=energy, iterations=it, residual=rnorm, seconds=wall,
+        return dict(energy=energy, iterations=it, residual=rnorm, seconds=wall, precision=precision,
+                    stalled=prog['stalled'],
                     iters_per_s=it / wall, flops_per_iter=self.flops, kernel_calls=self.nblocks,
                     preconditioner='learned' if precond is not None else 'diagonal',
                     fallback_at=learn['at'])

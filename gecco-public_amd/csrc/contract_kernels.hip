@@ -1,9 +1,9 @@
-// Grouped block-contraction kernel for gfx950 (MI355X), FP64.
+// Grouped block-contraction kernel for gfx950 (MI355X), FP64 or FP32 lists.
 //
 // One launch processes every (block, output tile, K chunk) task of a plan:
 //   gather A(k,m), B(k,n) tiles through signed index tables into LDS
 //   (replaces contract/collect_block_impr.f:2-415),
-//   FP64 MFMA 16x16x4 accumulation (replaces dgemm('t','n') at
+//   MFMA 16x16x4 accumulation in the list precision (replaces dgemm('t','n') at
 //   contract/contr_blk1blk2_blocked_mm.f:465,523),
 //   signed scatter-accumulate of the tile into the result list
 //   (replaces contract/scatter_block.f:2-174).
@@ -17,6 +17,25 @@
 namespace gecco {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Element types: lists in float64 or float32 (a plan's precision).  MFMA
+// 16x16x4 in the element type; accumulators of 4 elements per lane.
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef v4d type; };
+template <> struct Acc<float> { typedef v4f type; };
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Row of accumulator register r in lane group g = lane / 16 of a 16 x 16
+// MFMA tile (the column is lane % 16): f64 16x16x4 puts row 4 r + g in
+// register r, f32 16x16x4 row 4 g + r.
+template <typename T> __device__ __forceinline__ int mrow(int r, int g);
+template <> __device__ __forceinline__ int mrow<double>(int r, int g) { return 4 * r + g; }
+template <> __device__ __forceinline__ int mrow<float>(int r, int g) { return 4 * g + r; }
 
 // Global-memory loads, stores and atomics through global-address-space
 // pointers.  Operand, result and index pointers are read from device tables,
@@ -25,12 +44,19 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // for all outstanding prefetch loads and atomics as well.
 typedef const double __attribute__((address_space(1))) *gcdptr;
 typedef double __attribute__((address_space(1))) *gdptr;
+typedef const float __attribute__((address_space(1))) *gcfptr;
+typedef float __attribute__((address_space(1))) *gfptr;
 typedef const int32_t __attribute__((address_space(1))) *gciptr;
 __device__ __forceinline__ double ldg(const double *p) { return *(gcdptr)p; }
+__device__ __forceinline__ float ldg(const float *p) { return *(gcfptr)p; }
 __device__ __forceinline__ int32_t ldg(const int32_t *p) { return *(gciptr)p; }
 __device__ __forceinline__ void stg(double *p, double v) { *(gdptr)p = v; }
+__device__ __forceinline__ void stg(float *p, float v) { *(gfptr)p = v; }
 __device__ __forceinline__ void atomic_add_g(double *p, double v) {
     (void)__builtin_amdgcn_global_atomic_fadd_f64((gdptr)p, v);
+}
+__device__ __forceinline__ void atomic_add_g(float *p, float v) {
+    (void)__builtin_amdgcn_global_atomic_fadd_f32((gfptr)p, v);
 }
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
@@ -41,8 +67,9 @@ constexpr int LPT = BK * BM / NT;   // elements of each operand tile per thread
 // and have LPT different m/n, otherwise they share one m/n and have LPT k.  The
 // per-m/n half of a separable address is fixed for a task and computed once;
 // affine halves need no index load at all.
+template <typename T>
 struct OpView {
-    const double *base;
+    const T *base;
     const int32_t *first;
     int64_t ld, f0, f1;
     int table, faff;
@@ -53,7 +80,8 @@ struct OpView {
     int cval[LPT];       // element column valid and not forbidden
 };
 
-__device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const double *base, int x0,
+template <typename T>
+__device__ __forceinline__ void op_init(OpView<T> &v, const DevOperand &o, const T *base, int x0,
                                         int X, int tid) {
     v.base = base;
     v.first = o.first;
@@ -88,7 +116,8 @@ __device__ __forceinline__ void op_init(OpView &v, const DevOperand &o, const do
 // Issue the loads of one tile; the sign and the zero of forbidden or
 // out-of-range elements are applied at the LDS store (op_fix), so the loads
 // stay in flight while the MFMAs of the previous tile run.
-__device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double raw[LPT],
+template <typename T>
+__device__ __forceinline__ void op_load(const OpView<T> &v, int k0, int k1, T raw[LPT],
                                         uint32_t &flags) {
     flags = 0;
 #pragma unroll
@@ -116,9 +145,10 @@ __device__ __forceinline__ void op_load(const OpView &v, int k0, int k1, double 
     }
 }
 
-__device__ __forceinline__ double op_fix(double raw, uint32_t flags, int r) {
-    const double v = ((flags >> (r + LPT)) & 1u) ? -raw : raw;
-    return ((flags >> r) & 1u) ? v : 0.0;
+template <typename T>
+__device__ __forceinline__ T op_fix(T raw, uint32_t flags, int r) {
+    const T v = ((flags >> (r + LPT)) & 1u) ? -raw : raw;
+    return ((flags >> r) & 1u) ? v : (T)0;
 }
 
 // Instance groups: a task of group g >= 0 does nothing while skip[g] != 0.
@@ -126,11 +156,13 @@ __device__ __forceinline__ bool skipped(const int32_t *__restrict__ skip, int gr
     return skip != nullptr && grp >= 0 && ldg(skip + grp) != 0;
 }
 
-__global__ __launch_bounds__(NT) void grouped_contract_f64(
+template <typename T>
+__global__ __launch_bounds__(NT) void grouped_contract(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
-    __shared__ double As[BK][BM + 1];
-    __shared__ double Bs[BK][BN + 1];
+    T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
+    typedef typename Acc<T>::type AccT;
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -143,19 +175,19 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
         const DevBlock &b = blocks[task.blk];
         const int M = b.M, N = b.N;
         const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
-        OpView va, vb;
-        op_init(va, b.A, bufs[b.buf_a] + b.base_a, m0, M, tid);
-        op_init(vb, b.B, bufs[b.buf_b] + b.base_b, n0, N, tid);
+        OpView<T> va, vb;
+        op_init(va, b.A, (const T *)(bufs[b.buf_a] + b.base_a), m0, M, tid);
+        op_init(vb, b.B, (const T *)(bufs[b.buf_b] + b.base_b), n0, N, tid);
 
-        v4d acc[2][2];
+        AccT acc[2][2];
         for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < 2; ++j) acc[i][j] = AccT{0, 0, 0, 0};
         // bit 0: second 16-row half of the wave tile overlaps the block,
         // bit 1: second 16-column half does (the first halves always do or
         // the whole wave tile is padding, whose results are discarded)
         const uint32_t live = (uint32_t)(m0 + wm * 32 + 16 < M) | ((uint32_t)(n0 + wn * 32 + 16 < N) << 1);
 
-        double ra[LPT], rb[LPT];
+        T ra[LPT], rb[LPT];
         uint32_t fa, fb;
         op_load(va, task.k0, k1, ra, fa);
         op_load(vb, task.k0, k1, rb, fb);
@@ -174,24 +206,24 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 #pragma unroll
             for (int kq = 0; kq < BK / 4; ++kq) {
                 const int kr = kq * 4 + lrow;
-                double a0 = As[kr][wm * 32 + lcol];
-                double a1 = As[kr][wm * 32 + 16 + lcol];
-                double b0 = Bs[kr][wn * 32 + lcol];
-                double b1 = Bs[kr][wn * 32 + 16 + lcol];
-                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-                if (live & 2u) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-                if (live & 1u) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-                if (live == 3u) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+                T a0 = As[kr][wm * 32 + lcol];
+                T a1 = As[kr][wm * 32 + 16 + lcol];
+                T b0 = Bs[kr][wn * 32 + lcol];
+                T b1 = Bs[kr][wn * 32 + 16 + lcol];
+                acc[0][0] = mfma16(a0, b0, acc[0][0]);
+                if (live & 2u) acc[0][1] = mfma16(a0, b1, acc[0][1]);
+                if (live & 1u) acc[1][0] = mfma16(a1, b0, acc[1][0]);
+                if (live == 3u) acc[1][1] = mfma16(a1, b1, acc[1][1]);
             }
         }
 
-        // signed scatter-accumulate; output element (row, col) of a 16x16
-        // f64 MFMA tile sits in lane col + 16*(row%4), register row/4
+        // signed scatter-accumulate; register r of lane l holds row
+        // mrow<T>(r, l / 16), column l % 16 of its 16x16 MFMA tile
         const DevOperand oc = b.C;
-        double *pc = bufs[b.buf_c] + b.base_c;
-        const double xfac = b.xfac;
+        T *pc = bufs[b.buf_c] + b.base_c;
+        const T xfac = (T)b.xfac;
         int64_t addr[2][2][4];   // -1: padding or forbidden
-        double val[2][2][4];
+        T val[2][2][4];
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
             const int n = n0 + wn * 32 + ni * 16 + lcol;
@@ -203,7 +235,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 32 + mi * 16 + r * 4 + lrow;
+                    const int m = m0 + wm * 32 + mi * 16 + mrow<T>(r, lrow);
                     int64_t ad = -1;
                     bool neg = false;
                     if (m < M && n < N) {
@@ -221,7 +253,7 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                                      : (int64_t)(rr < 0 ? -rr : rr) + (int64_t)(cc < 0 ? -cc : cc) - 2;
                         }
                     }
-                    const double v = xfac * acc[mi][ni][r];
+                    const T v = xfac * acc[mi][ni][r];
                     addr[mi][ni][r] = ad;
                     val[mi][ni][r] = neg ? -v : v;
                 }
@@ -236,14 +268,14 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
                         if (addr[mi][ni][r] >= 0) atomic_add_g(pc + addr[mi][ni][r], val[mi][ni][r]);
         } else {
             // the task owns its result elements: loads first, then stores
-            double old[2][2][4];
+            T old[2][2][4];
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
                 for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        old[mi][ni][r] = addr[mi][ni][r] >= 0 ? ldg(pc + addr[mi][ni][r]) : 0.0;
+                        old[mi][ni][r] = addr[mi][ni][r] >= 0 ? ldg(pc + addr[mi][ni][r]) : (T)0;
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
@@ -275,12 +307,16 @@ __global__ __launch_bounds__(NT) void grouped_contract_f64(
 constexpr uint32_t BUF_OOB = 0x80000000u;
 constexpr int BUF_FLAGS = 0x00020000;   // raw buffer, gfx9-family descriptor word 3
 
-__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double &v) {
+    v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float &v) {
+    v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
-template <int X, int TBK>
+template <int X, int TBK, typename T>
 struct AffView {
+    static constexpr int ES = (int)sizeof(T);
     static constexpr int L = TBK * X / NT;  // elements per thread
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t off[L];  // byte offset of element r at the current K step
@@ -290,7 +326,7 @@ struct AffView {
 
     // operand block at blk (element (k, x) at o.f0 + o.s0 + o.f1*k + o.s1*x,
     // k < K, x < Xn); the first step starts at k = kstart
-    __device__ __forceinline__ void init(const double *blk, const DevOperand &o, int K, int x0,
+    __device__ __forceinline__ void init(const T *blk, const DevOperand &o, int K, int x0,
                                          int Xn, int kstart, int tid) {
         if (o.fast_first) {
             kk0 = tid % TBK; kstep = 0;
@@ -299,35 +335,35 @@ struct AffView {
             kk0 = tid / X; kstep = NT / X;
             xi0 = tid % X; xstep = 0;
         }
-        const int64_t extent = (o.f1 * (int64_t)(K - 1) + o.s1 * (int64_t)(Xn - 1) + 1) * 8;
+        const int64_t extent = (o.f1 * (int64_t)(K - 1) + o.s1 * (int64_t)(Xn - 1) + 1) * ES;
         rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)(blk + o.f0 + o.s0), (short)0, (int)extent,
                                                  BUF_FLAGS);
-        step = (uint32_t)(o.f1 * TBK * 8);
+        step = (uint32_t)(o.f1 * TBK * ES);
 #pragma unroll
         for (int r = 0; r < L; ++r) {
             const int x = x0 + xi0 + xstep * r, k = kstart + kk0 + kstep * r;
-            off[r] = x < Xn ? (uint32_t)((o.f1 * (int64_t)k + o.s1 * (int64_t)x) * 8) : BUF_OOB;
+            off[r] = x < Xn ? (uint32_t)((o.f1 * (int64_t)k + o.s1 * (int64_t)x) * ES) : BUF_OOB;
         }
     }
     // loads the step starting at block-local k0 (the range ends at k1) and
     // advances to the next step
-    __device__ __forceinline__ void load(int k0, int k1, double out[L]) {
+    __device__ __forceinline__ void load(int k0, int k1, T out[L]) {
         if (k0 + TBK <= k1) {
 #pragma unroll
             for (int r = 0; r < L; ++r) {
-                out[r] = buf_load_f64(rsrc, off[r]);
+                buf_load(rsrc, off[r], out[r]);
                 off[r] += step;
             }
         } else {
 #pragma unroll
             for (int r = 0; r < L; ++r) {
-                out[r] = buf_load_f64(rsrc, k0 + kk0 + kstep * r < k1 ? off[r] : BUF_OOB);
+                buf_load(rsrc, k0 + kk0 + kstep * r < k1 ? off[r] : BUF_OOB, out[r]);
                 off[r] += step;
             }
         }
     }
     template <int LD>
-    __device__ __forceinline__ void store(double *lds, const double in[L]) const {
+    __device__ __forceinline__ void store(T *lds, const T in[L]) const {
 #pragma unroll
         for (int r = 0; r < L; ++r) lds[(kk0 + kstep * r) * LD + xi0 + xstep * r] = in[r];
     }
@@ -369,12 +405,14 @@ __device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff
 // with the operands loaded from memory straight into the MFMA lane layout
 // (lane: k = lane / 16, m or n = lane % 16) and the next step's loads in
 // flight during the current step's MFMAs; no LDS traffic and no barrier per
-// step.  The four partial tiles are summed in LDS (1024 doubles) at the end.
+// step.  The four partial tiles are summed in LDS (1024 elements) at the end.
 // A step covers 16 * SKINNY_U consecutive k: 4 * SKINNY_U per wave.
 constexpr int SKINNY_U = 4;
 
+template <typename T>
 __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock *__restrict__ blocks,
-                                            double *const *__restrict__ bufs, double *lds, int tid) {
+                                            T *const *__restrict__ bufs, T *lds, int tid) {
+    typedef typename Acc<T>::type AccT;
     const int lane = tid & 63, wave = tid >> 6;
     const int km = lane >> 4, xl = lane & 15;
     const DevBlock &b = blocks[task.blk];
@@ -382,36 +420,36 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
     const int m = task.m0 + xl, n = task.n0 + xl;
     const bool mok = m < M, nok = n < N;
     const int koff = 4 * SKINNY_U * wave + km;   // k of MFMA u: step start + koff + 4u
-    v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+    AccT acc = AccT{0, 0, 0, 0};
     // range [task.k0, task.k1) from the start of block task.blk, along its chain
     int blk = task.blk, kb = 0, kl = task.k0;
     for (;;) {
         const DevBlock &cb = blocks[blk];
         const int kh = min(cb.K, task.k1 - kb);
         const DevOperand &oa = cb.A, &ob = cb.B;
-        const double *__restrict__ qa =
+        const T *__restrict__ qa =
             bufs[cb.buf_a] + cb.base_a + oa.f0 + oa.s0 + (mok ? oa.s1 * (int64_t)m : 0);
-        const double *__restrict__ qb =
+        const T *__restrict__ qb =
             bufs[cb.buf_b] + cb.base_b + ob.f0 + ob.s0 + (nok ? ob.s1 * (int64_t)n : 0);
         const int64_t fa = oa.f1, fb = ob.f1;
-        double a[SKINNY_U], bv[SKINNY_U];
-        auto load = [&](int k0, double *ra, double *rb) {
+        T a[SKINNY_U], bv[SKINNY_U];
+        auto load = [&](int k0, T *ra, T *rb) {
 #pragma unroll
             for (int u = 0; u < SKINNY_U; ++u) {
                 const int k = k0 + koff + 4 * u;
                 const bool ok = k < kh;
-                ra[u] = (ok && mok) ? ldg(qa + fa * (int64_t)k) : 0.0;
-                rb[u] = (ok && nok) ? ldg(qb + fb * (int64_t)k) : 0.0;
+                ra[u] = (ok && mok) ? ldg(qa + fa * (int64_t)k) : (T)0;
+                rb[u] = (ok && nok) ? ldg(qb + fb * (int64_t)k) : (T)0;
             }
         };
         load(kl, a, bv);
         for (int k0 = kl; k0 < kh; k0 += 16 * SKINNY_U) {
-            double an[SKINNY_U], bn[SKINNY_U];
+            T an[SKINNY_U], bn[SKINNY_U];
             const bool more = k0 + 16 * SKINNY_U < kh;
             if (more) load(k0 + 16 * SKINNY_U, an, bn);
 #pragma unroll
             for (int u = 0; u < SKINNY_U; ++u)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], acc, 0, 0, 0);
+                acc = mfma16(a[u], bv[u], acc);
             if (more) {
 #pragma unroll
                 for (int u = 0; u < SKINNY_U; ++u) { a[u] = an[u]; bv[u] = bn[u]; }
@@ -427,18 +465,18 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
     for (int r = 0; r < 4; ++r) lds[wave * 256 + r * 64 + lane] = acc[r];
     __syncthreads();
     // thread tid owns tile element (r, lane) = (tid / 64, tid % 64):
-    // m = m0 + 4 r + lane / 16, n = n0 + lane % 16
-    const double sum = lds[tid] + lds[256 + tid] + lds[512 + tid] + lds[768 + tid];
-    const int mm = task.m0 + (tid >> 6) * 4 + (lane >> 4), nn = task.n0 + (lane & 15);
+    // m = m0 + mrow(r, lane / 16), n = n0 + lane % 16
+    const T sum = lds[tid] + lds[256 + tid] + lds[512 + tid] + lds[768 + tid];
+    const int mm = task.m0 + mrow<T>(tid >> 6, lane >> 4), nn = task.n0 + (lane & 15);
     if (mm < M && nn < N) {
         const DevOperand &oc = b.C;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
         bool neg;
         const int64_t a = result_offset(oc, caff, mm, nn, neg);
         if (a >= 0) {
-            double *pc = bufs[b.buf_c] + b.base_c;
-            const double x = b.xfac * sum;
-            const double v = neg ? -x : x;
+            T *pc = bufs[b.buf_c] + b.base_c;
+            const T x = (T)b.xfac * sum;
+            const T v = neg ? -x : x;
             if (b.atomic) atomic_add_g(pc + a, v);
             else stg(pc + a, ldg(pc + a) + v);
         }
@@ -456,13 +494,14 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
 // (nl, ml) owns column n0 + nl and rows m0 + ml + ng * i, i < rpt.  The A
 // tile (K x dm) is staged in LDS, the thread's B column in registers.  The
 // result map is read n-fastest (table = 2 stores the table transposed).
-constexpr int DIRECT_A = 32 * 32;        // LDS doubles for the A tile
-constexpr int DIRECT_STAGE = 4096;       // LDS doubles for a staged result range
+constexpr int DIRECT_A = 32 * 32;        // LDS elements for the A tile
+constexpr int DIRECT_STAGE = 4096;       // LDS elements for a staged result range
 
+template <typename T>
 __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock *__restrict__ blocks,
-                                            double *const *__restrict__ bufs, double *lds, int tid) {
-    double *const As = lds;
-    double *const Cs = lds + DIRECT_A;
+                                            T *const *__restrict__ bufs, T *lds, int tid) {
+    T *const As = lds;
+    T *const Cs = lds + DIRECT_A;
     const DevBlock &b = blocks[task.blk];
     const int M = b.M, N = b.N, K = b.K;
     const int dn = task.pad & 511, ng = (task.pad >> 9) & 511, rpt = task.pad >> 18;
@@ -471,17 +510,17 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
     const bool active = ml < ng;
     const int m0 = task.m0, n = task.n0 + nl;
     const DevOperand &oa = b.A, &ob = b.B, &oc = b.C;
-    const double *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
-    const double *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
-    double *pc = bufs[b.buf_c] + b.base_c;
+    const T *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
+    const T *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
+    T *pc = bufs[b.buf_c] + b.base_c;
     __syncthreads();   // LDS of the previous task is no longer read
     for (int e = tid; e < K * dm; e += NT) {
         const int k = e / dm, r = e - k * dm;
         const int m = m0 + r;
-        As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : 0.0;
+        As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : (T)0;
     }
     const int span = task.span;
-    for (int i = tid; i < span; i += NT) Cs[i] = 0.0;
+    for (int i = tid; i < span; i += NT) Cs[i] = (T)0;
     __syncthreads();
     // result column data (separable / affine maps)
     int64_t cadd = 0;
@@ -497,14 +536,14 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
         }
     }
     // K outer, rpt rows inner: one B value live at a time
-    double acc[8];
+    T acc[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+    for (int i = 0; i < 8; ++i) acc[i] = (T)0;
     if (cok) {
-        const double *__restrict__ pbn = pb + ob.s1 * (int64_t)n;
+        const T *__restrict__ pbn = pb + ob.s1 * (int64_t)n;
 #pragma unroll 4
         for (int k = 0; k < K; ++k) {
-            const double bk = ldg(pbn + ob.f1 * (int64_t)k);
+            const T bk = ldg(pbn + ob.f1 * (int64_t)k);
 #pragma unroll
             for (int i = 0; i < 8; ++i)
                 if (i < rpt) acc[i] = fma(As[k * dm + ml + ng * i], bk, acc[i]);
@@ -531,21 +570,21 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
             a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
         }
         if (a < 0) continue;
-        const double x = b.xfac * acc[i];
-        const double v = neg ? -x : x;
+        const T x = (T)b.xfac * acc[i];
+        const T v = neg ? -x : x;
         if (span > 0) atomicAdd(Cs + (a - task.lo), v);
         else atomic_add_g(pc + a, v);
     }
     if (span > 0) {
         __syncthreads();
-        double *pr = pc + task.lo;
+        T *pr = pc + task.lo;
         if (task.owned) {
             for (int i0 = 0; i0 < span; i0 += 4 * NT) {
-                double o[4];
+                T o[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int i = i0 + u * NT + tid;
-                    o[u] = i < span ? ldg(pr + i) : 0.0;
+                    o[u] = i < span ? ldg(pr + i) : (T)0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -555,8 +594,8 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
             }
         } else {
             for (int i = tid; i < span; i += NT) {
-                const double v = Cs[i];
-                if (v != 0.0) atomic_add_g(pr + i, v);
+                const T v = Cs[i];
+                if (v != (T)0) atomic_add_g(pr + i, v);
             }
         }
     }
@@ -564,19 +603,20 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 
 // 128 x 128 tiles hold 128 accumulator registers per lane: the launch bound
 // asks for 2 waves per SIMD (at most 256 registers per lane in total).
-template <int TBM, int TBN, int TBK>
-__global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine_f64(
+template <int TBM, int TBN, int TBK, typename T>
+__global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
+    T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
+    typedef typename Acc<T>::type AccT;
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int TM = TBM / 32, TN = TBN / 32;
     constexpr int SA = TBK * LDA, SB = TBK * LDB;
     // double-buffered A and B tiles; after the K loop the array holds half
     // of the result tile (epilogue), or the partial tiles of a skinny task;
-    // a direct task uses DIRECT_A + DIRECT_STAGE doubles
+    // a direct task uses DIRECT_A + DIRECT_STAGE elements
     constexpr int LDS_T = 2 * SA + 2 * SB > (TBM / 2) * (TBN + 1) ? 2 * SA + 2 * SB : (TBM / 2) * (TBN + 1);
     constexpr int LDS_N = LDS_T > DIRECT_A + DIRECT_STAGE ? LDS_T : DIRECT_A + DIRECT_STAGE;
-    __shared__ double lds[LDS_N];
+    __shared__ T lds[LDS_N];
     static_assert(LDS_N >= 1024, "LDS too small for the skinny-task reduction");
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -605,23 +645,23 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         // start; [kl, kh): block-local range of the current step's block.
         int blk = task.blk, kb = 0, kcur = b.K;
         int kl = task.k0, kh = min(kcur, task.k1);
-        AffView<TBM, TBK> va;
-        AffView<TBN, TBK> vb;
+        AffView<TBM, TBK, T> va;
+        AffView<TBN, TBK, T> vb;
         va.init(bufs[b.buf_a] + b.base_a, b.A, kcur, m0, M, kl, tid);
         vb.init(bufs[b.buf_b] + b.base_b, b.B, kcur, n0, N, kl, tid);
 
-        v4d acc[TM][TN];
+        AccT acc[TM][TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
         // a wave whose 16-row or 16-column MFMA sub-tiles all lie outside
         // the block skips its MFMAs (scalar branch); otherwise every
         // sub-tile is computed, padding rows and columns from the zeros that
         // the masked loads put into LDS
         const bool wave_live = __builtin_amdgcn_readfirstlane(
             (m0 + wm * (TBM / 2) < M && n0 + wn * (TBN / 2) < N) ? 1 : 0);
-        double ra[AffView<TBM, TBK>::L], rb[AffView<TBN, TBK>::L];
+        T ra[AffView<TBM, TBK, T>::L], rb[AffView<TBN, TBK, T>::L];
         __syncthreads();   // LDS of the previous task is no longer read
         va.load(kl, kh, ra);
         vb.load(kl, kh, rb);
@@ -650,13 +690,13 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                 va.load(kl, kh, ra);
                 vb.load(kl, kh, rb);
             }
-            const double *A = lds + cur * SA;
-            const double *B = lds + 2 * SA + cur * SB;
+            const T *A = lds + cur * SA;
+            const T *B = lds + 2 * SA + cur * SB;
             if (wave_live) {
 #pragma unroll
                 for (int kq = 0; kq < TBK / 4; ++kq) {
                     const int kr = kq * 4 + lrow;
-                    double a[TM], bb[TN];
+                    T a[TM], bb[TN];
 #pragma unroll
                     for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
 #pragma unroll
@@ -665,7 +705,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     for (int i = 0; i < TM; ++i)
 #pragma unroll
                         for (int j = 0; j < TN; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = mfma16(a[i], bb[j], acc[i][j]);
                 }
             }
             if (more) {
@@ -678,8 +718,8 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         }
 
         const DevOperand &oc = b.C;
-        double *pc = bufs[b.buf_c] + b.base_c;
-        const double xfac = b.xfac;
+        T *pc = bufs[b.buf_c] + b.base_c;
+        const T xfac = (T)b.xfac;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
         // The tile goes through LDS as a row-major [row][col] array (row
         // stride TBN + 1 against bank conflicts), one half of the rows --
@@ -701,7 +741,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     for (int j = 0; j < TN; ++j)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            lds[(i * 16 + r * 4 + lrow) * LDE + wn * (TBN / 2) + j * 16 + lcol] =
+                            lds[(i * 16 + mrow<T>(r, lrow)) * LDE + wn * (TBN / 2) + j * 16 + lcol] =
                                 acc[i][j][r];
             }
             __syncthreads();   // (h = 0: the K loop ended with a barrier)
@@ -713,7 +753,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                 bool neg;
                 const int64_t a = result_offset(oc, caff, m, n, neg);
                 if (a < 0) continue;
-                const double x = xfac * lds[row * LDE + col];
+                const T x = xfac * lds[row * LDE + col];
                 atomic_add_g(pc + a, neg ? -x : x);
             }
         }
@@ -722,15 +762,16 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
 
 // Signed gather of operands into scratch (one launch for all jobs of a
 // plan); HBM-bound: one read of each allowed source element, one write.
-__global__ __launch_bounds__(256) void gather_operands_f64(
+template <typename T>
+__global__ __launch_bounds__(256) void gather_operands(
     const DevGather *__restrict__ jobs, const DevGatherTask *__restrict__ tasks, int ntasks,
-    double *const *__restrict__ bufs, double *__restrict__ scratch,
+    T *const *__restrict__ bufs, T *__restrict__ scratch,
     const int32_t *__restrict__ skip) {
     const DevGatherTask t = tasks[blockIdx.x];
     if (skipped(skip, t.grp)) return;
     const DevGather &g = jobs[t.job];
-    const double *__restrict__ src = bufs[g.src_buf] + g.src_base;
-    double *__restrict__ dst = scratch + g.dst_base;
+    const T *__restrict__ src = bufs[g.src_buf] + g.src_base;
+    T *__restrict__ dst = scratch + g.dst_base;
     const DevOperand &o = g.src;
     const uint32_t inner = g.kfast ? (uint32_t)g.K : (uint32_t)g.X;
     // signed source offset of element e (-1: forbidden); U elements per
@@ -769,9 +810,9 @@ __global__ __launch_bounds__(256) void gather_operands_f64(
             const uint32_t e = e0 + u * blockDim.x;
             addr[u] = e < (uint32_t)t.e1 ? source(e, neg[u]) : -1;
         }
-        double val[U];
+        T val[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) val[u] = addr[u] >= 0 ? ldg(src + addr[u]) : 0.0;
+        for (int u = 0; u < U; ++u) val[u] = addr[u] >= 0 ? ldg(src + addr[u]) : (T)0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t e = e0 + u * blockDim.x;
@@ -781,25 +822,26 @@ __global__ __launch_bounds__(256) void gather_operands_f64(
 }
 
 // List initialisation (copy with factor, or zero) ahead of the contractions.
-__global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict__ jobs,
+template <typename T>
+__global__ __launch_bounds__(256) void init_lists(const DevInit *__restrict__ jobs,
                                                       const DevInitTask *__restrict__ tasks,
-                                                      double *const *__restrict__ bufs,
+                                                      T *const *__restrict__ bufs,
                                                       const int32_t *__restrict__ skip) {
     const DevInitTask t = tasks[blockIdx.x];
     if (skipped(skip, t.grp)) return;
     const DevInit &j = jobs[t.job];
-    double *__restrict__ dst = bufs[j.dst] + j.off_dst;
+    T *__restrict__ dst = bufs[j.dst] + j.off_dst;
     // U elements per thread and round, their loads in flight together
     constexpr int U = 4;
     if (j.src >= 0) {
-        const double *__restrict__ src = bufs[j.src] + j.off_src;
-        const double f = j.factor;
+        const T *__restrict__ src = bufs[j.src] + j.off_src;
+        const T f = (T)j.factor;
         for (int64_t e0 = t.e0 + threadIdx.x; e0 < t.e1; e0 += U * blockDim.x) {
-            double x[U];
+            T x[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int64_t e = e0 + u * blockDim.x;
-                x[u] = e < t.e1 ? ldg(src + e) : 0.0;
+                x[u] = e < t.e1 ? ldg(src + e) : (T)0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -808,14 +850,15 @@ __global__ __launch_bounds__(256) void init_lists_f64(const DevInit *__restrict_
             }
         }
     } else {
-        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) stg(dst + e, 0.0);
+        for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) stg(dst + e, (T)0);
     }
 }
 
 // Partial trace rows: dst[row] += fact * sum_k sign_k * src[col_k] over the
 // row's entries (CSR, one thread per row, fixed order: no atomics).
-__global__ __launch_bounds__(256) void trace_rows_f64(const double *__restrict__ src,
-                                                      double *__restrict__ dst,
+template <typename T>
+__global__ __launch_bounds__(256) void trace_rows(const T *__restrict__ src,
+                                                      T *__restrict__ dst,
                                                       const int64_t *__restrict__ rowptr,
                                                       const int64_t *__restrict__ rowdst,
                                                       const int64_t *__restrict__ col,
@@ -823,21 +866,22 @@ __global__ __launch_bounds__(256) void trace_rows_f64(const double *__restrict__
                                                       double fact) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
          r += (int64_t)gridDim.x * blockDim.x) {
-        double acc = 0.0;
+        T acc = (T)0;
         for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
-            const double v = src[col[k]];
+            const T v = src[col[k]];
             acc += sgn[k] < 0 ? -v : v;
         }
-        dst[rowdst[r]] += fact * acc;
+        dst[rowdst[r]] += (T)fact * acc;
     }
 }
 
 // All spin-flip symmetrisation steps of a plan in one launch (lists of one
 // plan are distinct buffers or disjoint ranges; pairs of one list are
 // disjoint because the flip is an involution).
-__global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__restrict__ steps,
+template <typename T>
+__global__ __launch_bounds__(256) void sym_ab_batched(const DevSymStep *__restrict__ steps,
                                                           const DevSymTask *__restrict__ tasks,
-                                                          double *const *__restrict__ bufs,
+                                                          T *const *__restrict__ bufs,
                                                           const int32_t *__restrict__ i1,
                                                           const int32_t *__restrict__ i2,
                                                           const int8_t *__restrict__ fsign,
@@ -845,12 +889,12 @@ __global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__re
     const DevSymTask t = tasks[blockIdx.x];
     if (skipped(skip, t.grp)) return;
     const DevSymStep &st = steps[t.step];
-    double *__restrict__ x = bufs[st.buf] + st.base;
-    const double prefac = st.prefac;
+    T *__restrict__ x = bufs[st.buf] + st.base;
+    const T prefac = (T)st.prefac;
     for (int64_t p = t.p0 + threadIdx.x; p < t.p1; p += blockDim.x) {
         const int64_t a = ldg(i1 + p), c = ldg(i2 + p);
-        const double f = (double)fsign[p];
-        const double v = prefac * (ldg(x + a) + f * ldg(x + c));
+        const T f = (T)fsign[p];
+        const T v = prefac * (ldg(x + a) + f * ldg(x + c));
         stg(x + a, v);
         if (c != a) stg(x + c, f * v);
     }
@@ -858,12 +902,20 @@ __global__ __launch_bounds__(256) void sym_ab_batched_f64(const DevSymStep *__re
 
 }  // namespace gecco
 
+// Launchers: elem = 8 (float64 lists) or 4 (float32); bufs holds the
+// plan's buffer table of that element type.
+template <typename T> static T *const *tbl(void *const *b) { return reinterpret_cast<T *const *>(b); }
+
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
-                                    int ntasks, double *const *bufs, int grid,
-                                    const int32_t *skip, hipStream_t stream) {
+                                    int ntasks, void *const *bufs, int grid,
+                                    const int32_t *skip, int elem, hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::grouped_contract_f64, dim3(grid), dim3(gecco::NT), 0, stream,
-                       blocks, tasks, ntasks, bufs, skip);
+    if (elem == 4)
+        hipLaunchKernelGGL(gecco::grouped_contract<float>, dim3(grid), dim3(gecco::NT), 0, stream,
+                           blocks, tasks, ntasks, tbl<float>(bufs), skip);
+    else
+        hipLaunchKernelGGL(gecco::grouped_contract<double>, dim3(grid), dim3(gecco::NT), 0, stream,
+                           blocks, tasks, ntasks, tbl<double>(bufs), skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -878,63 +930,86 @@ extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
     return 0;
 }
 
-extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
-                                           const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int grid, int variant,
-                                           const int32_t *skip, hipStream_t stream) {
+template <typename T>
+static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *tasks, int ntasks,
+                         T *const *bufs, int grid, int variant, const int32_t *skip,
+                         hipStream_t stream) {
     using namespace gecco;
-    if (ntasks <= 0) return 0;
     switch (variant) {
-    case 0: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 32>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 1: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 2: hipLaunchKernelGGL((grouped_contract_affine_f64<64, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 3: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 64, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 4: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 16>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 5: hipLaunchKernelGGL((grouped_contract_affine_f64<128, 128, 8>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 0: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 1: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 2: hipLaunchKernelGGL((grouped_contract_affine<64, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     default: return 1;
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
+                                           const gecco::DevTask *tasks, int ntasks,
+                                           void *const *bufs, int grid, int variant,
+                                           const int32_t *skip, int elem, hipStream_t stream) {
+    if (ntasks <= 0) return 0;
+    if (elem == 4) return launch_affine(blocks, tasks, ntasks, tbl<float>(bufs), grid, variant, skip, stream);
+    return launch_affine(blocks, tasks, ntasks, tbl<double>(bufs), grid, variant, skip, stream);
 }
 
 
 extern "C" int gecco_direct_stage_cap(void) { return gecco::DIRECT_STAGE; }
 
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
-                                   int ntasks, double *const *bufs, double *scratch,
-                                   const int32_t *skip, hipStream_t stream) {
+                                   int ntasks, void *const *bufs, void *scratch,
+                                   const int32_t *skip, int elem, hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::gather_operands_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                       ntasks, bufs, scratch, skip);
+    if (elem == 4)
+        hipLaunchKernelGGL(gecco::gather_operands<float>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
+                           ntasks, tbl<float>(bufs), (float *)scratch, skip);
+    else
+        hipLaunchKernelGGL(gecco::gather_operands<double>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
+                           ntasks, tbl<double>(bufs), (double *)scratch, skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
-                                 int ntasks, double *const *bufs, const int32_t *skip,
+                                 int ntasks, void *const *bufs, const int32_t *skip, int elem,
                                  hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::init_lists_f64, dim3(ntasks), dim3(256), 0, stream, jobs, tasks, bufs,
-                       skip);
+    if (elem == 4)
+        hipLaunchKernelGGL(gecco::init_lists<float>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
+                           tbl<float>(bufs), skip);
+    else
+        hipLaunchKernelGGL(gecco::init_lists<double>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
+                           tbl<double>(bufs), skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
-extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
+extern "C" int gecco_launch_trace(const void *src, void *dst, const int64_t *rowptr,
                                   const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
-                                  int64_t nrows, double fact, hipStream_t stream) {
+                                  int64_t nrows, double fact, int elem, hipStream_t stream) {
     if (nrows <= 0) return 0;
     const int64_t grid = std::min<int64_t>((nrows + 255) / 256, 65536);
-    hipLaunchKernelGGL(gecco::trace_rows_f64, dim3((unsigned)grid), dim3(256), 0, stream, src, dst,
-                       rowptr, rowdst, col, sgn, nrows, fact);
+    if (elem == 4)
+        hipLaunchKernelGGL(gecco::trace_rows<float>, dim3((unsigned)grid), dim3(256), 0, stream,
+                           (const float *)src, (float *)dst, rowptr, rowdst, col, sgn, nrows, fact);
+    else
+        hipLaunchKernelGGL(gecco::trace_rows<double>, dim3((unsigned)grid), dim3(256), 0, stream,
+                           (const double *)src, (double *)dst, rowptr, rowdst, col, sgn, nrows, fact);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
-                                        double *const *bufs, const int32_t *i1, const int32_t *i2,
-                                        const int8_t *fsign, const int32_t *skip,
+                                        void *const *bufs, const int32_t *i1, const int32_t *i2,
+                                        const int8_t *fsign, const int32_t *skip, int elem,
                                         hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    hipLaunchKernelGGL(gecco::sym_ab_batched_f64, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
-                       bufs, i1, i2, fsign, skip);
+    if (elem == 4)
+        hipLaunchKernelGGL(gecco::sym_ab_batched<float>, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
+                           tbl<float>(bufs), i1, i2, fsign, skip);
+    else
+        hipLaunchKernelGGL(gecco::sym_ab_batched<double>, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
+                           tbl<double>(bufs), i1, i2, fsign, skip);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
-

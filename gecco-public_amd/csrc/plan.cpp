@@ -30,29 +30,30 @@
 #include "../../include/gecco_hip.h"
 #include "contract_types.h"
 
+// launchers (contract_kernels.hip); elem: 8 = float64 lists, 4 = float32
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
-                                    int ntasks, double *const *bufs, int grid,
-                                    const int32_t *skip, hipStream_t stream);
+                                    int ntasks, void *const *bufs, int grid,
+                                    const int32_t *skip, int elem, hipStream_t stream);
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
-                                           double *const *bufs, int grid, int variant,
-                                           const int32_t *skip, hipStream_t stream);
+                                           void *const *bufs, int grid, int variant,
+                                           const int32_t *skip, int elem, hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_direct_stage_cap(void);
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
-                                        double *const *bufs, const int32_t *i1, const int32_t *i2,
-                                        const int8_t *fsign, const int32_t *skip,
+                                        void *const *bufs, const int32_t *i1, const int32_t *i2,
+                                        const int8_t *fsign, const int32_t *skip, int elem,
                                         hipStream_t stream);
-extern "C" int gecco_launch_trace(const double *src, double *dst, const int64_t *rowptr,
+extern "C" int gecco_launch_trace(const void *src, void *dst, const int64_t *rowptr,
                                   const int64_t *rowdst, const int64_t *col, const int8_t *sgn,
-                                  int64_t nrows, double fact, hipStream_t stream);
+                                  int64_t nrows, double fact, int elem, hipStream_t stream);
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
-                                 int ntasks, double *const *bufs, const int32_t *skip,
+                                 int ntasks, void *const *bufs, const int32_t *skip, int elem,
                                  hipStream_t stream);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
-                                   int ntasks, double *const *bufs, double *scratch,
-                                   const int32_t *skip, hipStream_t stream);
+                                   int ntasks, void *const *bufs, void *scratch,
+                                   const int32_t *skip, int elem, hipStream_t stream);
 
 namespace gecco {
 
@@ -470,7 +471,7 @@ struct gecco_hip_plan {
     gecco::DevBlock *d_blocks = nullptr;
     gecco::DevTask *d_tasks = nullptr;
     int32_t *d_index = nullptr;
-    double **d_bufs = nullptr;
+    void **d_bufs = nullptr;
     // task classes, in this order in d_tasks: strided-GEMM launch (GEMM
     // tiles, 16 x 16 wave-split-K tiles and direct short-contraction tasks),
     // gather kernel
@@ -478,7 +479,7 @@ struct gecco_hip_plan {
     // operand materialisation (signed gather into plan-owned scratch)
     gecco::DevGather *d_gjobs = nullptr;
     gecco::DevGatherTask *d_gtasks = nullptr;
-    double *d_scratch = nullptr;
+    void *d_scratch = nullptr;
     int64_t ngjobs = 0, ngtasks = 0, scratch_elems = 0;
     int64_t materialize_min = 1;    // min(M, N) from which gathers are materialised
     bool chain = true;              // link strided-GEMM blocks with a common result tile
@@ -494,7 +495,7 @@ struct gecco_hip_plan {
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
-    std::vector<double *> last_bufs;
+    std::vector<void *> last_bufs;
     // optional HIP-event timing of the launches of execute (per launch class:
     // 0 strided-GEMM kernel, 1 gather kernel, 2 spin-flip symmetrisation,
     // 3 operand materialisation, 4 unused: the direct short-contraction tasks
@@ -511,6 +512,8 @@ struct gecco_hip_plan {
     int32_t cur_group = 0, max_group = 0;
     // keep the host index data after finalize (gecco_hip_plan_retune)
     bool keep_host = false;
+    // element size of the lists (8: float64, 4: float32)
+    int elem = 8;
     const int32_t *d_skip = nullptr;
     int64_t nskip = 0;
 };
@@ -692,7 +695,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
         nb_max = std::max<int64_t>(nb_max, std::max(b.buf1, std::max(b.buf2, b.buf12)) + 1);
         flops += 2.0 * (double)b.M * (double)b.N * (double)b.K;
-        payload += 8.0 * ((double)b.A.r.maxaddr + 1 + (double)b.B.r.maxaddr + 1 + 2.0 * (double)b.len12);
+        payload += (double)p->elem * ((double)b.A.r.maxaddr + 1 + (double)b.B.r.maxaddr + 1 + 2.0 * (double)b.len12);
         ranges.push_back({{(int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12,
                            (int64_t)b.buf12 * ((int64_t)1 << 40) + b.off12 + b.len12}, i});
     }
@@ -715,11 +718,11 @@ static int plan_finalize(gecco_hip_plan *p) {
         auto strided = [&](const HostOperand &o, int64_t X) {
             int64_t f0, f1, s0, s1;
             if (o.r.table || !affine_of(o.r.first, f0, f1) || !affine_of(o.r.second, s0, s1)) return false;
-            return f1 >= 0 && s1 >= 0 && (f1 * (b.K - 1) + s1 * (X - 1) + 1) * 8 <= max_extent;
+            return f1 >= 0 && s1 >= 0 && (f1 * (b.K - 1) + s1 * (X - 1) + 1) * p->elem <= max_extent;
         };
         const bool sa = strided(b.A, b.M), sb = strided(b.B, b.N);
         const bool big = p->materialize_min > 0 && std::min(b.M, b.N) >= p->materialize_min;
-        const bool fa = b.K * b.M * 8 <= max_extent, fb = b.K * b.N * 8 <= max_extent;
+        const bool fa = b.K * b.M * p->elem <= max_extent, fb = b.K * b.N * p->elem <= max_extent;
         const bool ca = sa || (big && fa), cb = sb || (big && fb);
         b.strided = ca && cb;
         b.mat_a = b.strided && !sa;
@@ -1082,7 +1085,7 @@ static int plan_finalize(gecco_hip_plan *p) {
                 for (int64_t e0 = 0; e0 < n; e0 += chunk)
                     gt.push_back({(int32_t)j, (int32_t)e0, (int32_t)std::min(n, e0 + chunk),
                                   pass == 0 ? -1 : job_grp[j]});
-                if (pass == 1) gbytes += 16.0 * (double)n;
+                if (pass == 1) gbytes += 2.0 * p->elem * (double)n;
                 if (pass == 0) p->ngtasks_static = (int64_t)gt.size();
             }
         p->ngjobs = (int64_t)jobs.size();
@@ -1090,7 +1093,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         p->scratch_elems = next;
         p->gather_bytes = gbytes;
         if (!jobs.empty()) {
-            if (check(hipMalloc(&p->d_scratch, (size_t)next * sizeof(double)), "hipMalloc(scratch)") ||
+            if (check(hipMalloc(&p->d_scratch, (size_t)next * p->elem), "hipMalloc(scratch)") ||
                 check(hipMalloc(&p->d_gjobs, jobs.size() * sizeof(DevGather)), "hipMalloc(gather jobs)") ||
                 check(hipMemcpy(p->d_gjobs, jobs.data(), jobs.size() * sizeof(DevGather), hipMemcpyHostToDevice), "hipMemcpy(gather jobs)") ||
                 check(hipMalloc(&p->d_gtasks, gt.size() * sizeof(DevGatherTask)), "hipMalloc(gather tasks)") ||
@@ -1103,7 +1106,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     if (check(hipMalloc(&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask)), "hipMalloc(tasks)")) return g_err;
     if (!tasks.empty() && check(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice), "hipMemcpy(tasks)")) return g_err;
     p->nbufs_max = nb_max;
-    if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(double *)), "hipMalloc(bufs)")) return g_err;
+    if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(void *)), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
     for (auto &tr : p->traces) {
         if (!tr.nrows) continue;
@@ -1173,7 +1176,7 @@ static int plan_finalize(gecco_hip_plan *p) {
                 std::vector<int64_t>().swap(y.i2);
                 std::vector<int8_t>().swap(y.f);
             }
-            p->payload += 16.0 * (double)y.npair;
+            p->payload += 2.0 * p->elem * (double)y.npair;
             p->index_bytes += 9.0 * (double)y.npair;
         }
         p->nsymtasks = (int64_t)stasks.size();
@@ -1215,12 +1218,12 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (p->d_skip && !p->traces.empty())
         return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace steps");
     if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty()) return ok();
-    std::vector<double *> want(bufs, bufs + p->nbufs_max);
+    std::vector<void *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
     if (want != p->last_bufs) {
         // slot table changes only when the caller passes different buffers;
         // the copy is ordered on the same stream before the launch
-        if (check(hipMemcpyAsync(p->d_bufs, want.data(), want.size() * sizeof(double *), hipMemcpyHostToDevice, s), "hipMemcpyAsync(bufs)")) return g_err;
+        if (check(hipMemcpyAsync(p->d_bufs, want.data(), want.size() * sizeof(void *), hipMemcpyHostToDevice, s), "hipMemcpyAsync(bufs)")) return g_err;
         if (check(hipStreamSynchronize(s), "hipStreamSynchronize")) return g_err;
         p->last_bufs = want;
         p->static_done = false;   // constant slots may hold other data now
@@ -1236,13 +1239,13 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         return 0;
     };
     if (p->nitasks &&
-        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, p->d_skip, s))
+        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, p->d_skip, p->elem, s))
         return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
     const int64_t g0 = p->static_done ? p->ngtasks_static : 0;
     if (p->ngtasks > g0) {
         if (mark(3, true)) return g_err;
         if (gecco_launch_gather(p->d_gjobs, p->d_gtasks + g0, (int)(p->ngtasks - g0), p->d_bufs,
-                                p->d_scratch, p->d_skip, s))
+                                p->d_scratch, p->d_skip, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "gather launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(3, false)) return g_err;
     }
@@ -1250,25 +1253,27 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
-                                        p->affine_variant, p->d_skip, s))
+                                        p->affine_variant, p->d_skip, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
     if (ng) {
         if (mark(1, true)) return g_err;
-        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, p->d_skip, s))
+        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, p->d_skip,
+                                 p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
     for (auto &tr : p->traces)
-        if (tr.nrows && gecco_launch_trace(bufs[tr.buf_src] + tr.off_src, bufs[tr.buf_dst] + tr.off_dst,
+        if (tr.nrows && gecco_launch_trace((const char *)bufs[tr.buf_src] + tr.off_src * p->elem,
+                                           (char *)bufs[tr.buf_dst] + tr.off_dst * p->elem,
                                            tr.d_rowptr, tr.d_rowdst, tr.d_col, tr.d_sgn, tr.nrows,
-                                           tr.fact, s))
+                                           tr.fact, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "trace launch failed");
     if (p->nsymtasks) {
         if (mark(2, true)) return g_err;
         if (gecco_launch_sym_batched(p->d_symsteps, p->d_symtasks, (int)p->nsymtasks, p->d_bufs,
-                                     p->d_sym_i1, p->d_sym_i2, p->d_sym_f, p->d_skip, s))
+                                     p->d_sym_i1, p->d_sym_i2, p->d_sym_f, p->d_skip, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
         if (mark(2, false)) return g_err;
     }
@@ -1632,6 +1637,15 @@ int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t tar
     if (target_tasks > 0) plan->target_tasks = target_tasks;
     if (min_kchunk > 0) plan->min_kchunk = std::max<int64_t>(32, min_kchunk);
     if (max_kchunk > 0) plan->max_kchunk = std::max<int64_t>(64, max_kchunk);
+    return 0;
+}
+
+int gecco_hip_plan_set_precision(gecco_hip_plan *plan, int64_t elem_bytes) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (elem_bytes != 4 && elem_bytes != 8)
+        return fail(GECCO_HIP_EINVAL, "precision: 8 (float64) or 4 (float32) bytes per element");
+    plan->elem = (int)elem_bytes;
     return 0;
 }
 

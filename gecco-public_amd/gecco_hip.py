@@ -71,7 +71,7 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
              'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
              'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning',
-             'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune')
+             'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune', 'gecco_hip_plan_set_precision')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -84,6 +84,7 @@ _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_set_group.argtypes = [ctypes.c_void_p, ctypes.c_int32]
 _lib.gecco_hip_plan_set_tuning.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
 _lib.gecco_hip_plan_keep_host.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_set_precision.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_retune.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
 _lib.gecco_hip_plan_set_skip.argtypes = [ctypes.c_void_p, ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
@@ -201,6 +202,10 @@ class Plan:
         _check(_lib.gecco_hip_plan_finalize(self._h))
 
     def execute(self, bufs, stream=None):
+        elem = getattr(self, '_elem', 8)
+        for b in bufs:
+            if hasattr(b, 'element_size') and b.element_size() != elem:
+                raise GeccoHipError(f'plan of {8 * elem}-bit lists given a {b.dtype} buffer')
         arr = (ctypes.c_void_p * len(bufs))(*[_ptr(b).value for b in bufs])
         s = ctypes.c_void_p(0 if stream is None else int(stream))
         _check(_lib.gecco_hip_plan_execute(self._h, arr, len(bufs), s))
@@ -273,6 +278,12 @@ class Plan:
         v = lambda x: -1 if x is None else int(x)
         _check(_lib.gecco_hip_plan_set_tuning(self._h, v(variant), v(target_tasks), v(min_kchunk),
                                               v(max_kchunk)))
+
+    def set_precision(self, precision):
+        """'f64' (default) or 'f32' lists; before finalize."""
+        elem = {'f64': 8, 'f32': 4}[precision]
+        _check(_lib.gecco_hip_plan_set_precision(self._h, elem))
+        self._elem = elem
 
     def keep_host(self, on=True):
         """Before finalize: keep the host index data (for retune); after

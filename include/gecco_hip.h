@@ -225,6 +225,12 @@ int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t tar
  * finalize releases that data.  retune frees the plan's device data and
  * finalizes anew (the next execute uploads the buffer table again). */
 int gecco_hip_plan_keep_host(gecco_hip_plan *plan, int64_t on);
+/* Element type of every list the plan touches: 8 = float64 (default), 4 =
+ * float32 (before finalize).  A float32 plan reads and writes float lists
+ * (the pointers passed to execute then point at float data) and runs its
+ * MFMAs as f32 16x16x4 with float32 accumulation; factors stay float64
+ * arguments.  No counterpart in the reference, which is float64 only. */
+int gecco_hip_plan_set_precision(gecco_hip_plan *plan, int64_t elem_bytes);
 int gecco_hip_plan_retune(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
                           int64_t min_kchunk, int64_t max_kchunk);
 int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags);

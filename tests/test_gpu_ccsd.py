@@ -193,3 +193,30 @@ def test_autotuned_plans_give_the_same_solve():
     c.build_plans(c.tuning)
     res2 = c.solve(tol=1e-10, maxit=80)
     assert abs(res2['energy'] - ref['energy']) < 1e-11
+
+
+def test_mixed_precision_solve_reaches_the_f64_energy():
+    """Float32 residuals until ||g|| < 1e-5, then float64 refinement: the
+    same converged energy as the float64 solve (iterative refinement of the
+    residual)."""
+    _torch()
+    import ccsd
+    h = synthetic(5, 15, 45, nsym=2, seed=8, sigma=0.1)
+    c = ccsd.CCSD(h)
+    ref = c.solve(tol=1e-10, maxit=100)
+    for mode in ('graph', 'sync'):
+        res = c.solve_mixed(tol=1e-10, switch=1e-5, maxit=100, mode=mode)
+        print(mode, 'f64', ref['iterations'], 'mixed', res['f32_iterations'], '+', res['f64_iterations'],
+              flush=True)
+        assert res['residual'] < 1e-10
+        assert abs(res['energy'] - ref['energy']) < 1e-10
+        assert res['f32_iterations'] > 0 and res['f64_iterations'] < ref['iterations']
+    # below its floor the float32 residual stalls: the guard ends the phase
+    st = c.solve(tol=1e-13, maxit=150, precision='f32', stall=5)
+    assert st['stalled'] and st['iterations'] < 150 and st['residual'] > 1e-13
+    # a float32 evaluation differs from the float64 one at float32 rounding
+    c.amp.normal_(0, 0.01)
+    c.evaluate()
+    r64 = c.res.clone()
+    c.evaluate32()
+    assert float((c.res - r64).abs().max()) <= 1e-5 * float(r64.abs().max())

@@ -233,6 +233,43 @@ def test_triples_match_oracle(nsym, nocc, nvir, name, use_tr):
     assert _relerr(dr.cpu().numpy(), ref) <= TOL
 
 
+@pytest.mark.parametrize('nsym,nocc,nvir', [(2, 4, 6), (4, 6, 10)])
+@pytest.mark.parametrize('name', list(CASES) + list(T3_CASES))
+def test_f32_plan_matches_oracle(nsym, nocc, nvir, name):
+    """A float32 plan (float32 lists, f32 MFMA) against the oracle's float64
+    contraction of the same (float32-representable) inputs.  Tolerance:
+    float32 rounding of the products and sums, max |err| <= 2e-5 max |ref|."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    print(f'f32 {name} nsym={nsym} nocc={nocc} nvir={nvir}', flush=True)
+    cases = {**CASES, **T3_CASES}
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = cases[name](orbs)
+    x1 = spin_symmetric(a, 81).astype(np.float32).astype(np.float64)
+    x2 = spin_symmetric(b, 82).astype(np.float32).astype(np.float64)
+    x0 = spin_symmetric(r, 83).astype(np.float32).astype(np.float64)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=True, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    A, B, R = (gecco_hip.MEList(o, m.occ_c, m.occ_a) for m in (a, b, r))
+    plan = gecco_hip.Plan()
+    plan.set_precision('f32')
+    plan.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, True)
+    plan.add_sym_ab(R, 2)
+    plan.finalize()
+    dev = torch.device('cuda:0')
+    f = lambda v: torch.from_numpy(v.astype(np.float32)).to(dev)
+    dr = f(x0)
+    with pytest.raises(gecco_hip.GeccoHipError):
+        plan.execute([torch.from_numpy(x1).to(dev), f(x2), dr])      # float64 buffer
+    plan.execute([f(x1), f(x2), dr])
+    torch.cuda.synchronize()
+    err = _relerr(dr.cpu().numpy().astype(np.float64), ref)
+    print(f'rel err {err:.2e}')
+    assert err <= 2e-5
+
+
 def test_all_triples_terms_one_launch():
     """The five triples terms into one R3 list in one grouped launch (the
     triples bench's plan) against the sum of the oracle's terms."""
