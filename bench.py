@@ -62,7 +62,7 @@ METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
 # profiles/); for the CCSD iteration it is the average over the three
 # formula-level launches of the strided-GEMM kernel
 MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 565.6e6,
-                    ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine_f64'): 925.3e6}
+                    ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine<64, 64, 16, double>'): 924.2e6}
 GOLDEN = os.path.join(ROOT, 'tests', 'golden', 'ccsd_o20_v100_s4.json')
 
 
@@ -158,7 +158,7 @@ def run_ladder(args):
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
                      'traffic': MEASURED_TRAFFIC.get(('ladder', args.nocc, args.nvir, args.nsym)),
-                     'kernel': 'grouped_contract_affine_f64', 'kernel_ms': round(kern_ms, 5)},
+                     'kernel': 'grouped_contract_affine<64, 64, 16, double>', 'kernel_ms': round(kern_ms, 5)},
     }
     if not args.no_cpu:
         v, dt, sample = cpu_ladder(args.nocc, args.nvir, args.nsym)
@@ -201,7 +201,7 @@ def kernel_times(plans, run, steps):
     _, dom, dom_fl = max(cand)
     dom_ms = ms[dom] / max(1, nl[dom])
     achieved = dom_fl / (ms[dom] / steps * 1e-3) / 1e12 if ms[dom] > 0 else 0.0
-    return dict(ms=ms, launches=nl, names=['grouped_contract_affine_f64', 'grouped_contract_f64'],
+    return dict(ms=ms, launches=nl, names=['grouped_contract_affine<64, 64, 16, double>', 'grouped_contract<double>'],
                 dom=dom, dom_ms=dom_ms, achieved=achieved, gather_bytes=gbytes)
 
 
@@ -244,6 +244,8 @@ def run_ccsd(args):
     solver = ccsd.CCSD(ham)
     setup_s = time.perf_counter() - t0
     torch.cuda.synchronize()
+    step = lambda m: print(f'# {m}', file=sys.stderr, flush=True)
+    step(f'setup {setup_s:.1f} s')
     tune_report, tune_s = None, 0.0
     if args.tune:
         t0 = time.perf_counter()
@@ -259,6 +261,7 @@ def run_ccsd(args):
     solver.solve(tol=0.0, maxit=args.steps, restart=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    step(f'timed steps {wall:.3f} s')
     # kernel durations (roofline): as many further iterations launched one by
     # one (the same kernels as the replayed graph), timed with HIP events on
     # the launch stream
@@ -269,12 +272,14 @@ def run_ccsd(args):
     # time to convergence from T = 0
     torch.cuda.synchronize()
     res = solver.solve(tol=args.tol, maxit=200)
+    step(f'solve {res["iterations"]} iterations')
     # mixed precision: float32 lists until ||g|| < 1e-6, then float64
     # refinement; and the float32 iteration rate (graph replays)
     t0 = time.perf_counter()
     solver.build_f32()
     f32_setup = time.perf_counter() - t0
     mixed = solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200)
+    step(f'mixed solve {mixed["f32_iterations"]} + {mixed["f64_iterations"]} iterations')
     solver.solve(tol=0.0, maxit=args.warmup, precision='f32')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -299,10 +304,10 @@ def run_ccsd(args):
                    'kernel_calls_per_iter': solver.nblocks,
                    'gflop_per_iter': round(solver.flops / 1e9, 3)},
         'contraction_tflops': round(solver.flops * args.steps / wall / 1e12, 3),
-        'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
-                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
-                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
-                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double>': round(ms[0] / args.steps, 4),
+                                'grouped_contract<double>': round(ms[1] / args.steps, 4),
+                                'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
+                                'gather_operands<double>': round(ms[3] / args.steps, 4)},
         'gather_operands_gbps': round(gbytes / (ms[3] / args.steps * 1e-3) / 1e9, 1) if ms[3] > 0 else None,
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
@@ -444,10 +449,10 @@ def run_triples(args):
                              'V(vv,vo).T2', 'V(ov,oo).T2'],
                    'kernel_calls': solver.nblocks, 'gflop_per_step': round(solver.flops / 1e9, 3),
                    't3_elements': solver.L['T3'].length},
-        'kernels_ms_per_step': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
-                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
-                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
-                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'kernels_ms_per_step': {'grouped_contract_affine<64, 64, 16, double>': round(ms[0] / args.steps, 4),
+                                'grouped_contract<double>': round(ms[1] / args.steps, 4),
+                                'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
+                                'gather_operands<double>': round(ms[3] / args.steps, 4)},
         'roofline': {'bound': 'mfma', 'achieved': round(kt['achieved'], 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(kt['achieved'] / PEAK_FP64, 4),
                      'traffic': MEASURED_TRAFFIC.get(('triples', args.nocc, args.nvir, args.nsym,
@@ -630,10 +635,10 @@ def run_batched(args):
                    'gflop_per_iter': round(b.flops / 1e9, 3)},
         'instance_iters_per_s': round(args.instances * args.steps / wall, 2),
         'contraction_tflops': round(b.flops * args.steps / wall / 1e12, 3),
-        'kernels_ms_per_iter': {'grouped_contract_affine_f64': round(ms[0] / args.steps, 4),
-                                'grouped_contract_f64': round(ms[1] / args.steps, 4),
-                                'sym_ab_batched_f64': round(ms[2] / args.steps, 4),
-                                'gather_operands_f64': round(ms[3] / args.steps, 4)},
+        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double>': round(ms[0] / args.steps, 4),
+                                'grouped_contract<double>': round(ms[1] / args.steps, 4),
+                                'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
+                                'gather_operands<double>': round(ms[3] / args.steps, 4)},
         'roofline': {'bound': 'mfma', 'achieved': round(kt['achieved'], 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(kt['achieved'] / PEAK_FP64, 4),
                      'traffic': MEASURED_TRAFFIC.get(('batched', args.instances, args.nsym, names[dom])),
