@@ -596,6 +596,20 @@ def run_batched(args):
     res = b.solve(tol=args.tol, maxit=200, timing=True)
     print(f'# batched solve: {res["batch_iterations"]} batched iterations, {res["seconds"]:.3f} s',
           file=sys.stderr, flush=True)
+    # mixed precision: float32 lists to 1e-5, then float64 refinement; and
+    # the float32 batched iteration rate
+    t0 = time.perf_counter()
+    b.build_f32()
+    print(f'# float32 lists and plans {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
+    mixed = b.solve_mixed(tol=args.tol, switch=1e-5, maxit=200)
+    print(f'# mixed solve {mixed["f32_batch_iterations"]} + {mixed["f64_batch_iterations"]} batched '
+          f'iterations, {mixed["seconds"]:.3f} s', file=sys.stderr, flush=True)
+    b.solve(tol=0.0, maxit=args.warmup, precision='f32')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
+    torch.cuda.synchronize()
+    wall32 = time.perf_counter() - t0
     # the same instances one after another (per-instance plans and graphs)
     seq_s, seq_iters, seq_dE = 0.0, 0, 0.0
     seq_ms = []             # per-iteration milliseconds of each instance alone
@@ -650,6 +664,13 @@ def run_batched(args):
                         'speedup_vs_sequential': round(seq_s / res['seconds'], 3),
                         'batched_gpu_ms': round(sum(res['iteration_ms']), 2),
                         'max_abs_dE_vs_sequential': seq_dE},
+        'mixed_precision': {'switch': 1e-5, 'f32_batch_iterations': mixed['f32_batch_iterations'],
+                            'f64_batch_iterations': mixed['f64_batch_iterations'],
+                            'seconds': round(mixed['seconds'], 4), 'all_converged': all(mixed['converged']),
+                            'max_abs_dE_vs_f64': max(abs(x - y) for x, y in
+                                                     zip(mixed['energies'], res['energies'])),
+                            'f32_batched_iters_per_s': round(args.steps / wall32, 3),
+                            'f32_contraction_tflops': round(b.flops * args.steps / wall32 / 1e12, 3)},
         'setup_seconds': round(setup_s, 2),
     }
     if not args.no_cpu:

@@ -56,23 +56,97 @@ class BatchedCCSD:
         self._graph = None
         self._diis = None
 
-    def evaluate(self, stream=None):
+    def _record(self, precision):
+        import gecco_hip
+        nl = self.nlists
+        plans = []
+        for li in range(len(self.inst[0].schedule)):
+            p = gecco_hip.Plan()
+            p.set_precision(precision)
+            for k, c in enumerate(self.inst):
+                p.set_group(k)
+                c.record_level(p, c.schedule[li], k * nl)
+            p.finalize()
+            if self.skip_converged:
+                p.set_skip(self.done)
+            plans.append(p)
+        return plans
+
+    def build_f32(self):
+        """Float32 lists of every instance and the combined float32 plans
+        (same instance groups and skip flags)."""
+        saved = [(c.flops, c.nblocks) for c in self.inst]
+        for c in self.inst:
+            c.build_f32(plans=False)
+        self.bufs32 = [b for c in self.inst for b in c.bufs32]
+        self.plans32 = self._record('f32')
+        for c, (f, nb) in zip(self.inst, saved):
+            c.flops, c.nblocks = f, nb
+
+    def evaluate(self, stream=None, precision='f64'):
+        if precision == 'f32':
+            if getattr(self, 'plans32', None) is None:
+                self.build_f32()
+            for c in self.inst:
+                c.amp32.copy_(c.amp)
+            for p in self.plans32:
+                p.execute(self.bufs32, stream)
+            for c in self.inst:
+                c.res.copy_(c.res32)
+                c.buf['E'].copy_(c.buf32['E'])
+            return
         for p in self.plans:
             p.execute(self.bufs, stream)
 
-    def solve(self, tol=1e-8, maxit=100, ndiis=8, mode='graph', restart=True, timing=False):
+    def tune(self, reps=3, log=None):
+        """Launch settings of every combined level plan by timing
+        (autotune.tune_inplace on a re-recorded plan that keeps its host
+        data); returns the per-level choices."""
+        import gecco_hip
+        import autotune
+        nl = self.nlists
+        report = []
+        for li in range(len(self.plans)):
+            p = gecco_hip.Plan()
+            p.keep_host(True)
+            saved = [(c.flops, c.nblocks) for c in self.inst]
+            for k, c in enumerate(self.inst):
+                p.set_group(k)
+                c.record_level(p, c.schedule[li], k * nl)
+            for c, (f, nb) in zip(self.inst, saved):
+                c.flops, c.nblocks = f, nb
+            p.finalize()
+            if self.skip_converged:
+                p.set_skip(self.done)
+            for q in self.plans[:li]:
+                q.execute(self.bufs)
+            opts, ms, table = autotune.tune_inplace(p, self.bufs, reps=reps, log=log)
+            p.keep_host(False)
+            self.plans[li] = p
+            report.append({'level': li, 'chosen': opts, 'ms': round(ms, 3),
+                           'default_ms': round(table[0][1], 3), 'candidates': len(table)})
+        self._graph = None
+        return report
+
+    def solve(self, tol=1e-8, maxit=100, ndiis=8, mode='graph', restart=True, timing=False,
+              precision='f64', keep_amplitudes=False):
         """Iterate every instance to ||g|| < tol (at most maxit iterations);
         mode 'graph' replays the captured batched iteration, 'async' launches
         it eagerly.  Returns per-instance energies, iterations and residuals,
         and the batch's wall time (timing=True: also the GPU milliseconds of
-        every batched iteration, 'iteration_ms')."""
+        every batched iteration, 'iteration_ms').  precision 'f32': float32
+        lists (evaluate(precision='f32')), float64 amplitudes and DIIS;
+        keep_amplitudes: restart the subspaces from the current amplitudes."""
         import gecco_hip
         torch = self.torch
         if mode not in ('async', 'graph'):
             raise ValueError(f'unknown solve mode {mode!r}')
+        if precision == 'f32' and getattr(self, 'plans32', None) is None:
+            self.build_f32()
         if restart or self._diis is None:
             for c in self.inst:
-                c.amp.zero_()
+                if not keep_amplitudes:
+                    c.amp.zero_()
             self._diis = [gecco_hip.DIIS(c.amp.numel(), ndiis) for c in self.inst]
             self.done.zero_()
             for k, d in enumerate(self._diis):
@@ -82,7 +156,7 @@ class BatchedCCSD:
         stream = torch.cuda.current_stream().cuda_stream
 
         def iteration(s):
-            self.evaluate(s)
+            self.evaluate(s, precision)
             for c, d in zip(self.inst, self._diis):
                 d.step_async(c.amp, c.res, c.diag, energy=c.buf['E'], tol=tol, stream=s)
 
@@ -98,7 +172,7 @@ class BatchedCCSD:
         ev0 = torch.cuda.Event(enable_timing=timing)
         ev0.record()
         graph = self._graph[1] if (mode == 'graph' and self._graph is not None and
-                                   self._graph[0] == tol) else None
+                                   self._graph[0] == (tol, precision)) else None
         it = 0
         for it in range(1, maxit + 1):
             if graph is not None:
@@ -110,7 +184,7 @@ class BatchedCCSD:
                     graph = torch.cuda.CUDAGraph()
                     with torch.cuda.graph(graph):
                         iteration(torch.cuda.current_stream().cuda_stream)
-                    self._graph = (tol, graph)
+                    self._graph = ((tol, precision), graph)
             ev = torch.cuda.Event(enable_timing=timing)
             ev.record()
             events.append(ev)
@@ -138,4 +212,19 @@ class BatchedCCSD:
                    batch_iterations=done, flops_per_iter=self.flops, kernel_calls=self.nblocks)
         if timing:
             out['iteration_ms'] = [a.elapsed_time(b) for a, b in zip([ev0] + events[:-1], events)]
+        return out
+
+    def solve_mixed(self, tol=1e-8, switch=1e-5, maxit=100, maxit_f32=60, ndiis=8, mode='graph'):
+        """Float32 lists until every instance has ||g|| < switch (at most
+        maxit_f32 batched iterations; an instance that meets switch is
+        skipped meanwhile), then float64 refinement from those amplitudes
+        with fresh subspaces to ||g|| < tol."""
+        r32 = self.solve(tol=switch, maxit=maxit_f32, ndiis=ndiis, mode=mode, precision='f32')
+        r64 = self.solve(tol=tol, maxit=maxit, ndiis=ndiis, mode=mode, precision='f64',
+                         keep_amplitudes=True)
+        out = dict(r64)
+        out.update(f32_batch_iterations=r32['batch_iterations'], f32_seconds=r32['seconds'],
+                   f64_batch_iterations=r64['batch_iterations'], f64_seconds=r64['seconds'],
+                   seconds=r32['seconds'] + r64['seconds'], switch=switch,
+                   f32_converged=r32['converged'])
         return out

@@ -332,9 +332,10 @@ class CCSD:
             p.execute(self.bufs, stream)
 
     # ------------------------------------------------------ float32 lists
-    def build_f32(self):
+    def build_f32(self, plans=True):
         """Float32 copies of every list (integrals rounded once) and plans
-        that run the formula on them: the bulk of a mixed-precision solve."""
+        that run the formula on them: the bulk of a mixed-precision solve
+        (plans=False: the lists only, for a batched driver)."""
         import gecco_hip
         torch = self.torch
         lists = self.formula['lists']
@@ -348,6 +349,8 @@ class CCSD:
             else:
                 self.buf32[n] = self.buf[n].to(torch.float32)
         self.bufs32 = [self.buf32[n] for n in lists]
+        if not plans:
+            return
         flops, nblocks = self.flops, self.nblocks
         self.plans32 = []
         for li, level in enumerate(self.schedule):

@@ -168,6 +168,15 @@ def test_batched_instances_match_separate_solves():
     res2 = b.solve(tol=1e-10, maxit=80)
     assert max(abs(x - y) for x, y in zip(res2['energies'], res['energies'])) < 1e-12
     assert res2['iterations'] == res['iterations']
+    # mixed precision: float32 lists to 1e-5, float64 refinement
+    rm = b.solve_mixed(tol=1e-10, switch=1e-5, maxit=80)
+    assert all(rm['converged'])
+    assert max(abs(x - r['energy']) for x, r in zip(rm['energies'], sep)) < 1e-10
+    # autotuned combined plans: the same energies
+    report = b.tune(reps=2)
+    assert len(report) == len(b.plans)
+    rt = b.solve(tol=1e-10, maxit=80)
+    assert max(abs(x - y) for x, y in zip(rt['energies'], res['energies'])) < 1e-11
     # without skipping converged instances: the same energies and counts
     b2 = BatchedCCSD(hams, skip_converged=False)
     res3 = b2.solve(tol=1e-10, maxit=80)
