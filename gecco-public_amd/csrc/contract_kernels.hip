@@ -323,12 +323,14 @@ struct AffView {
     uint32_t step;    // bytes per K step
     int kk0, kstep;   // k of element r relative to the step start: kk0 + kstep*r
     int xi0, xstep;   // m/n (relative to the tile) of element r: xi0 + xstep*r
+    bool ff;          // the operand's fast_first: which of the two layouts above
 
     // operand block at blk (element (k, x) at o.f0 + o.s0 + o.f1*k + o.s1*x,
     // k < K, x < Xn); the first step starts at k = kstart
     __device__ __forceinline__ void init(const T *blk, const DevOperand &o, int K, int x0,
                                          int Xn, int kstart, int tid) {
-        if (o.fast_first) {
+        ff = o.fast_first != 0;
+        if (ff) {
             kk0 = tid % TBK; kstep = 0;
             xi0 = tid / TBK; xstep = NT / TBK;
         } else {
@@ -361,6 +363,15 @@ struct AffView {
                 off[r] += step;
             }
         }
+    }
+    // store of data loaded under layout fl (a step loaded before the view
+    // moved on to the next block of a chain, whose layout may differ)
+    template <int LD>
+    __device__ __forceinline__ static void store_as(T *lds, const T in[L], bool fl, int tid) {
+        const int k0 = fl ? tid % TBK : tid / X, ks = fl ? 0 : NT / X;
+        const int x0 = fl ? tid / TBK : tid % X, xs = fl ? NT / TBK : 0;
+#pragma unroll
+        for (int r = 0; r < L; ++r) lds[(k0 + ks * r) * LD + x0 + xs * r] = in[r];
     }
     template <int LD>
     __device__ __forceinline__ void store(T *lds, const T in[L]) const {
@@ -603,7 +614,9 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 
 // 128 x 128 tiles hold 128 accumulator registers per lane: the launch bound
 // asks for 2 waves per SIMD (at most 256 registers per lane in total).
-template <int TBM, int TBN, int TBK, typename T>
+// PF: K steps whose operand loads are in flight while a step's MFMAs run
+// (1: the next step; 2: the next two, in two register sets).
+template <int TBM, int TBN, int TBK, typename T, int PF = 1>
 __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
     T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
@@ -661,6 +674,84 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         // the masked loads put into LDS
         const bool wave_live = __builtin_amdgcn_readfirstlane(
             (m0 + wm * (TBM / 2) < M && n0 + wn * (TBN / 2) < N) ? 1 : 0);
+        if constexpr (PF == 2) {
+            // fetch(): loads the step at the current position and advances
+            // the position (along the chain); false once the range is done
+            bool have = true;
+            // fa / fb: the layouts the step was loaded under
+            auto fetch = [&](T *xa, T *xb, bool &fa, bool &fb) -> bool {
+                if (!have) return false;
+                fa = va.ff;
+                fb = vb.ff;
+                va.load(kl, kh, xa);
+                vb.load(kl, kh, xb);
+                kl += TBK;
+                if (kl >= kh) {
+                    kb += kcur;
+                    blk = blocks[blk].next;
+                    have = blk >= 0 && kb < task.k1;
+                    if (have) {
+                        const DevBlock &nb = blocks[blk];
+                        kcur = nb.K;
+                        kl = 0;
+                        kh = min(kcur, task.k1 - kb);
+                        va.init(bufs[nb.buf_a] + nb.base_a, nb.A, kcur, m0, M, 0, tid);
+                        vb.init(bufs[nb.buf_b] + nb.base_b, nb.B, kcur, n0, N, 0, tid);
+                    }
+                }
+                return true;
+            };
+            auto mma = [&](int c) {
+                const T *A = lds + c * SA;
+                const T *B = lds + 2 * SA + c * SB;
+                if (!wave_live) return;
+#pragma unroll
+                for (int kq = 0; kq < TBK / 4; ++kq) {
+                    const int kr = kq * 4 + lrow;
+                    T a[TM], bb[TN];
+#pragma unroll
+                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
+#pragma unroll
+                    for (int i = 0; i < TM; ++i)
+#pragma unroll
+                        for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(a[i], bb[j], acc[i][j]);
+                }
+            };
+            typedef AffView<TBM, TBK, T> VA;
+            typedef AffView<TBN, TBK, T> VB;
+            T xa[VA::L], xb[VB::L], ya[VA::L], yb[VB::L];
+            bool fxa, fxb, fya, fyb;
+            __syncthreads();   // LDS of the previous task is no longer read
+            fetch(xa, xb, fxa, fxb);     // the task's range is never empty
+            VA::template store_as<LDA>(lds, xa, fxa, tid);
+            VB::template store_as<LDB>(lds + 2 * SA, xb, fxb, tid);
+            __syncthreads();
+            bool okx = fetch(xa, xb, fxa, fxb);
+            bool oky = fetch(ya, yb, fya, fyb);
+            int cur = 0;
+            for (;;) {
+                mma(cur);
+                if (okx) {
+                    VA::template store_as<LDA>(lds + (cur ^ 1) * SA, xa, fxa, tid);
+                    VB::template store_as<LDB>(lds + 2 * SA + (cur ^ 1) * SB, xb, fxb, tid);
+                }
+                __syncthreads();
+                cur ^= 1;
+                if (!okx) break;
+                okx = fetch(xa, xb, fxa, fxb);
+                mma(cur);
+                if (oky) {
+                    VA::template store_as<LDA>(lds + (cur ^ 1) * SA, ya, fya, tid);
+                    VB::template store_as<LDB>(lds + 2 * SA + (cur ^ 1) * SB, yb, fyb, tid);
+                }
+                __syncthreads();
+                cur ^= 1;
+                if (!oky) break;
+                oky = fetch(ya, yb, fya, fyb);
+            }
+        } else {
         T ra[AffView<TBM, TBK, T>::L], rb[AffView<TBN, TBK, T>::L];
         __syncthreads();   // LDS of the previous task is no longer read
         va.load(kl, kh, ra);
@@ -715,6 +806,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
             __syncthreads();
             cur ^= 1;
             if (!more) break;
+        }
         }
 
         const DevOperand &oc = b.C;
@@ -922,7 +1014,8 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
 // Tile variants of the affine kernel: {TBM, TBN, TBK}.
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
     static const int shapes[][3] = {{64, 64, 32}, {64, 64, 16}, {64, 128, 16},
-                                    {128, 64, 16}, {128, 128, 16}, {128, 128, 8}};
+                                    {128, 64, 16}, {128, 128, 16}, {128, 128, 8},
+                                    {64, 64, 16}, {64, 64, 32}};
     if (v < 0 || v >= (int)(sizeof shapes / sizeof shapes[0])) return 1;
     *bm = shapes[v][0];
     *bn = shapes[v][1];
@@ -942,6 +1035,9 @@ static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *ta
     case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    // two K steps of loads in flight
+    case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     default: return 1;
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;

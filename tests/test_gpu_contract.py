@@ -304,7 +304,7 @@ def test_all_triples_terms_one_launch():
     assert _relerr(bufs[0].cpu().numpy(), ref) <= TOL
 
 
-@pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize('name', ['ladder', 'hh_ladder', 'energy', 'ring'])
 def test_affine_tile_variants(variant, name, monkeypatch):
     """Every tile variant of the strided-GEMM kernel gives the oracle result."""
