@@ -286,6 +286,15 @@ def run_ccsd(args):
     solver.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
     torch.cuda.synchronize()
     wall32 = time.perf_counter() - t0
+    # split-bfloat16 GEMM tiles in the float32 phase
+    mixed_bf = solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200, bulk='bf16x3')
+    solver.solve(tol=0.0, maxit=args.warmup, precision='f32')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
+    torch.cuda.synchronize()
+    wall_bf = time.perf_counter() - t0
+    step(f'split-bf16 mixed solve {mixed_bf["f32_iterations"]} + {mixed_bf["f64_iterations"]} iterations')
     ref = None
     if os.path.exists(GOLDEN):
         g = json.load(open(GOLDEN))
@@ -326,7 +335,12 @@ def run_ccsd(args):
                             'abs_dE_vs_f64': abs(mixed['energy'] - res['energy']),
                             'abs_dE_vs_oracle': None if ref is None else abs(mixed['energy'] - ref),
                             'f32_iters_per_s': round(args.steps / wall32, 3),
-                            'f32_setup_seconds': round(f32_setup, 2)},
+                            'f32_setup_seconds': round(f32_setup, 2),
+                            'split_bf16': {'f32_phase_iterations': mixed_bf['f32_iterations'],
+                                           'f64_iterations': mixed_bf['f64_iterations'],
+                                           'seconds': round(mixed_bf['seconds'], 4),
+                                           'abs_dE_vs_f64': abs(mixed_bf['energy'] - res['energy']),
+                                           'iters_per_s': round(args.steps / wall_bf, 3)}},
         'setup_seconds': round(setup_s, 2),
     }
     if tune_report is not None:
@@ -428,6 +442,23 @@ def run_triples(args):
     # the worst single element relative to its own magnitude above 1e-3 max
     big = r64.abs() > 1e-3 * r64.abs().max()
     f32_err['worst_element_rel'] = float(((r32 - r64).abs()[big] / r64.abs()[big]).max())
+    # split-bfloat16 GEMM tiles
+    solver.build_f32(bulk='bf16x3')
+    for n in ('T2', 'T3'):
+        solver.buf32[n].copy_(solver.buf[n])
+    for _ in range(args.warmup):
+        for p in solver.plans32:
+            p.execute(solver.bufs32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in solver.plans32:
+            p.execute(solver.bufs32)
+    torch.cuda.synchronize()
+    wall_bf = time.perf_counter() - t0
+    rbf = solver.buf32['R3'].double()
+    bf_err = {'max_abs_rel': float((rbf - r64).abs().max() / r64.abs().max()),
+              'norm_rel': float((rbf - r64).norm() / r64.norm())}
     # size-independent check: every term is linear in (T2, T3)
     r1 = solver.buf['R3'].clone()
     for n in ('T2', 'T3'):
@@ -461,6 +492,8 @@ def run_triples(args):
         'linearity_rel_err': lin,
         'f32': {'tflops': round(solver.flops * args.steps / wall32 / 1e12, 4),
                 'ms_per_step': round(wall32 * 1e3 / args.steps, 4), 'error_vs_f64': f32_err},
+        'split_bf16': {'tflops': round(solver.flops * args.steps / wall_bf / 1e12, 4),
+                       'ms_per_step': round(wall_bf * 1e3 / args.steps, 4), 'error_vs_f64': bf_err},
         'setup_seconds': round(setup_s, 2),
     }
     if tune_report is not None:

@@ -264,7 +264,7 @@ class CCSD:
                 p.add_sym_ab(self.L[res], base + self.slot[res])
 
     def solve_mixed(self, tol=1e-8, switch=1e-6, maxit=100, ndiis=8, mode='graph', verbose=False,
-                    stall=6):
+                    stall=6, bulk=None):
         """Mixed-precision solve: float32 residuals (solve(precision='f32'))
         until ||g|| < switch (or the float32 residual stalls: `stall`
         iterations without a 10% gain), then float64 iterations from those
@@ -275,6 +275,9 @@ class CCSD:
         continuing the float32 one.
         Returns the float64 phase's result with 'f32_iterations',
         'f32_seconds' and the total 'seconds' / 'iterations'."""
+        if bulk is not None and getattr(self, 'bulk', None) != bulk:
+            self.build_f32(bulk=bulk)       # bulk: 'f32' or 'bf16x3' float32 plans
+            self._graph = None
         r32 = self.solve(tol=switch, maxit=maxit, ndiis=ndiis, mode=mode, precision='f32',
                          verbose=verbose, stall=stall)
         r64 = self.solve(tol=tol, maxit=maxit, ndiis=ndiis, mode=mode, precision='f64',
@@ -332,10 +335,11 @@ class CCSD:
             p.execute(self.bufs, stream)
 
     # ------------------------------------------------------ float32 lists
-    def build_f32(self, plans=True):
+    def build_f32(self, plans=True, bulk='f32'):
         """Float32 copies of every list (integrals rounded once) and plans
         that run the formula on them: the bulk of a mixed-precision solve
-        (plans=False: the lists only, for a batched driver)."""
+        (plans=False: the lists only, for a batched driver; bulk 'bf16x3':
+        the GEMM tiles as split-bfloat16 three-term products)."""
         import gecco_hip
         torch = self.torch
         lists = self.formula['lists']
@@ -355,10 +359,11 @@ class CCSD:
         self.plans32 = []
         for li, level in enumerate(self.schedule):
             p = gecco_hip.Plan()
-            p.set_precision('f32')
+            p.set_precision(bulk)
             self.record_level(p, level, 0)
             p.finalize()
             self.plans32.append(p)
+        self.bulk = bulk
         self.flops, self.nblocks = flops, nblocks
 
     def evaluate32(self, stream=None):

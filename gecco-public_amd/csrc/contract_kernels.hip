@@ -33,6 +33,14 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
 // Row of accumulator register r in lane group g = lane / 16 of a 16 x 16
 // MFMA tile (the column is lane % 16): f64 16x16x4 puts row 4 r + g in
 // register r, f32 16x16x4 row 4 g + r.
+typedef short v4s __attribute__((ext_vector_type(4)));
+// bfloat16 of a float (nearest, ties to even) and back
+__device__ __forceinline__ unsigned short bf16_of(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 template <typename T> __device__ __forceinline__ int mrow(int r, int g);
 template <> __device__ __forceinline__ int mrow<double>(int r, int g) { return 4 * r + g; }
 template <> __device__ __forceinline__ int mrow<float>(int r, int g) { return 4 * g + r; }
@@ -615,7 +623,12 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 // 128 x 128 tiles hold 128 accumulator registers per lane: the launch bound
 // asks for 2 waves per SIMD (at most 256 registers per lane in total).
 // PF: K steps whose operand loads are in flight while a step's MFMAs run
-// (1: the next step; 2: the next two, in two register sets).
+// (1: the next step; 2: the next two, in two register sets).  PF = 3
+// (float lists, TBK = 16): the next step, and the GEMM tiles run as split-
+// bfloat16 products -- each operand element x = hi + lo with hi = bf16(x),
+// lo = bf16(x - hi) stored k-contiguous in LDS, and the tile product as
+// lo.hi + hi.lo + hi.hi on v_mfma_f32_16x16x16_bf16 (float32 accumulation);
+// the short and thin tasks of the launch stay float32.
 template <int TBM, int TBN, int TBK, typename T, int PF = 1>
 __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
@@ -674,7 +687,89 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         // the masked loads put into LDS
         const bool wave_live = __builtin_amdgcn_readfirstlane(
             (m0 + wm * (TBM / 2) < M && n0 + wn * (TBN / 2) < N) ? 1 : 0);
-        if constexpr (PF == 2) {
+        if constexpr (PF == 3) {
+            static_assert(sizeof(T) == 4 && TBK == 16, "split-bfloat16 tiles: float lists, TBK 16");
+            constexpr int LK = TBK + 4;                       // padded k row (bf16)
+            constexpr int HALF = 2 * (TBM + TBN) * LK;        // one buffer: A hi, A lo, B hi, B lo
+            static_assert(2 * HALF * 2 <= LDS_N * (int)sizeof(T), "LDS too small for split tiles");
+            unsigned short *const L16 = reinterpret_cast<unsigned short *>(lds);
+            auto split_store = [&](int c, const T *xa, const T *xb) {
+                unsigned short *ah = L16 + c * HALF, *al = ah + TBM * LK;
+                unsigned short *bh = al + TBM * LK, *bl = bh + TBN * LK;
+#pragma unroll
+                for (int r = 0; r < AffView<TBM, TBK, T>::L; ++r) {
+                    const int k = va.kk0 + va.kstep * r, x = va.xi0 + va.xstep * r;
+                    const unsigned short h = bf16_of(xa[r]);
+                    ah[x * LK + k] = h;
+                    al[x * LK + k] = bf16_of(xa[r] - bf16_val(h));
+                }
+#pragma unroll
+                for (int r = 0; r < AffView<TBN, TBK, T>::L; ++r) {
+                    const int k = vb.kk0 + vb.kstep * r, x = vb.xi0 + vb.xstep * r;
+                    const unsigned short h = bf16_of(xb[r]);
+                    bh[x * LK + k] = h;
+                    bl[x * LK + k] = bf16_of(xb[r] - bf16_val(h));
+                }
+            };
+            T ra[AffView<TBM, TBK, T>::L], rb[AffView<TBN, TBK, T>::L];
+            __syncthreads();   // LDS of the previous task is no longer read
+            va.load(kl, kh, ra);
+            vb.load(kl, kh, rb);
+            split_store(0, ra, rb);
+            __syncthreads();
+            int cur = 0;
+            const int kg = 4 * (lane >> 4);                  // this lane's 4 k of the 16
+            for (;;) {
+                bool more = true;
+                kl += TBK;
+                if (kl >= kh) {
+                    kb += kcur;
+                    blk = blocks[blk].next;
+                    more = blk >= 0 && kb < task.k1;
+                    if (more) {
+                        const DevBlock &nb = blocks[blk];
+                        kcur = nb.K;
+                        kl = 0;
+                        kh = min(kcur, task.k1 - kb);
+                        va.init(bufs[nb.buf_a] + nb.base_a, nb.A, kcur, m0, M, 0, tid);
+                        vb.init(bufs[nb.buf_b] + nb.base_b, nb.B, kcur, n0, N, 0, tid);
+                    }
+                }
+                if (more) {
+                    va.load(kl, kh, ra);
+                    vb.load(kl, kh, rb);
+                }
+                if (wave_live) {
+                    const unsigned short *ah = L16 + cur * HALF, *al = ah + TBM * LK;
+                    const unsigned short *bh = al + TBM * LK, *bl = bh + TBN * LK;
+                    v4s a_h[TM], a_l[TM], b_h[TN], b_l[TN];
+#pragma unroll
+                    for (int i = 0; i < TM; ++i) {
+                        const int row = wm * (TBM / 2) + i * 16 + lcol;
+                        a_h[i] = *reinterpret_cast<const v4s *>(ah + row * LK + kg);
+                        a_l[i] = *reinterpret_cast<const v4s *>(al + row * LK + kg);
+                    }
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        const int row = wn * (TBN / 2) + j * 16 + lcol;
+                        b_h[j] = *reinterpret_cast<const v4s *>(bh + row * LK + kg);
+                        b_l[j] = *reinterpret_cast<const v4s *>(bl + row * LK + kg);
+                    }
+#pragma unroll
+                    for (int i = 0; i < TM; ++i)
+#pragma unroll
+                        for (int j = 0; j < TN; ++j) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_l[i], b_h[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_h[i], b_l[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_h[i], b_h[j], acc[i][j], 0, 0, 0);
+                        }
+                }
+                if (more) split_store(cur ^ 1, ra, rb);
+                __syncthreads();
+                cur ^= 1;
+                if (!more) break;
+            }
+        } else if constexpr (PF == 2) {
             // fetch(): loads the step at the current position and advances
             // the position (along the chain); false once the range is done
             bool have = true;
@@ -1015,7 +1110,7 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
     static const int shapes[][3] = {{64, 64, 32}, {64, 64, 16}, {64, 128, 16},
                                     {128, 64, 16}, {128, 128, 16}, {128, 128, 8},
-                                    {64, 64, 16}, {64, 64, 32}};
+                                    {64, 64, 16}, {64, 64, 32}, {64, 64, 16}};
     if (v < 0 || v >= (int)(sizeof shapes / sizeof shapes[0])) return 1;
     *bm = shapes[v][0];
     *bn = shapes[v][1];
@@ -1038,6 +1133,14 @@ static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *ta
     // two K steps of loads in flight
     case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    // split-bfloat16 GEMM tiles (float lists only)
+    case 8:
+        if constexpr (sizeof(T) == 4) {
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 3>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip);
+            break;
+        } else {
+            return 1;
+        }
     default: return 1;
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;

@@ -808,6 +808,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     int abm = 64, abn = 64, abk = 32;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
         return fail(GECCO_HIP_EINVAL, "unknown tile variant %d", p->affine_variant);
+    if (p->affine_variant == 8 && p->elem != 4)
+        return fail(GECCO_HIP_EINVAL, "split-bfloat16 tiles (variant 8) need a float32 plan");
     // class of a task's block: 0 strided-GEMM launch, 3 gather kernel; tile
     // shape per block
     std::vector<int> cls(p->blocks.size());

@@ -280,9 +280,13 @@ class Plan:
                                               v(max_kchunk)))
 
     def set_precision(self, precision):
-        """'f64' (default) or 'f32' lists; before finalize."""
-        elem = {'f64': 8, 'f32': 4}[precision]
+        """'f64' (default), 'f32' or 'bf16x3' (float32 lists whose GEMM
+        tiles run as split-bfloat16 three-term products: tile variant 8);
+        before finalize."""
+        elem = {'f64': 8, 'f32': 4, 'bf16x3': 4}[precision]
         _check(_lib.gecco_hip_plan_set_precision(self._h, elem))
+        if precision == 'bf16x3':
+            self.set_tuning(variant=8)
         self._elem = elem
 
     def keep_host(self, on=True):

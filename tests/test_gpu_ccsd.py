@@ -220,6 +220,11 @@ def test_mixed_precision_solve_reaches_the_f64_energy():
         assert res['residual'] < 1e-10
         assert abs(res['energy'] - ref['energy']) < 1e-10
         assert res['f32_iterations'] > 0 and res['f64_iterations'] < ref['iterations']
+    # split-bfloat16 bulk, float64 refinement: the same energy
+    rb = c.solve_mixed(tol=1e-10, switch=1e-5, maxit=100, bulk='bf16x3')
+    print('bf16x3 mixed', rb['f32_iterations'], '+', rb['f64_iterations'], flush=True)
+    assert rb['residual'] < 1e-10 and abs(rb['energy'] - ref['energy']) < 1e-10
+    c.build_f32(bulk='f32')
     # below its floor the float32 residual stalls: the guard ends the phase
     st = c.solve(tol=1e-13, maxit=150, precision='f32', stall=5)
     assert st['stalled'] and st['iterations'] < 150 and st['residual'] > 1e-13
