@@ -9,7 +9,9 @@ subspace (ccsd.CCSD with plans=False); instance k's lists take the buffer
 slots k * nlists + list index of the shared plans and it is instance group
 k of every plan.  Each instance's asynchronous DIIS step sets its entry of a
 device flag array when it meets the tolerance (leaving the amplitudes
-unchanged), and every launch skips the work of the flagged instances, so a
+unchanged; all instances' DIIS steps run as one group of three launches,
+gecco_hip.DIISGroup), and every launch skips the work of the flagged
+instances, so a
 converged instance costs nothing further while the replayed graph stays the
 same; the batch iterates until every instance has converged or maxit is
 reached.
@@ -151,14 +153,17 @@ class BatchedCCSD:
             self.done.zero_()
             for k, d in enumerate(self._diis):
                 d.set_flag(self.done[k:k + 1])
+            # every instance's DIIS step in three launches
+            self._group = gecco_hip.DIISGroup(self._diis, [c.amp for c in self.inst],
+                                              [c.res for c in self.inst], [c.diag for c in self.inst],
+                                              [c.buf['E'] for c in self.inst])
             self._steps = 0
             self._graph = None
         stream = torch.cuda.current_stream().cuda_stream
 
         def iteration(s):
             self.evaluate(s, precision)
-            for c, d in zip(self.inst, self._diis):
-                d.step_async(c.amp, c.res, c.diag, energy=c.buf['E'], tol=tol, stream=s)
+            self._group.step_async(tol, s)
 
         n = len(self.inst)
         conv_at = [None] * n

@@ -947,15 +947,50 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
     }
 }
 
+// Streaming launches (gathers, list initialisation, projection) with many
+// tasks: workgroup w of G takes tasks w, w + G, w + 2G, ... (per of them;
+// strided, so that a contiguous run of live tasks -- one instance's lists
+// -- spreads over all workgroups).  Its threads read the task headers
+// together and keep the live ones (group not flagged in skip) in LDS, so
+// tasks of converged instances cost one parallel read instead of a
+// workgroup each.  per = 1 (at most 8192 tasks): one task per workgroup.
+// The task bodies hold no barriers and write disjoint elements, so the
+// order of the live tasks does not matter.
+template <typename TaskT, typename F>
+__device__ __forceinline__ void for_live_tasks(const TaskT *__restrict__ tasks, int ntasks, int per,
+                                               const int32_t *__restrict__ skip, F &&f) {
+    if (per == 1) {
+        const TaskT t = tasks[blockIdx.x];
+        if (!skipped(skip, t.grp)) f(t);
+        return;
+    }
+    __shared__ int live[256];
+    __shared__ int nlive;
+    if (threadIdx.x == 0) nlive = 0;
+    __syncthreads();
+    const int mine = (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
+    if ((int)threadIdx.x < per && mine < ntasks && !skipped(skip, tasks[mine].grp))
+        live[atomicAdd(&nlive, 1)] = mine;
+    __syncthreads();
+    const int nl = nlive;
+    for (int i = 0; i < nl; ++i) {
+        const TaskT t = tasks[live[i]];
+        f(t);
+    }
+}
+
+static inline int tasks_per_group(int ntasks) {
+    return ntasks <= 8192 ? 1 : min(256, (ntasks + 8191) / 8192);
+}
+
 // Signed gather of operands into scratch (one launch for all jobs of a
 // plan); HBM-bound: one read of each allowed source element, one write.
 template <typename T>
 __global__ __launch_bounds__(256) void gather_operands(
     const DevGather *__restrict__ jobs, const DevGatherTask *__restrict__ tasks, int ntasks,
     T *const *__restrict__ bufs, T *__restrict__ scratch,
-    const int32_t *__restrict__ skip) {
-    const DevGatherTask t = tasks[blockIdx.x];
-    if (skipped(skip, t.grp)) return;
+    const int32_t *__restrict__ skip, int per) {
+  for_live_tasks(tasks, ntasks, per, skip, [&](const DevGatherTask &t) {
     const DevGather &g = jobs[t.job];
     const T *__restrict__ src = bufs[g.src_buf] + g.src_base;
     T *__restrict__ dst = scratch + g.dst_base;
@@ -1006,6 +1041,7 @@ __global__ __launch_bounds__(256) void gather_operands(
             if (e < (uint32_t)t.e1) stg(dst + e, neg[u] ? -val[u] : val[u]);
         }
     }
+  });
 }
 
 // List initialisation (copy with factor, or zero) ahead of the contractions.
@@ -1013,9 +1049,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void init_lists(const DevInit *__restrict__ jobs,
                                                       const DevInitTask *__restrict__ tasks,
                                                       T *const *__restrict__ bufs,
-                                                      const int32_t *__restrict__ skip) {
-    const DevInitTask t = tasks[blockIdx.x];
-    if (skipped(skip, t.grp)) return;
+                                                      const int32_t *__restrict__ skip, int ntasks,
+                                                      int per) {
+  for_live_tasks(tasks, ntasks, per, skip, [&](const DevInitTask &t) {
     const DevInit &j = jobs[t.job];
     T *__restrict__ dst = bufs[j.dst] + j.off_dst;
     // U elements per thread and round, their loads in flight together
@@ -1039,6 +1075,7 @@ __global__ __launch_bounds__(256) void init_lists(const DevInit *__restrict__ jo
     } else {
         for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) stg(dst + e, (T)0);
     }
+  });
 }
 
 // Partial trace rows: dst[row] += fact * sum_k sign_k * src[col_k] over the
@@ -1072,9 +1109,9 @@ __global__ __launch_bounds__(256) void sym_ab_batched(const DevSymStep *__restri
                                                           const int32_t *__restrict__ i1,
                                                           const int32_t *__restrict__ i2,
                                                           const int8_t *__restrict__ fsign,
-                                                          const int32_t *__restrict__ skip) {
-    const DevSymTask t = tasks[blockIdx.x];
-    if (skipped(skip, t.grp)) return;
+                                                          const int32_t *__restrict__ skip, int ntasks,
+                                                          int per) {
+  for_live_tasks(tasks, ntasks, per, skip, [&](const DevSymTask &t) {
     const DevSymStep &st = steps[t.step];
     T *__restrict__ x = bufs[st.buf] + st.base;
     const T prefac = (T)st.prefac;
@@ -1085,6 +1122,7 @@ __global__ __launch_bounds__(256) void sym_ab_batched(const DevSymStep *__restri
         stg(x + a, v);
         if (c != a) stg(x + c, f * v);
     }
+  });
 }
 
 }  // namespace gecco
@@ -1162,12 +1200,13 @@ extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::De
                                    int ntasks, void *const *bufs, void *scratch,
                                    const int32_t *skip, int elem, hipStream_t stream) {
     if (ntasks <= 0) return 0;
+    const int per = gecco::tasks_per_group(ntasks), grid = (ntasks + per - 1) / per;
     if (elem == 4)
-        hipLaunchKernelGGL(gecco::gather_operands<float>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                           ntasks, tbl<float>(bufs), (float *)scratch, skip);
+        hipLaunchKernelGGL(gecco::gather_operands<float>, dim3(grid), dim3(256), 0, stream, jobs, tasks,
+                           ntasks, tbl<float>(bufs), (float *)scratch, skip, per);
     else
-        hipLaunchKernelGGL(gecco::gather_operands<double>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                           ntasks, tbl<double>(bufs), (double *)scratch, skip);
+        hipLaunchKernelGGL(gecco::gather_operands<double>, dim3(grid), dim3(256), 0, stream, jobs, tasks,
+                           ntasks, tbl<double>(bufs), (double *)scratch, skip, per);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -1175,12 +1214,13 @@ extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevIni
                                  int ntasks, void *const *bufs, const int32_t *skip, int elem,
                                  hipStream_t stream) {
     if (ntasks <= 0) return 0;
+    const int per = gecco::tasks_per_group(ntasks), grid = (ntasks + per - 1) / per;
     if (elem == 4)
-        hipLaunchKernelGGL(gecco::init_lists<float>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                           tbl<float>(bufs), skip);
+        hipLaunchKernelGGL(gecco::init_lists<float>, dim3(grid), dim3(256), 0, stream, jobs, tasks,
+                           tbl<float>(bufs), skip, ntasks, per);
     else
-        hipLaunchKernelGGL(gecco::init_lists<double>, dim3(ntasks), dim3(256), 0, stream, jobs, tasks,
-                           tbl<double>(bufs), skip);
+        hipLaunchKernelGGL(gecco::init_lists<double>, dim3(grid), dim3(256), 0, stream, jobs, tasks,
+                           tbl<double>(bufs), skip, ntasks, per);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -1204,11 +1244,12 @@ extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const int8_t *fsign, const int32_t *skip, int elem,
                                         hipStream_t stream) {
     if (ntasks <= 0) return 0;
+    const int per = gecco::tasks_per_group(ntasks), grid = (ntasks + per - 1) / per;
     if (elem == 4)
-        hipLaunchKernelGGL(gecco::sym_ab_batched<float>, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
-                           tbl<float>(bufs), i1, i2, fsign, skip);
+        hipLaunchKernelGGL(gecco::sym_ab_batched<float>, dim3(grid), dim3(256), 0, stream, steps, tasks,
+                           tbl<float>(bufs), i1, i2, fsign, skip, ntasks, per);
     else
-        hipLaunchKernelGGL(gecco::sym_ab_batched<double>, dim3(ntasks), dim3(256), 0, stream, steps, tasks,
-                           tbl<double>(bufs), i1, i2, fsign, skip);
+        hipLaunchKernelGGL(gecco::sym_ab_batched<double>, dim3(grid), dim3(256), 0, stream, steps, tasks,
+                           tbl<double>(bufs), i1, i2, fsign, skip, ntasks, per);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }

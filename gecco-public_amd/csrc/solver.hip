@@ -224,11 +224,10 @@ __global__ __launch_bounds__(NTH) void diis_prepare_async_f64(
 // workgroup), the status record, and the slot of the next step's vector:
 // a free one, or the oldest one's when full, whose row and column of B are
 // dropped (optc_update_redsp1 'shift').
-__global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
-                                                       const double *__restrict__ energy,
-                                                       DevState *__restrict__ st, int maxsub,
-                                                       double tol, StatusRec *ring,
-                                                       int32_t *__restrict__ flag) {
+__device__ __forceinline__ void finish_body(const double *__restrict__ part, int ngroup,
+                                            const double *__restrict__ energy,
+                                            DevState *__restrict__ st, int maxsub, double tol,
+                                            StatusRec *ring, int32_t *__restrict__ flag) {
     // a flag set before this step (the instance converged and its residual
     // is no longer re-evaluated): repeat the converged record, leave the
     // subspace and t as they are
@@ -370,6 +369,54 @@ __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict_
     }
     st->order[mm] = slot;
     st->m = mm + 1;
+}
+
+__global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
+                                                       const double *__restrict__ energy,
+                                                       DevState *__restrict__ st, int maxsub,
+                                                       double tol, StatusRec *ring,
+                                                       int32_t *__restrict__ flag) {
+    finish_body(part, ngroup, energy, st, maxsub, tol, ring, flag);
+}
+
+// Grouped asynchronous steps of many subspaces (independent problems):
+// blockIdx.y (prepare, expand) or blockIdx.x (finish) selects the member.
+struct GroupMember {
+    double *t;
+    const double *rest, *diag, *scale, *energy;
+    double *rsub, *vsub, *part;
+    DevState *st;
+    StatusRec *ring;
+    int32_t *flag;
+    int64_t n;
+    int maxsub, pad;
+};
+
+__global__ __launch_bounds__(NTH) void diis_prepare_group_f64(const GroupMember *__restrict__ mem) {
+    const GroupMember &g = mem[blockIdx.y];
+    if (g.flag && *g.flag) return;
+    Slots sl;
+    sl.m = g.st->m;
+    for (int j = 0; j < MAXSUB; ++j) sl.s[j] = j < sl.m ? g.st->order[j] : 0;
+    prepare_body(g.t, g.rest, g.diag, g.scale, g.rsub, g.vsub, g.n, sl, g.part);
+}
+
+__global__ __launch_bounds__(NTH) void diis_finish_group_f64(const GroupMember *__restrict__ mem,
+                                                             int ngroup, double tol) {
+    const GroupMember &g = mem[blockIdx.x];
+    finish_body(g.part, ngroup, g.energy, g.st, g.maxsub, tol, g.ring, g.flag);
+}
+
+__global__ __launch_bounds__(NTH) void diis_expand_group_f64(const GroupMember *__restrict__ mem) {
+    const GroupMember &g = mem[blockIdx.y];
+    if (g.st->skip) return;
+    Weights wt;
+    wt.m = g.st->mw;
+    for (int j = 0; j < MAXSUB; ++j) {
+        wt.s[j] = j < wt.m ? g.st->wslot[j] : 0;
+        wt.w[j] = j < wt.m ? g.st->w[j] : 0.0;
+    }
+    expand_body(g.t, g.vsub, g.n, wt);
 }
 
 __global__ __launch_bounds__(NTH) void diis_expand_async_f64(double *__restrict__ t,
@@ -595,6 +642,79 @@ int gecco_hip_diis_step_async(gecco_hip_diis *d, double *t, const double *rest, 
                        d->maxsub, tol, d->ring, d->flag);
     hipLaunchKernelGGL(diis_expand_async_f64, dim3(ng), dim3(NTH), 0, s, t, d->vsub, d->n, d->dstate);
     return check(hipGetLastError(), "diis async kernels");
+}
+
+struct gecco_hip_diis_group {
+    std::vector<gecco_hip_diis *> members;
+    GroupMember *d_mem = nullptr;
+    int ng = 1;
+};
+
+int gecco_hip_diis_group_create(gecco_hip_diis *const *d, int64_t count, double *const *t,
+                                const double *const *rest, const double *const *diag,
+                                const double *const *energy, gecco_hip_diis_group **out) {
+    gecco::clear_error();
+    if (!d || !t || !rest || !diag || !out || count <= 0 || count > 65535)
+        return fail(GECCO_HIP_EINVAL, "diis_group_create: bad arguments");
+    std::vector<GroupMember> mem((size_t)count);
+    int64_t nmax = 1;
+    for (int64_t k = 0; k < count; ++k) {
+        gecco_hip_diis *x = d[k];
+        if (!x || !t[k] || !rest[k] || !diag[k])
+            return fail(GECCO_HIP_EINVAL, "diis_group_create: member %lld incomplete", (long long)k);
+        if (!x->order.empty())
+            return fail(GECCO_HIP_ESTATE, "diis_group_create: member %lld holds synchronous steps", (long long)k);
+        GroupMember &g = mem[(size_t)k];
+        std::memset(&g, 0, sizeof g);
+        g.t = t[k];
+        g.rest = rest[k];
+        g.diag = diag[k];
+        g.scale = x->scale;
+        g.energy = energy ? energy[k] : nullptr;
+        g.rsub = x->rsub;
+        g.vsub = x->vsub;
+        g.part = x->part;
+        g.st = x->dstate;
+        g.ring = x->ring;
+        g.flag = x->flag;
+        g.n = x->n;
+        g.maxsub = x->maxsub;
+        nmax = std::max<int64_t>(nmax, x->n);
+    }
+    auto *grp = new gecco_hip_diis_group();
+    grp->members.assign(d, d + count);
+    grp->ng = (int)std::min<int64_t>(NGROUP, std::max<int64_t>(1, (nmax + NTH - 1) / NTH));
+    if (check(hipMalloc(&grp->d_mem, mem.size() * sizeof(GroupMember)), "hipMalloc(diis group)") ||
+        check(hipMemcpy(grp->d_mem, mem.data(), mem.size() * sizeof(GroupMember), hipMemcpyHostToDevice),
+              "hipMemcpy(diis group)")) {
+        if (grp->d_mem) (void)hipFree(grp->d_mem);
+        delete grp;
+        return GECCO_HIP_EDEVICE;
+    }
+    *out = grp;
+    return 0;
+}
+
+int gecco_hip_diis_group_destroy(gecco_hip_diis_group *g) {
+    if (!g) return 0;
+    if (g->d_mem) (void)hipFree(g->d_mem);
+    delete g;
+    return 0;
+}
+
+int gecco_hip_diis_group_step_async(gecco_hip_diis_group *g, double tol, void *stream) {
+    gecco::clear_error();
+    if (!g) return fail(GECCO_HIP_EINVAL, "null diis group");
+    for (auto *x : g->members) {
+        if (!x->order.empty()) return fail(GECCO_HIP_ESTATE, "diis group member holds synchronous steps");
+        x->async = true;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned cnt = (unsigned)g->members.size();
+    hipLaunchKernelGGL(diis_prepare_group_f64, dim3(g->ng, cnt), dim3(NTH), 0, s, g->d_mem);
+    hipLaunchKernelGGL(diis_finish_group_f64, dim3(cnt), dim3(NTH), 0, s, g->d_mem, g->ng, tol);
+    hipLaunchKernelGGL(diis_expand_group_f64, dim3(g->ng, cnt), dim3(NTH), 0, s, g->d_mem);
+    return check(hipGetLastError(), "diis group kernels");
 }
 
 int gecco_hip_diis_set_scale(gecco_hip_diis *d, const double *scale) {

@@ -354,7 +354,8 @@ MODEL_SYMBOLS = ('gecco_hip_orbs_create', 'gecco_hip_orbs_destroy', 'gecco_hip_o
 SOLVER_SYMBOLS = ('gecco_hip_diis_create', 'gecco_hip_diis_destroy', 'gecco_hip_diis_reset',
                   'gecco_hip_diis_step', 'gecco_hip_diis_weights', 'gecco_hip_diis_matrix',
                   'gecco_hip_diis_step_async', 'gecco_hip_diis_status', 'gecco_hip_diis_set_flag',
-                  'gecco_hip_diis_set_scale')
+                  'gecco_hip_diis_set_scale', 'gecco_hip_diis_group_create',
+                  'gecco_hip_diis_group_step_async', 'gecco_hip_diis_group_destroy')
 MLP_SYMBOLS = ('gecco_hip_mlp_infer', 'gecco_hip_mlp_scale', 'gecco_hip_mlp_grad', 'gecco_hip_mlp_adam')
 MLP_P = 17281
 MLP_SCALE_WORK = 2 + 2 * 512
@@ -410,6 +411,11 @@ _lib.gecco_hip_diis_step_async.argtypes = [VP, VP, VP, VP, VP, ctypes.c_double, 
 _lib.gecco_hip_diis_status.argtypes = [VP, I64, PD, PD, PI64]
 _lib.gecco_hip_diis_set_flag.argtypes = [VP, VP]
 _lib.gecco_hip_diis_set_scale.argtypes = [VP, VP]
+_lib.gecco_hip_diis_group_create.argtypes = [ctypes.POINTER(VP), I64, ctypes.POINTER(VP),
+                                             ctypes.POINTER(VP), ctypes.POINTER(VP),
+                                             ctypes.POINTER(VP), ctypes.POINTER(VP)]
+_lib.gecco_hip_diis_group_step_async.argtypes = [VP, ctypes.c_double, VP]
+_lib.gecco_hip_diis_group_destroy.argtypes = [VP]
 _lib.gecco_hip_mlp_infer.argtypes = [VP, VP, I64, VP, VP]
 _lib.gecco_hip_mlp_scale.argtypes = [VP, VP, VP, VP, I64, ctypes.c_double, VP, VP, VP]
 _lib.gecco_hip_mlp_grad.argtypes = [VP, VP, VP, VP, I64, VP, VP, VP, VP]
@@ -745,3 +751,34 @@ def mlp_adam(params, m, v, grad, step, lr, stream=None):
         _mlp_check(t, 'torch.float32', MLP_P, f'mlp_adam {nm}')
     _check(_lib.gecco_hip_mlp_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(grad), int(step), float(lr),
                                    VP(0 if stream is None else int(stream))))
+
+
+class DIISGroup:
+    """Asynchronous steps of many DIIS subspaces in three launches
+    (gecco_hip_diis_group_*): member k steps t[k] with rest[k], diag[k] and
+    energy[k].  Set each member's flag and scale before creating the group."""
+
+    def __init__(self, diis, t, rest, diag, energy=None):
+        n = len(diis)
+        for k in range(n):
+            for x in (t[k], rest[k], diag[k]):
+                if x.numel() != diis[k].n:
+                    raise GeccoHipError('DIISGroup: vector length mismatch')
+        arr = lambda xs: (VP * n)(*[None if x is None else _ptr(x).value for x in xs])
+        self._keep = (list(diis), list(t), list(rest), list(diag), list(energy or []))
+        h = VP()
+        hd = (VP * n)(*[d._h.value for d in diis])
+        _check(_lib.gecco_hip_diis_group_create(hd, n, arr(t), arr(rest),
+                                                arr(diag), None if energy is None else arr(energy),
+                                                ctypes.byref(h)))
+        self._h = h
+
+    def step_async(self, tol=0.0, stream=None):
+        _check(_lib.gecco_hip_diis_group_step_async(self._h, float(tol),
+                                                    VP(0 if stream is None else int(stream))))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            _lib.gecco_hip_diis_group_destroy(h)
+            self._h = None

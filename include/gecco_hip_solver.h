@@ -87,6 +87,21 @@ int gecco_hip_diis_set_flag(gecco_hip_diis *d, int32_t *flag);
  * D only). */
 int gecco_hip_diis_set_scale(gecco_hip_diis *d, const double *scale);
 
+/* Many subspaces stepped together (independent problems, e.g. batched CC
+ * instances): member k steps t[k] with rest[k], diag[k] and energy[k]
+ * (energy may be NULL) exactly as gecco_hip_diis_step_async would, but one
+ * step of the whole group is three launches (all members' fused passes, one
+ * workgroup per member for the small solve, all expansions).  Each member
+ * keeps its own records (gecco_hip_diis_status), flag and scale; set those
+ * before creating the group, and recreate it after a member's reset.  No
+ * counterpart in the reference. */
+typedef struct gecco_hip_diis_group gecco_hip_diis_group;
+int gecco_hip_diis_group_create(gecco_hip_diis *const *d, int64_t count, double *const *t,
+                                const double *const *rest, const double *const *diag,
+                                const double *const *energy, gecco_hip_diis_group **out);
+int gecco_hip_diis_group_step_async(gecco_hip_diis_group *g, double tol, void *stream);
+int gecco_hip_diis_group_destroy(gecco_hip_diis_group *g);
+
 /* Weights of the last step (w[0..m-1], oldest vector first) and the
  * current subspace matrix (b[m*m], row-major), for checks. */
 int gecco_hip_diis_weights(const gecco_hip_diis *d, double *w, int64_t nw, int64_t *m);

@@ -216,3 +216,40 @@ def test_async_flag_freezes_converged_steps():
     assert not torch.equal(t, t_conv)
     with pytest.raises(gecco_hip.GeccoHipError):
         diis.set_flag(torch.zeros(2, dtype=torch.int32, device=dev))
+
+
+def test_diis_group_matches_single_steps():
+    """Three subspaces of different lengths stepped as one group give the
+    same amplitudes and records as separate asynchronous steps."""
+    torch = _torch()
+    import gecco_hip
+    dev = torch.device('cuda:0')
+    ns = [1000, 70001, 313]
+    rng = np.random.default_rng(17)
+    probs = []
+    for n in ns:
+        d = rng.uniform(0.5, 3.0, n)
+        b = 0.1 * rng.standard_normal(n)
+        c = 0.05 * rng.standard_normal(n)
+        probs.append((torch.from_numpy(d).to(dev), b, c))
+    single = [gecco_hip.DIIS(n, 4) for n in ns]
+    grouped = [gecco_hip.DIIS(n, 4) for n in ns]
+    t1 = [torch.zeros(n, dtype=torch.float64, device=dev) for n in ns]
+    t2 = [torch.zeros(n, dtype=torch.float64, device=dev) for n in ns]
+    rest = [torch.zeros(n, dtype=torch.float64, device=dev) for n in ns]
+    e = [torch.tensor([-1.0 - k], dtype=torch.float64, device=dev) for k in range(3)]
+    group = gecco_hip.DIISGroup(grouped, t2, rest, [p[0] for p in probs], e)
+    for it in range(7):
+        for k, (d, b, c) in enumerate(probs):
+            tt = t1[k].cpu().numpy()
+            rest[k].copy_(torch.from_numpy(b + c * tt * tt + 0.01 * np.roll(tt, 1)))
+            t2[k].copy_(t1[k])
+            single[k].step_async(t1[k], rest[k], d, energy=e[k])
+        group.step_async()
+        torch.cuda.synchronize()
+        for k in range(3):
+            g1, e1, c1 = single[k].status(it)
+            g2, e2, c2 = grouped[k].status(it)
+            assert c1 == c2 == 0 and e1 == e2 == -1.0 - k
+            assert abs(g1 - g2) <= 1e-12 * g1
+            assert torch.equal(t1[k], t2[k]), (it, k)
