@@ -492,6 +492,10 @@ struct gecco_hip_plan {
     int64_t target_tasks = 2048;
     int64_t min_kchunk = 256;
     int64_t max_kchunk = 1024;      // longest contraction range of one strided-GEMM task
+    // order of the strided-GEMM launch's tasks: 0 longest first; 1 patches
+    // of tiles that share operand panels kept on one XCD (workgroup w runs
+    // on XCD w mod 8), patches spread over the XCDs by cost
+    int xcd_order = 1;
     int64_t nbufs_max = 0;
     int grid = 0;
     double flops = 0, payload = 0, index_bytes = 0;
@@ -853,6 +857,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     std::vector<DevTask> tasks;
     std::vector<DevBlock> dblocks(p->blocks.size());
     std::vector<int64_t> nsplit(p->blocks.size(), 1);
+    std::vector<int64_t> patch;     // patch id of every task
+    int64_t npatch = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
         const bool head = chain_head[i] == i;
@@ -892,6 +898,11 @@ static int plan_finalize(gecco_hip_plan *p) {
         d.atomic = (ns > 1 || !b.injective || overlap[i]) ? 1 : 0;
         d.next = b.next = chain_next[i];
         if (!head) continue;
+        // patches of PM x PN tiles with one contraction range share their
+        // A panels (PN times) and B panels (PM times); about 16 tiles each
+        const int64_t mt = (b.M + tile_m(i) - 1) / tile_m(i), nt = (b.N + tile_n(i) - 1) / tile_n(i);
+        const int64_t PN = std::min<int64_t>(nt, 4), PM = std::min<int64_t>(mt, std::max<int64_t>(1, 16 / PN));
+        const int64_t cm = (mt + PM - 1) / PM, cn = (nt + PN - 1) / PN;
         // a task's range [k0, k1) counts from the start of the chain block
         // it begins in
         for (int64_t m0 = 0; m0 < b.M; m0 += tile_m(i))
@@ -905,7 +916,9 @@ static int plan_finalize(gecco_hip_plan *p) {
                     }
                     tasks.push_back({(int32_t)j, (int32_t)m0, (int32_t)n0, (int32_t)(c0 - start),
                                      (int32_t)(c1 - start)});
+                    patch.push_back(npatch + (s * cm + m0 / tile_m(i) / PM) * cn + n0 / tile_n(i) / PN);
                 }
+        npatch += ns * cm * cn;
     }
     // Result range of every task.  Tasks of the direct kernel whose result
     // map is a table or not injective stage their tile in LDS when the range
@@ -926,7 +939,7 @@ static int plan_finalize(gecco_hip_plan *p) {
             // 16 x 16 wave-split-K task
             task.pad = b.direct ? dpack[i] : b.skinny ? 1 : 0;
             task.grp = b.group;
-            task.pad2 = 0;
+            task.pad2 = (int32_t)patch[t];   // until the launch order is set
             task.lo = 0;
             const int64_t m1 = std::min<int64_t>(b.M, task.m0 + tile_m(i));
             const int64_t n1 = std::min<int64_t>(b.N, task.n0 + tile_n(i));
@@ -964,6 +977,62 @@ static int plan_finalize(gecco_hip_plan *p) {
     });
     int64_t ncls[4] = {0, 0, 0, 0};
     for (auto &t : tasks) ncls[cls[t.blk]]++;
+    // XCD-grouped order of the strided-GEMM launch (one workgroup per task,
+    // dispatched round-robin over the 8 XCDs, each with its own L2): the
+    // patches go, longest task first, to the XCD with the least work so far
+    // (cost = contraction length), each XCD keeps its patches in that order
+    // with a patch's tasks together, and the launch takes the XCDs' lists in
+    // turn so that workgroup w gets the next task of XCD w mod 8.
+    if (p->xcd_order && ncls[0] > 0) {
+        constexpr int NX = 8;
+        const size_t n0 = (size_t)ncls[0];
+        std::vector<std::pair<int32_t, size_t>> byp(n0);   // (patch, position) in cost order
+        for (size_t t = 0; t < n0; ++t) byp[t] = {tasks[t].pad2, t};
+        std::stable_sort(byp.begin(), byp.end(), [&](const std::pair<int32_t, size_t> &x,
+                                                     const std::pair<int32_t, size_t> &y) {
+            return x.first < y.first;
+        });
+        struct Patch { size_t first, last; int64_t cost, top; };
+        std::vector<Patch> pts;
+        for (size_t a = 0; a < n0;) {
+            size_t e = a;
+            Patch q{a, a, 0, 0};
+            while (e < n0 && byp[e].first == byp[a].first) {
+                const int64_t c = cost(tasks[byp[e].second]) + 1;
+                q.cost += c;
+                q.top = std::max(q.top, c);
+                ++e;
+            }
+            q.last = e;
+            pts.push_back(q);
+            a = e;
+        }
+        std::stable_sort(pts.begin(), pts.end(), [](const Patch &x, const Patch &y) {
+            return x.top != y.top ? x.top > y.top : x.cost > y.cost;
+        });
+        std::vector<std::vector<size_t>> lists(NX);
+        int64_t load[NX] = {0};
+        for (auto &q : pts) {
+            int x = 0;
+            for (int y = 1; y < NX; ++y)
+                if (load[y] < load[x]) x = y;
+            load[x] += q.cost;
+            for (size_t e = q.first; e < q.last; ++e) lists[(size_t)x].push_back(byp[e].second);
+        }
+        std::vector<DevTask> ord;
+        ord.reserve(n0);
+        size_t at[NX] = {0};
+        for (size_t w = 0; w < n0; ++w) {
+            int x = (int)(w % NX);
+            if (at[x] == lists[(size_t)x].size()) {      // list used up: the longest remaining one
+                for (int y = 0; y < NX; ++y)
+                    if (lists[(size_t)y].size() - at[y] > lists[(size_t)x].size() - at[x]) x = y;
+            }
+            ord.push_back(tasks[lists[(size_t)x][at[x]++]]);
+        }
+        std::copy(ord.begin(), ord.end(), tasks.begin());
+    }
+    for (auto &t : tasks) t.pad2 = 0;
     p->flops_affine = p->flops_general = p->flops_direct = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         const double f = 2.0 * (double)p->blocks[i].M * (double)p->blocks[i].N * (double)p->blocks[i].K;
@@ -1490,6 +1559,7 @@ int gecco_hip_plan_create(gecco_hip_plan **plan) {
     *plan = new gecco_hip_plan();
     if (const char *e = getenv("GECCO_HIP_AFFINE_TILE")) (*plan)->affine_variant = atoi(e);
     if (const char *e = getenv("GECCO_HIP_TARGET_TASKS")) (*plan)->target_tasks = std::max(1L, atol(e));
+    if (const char *e = getenv("GECCO_HIP_XCD_ORDER")) (*plan)->xcd_order = atoi(e) != 0;
     if (const char *e = getenv("GECCO_HIP_MIN_KCHUNK")) (*plan)->min_kchunk = std::max(32L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MAX_KCHUNK")) (*plan)->max_kchunk = std::max(64L, atol(e));
     if (const char *e = getenv("GECCO_HIP_MATERIALIZE_MIN")) (*plan)->materialize_min = std::max(0L, atol(e));
