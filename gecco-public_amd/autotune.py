@@ -1,8 +1,9 @@
 """Autotuner for the grouped contraction launches.
 
 A plan's strided-GEMM launch is shaped by its tile variant (64x64x16,
-64x64x32, 64x128x16, 128x64x16, 128x128x16, and 64x64x16 / 64x64x32 with two
-K steps of loads in flight), the number of tasks the
+64x64x16 with each wave 16 rows across the tile, 64x64x32, 64x128x16,
+128x64x16, 128x128x16, and 64x64x16 / 64x64x32 with two K steps of loads in
+flight), the number of tasks the
 contraction ranges are split towards, and the longest contraction range of
 one task (gecco_hip_plan_set_tuning).  The best choice depends on the block
 shapes of the formula level and on the GPU, so the tuner builds the plan for
@@ -12,7 +13,7 @@ defaults); a candidate replaces the current best only if it is faster by more
 than `margin`.  No counterpart in the reference, whose batching constants are
 fixed input keywords.
 """
-AXES = (('variant', (1, 6, 7, 0, 2, 3, 4)),
+AXES = (('variant', (1, 9, 6, 7, 0, 2, 3, 4)),
         ('target_tasks', (2048, 1024, 4096, 8192)),
         ('max_kchunk', (1024, 512, 2048)))
 DEFAULT = {'variant': 1, 'target_tasks': 2048, 'max_kchunk': 1024}

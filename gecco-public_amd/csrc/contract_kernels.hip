@@ -629,13 +629,27 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 // lo = bf16(x - hi) stored k-contiguous in LDS, and the tile product as
 // lo.hi + hi.lo + hi.hi on v_mfma_f32_16x16x16_bf16 (float32 accumulation);
 // the short and thin tasks of the launch stay float32.
-template <int TBM, int TBN, int TBK, typename T, int PF = 1>
-__global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_contract_affine(
+// WM: waves along M (2: each wave a (TBM/2) x (TBN/2) quarter; 4: each wave
+// TBM/4 rows across the whole tile width, so that a wave whose rows all lie
+// beyond the block -- M = 100 in 64-row tiles: rows 112..127 -- skips its
+// MFMAs).
+// Waves per SIMD the register allocation aims at: 64 x 64 x 16 tiles with
+// one K step in flight fit 6 in float32 (78 registers) and 4 in float64 (the
+// float64 quarter-tile layout reaches 4 without the hint).
+template <int TBM, int TBN, int TBK, typename T, int PF, int WM>
+constexpr int AFFINE_WAVES = TBM * TBN >= 128 * 128 ? 2
+                           : !(TBM == 64 && TBN == 64 && TBK == 16 && PF == 1) ? 1
+                           : sizeof(T) == 4 ? 6 : WM == 4 ? 4 : 1;
+
+template <int TBM, int TBN, int TBK, typename T, int PF = 1, int WM = 2>
+__global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
     T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
     typedef typename Acc<T>::type AccT;
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
-    constexpr int TM = TBM / 32, TN = TBN / 32;
+    constexpr int WN = 4 / WM, RM = TBM / WM, RN = TBN / WN;   // rows and columns per wave
+    constexpr int TM = RM / 16, TN = RN / 16;
+    static_assert(WM * WN == 4 && TM >= 1 && TN >= 1, "wave layout");
     constexpr int SA = TBK * LDA, SB = TBK * LDB;
     // double-buffered A and B tiles; after the K loop the array holds half
     // of the result tile (epilogue), or the partial tiles of a skinny task;
@@ -647,7 +661,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
     const int lrow = lane >> 4, lcol = lane & 15;
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
@@ -689,7 +703,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         // kernel take 163 instead of 126 registers, 3 waves/SIMD instead of
         // 4, and the launch 19% slower)
         const bool wave_live = __builtin_amdgcn_readfirstlane(
-            (m0 + wm * (TBM / 2) < M && n0 + wn * (TBN / 2) < N) ? 1 : 0);
+            (m0 + wm * RM < M && n0 + wn * RN < N) ? 1 : 0);
         if constexpr (PF == 3) {
             static_assert(sizeof(T) == 4 && TBK == 16, "split-bfloat16 tiles: float lists, TBK 16");
             constexpr int LK = TBK + 4;                       // padded k row (bf16)
@@ -748,13 +762,13 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     v4s a_h[TM], a_l[TM], b_h[TN], b_l[TN];
 #pragma unroll
                     for (int i = 0; i < TM; ++i) {
-                        const int row = wm * (TBM / 2) + i * 16 + lcol;
+                        const int row = wm * RM + i * 16 + lcol;
                         a_h[i] = *reinterpret_cast<const v4s *>(ah + row * LK + kg);
                         a_l[i] = *reinterpret_cast<const v4s *>(al + row * LK + kg);
                     }
 #pragma unroll
                     for (int j = 0; j < TN; ++j) {
-                        const int row = wn * (TBN / 2) + j * 16 + lcol;
+                        const int row = wn * RN + j * 16 + lcol;
                         b_h[j] = *reinterpret_cast<const v4s *>(bh + row * LK + kg);
                         b_l[j] = *reinterpret_cast<const v4s *>(bl + row * LK + kg);
                     }
@@ -808,9 +822,9 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     const int kr = kq * 4 + lrow;
                     T a[TM], bb[TN];
 #pragma unroll
-                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
+                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * RM + i * 16 + lcol];
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
+                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * RN + j * 16 + lcol];
 #pragma unroll
                     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -887,9 +901,9 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
                     const int kr = kq * 4 + lrow;
                     T a[TM], bb[TN];
 #pragma unroll
-                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * (TBM / 2) + i * 16 + lcol];
+                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * RM + i * 16 + lcol];
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * (TBN / 2) + j * 16 + lcol];
+                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * RN + j * 16 + lcol];
 #pragma unroll
                     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -913,7 +927,7 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
         // The tile goes through LDS as a row-major [row][col] array (row
         // stride TBN + 1 against bank conflicts), one half of the rows --
-        // those of the waves with wm = h -- at a time; then each thread adds
+        // those of the waves whose rows lie in half h -- at a time; then each thread adds
         // elements n-fastest, so the result-map reads are coalesced.  Every
         // element is a non-returning atomic add, also when the task owns its
         // result elements (injective map, one K chunk, no overlapping
@@ -924,14 +938,14 @@ __global__ __launch_bounds__(NT, (TBM * TBN >= 128 * 128 ? 2 : 1)) void grouped_
         constexpr int HR = TBM / 2, LDE = TBN + 1;
         for (int h = 0; h < 2; ++h) {
             if (h) __syncthreads();   // the previous half has been read
-            if (wm == h) {
+            if ((wm * RM) / HR == h) {
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
                     for (int j = 0; j < TN; ++j)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            lds[(i * 16 + mrow<T>(r, lrow)) * LDE + wn * (TBN / 2) + j * 16 + lcol] =
+                            lds[(wm * RM - h * HR + i * 16 + mrow<T>(r, lrow)) * LDE + wn * RN + j * 16 + lcol] =
                                 acc[i][j][r];
             }
             __syncthreads();   // (h = 0: the K loop ended with a barrier)
@@ -1151,7 +1165,7 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
     static const int shapes[][3] = {{64, 64, 32}, {64, 64, 16}, {64, 128, 16},
                                     {128, 64, 16}, {128, 128, 16}, {128, 128, 8},
-                                    {64, 64, 16}, {64, 64, 32}, {64, 64, 16}};
+                                    {64, 64, 16}, {64, 64, 32}, {64, 64, 16}, {64, 64, 16}};
     if (v < 0 || v >= (int)(sizeof shapes / sizeof shapes[0])) return 1;
     *bm = shapes[v][0];
     *bn = shapes[v][1];
@@ -1174,6 +1188,8 @@ static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *ta
     // two K steps of loads in flight
     case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    // 64 x 64 x 16 with the waves stacked along M
+    case 9: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
     // split-bfloat16 GEMM tiles (float lists only)
     case 8:
         if constexpr (sizeof(T) == 4) {

@@ -214,8 +214,9 @@ int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
  * 2 64x128x16, 3 128x64x16, 4 128x128x16, 5 128x128x8, 6 64x64x16 and
  * 7 64x64x32 with the loads of two K steps in flight; 8, float32 plans
  * only: 64x64x16 GEMM tiles as split-bfloat16 three-term products x = hi + lo,
- * lo.hi + hi.lo + hi.hi on BF16 MFMA with float32 accumulation), the number
- * of tasks
+ * lo.hi + hi.lo + hi.hi on BF16 MFMA with float32 accumulation; 9 64x64x16
+ * with each wave 16 rows across the tile, so that waves beyond the block's
+ * rows skip their MFMAs), the number of tasks
  * the contraction ranges are split towards (default 2048), and the shortest
  * and longest contraction range of one task (defaults 256, 1024).  The
  * autotuner (gecco-public_amd/autotune.py) picks them per plan by timing.

@@ -351,7 +351,7 @@ def test_all_triples_terms_one_launch():
     assert _relerr(bufs[0].cpu().numpy(), ref) <= TOL
 
 
-@pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize('variant', [0, 1, 2, 3, 4, 5, 6, 7, 9])
 @pytest.mark.parametrize('name', ['ladder', 'hh_ladder', 'energy', 'ring'])
 def test_affine_tile_variants(variant, name, monkeypatch):
     """Every tile variant of the strided-GEMM kernel gives the oracle result."""
@@ -416,7 +416,7 @@ def test_materialised_gathers_share_jobs():
 
 
 @pytest.mark.parametrize('materialize,variant,chain', [(0, 1, 1), (1, 1, 1), (1, 0, 1), (1, 4, 1),
-                                                    (1, 1, 0)])
+                                                    (1, 1, 0), (1, 9, 1)])
 def test_ccsd_terms_match_oracle(materialize, variant, chain, monkeypatch):
     """Every binary contraction of the CCSD formula table, as kernel calls of
     the product enumerator, against the oracle's contr_op1op2 restatement.
