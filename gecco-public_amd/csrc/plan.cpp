@@ -732,8 +732,8 @@ static int plan_finalize(gecco_hip_plan *p) {
         b.mat_a = b.strided && !sa;
         b.mat_b = b.strided && !sb;
         // short contractions with a thin result side (or a very short
-        // contraction) run on the direct kernel; see
-        // grouped_contract_direct_f64
+        // contraction) run as direct tasks of the strided-GEMM launch
+        // (direct_task in contract_kernels.hip)
         b.direct = b.strided && b.K <= p->direct_kmax &&
                    (std::min(b.M, b.N) <= 32 || b.K <= 8);
         is_aff[i] = b.strided && !b.direct ? 1 : 0;
