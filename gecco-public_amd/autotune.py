@@ -40,6 +40,8 @@ def tune_inplace(plan, bufs, axes=AXES, start=None, reps=5, margin=0.01, log=Non
     with the best settings.  Returns (options, ms, table); log(text), if
     given, is called after every candidate."""
     best_opts = dict(DEFAULT if start is None else start)
+    if start is None and getattr(plan, '_elem', 8) == 4:
+        best_opts['variant'] = 9          # the float32 plans' default tile
     plan.retune(**best_opts)
     best_ms = time_plan(plan, bufs, reps)
     table = [(dict(best_opts), best_ms)]

@@ -488,7 +488,7 @@ struct gecco_hip_plan {
     int64_t nchained = 0;           // blocks linked behind a chain head
     int64_t nstaged = 0;            // tasks with an LDS-staged result range
     double gather_bytes = 0;
-    int affine_variant = 1;
+    int affine_variant = -1;        // tile variant; -1: 1 for float64 plans, 9 for float32
     int64_t target_tasks = 2048;
     int64_t min_kchunk = 256;
     int64_t max_kchunk = 1024;      // longest contraction range of one strided-GEMM task
@@ -810,6 +810,9 @@ static int plan_finalize(gecco_hip_plan *p) {
         for (size_t i = 0; i < p->blocks.size(); ++i) overlap[i] = overlap[chain_head[i]];
     }
     int abm = 64, abn = 64, abk = 32;
+    // float32 64 x 64 tiles with the waves stacked along M were as fast or
+    // faster on every CCSD level (the 100-row ladder blocks: 3.5%)
+    if (p->affine_variant < 0) p->affine_variant = p->elem == 4 ? 9 : 1;
     if (gecco_affine_variant_shape(p->affine_variant, &abm, &abn, &abk))
         return fail(GECCO_HIP_EINVAL, "unknown tile variant %d", p->affine_variant);
     if (p->affine_variant == 8 && p->elem != 4)

@@ -210,7 +210,8 @@ int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
 int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
 /* Launch shape of the strided-GEMM kernel (before finalize; a negative or
  * zero argument keeps the current value): tile variant (see
- * gecco_hip_plan_block_info's tile shapes: 0 64x64x32, 1 64x64x16 (default),
+ * gecco_hip_plan_block_info's tile shapes: 0 64x64x32, 1 64x64x16 (default
+ * for float64 plans; float32 plans default to 9),
  * 2 64x128x16, 3 128x64x16, 4 128x128x16, 5 128x128x8, 6 64x64x16 and
  * 7 64x64x32 with the loads of two K steps in flight; 8, float32 plans
  * only: 64x64x16 GEMM tiles as split-bfloat16 three-term products x = hi + lo,
