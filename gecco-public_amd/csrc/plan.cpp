@@ -677,6 +677,62 @@ static void plan_release_host(gecco_hip_plan *p) {
     p->keep_host = false;
 }
 
+// XCD-grouped launch order of n tasks given in cost order (longest first):
+// patch[t] = patch of task t (tiles that share operand panels), cost[t] its
+// length.  A workgroup w of a launch with one workgroup per task runs on XCD
+// w mod 8, each XCD with its own L2.  The patches go, longest task first, to
+// the XCD with the least work so far (cost + 1 per task); each XCD keeps its
+// patches in that order, a patch's tasks together and in their given order;
+// the launch takes the XCDs' lists in turn (position w: the next task of XCD
+// w mod 8, or of the list with most tasks left once that one is used up).
+// perm[w] = the task at position w.
+static void xcd_task_order(int64_t n, const int32_t *patch, const int64_t *cost, int64_t *perm) {
+    constexpr int NX = 8;
+    const size_t n0 = (size_t)n;
+    std::vector<std::pair<int32_t, size_t>> byp(n0);   // (patch, task) with tasks in cost order
+    for (size_t t = 0; t < n0; ++t) byp[t] = {patch[t], t};
+    std::stable_sort(byp.begin(), byp.end(), [](const std::pair<int32_t, size_t> &x,
+                                                 const std::pair<int32_t, size_t> &y) {
+        return x.first < y.first;
+    });
+    struct Patch { size_t first, last; int64_t cost, top; };
+    std::vector<Patch> pts;
+    for (size_t a = 0; a < n0;) {
+        size_t e = a;
+        Patch q{a, a, 0, 0};
+        while (e < n0 && byp[e].first == byp[a].first) {
+            const int64_t c = cost[byp[e].second] + 1;
+            q.cost += c;
+            q.top = std::max(q.top, c);
+            ++e;
+        }
+        q.last = e;
+        pts.push_back(q);
+        a = e;
+    }
+    std::stable_sort(pts.begin(), pts.end(), [](const Patch &x, const Patch &y) {
+        return x.top != y.top ? x.top > y.top : x.cost > y.cost;
+    });
+    std::vector<std::vector<size_t>> lists(NX);
+    int64_t load[NX] = {0};
+    for (auto &q : pts) {
+        int x = 0;
+        for (int y = 1; y < NX; ++y)
+            if (load[y] < load[x]) x = y;
+        load[x] += q.cost;
+        for (size_t e = q.first; e < q.last; ++e) lists[(size_t)x].push_back(byp[e].second);
+    }
+    size_t at[NX] = {0};
+    for (size_t w = 0; w < n0; ++w) {
+        int x = (int)(w % NX);
+        if (at[x] == lists[(size_t)x].size()) {
+            for (int y = 0; y < NX; ++y)
+                if (lists[(size_t)y].size() - at[y] > lists[(size_t)x].size() - at[x]) x = y;
+        }
+        perm[w] = (int64_t)lists[(size_t)x][at[x]++];
+    }
+}
+
 static int plan_finalize(gecco_hip_plan *p) {
     if (p->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
     // index pool
@@ -987,52 +1043,14 @@ static int plan_finalize(gecco_hip_plan *p) {
     // with a patch's tasks together, and the launch takes the XCDs' lists in
     // turn so that workgroup w gets the next task of XCD w mod 8.
     if (p->xcd_order && ncls[0] > 0) {
-        constexpr int NX = 8;
         const size_t n0 = (size_t)ncls[0];
-        std::vector<std::pair<int32_t, size_t>> byp(n0);   // (patch, position) in cost order
-        for (size_t t = 0; t < n0; ++t) byp[t] = {tasks[t].pad2, t};
-        std::stable_sort(byp.begin(), byp.end(), [&](const std::pair<int32_t, size_t> &x,
-                                                     const std::pair<int32_t, size_t> &y) {
-            return x.first < y.first;
-        });
-        struct Patch { size_t first, last; int64_t cost, top; };
-        std::vector<Patch> pts;
-        for (size_t a = 0; a < n0;) {
-            size_t e = a;
-            Patch q{a, a, 0, 0};
-            while (e < n0 && byp[e].first == byp[a].first) {
-                const int64_t c = cost(tasks[byp[e].second]) + 1;
-                q.cost += c;
-                q.top = std::max(q.top, c);
-                ++e;
-            }
-            q.last = e;
-            pts.push_back(q);
-            a = e;
-        }
-        std::stable_sort(pts.begin(), pts.end(), [](const Patch &x, const Patch &y) {
-            return x.top != y.top ? x.top > y.top : x.cost > y.cost;
-        });
-        std::vector<std::vector<size_t>> lists(NX);
-        int64_t load[NX] = {0};
-        for (auto &q : pts) {
-            int x = 0;
-            for (int y = 1; y < NX; ++y)
-                if (load[y] < load[x]) x = y;
-            load[x] += q.cost;
-            for (size_t e = q.first; e < q.last; ++e) lists[(size_t)x].push_back(byp[e].second);
-        }
-        std::vector<DevTask> ord;
-        ord.reserve(n0);
-        size_t at[NX] = {0};
-        for (size_t w = 0; w < n0; ++w) {
-            int x = (int)(w % NX);
-            if (at[x] == lists[(size_t)x].size()) {      // list used up: the longest remaining one
-                for (int y = 0; y < NX; ++y)
-                    if (lists[(size_t)y].size() - at[y] > lists[(size_t)x].size() - at[x]) x = y;
-            }
-            ord.push_back(tasks[lists[(size_t)x][at[x]++]]);
-        }
+        std::vector<int32_t> pid(n0);
+        std::vector<int64_t> tc(n0);
+        for (size_t t = 0; t < n0; ++t) { pid[t] = tasks[t].pad2; tc[t] = cost(tasks[t]); }
+        std::vector<int64_t> perm(n0);
+        xcd_task_order((int64_t)n0, pid.data(), tc.data(), perm.data());
+        std::vector<DevTask> ord(n0);
+        for (size_t w = 0; w < n0; ++w) ord[w] = tasks[(size_t)perm[w]];
         std::copy(ord.begin(), ord.end(), tasks.begin());
     }
     for (auto &t : tasks) t.pad2 = 0;
@@ -1848,5 +1866,12 @@ int gecco_hip_memcpy_dtoh(void *dst, const void *src, int64_t bytes) {
     return check(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 int gecco_hip_synchronize(void) { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int gecco_hip_task_order(int64_t n, const int32_t *patch, const int64_t *cost, int64_t *perm) {
+    g_err = 0;
+    if (n < 0 || (n > 0 && (!patch || !cost || !perm))) return fail(GECCO_HIP_EINVAL, "bad task order arguments");
+    xcd_task_order(n, patch, cost, perm);
+    return 0;
+}
 
 }  // extern "C"

@@ -71,13 +71,15 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_workspace', 'gecco_hip_plan_block_index',
              'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
              'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning',
-             'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune', 'gecco_hip_plan_set_precision')
+             'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune', 'gecco_hip_plan_set_precision',
+             'gecco_hip_task_order')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
 for _n in C_SYMBOLS[1:]:
     getattr(_lib, _n).restype = ctypes.c_int
 _lib.gecco_hip_plan_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+_lib.gecco_hip_task_order.argtypes = [I64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
 _lib.gecco_hip_plan_destroy.argtypes = [ctypes.c_void_p]
 _lib.gecco_hip_plan_add_block.argtypes = [ctypes.c_void_p, ctypes.POINTER(BlockArgs)]
 _lib.gecco_hip_plan_finalize.argtypes = [ctypes.c_void_p]
@@ -782,3 +784,17 @@ class DIISGroup:
         if h is not None and h.value:
             _lib.gecco_hip_diis_group_destroy(h)
             self._h = None
+
+
+def task_order(patch, cost):
+    """Launch order of the strided-GEMM launch's tasks (gecco_hip_task_order;
+    host only): patch and cost per task, tasks in cost order.  Returns the
+    task index at every launch position (int64 numpy array)."""
+    import numpy as np
+    patch = np.ascontiguousarray(patch, dtype=np.int32)
+    cost = np.ascontiguousarray(cost, dtype=np.int64)
+    if patch.shape != cost.shape or patch.ndim != 1:
+        raise ValueError('patch and cost must be 1-d arrays of one length')
+    perm = np.empty(len(patch), dtype=np.int64)
+    _check(_lib.gecco_hip_task_order(len(patch), patch.ctypes.data, cost.ctypes.data, perm.ctypes.data))
+    return perm

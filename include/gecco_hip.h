@@ -225,6 +225,17 @@ int gecco_hip_plan_set_group(gecco_hip_plan *plan, int32_t group);
  * cnt_block are input keywords, contract/contr_blk1blk2_blocked_mm.f:184-227). */
 int gecco_hip_plan_set_tuning(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
                               int64_t min_kchunk, int64_t max_kchunk);
+/* The launch order the plans give the strided-GEMM launch's tasks (host
+ * only, no device work): n tasks in cost order (longest first), patch[t] the
+ * patch of task t (up to 16 tiles of one chain and contraction range, which
+ * share operand panels), cost[t] its contraction length.  Workgroup w of
+ * the launch runs on XCD w mod 8.  Patches go, longest task first, to the XCD
+ * with the least work so far; an XCD's patches keep that order with their
+ * tasks together; perm[w] is the task at launch position w.  No counterpart
+ * in the reference (a serial loop over blocks); exported so that the order
+ * can be checked without a GPU.  GECCO_HIP_XCD_ORDER=0 turns the order off
+ * for plans created afterwards. */
+int gecco_hip_task_order(int64_t n, const int32_t *patch, const int64_t *cost, int64_t *perm);
 /* keep_host(1) before finalize: the plan keeps its host index data, so that
  * gecco_hip_plan_retune can finalize it again with other launch settings
  * (arguments as set_tuning) without recording it again; keep_host(0) after
