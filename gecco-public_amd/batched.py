@@ -167,6 +167,7 @@ class BatchedCCSD:
 
         n = len(self.inst)
         conv_at = [None] * n
+        singular_at = [None] * n
         energy = [None] * n
         resid = [None] * n
         events = []
@@ -202,6 +203,11 @@ class BatchedCCSD:
                         continue
                     g, e, code = d.status(base + done)
                     energy[k], resid[k] = e, g
+                    if code < 0 and singular_at[k] is None:
+                        # a singular subspace matrix stops this instance as
+                        # the reference stops (solve/optc_diis_extr.f:70);
+                        # its amplitudes stay, it is not converged
+                        singular_at[k] = done + 1
                     if code != 0:
                         conv_at[k] = done + 1
                 done += 1
@@ -213,7 +219,8 @@ class BatchedCCSD:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out = dict(energies=energy, iterations=[x if x is not None else it for x in conv_at],
-                   residuals=resid, converged=[x is not None for x in conv_at], seconds=wall,
+                   residuals=resid, seconds=wall, singular_at=singular_at,
+                   converged=[x is not None and s is None for x, s in zip(conv_at, singular_at)],
                    batch_iterations=done, flops_per_iter=self.flops, kernel_calls=self.nblocks)
         if timing:
             out['iteration_ms'] = [a.elapsed_time(b) for a, b in zip([ev0] + events[:-1], events)]

@@ -526,6 +526,12 @@ class CCSD:
                         on_iter(done, energy, rnorm)
                     if verbose:
                         print(f'  it {done:3d}  E = {energy:.12f}  |R| = {rnorm:.3e}', flush=True)
+                    if code < 0:
+                        # the reference stops on a singular subspace matrix
+                        # (solve/optc_diis_extr.f:70); so does the sync step
+                        torch.cuda.synchronize()
+                        raise gecco_hip.GeccoHipError(
+                            f'DIIS matrix is singular at iteration {done} (||g|| = {rnorm:.3e})')
                     if code != 0:
                         conv = True
                         break

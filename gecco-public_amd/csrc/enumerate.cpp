@@ -335,6 +335,7 @@ struct gecco_hip_list {
     std::map<std::vector<int>, int> find;  // (idxmsa, igama, imc, gc, ima, ga) -> dist
     std::map<std::pair<int, int>, int64_t> blen;  // (idxmsa, igama) -> length
     int64_t length = 0;
+    int tra = 0;   // adjoint view of a stored list (gecco_hip_list_adjoint)
 };
 
 namespace gecco {
@@ -602,6 +603,7 @@ static int emit(ContrCtx &c, double xfac, const Dist &d1, const Dist &d2, const 
     gecco_hip_block_args a;
     std::memset(&a, 0, sizeof a);
     a.xfac = xfac;
+    a.tra_op1 = c.L1->tra; a.tra_op2 = c.L2->tra; a.tra_op1op2 = c.L12->tra;
     a.ncblk_op1 = (int64_t)c.L1->hc.size(); a.nablk_op1 = (int64_t)c.L1->ha.size();
     a.ncblk_ex1 = (int64_t)n1c; a.nablk_ex1 = (int64_t)c.ex1a.occ.size();
     a.ncblk_op2 = (int64_t)c.L2->hc.size(); a.nablk_op2 = (int64_t)c.L2->ha.size();
@@ -1283,6 +1285,55 @@ int gecco_hip_list_create(gecco_hip_orbs *o, const int64_t *occ_c, const int64_t
     return gecco_hip_list_create_n(o, 1, occ, absym, out);
 }
 
+int gecco_hip_list_adjoint(const gecco_hip_list *L, gecco_hip_list **out) {
+    // C and A sides swap; every distribution keeps its stored offset and
+    // length, its component labels swap sides, and its lengths and leading
+    // dimensions follow the swapped component order.  The block labels
+    // (Ms(A), IRREP(A)) of the view equal the stored ones: Ms(C) = Ms(A) and
+    // IRREP(C) = IRREP(A) in every block of a list with Ms(total) = 0 and
+    // IRREP(total) = 1, and msa2idxms4op gives the view's block the stored
+    // block's index for either order of (occ_a, occ_c).
+    if (!L || !out) return set_error(GECCO_HIP_EINVAL, "list_adjoint: null argument");
+    if (L->njoined != 1) return set_error(GECCO_HIP_EINVAL, "list_adjoint: one-vertex lists only");
+    auto *A = new gecco_hip_list();
+    A->orbs = L->orbs;
+    for (int h = 0; h < 5; ++h) { A->occ_c[h] = L->occ_a[h]; A->occ_a[h] = L->occ_c[h]; }
+    A->njoined = 1;
+    A->vocc = L->vocc;
+    for (auto &v : A->vocc)
+        for (int h = 0; h < 5; ++h) std::swap(v[h], v[5 + h]);
+    A->absym = L->absym;
+    A->oc = L->oa; A->hc = L->ha; A->vc = L->va;
+    A->oa = L->oc; A->ha = L->hc; A->va = L->vc;
+    A->nc = L->na; A->na = L->nc;
+    const size_t ncs = L->oc.size();
+    for (const Dist &d : L->dists) {
+        Dist e;
+        e.msc = d.msa; e.msa = d.msc; e.gc = d.ga; e.ga = d.gc;
+        e.imc = d.ima; e.ima = d.imc;
+        e.lens.assign(d.lens.begin() + ncs, d.lens.end());
+        e.lens.insert(e.lens.end(), d.lens.begin(), d.lens.begin() + ncs);
+        e.ldc = d.lda; e.lda = d.ldc;
+        e.off = d.off; e.len = d.len;
+        A->dists.push_back(std::move(e));
+    }
+    for (const auto &kv : L->find) {
+        const std::vector<int> &k = kv.first;    // idxmsa, igama, imc, gc, ima, ga
+        const size_t nca = L->oa.size();
+        std::vector<int> key{k[0], k[1]};
+        key.insert(key.end(), k.begin() + 2 + 2 * ncs, k.begin() + 2 + 2 * ncs + nca);
+        key.insert(key.end(), k.begin() + 2 + 2 * ncs + nca, k.end());
+        key.insert(key.end(), k.begin() + 2, k.begin() + 2 + ncs);
+        key.insert(key.end(), k.begin() + 2 + ncs, k.begin() + 2 + 2 * ncs);
+        A->find[key] = kv.second;
+    }
+    A->blen = L->blen;
+    A->length = L->length;
+    A->tra = !L->tra;
+    *out = A;
+    return 0;
+}
+
 int gecco_hip_list_destroy(gecco_hip_list *L) {
     delete L;
     return 0;
@@ -1470,6 +1521,7 @@ int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
     if (!plan || !op || !trop || !cnt) return set_error(GECCO_HIP_EINVAL, "add_trace: null argument");
     if (op->orbs != trop->orbs) return set_error(GECCO_HIP_EINVAL, "add_trace: lists of different orbital sets");
     if (op->njoined > 1 || trop->njoined > 1) return set_error(GECCO_HIP_EINVAL, "add_trace: one-vertex lists only");
+    if (op->tra || trop->tra) return set_error(GECCO_HIP_EINVAL, "add_trace: stored lists only, no adjoint views");
     int c5[5] = {0};
     for (int h = 1; h <= 4; ++h) {
         c5[h] = (int)cnt[h - 1];
@@ -1493,6 +1545,7 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
     if (!plan || !ori || !reo || !shift_c || !shift_a)
         return set_error(GECCO_HIP_EINVAL, "add_reorder: null argument");
     if (ori->orbs != reo->orbs) return set_error(GECCO_HIP_EINVAL, "add_reorder: lists of different orbital sets");
+    if (ori->tra || reo->tra) return set_error(GECCO_HIP_EINVAL, "add_reorder: stored lists only, no adjoint views");
     if (from_vtx < 0 || from_vtx >= ori->njoined || to_vtx < 0 || to_vtx >= reo->njoined)
         return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex out of range");
     if (sign_reo != 1 && sign_reo != -1) return set_error(GECCO_HIP_EINVAL, "add_reorder: sign must be +1 or -1");
@@ -1514,6 +1567,7 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
 int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *L, int32_t buf, int64_t off,
                               double prefac, double relfac) {
     if (!plan || !L) return set_error(GECCO_HIP_EINVAL, "add_sym_ab: null argument");
+    if (L->tra) return set_error(GECCO_HIP_EINVAL, "add_sym_ab: give the stored list, not its adjoint view");
     std::vector<int64_t> i1, i2;
     std::vector<int8_t> f;
     int rc = sym_pairs(L, off, i1, i2, f, relfac);

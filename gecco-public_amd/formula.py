@@ -30,8 +30,10 @@ and command_del_intermediate have no work left to do).
 Schedule: items run in levels; the items of a level are independent and
 run as one grouped launch.  A CONTR item goes into the first level after
 every earlier (in list order) write of its operands -- all contributions to
-a list commute, so several writers share a level -- and not before an
-earlier item that reads its target.  An INIT goes after the earlier reads
+a list commute, so several writers share a level -- and into a level after
+every earlier item that reads its target (the reader must see the target
+as it was before this contribution; a level's contributions are added while
+its other items read).  An INIT goes after the earlier reads
 and writes of its target and before the later ones.  The spin-flip
 projection is linear and idempotent and the initial contents of every list
 are fixed points of it, so contributions of several levels are projected
@@ -110,8 +112,9 @@ def schedule(items):
     A contraction reads its operands as they are at the start of its level,
     so with items taken in list order:
       read after write   an operand's earlier writes are in earlier levels;
-      write after read   the target's earlier readers are in the same or
-                         earlier levels;
+      write after read   the target's earlier readers are in earlier
+                         levels (the writer adds into the target while the
+                         readers of its own level run);
       after an INIT      the target's last INIT is in the same or an earlier
                          level.
     Contributions to one list commute and need no order among themselves.
@@ -134,7 +137,7 @@ def schedule(items):
                 last_read[src] = max(last_read.get(src, -1), lv)
             continue
         lv = max(last_write.get(it['op1'], -1) + 1, last_write.get(it['op2'], -1) + 1,
-                 last_read.get(t, 0), last_init.get(t, 0))
+                 last_read.get(t, -1) + 1, last_init.get(t, 0))
         contrs.setdefault(lv, []).append(it)
         last_write[t] = max(last_write.get(t, -1), lv)
         for op in (it['op1'], it['op2']):

@@ -539,6 +539,22 @@ class MEList:
                                             out.ctypes.data_as(PD), out.size))
         return out[:self.length]
 
+    def adjoint(self):
+        """Adjoint view X^+ of this one-vertex list (gecco_hip_list_adjoint):
+        C and A swapped, the same storage; a contraction that takes the view
+        runs its kernel calls with the transpose flag (the vertex's dagger)."""
+        v = MEList.__new__(MEList)
+        v.orbs = self.orbs
+        v.vocc_c, v.vocc_a = [dict(x) for x in self.vocc_a], [dict(x) for x in self.vocc_c]
+        v.njoined = self.njoined
+        v.occ_c, v.occ_a = dict(self.occ_a), dict(self.occ_c)
+        v.absym = self.absym
+        v._h = VP()
+        _check(_lib.gecco_hip_list_adjoint(self._h, ctypes.byref(v._h)))
+        v.length = _lib.gecco_hip_list_length(v._h)
+        v.base = self        # keeps the stored list alive
+        return v
+
     def comp_spaces(self):
         """Space (1=H, 2=P) of each electron column of spinorbs()."""
         from itertools import chain

@@ -12,7 +12,8 @@
  *
  * Scope: one-vertex operators (njoined = 1) over hole (H, hpvx 1) and
  * particle (P, hpvx 2) spaces with one orbital subspace each, Ms(total) 0,
- * totally symmetric (gamt 1), as in closed-shell CC.
+ * totally symmetric (gamt 1), as in closed-shell CC; operands and results
+ * may be adjoint views (gecco_hip_list_adjoint).
  */
 #ifndef GECCO_HIP_MODEL_H
 #define GECCO_HIP_MODEL_H
@@ -57,6 +58,17 @@ int gecco_hip_list_create(gecco_hip_orbs *orbs, const int64_t *occ_c, const int6
  * reorders (gecco_hip_plan_add_reorder) take any. */
 int gecco_hip_list_create_n(gecco_hip_orbs *orbs, int64_t njoined, const int64_t *occ,
                             int64_t absym, gecco_hip_list **out);
+/* Adjoint view X^+ of a one-vertex list X, for a contraction operand or
+ * result used with the dagger flag (contract/get_bc_info3.f:173,191,391:
+ * tra_op1 / tra_op2 / tra_op1op2 = vertex%dagger, passed to the kernels by
+ * contract/contr_op1op2_wmaps_c.f:844-855,1903,1931).  The view's C side is
+ * X's A side and vice versa; element (c; a) of the view is X's stored
+ * element (a; c), addressed through the transposed leading dimensions of
+ * contract/set_op_ldim_c.f:51-70 with no sign.  The view shares X's layout
+ * (offsets, length): give gecco_hip_plan_add_contraction the view and the
+ * buffer slot of X; spin-flip symmetrisation, traces and reorders take the
+ * stored list.  Destroy the view with gecco_hip_list_destroy. */
+int gecco_hip_list_adjoint(const gecco_hip_list *list, gecco_hip_list **out);
 int gecco_hip_list_destroy(gecco_hip_list *list);
 int64_t gecco_hip_list_length(const gecco_hip_list *list);
 int64_t gecco_hip_list_ndist(const gecco_hip_list *list);

@@ -35,7 +35,47 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 
+_BLAS = None
+
+
+def blas():
+    """numpy's bundled OpenBLAS (ILP64, symbols with a 64_ suffix), or None:
+    (library handle, address of its Fortran-interface dgemm)."""
+    global _BLAS
+    if _BLAS is None:
+        import glob
+        libs = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), 'numpy.libs')
+        found = sorted(glob.glob(os.path.join(libs, 'libscipy_openblas64_*.so')))
+        _BLAS = (None, None)
+        if found:
+            h = ctypes.CDLL(found[0])
+            if hasattr(h, 'scipy_dgemm_64_'):
+                _BLAS = (h, ctypes.cast(h.scipy_dgemm_64_, ctypes.c_void_p).value)
+    return _BLAS if _BLAS[0] is not None else None
+
+
+def blas_threads(n):
+    """Thread count of the bundled OpenBLAS (affects numpy too)."""
+    b = blas()
+    if b is not None and hasattr(b[0], 'scipy_openblas_set_num_threads64_'):
+        b[0].scipy_openblas_set_num_threads64_(ctypes.c_int(int(n)))
+
+
+_USE_BLAS = True
+
+
+def use_blas(on):
+    """Route 3's dgemm from the bundled ILP64 OpenBLAS (on, the default when
+    present; the reference links a BLAS dgemm) or the plain loop (off)."""
+    global _USE_BLAS
+    _USE_BLAS = bool(on)
+    if _LIB is not None:
+        b = blas() if _USE_BLAS else None
+        _LIB.orc_set_dgemm(b[1] if b is not None else None)
+
+
 def lib():
+    """The C restatement (see use_blas for its dgemm)."""
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, '_build', 'libgecco_oracle.so')
@@ -44,6 +84,8 @@ def lib():
             subprocess.check_call(['make', '-s', '-C', _HERE])
         _LIB = ctypes.CDLL(path)
         _LIB.orc_contr_blk1blk2_blocked_mm.restype = ctypes.c_int
+        _LIB.orc_set_dgemm.argtypes = [ctypes.c_void_p]
+        use_blas(_USE_BLAS)
     return _LIB
 
 
@@ -86,6 +128,8 @@ class Contraction:
 
     cnt_c: occupation (hpvx -> n) of the contracted part on Op1's C side
            (it meets Op2's A side); cnt_a: on Op1's A side (meets Op2's C).
+    An operand may be an oracle.melist.Adjoint (the vertex's dagger flag):
+    its kernel calls then carry tra_op1 / tra_op2 / tra_op1op2.
     Operand roles follow contract/contr_op1op2_wmaps_c.f:1541-1551,1657-1668:
       op1 C = CNT C + EX1 C, op1 A = CNT A + EX1 A,
       op2 C = CNT A + EX2 C, op2 A = CNT C + EX2 A,
@@ -302,6 +346,9 @@ class Contraction:
                     map2a = self._strmap_blk(op2.ha, self.mi_2a, s_cc, s_ex2a)
                     out.append(dict(
                         xfac=self.fact * fac, fac_scal=fac,
+                        tra_op1=int(getattr(op1, 'transposed', False)),
+                        tra_op2=int(getattr(op2, 'transposed', False)),
+                        tra_op1op2=int(getattr(res, 'transposed', False)),
                         off1=d1['off'], len1=d1['len'], off2=d2['off'], len2=d2['len'],
                         off12=dres['off'], len12=dres['len'],
                         ncblk_op1=len(op1.hc), nablk_op1=len(op1.ha),
@@ -339,8 +386,8 @@ class Contraction:
             call_kernel(L, b, x1, x2, xr, route, lenblock, lenblock_cnt, lenscr)
         if use_tr:
             use_tr_here = self.op1.absym * self.op2.absym == self.res.absym != 0
-            if use_tr_here:
-                sym_ab(self.res, xr, 0.5, float(self.res.absym))
+            if use_tr_here:     # the projection of the stored list
+                sym_ab(getattr(self.res, 'base', self.res), xr, 0.5, float(self.res.absym))
         return ncall
 
     def flops(self, use_tr=True):
@@ -380,6 +427,7 @@ def call_kernel(L, b, x1, x2, xr, route, lenblock=200, lenblock_cnt=-1, lenscr=N
     p2 = ctypes.c_void_p(x2.ctypes.data + 8 * b['off2'])
     p12 = ctypes.c_void_p(xr.ctypes.data + 8 * b['off12'])
     I = ctypes.c_int64
+    tra = [I(int(b.get(k, 0))) for k in ('tra_op1', 'tra_op2', 'tra_op1op2')]
     if route == 3:
         nex1 = int(np.prod(b['lstr_ex1']))
         nex2 = int(np.prod(b['lstr_ex2']))
@@ -389,14 +437,14 @@ def call_kernel(L, b, x1, x2, xr, route, lenblock=200, lenblock_cnt=-1, lenscr=N
                 lenblock * ncnt + 16
         scr = np.zeros(lenscr)
         rc = L.orc_contr_blk1blk2_blocked_mm(
-            ctypes.c_double(b['xfac']), p12, p1, p2, I(0), I(0), I(0),
+            ctypes.c_double(b['xfac']), p12, p1, p2, *tra,
             _ptr(scr), I(lenscr), I(lenblock), I(lenblock_cnt),
             *[I(d) for d in dims], *[_ptr(a) for a in arrs])
         if rc != 0:
             raise RuntimeError('contr_blk1blk2_blocked_mm: insufficient scratch')
     else:
         L.orc_contr_blk1blk2_wmaps_c(
-            ctypes.c_double(b['xfac']), p12, p1, p2, I(0), I(0), I(0),
+            ctypes.c_double(b['xfac']), p12, p1, p2, *tra,
             *[I(d) for d in dims], *[_ptr(a) for a in arrs])
 
 

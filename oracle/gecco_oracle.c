@@ -17,7 +17,8 @@
  *   orc_collect_block_impr         contract/collect_block_impr.f:2-415
  *   orc_scatter_block              contract/scatter_block.f:2-174
  *   orc_contr_blk1blk2_blocked_mm  contract/contr_blk1blk2_blocked_mm.f:2-605
- *                                  (dgemm('t','n') restated as a plain loop)
+ *                                  (dgemm('t','n') from an ILP64 BLAS set with
+ *                                  orc_set_dgemm, else a plain loop)
  *   orc_contr_blk1blk2_wmaps_c     contract/contr_blk1blk2_wmaps_c.f:2-413
  *
  * Parity status: pinned.  tests/test_oracle_contract.py checks route 3 against
@@ -365,9 +366,26 @@ void orc_scatter_block(
     }
 }
 
-/* dgemm('t','n',m,n,k,alpha,A,lda,B,ldb,0,C,ldc) as a plain loop */
+/* A BLAS dgemm with 8-byte integers (the reference links an ILP64 BLAS,
+ * config/x86_64-linux.conf; contr_blk1blk2_blocked_mm.f:465,523 call
+ * dgemm('t','n',...)), given by the caller as a function pointer; NULL keeps
+ * the plain loop below. */
+typedef void (*dgemm64_fn)(const char *, const char *, const I *, const I *, const I *,
+                           const double *, const double *, const I *, const double *,
+                           const I *, const double *, double *, const I *, size_t, size_t);
+static dgemm64_fn blas_dgemm = 0;
+
+void orc_set_dgemm(void *fn) { blas_dgemm = (dgemm64_fn)fn; }
+
+/* dgemm('t','n',m,n,k,alpha,A,lda,B,ldb,0,C,ldc): the BLAS routine when one
+ * is set, else a plain loop */
 static void ref_dgemm_tn(I m, I n, I k, double alpha, const double *A, I lda,
                          const double *B, I ldb, double *C, I ldc) {
+    if (blas_dgemm) {
+        const double beta = 0.0;
+        blas_dgemm("T", "N", &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc, 1, 1);
+        return;
+    }
     for (I j = 0; j < n; ++j)
         for (I i = 0; i < m; ++i) {
             double s = 0.0;

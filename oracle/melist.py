@@ -271,3 +271,43 @@ class MEList:
     def block_len(self, msa_tot, igama):
         idxmsa = msa2idxms4op(msa_tot, 0, self.na, self.nc)
         return self.gam_ms.get((igama, idxmsa), (0, 0))[1]
+
+
+class Adjoint:
+    """The adjoint X^+ of a one-vertex list X, as a contraction uses an
+    operand whose vertex carries the dagger flag (contract/get_bc_info3.f:173,
+    191: tra_op = vertex%dagger; the result with contr%dagger, :391).
+
+    Its C side is X's A side and its A side X's C side.  Element (c; a) of X^+
+    is the stored element (a; c) of X: the reference reaches it through the
+    transposed leading dimensions of contract/set_op_ldim_c.f:51-70 and the
+    C/A-swapped distribution lookup of operators/idx_msgmdst2.f (called with
+    the transpose flag, contract/contr_op1op2_wmaps_c.f:1644,1759), with no
+    sign.  With Ms(total) = 0 and a totally symmetric X, Ms(C) = Ms(A) and
+    IRREP(C) = IRREP(A), so block (Ms(A), IRREP(A)) of X^+ is X's block with
+    the same labels (contract/contr_op1op2_wmaps_c.f:427-429,1127-1146 flip
+    mst and take IRREP(C) in that case)."""
+
+    transposed = True
+
+    def __init__(self, mel):
+        if mel.njoined != 1:
+            raise ValueError('Adjoint: one-vertex lists only')
+        self.base = mel
+        self.orbs = mel.orbs
+        self.njoined = 1
+        self.occ_c, self.occ_a = dict(mel.occ_a), dict(mel.occ_c)
+        self.oc, self.hc, self.oa, self.ha = list(mel.oa), list(mel.ha), list(mel.oc), list(mel.hc)
+        self.nc, self.na = mel.na, mel.nc
+        self.absym = mel.absym
+        self.length = mel.length
+
+    def find(self, msa_tot, igama, imc, gc, ima, ga):
+        d = self.base.find(msa_tot, igama, ima, ga, imc, gc)
+        if d is None:
+            return None
+        nc = len(self.base.oc)
+        return dict(off=d['off'], len=d['len'], lens=d['lens'][nc:] + d['lens'][:nc])
+
+    def block_len(self, msa_tot, igama):
+        return self.base.block_len(msa_tot, igama)

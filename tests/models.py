@@ -117,3 +117,82 @@ def t3_ovoo_t2(orbs):
 
 T3_CASES = {'t3_pp_ladder': t3_pp_ladder, 't3_hh_ladder': t3_hh_ladder, 't3_ring': t3_ring,
             't3_vvvo_t2': t3_vvvo_t2, 't3_ovoo_t2': t3_ovoo_t2}
+
+
+# Contractions with transposed operands: an operand used as the adjoint of
+# its stored list (the vertex's dagger flag; tra_op1 / tra_op2 / tra_op1op2
+# in the kernel calls).  Each returns the contraction, the stored lists
+# (op1, op2, result) and which of them are used as adjoints.
+def energy_t1(orbs):
+    """E += sum <ij||ab> t_ij^ab with <ij||ab> read as the adjoint of the
+    stored <ab||ij> list (C = PP, A = HH)."""
+    from oracle.melist import Adjoint
+    V = MEList(orbs, {P: 2}, {H: 2})
+    T = MEList(orbs, {P: 2}, {H: 2})
+    E = MEList(orbs, {}, {})
+    return Contraction(Adjoint(V), T, E, {H: 2}, {P: 2}), (V, T, E), (True, False, False)
+
+
+def ring_t2(orbs):
+    """R(ab,ij) += sum_kc T(ac,ik) W^+(kb,cj), W stored with C = PH, A = PH."""
+    from oracle.melist import Adjoint
+    T = MEList(orbs, {P: 2}, {H: 2})
+    W = MEList(orbs, {H: 1, P: 1}, {H: 1, P: 1})
+    R = MEList(orbs, {P: 2}, {H: 2})
+    return Contraction(T, Adjoint(W), R, {P: 1}, {H: 1}), (T, W, R), (False, True, False)
+
+
+def t1_ovoo_t2(orbs):
+    """R(ab,ij) += T1 . Hooov^+ (C = PH, A = HH used; stored C = HH,
+    A = PH): creator and annihilator occupations of different type."""
+    from oracle.melist import Adjoint
+    T1 = MEList(orbs, {P: 1}, {H: 1})
+    V = MEList(orbs, {H: 2}, {P: 1, H: 1})
+    R = MEList(orbs, {P: 2}, {H: 2})
+    return Contraction(T1, Adjoint(V), R, {}, {H: 1}), (T1, V, R), (False, True, False)
+
+
+def ladder_t12(orbs):
+    """R^+ += V . T: the pp ladder written into the adjoint of a list stored
+    with C = HH, A = PP."""
+    from oracle.melist import Adjoint
+    V = MEList(orbs, {P: 2}, {P: 2})
+    T = MEList(orbs, {P: 2}, {H: 2})
+    R = MEList(orbs, {H: 2}, {P: 2})
+    return Contraction(V, T, Adjoint(R), {}, {P: 2}), (V, T, R), (False, False, True)
+
+
+def fock_t12(orbs):
+    """R^+ += T . F^+ with F stored C = P, A = P and R stored C = HH, A = PP:
+    transposed op2 and result in one kernel call."""
+    from oracle.melist import Adjoint
+    T = MEList(orbs, {P: 2}, {H: 2})
+    F = MEList(orbs, {P: 1}, {P: 1})
+    R = MEList(orbs, {H: 2}, {P: 2})
+    return Contraction(T, Adjoint(F), Adjoint(R), {P: 1}, {}), (T, F, R), (False, True, True)
+
+
+def vvvo_t1(orbs):
+    """R1(a,i) += 1/2 sum_{m,e<f} <am||ef> t_im^ef with <am||ef> read as the
+    adjoint of a list stored with C = PP, A = PH (<ef||am>)."""
+    from oracle.melist import Adjoint
+    V = MEList(orbs, {P: 2}, {P: 1, H: 1})
+    T = MEList(orbs, {P: 2}, {H: 2})
+    R = MEList(orbs, {P: 1}, {H: 1})
+    return Contraction(Adjoint(V), T, R, {H: 1}, {P: 2}), (V, T, R), (True, False, False)
+
+
+TRA_CASES = {'energy_t1': energy_t1, 'ring_t2': ring_t2, 't1_ovoo_t2': t1_ovoo_t2,
+             'ladder_t12': ladder_t12, 'fock_t12': fock_t12, 'vvvo_t1': vvvo_t1}
+
+
+def transposed_data(mel, x):
+    """The list of mel's adjoint occupation (C and A swapped) holding the
+    elements of x transposed, through the dense antisymmetric form: the
+    explicit counterpart of reading x through an Adjoint view."""
+    from oracle.contract import to_dense, from_dense
+    T = MEList(mel.orbs, mel.occ_a, mel.occ_c, mel.absym)
+    D = to_dense(mel, x)
+    nc = sum(mel.oc)
+    D = D.transpose(list(range(nc, D.ndim)) + list(range(nc)))
+    return T, from_dense(T, D)

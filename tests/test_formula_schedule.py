@@ -43,22 +43,81 @@ def test_dependency_rules():
     items = [F.init('X'), F.init('Y'), F.init('Z'),
              c('X', 'A', 'B', {}, {}, 1.0),       # level 0
              c('Y', 'X', 'B', {}, {}, 1.0),       # reads X: level 1
-             c('X', 'A', 'A', {}, {}, 1.0),       # writes X after a read of X: level 1 (not 0)
+             c('X', 'A', 'A', {}, {}, 1.0),       # writes X after a read of X: level 2
              c('Z', 'A', 'B', {}, {}, 1.0),       # independent: level 0
              c('Z', 'Y', 'A', {}, {}, 1.0),       # reads Y: level 2
-             c('Y', 'A', 'A', {}, {}, 1.0),       # writes Y after its read: level 2
+             c('Y', 'A', 'A', {}, {}, 1.0),       # writes Y after its read: level 3
              F.init('W', 'Z', 2.0),               # after Z's writes: level 3
              c('W', 'A', 'A', {}, {}, 1.0)]       # after W's init: level 3
     lv = F.schedule(items)
     assert _names(lv) == [[('X', 'A', 'B'), ('Z', 'A', 'B')],
-                          [('Y', 'X', 'B'), ('X', 'A', 'A')],
-                          [('Z', 'Y', 'A'), ('Y', 'A', 'A')],
-                          [('W', 'A', 'A')]]
+                          [('Y', 'X', 'B')],
+                          [('X', 'A', 'A'), ('Z', 'Y', 'A')],
+                          [('Y', 'A', 'A'), ('W', 'A', 'A')]]
     assert [it['target'] for it in lv[0]['init']] == ['X', 'Y', 'Z']
     assert [it['target'] for it in lv[3]['init']] == ['W']
-    # X: written in 0 and 1, read in 1 -> projected after 0 and after 1
-    assert lv[0]['sym'] == ['X'] and lv[1]['sym'] == ['Y', 'X']
-    assert lv[2]['sym'] == ['Z', 'Y'] and lv[3]['sym'] == ['W']
+    # X: written in 0 and 2, read in 1 -> projected after 0 and after 2
+    assert lv[0]['sym'] == ['X'] and lv[1]['sym'] == ['Y']
+    assert lv[2]['sym'] == ['X', 'Z'] and lv[3]['sym'] == ['Y', 'W']
+
+
+def _run_sequential(items, lists):
+    x = {k: v.copy() for k, v in lists.items()}
+    for it in items:
+        if it['cmd'] == 'INIT':
+            x[it['target']] = it['factor'] * x[it['source']] if 'source' in it else 0.0 * x[it['target']]
+        elif it['cmd'] == 'CONTR':
+            x[it['target']] = x[it['target']] + it['factor'] * x[it['op1']] * x[it['op2']]
+    return x
+
+
+def _run_levels(levels, lists):
+    """A level's INITs run in order, then its contractions as one grouped
+    launch: every task reads its operands while the others add into their
+    targets, so no list may be both read and written in one level."""
+    x = {k: v.copy() for k, v in lists.items()}
+    for lv in levels:
+        for it in lv['init']:
+            x[it['target']] = it['factor'] * x[it['source']] if 'source' in it else 0.0 * x[it['target']]
+        reads = {c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']}
+        writes = {c['target'] for c in lv['contr']}
+        assert not reads & writes, (reads & writes, lv)
+        add = {}
+        for c in lv['contr']:
+            add[c['target']] = add.get(c['target'], 0.0) + c['factor'] * x[c['op1']] * x[c['op2']]
+        for k, v in add.items():
+            x[k] = x[k] + v
+    return x
+
+
+def test_scheduled_levels_equal_the_sequential_formula():
+    """Random formulas over a few lists (bilinear items as elementwise
+    products): executing the scheduled levels -- a level's contractions
+    grouped, reading the lists as they stand at the level's start -- gives
+    the item-by-item result, and no level reads a list it writes.  The
+    earlier rule that let a writer share a level with an earlier reader of
+    its target fails this test."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    names = ['A', 'B', 'X', 'Y', 'Z', 'W']
+    for trial in range(300):
+        items = []
+        for _ in range(int(rng.integers(3, 14))):
+            t = names[2 + int(rng.integers(0, 4))]
+            if rng.random() < 0.2:
+                src = names[int(rng.integers(0, 6))]
+                items.append(F.init(t, src, float(rng.normal())) if src != t and rng.random() < 0.6
+                             else F.init(t))
+            else:
+                o1, o2 = (names[int(i)] for i in rng.integers(0, 6, 2))
+                if t in (o1, o2):
+                    continue
+                items.append(F.contr(t, o1, o2, {}, {}, float(rng.normal())))
+        lists = {n: rng.normal(size=3) for n in names}
+        seq = _run_sequential(items, lists)
+        lev = _run_levels(F.schedule(items), lists)
+        for n in names:
+            assert np.allclose(seq[n], lev[n], rtol=1e-12, atol=1e-12), (trial, n, items)
 
 
 def test_formula_files(tmp_path):

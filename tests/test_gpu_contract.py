@@ -540,3 +540,59 @@ def test_reorder_matches_oracle(name, nsym):
     p.execute(bufs)
     torch.cuda.synchronize()
     assert _relerr(bufs[1].cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 6, 10)])
+@pytest.mark.parametrize('name', ['energy_t1', 'ring_t2', 't1_ovoo_t2', 'ladder_t12', 'fock_t12',
+                                  'vvvo_t1'])
+@pytest.mark.parametrize('use_tr', [True, False])
+def test_transposed_operands_match_oracle(nsym, nocc, nvir, name, use_tr):
+    """Kernel calls with tra_op1 / tra_op2 / tra_op1op2 set (operands and
+    results used as adjoints, contract/contr_op1op2_wmaps_c.f:844-855): the
+    oracle's calls recorded one by one into a plan, the product enumerator
+    given adjoint views (gecco_hip_list_adjoint), and the Fortran-signature
+    entry point contr_blk1blk2_blocked_mm_ on device pointers, each against
+    the oracle."""
+    torch = _torch()
+    import gecco_hip
+    from models import TRA_CASES, round_robin
+    print(f'transposed case {name} nsym={nsym} nocc={nocc} nvir={nvir} use_tr={use_tr}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    c, stored, tra = TRA_CASES[name](orbs)
+    xs = [spin_symmetric(m, 91 + k) for k, m in enumerate(stored)]
+    blocks = c.blocks(use_tr)
+    assert blocks and all(bl['tra_op1'] == tra[0] and bl['tra_op2'] == tra[1] and
+                          bl['tra_op1op2'] == tra[2] for bl in blocks)
+    dev = torch.device('cuda:0')
+    d1, d2 = (torch.from_numpy(x).to(dev) for x in xs[:2])
+    # the oracle's kernel calls, one plan
+    ref = _oracle(c, xs[0], xs[1], stored[2].length, blocks)
+    dr = torch.zeros(stored[2].length, dtype=torch.float64, device=dev)
+    plan = gecco_hip.Plan()
+    for bl in blocks:
+        plan.add(bl)
+    plan.finalize()
+    plan.execute([d1, d2, dr])
+    torch.cuda.synchronize()
+    assert _relerr(dr.cpu().numpy(), ref) <= TOL
+    # the Fortran-signature kernel entry, block by block on device memory
+    dr.zero_()
+    for bl in blocks:
+        gecco_hip.contr_blk1blk2_blocked_mm(bl, d1[bl['off1']:], d2[bl['off2']:], dr[bl['off12']:])
+    torch.cuda.synchronize()
+    assert _relerr(dr.cpu().numpy(), ref) <= TOL
+    # the product's enumerator on adjoint views, with the projection
+    full = xs[2].copy()
+    c.run(xs[0], xs[1], full, route=3, use_tr=use_tr, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    lists = [gecco_hip.MEList(o, m.occ_c, m.occ_a) for m in stored]
+    used = [L.adjoint() if t else L for L, t in zip(lists, tra)]
+    plan = gecco_hip.Plan()
+    plan.add_contraction(used[0], used[1], used[2], c.cnt_c, c.cnt_a, 1.0, use_tr)
+    if use_tr:
+        plan.add_sym_ab(lists[2], 2)
+    plan.finalize()
+    dr = torch.from_numpy(xs[2]).to(dev)
+    plan.execute([d1, d2, dr])
+    torch.cuda.synchronize()
+    assert _relerr(dr.cpu().numpy(), full) <= TOL

@@ -253,3 +253,48 @@ def test_diis_group_matches_single_steps():
             assert c1 == c2 == 0 and e1 == e2 == -1.0 - k
             assert abs(g1 - g2) <= 1e-12 * g1
             assert torch.equal(t1[k], t2[k]), (it, k)
+
+
+def test_singular_subspace_is_reported_and_stops_the_solve():
+    """Two bitwise-identical preconditioned steps make the augmented DIIS
+    matrix exactly singular.  The reference stops there
+    (solve/optc_diis_extr.f:70); the synchronous step raises, the
+    asynchronous record reads -1, and ccsd.solve raises in every mode
+    instead of reporting convergence."""
+    torch = _torch()
+    import gecco_hip
+    import ccsd
+    from synth import synthetic
+    dev = torch.device('cuda:0')
+    n = 5000
+    g = torch.linspace(-1e-2, 1e-2, n, dtype=torch.float64, device=dev) + 1e-3
+    d = torch.full((n,), 2.0, dtype=torch.float64, device=dev)
+    t0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    # synchronous steps: the second sees the same g at the same t
+    diis = gecco_hip.DIIS(n, 4)
+    t = t0.clone()
+    diis.step(t, g, d, tol=0.0)
+    t.copy_(t0)
+    with pytest.raises(gecco_hip.GeccoHipError, match='singular'):
+        diis.step(t, g, d, tol=0.0)
+    # asynchronous steps: the record of the second step is -1
+    diis = gecco_hip.DIIS(n, 4)
+    t = t0.clone()
+    diis.step_async(t, g, d, tol=0.0)
+    t.copy_(t0)
+    diis.step_async(t, g, d, tol=0.0)
+    torch.cuda.synchronize()
+    assert diis.status(0)[2] == 0 and diis.status(1)[2] == -1
+    # the solver: every evaluation returns g at T = 0
+    ham = synthetic(2, 5, 15, nsym=1, seed=3, sigma=0.1)
+    c = ccsd.CCSD(ham)
+    g0 = torch.linspace(1e-4, 2e-3, c.amp.numel(), dtype=torch.float64, device=dev)
+
+    def evaluate(stream=None):
+        c.amp.zero_()
+        c.res.copy_(g0)
+    c.evaluate = evaluate
+    for mode in ('sync', 'async', 'graph'):
+        print('singular subspace, mode', mode, flush=True)
+        with pytest.raises(gecco_hip.GeccoHipError, match='singular'):
+            c.solve(tol=1e-14, maxit=6, mode=mode)

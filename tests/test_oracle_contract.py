@@ -102,3 +102,58 @@ def test_ladder_flop_count():
     c, *_ = ladder(orbitals(8, 32, 1))
     assert c.flops(use_tr=True) == 147994624
     assert c.flops(use_tr=False) == 2 * (2 * 496 * 28 * 496 + 1024 * 64 * 1024)
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 4), (2, 3, 5), (4, 4, 5)])
+@pytest.mark.parametrize('name', sorted(__import__('models').TRA_CASES))
+@pytest.mark.parametrize('use_tr', [False, True])
+@pytest.mark.parametrize('route', [3, 2])
+def test_transposed_operands_equal_explicit_transposes(nsym, nocc, nvir, name, use_tr, route):
+    """A kernel call with tra_op1 / tra_op2 / tra_op1op2 (the reference's
+    dagger flags, contract/contr_op1op2_wmaps_c.f:844-855, addressed by the
+    transposed branch of contract/set_op_ldim_c.f) gives what the same
+    contraction gives on explicitly transposed lists (each stored element
+    (a; c) placed at (c; a) of a list of the swapped occupation, through the
+    dense antisymmetric form).  The non-transposed contractions are pinned
+    by the dense einsums above."""
+    from models import TRA_CASES, transposed_data
+    from oracle.contract import Contraction
+    from oracle.melist import MEList
+    orbs = orbitals(nocc, nvir, nsym)
+    c, stored, tra = TRA_CASES[name](orbs)
+    assert any(b['tra_op1'] or b['tra_op2'] or b['tra_op1op2'] for b in c.blocks(use_tr)) or \
+        not c.blocks(use_tr)
+    xs = [spin_symmetric(m, 21 + k) for k, m in enumerate(stored)]
+    xs[2] = spin_symmetric(stored[2], 31, 0.1)           # results accumulate
+    ops, xe = [], []
+    for m, x, t in zip(stored, xs, tra):
+        if t:
+            L, y = transposed_data(m, x)
+        else:
+            L, y = m, x.copy()
+        ops.append(L)
+        xe.append(y)
+    ce = Contraction(ops[0], ops[1], ops[2], c.cnt_c, c.cnt_a, c.fact)
+    got = xs[2].copy()
+    c.run(xs[0], xs[1], got, route=route, use_tr=use_tr)
+    ce.run(xe[0], xe[1], xe[2], route=route, use_tr=use_tr)
+    ref = transposed_data(ops[2], xe[2])[1] if tra[2] else xe[2]
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), np.abs(got - ref).max()
+    assert np.abs(ref - xs[2]).max() > 1e-3 * np.abs(ref).max()    # the contraction added something
+
+
+def test_transposed_layout_differs_from_plain():
+    """The transposed leading dimensions matter: reading a stored list through
+    its adjoint with the transpose flag cleared gives a different result
+    (so the check above would catch a dropped flag)."""
+    from models import TRA_CASES
+    from oracle.contract import Contraction
+    orbs = orbitals(3, 5, 2)
+    c, stored, tra = TRA_CASES['ring_t2'](orbs)
+    xs = [spin_symmetric(m, 41 + k) for k, m in enumerate(stored)]
+    got = np.zeros(stored[2].length)
+    c.run(xs[0], xs[1], got, route=3, use_tr=False)
+    blocks = [dict(b, tra_op2=0) for b in c.blocks(False)]
+    bad = np.zeros(stored[2].length)
+    c.run(xs[0], xs[1], bad, route=3, use_tr=False, blocks=blocks)
+    assert np.abs(got - bad).max() > 1e-3 * np.abs(got).max()

@@ -77,3 +77,31 @@ def test_plan_emulation_matches_oracle(nsym, nocc, nvir, name, use_tr):
     if use_tr:
         sym_ab(r, bufs[2], 0.5, float(r.absym))
     assert np.abs(bufs[2] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 3, 4), (4, 5, 9)])
+@pytest.mark.parametrize('name', ['energy_t1', 'ring_t2', 't1_ovoo_t2', 'ladder_t12', 'fock_t12', 'vvvo_t1'])
+@pytest.mark.parametrize('use_tr', [True, False])
+def test_transposed_operands_emulate_the_oracle(nsym, nocc, nvir, name, use_tr):
+    """Operands and results given as adjoint views (gecco_hip_list_adjoint):
+    the product's enumerator records the oracle's kernel calls (count and
+    FLOP) with the transpose flags, and its resolved addresses -- the
+    transposed branch of set_op_ldim_c in the plan -- reproduce the oracle."""
+    import gecco_hip
+    from models import TRA_CASES
+    oo = Orbitals(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    c, stored, tra = TRA_CASES[name](oo)
+    xs = [spin_symmetric(m, 81 + k) for k, m in enumerate(stored)]
+    ref = xs[2].copy()
+    c.run(xs[0], xs[1], ref, route=3, use_tr=use_tr, screen=False)
+    lists = [gecco_hip.MEList(o, m.occ_c, m.occ_a) for m in stored]
+    used = [L.adjoint() if t else L for L, t in zip(lists, tra)]
+    p = gecco_hip.Plan()
+    nb, fl = p.add_contraction(used[0], used[1], used[2], c.cnt_c, c.cnt_a, 1.0, use_tr)
+    assert nb == len(c.blocks(use_tr)) and fl == c.flops(use_tr)
+    bufs = [x.copy() for x in xs]
+    emulate(p, nb, bufs)
+    if use_tr:
+        sym_ab(stored[2], bufs[2], 0.5, float(stored[2].absym))
+    assert np.abs(bufs[2] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
