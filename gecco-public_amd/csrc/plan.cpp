@@ -619,34 +619,44 @@ static int check(hipError_t e, const char *what) {
     return ok();
 }
 
+static int dmalloc(gecco_hip_plan *p, void **ptr, size_t bytes, const char *what) {
+    (void)p;
+    return check(hipMalloc(ptr, std::max<size_t>(bytes, 1)), what);
+}
+
+static void dfree(gecco_hip_plan *p, void *ptr) {
+    (void)p;
+    if (ptr) (void)hipFree(ptr);
+}
+
 static void plan_free_device(gecco_hip_plan *p) {
-    if (p->d_blocks) (void)hipFree(p->d_blocks);
-    if (p->d_tasks) (void)hipFree(p->d_tasks);
-    if (p->d_index) (void)hipFree(p->d_index);
-    if (p->d_bufs) (void)hipFree(p->d_bufs);
+    if (p->d_blocks) dfree(p, p->d_blocks);
+    if (p->d_tasks) dfree(p, p->d_tasks);
+    if (p->d_index) dfree(p, p->d_index);
+    if (p->d_bufs) dfree(p, p->d_bufs);
     for (auto &tr : p->traces) {
-        if (tr.d_rowptr) (void)hipFree(tr.d_rowptr);
-        if (tr.d_rowdst) (void)hipFree(tr.d_rowdst);
-        if (tr.d_col) (void)hipFree(tr.d_col);
-        if (tr.d_sgn) (void)hipFree(tr.d_sgn);
+        if (tr.d_rowptr) dfree(p, tr.d_rowptr);
+        if (tr.d_rowdst) dfree(p, tr.d_rowdst);
+        if (tr.d_col) dfree(p, tr.d_col);
+        if (tr.d_sgn) dfree(p, tr.d_sgn);
         tr.d_rowptr = tr.d_rowdst = tr.d_col = nullptr;
         tr.d_sgn = nullptr;
     }
-    if (p->d_inits) (void)hipFree(p->d_inits);
-    if (p->d_itasks) (void)hipFree(p->d_itasks);
+    if (p->d_inits) dfree(p, p->d_inits);
+    if (p->d_itasks) dfree(p, p->d_itasks);
     p->d_inits = nullptr;
     p->d_itasks = nullptr;
-    if (p->d_gjobs) (void)hipFree(p->d_gjobs);
-    if (p->d_gtasks) (void)hipFree(p->d_gtasks);
-    if (p->d_scratch) (void)hipFree(p->d_scratch);
+    if (p->d_gjobs) dfree(p, p->d_gjobs);
+    if (p->d_gtasks) dfree(p, p->d_gtasks);
+    if (p->d_scratch) dfree(p, p->d_scratch);
     p->d_gjobs = nullptr;
     p->d_gtasks = nullptr;
     p->d_scratch = nullptr;
-    if (p->d_symsteps) (void)hipFree(p->d_symsteps);
-    if (p->d_symtasks) (void)hipFree(p->d_symtasks);
-    if (p->d_sym_i1) (void)hipFree(p->d_sym_i1);
-    if (p->d_sym_i2) (void)hipFree(p->d_sym_i2);
-    if (p->d_sym_f) (void)hipFree(p->d_sym_f);
+    if (p->d_symsteps) dfree(p, p->d_symsteps);
+    if (p->d_symtasks) dfree(p, p->d_symtasks);
+    if (p->d_sym_i1) dfree(p, p->d_sym_i1);
+    if (p->d_sym_i2) dfree(p, p->d_sym_i2);
+    if (p->d_sym_f) dfree(p, p->d_sym_f);
     p->d_symsteps = nullptr;
     p->d_symtasks = nullptr;
     p->d_sym_i1 = p->d_sym_i2 = nullptr;
@@ -1071,8 +1081,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     }
     if (pool.empty()) pool.push_back(0);
     int32_t *d_index = nullptr;
-    if (check(hipMalloc(&d_index, pool.size() * sizeof(int32_t)), "hipMalloc(index)")) return g_err;
-    if (check(hipMemcpy(d_index, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy(index)")) { (void)hipFree(d_index); return g_err; }
+    if (dmalloc(p, (void **)&d_index, pool.size() * sizeof(int32_t), "hipMalloc(index)")) return g_err;
+    if (check(hipMemcpy(d_index, pool.data(), pool.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy(index)")) { dfree(p, d_index); return g_err; }
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
         DevBlock &d = dblocks[i];
@@ -1185,27 +1195,27 @@ static int plan_finalize(gecco_hip_plan *p) {
         p->scratch_elems = next;
         p->gather_bytes = gbytes;
         if (!jobs.empty()) {
-            if (check(hipMalloc(&p->d_scratch, (size_t)next * p->elem), "hipMalloc(scratch)") ||
-                check(hipMalloc(&p->d_gjobs, jobs.size() * sizeof(DevGather)), "hipMalloc(gather jobs)") ||
+            if (dmalloc(p, (void **)&p->d_scratch, (size_t)next * p->elem, "hipMalloc(scratch)") ||
+                dmalloc(p, (void **)&p->d_gjobs, jobs.size() * sizeof(DevGather), "hipMalloc(gather jobs)") ||
                 check(hipMemcpy(p->d_gjobs, jobs.data(), jobs.size() * sizeof(DevGather), hipMemcpyHostToDevice), "hipMemcpy(gather jobs)") ||
-                check(hipMalloc(&p->d_gtasks, gt.size() * sizeof(DevGatherTask)), "hipMalloc(gather tasks)") ||
+                dmalloc(p, (void **)&p->d_gtasks, gt.size() * sizeof(DevGatherTask), "hipMalloc(gather tasks)") ||
                 check(hipMemcpy(p->d_gtasks, gt.data(), gt.size() * sizeof(DevGatherTask), hipMemcpyHostToDevice), "hipMemcpy(gather tasks)"))
                 return g_err;
         }
     }
-    if (check(hipMalloc(&p->d_blocks, std::max<size_t>(1, dblocks.size()) * sizeof(DevBlock)), "hipMalloc(blocks)")) return g_err;
+    if (dmalloc(p, (void **)&p->d_blocks, std::max<size_t>(1, dblocks.size()) * sizeof(DevBlock), "hipMalloc(blocks)")) return g_err;
     if (!dblocks.empty() && check(hipMemcpy(p->d_blocks, dblocks.data(), dblocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice), "hipMemcpy(blocks)")) return g_err;
-    if (check(hipMalloc(&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask)), "hipMalloc(tasks)")) return g_err;
+    if (dmalloc(p, (void **)&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask), "hipMalloc(tasks)")) return g_err;
     if (!tasks.empty() && check(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice), "hipMemcpy(tasks)")) return g_err;
     p->nbufs_max = nb_max;
-    if (check(hipMalloc(&p->d_bufs, (nb_max + 1) * sizeof(void *)), "hipMalloc(bufs)")) return g_err;
+    if (dmalloc(p, (void **)&p->d_bufs, (nb_max + 1) * sizeof(void *), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
     for (auto &tr : p->traces) {
         if (!tr.nrows) continue;
-        if (check(hipMalloc(&tr.d_rowptr, (tr.nrows + 1) * 8), "hipMalloc(trace)") ||
-            check(hipMalloc(&tr.d_rowdst, tr.nrows * 8), "hipMalloc(trace)") ||
-            check(hipMalloc(&tr.d_col, tr.nnz * 8), "hipMalloc(trace)") ||
-            check(hipMalloc(&tr.d_sgn, tr.nnz), "hipMalloc(trace)") ||
+        if (dmalloc(p, (void **)&tr.d_rowptr, (tr.nrows + 1) * 8, "hipMalloc(trace)") ||
+            dmalloc(p, (void **)&tr.d_rowdst, tr.nrows * 8, "hipMalloc(trace)") ||
+            dmalloc(p, (void **)&tr.d_col, tr.nnz * 8, "hipMalloc(trace)") ||
+            dmalloc(p, (void **)&tr.d_sgn, tr.nnz, "hipMalloc(trace)") ||
             check(hipMemcpy(tr.d_rowptr, tr.rowptr.data(), (tr.nrows + 1) * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
             check(hipMemcpy(tr.d_rowdst, tr.rowdst.data(), tr.nrows * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
             check(hipMemcpy(tr.d_col, tr.col.data(), tr.nnz * 8, hipMemcpyHostToDevice), "hipMemcpy(trace)") ||
@@ -1226,9 +1236,9 @@ static int plan_finalize(gecco_hip_plan *p) {
                 it.push_back({(int32_t)j, p->init_group[j], e0, std::min(p->inits[j].len, e0 + chunk)});
         p->nitasks = (int64_t)it.size();
         if (p->nitasks &&
-            (check(hipMalloc(&p->d_inits, p->inits.size() * sizeof(DevInit)), "hipMalloc(inits)") ||
+            (dmalloc(p, (void **)&p->d_inits, p->inits.size() * sizeof(DevInit), "hipMalloc(inits)") ||
              check(hipMemcpy(p->d_inits, p->inits.data(), p->inits.size() * sizeof(DevInit), hipMemcpyHostToDevice), "hipMemcpy(inits)") ||
-             check(hipMalloc(&p->d_itasks, it.size() * sizeof(DevInitTask)), "hipMalloc(init tasks)") ||
+             dmalloc(p, (void **)&p->d_itasks, it.size() * sizeof(DevInitTask), "hipMalloc(init tasks)") ||
              check(hipMemcpy(p->d_itasks, it.data(), it.size() * sizeof(DevInitTask), hipMemcpyHostToDevice), "hipMemcpy(init tasks)")))
             return g_err;
     }
@@ -1273,13 +1283,13 @@ static int plan_finalize(gecco_hip_plan *p) {
         }
         p->nsymtasks = (int64_t)stasks.size();
         if (p->nsymtasks &&
-            (check(hipMalloc(&p->d_symsteps, steps.size() * sizeof(DevSymStep)), "hipMalloc(sym)") ||
+            (dmalloc(p, (void **)&p->d_symsteps, steps.size() * sizeof(DevSymStep), "hipMalloc(sym)") ||
              check(hipMemcpy(p->d_symsteps, steps.data(), steps.size() * sizeof(DevSymStep), hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
-             check(hipMalloc(&p->d_symtasks, stasks.size() * sizeof(DevSymTask)), "hipMalloc(sym)") ||
+             dmalloc(p, (void **)&p->d_symtasks, stasks.size() * sizeof(DevSymTask), "hipMalloc(sym)") ||
              check(hipMemcpy(p->d_symtasks, stasks.data(), stasks.size() * sizeof(DevSymTask), hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
-             check(hipMalloc(&p->d_sym_i1, a1.size() * 4), "hipMalloc(sym)") ||
-             check(hipMalloc(&p->d_sym_i2, a2.size() * 4), "hipMalloc(sym)") ||
-             check(hipMalloc(&p->d_sym_f, ff.size()), "hipMalloc(sym)") ||
+             dmalloc(p, (void **)&p->d_sym_i1, a1.size() * 4, "hipMalloc(sym)") ||
+             dmalloc(p, (void **)&p->d_sym_i2, a2.size() * 4, "hipMalloc(sym)") ||
+             dmalloc(p, (void **)&p->d_sym_f, ff.size(), "hipMalloc(sym)") ||
              check(hipMemcpy(p->d_sym_i1, a1.data(), a1.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
              check(hipMemcpy(p->d_sym_i2, a2.data(), a2.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(sym)") ||
              check(hipMemcpy(p->d_sym_f, ff.data(), ff.size(), hipMemcpyHostToDevice), "hipMemcpy(sym)")))
@@ -1373,7 +1383,6 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
 }
 
 // ------------------------------------------------------------ staging
-static std::mutex g_stage_mu;
 struct Stage {
     double *p = nullptr;
     size_t n = 0;
@@ -1402,8 +1411,77 @@ static bool is_device_ptr(const void *p) {
     return attr.type == hipMemoryTypeDevice;
 }
 
-// One kernel call through the Fortran signature: stage host operands,
-// run the one-block plan, copy the result back.
+// ------------------------------------------------ cached single-call plans
+// GeCCo makes the same kernel calls (maps, lengths, factor) every iteration.
+// A one-call plan is kept per distinct call -- keyed by the resolved index
+// data of its operands, its dimensions, its factor and the device -- so a
+// repeated call does no device allocation and no upload: it only runs.
+static uint64_t mix64(uint64_t h, uint64_t x) {
+    h ^= x + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    h ^= h >> 31;
+    h *= 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 29;
+    return h;
+}
+
+struct CallKey {
+    uint64_t h1 = 0x243f6a8885a308d3ull, h2 = 0x13198a2e03707344ull, n = 0;
+    void add(uint64_t x) {
+        h1 = mix64(h1, x);
+        h2 = mix64(h2 ^ 0x5bd1e995u, x * 0xff51afd7ed558ccdull + n);
+        ++n;
+    }
+    void add(const std::vector<int32_t> &v) {
+        add((uint64_t)v.size());
+        size_t i = 0;
+        for (; i + 1 < v.size(); i += 2) add((uint64_t)(uint32_t)v[i] | ((uint64_t)(uint32_t)v[i + 1] << 32));
+        if (i < v.size()) add((uint64_t)(uint32_t)v[i]);
+    }
+    bool operator==(const CallKey &o) const { return h1 == o.h1 && h2 == o.h2 && n == o.n; }
+};
+
+struct CallKeyHash {
+    size_t operator()(const CallKey &k) const { return (size_t)(k.h1 ^ (k.h2 * 31)); }
+};
+
+static CallKey call_key(const HostBlock &hb, int dev) {
+    CallKey k;
+    uint64_t xf;
+    std::memcpy(&xf, &hb.xfac, 8);
+    k.add(xf);
+    k.add((uint64_t)dev);
+    const int64_t dims[5] = {hb.M, hb.N, hb.K, hb.len12, (int64_t)hb.swap | ((int64_t)hb.injective << 1)};
+    for (int64_t d : dims) k.add((uint64_t)d);
+    for (const HostOperand *o : {&hb.A, &hb.B, &hb.C}) {
+        k.add((uint64_t)o->r.table | ((uint64_t)o->fast_first << 1));
+        k.add((uint64_t)o->r.F);
+        k.add((uint64_t)o->r.S);
+        k.add(o->r.first);
+        k.add(o->r.second);
+    }
+    return k;
+}
+
+static std::mutex g_calls_mu;
+static std::unordered_map<CallKey, gecco_hip_plan *, CallKeyHash> g_calls;
+static double g_calls_bytes = 0;
+static const size_t CALLS_MAX = 16384;           // cached plans
+static const double CALLS_MAX_BYTES = 1.0e9;     // their device index bytes
+
+static void calls_clear() {
+    for (auto &kv : g_calls) {
+        plan_free_device(kv.second);
+        delete kv.second;
+    }
+    g_calls.clear();
+    g_calls_bytes = 0;
+}
+
+// One kernel call through the Fortran signature: stage host operands, run
+// the call's cached plan (built and finalized on its first occurrence),
+// copy a host result back.  The launch is on the null stream; the copy back
+// waits for it, device-resident results are ordered before later work on
+// that stream, and nothing else synchronises.
 static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const double *xop1,
                        const double *xop2) {
     gecco_hip_block_args a = a0;
@@ -1414,11 +1492,28 @@ static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const do
     const int64_t len1 = prod(a.lstrop1, a.ncblk_op1 + a.nablk_op1);
     const int64_t len2 = prod(a.lstrop2, a.ncblk_op2 + a.nablk_op2);
     const int64_t len12 = prod(a.lstrop1op2, a.ncblk_op1op2 + a.nablk_op1op2);
-    gecco_hip_plan plan;
-    plan.blocks.emplace_back();
-    if (build_block(&a, plan.blocks.back())) return g_err;
-    if (plan_finalize(&plan)) { plan_free_device(&plan); return g_err; }
-    std::lock_guard<std::mutex> lk(g_stage_mu);
+    HostBlock hb;
+    if (build_block(&a, hb)) return g_err;
+    int dev = 0;
+    if (check(hipGetDevice(&dev), "hipGetDevice")) return g_err;
+    const CallKey key = call_key(hb, dev);
+    std::lock_guard<std::mutex> lk(g_calls_mu);
+    gecco_hip_plan *plan = nullptr;
+    auto it = g_calls.find(key);
+    if (it != g_calls.end()) {
+        plan = it->second;
+    } else {
+        if (g_calls.size() >= CALLS_MAX || g_calls_bytes >= CALLS_MAX_BYTES) calls_clear();
+        plan = new gecco_hip_plan();
+        plan->blocks.push_back(std::move(hb));
+        if (plan_finalize(plan)) {
+            plan_free_device(plan);
+            delete plan;
+            return g_err;
+        }
+        g_calls.emplace(key, plan);
+        g_calls_bytes += plan->index_bytes + plan->scratch_elems * 8.0 + 1024.0;
+    }
     double *d1 = const_cast<double *>(xop1), *d2 = const_cast<double *>(xop2), *d12 = xop1op2;
     const bool h1 = !is_device_ptr(xop1), h2 = !is_device_ptr(xop2), h12 = !is_device_ptr(xop1op2);
     int rc = 0;
@@ -1426,10 +1521,8 @@ static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const do
     if (h2 && !rc) rc = stage_get(1, len2, &d2) || check(hipMemcpy(d2, xop2, len2 * 8, hipMemcpyHostToDevice), "stage op2");
     if (h12 && !rc) rc = stage_get(2, len12, &d12) || check(hipMemcpy(d12, xop1op2, len12 * 8, hipMemcpyHostToDevice), "stage res");
     double *bufs[3] = {d1, d2, d12};
-    if (!rc) rc = plan_execute(&plan, bufs, 3, 0);
-    if (!rc) rc = check(hipDeviceSynchronize(), "kernel");
+    if (!rc) rc = plan_execute(plan, bufs, 3, 0);
     if (!rc && h12) rc = check(hipMemcpy(xop1op2, d12, len12 * 8, hipMemcpyDeviceToHost), "unstage res");
-    plan_free_device(&plan);
     return rc ? g_err : ok();
 }
 
@@ -1572,6 +1665,14 @@ void contr_blk1blk2_wmaps_c_(
     std::memset(&a, 0, sizeof a);
     FILL_ARGS(a)
     single_call(a, xop1op2, xop1, xop2);
+}
+
+int gecco_hip_call_cache(int64_t *entries, int64_t clear) {
+    g_err = 0;
+    std::lock_guard<std::mutex> lk(g_calls_mu);
+    if (clear) calls_clear();
+    if (entries) *entries = (int64_t)g_calls.size();
+    return ok();
 }
 
 int gecco_hip_plan_create(gecco_hip_plan **plan) {

@@ -177,6 +177,14 @@ def contr_blk1blk2_blocked_mm(blk, xop1, xop2, xop1op2):
     _blocked_mm_call('contr_blk1blk2_blocked_mm_', blk, _ptr(xop1), _ptr(xop2), _ptr(xop1op2))
 
 
+def call_cache(clear=False):
+    """Number of one-call plans cached by the Fortran-signature entries
+    (after freeing them all when clear)."""
+    n = I64()
+    _check(_lib.gecco_hip_call_cache(ctypes.byref(n), int(bool(clear))))
+    return n.value
+
+
 def contr_blk1blk2_wmaps_c(blk, xop1, xop2, xop1op2):
     """Route-2 kernel call (same arithmetic result as route 3)."""
     _blocked_mm_call('contr_blk1blk2_wmaps_c_', blk, _ptr(xop1), _ptr(xop2), _ptr(xop1op2))

@@ -45,7 +45,12 @@ int gecco_hip_last_error(char *msg, int64_t msglen);
  * Replaces contract/contr_blk1blk2_blocked_mm.f:2-20 (called from
  * contract/contr_op1op2_wmaps_c.f:1927).  xscr/lenscr/lenblock/lenblock_cnt
  * are accepted for signature compatibility; the GPU kernel needs no host
- * scratch and its tiling does not depend on them. */
+ * scratch and its tiling does not depend on them.  The first call with a
+ * given set of maps, lengths and factor builds a one-call plan (device index
+ * tables) that later identical calls reuse, so a call GeCCo repeats every
+ * iteration does no device allocation and no upload; the kernels run on the
+ * null stream and only a host-resident result is copied back (which waits
+ * for them). */
 void contr_blk1blk2_blocked_mm_(
     const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
     const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2,
@@ -93,6 +98,12 @@ void contr_blk1blk2_wmaps_c_(
     const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a,
     const int64_t *map_ex1cntc, const int64_t *map_ex1cnta,
     const int64_t *map_ex2cntc, const int64_t *map_ex2cnta);
+
+/* The one-call plans kept by contr_blk1blk2_blocked_mm_ and
+ * contr_blk1blk2_wmaps_c_ (at most 16384, or about 1 GB of device index
+ * data; beyond that the cache is emptied): *entries = their number after
+ * the call; clear != 0 frees them first.  No counterpart in the reference. */
+int gecco_hip_call_cache(int64_t *entries, int64_t clear);
 
 /* ------------------------------------------------------------ plan API */
 typedef struct gecco_hip_plan gecco_hip_plan;

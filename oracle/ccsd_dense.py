@@ -21,6 +21,12 @@ Spin-orbital index within a space: so = 2*orb + spin (spin 0 alpha,
 import numpy as np
 
 
+def _einsum(*args):
+    """np.einsum with its contraction order chosen for BLAS (results equal
+    the left-to-right evaluation up to rounding)."""
+    return np.einsum(*args, optimize=True)
+
+
 def spin_integrals(ham):
     """Antisymmetrised <pq||rs> blocks and orbital energies, spin-orbital."""
     no, nv = ham.nocc, ham.nvir
@@ -56,8 +62,8 @@ class CCSD:
                       - self.ev[None, None, :, None] - self.ev[None, None, None, :])
 
     def energy(self, t1, t2):
-        return 0.25 * np.einsum('ijab,ijab', self.oovv, t2) + \
-            0.5 * np.einsum('ijab,ia,jb', self.oovv, t1, t1)
+        return 0.25 * _einsum('ijab,ijab', self.oovv, t2) + \
+            0.5 * _einsum('ijab,ia,jb', self.oovv, t1, t1)
 
     def residual(self, t1, t2):
         """Stanton-Gauss-Watts-Bartlett 1991 with a diagonal Fock operator.
@@ -67,38 +73,38 @@ class CCSD:
         ooov = g[o, o, o, v]; oovv = self.oovv; ovvv = g[o, v, v, v]
         oooo = g[o, o, o, o]; vvvv = g[v, v, v, v]; ovvo = g[o, v, v, o]
         ovoo = g[o, v, o, o]; vvvo = g[v, v, v, o]; oovo = g[o, o, v, o]
-        tau = t2 + np.einsum('ia,jb->ijab', t1, t1) - np.einsum('ib,ja->ijab', t1, t1)
-        taut = t2 + 0.5 * (np.einsum('ia,jb->ijab', t1, t1) - np.einsum('ib,ja->ijab', t1, t1))
-        Fae = np.einsum('mf,mafe->ae', t1, ovvv) - 0.5 * np.einsum('mnaf,mnef->ae', taut, oovv)
-        Fmi = np.einsum('ne,mnie->mi', t1, ooov) + 0.5 * np.einsum('inef,mnef->mi', taut, oovv)
-        Fme = np.einsum('nf,mnef->me', t1, oovv)
-        Wmnij = oooo + np.einsum('je,mnie->mnij', t1, ooov) - np.einsum('ie,mnje->mnij', t1, ooov) \
-            + 0.25 * np.einsum('ijef,mnef->mnij', tau, oovv)
-        Wabef = vvvv - np.einsum('mb,amef->abef', t1, g[v, o, v, v]) \
-            + np.einsum('ma,bmef->abef', t1, g[v, o, v, v]) \
-            + 0.25 * np.einsum('mnab,mnef->abef', tau, oovv)
-        Wmbej = ovvo + np.einsum('jf,mbef->mbej', t1, ovvv) - np.einsum('nb,mnej->mbej', t1, oovo) \
-            - np.einsum('jnfb,mnef->mbej', 0.5 * t2 + np.einsum('jf,nb->jnfb', t1, t1), oovv)
+        tau = t2 + _einsum('ia,jb->ijab', t1, t1) - _einsum('ib,ja->ijab', t1, t1)
+        taut = t2 + 0.5 * (_einsum('ia,jb->ijab', t1, t1) - _einsum('ib,ja->ijab', t1, t1))
+        Fae = _einsum('mf,mafe->ae', t1, ovvv) - 0.5 * _einsum('mnaf,mnef->ae', taut, oovv)
+        Fmi = _einsum('ne,mnie->mi', t1, ooov) + 0.5 * _einsum('inef,mnef->mi', taut, oovv)
+        Fme = _einsum('nf,mnef->me', t1, oovv)
+        Wmnij = oooo + _einsum('je,mnie->mnij', t1, ooov) - _einsum('ie,mnje->mnij', t1, ooov) \
+            + 0.25 * _einsum('ijef,mnef->mnij', tau, oovv)
+        Wabef = vvvv - _einsum('mb,amef->abef', t1, g[v, o, v, v]) \
+            + _einsum('ma,bmef->abef', t1, g[v, o, v, v]) \
+            + 0.25 * _einsum('mnab,mnef->abef', tau, oovv)
+        Wmbej = ovvo + _einsum('jf,mbef->mbej', t1, ovvv) - _einsum('nb,mnej->mbej', t1, oovo) \
+            - _einsum('jnfb,mnef->mbej', 0.5 * t2 + _einsum('jf,nb->jnfb', t1, t1), oovv)
         # T1 equation (eq. 1 with f_ia = 0, off-diagonal f = 0)
-        r1 = np.einsum('ie,ae->ia', t1, Fae) - np.einsum('ma,mi->ia', t1, Fmi) \
-            + np.einsum('imae,me->ia', t2, Fme) - np.einsum('nf,naif->ia', t1, g[o, v, o, v]) \
-            - 0.5 * np.einsum('imef,maef->ia', t2, ovvv) \
-            - 0.5 * np.einsum('mnae,nmei->ia', t2, oovo)
+        r1 = _einsum('ie,ae->ia', t1, Fae) - _einsum('ma,mi->ia', t1, Fmi) \
+            + _einsum('imae,me->ia', t2, Fme) - _einsum('nf,naif->ia', t1, g[o, v, o, v]) \
+            - 0.5 * _einsum('imef,maef->ia', t2, ovvv) \
+            - 0.5 * _einsum('mnae,nmei->ia', t2, oovo)
         # T2 equation (eq. 2)
-        Fbe = Fae - 0.5 * np.einsum('mb,me->be', t1, Fme)
-        Fmj = Fmi + 0.5 * np.einsum('je,me->mj', t1, Fme)
-        x = np.einsum('ijae,be->ijab', t2, Fbe)
+        Fbe = Fae - 0.5 * _einsum('mb,me->be', t1, Fme)
+        Fmj = Fmi + 0.5 * _einsum('je,me->mj', t1, Fme)
+        x = _einsum('ijae,be->ijab', t2, Fbe)
         r2 = g[o, o, v, v].copy()
         r2 += x - x.transpose(0, 1, 3, 2)
-        x = np.einsum('imab,mj->ijab', t2, Fmj)
+        x = _einsum('imab,mj->ijab', t2, Fmj)
         r2 -= x - x.transpose(1, 0, 2, 3)
-        r2 += 0.5 * np.einsum('mnab,mnij->ijab', tau, Wmnij)
-        r2 += 0.5 * np.einsum('ijef,abef->ijab', tau, Wabef)
-        x = np.einsum('imae,mbej->ijab', t2, Wmbej) - np.einsum('ie,ma,mbej->ijab', t1, t1, ovvo)
+        r2 += 0.5 * _einsum('mnab,mnij->ijab', tau, Wmnij)
+        r2 += 0.5 * _einsum('ijef,abef->ijab', tau, Wabef)
+        x = _einsum('imae,mbej->ijab', t2, Wmbej) - _einsum('ie,ma,mbej->ijab', t1, t1, ovvo)
         r2 += x - x.transpose(0, 1, 3, 2) - x.transpose(1, 0, 2, 3) + x.transpose(1, 0, 3, 2)
-        x = np.einsum('ie,abej->ijab', t1, vvvo)
+        x = _einsum('ie,abej->ijab', t1, vvvo)
         r2 += x - x.transpose(1, 0, 2, 3)
-        x = np.einsum('ma,mbij->ijab', t1, ovoo)
+        x = _einsum('ma,mbij->ijab', t1, ovoo)
         r2 -= x - x.transpose(0, 1, 3, 2)
         return r1, r2
 

@@ -1,7 +1,7 @@
 """Writes the FP64 oracle CCSD energy of a synthetic Hamiltonian as a golden
 fixture: the package's CCSD formula table evaluated with the oracle's
-restatement of contr_op1op2 / contr_blk1blk2_blocked_mm (oracle/), Jacobi
-steps with Pulay DIIS, converged to ||Omega|| < 1e-10.
+restatement of contr_op1op2 / contr_blk1blk2_blocked_mm (oracle/ccsd_route3.py),
+Jacobi steps with Pulay DIIS, converged to ||Omega|| < 1e-10.
 
 Usage: python tests/golden/make_ccsd_reference.py NOCC NVIR NAUX NSYM SEED SIGMA OUT.json
 """
@@ -10,43 +10,13 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, 'gecco-public_amd'))
 
 import ccsd  # noqa: E402
 from synth import synthetic  # noqa: E402
-from oracle.melist import Orbitals, MEList  # noqa: E402
-from oracle.contract import Contraction, sym_ab_pairs, apply_sym_pairs  # noqa: E402
-from oracle.strings import strings_of, to_spinorbs  # noqa: E402
-
-H, P = 1, 2
-
-
-def list_spinorbs(L):
-    """(length, nel) spin-orbital indices of every element (component order)."""
-    nel = L.nc + L.na
-    out = np.zeros((L.length, nel), dtype=np.int64)
-    for dl in L.blocks.values():
-        for d in dl:
-            comps = []
-            for k in range(len(L.oc)):
-                comps.append(strings_of(L.orbs.spaces[L.hc[k]], L.oc[k], d['msc'][k], d['gamc'][k]))
-            for k in range(len(L.oa)):
-                comps.append(strings_of(L.orbs.spaces[L.ha[k]], L.oa[k], d['msa'][k], d['gama'][k]))
-            arrs = [np.array([[2 * (o - 1) + (0 if s == 1 else 1) for o, s in to_spinorbs(x)]
-                              for x in c], dtype=np.int64).reshape(len(c), -1) for c in comps]
-            ld = d['ldc'] + d['lda']
-            grids = np.meshgrid(*[np.arange(len(c)) for c in comps], indexing='ij')
-            offs = d['off'] + sum(g * l for g, l in zip(grids, ld))
-            cols = []
-            for a, g in zip(arrs, grids):
-                cols.append(a[g])
-            block = np.concatenate([c[..., None] if c.ndim == len(comps) else c for c in cols], axis=-1)
-            out[offs.ravel()] = block.reshape(-1, nel)
-    return out
+from oracle.ccsd_route3 import Route3CCSD  # noqa: E402
 
 
 def main():
@@ -55,91 +25,9 @@ def main():
     out_path = sys.argv[7]
     t0 = time.time()
     ham = synthetic(nocc, nvir, naux, nsym=nsym, seed=seed, sigma=sigma)
-    n = nocc + nvir
-    eri = (ham.B.reshape(naux, n * n).T @ ham.B.reshape(naux, n * n)).reshape(n, n, n, n)
-    orbs = Orbitals(ham.irrep_o, ham.irrep_v, nsym)
-    L = {k: MEList(orbs, c, a) for k, (c, a) in ccsd.LISTS.items()}
-    X = {}
-    for name in ccsd.INTEGRALS:
-        so = list_spinorbs(L[name])
-        sp = []
-        for side in (L[name].occ_c, L[name].occ_a):
-            for h in (4, 2, 3, 1):
-                sp += [h] * side.get(h, 0)
-        g = [(so[:, q] >> 1) + (nocc if sp[q] == P else 0) for q in range(4)]
-        s = [so[:, q] & 1 for q in range(4)]
-        X[name] = eri[g[0], g[2], g[1], g[3]] * ((s[0] == s[2]) & (s[1] == s[3])) - \
-            eri[g[0], g[3], g[1], g[2]] * ((s[0] == s[3]) & (s[1] == s[2]))
-    eps = np.concatenate([ham.eps_o, ham.eps_v])
-    D = {}
-    for name in ('T1', 'T2'):
-        so = list_spinorbs(L[name])
-        nc = L[name].nc
-        sp = []
-        for side in (L[name].occ_c, L[name].occ_a):
-            for h in (4, 2, 3, 1):
-                sp += [h] * side.get(h, 0)
-        d = np.zeros(L[name].length)
-        for q in range(so.shape[1]):
-            e = eps[(so[:, q] >> 1) + (nocc if sp[q] == P else 0)]
-            d += -e if q < nc else e
-        D[name] = d
-    contractions = []
-    for level in ccsd.LEVELS:
-        lv = []
-        for res, a, b, cc, ca, fact in level:
-            c = Contraction(L[a], L[b], L[res], cc, ca, fact)
-            lv.append((res, a, b, c, c.blocks(True)))
-        contractions.append(lv)
-    pairs = {}
-    for lv in contractions:
-        for res, *_ in lv:
-            if res not in pairs:
-                pairs[res] = sym_ab_pairs(L[res], 1.0)
+    r = Route3CCSD(ham, ccsd.FORMULA, ccsd.SCHEDULE)
     print(f'setup {time.time() - t0:.1f} s', flush=True)
-    X['T1'] = np.zeros(L['T1'].length)
-    X['T2'] = np.zeros(L['T2'].length)
-    n1 = L['T1'].length
-    Dcat = np.concatenate([D['T1'], D['T2']])
-    hx, he = [], []
-    energy = None
-    for it in range(1, 200):
-        for name in ccsd.LISTS:
-            if name in ccsd.INIT:
-                src, f = ccsd.INIT[name]
-                X[name] = f * X[src]
-            elif name not in ccsd.INTEGRALS and name not in ('T1', 'T2'):
-                X[name] = np.zeros(L[name].length)
-        for lv in contractions:
-            written = []
-            for res, a, b, c, blocks in lv:
-                c.run(X[a], X[b], X[res], route=3, use_tr=False, blocks=blocks)
-                if res not in written:
-                    written.append(res)
-            for res in written:
-                apply_sym_pairs(X[res], pairs[res], 0.5)
-        t = np.concatenate([X['T1'], X['T2']])
-        omega = np.concatenate([X['R1'], X['R2']]) - Dcat * t
-        rn = float(np.linalg.norm(omega))
-        energy = float(X['E'][0])
-        print(f'it {it:3d} E {energy:.14f} |R| {rn:.3e} ({time.time() - t0:.0f} s)', flush=True)
-        if rn < 1e-10:
-            break
-        step = omega / Dcat
-        hx.append(t + step)
-        he.append(step)
-        hx, he = hx[-8:], he[-8:]
-        m = len(hx)
-        B = -np.ones((m + 1, m + 1))
-        B[m, m] = 0.0
-        for i in range(m):
-            for j in range(m):
-                B[i, j] = he[i] @ he[j]
-        r = np.zeros(m + 1)
-        r[m] = -1.0
-        coef = np.linalg.solve(B, r)[:m]
-        new = sum(ci * xi for ci, xi in zip(coef, hx))
-        X['T1'], X['T2'] = new[:n1].copy(), new[n1:].copy()
+    energy, it, rn = r.solve(tol=1e-10, log=lambda m: print(m, flush=True))
     json.dump({'recipe': {'nocc': nocc, 'nvir': nvir, 'naux': naux, 'nsym': nsym, 'seed': seed,
                           'sigma': sigma, 'occ_range': [-2.0, -0.3], 'vir_range': [0.2, 3.0]},
                'energy': energy, 'iterations': it, 'residual': rn,

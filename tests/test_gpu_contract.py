@@ -596,3 +596,39 @@ def test_transposed_operands_match_oracle(nsym, nocc, nvir, name, use_tr):
     plan.execute([d1, d2, dr])
     torch.cuda.synchronize()
     assert _relerr(dr.cpu().numpy(), full) <= TOL
+
+
+def test_fortran_entry_reuses_its_plans():
+    """The Fortran-signature entries keep one plan per distinct kernel call:
+    repeating the calls (host and device operands) adds the same result
+    again without new plans, and another factor is another plan."""
+    torch = _torch()
+    import gecco_hip
+    orbs = orbitals(4, 6, 2)
+    c, a, b, r = CASES['ring'](orbs)
+    x1 = spin_symmetric(a, 35)
+    x2 = spin_symmetric(b, 36)
+    blocks = c.blocks(True)
+    ref = _oracle(c, x1, x2, r.length, blocks)
+    gecco_hip.call_cache(clear=True)
+    dev = torch.device('cuda:0')
+    d1, d2 = torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev)
+    dr = torch.zeros(r.length, dtype=torch.float64, device=dev)
+    got = np.zeros(r.length)
+    counts = []
+    for rep in range(3):
+        for bl in blocks:
+            gecco_hip.contr_blk1blk2_blocked_mm(bl, d1[bl['off1']:], d2[bl['off2']:], dr[bl['off12']:])
+            gecco_hip.contr_blk1blk2_wmaps_c(bl, x1[bl['off1']:], x2[bl['off2']:], got[bl['off12']:])
+        counts.append(gecco_hip.call_cache())
+    torch.cuda.synchronize()
+    assert counts[0] == len(blocks) and counts[1] == counts[0] and counts[2] == counts[0]
+    assert _relerr(dr.cpu().numpy(), 3 * ref) <= TOL
+    assert _relerr(got, 3 * ref) <= TOL
+    for bl in blocks:
+        bl2 = dict(bl, xfac=-0.5 * bl['xfac'])
+        gecco_hip.contr_blk1blk2_blocked_mm(bl2, d1[bl['off1']:], d2[bl['off2']:], dr[bl['off12']:])
+    torch.cuda.synchronize()
+    assert gecco_hip.call_cache() == 2 * len(blocks)
+    assert _relerr(dr.cpu().numpy(), 2.5 * ref) <= TOL
+    assert gecco_hip.call_cache(clear=True) == 0
