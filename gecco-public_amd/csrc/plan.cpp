@@ -520,6 +520,10 @@ struct gecco_hip_plan {
     int elem = 8;
     const int32_t *d_skip = nullptr;
     int64_t nskip = 0;
+    // device memory from a caller-given region instead of hipMalloc (the
+    // allocation-free entry point): base, size, bytes handed out
+    char *arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
 };
 
 namespace gecco {
@@ -619,14 +623,23 @@ static int check(hipError_t e, const char *what) {
     return ok();
 }
 
+// Device memory of a plan: hipMalloc, or, for a plan that owns none, a
+// bump allocation inside a caller-given region (256-byte aligned pieces).
 static int dmalloc(gecco_hip_plan *p, void **ptr, size_t bytes, const char *what) {
-    (void)p;
+    if (p->arena) {
+        const size_t off = (p->arena_used + 255) & ~(size_t)255;
+        if (off + bytes > p->arena_bytes)
+            return fail(GECCO_HIP_EINVAL, "%s: %zu bytes needed beyond %zu in use exceed the %zu-byte "
+                        "scratch of the allocation-free entry point", what, bytes, off, p->arena_bytes);
+        *ptr = p->arena + off;
+        p->arena_used = off + bytes;
+        return ok();
+    }
     return check(hipMalloc(ptr, std::max<size_t>(bytes, 1)), what);
 }
 
 static void dfree(gecco_hip_plan *p, void *ptr) {
-    (void)p;
-    if (ptr) (void)hipFree(ptr);
+    if (!p->arena && ptr) (void)hipFree(ptr);
 }
 
 static void plan_free_device(gecco_hip_plan *p) {
@@ -1526,6 +1539,54 @@ static int single_call(const gecco_hip_block_args &a0, double *xop1op2, const do
     return rc ? g_err : ok();
 }
 
+// ------------------------------------------ the allocation-free entry point
+// Device memory of the calls of contr_blk1blk2_blocked_mm_noalloc_ (index
+// tables, task lists, materialised operands, the buffer-slot table).  Each
+// device holds its own copy for the life of the process; its address on a
+// device comes from hipGetSymbolAddress on the first call there.
+#define NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, char *> g_scratch_ptr;   // device -> its copy of scratch
+
+static int scratch_address(char **out) {
+    int dev = 0;
+    if (check(hipGetDevice(&dev), "hipGetDevice")) return g_err;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void *ptr = nullptr;
+        if (check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return g_err;
+        it = g_scratch_ptr.emplace(dev, (char *)ptr).first;
+    }
+    *out = it->second;
+    return ok();
+}
+
+static int noalloc_call(const gecco_hip_block_args &a0, double *xop1op2, const double *xop1,
+                        const double *xop2) {
+    if (!is_device_ptr(xop1) || !is_device_ptr(xop2) || !is_device_ptr(xop1op2))
+        return fail(GECCO_HIP_EINVAL, "contr_blk1blk2_blocked_mm_noalloc_: operands must be device memory");
+    gecco_hip_block_args a = a0;
+    a.buf_op1 = 0;
+    a.buf_op2 = 1;
+    a.buf_op1op2 = 2;
+    a.off_op1 = a.off_op2 = a.off_op1op2 = 0;
+    char *base = nullptr;
+    if (scratch_address(&base)) return g_err;
+    gecco_hip_plan plan;
+    plan.arena = base;
+    plan.arena_bytes = NOALLOC_SCRATCH_BYTES;
+    plan.blocks.emplace_back();
+    if (build_block(&a, plan.blocks.back())) return g_err;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    // the previous call's kernels may still read the scratch
+    if (check(hipStreamSynchronize(0), "hipStreamSynchronize")) return g_err;
+    if (plan_finalize(&plan)) return g_err;
+    double *bufs[3] = {const_cast<double *>(xop1), const_cast<double *>(xop2), xop1op2};
+    return plan_execute(&plan, bufs, 3, 0);
+}
+
 int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *args) {
     if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
     HostBlock hb;
@@ -1666,6 +1727,33 @@ void contr_blk1blk2_wmaps_c_(
     FILL_ARGS(a)
     single_call(a, xop1op2, xop1, xop2);
 }
+
+void contr_blk1blk2_blocked_mm_noalloc_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2, double *xscr,
+    const int64_t *lenscr, const int64_t *lenblock, const int64_t *lenblock_cnt,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1, const int64_t *ncblk_ex1,
+    const int64_t *nablk_ex1, const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2, const int64_t *ncblk_cnt,
+    const int64_t *nablk_cnt, const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a, const int64_t *hpvxop2c,
+    const int64_t *hpvxop2a, const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a, const int64_t *map_info_1c,
+    const int64_t *map_info_1a, const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a, const int64_t *map_ex1cntc,
+    const int64_t *map_ex1cnta, const int64_t *map_ex2cntc, const int64_t *map_ex2cnta) {
+    (void)xscr; (void)lenscr; (void)lenblock; (void)lenblock_cnt;
+    g_err = 0;
+    g_msg.clear();
+    gecco_hip_block_args a;
+    std::memset(&a, 0, sizeof a);
+    FILL_ARGS(a)
+    noalloc_call(a, xop1op2, xop1, xop2);
+}
+
+int64_t gecco_hip_noalloc_scratch_bytes(void) { return (int64_t)NOALLOC_SCRATCH_BYTES; }
 
 int gecco_hip_call_cache(int64_t *entries, int64_t clear) {
     g_err = 0;

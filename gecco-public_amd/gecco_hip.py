@@ -150,7 +150,7 @@ def _blocked_mm_call(sym, blk, x1, x2, xr):
     ref = lambda v: ctypes.byref(I64(int(v)))
     args = [ctypes.byref(ctypes.c_double(float(blk['xfac']))), xr, x1, x2,
             ref(blk.get('tra_op1', 0)), ref(blk.get('tra_op2', 0)), ref(blk.get('tra_op1op2', 0))]
-    if sym == 'contr_blk1blk2_blocked_mm_':
+    if sym in ('contr_blk1blk2_blocked_mm_', 'contr_blk1blk2_blocked_mm_noalloc_'):
         scr = ctypes.c_double(0.0)
         args += [ctypes.byref(scr), ref(1), ref(200), ref(-1)]
     args += [ref(blk[k]) for k in DIM_KEYS]
@@ -177,12 +177,25 @@ def contr_blk1blk2_blocked_mm(blk, xop1, xop2, xop1op2):
     _blocked_mm_call('contr_blk1blk2_blocked_mm_', blk, _ptr(xop1), _ptr(xop2), _ptr(xop1op2))
 
 
+def contr_blk1blk2_blocked_mm_noalloc(blk, xop1, xop2, xop1op2):
+    """The allocation-free variant (contr_blk1blk2_blocked_mm_noalloc_):
+    device tensors only; no device memory is allocated."""
+    _blocked_mm_call('contr_blk1blk2_blocked_mm_noalloc_', blk, _ptr(xop1), _ptr(xop2), _ptr(xop1op2))
+
+
 def call_cache(clear=False):
     """Number of one-call plans cached by the Fortran-signature entries
     (after freeing them all when clear)."""
     n = I64()
     _check(_lib.gecco_hip_call_cache(ctypes.byref(n), int(bool(clear))))
     return n.value
+
+
+def noalloc_scratch_bytes():
+    """Size of the fixed device array of the allocation-free entry."""
+    f = _lib.gecco_hip_noalloc_scratch_bytes
+    f.restype = I64
+    return int(f())
 
 
 def contr_blk1blk2_wmaps_c(blk, xop1, xop2, xop1op2):

@@ -105,6 +105,46 @@ void contr_blk1blk2_wmaps_c_(
  * the call; clear != 0 frees them first.  No counterpart in the reference. */
 int gecco_hip_call_cache(int64_t *entries, int64_t clear);
 
+/* Allocation-free variant of contr_blk1blk2_blocked_mm_ (same arguments,
+ * same result).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree: the
+ * call's index tables, task lists, materialised operands and buffer-slot
+ * table are placed in a fixed device array of
+ * gecco_hip_noalloc_scratch_bytes() bytes (64 MiB) that every device holds
+ * for the life of the process; a call that needs more fails with
+ * GECCO_HIP_EINVAL (gecco_hip_last_error).  xop1, xop2 and xop1op2 must be
+ * device memory.  The array is shared by every call on a device and keeps
+ * the last call's contents (nothing in it has to start at zero: each call
+ * writes all it reads); a call first waits for the null stream, so calls
+ * from one thread follow each other safely, but calls that overlap on one
+ * device from several threads are serialised by a lock only within this
+ * process.  The kernels run on the null stream. */
+void contr_blk1blk2_blocked_mm_noalloc_(
+    const double *xfac, double *xop1op2, const double *xop1, const double *xop2,
+    const int64_t *tra_op1, const int64_t *tra_op2, const int64_t *tra_op1op2,
+    double *xscr, const int64_t *lenscr, const int64_t *lenblock,
+    const int64_t *lenblock_cnt,
+    const int64_t *ncblk_op1, const int64_t *nablk_op1,
+    const int64_t *ncblk_ex1, const int64_t *nablk_ex1,
+    const int64_t *ncblk_op2, const int64_t *nablk_op2,
+    const int64_t *ncblk_ex2, const int64_t *nablk_ex2,
+    const int64_t *ncblk_cnt, const int64_t *nablk_cnt,
+    const int64_t *ncblk_op1op2, const int64_t *nablk_op1op2,
+    const int64_t *hpvxop1c, const int64_t *hpvxop1a,
+    const int64_t *hpvxop2c, const int64_t *hpvxop2a,
+    const int64_t *hpvxop1op2c, const int64_t *hpvxop1op2a,
+    const int64_t *lstrop1, const int64_t *lstrop2, const int64_t *lstrop1op2,
+    const int64_t *lstr_ex1, const int64_t *lstr_ex2, const int64_t *lstr_cnt,
+    const int64_t *map_info_12c, const int64_t *map_info_12a,
+    const int64_t *map_info_1c, const int64_t *map_info_1a,
+    const int64_t *map_info_2c, const int64_t *map_info_2a,
+    const int64_t *map_ex1ex2c, const int64_t *map_ex1ex2a,
+    const int64_t *map_ex1cntc, const int64_t *map_ex1cnta,
+    const int64_t *map_ex2cntc, const int64_t *map_ex2cnta);
+/* Size in bytes of the device array used by the allocation-free entry. */
+int64_t gecco_hip_noalloc_scratch_bytes(void);
+
 /* ------------------------------------------------------------ plan API */
 typedef struct gecco_hip_plan gecco_hip_plan;
 
