@@ -205,34 +205,18 @@ def kernel_times(plans, run, steps):
                 dom=dom, dom_ms=dom_ms, achieved=achieved, gather_bytes=gbytes)
 
 
-def cpu_ccsd_sample(ham, flops_iter):
-    """Oracle route 3 (one core) on the last level of the CCSD formula (the
-    residual contractions, about half of an iteration's FLOP); returns the
-    per-iteration rate scaled by FLOP, the seconds and a description."""
-    import numpy as np
-    import ccsd
-    from oracle.melist import Orbitals, MEList
-    from oracle.contract import Contraction
-    orbs = Orbitals(ham.irrep_o, ham.irrep_v, ham.nsym)
-    L = {k: MEList(orbs, c, a) for k, (c, a) in ccsd.LISTS.items()}
-    rng = np.random.default_rng(2)
-    X = {k: 0.01 * rng.standard_normal(L[k].length) for k in L}
-    terms = []
-    fl = 0.0
-    for res, a, b, cc, ca, fact in ccsd.LEVELS[-1]:
-        c = Contraction(L[a], L[b], L[res], cc, ca, fact)
-        blocks = c.blocks(True)
-        fl += sum(2.0 * np.prod(x['lstr_ex1']) * np.prod(x['lstr_ex2']) * np.prod(x['lstr_cnt'])
-                  for x in blocks)
-        terms.append((res, a, b, c, blocks))
-    t0 = time.perf_counter()
-    for res, a, b, c, blocks in terms:
-        c.run(X[a], X[b], X[res], route=3, use_tr=False, screen=False, blocks=blocks)
-    dt = time.perf_counter() - t0
-    per_iter = dt * flops_iter / fl
-    return 1.0 / per_iter, dt, (f'the {len(terms)} residual contractions of the last formula level '
-                                f'({fl / 1e9:.2f} of {flops_iter / 1e9:.2f} GFLOP per iteration), oracle '
-                                'route 3 with a plain-loop dgemm, scaled to one iteration by FLOP')
+def cpu_ccsd_iteration(args):
+    """One whole CCSD formula iteration of the oracle's route 3 on one host
+    core (oracle/cpu_baseline.py in a child process: the bundled OpenBLAS
+    dgemm limited to one thread before numpy loads), best of two."""
+    import subprocess
+    cmd = [sys.executable, '-m', 'oracle.cpu_baseline', str(args.nocc), str(args.nvir), str(args.naux),
+           str(args.nsym), str(args.seed), str(args.sigma), '2']
+    out = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError('cpu baseline failed: ' + out.stderr[-2000:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r
 
 
 def run_ccsd(args):
@@ -258,10 +242,11 @@ def run_ccsd(args):
     solver.solve(tol=0.0, maxit=args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    solver.solve(tol=0.0, maxit=args.steps, restart=False)
+    timed = solver.solve(tol=0.0, maxit=args.steps, restart=False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    step(f'timed steps {wall:.3f} s')
+    done = timed['iterations']         # all of them unless the solve stopped early
+    step(f'timed steps {wall:.3f} s, {done} iterations')
     # kernel durations (roofline): as many further iterations launched one by
     # one (the same kernels as the replayed graph), timed with HIP events on
     # the launch stream
@@ -269,32 +254,41 @@ def run_ccsd(args):
                                                         mode='async'), args.steps)
     ms, names, dom, dom_ms, achieved, gbytes = (kt[k] for k in ('ms', 'names', 'dom', 'dom_ms',
                                                                 'achieved', 'gather_bytes'))
-    # time to convergence from T = 0
-    torch.cuda.synchronize()
-    res = solver.solve(tol=args.tol, maxit=200)
-    step(f'solve {res["iterations"]} iterations')
-    # mixed precision: float32 lists until ||g|| < 1e-6, then float64
-    # refinement; and the float32 iteration rate (graph replays)
+    # time to convergence from T = 0: each scheme runs twice (its first run
+    # captures its graphs) and the second run is the one reported
+    def twice(f):
+        f()
+        torch.cuda.synchronize()
+        return f()
+    res = twice(lambda: solver.solve(tol=args.tol, maxit=200))
+    step(f'float64 solve {res["iterations"]} iterations {res["seconds"]:.4f} s')
     t0 = time.perf_counter()
-    solver.build_f32()
+    solver.build_f32(bulk='f32')
     f32_setup = time.perf_counter() - t0
-    mixed = solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200)
+    # float32 residuals until ||g|| < 1e-6, then float64 iterations
+    mixed = twice(lambda: solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200, bulk='f32'))
     step(f'mixed solve {mixed["f32_iterations"]} + {mixed["f64_iterations"]} iterations')
+    # float64 only for two residual evaluations: float32 phase, the step of
+    # the linearised formula in float32, the final evaluation
+    refined = twice(lambda: solver.solve_refined(tol=args.tol, switch=1e-6, bulk='f32'))
+    step(f'refined solve {refined["f32_iterations"]} + {refined["linear_iterations"]} iterations, '
+         f'{refined["f64_evaluations"]} float64 evaluations')
     solver.solve(tol=0.0, maxit=args.warmup, precision='f32')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
     torch.cuda.synchronize()
     wall32 = time.perf_counter() - t0
-    # split-bfloat16 GEMM tiles in the float32 phase
-    mixed_bf = solver.solve_mixed(tol=args.tol, switch=1e-6, maxit=200, bulk='bf16x3')
+    # the same with split-bfloat16 GEMM tiles in the low-precision phases
+    refined_bf = twice(lambda: solver.solve_refined(tol=args.tol, switch=1e-6, bulk='bf16x3'))
+    step(f'refined split-bf16 solve {refined_bf["f32_iterations"]} + {refined_bf["linear_iterations"]} '
+         'iterations')
     solver.solve(tol=0.0, maxit=args.warmup, precision='f32')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solver.solve(tol=0.0, maxit=args.steps, precision='f32', restart=False)
     torch.cuda.synchronize()
     wall_bf = time.perf_counter() - t0
-    step(f'split-bf16 mixed solve {mixed_bf["f32_iterations"]} + {mixed_bf["f64_iterations"]} iterations')
     ref = None
     if os.path.exists(GOLDEN):
         g = json.load(open(GOLDEN))
@@ -303,16 +297,16 @@ def run_ccsd(args):
                 (args.nocc, args.nvir, args.naux, args.nsym, args.seed, args.sigma):
             ref = g['energy']
     out = {
-        'metric': METRIC, 'value': round(args.steps / wall, 3), 'unit': 'iters/s',
-        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(wall * 1e3 / args.steps, 4), 'higher_is_better': True,
+        'metric': METRIC, 'value': round(done / wall, 3), 'unit': 'iters/s',
+        'n_gpus': 1, 'steps': done, 'warmup': args.warmup,
+        'ms_per_step': round(wall * 1e3 / done, 4), 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': f'synthetic DF Hamiltonian (seed {args.seed}, naux {args.naux}, sigma {args.sigma})',
         'config': {'workload': 'ccsd_iteration', 'nocc': args.nocc, 'nvir': args.nvir,
                    'naux': args.naux, 'nsym': args.nsym, 'use_tr': True,
                    'kernel_calls_per_iter': solver.nblocks,
                    'gflop_per_iter': round(solver.flops / 1e9, 3)},
-        'contraction_tflops': round(solver.flops * args.steps / wall / 1e12, 3),
+        'contraction_tflops': round(solver.flops * done / wall / 1e12, 3),
         'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double, 1, 2>': round(ms[0] / args.steps, 4),
                                 'grouped_contract<double>': round(ms[1] / args.steps, 4),
                                 'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
@@ -328,28 +322,52 @@ def run_ccsd(args):
                         'residual': res['residual'],
                         'oracle_energy': ref,
                         'abs_dE': None if ref is None else abs(res['energy'] - ref)},
-        'mixed_precision': {'switch': 1e-6, 'f32_iterations': mixed['f32_iterations'],
-                            'f64_iterations': mixed['f64_iterations'],
-                            'seconds': round(mixed['seconds'], 4), 'energy': mixed['energy'],
-                            'residual': mixed['residual'],
-                            'abs_dE_vs_f64': abs(mixed['energy'] - res['energy']),
-                            'abs_dE_vs_oracle': None if ref is None else abs(mixed['energy'] - ref),
-                            'f32_iters_per_s': round(args.steps / wall32, 3),
-                            'f32_setup_seconds': round(f32_setup, 2),
-                            'split_bf16': {'f32_phase_iterations': mixed_bf['f32_iterations'],
-                                           'f64_iterations': mixed_bf['f64_iterations'],
-                                           'seconds': round(mixed_bf['seconds'], 4),
-                                           'abs_dE_vs_f64': abs(mixed_bf['energy'] - res['energy']),
-                                           'iters_per_s': round(args.steps / wall_bf, 3)}},
+        'time_to_convergence': {
+            'tol': args.tol, 'oracle_energy': ref,
+            'schemes': {
+                'f64': {'seconds': round(res['seconds'], 5), 'iterations': res['iterations'],
+                        'f64_evaluations': res['iterations'],
+                        'abs_dE_vs_oracle': None if ref is None else abs(res['energy'] - ref)},
+                'mixed_f32_then_f64': {'seconds': round(mixed['seconds'], 5),
+                                       'f32_iterations': mixed['f32_iterations'],
+                                       'f64_evaluations': mixed['f64_iterations'],
+                                       'abs_dE_vs_oracle': None if ref is None else abs(mixed['energy'] - ref)},
+                'refined_f32': {'seconds': round(refined['seconds'], 5),
+                                'f32_iterations': refined['f32_iterations'],
+                                'linear_iterations': refined['linear_iterations'],
+                                'f64_evaluations': refined['f64_evaluations'],
+                                'residual': refined['residual'],
+                                'abs_dE_vs_oracle': None if ref is None else abs(refined['energy'] - ref)},
+                'refined_split_bf16': {'seconds': round(refined_bf['seconds'], 5),
+                                       'f32_iterations': refined_bf['f32_iterations'],
+                                       'linear_iterations': refined_bf['linear_iterations'],
+                                       'f64_evaluations': refined_bf['f64_evaluations'],
+                                       'residual': refined_bf['residual'],
+                                       'abs_dE_vs_oracle': None if ref is None else
+                                       abs(refined_bf['energy'] - ref)}},
+            'speedup_vs_plain_f64': {k: round(res['seconds'] / v['seconds'], 3) for k, v in
+                                     (('mixed_f32_then_f64', mixed), ('refined_f32', refined),
+                                      ('refined_split_bf16', refined_bf))},
+            'f32_iters_per_s': round(args.steps / wall32, 3),
+            'split_bf16_iters_per_s': round(args.steps / wall_bf, 3),
+            'linearised_gflop_per_iter': round(solver.lin_flops / 1e9, 3),
+            'f32_setup_seconds': round(f32_setup, 2)},
         'setup_seconds': round(setup_s, 2),
     }
     if tune_report is not None:
         out['tuning'] = {'seconds': round(tune_s, 1), 'levels': tune_report}
     if not args.no_cpu:
-        v, dt, sample = cpu_ccsd_sample(ham, solver.flops)
-        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'iters/s', 'cores': 1, 'kind': 'port',
-                               'sample': sample, 'seconds': round(dt, 2),
-                               'cpu': platform.processor() or platform.machine()}
+        step('cpu baseline (one core, oracle route 3)')
+        r = cpu_ccsd_iteration(args)
+        out['cpu_baseline'] = {
+            'value': round(1.0 / r['seconds_per_iteration'], 6), 'unit': 'iters/s', 'cores': 1,
+            'kind': 'port',
+            'sample': f"one whole CCSD formula iteration ({r['kernel_calls']} kernel calls, "
+                      f"{r['gflop_per_iteration']:.2f} GFLOP) of the oracle's route 3 (collect, dgemm, "
+                      f"scatter per kernel call) with {r['blas']}, best of {r['iterations_timed']}",
+            'seconds': round(sum(r['times']), 3), 'setup_seconds': round(r['setup_seconds'], 1),
+            'gflops': round(r['gflop_per_iteration'] / r['seconds_per_iteration'], 3),
+            'cpu': r['cpu'], 'nproc': r['nproc']}
     return out
 
 

@@ -366,6 +366,190 @@ class CCSD:
         self.bulk = bulk
         self.flops, self.nblocks = flops, nblocks
 
+    # ------------------------------------------- refinement of the residual
+    def build_linear(self, bulk='f32'):
+        """Plans of the formula linearised about the current lists
+        (formula.linearize over T1, T2): base lists are the float32 lists
+        (build_f32), each list X that depends on the amplitudes gets a float32
+        delta list dX, and one pass gives dR1|dR2 = J dT1|dT2, the directional
+        derivative of the residual lists along the amplitude step in
+        dT1|dT2.  bulk: 'f32' or 'bf16x3' (split-BF16 GEMM tiles)."""
+        import gecco_hip
+        torch = self.torch
+        if getattr(self, 'buf32', None) is None:
+            self.build_f32(plans=False)
+        items, dnames = _formula.linearize(self.formula, ['T1', 'T2'])
+        lists = list(self.formula['lists'])
+        nl = len(lists)
+        base = {d: d[1:] for d in dnames}
+        self.damp32 = torch.zeros(self.n1 + self.n2, dtype=torch.float32, device=self.dev)
+        self.dres32 = torch.zeros_like(self.damp32)
+        dbuf = {}
+        for d in dnames:
+            n = base[d]
+            if n in ('T1', 'T2', 'R1', 'R2'):
+                v = self.damp32 if n[0] == 'T' else self.dres32
+                dbuf[d] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+            else:
+                dbuf[d] = torch.zeros(max(1, self.L[n].length), dtype=torch.float32, device=self.dev)
+        slot = dict(self.slot)
+        slot.update({d: nl + k for k, d in enumerate(dnames)})
+        self.lin_bufs = self.bufs32 + [dbuf[d] for d in dnames]
+        lst = lambda x: self.L[base.get(x, x)]
+        flops = 0.0
+        self.lin_plans = []
+        for lv in _formula.schedule(items):
+            p = gecco_hip.Plan()
+            p.set_precision(bulk)
+            for n in self.constant:
+                p.set_constant(slot[n])
+            for it in lv['init']:
+                if 'source' in it:
+                    p.add_init(slot[it['target']], lst(it['target']).length, buf_src=slot[it['source']],
+                               factor=it['factor'])
+                else:
+                    p.add_init(slot[it['target']], lst(it['target']).length)
+            for c in lv['contr']:
+                a, b, t = c['op1'], c['op2'], c['target']
+                nb, fl = p.add_contraction(lst(a), lst(b), lst(t), c['cnt_c'], c['cnt_a'], c['factor'],
+                                           self.use_tr, buf=(slot[a], slot[b], slot[t]))
+                flops += fl
+            if self.use_tr:
+                for x in lv['sym']:
+                    p.add_sym_ab(lst(x), slot[x])
+            p.finalize()
+            self.lin_plans.append(p)
+        self.lin_flops = flops
+        self.lin_bulk = bulk
+        self._lin_state = None      # its captured graph ran the previous plans
+
+    def _gradient64(self):
+        """One float64 evaluation at the current amplitudes: g = D t + R and
+        the energy (a float64 residual evaluation)."""
+        self.evaluate()
+        g = self.diag * self.amp + self.res
+        return g, float(self.buf['E'][0])
+
+    def _solve_linear(self, g, tol, maxit=60, ndiis=8):
+        """Solve J d = -g for the amplitude step d (float64 vector; J from
+        the linearised plans about the current lists, in the plans' bulk
+        precision): DIIS with the diagonal preconditioner on the linear
+        function g + J d = D d + (g + dR(d)).  The iteration is captured once
+        as a HIP graph (per tolerance) over persistent vectors and replayed;
+        each solve resets the subspace and starts from d = 0.  Returns (d,
+        iterations, ||g + J d||); d is overwritten by the next solve."""
+        import gecco_hip
+        torch = self.torch
+        st = getattr(self, '_lin_state', None)
+        if st is None or st['ndiis'] != ndiis:
+            n = self.amp.numel()
+            st = dict(ndiis=ndiis, delta=torch.zeros(n, dtype=torch.float64, device=self.dev),
+                      rest=torch.zeros(n, dtype=torch.float64, device=self.dev),
+                      g=torch.zeros(n, dtype=torch.float64, device=self.dev),
+                      diis=gecco_hip.DIIS(n, ndiis), graph=None)
+            self._lin_state = st
+        delta, rest, gb, diis = st['delta'], st['rest'], st['g'], st['diis']
+        gb.copy_(g)
+        delta.zero_()
+        torch.cuda.synchronize()
+        diis.reset()
+
+        def iteration(s):
+            self.damp32.copy_(delta)
+            for p in self.lin_plans:
+                p.execute(self.lin_bufs, s)
+            rest.copy_(self.dres32)
+            rest.add_(gb)
+            diis.step_async(delta, rest, self.diag, tol=tol, stream=s)
+        stream = torch.cuda.current_stream()
+        graph = st['graph'][1] if st['graph'] is not None and st['graph'][0] == tol else None
+        events = []
+        rn, it, done = float('inf'), 0, 0
+        for it in range(1, maxit + 1):
+            if graph is None:
+                iteration(stream.cuda_stream)
+                stream.synchronize()
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    iteration(torch.cuda.current_stream().cuda_stream)
+                st['graph'] = (tol, graph)
+            else:
+                graph.replay()
+            ev = torch.cuda.Event()
+            ev.record()
+            events.append(ev)
+            stop = False
+            while done < it - 1 or (it == maxit and done < it):
+                events[done].synchronize()
+                rn, _, code = diis.status(done)
+                done += 1
+                if code < 0:
+                    raise gecco_hip.GeccoHipError(f'linear DIIS matrix is singular at iteration {done}')
+                if code == 1:
+                    stop = True
+                    break
+            if stop:
+                it = done
+                break
+        torch.cuda.synchronize()
+        return delta, it, rn
+
+    def solve_refined(self, tol=1e-8, switch=1e-6, maxit=100, bulk='f32', inner_bulk=None,
+                      max_f64=4, stall=6, verbose=False):
+        """Mixed-precision solve with float64 kept for the energy, the DIIS
+        solves and the last residual evaluations:
+          1. float32 (or split-BF16) residuals with float64 DIIS until
+             ||g|| < switch or the low-precision residual stalls
+             (solve(precision='f32'));
+          2. a float64 evaluation g(t) and the energy;
+          3. the step d from J d = -g, J the formula linearised about t
+             (build_linear), solved in the low precision -- d is small, so
+             its rounding error is small in absolute terms -- then t += d;
+          4. a float64 evaluation; done when ||g|| < tol, else back to 3
+             (at most max_f64 float64 evaluations).
+        This is iterative refinement of the amplitude residual: each float64
+        evaluation corrects the low-precision Jacobian's step, and from a
+        residual near the float32 floor one correction reaches 1e-8 or
+        better, so a solve normally makes two float64 evaluations.
+        Returns the result of the last evaluation with the counts of each
+        phase."""
+        torch = self.torch
+        inner_bulk = inner_bulk or bulk
+        fresh = getattr(self, 'bulk', None) != bulk or getattr(self, 'plans32', None) is None
+        if fresh:      # new float32 lists: the linearised plans read them too
+            self.build_f32(bulk=bulk)
+            self._graph = None
+        if fresh or getattr(self, 'lin_bulk', None) != inner_bulk:
+            self.build_linear(inner_bulk)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r32 = self.solve(tol=switch, maxit=maxit, precision='f32', stall=stall, verbose=verbose)
+        inner = []
+        nf64 = 0
+        while True:
+            g, energy = self._gradient64()
+            nf64 += 1
+            gn = float(g.norm())
+            if verbose:
+                print(f'  float64 evaluation {nf64}: E = {energy:.12f}  |g| = {gn:.3e}', flush=True)
+            if gn < tol or nf64 >= max_f64:
+                break
+            # base lists: the float64 lists at t, rounded once
+            for n in self.formula['lists']:
+                if n not in self.constant and n not in ('T1', 'T2', 'R1', 'R2'):
+                    self.buf32[n].copy_(self.buf[n])
+            self.amp32.copy_(self.amp)
+            self.res32.copy_(self.res)
+            d, k, rn = self._solve_linear(g, 0.3 * tol, maxit=maxit)
+            inner.append(k)
+            self.amp.add_(d)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return dict(energy=energy, residual=gn, seconds=wall, f64_evaluations=nf64,
+                    f32_iterations=r32['iterations'], linear_iterations=inner,
+                    iterations=r32['iterations'] + sum(inner), precision='refined', bulk=bulk,
+                    inner_bulk=inner_bulk, switch=switch, converged=gn < tol)
+
     def evaluate32(self, stream=None):
         """evaluate() on the float32 lists: T1|T2 rounded to float32, the
         residual and energy returned in float64."""

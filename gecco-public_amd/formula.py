@@ -153,3 +153,57 @@ def schedule(items):
             if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
                 lv['sym'].append(res)
     return levels
+
+
+def delta_name(n):
+    return 'd' + n
+
+
+def linearize(formula, varying):
+    """The linearisation of a formula about a fixed point of its lists: for
+    every list X that depends on the `varying` lists, a list dX with
+      INIT X = 0           ->  dX = 0
+      INIT X = f * S       ->  dX = f * dS          (0 when S is fixed)
+      CONTR X += f A . B   ->  dX += f dA . B + f A . dB
+    (terms of fixed operands dropped), so that with the base lists holding
+    the formula's values at t, the delta lists hold the directional
+    derivative of every list along dt (the varying lists' deltas).  Returns
+    (items over base and delta lists, the names of the delta lists).  The
+    base lists are read only; the items schedule like any formula.
+
+    The base value of a list is its value after the whole formula, so every
+    read of a list must see that value: a formula that writes (or
+    initialises) a list after an item has read it is refused
+    (ValueError)."""
+    read = set()
+    for it in formula['items']:
+        if it['target'] in read:
+            raise ValueError(f'linearize: {it["target"]} is written after it is read')
+        if it['cmd'] == 'CONTR':
+            read.update((it['op1'], it['op2']))
+        elif it.get('source'):
+            read.add(it['source'])
+    varies = set(varying)
+    for it in formula['items']:        # lists written from varying ones
+        if it['cmd'] == 'CONTR' and (it['op1'] in varies or it['op2'] in varies):
+            varies.add(it['target'])
+        elif it['cmd'] == 'INIT' and it.get('source') in varies:
+            varies.add(it['target'])
+    items = []
+    for it in formula['items']:
+        t = it['target']
+        if t not in varies or t in varying:
+            continue
+        if it['cmd'] == 'INIT':
+            if it.get('source') in varies:
+                items.append(init(delta_name(t), delta_name(it['source']), it['factor']))
+            else:
+                items.append(init(delta_name(t)))
+        elif it['cmd'] == 'CONTR':
+            if it['op1'] in varies:
+                items.append(contr(delta_name(t), delta_name(it['op1']), it['op2'], it['cnt_c'],
+                                   it['cnt_a'], it['factor']))
+            if it['op2'] in varies:
+                items.append(contr(delta_name(t), it['op1'], delta_name(it['op2']), it['cnt_c'],
+                                   it['cnt_a'], it['factor']))
+    return items, sorted(delta_name(n) for n in varies)

@@ -224,6 +224,13 @@ def test_mixed_precision_solve_reaches_the_f64_energy():
     rb = c.solve_mixed(tol=1e-10, switch=1e-5, maxit=100, bulk='bf16x3')
     print('bf16x3 mixed', rb['f32_iterations'], '+', rb['f64_iterations'], flush=True)
     assert rb['residual'] < 1e-10 and abs(rb['energy'] - ref['energy']) < 1e-10
+    # float64 for two residual evaluations only: the float32 phase, one
+    # float64 evaluation, the step from the linearised formula in float32,
+    # the final float64 evaluation
+    rr = c.solve_refined(tol=1e-10, switch=1e-6)
+    print('refined', rr['f32_iterations'], rr['linear_iterations'], rr['f64_evaluations'], flush=True)
+    assert rr['converged'] and rr['f64_evaluations'] <= 2 and rr['residual'] < 1e-10
+    assert abs(rr['energy'] - ref['energy']) < 1e-10
     c.build_f32(bulk='f32')
     # below its floor the float32 residual stalls: the guard ends the phase
     st = c.solve(tol=1e-13, maxit=150, precision='f32', stall=5)

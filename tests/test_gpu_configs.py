@@ -57,6 +57,14 @@ def test_ccsd_full_size_matches_oracle_energy():
     print('mixed', mixed['f32_iterations'], mixed['f64_iterations'], mixed['energy'], flush=True)
     assert mixed['residual'] < 1e-8
     assert abs(mixed['energy'] - gold['energy']) <= 1e-8
+    # float64 only for the energy, the DIIS solves and two residual
+    # evaluations: float32 phase, then one linearised correction
+    for bulk in ('f32', 'bf16x3'):
+        ref = c.solve_refined(tol=1e-8, bulk=bulk)
+        print('refined', bulk, ref['f32_iterations'], ref['linear_iterations'], ref['f64_evaluations'],
+              ref['residual'], ref['energy'], flush=True)
+        assert ref['converged'] and ref['f64_evaluations'] <= 2
+        assert abs(ref['energy'] - gold['energy']) <= 1e-8
 
 
 def test_batched_instances_at_full_size():
