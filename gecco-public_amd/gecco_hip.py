@@ -183,6 +183,37 @@ def contr_blk1blk2_blocked_mm_noalloc(blk, xop1, xop2, xop1op2):
     _blocked_mm_call('contr_blk1blk2_blocked_mm_noalloc_', blk, _ptr(xop1), _ptr(xop2), _ptr(xop1op2))
 
 
+def _iocc(c, a):
+    """GeCCo's iocc(ngastp, 2) of one vertex (column 1 C, column 2 A, rows
+    H, P, V, X) from occupation dicts, as int64 in Fortran order."""
+    return np.concatenate([_occ4(c), _occ4(a)]).astype(np.int64)
+
+
+def contr_op1op2(xfac, op1, op2, op1op2, cnt, xop1, xop2, xop1op2, irreps, update=True,
+                 tra=(False, False, False), use_tr=True):
+    """One contraction with contr_op1op2's semantics through
+    gecco_hip_contr_op1op2_: op1/op2/op1op2 = (C occupation, A occupation)
+    of the operators as the contraction uses them, cnt = (contracted part of
+    op1's C side, of op1's A side), xop* the whole lists (numpy host arrays
+    or torch CUDA tensors; xop1op2 updated in place), irreps = (nsym,
+    IRREPs of the hole orbitals, IRREPs of the particle orbitals), tra the
+    dagger flags (the list stored as the adjoint)."""
+    nsym, ih, ip = irreps
+    ih = np.ascontiguousarray(ih, dtype=np.int64)
+    ip = np.ascontiguousarray(ip, dtype=np.int64)
+    occ = [_iocc(*o) for o in (op1, op2, op1op2, cnt)]
+    f = _lib.gecco_hip_contr_op1op2_
+    f.restype = None
+    ref = lambda v: ctypes.byref(I64(int(v)))
+    err = I64()
+    f(ctypes.byref(ctypes.c_double(float(xfac))), ref(bool(update)),
+      *[o.ctypes.data_as(PI64) for o in occ], ref(tra[0]), ref(tra[1]), ref(tra[2]), ref(bool(use_tr)),
+      ref(nsym), ref(len(ih)), ih.ctypes.data_as(PI64), ref(len(ip)), ip.ctypes.data_as(PI64),
+      _ptr(xop1), _ptr(xop2), _ptr(xop1op2), ctypes.byref(err))
+    if err.value:
+        raise GeccoHipError(f'gecco_hip_contr_op1op2_: error {err.value}: {last_error()[1]}')
+
+
 def call_cache(clear=False):
     """Number of one-call plans cached by the Fortran-signature entries
     (after freeing them all when clear)."""

@@ -138,6 +138,44 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
 int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *list, int32_t buf,
                               int64_t off, double prefac, double relfac);
 
+/* One binary contraction  Res (+)= xfac * Op1 . Op2  with the semantics of
+ * contr_op1op2 (contract/contr_op1op2.f:2-17, called from
+ * contract/fs_contr_drv.f:273-288), its arguments flattened from GeCCo's
+ * derived types so that a thin Fortran shim can pass them (Fortran calling
+ * convention: every argument by reference, INTEGER = int64):
+ *   xfac                 the factor (fact * bc_sign of the formula item)
+ *   update               0: the result list is zeroed first
+ *                        (contract/contr_op1op2_wmaps_c.f:644,680), else
+ *                        the contraction is added to it
+ *   iocc_op1, iocc_op2, iocc_op1op2
+ *                        occupations (ngastp = 4, 2) of the operators as the
+ *                        contraction uses them -- iocc(:,1) C over H, P, V,
+ *                        X, iocc(:,2) A -- one vertex each, H and P only
+ *   iocc_cnt             the contracted strings: (:,1) on Op1's C side,
+ *                        (:,2) on Op1's A side
+ *   tra_op1, tra_op2, tra_op1op2
+ *                        the dagger flags: the list is stored as the adjoint
+ *                        (its C and A occupations swapped)
+ *   use_tr               spin-flip symmetry (the use_tr keyword), with the
+ *                        result's symmetrisation
+ *   nsym, norb_h, igam_h[norb_h], norb_p, igam_p[norb_p]
+ *                        IRREPs of the hole and particle orbitals (ordered by
+ *                        IRREP), from orb_info
+ *   xop1, xop2, xop1op2  the whole lists in GeCCo's layout, host or device
+ *                        memory (host buffers are staged)
+ *   ierr                 0, or a GECCO_HIP_E* code (gecco_hip_last_error
+ *                        has the message; the shim turns it into quit).
+ * Every distinct contraction keeps its lists and plan for the life of the
+ * process.  The kernels run on the null stream; a host result is copied back
+ * before the call returns. */
+void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const int64_t *iocc_op1,
+                             const int64_t *iocc_op2, const int64_t *iocc_op1op2,
+                             const int64_t *iocc_cnt, const int64_t *tra_op1, const int64_t *tra_op2,
+                             const int64_t *tra_op1op2, const int64_t *use_tr, const int64_t *nsym,
+                             const int64_t *norb_h, const int64_t *igam_h, const int64_t *norb_p,
+                             const int64_t *igam_p, const double *xop1, const double *xop2,
+                             double *xop1op2, int64_t *ierr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,3 +116,25 @@ def test_model_enumerator_counts_match_oracle():
                 nb, fl = p.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, use_tr)
                 assert nb == len(c.blocks(use_tr)), (name, nsym, use_tr)
                 assert fl == c.flops(use_tr)
+
+
+def test_fortran_caller_links_the_library():
+    """The F77-style test caller (tests/fortran/contr_driver.f, built with
+    flang and 8-byte default INTEGER by __graft_entry__.build()) leaves the
+    reference's kernel names and the flattened contr_op1op2 entry undefined,
+    and libgecco_hip.so exports them: GeCCo's own calls bind unchanged."""
+    import subprocess
+    import gecco_hip
+    import pytest
+    driver = os.path.join(ROOT, 'gecco-public_amd', 'fortran', '_build', 'contr_driver')
+    if not os.path.exists(driver):
+        pytest.fail('tests/fortran/contr_driver.f is not built (__graft_entry__.build())')
+    und = subprocess.run(['nm', '-u', driver], capture_output=True, text=True).stdout.split()
+    lib = ctypes.CDLL(gecco_hip.LIBPATH)
+    for sym in ('contr_blk1blk2_blocked_mm_', 'contr_blk1blk2_wmaps_c_', 'gecco_hip_contr_op1op2_',
+                'gecco_hip_last_error'):
+        assert sym in und, sym
+        assert hasattr(lib, sym), sym
+    mods = os.path.join(ROOT, 'gecco-public_amd', 'fortran', '_build')
+    assert os.path.exists(os.path.join(mods, 'gecco_hip_f.mod'))
+    assert os.path.exists(os.path.join(mods, 'gecco_hip_kernels_f.mod'))

@@ -632,3 +632,42 @@ def test_fortran_entry_reuses_its_plans():
     assert gecco_hip.call_cache() == 2 * len(blocks)
     assert _relerr(dr.cpu().numpy(), 2.5 * ref) <= TOL
     assert gecco_hip.call_cache(clear=True) == 0
+
+
+@pytest.mark.parametrize('name', sorted(CASES) + ['energy_t1', 'ring_t2', 't1_ovoo_t2', 'ladder_t12',
+                                                  'fock_t12', 'vvvo_t1'])
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (4, 6, 10)])
+def test_contr_op1op2_entry_matches_oracle(name, nsym, nocc, nvir):
+    """gecco_hip_contr_op1op2_ (contr_op1op2's arguments flattened:
+    occupations, dagger flags, update, use_tr, orbital IRREPs, whole lists)
+    on host arrays and on device tensors, accumulating (update) and
+    overwriting the result, against the oracle's contr_op1op2
+    restatement."""
+    torch = _torch()
+    import gecco_hip
+    from models import TRA_CASES, round_robin
+    print(f'contr_op1op2 {name} nsym={nsym} nocc={nocc} nvir={nvir}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    if name in CASES:
+        c, a, b, r = CASES[name](orbs)
+        stored, tra = (a, b, r), (False, False, False)
+    else:
+        c, stored, tra = TRA_CASES[name](orbs)
+    used = (c.op1, c.op2, c.res)
+    irreps = (nsym, round_robin(nocc, nsym), round_robin(nvir, nsym, 1))
+    xs = [spin_symmetric(m, 51 + k) for k, m in enumerate(stored)]
+    for update in (True, False):
+        ref = xs[2].copy() if update else np.zeros(stored[2].length)
+        c.run(xs[0], xs[1], ref, route=3, use_tr=True, screen=False)
+        args = (0.75, (used[0].occ_c, used[0].occ_a), (used[1].occ_c, used[1].occ_a),
+                (used[2].occ_c, used[2].occ_a), (c.cnt_c, c.cnt_a))
+        host = xs[2].copy()
+        ref75 = xs[2] + 0.75 * (ref - xs[2]) if update else 0.75 * ref
+        for rep in range(2):        # the second call reuses the kept plan
+            host = xs[2].copy()
+            gecco_hip.contr_op1op2(*args, xs[0], xs[1], host, irreps, update=update, tra=tra)
+            assert _relerr(host, ref75) <= TOL, (update, rep)
+        dev = [torch.from_numpy(x).cuda() for x in xs]
+        gecco_hip.contr_op1op2(*args, dev[0], dev[1], dev[2], irreps, update=update, tra=tra)
+        torch.cuda.synchronize()
+        assert _relerr(dev[2].cpu().numpy(), ref75) <= TOL
