@@ -24,7 +24,7 @@ struct OrbsEntry {
 
 struct CallEntry {
     gecco_hip_plan *plan = nullptr;
-    std::vector<gecco_hip_list *> views;    // adjoint views owned by the entry
+    std::vector<gecco_hip_list *> views;    // adjoint views (or reorder lists) owned by the entry
     int64_t len1 = 0, len2 = 0, len12 = 0;
 };
 
@@ -162,6 +162,80 @@ void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const in
     double *bufs[3] = {d1, d2, d12};
     if (!rc) rc = gecco_hip_plan_execute(ce.plan, bufs, 3, nullptr);
     if (!rc && h12) rc = gecco_hip_memcpy_dtoh(xop1op2, d12, ce.len12 * 8);
+    if (rc) {
+        *ierr = gecco_hip_last_error(nullptr, 0);
+        if (!*ierr) *ierr = GECCO_HIP_EDEVICE;
+    }
+}
+
+void gecco_hip_reo_op_(const double *fac, const int64_t *update, const int64_t *sign_reo,
+                       const int64_t *njoined_ori, const int64_t *iocc_ori, const int64_t *njoined_reo,
+                       const int64_t *iocc_reo, const int64_t *from_vtx, const int64_t *to_vtx,
+                       const int64_t *iocc_shift, const int64_t *nsym, const int64_t *norb_h,
+                       const int64_t *igam_h, const int64_t *norb_p, const int64_t *igam_p,
+                       const double *xori, double *xreo, int64_t *ierr) {
+    *ierr = 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::vector<int64_t> okey{*nsym, *norb_h, *norb_p};
+    okey.insert(okey.end(), igam_h, igam_h + *norb_h);
+    okey.insert(okey.end(), igam_p, igam_p + *norb_p);
+    OrbsEntry &oe = g_orbs[okey];
+    if (!oe.orbs && gecco_hip_orbs_create(*nsym, *norb_h, igam_h, *norb_p, igam_p, &oe.orbs)) {
+        *ierr = GECCO_HIP_EINVAL;
+        return;
+    }
+    const int64_t nj[2] = {*njoined_ori, *njoined_reo};
+    if (nj[0] < 1 || nj[0] > 8 || nj[1] < 1 || nj[1] > 8) {
+        *ierr = GECCO_HIP_EINVAL;
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    uint64_t xbits;
+    std::memcpy(&xbits, fac, 8);
+    std::vector<int64_t> ckey(okey);
+    ckey.insert(ckey.end(), {-1, (int64_t)dev, (int64_t)xbits, *update != 0, *sign_reo, *from_vtx,
+                             *to_vtx, nj[0], nj[1]});
+    ckey.insert(ckey.end(), iocc_ori, iocc_ori + 8 * nj[0]);
+    ckey.insert(ckey.end(), iocc_reo, iocc_reo + 8 * nj[1]);
+    ckey.insert(ckey.end(), iocc_shift, iocc_shift + 8);
+    CallEntry &ce = g_calls[ckey];
+    if (!ce.plan) {
+        // lists of several vertices: iocc(4, 2, njoined) is the vertex-wise
+        // (C, A) occupation, the layout gecco_hip_list_create_n takes
+        gecco_hip_list *L[2] = {nullptr, nullptr};
+        const int64_t *occ[2] = {iocc_ori, iocc_reo};
+        int rc = 0;
+        for (int q = 0; q < 2 && !rc; ++q) rc = gecco_hip_list_create_n(oe.orbs, nj[q], occ[q], 1, &L[q]);
+        gecco_hip_plan *p = nullptr;
+        if (!rc) rc = gecco_hip_plan_create(&p);
+        const int64_t lr = rc ? 0 : gecco_hip_list_length(L[1]);
+        if (!rc && !*update) rc = gecco_hip_plan_add_init(p, 1, 0, lr, -1, 0, 0.0);
+        int64_t nt = 0;
+        if (!rc)
+            rc = gecco_hip_plan_add_reorder(p, L[0], L[1], *from_vtx - 1, *to_vtx - 1, iocc_shift,
+                                            iocc_shift + 4, *fac, *sign_reo, 0, 0, 1, 0, &nt);
+        if (!rc) rc = gecco_hip_plan_finalize(p);
+        if (rc) {
+            if (p) gecco_hip_plan_destroy(p);
+            for (auto *l : L) if (l) gecco_hip_list_destroy(l);
+            g_calls.erase(ckey);
+            *ierr = rc;
+            return;
+        }
+        ce.plan = p;
+        ce.len1 = gecco_hip_list_length(L[0]);
+        ce.len12 = lr;
+        ce.views.assign(L, L + 2);
+    }
+    double *d1 = const_cast<double *>(xori), *d2 = xreo;
+    const bool h1 = !on_device(xori), h2 = !on_device(xreo);
+    int rc = 0;
+    if (h1 && !rc) rc = stage(0, ce.len1, &d1) || gecco_hip_memcpy_htod(d1, xori, ce.len1 * 8);
+    if (h2 && !rc) rc = stage(2, ce.len12, &d2) || gecco_hip_memcpy_htod(d2, xreo, ce.len12 * 8);
+    double *bufs[2] = {d1, d2};
+    if (!rc) rc = gecco_hip_plan_execute(ce.plan, bufs, 2, nullptr);
+    if (!rc && h2) rc = gecco_hip_memcpy_dtoh(xreo, d2, ce.len12 * 8);
     if (rc) {
         *ierr = gecco_hip_last_error(nullptr, 0);
         if (!*ierr) *ierr = GECCO_HIP_EDEVICE;

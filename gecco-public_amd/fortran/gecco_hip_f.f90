@@ -30,6 +30,20 @@ module gecco_hip_f
       integer(c_int64_t), intent(out) :: ierr
     end subroutine
 
+    ! reorder of a whole list with reo_op_wmaps_c's semantics, flattened
+    ! (include/gecco_hip_model.h); vertices numbered from 1
+    subroutine gecco_hip_reo_op(fac, update, sign_reo, njoined_ori, iocc_ori, &
+         njoined_reo, iocc_reo, from_vtx, to_vtx, iocc_shift, nsym, norb_h, igam_h, &
+         norb_p, igam_p, xori, xreo, ierr) bind(C, name='gecco_hip_reo_op_')
+      import :: c_double, c_int64_t
+      real(c_double), intent(in) :: fac, xori(*)
+      real(c_double), intent(inout) :: xreo(*)
+      integer(c_int64_t), intent(in) :: update, sign_reo, njoined_ori, iocc_ori(4, 2, *), &
+           njoined_reo, iocc_reo(4, 2, *), from_vtx, to_vtx, iocc_shift(4, 2), nsym, &
+           norb_h, igam_h(*), norb_p, igam_p(*)
+      integer(c_int64_t), intent(out) :: ierr
+    end subroutine
+
     ! the allocation-free route-3 call (device operands only)
     subroutine contr_blk1blk2_blocked_mm_noalloc(xfac, xop1op2, xop1, xop2, &
          tra_op1, tra_op2, tra_op1op2, xscr, lenscr, lenblock, lenblock_cnt, &

@@ -214,6 +214,30 @@ def contr_op1op2(xfac, op1, op2, op1op2, cnt, xop1, xop2, xop1op2, irreps, updat
         raise GeccoHipError(f'gecco_hip_contr_op1op2_: error {err.value}: {last_error()[1]}')
 
 
+def reo_op(fac, ori, reo, frm, to, shift, xori, xreo, irreps, update=True, sign_reo=1):
+    """Reorder of a whole list with reo_op_wmaps_c's semantics through
+    gecco_hip_reo_op_: ori/reo = (C occupations, A occupations) per vertex
+    (lists of dicts), the shift string (C dict, A dict) moved from vertex frm
+    to vertex to (0-based here), xori/xreo whole lists (xreo updated in
+    place), irreps as for contr_op1op2."""
+    nsym, ih, ip = irreps
+    ih = np.ascontiguousarray(ih, dtype=np.int64)
+    ip = np.ascontiguousarray(ip, dtype=np.int64)
+    occ = [np.concatenate([_iocc(c, a) for c, a in zip(*o)]) for o in (ori, reo)]
+    sh = _iocc(*shift)
+    f = _lib.gecco_hip_reo_op_
+    f.restype = None
+    ref = lambda v: ctypes.byref(I64(int(v)))
+    err = I64()
+    f(ctypes.byref(ctypes.c_double(float(fac))), ref(bool(update)), ref(sign_reo),
+      ref(len(ori[0])), occ[0].ctypes.data_as(PI64), ref(len(reo[0])), occ[1].ctypes.data_as(PI64),
+      ref(frm + 1), ref(to + 1), sh.ctypes.data_as(PI64), ref(nsym), ref(len(ih)),
+      ih.ctypes.data_as(PI64), ref(len(ip)), ip.ctypes.data_as(PI64), _ptr(xori), _ptr(xreo),
+      ctypes.byref(err))
+    if err.value:
+        raise GeccoHipError(f'gecco_hip_reo_op_: error {err.value}: {last_error()[1]}')
+
+
 def call_cache(clear=False):
     """Number of one-call plans cached by the Fortran-signature entries
     (after freeing them all when clear)."""

@@ -176,6 +176,22 @@ void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const in
                              const int64_t *igam_p, const double *xop1, const double *xop2,
                              double *xop1op2, int64_t *ierr);
 
+/* Reorder of a whole list with the semantics of reo_op_wmaps_c
+ * (contract/reo_op_wmaps_c.f:2-9, called from contract/fs_reo_drv.f:166),
+ * its derived-type arguments flattened as for gecco_hip_contr_op1op2_:
+ *   REO (+)= fac * sign_reo * (ORI with the shift string moved from vertex
+ *   from_vtx to vertex to_vtx), through gecco_hip_plan_add_reorder;
+ *   iocc_ori(4, 2, njoined_ori), iocc_reo(4, 2, njoined_reo): vertex-wise
+ *   C and A occupations over H, P, V, X; iocc_shift(4, 2): the moved string
+ *   (reo_info's shift occupation); vertices numbered from 1; update 0 zeroes
+ *   REO first; orbital IRREPs, buffers and ierr as gecco_hip_contr_op1op2_. */
+void gecco_hip_reo_op_(const double *fac, const int64_t *update, const int64_t *sign_reo,
+                       const int64_t *njoined_ori, const int64_t *iocc_ori, const int64_t *njoined_reo,
+                       const int64_t *iocc_reo, const int64_t *from_vtx, const int64_t *to_vtx,
+                       const int64_t *iocc_shift, const int64_t *nsym, const int64_t *norb_h,
+                       const int64_t *igam_h, const int64_t *norb_p, const int64_t *igam_p,
+                       const double *xori, double *xreo, int64_t *ierr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -540,6 +540,22 @@ def test_reorder_matches_oracle(name, nsym):
     p.execute(bufs)
     torch.cuda.synchronize()
     assert _relerr(bufs[1].cpu().numpy(), ref) <= TOL
+    # the same through the flattened reo_op_wmaps_c entry, host and device
+    # buffers, accumulating and overwriting
+    irreps = (nsym, round_robin(3, nsym), round_robin(6, nsym, 1))
+    vocc = lambda c, a: ([c] if isinstance(c, dict) else c, [a] if isinstance(a, dict) else a)
+    y = y0.copy()
+    gecco_hip.reo_op(0.25, vocc(oc, oa), vocc(rc, ra), frm, to, (kc, ka), x, y, irreps, sign_reo=-1)
+    assert _relerr(y, ref) <= TOL
+    y = y0.copy()
+    gecco_hip.reo_op(0.25, vocc(oc, oa), vocc(rc, ra), frm, to, (kc, ka), x, y, irreps, update=False,
+                     sign_reo=-1)
+    assert _relerr(y, ref - y0) <= TOL
+    dy = torch.from_numpy(y0).to(dev)
+    gecco_hip.reo_op(0.25, vocc(oc, oa), vocc(rc, ra), frm, to, (kc, ka), bufs[0], dy, irreps,
+                     sign_reo=-1)
+    torch.cuda.synchronize()
+    assert _relerr(dy.cpu().numpy(), ref) <= TOL
 
 
 @pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 6, 10)])
