@@ -163,16 +163,29 @@ class CCSD:
         self.orbs = gecco_hip.Orbs(ham.irrep_o, ham.irrep_v, ham.nsym)
         self.L = {n: gecco_hip.MEList(self.orbs, c, a) for n, (c, a) in lists.items()}
         self.slot = {n: i for i, n in enumerate(lists)}
-        # T1|T2 and R1|R2 are contiguous: the amplitude vector and the
-        # residual (without its diagonal part) of the solver
+        # T1|T2|... and R1|R2|... are contiguous: the amplitude vector and
+        # the residual (without its diagonal part) of the solver; a formula
+        # with T3 (CCSDT) adds T3/R3 behind them
+        self.tnames = [f'T{k}' for k in range(1, 10) if f'T{k}' in lists]
+        self.rnames = [f'R{k}' for k in range(1, len(self.tnames) + 1)]
+        for n in self.rnames:
+            if n not in lists:
+                raise ValueError(f'formula lacks the list {n}')
         self.n1, self.n2 = self.L['T1'].length, self.L['T2'].length
-        self.amp = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
-        self.res = torch.zeros(self.n1 + self.n2, dtype=torch.float64, device=self.dev)
+        sizes = [self.L[n].length for n in self.tnames]
+        self.seg = {}
+        o = 0
+        for k, n in enumerate(self.tnames):
+            self.seg[n] = self.seg[self.rnames[k]] = (o, o + sizes[k])
+            o += sizes[k]
+        self.namp = o
+        self.amp = torch.zeros(o, dtype=torch.float64, device=self.dev)
+        self.res = torch.zeros(o, dtype=torch.float64, device=self.dev)
         self.buf = {}
         for n in lists:
-            if n in ('T1', 'T2', 'R1', 'R2'):
+            if n in self.seg:
                 v = self.amp if n[0] == 'T' else self.res
-                self.buf[n] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+                self.buf[n] = v[self.seg[n][0]:self.seg[n][1]]
             else:
                 self.buf[n] = torch.zeros(max(1, self.L[n].length), dtype=torch.float64,
                                           device=self.dev)
@@ -323,7 +336,7 @@ class CCSD:
         """Diagonal preconditioner of T1|T2 (dia4op_ev): eps of the C strings
         minus eps of the A strings, built by the library on the host."""
         ham = self.ham
-        d = np.concatenate([self.L[n].diagonal(ham.eps_o, ham.eps_v) for n in ('T1', 'T2')])
+        d = np.concatenate([self.L[n].diagonal(ham.eps_o, ham.eps_v) for n in self.tnames])
         self.diag = self.torch.from_numpy(d).to(self.dev)
 
     # ---------------------------------------------------------- iteration
@@ -343,13 +356,13 @@ class CCSD:
         import gecco_hip
         torch = self.torch
         lists = self.formula['lists']
-        self.amp32 = torch.zeros(self.n1 + self.n2, dtype=torch.float32, device=self.dev)
+        self.amp32 = torch.zeros(self.namp, dtype=torch.float32, device=self.dev)
         self.res32 = torch.zeros_like(self.amp32)
         self.buf32 = {}
         for n in lists:
-            if n in ('T1', 'T2', 'R1', 'R2'):
+            if n in self.seg:
                 v = self.amp32 if n[0] == 'T' else self.res32
-                self.buf32[n] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+                self.buf32[n] = v[self.seg[n][0]:self.seg[n][1]]
             else:
                 self.buf32[n] = self.buf[n].to(torch.float32)
         self.bufs32 = [self.buf32[n] for n in lists]
@@ -378,18 +391,18 @@ class CCSD:
         torch = self.torch
         if getattr(self, 'buf32', None) is None:
             self.build_f32(plans=False)
-        items, dnames = _formula.linearize(self.formula, ['T1', 'T2'])
+        items, dnames = _formula.linearize(self.formula, self.tnames)
         lists = list(self.formula['lists'])
         nl = len(lists)
         base = {d: d[1:] for d in dnames}
-        self.damp32 = torch.zeros(self.n1 + self.n2, dtype=torch.float32, device=self.dev)
+        self.damp32 = torch.zeros(self.namp, dtype=torch.float32, device=self.dev)
         self.dres32 = torch.zeros_like(self.damp32)
         dbuf = {}
         for d in dnames:
             n = base[d]
-            if n in ('T1', 'T2', 'R1', 'R2'):
+            if n in self.seg:
                 v = self.damp32 if n[0] == 'T' else self.dres32
-                dbuf[d] = v[:self.n1] if n[1] == '1' else v[self.n1:]
+                dbuf[d] = v[self.seg[n][0]:self.seg[n][1]]
             else:
                 dbuf[d] = torch.zeros(max(1, self.L[n].length), dtype=torch.float32, device=self.dev)
         slot = dict(self.slot)
@@ -536,7 +549,7 @@ class CCSD:
                 break
             # base lists: the float64 lists at t, rounded once
             for n in self.formula['lists']:
-                if n not in self.constant and n not in ('T1', 'T2', 'R1', 'R2'):
+                if n not in self.constant and n not in self.seg:
                     self.buf32[n].copy_(self.buf[n])
             self.amp32.copy_(self.amp)
             self.res32.copy_(self.res)

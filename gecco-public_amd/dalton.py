@@ -116,14 +116,19 @@ def read_motwoint(path, n):
     return eri, count
 
 
-def hamiltonian_from_dalton(sirifc, motwoint, freeze_below=FREEZE_BELOW, check_tol=1e-9):
+def hamiltonian_from_dalton(sirifc, motwoint, freeze_below=FREEZE_BELOW, check_tol=1e-9, frozen=None,
+                            deleted=None):
     """Closed-shell DenseHamiltonian for the CCSD driver from SIRIFC and
     MOTWOINT.  The SCF energy rebuilt from the Fock diagonal and the
     integrals, E = E_nuc + sum_i (2 f_ii - sum_j (2 (ii|jj) - (ij|ji))), and
     the Fock diagonal against SIRIFC's orbital energies check the decoding
     (ValueError beyond check_tol).  Occupied orbitals of energy below
-    freeze_below are frozen; the correlated orbitals are the other occupied
-    ones and the virtual ones, in file (IRREP) order."""
+    freeze_below are frozen, or, with frozen = counts per IRREP, the lowest
+    frozen[s] orbitals of IRREP s (the orb_space shell type=frozen,def=(...)
+    of the reference's input); deleted = counts per IRREP drops the highest
+    deleted[s] orbitals of IRREP s (shell type=deleted).  The correlated
+    orbitals are the other occupied ones and the remaining virtual ones, in
+    file (IRREP) order."""
     sir = read_sirifc(sirifc)
     norb, nrhf, nsym = sir['norb'], sir['nrhf'], sir['nsym']
     n = int(norb.sum())
@@ -140,9 +145,19 @@ def hamiltonian_from_dalton(sirifc, motwoint, freeze_below=FREEZE_BELOW, check_t
                          f'{sir["emcscf"]:.12f}')
     if np.abs(np.diag(F) - sir['eorb']).max() > check_tol:
         raise ValueError('Fock diagonal differs from the orbital energies')
-    frozen = [i for i in occ if F[i, i] < freeze_below]
+    start = [int(norb[:s].sum()) for s in range(nsym)]
+    if frozen is None:
+        frozen = [i for i in occ if F[i, i] < freeze_below]
+    else:
+        frozen = [start[s] + k for s in range(nsym) for k in range(int(frozen[s]) if s < len(frozen) else 0)]
+    gone = set()
+    if deleted is not None:
+        gone = {start[s] + int(norb[s]) - 1 - k for s in range(min(nsym, len(deleted)))
+                for k in range(int(deleted[s]))}
+        if gone & set(occ):
+            raise ValueError('deleted orbitals overlap the occupied ones')
     act_o = [int(i) for i in occ if i not in frozen]
-    act_v = [p for p in range(n) if p not in set(occ)]
+    act_v = [p for p in range(n) if p not in set(occ) and p not in gone]
     act = act_o + act_v
     return DenseHamiltonian(np.diag(F)[act_o], np.diag(F)[act_v], irrep[act_o], irrep[act_v],
                             eri[np.ix_(act, act, act, act)], nsym, sir['emcscf'])

@@ -103,8 +103,11 @@ class Route3CCSD:
         self.L = {k: MEList(orbs, c, a) for k, (c, a) in formula['lists'].items()}
         self.constant = list(formula['constant'])
         self.X = integral_lists(self.L, ham, self.constant)
-        self.D = np.concatenate([diagonal(self.L[n], ham) for n in ('T1', 'T2')])
-        self.n1 = self.L['T1'].length
+        # amplitude lists T1, T2, ... (as many as the formula has)
+        self.tnames = [f'T{k}' for k in range(1, 10) if f'T{k}' in self.L]
+        self.rnames = [f'R{k}' for k in range(1, len(self.tnames) + 1)]
+        self.D = np.concatenate([diagonal(self.L[n], ham) for n in self.tnames])
+        self.bounds = np.cumsum([0] + [self.L[n].length for n in self.tnames])
         self.levels = []
         self.flops = 0.0
         self.calls = 0
@@ -120,12 +123,13 @@ class Route3CCSD:
                 cs.append((it['target'], it['op1'], it['op2'], c, blocks))
             self.levels.append((lv['init'], cs, lv['sym']))
         self.pairs = {n: sym_ab_pairs(self.L[n], 1.0) for _, _, sym in self.levels for n in sym}
-        for n in ('T1', 'T2'):
+        for n in self.tnames:
             self.X[n] = np.zeros(self.L[n].length)
 
     def evaluate(self):
-        """One pass of the formula for the amplitudes in X['T1'], X['T2']:
-        returns the residual Omega = R - D t (the function g of the solver)."""
+        """One pass of the formula for the amplitudes in X['T1'], X['T2'],
+        ...: returns the residual Omega = R - D t (the function g of the
+        solver) over all ranks."""
         X, L = self.X, self.L
         for inits, cs, sym in self.levels:
             for it in inits:
@@ -135,8 +139,8 @@ class Route3CCSD:
                 c.run(X[a], X[b], X[res], route=3, use_tr=False, screen=False, blocks=blocks)
             for n in sym:
                 apply_sym_pairs(X[n], self.pairs[n], 0.5)
-        t = np.concatenate([X['T1'], X['T2']])
-        return np.concatenate([X['R1'], X['R2']]) - self.D * t
+        t = np.concatenate([X[n] for n in self.tnames])
+        return np.concatenate([X[n] for n in self.rnames]) - self.D * t
 
     def solve(self, tol=1e-10, maxit=200, ndiis=8, log=None):
         X = self.X
@@ -150,7 +154,7 @@ class Route3CCSD:
                 log(f'it {it:3d} E {energy:.14f} |R| {rn:.3e} ({time.time() - t0:.0f} s)')
             if rn < tol:
                 return energy, it, rn
-            t = np.concatenate([X['T1'], X['T2']])
+            t = np.concatenate([X[n] for n in self.tnames])
             step = omega / self.D
             hx.append(t + step)
             he.append(step)
@@ -165,5 +169,6 @@ class Route3CCSD:
             r[m] = -1.0
             coef = np.linalg.solve(B, r)[:m]
             new = sum(ci * xi for ci, xi in zip(coef, hx))
-            X['T1'], X['T2'] = new[:self.n1].copy(), new[self.n1:].copy()
+            for k, n in enumerate(self.tnames):
+                X[n] = new[self.bounds[k]:self.bounds[k + 1]].copy()
         return energy, maxit, rn

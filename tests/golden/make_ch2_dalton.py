@@ -17,6 +17,7 @@ import os
 REF = '/root/reference/test/molecules/CH2-DZ-DALTON'
 HERE = os.path.dirname(os.path.abspath(__file__))
 E_CCSD_REF = -38.907618798245        # test/CCPT-energy.e.ref:690
+E_CCSDT_REF = -38.859570692641       # test/CC-energy-I.e.ref:606
 
 
 def main():
@@ -24,7 +25,10 @@ def main():
         data = open(os.path.join(REF, name), 'rb').read()
         with gzip.GzipFile(os.path.join(HERE, f'ch2_dz_{name}.gz'), 'wb', mtime=0) as f:
             f.write(data)
-    json.dump({'e_ccsd_ref': E_CCSD_REF, 'source': 'test/CCPT-energy.e.ref:690 (CC energy)'},
+    json.dump({'e_ccsd_ref': E_CCSD_REF, 'source': 'test/CCPT-energy.e.ref:690 (CC energy)',
+               'e_ccsdt_ref': E_CCSDT_REF,
+               'e_ccsdt_source': 'test/CC-energy-I.e.ref:606 (CC maxexc=3, frozen (1,0,0,0), '
+                                 'deleted (2,1,0,0))'},
               open(os.path.join(HERE, 'ch2_dz.json'), 'w'), indent=1)
     print('wrote', HERE)
 

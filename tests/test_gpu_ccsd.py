@@ -241,3 +241,33 @@ def test_mixed_precision_solve_reaches_the_f64_energy():
     r64 = c.res.clone()
     c.evaluate32()
     assert float((c.res - r64).abs().max()) <= 1e-5 * float(r64.abs().max())
+
+
+def test_ccsdt_ch2_dz_matches_reference_energy():
+    """CCSDT on the engine: the generated T1/T2/T3 formula (ccgen.generate(3))
+    for CH2/DZ with frozen (1,0,0,0) and deleted (2,1,0,0) orbitals gives the
+    reference's known answer -38.859570692641 Eh (test/CC-energy-I.e.ref:606)
+    in the float64 solve, and the refined mixed-precision solve (float32
+    phase, two float64 residual evaluations) gives the same energy."""
+    import json
+    import os
+    _torch()
+    import ccgen
+    import ccsd
+    from dalton import hamiltonian_from_dalton
+    gold = os.path.join(os.path.dirname(__file__), 'golden')
+    ham = hamiltonian_from_dalton(os.path.join(gold, 'ch2_dz_SIRIFC.gz'),
+                                  os.path.join(gold, 'ch2_dz_MOTWOINT.gz'), frozen=(1, 0, 0, 0),
+                                  deleted=(2, 1, 0, 0))
+    ref = json.load(open(os.path.join(gold, 'ch2_dz.json')))['e_ccsdt_ref']
+    print('ch2/dz ccsdt', ham.nocc, ham.nvir, flush=True)
+    c = ccsd.CCSD(ham, formula=ccgen.generate(3))
+    assert c.tnames == ['T1', 'T2', 'T3']
+    res = c.solve(tol=1e-11, maxit=100)
+    print('ccsdt', res['iterations'], ham.e_ref + res['energy'], flush=True)
+    assert res['residual'] < 1e-11
+    assert abs(ham.e_ref + res['energy'] - ref) < 1e-10
+    rr = c.solve_refined(tol=1e-10, switch=1e-6)
+    print('ccsdt refined', rr['f32_iterations'], rr['linear_iterations'], rr['f64_evaluations'], flush=True)
+    assert rr['converged'] and rr['f64_evaluations'] <= 2
+    assert abs(ham.e_ref + rr['energy'] - ref) < 1e-10
