@@ -532,11 +532,34 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
     const T *__restrict__ pa = bufs[b.buf_a] + b.base_a + oa.f0 + oa.s0;
     const T *__restrict__ pb = bufs[b.buf_b] + b.base_b + ob.f0 + ob.s0;
     T *pc = bufs[b.buf_c] + b.base_c;
+    // The A tile (K * dm <= DIRECT_A = 4 NT elements) and the first DB
+    // values of the thread's B column: every load in flight at once, issued
+    // before the barrier that frees the LDS of the previous task (these tasks
+    // are memory-bound; a loop of load-then-store had kept one load in flight
+    // per thread and left them waiting on latency).
+    constexpr int DB = 16;
+    const bool colok = active && n < N;
+    const T *__restrict__ pbn = pb + ob.s1 * (int64_t)n;
+    T bv[DB];
+#pragma unroll
+    for (int u = 0; u < DB; ++u) bv[u] = colok && u < K ? ldg(pbn + ob.f1 * (int64_t)u) : (T)0;
+    constexpr int AU = DIRECT_A / NT;
+    const int KD = K * dm;
+    T av[AU];
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+        const int e = tid + u * NT;
+        av[u] = (T)0;
+        if (e < KD) {
+            const int k = e / dm, m = m0 + (e - k * dm);
+            if (m < M) av[u] = ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m));
+        }
+    }
     __syncthreads();   // LDS of the previous task is no longer read
-    for (int e = tid; e < K * dm; e += NT) {
-        const int k = e / dm, r = e - k * dm;
-        const int m = m0 + r;
-        As[e] = m < M ? ldg(pa + (oa.f1 * (int64_t)k + oa.s1 * (int64_t)m)) : (T)0;
+#pragma unroll
+    for (int u = 0; u < AU; ++u) {
+        const int e = tid + u * NT;
+        if (e < KD) As[e] = av[u];
     }
     const int span = task.span;
     for (int i = tid; i < span; i += NT) Cs[i] = (T)0;
@@ -559,13 +582,27 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc[i] = (T)0;
     if (cok) {
-        const T *__restrict__ pbn = pb + ob.s1 * (int64_t)n;
-#pragma unroll 4
-        for (int k = 0; k < K; ++k) {
-            const T bk = ldg(pbn + ob.f1 * (int64_t)k);
+        // the B column in rounds of DB loads in flight (the first round is
+        // in bv already); per round the rows one after another (a scheduling
+        // barrier per row keeps the compiler from hoisting every row's LDS
+        // reads of the round at once, which took the kernel past its
+        // register budget)
+        for (int k0 = 0; k0 < K; k0 += DB) {
+            if (k0 > 0) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (i < rpt) acc[i] = fma(As[k * dm + ml + ng * i], bk, acc[i]);
+                for (int u = 0; u < DB; ++u) bv[u] = k0 + u < K ? ldg(pbn + ob.f1 * (int64_t)(k0 + u)) : (T)0;
+            }
+            const int ku = min(DB, K - k0);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < rpt) {
+                    const T *ar = As + k0 * dm + ml + ng * i;
+#pragma unroll
+                    for (int u = 0; u < DB; ++u)
+                        if (u < ku) acc[i] = fma(ar[u * dm], bv[u], acc[i]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     // results: into the LDS stage (span > 0), else non-returning atomic adds
@@ -634,12 +671,13 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
 // beyond the block -- M = 100 in 64-row tiles: rows 112..127 -- skips its
 // MFMAs).
 // Waves per SIMD the register allocation aims at: 64 x 64 x 16 tiles with
-// one K step in flight fit 6 in float32 (78 registers) and 4 in float64 (the
-// float64 quarter-tile layout reaches 4 without the hint).
+// one K step in flight fit 6 in float32 (80 registers) and 4 in float64 (104
+// registers; without the hint the direct tasks' batched loads would take the
+// float64 kernel to 133 registers and 3 waves).
 template <int TBM, int TBN, int TBK, typename T, int PF, int WM>
 constexpr int AFFINE_WAVES = TBM * TBN >= 128 * 128 ? 2
                            : !(TBM == 64 && TBN == 64 && TBK == 16 && PF == 1) ? 1
-                           : sizeof(T) == 4 ? 6 : WM == 4 ? 4 : 1;
+                           : sizeof(T) == 4 ? 6 : 4;
 
 template <int TBM, int TBN, int TBK, typename T, int PF = 1, int WM = 2>
 __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void grouped_contract_affine(

@@ -756,8 +756,59 @@ static void xcd_task_order(int64_t n, const int32_t *patch, const int64_t *cost,
     }
 }
 
+// Kernel orientation turned around: A and B (and M and N) exchanged, the
+// result map transposed -- the same product C(m, n) = sum_k A(k, m) B(k, n)
+// read as C^T(n, m) = sum_k B(k, n) A(k, m).
+static void flip_orientation(HostBlock &b) {
+    std::swap(b.M, b.N);
+    std::swap(b.buf1, b.buf2);
+    std::swap(b.off1, b.off2);
+    std::swap(b.A, b.B);
+    Resolved &c = b.C.r;
+    if (!c.table) {
+        std::swap(c.first, c.second);
+    } else {
+        std::vector<int32_t> tt(c.F * c.S);
+        for (int64_t s = 0; s < c.S; ++s)
+            for (int64_t f = 0; f < c.F; ++f) tt[s + c.S * f] = c.first[f + c.F * s];
+        c.first.swap(tt);
+    }
+    std::swap(c.F, c.S);
+    b.swap = !b.swap;
+}
+
+// Whether a block's operands reach the strided-GEMM launch: each is a
+// strided view of its list (affine index maps; *sa / *sb), or is gathered
+// into dense scratch first (min(M, N) >= materialize_min).  The kernels
+// address an operand block with unsigned 32-bit byte offsets: non-negative
+// strides, extents up to 2^30 bytes.
+static bool strided_operands(const gecco_hip_plan *p, const HostBlock &b, bool *sa_out = nullptr,
+                             bool *sb_out = nullptr) {
+    const int64_t max_extent = (int64_t)1 << 30;
+    auto strided = [&](const HostOperand &o, int64_t X) {
+        int64_t f0, f1, s0, s1;
+        if (o.r.table || !affine_of(o.r.first, f0, f1) || !affine_of(o.r.second, s0, s1)) return false;
+        return f1 >= 0 && s1 >= 0 && (f1 * (b.K - 1) + s1 * (X - 1) + 1) * p->elem <= max_extent;
+    };
+    const bool sa = strided(b.A, b.M), sb = strided(b.B, b.N);
+    const bool big = p->materialize_min > 0 && std::min(b.M, b.N) >= p->materialize_min;
+    const bool fa = b.K * b.M * p->elem <= max_extent, fb = b.K * b.N * p->elem <= max_extent;
+    if (sa_out) *sa_out = sa;
+    if (sb_out) *sb_out = sb;
+    return (sa || (big && fa)) && (sb || (big && fb));
+}
+
 static int plan_finalize(gecco_hip_plan *p) {
     if (p->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    // Short contractions (the direct tasks below) with at most 8 result
+    // columns but more rows: turned so that the long side runs along the
+    // lanes.  A direct task's thread owns one column, loads that column of
+    // B with all its K values in flight and computes the task's few rows
+    // from the A tile in LDS; with the long side along the rows, each task
+    // would stage only DIRECT_A / K rows of the long operand and spend its
+    // time on latency (e.g. T1 x <oo||vv>: 2500 x 5 results, K = 25).
+    for (auto &b : p->blocks)
+        if (b.N < b.M && b.N <= 8 && b.K <= p->direct_kmax && strided_operands(p, b)) flip_orientation(b);
     // index pool
     std::vector<int32_t> pool;
     auto put = [&](const std::vector<int32_t> &v) {
@@ -795,19 +846,8 @@ static int plan_finalize(gecco_hip_plan *p) {
     std::vector<uint8_t> is_aff(p->blocks.size(), 0);
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         auto &b = p->blocks[i];
-        // the strided-GEMM kernel addresses an operand block with unsigned
-        // 32-bit byte offsets: non-negative strides, extents up to 2^30 bytes
-        const int64_t max_extent = (int64_t)1 << 30;
-        auto strided = [&](const HostOperand &o, int64_t X) {
-            int64_t f0, f1, s0, s1;
-            if (o.r.table || !affine_of(o.r.first, f0, f1) || !affine_of(o.r.second, s0, s1)) return false;
-            return f1 >= 0 && s1 >= 0 && (f1 * (b.K - 1) + s1 * (X - 1) + 1) * p->elem <= max_extent;
-        };
-        const bool sa = strided(b.A, b.M), sb = strided(b.B, b.N);
-        const bool big = p->materialize_min > 0 && std::min(b.M, b.N) >= p->materialize_min;
-        const bool fa = b.K * b.M * p->elem <= max_extent, fb = b.K * b.N * p->elem <= max_extent;
-        const bool ca = sa || (big && fa), cb = sb || (big && fb);
-        b.strided = ca && cb;
+        bool sa, sb;
+        b.strided = strided_operands(p, b, &sa, &sb);
         b.mat_a = b.strided && !sa;
         b.mat_b = b.strided && !sb;
         // short contractions with a thin result side (or a very short
