@@ -145,9 +145,13 @@ class CCSD:
     H<spaces of C><spaces of A> as above (filled from ham).  plans=False
     sets up the lists, integrals and preconditioner only (for a batched
     driver that builds the plans of several instances together).
+    pool=True lets intermediates whose lifetimes do not meet share device
+    memory (formula.lifetimes / formula.pool_layout: command_new_intermediate
+    and command_del_intermediate); the refined solve, whose linearised
+    formula reads every intermediate's final value, then refuses to run.
     """
 
-    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True):
+    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True, pool=False):
         import torch
         import gecco_hip
         self.torch = torch
@@ -181,14 +185,14 @@ class CCSD:
         self.namp = o
         self.amp = torch.zeros(o, dtype=torch.float64, device=self.dev)
         self.res = torch.zeros(o, dtype=torch.float64, device=self.dev)
-        self.buf = {}
-        for n in lists:
-            if n in self.seg:
-                v = self.amp if n[0] == 'T' else self.res
-                self.buf[n] = v[self.seg[n][0]:self.seg[n][1]]
-            else:
-                self.buf[n] = torch.zeros(max(1, self.L[n].length), dtype=torch.float64,
-                                          device=self.dev)
+        # intermediates with a lifetime share one pool when asked to
+        self.pool_spans, self.pool_off, self.pool_len = {}, {}, 0
+        if pool:
+            spans = _formula.lifetimes(self.formula, keep=self.tnames + self.rnames + ['E'])
+            self.pool_spans = {n: v for n, v in spans.items() if n not in self.seg}
+            self.pool_off, self.pool_len = _formula.pool_layout(
+                self.pool_spans, {n: self.L[n].length for n in self.pool_spans})
+        self.buf = self._list_buffers(torch.float64, self.amp, self.res)
         t0 = time.perf_counter()
         self._integrals()
         self._denominators()
@@ -202,6 +206,32 @@ class CCSD:
             self.build_plans()
         self.t_setup_plans = time.perf_counter() - t0
         self.bufs = [self.buf[n] for n in lists]
+
+    def _list_buffers(self, dtype, amp, res, source=None):
+        """A buffer per list: amplitude and residual lists as views of amp
+        and res, pooled intermediates as views of one pool, the others
+        their own zeros (or source[n] converted to dtype: the integrals)."""
+        torch = self.torch
+        pool = torch.zeros(max(1, self.pool_len), dtype=dtype, device=self.dev) if self.pool_spans else None
+        out = {}
+        for n in self.formula['lists']:
+            ln = max(1, self.L[n].length)
+            if n in self.seg:
+                v = amp if n[0] == 'T' else res
+                out[n] = v[self.seg[n][0]:self.seg[n][1]]
+            elif n in self.pool_spans:
+                out[n] = pool[self.pool_off[n]:self.pool_off[n] + ln]
+            elif source is not None:
+                out[n] = source[n].to(dtype)
+            else:
+                out[n] = torch.zeros(ln, dtype=dtype, device=self.dev)
+        return out
+
+    def list_bytes(self):
+        """Device bytes of the float64 lists (the pool counted once)."""
+        own = sum(max(1, self.L[n].length) for n in self.formula['lists']
+                  if n not in self.seg and n not in self.pool_spans)
+        return 8 * (own + 2 * self.namp + (max(1, self.pool_len) if self.pool_spans else 0))
 
     def build_plans(self, tuning=None):
         """One plan per scheduled level of this instance alone (buffer slots
@@ -265,6 +295,11 @@ class CCSD:
                            factor=it['factor'])
             else:
                 p.add_init(base + self.slot[n], self.L[n].length)
+        for it in level.get('add', []):
+            n, src = it['target'], it['source']
+            if self.L[src].length != self.L[n].length:
+                raise ValueError(f'ADD {src} into {n}: lists of different layout')
+            p.add_sum(base + self.slot[n], self.L[n].length, base + self.slot[src], factor=it['factor'])
         for c in level['contr']:
             res, a, b = c['target'], c['op1'], c['op2']
             nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], c['cnt_c'], c['cnt_a'],
@@ -358,13 +393,7 @@ class CCSD:
         lists = self.formula['lists']
         self.amp32 = torch.zeros(self.namp, dtype=torch.float32, device=self.dev)
         self.res32 = torch.zeros_like(self.amp32)
-        self.buf32 = {}
-        for n in lists:
-            if n in self.seg:
-                v = self.amp32 if n[0] == 'T' else self.res32
-                self.buf32[n] = v[self.seg[n][0]:self.seg[n][1]]
-            else:
-                self.buf32[n] = self.buf[n].to(torch.float32)
+        self.buf32 = self._list_buffers(torch.float32, self.amp32, self.res32, source=self.buf)
         self.bufs32 = [self.buf32[n] for n in lists]
         if not plans:
             return
@@ -389,6 +418,9 @@ class CCSD:
         dT1|dT2.  bulk: 'f32' or 'bf16x3' (split-BF16 GEMM tiles)."""
         import gecco_hip
         torch = self.torch
+        if self.pool_spans:
+            raise ValueError('build_linear: the linearised formula reads every intermediate at the '
+                             'current amplitudes; pooled intermediates (pool=True) do not keep them')
         if getattr(self, 'buf32', None) is None:
             self.build_f32(plans=False)
         items, dnames = _formula.linearize(self.formula, self.tnames)
@@ -422,6 +454,9 @@ class CCSD:
                                factor=it['factor'])
                 else:
                     p.add_init(slot[it['target']], lst(it['target']).length)
+            for it in lv['add']:
+                p.add_sum(slot[it['target']], lst(it['target']).length, slot[it['source']],
+                          factor=it['factor'])
             for c in lv['contr']:
                 a, b, t = c['op1'], c['op2'], c['target']
                 nb, fl = p.add_contraction(lst(a), lst(b), lst(t), c['cnt_c'], c['cnt_a'], c['factor'],

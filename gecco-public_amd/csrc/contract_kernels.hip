@@ -1121,10 +1121,18 @@ __global__ __launch_bounds__(256) void init_lists(const DevInit *__restrict__ jo
                 const int64_t e = e0 + u * blockDim.x;
                 x[u] = e < t.e1 ? ldg(src + e) : (T)0;
             }
+            if (j.accumulate) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t e = e0 + u * blockDim.x;
-                if (e < t.e1) stg(dst + e, f * x[u]);
+                for (int u = 0; u < U; ++u) {
+                    const int64_t e = e0 + u * blockDim.x;
+                    if (e < t.e1 && x[u] != (T)0) atomic_add_g(dst + e, f * x[u]);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t e = e0 + u * blockDim.x;
+                    if (e < t.e1) stg(dst + e, f * x[u]);
+                }
             }
         }
     } else {

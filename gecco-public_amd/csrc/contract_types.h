@@ -76,6 +76,9 @@ struct DevInit {
     int64_t off_dst, off_src, len;
     double factor;
     int32_t dst, src;
+    // 0: dst = factor * src (or 0 without a source); 1: dst += factor * src
+    // (atomic adds, so that several sums into one list may share a launch)
+    int32_t accumulate, pad;
 };
 
 struct DevInitTask {

@@ -1939,6 +1939,29 @@ int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_d
     return 0;
 }
 
+int gecco_hip_plan_add_sum(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
+                           int32_t buf_src, int64_t off_src, double factor) {
+    g_err = 0;
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (buf_dst < 0 || off_dst < 0 || len < 0 || buf_src < 0 || off_src < 0)
+        return fail(GECCO_HIP_EINVAL, "add_sum: bad slot, offset or length");
+    if (buf_src == buf_dst && off_src < off_dst + len && off_dst < off_src + len)
+        return fail(GECCO_HIP_EINVAL, "add_sum: source and destination overlap");
+    DevInit j;
+    std::memset(&j, 0, sizeof j);
+    j.dst = buf_dst;
+    j.src = buf_src;
+    j.off_dst = off_dst;
+    j.off_src = off_src;
+    j.len = len;
+    j.factor = factor;
+    j.accumulate = 1;
+    plan->inits.push_back(j);
+    plan->init_group.push_back(plan->cur_group);
+    return 0;
+}
+
 int gecco_hip_plan_set_constant(gecco_hip_plan *plan, int32_t slot) {
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
     if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");

@@ -20,12 +20,24 @@ commands:
   {"cmd": "SYMMETRISE", "target": L}            command_symmetrise (5): the
                                                 spin-flip projection of L
                                                 (contract/sym_ab_blk.f)
+  {"cmd": "ADD", "target": L, "source": S,      command_add_intm (7),
+   "factor": f}                                 contract/fs_add_drv.f: L += f * S
+                                                (S with L's occupation)
+  {"cmd": "BC", ... as CONTR}                   command_bc (10): L = f A . B,
+                                                i.e. INIT L = 0 then the CONTR
+  {"cmd": "NEW", "target": L}                   command_new_intermediate (2) and
+  {"cmd": "DELETE", "target": L}                command_del_intermediate (3)
+                                                (contract/fs_newintm_drv.f,
+                                                fs_delintm_drv.f): the span of
+                                                items in which intermediate L
+                                                exists
 
 Occupations are dicts over the spaces {"H": n, "P": n}.  A formula file is
 JSON: {"lists": {name: {"C": occ, "A": occ}}, "constant": [names],
-"items": [...]}.  Intermediates are created with the lists and kept for the
-whole formula (the engine's lists are device-resident; command_new_intermediate
-and command_del_intermediate have no work left to do).
+"items": [...]}.  Every list is declared with the formula; NEW and DELETE
+(or, without them, the items themselves) bound the levels in which an
+intermediate holds data (`lifetimes`), so that a driver may let intermediates
+whose lifetimes do not meet share device memory (`pool_layout`).
 
 Schedule: items run in levels; the items of a level are independent and
 run as one grouped launch.  A CONTR item goes into the first level after
@@ -69,13 +81,46 @@ def init(target, source=None, factor=1.0):
     return it
 
 
+def add(target, source, factor=1.0):
+    """An ADD item: target += factor * source."""
+    return {'cmd': 'ADD', 'target': target, 'source': source, 'factor': float(factor)}
+
+
+def bc(target, op1, op2, cnt_c, cnt_a, factor):
+    """A BC item: target = factor * op1 . op2 (no update)."""
+    return dict(contr(target, op1, op2, cnt_c, cnt_a, factor), cmd='BC')
+
+
+def new(target):
+    return {'cmd': 'NEW', 'target': target}
+
+
+def delete(target):
+    return {'cmd': 'DELETE', 'target': target}
+
+
+_COMMANDS = ('INIT', 'CONTR', 'SYMMETRISE', 'ADD', 'BC', 'NEW', 'DELETE')
+
+
+def expand(items):
+    """The items with every BC written as INIT (zero) then CONTR."""
+    out = []
+    for it in items:
+        if it['cmd'] == 'BC':
+            out.append(init(it['target']))
+            out.append(dict(it, cmd='CONTR'))
+        else:
+            out.append(it)
+    return out
+
+
 def dump(formula, path):
     """Write a formula ({"lists", "constant", "items"}) as JSON."""
     out = {'lists': {n: {'C': _occ_out(c), 'A': _occ_out(a)} for n, (c, a) in formula['lists'].items()},
            'constant': list(formula.get('constant', [])), 'items': []}
     for it in formula['items']:
         it = dict(it)
-        if it['cmd'] == 'CONTR':
+        if it['cmd'] in ('CONTR', 'BC'):
             it['cnt_c'] = _occ_out(it['cnt_c'])
             it['cnt_a'] = _occ_out(it['cnt_a'])
         out['items'].append(it)
@@ -90,11 +135,13 @@ def load(path):
     lists = {n: (_occ_in(v['C']), _occ_in(v['A'])) for n, v in d['lists'].items()}
     items = []
     for it in d['items']:
-        if it['cmd'] not in ('INIT', 'CONTR', 'SYMMETRISE'):
+        if it['cmd'] not in _COMMANDS:
             raise ValueError(f'unknown formula command {it["cmd"]!r}')
-        if it['cmd'] == 'CONTR':
-            it = contr(it['target'], it['op1'], it['op2'], _occ_in(it['cnt_c']), _occ_in(it['cnt_a']),
-                       it['factor'])
+        if it['cmd'] in ('CONTR', 'BC'):
+            it = dict(contr(it['target'], it['op1'], it['op2'], _occ_in(it['cnt_c']), _occ_in(it['cnt_a']),
+                            it['factor']), cmd=it['cmd'])
+        elif it['cmd'] == 'ADD':
+            it = add(it['target'], it['source'], it.get('factor', 1.0))
         items.append(it)
     for it in items:
         for key in ('target', 'op1', 'op2', 'source'):
@@ -105,9 +152,10 @@ def load(path):
 
 def schedule(items):
     """Levels of a formula: a list of dicts {"init": [INIT items],
-    "contr": [CONTR items], "sym": [list names]} in execution order (the
-    INITs of a level run before its contractions, the projections after
-    them).
+    "add": [ADD items], "contr": [CONTR items], "sym": [list names]} in
+    execution order (the INITs and ADDs of a level run in one launch before
+    its contractions, the projections after them; BC items count as INIT
+    then CONTR, NEW and DELETE do no work).
 
     A contraction reads its operands as they are at the start of its level,
     so with items taken in list order:
@@ -119,40 +167,157 @@ def schedule(items):
                          level.
     Contributions to one list commute and need no order among themselves.
     An INIT follows the earlier reads and writes of its target and the
-    writes of its source."""
+    writes of its source.  An ADD reads its source like a contraction and
+    runs in the launch of the level's INITs, so it follows its target's
+    last INIT by a level."""
+    return _place(items)[0]
+
+
+def _place(items):
+    """schedule() and the level of every item of expand(items) (None for
+    SYMMETRISE, NEW and DELETE).
+
+    An INIT to zero is placed late: in the level of the first item that
+    writes its list (a CONTR shares that level, the INITs running first;
+    an ADD runs in the INIT launch, so the INIT goes one level earlier),
+    or, when a read or another INIT of the list comes first, as early as
+    its own constraints allow.  A list then holds data only from the level
+    where it is first needed (lifetimes)."""
+    items = expand(items)
     last_write, last_read, last_init = {}, {}, {}     # list -> level
-    inits, contrs = {}, {}
-    for it in items:
+    inits, contrs, adds = {}, {}, {}
+    where = [None] * len(items)
+    pending = {}          # list -> (item index, earliest level) of a deferred INIT
+
+    def put_init(t, lv):
+        k, _ = pending.pop(t)
+        inits.setdefault(lv, []).append(items[k])
+        where[k] = lv
+        last_write[t] = max(last_write.get(t, -1), lv)
+        last_init[t] = lv
+
+    def flush(n):
+        if n in pending:
+            put_init(n, pending[n][1])
+
+    for k, it in enumerate(items):
         t = it['target']
-        if it['cmd'] == 'SYMMETRISE':
+        if it['cmd'] in ('SYMMETRISE', 'NEW', 'DELETE'):
+            continue
+        if it['cmd'] == 'ADD':
+            src = it['source']
+            flush(src)
+            lv = max(last_write.get(src, -1) + 1, last_read.get(t, -1) + 1, last_init.get(t, -1) + 1)
+            if t in pending:
+                put_init(t, max(pending[t][1], lv - 1))
+                lv = max(lv, last_init[t] + 1)
+            adds.setdefault(lv, []).append(it)
+            last_write[t] = max(last_write.get(t, -1), lv)
+            last_read[src] = max(last_read.get(src, -1), lv)
+            where[k] = lv
             continue
         if it['cmd'] == 'INIT':
             src = it.get('source')
+            flush(t)
+            if src:
+                flush(src)
             lv = max(last_read.get(t, -1) + 1, last_write.get(t, -1) + 1,
                      last_write.get(src, -1) + 1 if src else 0)
+            if not src:
+                pending[t] = (k, lv)
+                continue
             inits.setdefault(lv, []).append(it)
             last_write[t] = max(last_write.get(t, -1), lv)
             last_init[t] = lv
-            if src:
-                last_read[src] = max(last_read.get(src, -1), lv)
+            last_read[src] = max(last_read.get(src, -1), lv)
+            where[k] = lv
             continue
+        flush(it['op1'])
+        flush(it['op2'])
         lv = max(last_write.get(it['op1'], -1) + 1, last_write.get(it['op2'], -1) + 1,
                  last_read.get(t, -1) + 1, last_init.get(t, 0))
+        if t in pending:
+            put_init(t, max(pending[t][1], lv))
+            lv = max(lv, last_init[t])
         contrs.setdefault(lv, []).append(it)
         last_write[t] = max(last_write.get(t, -1), lv)
         for op in (it['op1'], it['op2']):
             last_read[op] = max(last_read.get(op, -1), lv)
-    nlev = max(list(contrs) + list(inits) + [-1]) + 1
-    levels = [{'init': inits.get(l, []), 'contr': contrs.get(l, []), 'sym': []} for l in range(nlev)]
-    writes = [{c['target'] for c in lv['contr']} for lv in levels]
-    reads = [{c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']} for lv in levels]
+        where[k] = lv
+    for t in list(pending):
+        flush(t)
+    nlev = max(list(contrs) + list(inits) + list(adds) + [-1]) + 1
+    levels = [{'init': inits.get(l, []), 'add': adds.get(l, []), 'contr': contrs.get(l, []), 'sym': []}
+              for l in range(nlev)]
+    writes = [{c['target'] for c in lv['contr'] + lv['add']} for lv in levels]
+    reads = [{c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']} |
+             {a['source'] for a in lv['add']} | {i['source'] for i in lv['init'] if 'source' in i}
+             for lv in levels]
     for li, lv in enumerate(levels):
-        for res in dict.fromkeys(c['target'] for c in lv['contr']):
+        for res in dict.fromkeys(c['target'] for c in lv['contr'] + lv['add']):
             nxt = next((l for l in range(li + 1, nlev) if res in writes[l]), None)
             last = nlev - 1 if nxt is None else nxt
             if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
                 lv['sym'].append(res)
-    return levels
+    return levels, items, where
+
+
+def lifetimes(formula, keep=()):
+    """The levels [first, last] in which each intermediate holds data: a
+    list (not constant, not in `keep`) whose first item, in formula order,
+    writes it without reading it -- an INIT, or a BC -- is defined from that
+    item's level to the level of its last item (read or write; its
+    projection runs at the end of a level that writes it).  Lists read
+    before any such write carry data between evaluations and have no
+    lifetime.  NEW and DELETE items, where given, must enclose every item
+    of their list (ValueError otherwise)."""
+    levels, items, where = _place(formula['items'])
+    fixed = set(formula.get('constant', ())) | set(keep)
+    first, last, born = {}, {}, {}
+    opened, closed = {}, {}
+    for k, (it, lv) in enumerate(zip(items, where)):
+        t = it['target']
+        if it['cmd'] == 'NEW':
+            opened[t] = k
+            continue
+        if it['cmd'] == 'DELETE':
+            closed[t] = k
+            continue
+        if lv is None:
+            continue
+        used = [t] + [it[x] for x in ('op1', 'op2', 'source') if x in it]
+        for n in used:
+            if n in closed and closed[n] < k:
+                raise ValueError(f'formula item {k} uses {n} after its DELETE')
+            if n in opened and opened[n] > k:
+                raise ValueError(f'formula item {k} uses {n} before its NEW')
+            if n not in born:
+                born[n] = n == t and it['cmd'] == 'INIT' and it.get('source') != t and \
+                    n not in used[1:]
+            first[n] = min(first.get(n, lv), lv)
+            last[n] = max(last.get(n, lv), lv)
+    return {n: (first[n], last[n]) for n in first if born.get(n) and n not in fixed}
+
+
+def pool_layout(spans, sizes, align=64):
+    """Offsets (elements) of lists in one shared pool such that lists
+    whose spans (level intervals, from lifetimes) meet never overlap in
+    memory: first fit, largest list first.  Returns ({name: offset},
+    pool length)."""
+    placed = []          # (offset, end, first, last)
+    off = {}
+    for n in sorted(spans, key=lambda x: (-sizes[x], x)):
+        f, l = spans[n]
+        need = max(1, int(sizes[n]))
+        busy = sorted((o, e) for o, e, f2, l2 in placed if not (l2 < f or l < f2))
+        at = 0
+        for o, e in busy:
+            if at + need <= o:
+                break
+            at = max(at, (e + align - 1) // align * align)
+        off[n] = at
+        placed.append((at, at + need, f, l))
+    return off, max([e for _, e, _, _ in placed] + [0])
 
 
 def delta_name(n):
@@ -164,6 +329,7 @@ def linearize(formula, varying):
     every list X that depends on the `varying` lists, a list dX with
       INIT X = 0           ->  dX = 0
       INIT X = f * S       ->  dX = f * dS          (0 when S is fixed)
+      ADD X += f * S       ->  dX += f * dS         (nothing when S is fixed)
       CONTR X += f A . B   ->  dX += f dA . B + f A . dB
     (terms of fixed operands dropped), so that with the base lists holding
     the formula's values at t, the delta lists hold the directional
@@ -175,8 +341,9 @@ def linearize(formula, varying):
     read of a list must see that value: a formula that writes (or
     initialises) a list after an item has read it is refused
     (ValueError)."""
+    fitems = [it for it in expand(formula['items']) if it['cmd'] not in ('NEW', 'DELETE', 'SYMMETRISE')]
     read = set()
-    for it in formula['items']:
+    for it in fitems:
         if it['target'] in read:
             raise ValueError(f'linearize: {it["target"]} is written after it is read')
         if it['cmd'] == 'CONTR':
@@ -184,17 +351,20 @@ def linearize(formula, varying):
         elif it.get('source'):
             read.add(it['source'])
     varies = set(varying)
-    for it in formula['items']:        # lists written from varying ones
+    for it in fitems:                  # lists written from varying ones
         if it['cmd'] == 'CONTR' and (it['op1'] in varies or it['op2'] in varies):
             varies.add(it['target'])
-        elif it['cmd'] == 'INIT' and it.get('source') in varies:
+        elif it['cmd'] in ('INIT', 'ADD') and it.get('source') in varies:
             varies.add(it['target'])
     items = []
-    for it in formula['items']:
+    for it in fitems:
         t = it['target']
         if t not in varies or t in varying:
             continue
-        if it['cmd'] == 'INIT':
+        if it['cmd'] == 'ADD':
+            if it['source'] in varies:
+                items.append(add(delta_name(t), delta_name(it['source']), it['factor']))
+        elif it['cmd'] == 'INIT':
             if it.get('source') in varies:
                 items.append(init(delta_name(t), delta_name(it['source']), it['factor']))
             else:

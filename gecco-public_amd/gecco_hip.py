@@ -481,6 +481,8 @@ _lib.gecco_hip_list_diagonal.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_init.argtypes = [VP, ctypes.c_int32, I64, I64, ctypes.c_int32, I64,
                                          ctypes.c_double]
 _lib.gecco_hip_plan_add_init.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_sum.argtypes = _lib.gecco_hip_plan_add_init.argtypes
+_lib.gecco_hip_plan_add_sum.restype = ctypes.c_int
 _lib.gecco_hip_diis_create.argtypes = [I64, I64, ctypes.POINTER(VP)]
 _lib.gecco_hip_diis_destroy.argtypes = [VP]
 _lib.gecco_hip_diis_reset.argtypes = [VP]
@@ -705,6 +707,18 @@ Plan.add_reorder = _plan_add_reorder
 Plan.add_trace = _plan_add_trace
 Plan.add_sym_ab = _plan_add_sym_ab
 Plan.add_init = _plan_add_init
+
+
+def _plan_add_sum(self, buf_dst, length, buf_src, factor=1.0, off_dst=0, off_src=0):
+    """dst += factor * src over `length` elements (an ADD item), in the
+    initialisation launch; atomic, so several sums into one list may share
+    the plan, but an initialisation of the same list must be in an earlier
+    plan."""
+    _check(_lib.gecco_hip_plan_add_sum(self._h, int(buf_dst), int(off_dst), int(length), int(buf_src),
+                                       int(off_src), float(factor)))
+
+
+Plan.add_sum = _plan_add_sum
 
 
 class DIIS:

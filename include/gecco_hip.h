@@ -226,6 +226,15 @@ int gecco_hip_plan_block_index(const gecco_hip_plan *plan, int64_t iblock, int64
  * Before finalize only. */
 int gecco_hip_plan_add_init(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
                             int32_t buf_src, int64_t off_src, double factor);
+/* Sum of one list into another of the same layout, in the same launch as
+ * the initialisations: dst[off_dst + i] += factor * src[off_src + i] for
+ * i < len (contract/fs_add_drv.f, command_add_intm of
+ * contract/frm_sched2.f:277-286, for lists of equal occupation).  The adds
+ * are atomic, so several sums into one list may share a plan; an
+ * initialisation of the same destination must be in an earlier plan (the
+ * jobs of one launch run concurrently).  Before finalize only. */
+int gecco_hip_plan_add_sum(gecco_hip_plan *plan, int32_t buf_dst, int64_t off_dst, int64_t len,
+                           int32_t buf_src, int64_t off_src, double factor);
 /* Declare that buffer slot `slot` holds the same data at every execute
  * (e.g. integral lists): the materialised gathers of that slot are built on
  * the first execute and again only when the slot's pointer changes.  The

@@ -121,8 +121,8 @@ class Route3CCSD:
                                   np.prod(b['lstr_cnt']) for b in blocks)
                 self.calls += len(blocks)
                 cs.append((it['target'], it['op1'], it['op2'], c, blocks))
-            self.levels.append((lv['init'], cs, lv['sym']))
-        self.pairs = {n: sym_ab_pairs(self.L[n], 1.0) for _, _, sym in self.levels for n in sym}
+            self.levels.append((lv['init'], lv.get('add', []), cs, lv['sym']))
+        self.pairs = {n: sym_ab_pairs(self.L[n], 1.0) for _, _, _, sym in self.levels for n in sym}
         for n in self.tnames:
             self.X[n] = np.zeros(self.L[n].length)
 
@@ -131,10 +131,12 @@ class Route3CCSD:
         ...: returns the residual Omega = R - D t (the function g of the
         solver) over all ranks."""
         X, L = self.X, self.L
-        for inits, cs, sym in self.levels:
+        for inits, adds, cs, sym in self.levels:
             for it in inits:
                 n = it['target']
                 X[n] = it['factor'] * X[it['source']] if 'source' in it else np.zeros(L[n].length)
+            for it in adds:                 # ADD items (contract/fs_add_drv.f)
+                X[it['target']] = X[it['target']] + it['factor'] * X[it['source']]
             for res, a, b, c, blocks in cs:
                 c.run(X[a], X[b], X[res], route=3, use_tr=False, screen=False, blocks=blocks)
             for n in sym:

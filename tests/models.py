@@ -196,3 +196,45 @@ def transposed_data(mel, x):
     nc = sum(mel.oc)
     D = D.transpose(list(range(nc, D.ndim)) + list(range(nc)))
     return T, from_dense(T, D)
+
+
+def ccsd_with_add_bc_new_delete():
+    """The CCSD formula (gecco-public_amd/ccsd.py) rewritten with the other
+    formula commands: copies that start an intermediate become INIT + ADD
+    (command_add_intm), the first contraction into a zeroed intermediate
+    becomes a BC (command_bc), and NEW/DELETE items bracket every
+    intermediate from its first to its last item.  Same residuals."""
+    import ccsd
+    import formula as F
+    items = []
+    first_write = {}
+    src = ccsd.FORMULA['items']
+    for k, it in enumerate(src):
+        if it['cmd'] == 'CONTR' and it['target'] not in first_write:
+            first_write[it['target']] = k
+    zero_init = {it['target'] for it in src if it['cmd'] == 'INIT' and 'source' not in it}
+    inter = [n for n in ccsd.LISTS if n not in ccsd.INTEGRALS and n not in ('T1', 'T2', 'R1', 'R2', 'E')]
+    last_use = {}
+    for k, it in enumerate(src):
+        for x in ('target', 'op1', 'op2', 'source'):
+            if it.get(x) in inter:
+                last_use[it[x]] = k
+    for k, it in enumerate(src):
+        t = it['target']
+        if it['cmd'] == 'INIT':
+            if t in inter:
+                items.append(F.new(t))
+            if 'source' in it:
+                items += [F.init(t), F.add(t, it['source'], it['factor'])]
+            elif t in first_write and t in inter:
+                pass                   # the BC below initialises it
+            else:
+                items.append(it)
+        elif it['cmd'] == 'CONTR' and t in zero_init and t in inter and first_write.get(t) == k:
+            items.append(F.bc(t, it['op1'], it['op2'], it['cnt_c'], it['cnt_a'], it['factor']))
+        else:
+            items.append(it)
+        for x in sorted({it.get(y) for y in ('target', 'op1', 'op2', 'source')} & set(inter)):
+            if last_use.get(x) == k:
+                items.append(F.delete(x))
+    return {'lists': dict(ccsd.LISTS), 'constant': list(ccsd.INTEGRALS), 'items': items}

@@ -3,6 +3,7 @@ formula items into grouped launches, the committed formula files, and the
 CCSD driver's levels derived from its flat formula."""
 import os
 
+import numpy as np
 import pytest
 
 import ccsd
@@ -29,10 +30,11 @@ def test_ccsd_levels_are_the_dependency_levels():
     for li in (1, 2):
         for c in levels[li]['contr']:
             assert c['op1'] in written[li - 1] or c['op2'] in written[li - 1], c
-    # every initialisation runs first, projections once per list
+    # a list's initialisation runs in the level of its first write (E is
+    # first written in level 1), projections once per list
     assert {it['target'] for it in levels[0]['init']} == \
-        {n for n in ccsd.LISTS if n not in ccsd.INTEGRALS and n not in ('T1', 'T2')}
-    assert not levels[1]['init'] and not levels[2]['init']
+        {n for n in ccsd.LISTS if n not in ccsd.INTEGRALS and n not in ('T1', 'T2', 'E')}
+    assert [it['target'] for it in levels[1]['init']] == ['E'] and not levels[2]['init']
     assert ccsd.SYM_AFTER == [['tau', 'taut', 'Fme', 'Z'],
                               ['Fae', 'F2ae', 'Fmi', 'F2mi', 'Wmnij', 'Zamij', 'Wmbej', 'E'],
                               ['R1', 'R2']]
@@ -54,7 +56,9 @@ def test_dependency_rules():
                           [('Y', 'X', 'B')],
                           [('X', 'A', 'A'), ('Z', 'Y', 'A')],
                           [('Y', 'A', 'A'), ('W', 'A', 'A')]]
-    assert [it['target'] for it in lv[0]['init']] == ['X', 'Y', 'Z']
+    # an INIT to zero runs in the level of its list's first write
+    assert [it['target'] for it in lv[0]['init']] == ['X', 'Z']
+    assert [it['target'] for it in lv[1]['init']] == ['Y']
     assert [it['target'] for it in lv[3]['init']] == ['W']
     # X: written in 0 and 2, read in 1 -> projected after 0 and after 2
     assert lv[0]['sym'] == ['X'] and lv[1]['sym'] == ['Y']
@@ -63,22 +67,50 @@ def test_dependency_rules():
 
 def _run_sequential(items, lists):
     x = {k: v.copy() for k, v in lists.items()}
-    for it in items:
+    for it in F.expand(items):
         if it['cmd'] == 'INIT':
             x[it['target']] = it['factor'] * x[it['source']] if 'source' in it else 0.0 * x[it['target']]
+        elif it['cmd'] == 'ADD':
+            x[it['target']] = x[it['target']] + it['factor'] * x[it['source']]
         elif it['cmd'] == 'CONTR':
             x[it['target']] = x[it['target']] + it['factor'] * x[it['op1']] * x[it['op2']]
     return x
 
 
-def _run_levels(levels, lists):
-    """A level's INITs run in order, then its contractions as one grouped
-    launch: every task reads its operands while the others add into their
-    targets, so no list may be both read and written in one level."""
-    x = {k: v.copy() for k, v in lists.items()}
+class _Store(dict):
+    """Lists as views: pooled ones into one shared array."""
+
+    def __init__(self, lists, offsets=None, pool=0):
+        super().__init__()
+        self.pool = np.full(max(pool, 1), np.nan)
+        for k, v in lists.items():
+            if offsets and k in offsets:
+                self.views = getattr(self, 'views', {})
+                dict.__setitem__(self, k, self.pool[offsets[k]:offsets[k] + len(v)])
+            else:
+                dict.__setitem__(self, k, v.copy())
+
+    def __setitem__(self, k, v):
+        self[k][...] = v
+
+
+def _run_levels(levels, lists, offsets=None, pool=0):
+    """A level's INITs and ADDs run as one launch (an INIT with a source
+    and an ADD read their sources as they stand at the level's start), then
+    its contractions as one grouped launch: every task reads its operands
+    while the others add into their targets, so no list may be both read
+    and written in one level.  offsets: pooled lists share one array."""
+    x = _Store(lists, offsets, pool)
     for lv in levels:
+        reads0 = {i['source'] for i in lv['init'] if 'source' in i} | {a['source'] for a in lv['add']}
+        writes0 = {i['target'] for i in lv['init']} | {a['target'] for a in lv['add']}
+        assert not reads0 & writes0, (reads0 & writes0, lv)
+        assert not {i['target'] for i in lv['init']} & {a['target'] for a in lv['add']}, lv
+        old = {k: np.array(x[k]) for k in reads0}
         for it in lv['init']:
-            x[it['target']] = it['factor'] * x[it['source']] if 'source' in it else 0.0 * x[it['target']]
+            x[it['target']] = it['factor'] * old[it['source']] if 'source' in it else 0.0
+        for a in lv['add']:
+            x[a['target']] = x[a['target']] + a['factor'] * old[a['source']]
         reads = {c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']}
         writes = {c['target'] for c in lv['contr']}
         assert not reads & writes, (reads & writes, lv)
@@ -102,22 +134,79 @@ def test_scheduled_levels_equal_the_sequential_formula():
     names = ['A', 'B', 'X', 'Y', 'Z', 'W']
     for trial in range(300):
         items = []
-        for _ in range(int(rng.integers(3, 14))):
-            t = names[2 + int(rng.integers(0, 4))]
-            if rng.random() < 0.2:
-                src = names[int(rng.integers(0, 6))]
-                items.append(F.init(t, src, float(rng.normal())) if src != t and rng.random() < 0.6
-                             else F.init(t))
-            else:
-                o1, o2 = (names[int(i)] for i in rng.integers(0, 6, 2))
-                if t in (o1, o2):
-                    continue
-                items.append(F.contr(t, o1, o2, {}, {}, float(rng.normal())))
+        items = _random_items(rng, names)
         lists = {n: rng.normal(size=3) for n in names}
         seq = _run_sequential(items, lists)
         lev = _run_levels(F.schedule(items), lists)
         for n in names:
             assert np.allclose(seq[n], lev[n], rtol=1e-12, atol=1e-12), (trial, n, items)
+
+
+def _random_items(rng, names):
+    """Random INIT (zero or copy), ADD, BC and CONTR items over the lists
+    names[2:] with operands from all of names."""
+    items = []
+    for _ in range(int(rng.integers(3, 16))):
+        t = names[2 + int(rng.integers(0, len(names) - 2))]
+        u = rng.random()
+        if u < 0.2:
+            src = names[int(rng.integers(0, len(names)))]
+            items.append(F.init(t, src, float(rng.normal())) if src != t and rng.random() < 0.6
+                         else F.init(t))
+        elif u < 0.3:
+            src = names[int(rng.integers(0, len(names)))]
+            if src != t:
+                items.append(F.add(t, src, float(rng.normal())))
+        else:
+            o1, o2 = (names[int(i)] for i in rng.integers(0, len(names), 2))
+            if t in (o1, o2):
+                continue
+            make = F.bc if rng.random() < 0.15 else F.contr
+            items.append(make(t, o1, o2, {}, {}, float(rng.normal())))
+    return items
+
+
+def test_pooled_intermediates_equal_the_sequential_formula():
+    """formula.lifetimes and formula.pool_layout: with every intermediate
+    that has a lifetime placed in one shared array (lists whose lifetimes
+    meet never overlap), executing the scheduled levels gives the outputs of
+    the item-by-item run.  Random formulas whose outputs are the lists named
+    in `keep`; A and B are constant inputs."""
+    rng = np.random.default_rng(11)
+    names = ['A', 'B', 'X', 'Y', 'Z', 'W', 'V', 'O']
+    shared = 0
+    for trial in range(400):
+        items = _random_items(rng, names)
+        form = {'items': items, 'constant': ['A', 'B']}
+        keep = ['O']
+        spans = F.lifetimes(form, keep=keep)
+        assert not set(spans) & {'A', 'B', 'O'}
+        sizes = {n: 3 for n in spans}
+        off, pool = F.pool_layout(spans, sizes, align=1)
+        for a in spans:               # lists alive together never overlap
+            for b in spans:
+                if a < b and not (spans[a][1] < spans[b][0] or spans[b][1] < spans[a][0]):
+                    assert off[a] + 3 <= off[b] or off[b] + 3 <= off[a], (a, b, spans, off)
+        shared += pool < 3 * len(spans)
+        lists = {n: rng.normal(size=3) for n in names}
+        seq = _run_sequential(items, lists)
+        lev = _run_levels(F.schedule(items), lists, off, pool)
+        for n in keep + ['A', 'B']:
+            assert np.allclose(seq[n], lev[n], rtol=1e-12, atol=1e-12), (trial, n, items, spans)
+    assert shared >= 3          # memory is actually shared in some of them
+
+
+def test_new_and_delete_bound_lifetimes():
+    c = F.contr
+    items = [F.new('X'), F.init('X'), c('X', 'A', 'B', {}, {}, 1.0), c('O', 'X', 'A', {}, {}, 1.0),
+             F.delete('X'), F.new('Y'), F.init('Y'), c('Y', 'O', 'A', {}, {}, 1.0), c('Q', 'Y', 'B', {}, {}, 1.0),
+             F.delete('Y')]
+    spans = F.lifetimes({'items': items, 'constant': ['A', 'B']}, keep=['O', 'Q'])
+    assert spans == {'X': (0, 1), 'Y': (2, 3)}
+    off, pool = F.pool_layout(spans, {'X': 10, 'Y': 10}, align=1)
+    assert off == {'X': 0, 'Y': 0} and pool == 10
+    with pytest.raises(ValueError):
+        F.lifetimes({'items': items + [c('O', 'X', 'A', {}, {}, 1.0)], 'constant': ['A', 'B']})
 
 
 def test_formula_files(tmp_path):
@@ -129,7 +218,7 @@ def test_formula_files(tmp_path):
     m = F.load(os.path.join(FORMULAS, 'mp2.json'))
     lv = F.schedule(m['items'])
     assert _names(lv) == [[('E', 'Hoovv', 'T2')]]
-    assert [it['target'] for it in lv[0]['init']] == ['R1', 'R2', 'E']
+    assert sorted(it['target'] for it in lv[0]['init']) == ['E', 'R1', 'R2']
     bad = tmp_path / 'bad.json'
     bad.write_text('{"lists": {}, "items": [{"cmd": "CONTR", "target": "Q", "op1": "A", "op2": "B", '
                    '"cnt_c": {}, "cnt_a": {}, "factor": 1}]}')
@@ -182,6 +271,11 @@ def test_linearised_formula_is_the_derivative():
         items = [F.init(n) for n in ('X', 'Y', 'Z')]
         for _ in range(int(rng.integers(2, 10))):
             t = names[3 + int(rng.integers(0, 3))]
+            if rng.random() < 0.25:           # ADD items: dX += f dS
+                src = names[int(rng.integers(0, 6))]
+                if src != t:
+                    items.append(F.add(t, src, float(rng.normal())))
+                continue
             o1, o2 = (names[int(i)] for i in rng.integers(0, 6, 2))
             if t in (o1, o2):
                 continue
@@ -191,3 +285,23 @@ def test_linearised_formula_is_the_derivative():
     with pytest.raises(ValueError):
         F.linearize({'items': [F.contr('Y', 'X', 'T', {}, {}, 1.0), F.contr('X', 'T', 'A', {}, {}, 1.0)]},
                     ['T'])
+
+
+def test_add_bc_new_delete_formula_gives_the_ccsd_energy():
+    """The CCSD formula written with ADD, BC, NEW and DELETE items
+    (tests/models.py) gives the hand formula's energy and residuals on the
+    oracle's route 3, and its NEW/DELETE spans give the intermediates
+    lifetimes."""
+    from models import ccsd_with_add_bc_new_delete
+    from oracle.ccsd_route3 import Route3CCSD
+    from synth import synthetic
+    f = ccsd_with_add_bc_new_delete()
+    cmds = {it['cmd'] for it in f['items']}
+    assert {'ADD', 'BC', 'NEW', 'DELETE'} <= cmds
+    ham = synthetic(3, 6, 18, nsym=2, seed=4, sigma=0.1)
+    e0, _, _ = Route3CCSD(ham, ccsd.FORMULA, ccsd.SCHEDULE).solve(tol=1e-11)
+    e1, _, _ = Route3CCSD(ham, f, F.schedule(f['items'])).solve(tol=1e-11)
+    assert abs(e0 - e1) < 1e-12
+    spans = F.lifetimes(f, keep=['T1', 'T2', 'R1', 'R2', 'E'])
+    assert set(spans) == {n for n in ccsd.LISTS if n not in ccsd.INTEGRALS and
+                          n not in ('T1', 'T2', 'R1', 'R2', 'E')}

@@ -271,3 +271,48 @@ def test_ccsdt_ch2_dz_matches_reference_energy():
     print('ccsdt refined', rr['f32_iterations'], rr['linear_iterations'], rr['f64_evaluations'], flush=True)
     assert rr['converged'] and rr['f64_evaluations'] <= 2
     assert abs(ham.e_ref + rr['energy'] - ref) < 1e-10
+
+
+def test_add_bc_new_delete_formula_on_the_engine():
+    """ADD items (atomic sums in the initialisation launch), BC and
+    NEW/DELETE: the CCSD formula written with them gives the hand formula's
+    energy on the GPU, and so does the same formula with its intermediates
+    in a shared pool (pool=True)."""
+    _torch()
+    import ccsd
+    from models import ccsd_with_add_bc_new_delete
+    ham = synthetic(4, 12, 36, nsym=2, seed=13, sigma=0.1)
+    print('add/bc/new/delete formula 4/12/2', flush=True)
+    e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
+    f = ccsd_with_add_bc_new_delete()
+    for pool in (False, True):
+        c = ccsd.CCSD(ham, formula=f, pool=pool)
+        r = c.solve(tol=1e-10, maxit=80)
+        print('pool', pool, r['iterations'], r['energy'] - e0, c.list_bytes(), flush=True)
+        assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11
+
+
+def test_ccsdt_pooled_intermediates():
+    """CCSDT (generated formula) with intermediates sharing memory by
+    lifetime: the CH2/DZ known answer (test/CC-energy-I.e.ref:606) with less
+    device memory for the lists than one buffer per list."""
+    import json
+    import os
+    _torch()
+    import ccgen
+    import ccsd
+    from dalton import hamiltonian_from_dalton
+    gold = os.path.join(os.path.dirname(__file__), 'golden')
+    ham = hamiltonian_from_dalton(os.path.join(gold, 'ch2_dz_SIRIFC.gz'),
+                                  os.path.join(gold, 'ch2_dz_MOTWOINT.gz'), frozen=(1, 0, 0, 0),
+                                  deleted=(2, 1, 0, 0))
+    ref = json.load(open(os.path.join(gold, 'ch2_dz.json')))['e_ccsdt_ref']
+    f = ccgen.generate(3)
+    c = ccsd.CCSD(ham, formula=f, pool=True)
+    full = ccsd.CCSD(ham, formula=f, plans=False).list_bytes()
+    print('ccsdt pooled bytes', c.list_bytes(), 'unpooled', full, flush=True)
+    assert c.list_bytes() < full
+    res = c.solve(tol=1e-11, maxit=100)
+    assert abs(ham.e_ref + res['energy'] - ref) < 1e-10
+    with pytest.raises(ValueError):
+        c.build_linear()
