@@ -23,11 +23,11 @@ sys.path.insert(0, HERE)
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--out', default=os.path.join(HERE, 'precond', 'ccsd_mlp.npz'))
-    ap.add_argument('--hamiltonians', type=int, default=16)
+    ap.add_argument('--hamiltonians', type=int, default=200)
     ap.add_argument('--nocc', type=int, nargs=2, default=[10, 24])
     ap.add_argument('--nvir', type=int, nargs=2, default=[50, 120])
-    ap.add_argument('--per-iter', type=int, default=8192)
-    ap.add_argument('--steps', type=int, default=2000)
+    ap.add_argument('--per-iter', type=int, default=256)
+    ap.add_argument('--steps', type=int, default=4000)
     ap.add_argument('--batch', type=int, default=65536)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--seed', type=int, default=100)
