@@ -6,7 +6,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libgecco_hip.so')
-SOURCES = ['contract_kernels.hip', 'plan.cpp', 'enumerate.cpp', 'solver.hip', 'mlp.hip', 'op1op2.cpp']
+SOURCES = ['contract_kernels.hip', 'plan.cpp', 'enumerate.cpp', 'solver.hip', 'mlp.hip', 'op1op2.cpp',
+           'collect_scatter.hip']
 HIPFLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
             '-munsafe-fp-atomics', '-Wall']

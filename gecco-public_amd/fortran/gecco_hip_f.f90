@@ -111,8 +111,8 @@ contains
 end module gecco_hip_f
 
 
-! Checked interfaces of the two kernel-call entries, for callers that want
-! them; an F77 caller with implicit interfaces (as GeCCo's enumerator) binds
+! Checked interfaces of the kernel-call entries and of the route-3 collect
+! and scatter steps, for callers that want them; an F77 caller with implicit interfaces (as GeCCo's enumerator) binds
 ! to the same symbols without this module, and a scope cannot use both.
 module gecco_hip_kernels_f
   use iso_c_binding
@@ -166,6 +166,42 @@ module gecco_hip_kernels_f
            map_info_1c(*), map_info_1a(*), map_info_2c(*), map_info_2a(*), &
            map_ex1ex2c(*), map_ex1ex2a(*), map_ex1cntc(*), map_ex1cnta(*), &
            map_ex2cntc(*), map_ex2cnta(*)
+    end subroutine
+
+    ! route-3 collect step (contract/collect_block_impr.f:2-17)
+    subroutine collect_block_impr(xblk, xop, kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, &
+         k_len, kca_trp, lc_st0, lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len, &
+         map_kl_c, map_kl_a, ncblk_op, nablk_op, ldim_opc, ldim_opa, ncblk_k, nablk_k, &
+         ncblk_l, nablk_l, lstr_kc, lstr_ka, lstr_lc, lstr_la, nstr_kc, nstr_ka, nstr_lc, &
+         nstr_la, ireo_kc, ireo_ka, ireo_lc, ireo_la) bind(C, name='collect_block_impr_')
+      import :: c_double, c_int64_t
+      real(c_double), intent(out) :: xblk(*)
+      real(c_double), intent(in) :: xop(*)
+      integer(c_int64_t), intent(in) :: kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, k_len, &
+           kca_trp, lc_st0, lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len, ncblk_op, nablk_op, &
+           ncblk_k, nablk_k, ncblk_l, nablk_l
+      integer(c_int64_t), intent(in) :: map_kl_c(*), map_kl_a(*), ldim_opc(*), ldim_opa(*), &
+           lstr_kc(*), lstr_ka(*), lstr_lc(*), lstr_la(*), nstr_kc(*), nstr_ka(*), &
+           nstr_lc(*), nstr_la(*), ireo_kc(*), ireo_ka(*), ireo_lc(*), ireo_la(*)
+    end subroutine
+
+    ! route-3 scatter step (contract/scatter_block.f:2-20)
+    subroutine scatter_block(xop, xblk, kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, &
+         k_len, lc_st0, lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len, &
+         map_kl_c, map_kl_a, ncblk_op, nablk_op, ldim_opc, ldim_opa, ncblk_k, nablk_k, &
+         ncblk_l, nablk_l, lstr_kc, lstr_ka, lstr_lc, lstr_la, nstr_kc, nstr_ka, nstr_lc, &
+         nstr_la, ireo_kc, ireo_ka, ireo_lc, ireo_la, istr_kc, istr_ka, istr_lc, istr_la) &
+         bind(C, name='scatter_block_')
+      import :: c_double, c_int64_t
+      real(c_double), intent(inout) :: xop(*)
+      real(c_double), intent(in) :: xblk(*)
+      integer(c_int64_t), intent(in) :: kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, k_len, &
+           lc_st0, lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len, ncblk_op, nablk_op, &
+           ncblk_k, nablk_k, ncblk_l, nablk_l
+      integer(c_int64_t), intent(in) :: map_kl_c(*), map_kl_a(*), ldim_opc(*), ldim_opa(*), &
+           lstr_kc(*), lstr_ka(*), lstr_lc(*), lstr_la(*), nstr_kc(*), nstr_ka(*), &
+           nstr_lc(*), nstr_la(*), ireo_kc(*), ireo_ka(*), ireo_lc(*), ireo_la(*)
+      integer(c_int64_t), intent(inout) :: istr_kc(*), istr_ka(*), istr_lc(*), istr_la(*)
     end subroutine
   end interface
 end module gecco_hip_kernels_f

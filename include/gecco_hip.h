@@ -99,6 +99,51 @@ void contr_blk1blk2_wmaps_c_(
     const int64_t *map_ex1cntc, const int64_t *map_ex1cnta,
     const int64_t *map_ex2cntc, const int64_t *map_ex2cnta);
 
+/* The collect step of route 3 (contract/collect_block_impr.f:2-17, called
+ * from contract/contr_blk1blk2_blocked_mm.f for each operand batch): the
+ * batch block xblk(k_len, l_len) = sign * xop(element) of the K strings
+ * [ka_st0/kc_st0 .. ka_nd0/kc_nd0] and L strings [la_st0/lc_st0 ..
+ * la_nd0/lc_nd0] through the string maps map_kl_c / map_kl_a, zero where a
+ * map entry is 0; kca_trp != 0 runs K's A strings fastest (the operand
+ * stored as Op(CNT^+)).  Same arguments and meaning as the reference, all
+ * by reference (int64).  The host computes each block element's source
+ * and sign; the GPU moves the data.  xblk and xop may be host arrays
+ * (staged) or device memory; errors are recorded for gecco_hip_last_error. */
+void collect_block_impr_(
+    double *xblk, const double *xop, const int64_t *kc_st0, const int64_t *kc_nd0,
+    const int64_t *kc_len, const int64_t *ka_st0, const int64_t *ka_nd0, const int64_t *ka_len,
+    const int64_t *k_len, const int64_t *kca_trp, const int64_t *lc_st0, const int64_t *lc_nd0,
+    const int64_t *lc_len, const int64_t *la_st0, const int64_t *la_nd0, const int64_t *la_len,
+    const int64_t *l_len, const int64_t *map_kl_c, const int64_t *map_kl_a,
+    const int64_t *ncblk_op, const int64_t *nablk_op, const int64_t *ldim_opc,
+    const int64_t *ldim_opa, const int64_t *ncblk_k, const int64_t *nablk_k,
+    const int64_t *ncblk_l, const int64_t *nablk_l, const int64_t *lstr_kc,
+    const int64_t *lstr_ka, const int64_t *lstr_lc, const int64_t *lstr_la,
+    const int64_t *nstr_kc, const int64_t *nstr_ka, const int64_t *nstr_lc,
+    const int64_t *nstr_la, const int64_t *ireo_kc, const int64_t *ireo_ka,
+    const int64_t *ireo_lc, const int64_t *ireo_la);
+
+/* The scatter step of route 3 (contract/scatter_block.f:2-20): xop(element)
+ * += sign * xblk(k, l) over the batch, through the same kind of maps as
+ * collect_block_impr_ (atomic adds on the GPU, so elements that several
+ * block entries reach sum in an unspecified order).  istr_kc, istr_ka,
+ * istr_lc, istr_la are the reference's work arrays and are not touched.
+ * Host arrays are staged, device pointers used in place. */
+void scatter_block_(
+    double *xop, const double *xblk, const int64_t *kc_st0, const int64_t *kc_nd0,
+    const int64_t *kc_len, const int64_t *ka_st0, const int64_t *ka_nd0, const int64_t *ka_len,
+    const int64_t *k_len, const int64_t *lc_st0, const int64_t *lc_nd0, const int64_t *lc_len,
+    const int64_t *la_st0, const int64_t *la_nd0, const int64_t *la_len, const int64_t *l_len,
+    const int64_t *map_kl_c, const int64_t *map_kl_a, const int64_t *ncblk_op,
+    const int64_t *nablk_op, const int64_t *ldim_opc, const int64_t *ldim_opa,
+    const int64_t *ncblk_k, const int64_t *nablk_k, const int64_t *ncblk_l,
+    const int64_t *nablk_l, const int64_t *lstr_kc, const int64_t *lstr_ka,
+    const int64_t *lstr_lc, const int64_t *lstr_la, const int64_t *nstr_kc,
+    const int64_t *nstr_ka, const int64_t *nstr_lc, const int64_t *nstr_la,
+    const int64_t *ireo_kc, const int64_t *ireo_ka, const int64_t *ireo_lc,
+    const int64_t *ireo_la, int64_t *istr_kc, int64_t *istr_ka, int64_t *istr_lc,
+    int64_t *istr_la);
+
 /* The one-call plans kept by contr_blk1blk2_blocked_mm_ and
  * contr_blk1blk2_wmaps_c_ (at most 16384, or about 1 GB of device index
  * data; beyond that the cache is emptied): *entries = their number after

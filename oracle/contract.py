@@ -85,8 +85,17 @@ def lib():
         _LIB = ctypes.CDLL(path)
         _LIB.orc_contr_blk1blk2_blocked_mm.restype = ctypes.c_int
         _LIB.orc_set_dgemm.argtypes = [ctypes.c_void_p]
+        _LIB.orc_set_collect_scatter.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         use_blas(_USE_BLAS)
     return _LIB
+
+
+def use_collect_scatter(collect=None, scatter=None):
+    """Route 3's collect and scatter steps through external functions with
+    the reference's Fortran argument lists (addresses; None: the
+    restatements) -- the test hook for the product's collect_block_impr_
+    and scatter_block_ entries."""
+    lib().orc_set_collect_scatter(collect, scatter)
 
 
 def _sub(a, b):

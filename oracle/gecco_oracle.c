@@ -377,6 +377,81 @@ static dgemm64_fn blas_dgemm = 0;
 
 void orc_set_dgemm(void *fn) { blas_dgemm = (dgemm64_fn)fn; }
 
+/* Test hook: route 3's collect and scatter steps through external functions
+ * with the reference's Fortran argument lists (every argument by reference;
+ * the product's collect_block_impr_ / scatter_block_ entries), so that a
+ * whole route-3 contraction checks them call by call; NULL restores the
+ * restatements above. */
+typedef void (*collect_fn)(double *, const double *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *);
+typedef void (*scatter_fn)(double *, const double *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, const I *, const I *, const I *, const I *,
+                           const I *, const I *, I *, I *, I *, I *);
+static collect_fn ext_collect = 0;
+static scatter_fn ext_scatter = 0;
+
+void orc_set_collect_scatter(void *collect, void *scatter) {
+    ext_collect = (collect_fn)collect;
+    ext_scatter = (scatter_fn)scatter;
+}
+
+static void collect(
+    double *xblk, const double *xop, I kc_st0, I kc_nd0, I kc_len, I ka_st0, I ka_nd0,
+    I ka_len, I k_len, I kca_trp, I lc_st0, I lc_nd0, I lc_len, I la_st0, I la_nd0, I la_len,
+    I l_len, const I *map_kl_c, const I *map_kl_a, I ncblk_op, I nablk_op, const I *ldim_opc,
+    const I *ldim_opa, I ncblk_k, I nablk_k, I ncblk_l, I nablk_l, const I *lstr_kc,
+    const I *lstr_ka, const I *lstr_lc, const I *lstr_la, const I *nstr_kc, const I *nstr_ka,
+    const I *nstr_lc, const I *nstr_la, const I *ireo_kc, const I *ireo_ka, const I *ireo_lc,
+    const I *ireo_la) {
+    if (!ext_collect) {
+        orc_collect_block_impr(xblk, xop, kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, k_len,
+                               kca_trp, lc_st0, lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len,
+                               map_kl_c, map_kl_a, ncblk_op, nablk_op, ldim_opc, ldim_opa, ncblk_k,
+                               nablk_k, ncblk_l, nablk_l, lstr_kc, lstr_ka, lstr_lc, lstr_la,
+                               nstr_kc, nstr_ka, nstr_lc, nstr_la, ireo_kc, ireo_ka, ireo_lc,
+                               ireo_la);
+        return;
+    }
+    ext_collect(xblk, xop, &kc_st0, &kc_nd0, &kc_len, &ka_st0, &ka_nd0, &ka_len, &k_len, &kca_trp,
+                &lc_st0, &lc_nd0, &lc_len, &la_st0, &la_nd0, &la_len, &l_len, map_kl_c, map_kl_a,
+                &ncblk_op, &nablk_op, ldim_opc, ldim_opa, &ncblk_k, &nablk_k, &ncblk_l, &nablk_l,
+                lstr_kc, lstr_ka, lstr_lc, lstr_la, nstr_kc, nstr_ka, nstr_lc, nstr_la, ireo_kc,
+                ireo_ka, ireo_lc, ireo_la);
+}
+
+static void scatter(
+    double *xop, const double *xblk, I kc_st0, I kc_nd0, I kc_len, I ka_st0, I ka_nd0, I ka_len,
+    I k_len, I lc_st0, I lc_nd0, I lc_len, I la_st0, I la_nd0, I la_len, I l_len,
+    const I *map_kl_c, const I *map_kl_a, I ncblk_op, I nablk_op, const I *ldim_opc,
+    const I *ldim_opa, I ncblk_k, I nablk_k, I ncblk_l, I nablk_l, const I *lstr_kc,
+    const I *lstr_ka, const I *lstr_lc, const I *lstr_la, const I *nstr_kc, const I *nstr_ka,
+    const I *nstr_lc, const I *nstr_la, const I *ireo_kc, const I *ireo_ka, const I *ireo_lc,
+    const I *ireo_la, I *istr_kc, I *istr_ka, I *istr_lc, I *istr_la) {
+    if (!ext_scatter) {
+        orc_scatter_block(xop, xblk, kc_st0, kc_nd0, kc_len, ka_st0, ka_nd0, ka_len, k_len, lc_st0,
+                          lc_nd0, lc_len, la_st0, la_nd0, la_len, l_len, map_kl_c, map_kl_a,
+                          ncblk_op, nablk_op, ldim_opc, ldim_opa, ncblk_k, nablk_k, ncblk_l,
+                          nablk_l, lstr_kc, lstr_ka, lstr_lc, lstr_la, nstr_kc, nstr_ka, nstr_lc,
+                          nstr_la, ireo_kc, ireo_ka, ireo_lc, ireo_la, istr_kc, istr_ka, istr_lc,
+                          istr_la);
+        return;
+    }
+    ext_scatter(xop, xblk, &kc_st0, &kc_nd0, &kc_len, &ka_st0, &ka_nd0, &ka_len, &k_len, &lc_st0,
+                &lc_nd0, &lc_len, &la_st0, &la_nd0, &la_len, &l_len, map_kl_c, map_kl_a, &ncblk_op,
+                &nablk_op, ldim_opc, ldim_opa, &ncblk_k, &nablk_k, &ncblk_l, &nablk_l, lstr_kc,
+                lstr_ka, lstr_lc, lstr_la, nstr_kc, nstr_ka, nstr_lc, nstr_la, ireo_kc, ireo_ka,
+                ireo_lc, ireo_la, istr_kc, istr_ka, istr_lc, istr_la);
+}
+
 /* dgemm('t','n',m,n,k,alpha,A,lda,B,ldb,0,C,ldc): the BLAS routine when one
  * is set, else a plain loop */
 static void ref_dgemm_tn(I m, I n, I k, double alpha, const double *A, I lda,
@@ -502,7 +577,7 @@ int orc_contr_blk1blk2_blocked_mm(
         orc_idxstnd_batch(&ncnt, &cc_bst, &ca_bst, &cc_bnd, &ca_bnd, cnt_batch,
                           nstr_cntc_tot, nstr_cnta_tot, maxlen_cnt_batch);
         if (op1shorter) {
-            orc_collect_block_impr(
+            collect(
                 xscr + idxopsscr - 1, xop1, cc_bst, cc_bnd, nstr_cntc_tot, ca_bst,
                 ca_bnd, nstr_cnta_tot, ncnt, 0, 1, nstr_ex1c_tot, nstr_ex1c_tot, 1,
                 nstr_ex1a_tot, nstr_ex1a_tot, nstr_ex1c_tot * nstr_ex1a_tot,
@@ -512,7 +587,7 @@ int orc_contr_blk1blk2_blocked_mm(
                 nstr_cnta1, nstr_ex1c1, nstr_ex1a1, ireo_cntc1, ireo_cnta1,
                 ireo_ex1c1, ireo_ex1a1);
         } else {
-            orc_collect_block_impr(
+            collect(
                 xscr + idxopsscr - 1, xop2, ca_bst, ca_bnd, nstr_cnta_tot, cc_bst,
                 cc_bnd, nstr_cntc_tot, ncnt, 1, 1, nstr_ex2c_tot, nstr_ex2c_tot, 1,
                 nstr_ex2a_tot, nstr_ex2a_tot, nstr_ex2c_tot * nstr_ex2a_tot,
@@ -527,7 +602,7 @@ int orc_contr_blk1blk2_blocked_mm(
             orc_idxstnd_batch(&nexl, &lc_bst, &la_bst, &lc_bnd, &la_bnd, exl_batch,
                               nstr_exlc_tot, nstr_exla_tot, maxlen_exl_batch);
             if (op1shorter) {
-                orc_collect_block_impr(
+                collect(
                     xscr + idxoplscr - 1, xop2, ca_bst, ca_bnd, nstr_cnta_tot,
                     cc_bst, cc_bnd, nstr_cntc_tot, ncnt, 1, lc_bst, lc_bnd,
                     nstr_exlc_tot, la_bst, la_bnd, nstr_exla_tot, nexl, map_ex2cntc,
@@ -537,7 +612,7 @@ int orc_contr_blk1blk2_blocked_mm(
                     nstr_ex2c2, nstr_ex2a2, ireo_cnta2, ireo_cntc2, ireo_ex2c2,
                     ireo_ex2a2);
             } else {
-                orc_collect_block_impr(
+                collect(
                     xscr + idxoplscr - 1, xop1, cc_bst, cc_bnd, nstr_cntc_tot,
                     ca_bst, ca_bnd, nstr_cnta_tot, ncnt, 0, lc_bst, lc_bnd,
                     nstr_exlc_tot, la_bst, la_bnd, nstr_exla_tot, nexl, map_ex1cntc,
@@ -557,7 +632,7 @@ int orc_contr_blk1blk2_blocked_mm(
                     ref_dgemm_tn(nexs, nexl, ncnt, xfac, xscr + idx - 1, ncnt,
                                  xscr + idxoplscr - 1, ncnt, xscr + idxop1op2scr - 1,
                                  nexs);
-                    orc_scatter_block(
+                    scatter(
                         xop1op2, xscr + idxop1op2scr - 1, sc_bst, sc_bnd,
                         nstr_exsc_tot, sa_bst, sa_bnd, nstr_exsa_tot, nexs, lc_bst,
                         lc_bnd, nstr_exlc_tot, la_bst, la_bnd, nstr_exla_tot, nexl,
@@ -570,7 +645,7 @@ int orc_contr_blk1blk2_blocked_mm(
                 } else {
                     ref_dgemm_tn(nexl, nexs, ncnt, xfac, xscr + idxoplscr - 1, ncnt,
                                  xscr + idx - 1, ncnt, xscr + idxop1op2scr - 1, nexl);
-                    orc_scatter_block(
+                    scatter(
                         xop1op2, xscr + idxop1op2scr - 1, lc_bst, lc_bnd,
                         nstr_exlc_tot, la_bst, la_bnd, nstr_exla_tot, nexl, sc_bst,
                         sc_bnd, nstr_exsc_tot, sa_bst, sa_bnd, nstr_exsa_tot, nexs,

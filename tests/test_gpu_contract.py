@@ -687,3 +687,39 @@ def test_contr_op1op2_entry_matches_oracle(name, nsym, nocc, nvir):
         gecco_hip.contr_op1op2(*args, dev[0], dev[1], dev[2], irreps, update=update, tra=tra)
         torch.cuda.synchronize()
         assert _relerr(dev[2].cpu().numpy(), ref75) <= TOL
+
+
+@pytest.mark.parametrize('name', ['ladder', 'ring', 't1t1', 'energy'])
+@pytest.mark.parametrize('lenblock,lenscr', [(200, None), (7, 900)])
+def test_collect_and_scatter_entries_inside_route_3(name, lenblock, lenscr):
+    """collect_block_impr_ and scatter_block_ (the reference's Fortran
+    argument lists, host arrays): the oracle's route 3 run with its collect
+    and scatter steps replaced by the two entries gives the oracle's own
+    result.  Small lenblock / lenscr split the strings and the contraction
+    range into batches (partial KA/LA ranges, kca_trp for Op2), so every
+    branch of the batch bookkeeping is used.  Collect moves data only (exact);
+    scatter adds atomically (summation order: 1e-13 relative)."""
+    _torch()
+    import ctypes
+    import gecco_hip
+    from oracle.contract import use_collect_scatter
+    if name not in CASES:
+        pytest.skip(f'no case {name}')
+    print(f'collect/scatter {name} lenblock={lenblock}', flush=True)
+    orbs = orbitals(4, 6, 2)
+    c, a, b, r = CASES[name](orbs)
+    x1 = spin_symmetric(a, 41)
+    x2 = spin_symmetric(b, 42)
+    x0 = spin_symmetric(r, 43)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=False, screen=False, lenblock=lenblock, lenscr=lenscr)
+    got = x0.copy()
+    fc = ctypes.cast(gecco_hip._lib.collect_block_impr_, ctypes.c_void_p).value
+    fs = ctypes.cast(gecco_hip._lib.scatter_block_, ctypes.c_void_p).value
+    try:
+        use_collect_scatter(fc, fs)
+        c.run(x1, x2, got, route=3, use_tr=False, screen=False, lenblock=lenblock, lenscr=lenscr)
+    finally:
+        use_collect_scatter(None, None)
+    assert gecco_hip._lib.gecco_hip_last_error(None, 0) == 0
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())

@@ -157,3 +157,40 @@ def test_transposed_layout_differs_from_plain():
     bad = np.zeros(stored[2].length)
     c.run(xs[0], xs[1], bad, route=3, use_tr=False, blocks=blocks)
     assert np.abs(got - bad).max() > 1e-3 * np.abs(got).max()
+
+
+def test_collect_scatter_hook_is_used():
+    """The route-3 test hook (oracle.contract.use_collect_scatter) really
+    replaces the collect and scatter steps: with a collect that leaves each
+    block at zero, or a scatter that adds nothing, the contraction adds
+    nothing; with the hook cleared, it adds the oracle's result."""
+    import ctypes
+    from models import CASES, orbitals, spin_symmetric
+    from oracle.contract import use_collect_scatter
+    c, a, b, r = CASES['ladder'](orbitals(4, 6, 2))
+    x1, x2 = spin_symmetric(a, 1), spin_symmetric(b, 2)
+    P = ctypes.c_void_p
+    calls = {'c': 0, 's': 0}
+
+    def coll(xblk, *args):
+        calls['c'] += 1                   # xblk keeps its zeros
+
+    def scat(*args):
+        calls['s'] += 1
+    cf = ctypes.CFUNCTYPE(None, *([P] * 39))(coll)
+    sf = ctypes.CFUNCTYPE(None, *([P] * 42))(scat)
+    for hook in ((cf, None), (None, sf)):
+        xr = np.zeros(r.length)
+        try:
+            use_collect_scatter(ctypes.cast(hook[0], P).value if hook[0] else None,
+                                ctypes.cast(hook[1], P).value if hook[1] else None)
+            c.run(x1, x2, xr, route=3, use_tr=False, screen=False)
+        finally:
+            use_collect_scatter(None, None)
+        if hook[0]:
+            assert calls['c'] > 0 and not xr.any()   # zero blocks: nothing added
+        else:
+            assert calls['s'] > 0 and not xr.any()
+    xr = np.zeros(r.length)
+    c.run(x1, x2, xr, route=3, use_tr=False, screen=False)
+    assert np.abs(xr).max() > 0
