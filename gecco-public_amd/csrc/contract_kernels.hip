@@ -164,10 +164,18 @@ __device__ __forceinline__ bool skipped(const int32_t *__restrict__ skip, int gr
     return skip != nullptr && grp >= 0 && ldg(skip + grp) != 0;
 }
 
+// Norm screening: the kernel call of block b is skipped when the squared
+// norm of its A or B operand block fell below 1e-28 (zero: per operand-block
+// flags of the plan's norm launch; nullptr when the plan does not screen).
+__device__ __forceinline__ bool screened(const int32_t *__restrict__ zero, const DevBlock &b) {
+    return zero != nullptr && ((b.zero_a >= 0 && ldg(zero + b.zero_a) != 0) ||
+                               (b.zero_b >= 0 && ldg(zero + b.zero_b) != 0));
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void grouped_contract(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
+    T *const *__restrict__ bufs, const int32_t *__restrict__ skip, const int32_t *__restrict__ zero) {
     typedef typename Acc<T>::type AccT;
     __shared__ T As[BK][BM + 1];
     __shared__ T Bs[BK][BN + 1];
@@ -181,6 +189,7 @@ __global__ __launch_bounds__(NT) void grouped_contract(
         const DevTask task = tasks[t];
         if (skipped(skip, task.grp)) continue;
         const DevBlock &b = blocks[task.blk];
+        if (screened(zero, b)) continue;
         const int M = b.M, N = b.N;
         const int m0 = task.m0, n0 = task.n0, k1 = task.k1;
         OpView<T> va, vb;
@@ -682,7 +691,7 @@ constexpr int AFFINE_WAVES = TBM * TBN >= 128 * 128 ? 2
 template <int TBM, int TBN, int TBK, typename T, int PF = 1, int WM = 2>
 __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    T *const *__restrict__ bufs, const int32_t *__restrict__ skip) {
+    T *const *__restrict__ bufs, const int32_t *__restrict__ skip, const int32_t *__restrict__ zero) {
     typedef typename Acc<T>::type AccT;
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int WN = 4 / WM, RM = TBM / WM, RN = TBN / WN;   // rows and columns per wave
@@ -704,7 +713,7 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
 
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const DevTask task = tasks[t];
-        if (skipped(skip, task.grp)) continue;
+        if (skipped(skip, task.grp) || screened(zero, blocks[task.blk])) continue;
         if (task.pad == 1) {   // tile of at most 16 x 16
             skinny_task(task, blocks, bufs, lds, tid);
             continue;
@@ -1196,14 +1205,15 @@ template <typename T> static T *const *tbl(void *const *b) { return reinterpret_
 
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, void *const *bufs, int grid,
-                                    const int32_t *skip, int elem, hipStream_t stream) {
+                                    const int32_t *skip, const int32_t *zero, int elem,
+                                    hipStream_t stream) {
     if (ntasks <= 0) return 0;
     if (elem == 4)
         hipLaunchKernelGGL(gecco::grouped_contract<float>, dim3(grid), dim3(gecco::NT), 0, stream,
-                           blocks, tasks, ntasks, tbl<float>(bufs), skip);
+                           blocks, tasks, ntasks, tbl<float>(bufs), skip, zero);
     else
         hipLaunchKernelGGL(gecco::grouped_contract<double>, dim3(grid), dim3(gecco::NT), 0, stream,
-                           blocks, tasks, ntasks, tbl<double>(bufs), skip);
+                           blocks, tasks, ntasks, tbl<double>(bufs), skip, zero);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -1222,24 +1232,24 @@ extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
 template <typename T>
 static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *tasks, int ntasks,
                          T *const *bufs, int grid, int variant, const int32_t *skip,
-                         hipStream_t stream) {
+                         const int32_t *zero, hipStream_t stream) {
     using namespace gecco;
     switch (variant) {
-    case 0: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 1: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 2: hipLaunchKernelGGL((grouped_contract_affine<64, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 0: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 1: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 2: hipLaunchKernelGGL((grouped_contract_affine<64, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
     // two K steps of loads in flight
-    case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
-    case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
     // 64 x 64 x 16 with the waves stacked along M
-    case 9: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip); break;
+    case 9: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
     // split-bfloat16 GEMM tiles (float lists only)
     case 8:
         if constexpr (sizeof(T) == 4) {
-            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 3>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip);
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 3>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero);
             break;
         } else {
             return 1;
@@ -1252,14 +1262,68 @@ static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *ta
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            void *const *bufs, int grid, int variant,
-                                           const int32_t *skip, int elem, hipStream_t stream) {
+                                           const int32_t *skip, const int32_t *zero, int elem,
+                                           hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    if (elem == 4) return launch_affine(blocks, tasks, ntasks, tbl<float>(bufs), grid, variant, skip, stream);
-    return launch_affine(blocks, tasks, ntasks, tbl<double>(bufs), grid, variant, skip, stream);
+    if (elem == 4) return launch_affine(blocks, tasks, ntasks, tbl<float>(bufs), grid, variant, skip, zero, stream);
+    return launch_affine(blocks, tasks, ntasks, tbl<double>(bufs), grid, variant, skip, zero, stream);
 }
 
 
 extern "C" int gecco_direct_stage_cap(void) { return gecco::DIRECT_STAGE; }
+
+namespace gecco {
+// Norm screening, first pass: partial[t] = sum of x^2 over chunk t (one
+// workgroup per chunk, fixed-order block reduction).
+template <typename T>
+__global__ __launch_bounds__(256) void norm_chunks(const DevNormTask *__restrict__ tasks,
+                                                   T *const *__restrict__ bufs, double *__restrict__ partial) {
+    const DevNormTask t = tasks[blockIdx.x];
+    const T *__restrict__ x = bufs[t.buf] + t.off;
+    double acc = 0.0;
+    for (int64_t e = t.e0 + threadIdx.x; e < t.e1; e += blockDim.x) {
+        const double v = (double)ldg(x + e);
+        acc += v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Second pass: zero[j] = 1 when the chunks [rows[j], rows[j+1]) of operand
+// block j sum to less than 1e-28 (contract/contr_op1op2_wmaps_c.f:1652-1655).
+__global__ __launch_bounds__(256) void norm_flags(const double *__restrict__ partial,
+                                                  const int64_t *__restrict__ rows, int64_t njobs,
+                                                  int32_t *__restrict__ zero) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    double s = 0.0;
+    for (int64_t t = rows[j]; t < rows[j + 1]; ++t) s += partial[t];
+    zero[j] = s < 1e-28 ? 1 : 0;
+}
+}  // namespace gecco
+
+extern "C" int gecco_launch_norms(const gecco::DevNormTask *tasks, int64_t ntasks, const int64_t *rows,
+                                  int64_t njobs, void *const *bufs, double *partial, int32_t *zero, int elem,
+                                  hipStream_t stream) {
+    if (njobs <= 0) return 0;
+    if (ntasks > 0) {
+        if (elem == 4)
+            hipLaunchKernelGGL(gecco::norm_chunks<float>, dim3((unsigned)ntasks), dim3(256), 0, stream, tasks,
+                               tbl<float>(bufs), partial);
+        else
+            hipLaunchKernelGGL(gecco::norm_chunks<double>, dim3((unsigned)ntasks), dim3(256), 0, stream, tasks,
+                               tbl<double>(bufs), partial);
+    }
+    hipLaunchKernelGGL(gecco::norm_flags, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0, stream, partial,
+                       rows, njobs, zero);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
 
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, void *const *bufs, void *scratch,

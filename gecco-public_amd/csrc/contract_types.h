@@ -38,7 +38,16 @@ struct DevBlock {
     // that add into the same result block through the same result map; a
     // task's contraction range continues into the next block of its chain
     int32_t next;
+    // norm-screening jobs of the A and B operand blocks (-1: not screened)
+    int32_t zero_a, zero_b;
     DevOperand A, B, C;               // A: (k,m)  B: (k,n)  C: (m,n)
+};
+
+// One chunk of the squared norm of an operand block (norm screening):
+// sum of x^2 over elements [e0, e1) of buffer slot buf from element off.
+struct DevNormTask {
+    int64_t off, e0, e1;
+    int32_t buf, pad;
 };
 
 struct DevTask {

@@ -126,6 +126,9 @@ void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const in
         }
         gecco_hip_plan *p = nullptr;
         int rc = gecco_hip_plan_create(&p);
+        // kernel calls with a zero operand block are skipped, as in the
+        // reference (contract/contr_op1op2_wmaps_c.f:1652-1655,1767-1770)
+        if (!rc) rc = gecco_hip_plan_set_screen(p, 1);
         // update = .false.: the result list starts from zero
         // (contract/contr_op1op2_wmaps_c.f:644,680)
         if (!rc && !*update) rc = gecco_hip_plan_add_init(p, 2, 0, lens[2], -1, 0, 0.0);

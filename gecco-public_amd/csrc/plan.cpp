@@ -22,7 +22,9 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -33,12 +35,17 @@
 // launchers (contract_kernels.hip); elem: 8 = float64 lists, 4 = float32
 extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::DevTask *tasks,
                                     int ntasks, void *const *bufs, int grid,
-                                    const int32_t *skip, int elem, hipStream_t stream);
+                                    const int32_t *skip, const int32_t *zero, int elem,
+                                    hipStream_t stream);
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            void *const *bufs, int grid, int variant,
-                                           const int32_t *skip, int elem, hipStream_t stream);
+                                           const int32_t *skip, const int32_t *zero, int elem,
+                                           hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
+extern "C" int gecco_launch_norms(const gecco::DevNormTask *tasks, int64_t ntasks, const int64_t *rows,
+                                  int64_t njobs, void *const *bufs, double *partial, int32_t *zero, int elem,
+                                  hipStream_t stream);
 extern "C" int gecco_direct_stage_cap(void);
 extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
                                         const gecco::DevSymTask *tasks, int ntasks,
@@ -411,6 +418,10 @@ struct HostBlock {
     double xfac;
     int32_t buf1, buf2, buf12;
     int64_t off1, off2, off12, len12;
+    // lengths of the operand blocks at off1 / off2 (the whole Ms/IRREP
+    // distribution block of each operand: the range the norm screening
+    // looks at)
+    int64_t len1 = 0, len2 = 0;
     int64_t M, N, K;    // kernel orientation
     bool swap;          // A=Op2, B=Op1 when the result is contiguous along EX1
     bool injective;
@@ -483,6 +494,16 @@ struct gecco_hip_plan {
     int64_t ngjobs = 0, ngtasks = 0, scratch_elems = 0;
     int64_t materialize_min = 1;    // min(M, N) from which gathers are materialised
     bool chain = true;              // link strided-GEMM blocks with a common result tile
+    // norm screening (contract/contr_op1op2_wmaps_c.f:1652-1655,1767-1770):
+    // a kernel call whose operand block has a squared norm below 1e-28 is
+    // skipped; the norms of the distinct operand blocks (jobs) are summed in
+    // chunks (partial) and compared (zero) before the contraction launches
+    bool screen = false;
+    gecco::DevNormTask *d_ntasks = nullptr;
+    double *d_npartial = nullptr;
+    int64_t *d_nrow = nullptr;              // chunk range of each job (CSR)
+    int32_t *d_nzero = nullptr;             // 1: the job's block is screened out
+    int64_t nnorm_tasks = 0, nnorm_jobs = 0;
     int64_t skinny_max = 16;        // min(M, N) up to which 16 x 16 wave-split-K tiles are used
     int64_t skinny_kchunk = 2048;   // contraction length of one such task
     int64_t nchained = 0;           // blocks linked behind a chain head
@@ -586,6 +607,11 @@ static int build_block(const gecco_hip_block_args *a, HostBlock &hb) {
     hb.injective = result_injective(r12, len12);
     hb.swap = first_is_contiguous(r12);
     hb.K = cnt.total();
+    int64_t lop1 = 1, lop2 = 1;
+    for (int64_t c = 0; c < a->ncblk_op1 + a->nablk_op1; ++c) lop1 *= a->lstrop1[c];
+    for (int64_t c = 0; c < a->ncblk_op2 + a->nablk_op2; ++c) lop2 *= a->lstrop2[c];
+    hb.len1 = hb.swap ? lop2 : lop1;
+    hb.len2 = hb.swap ? lop1 : lop2;
     if (!hb.swap) {
         hb.M = ex1.total(); hb.N = ex2.total();
         hb.buf1 = a->buf_op1; hb.off1 = a->off_op1;
@@ -645,6 +671,15 @@ static void dfree(gecco_hip_plan *p, void *ptr) {
 static void plan_free_device(gecco_hip_plan *p) {
     if (p->d_blocks) dfree(p, p->d_blocks);
     if (p->d_tasks) dfree(p, p->d_tasks);
+    if (p->d_ntasks) dfree(p, p->d_ntasks);
+    if (p->d_npartial) dfree(p, p->d_npartial);
+    if (p->d_nrow) dfree(p, p->d_nrow);
+    if (p->d_nzero) dfree(p, p->d_nzero);
+    p->d_ntasks = nullptr;
+    p->d_npartial = nullptr;
+    p->d_nrow = nullptr;
+    p->d_nzero = nullptr;
+    p->nnorm_tasks = p->nnorm_jobs = 0;
     if (p->d_index) dfree(p, p->d_index);
     if (p->d_bufs) dfree(p, p->d_bufs);
     for (auto &tr : p->traces) {
@@ -763,6 +798,7 @@ static void flip_orientation(HostBlock &b) {
     std::swap(b.M, b.N);
     std::swap(b.buf1, b.buf2);
     std::swap(b.off1, b.off2);
+    std::swap(b.len1, b.len2);
     std::swap(b.A, b.B);
     Resolved &c = b.C.r;
     if (!c.table) {
@@ -882,7 +918,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         for (size_t i = 0; i < p->blocks.size(); ++i) {
             chain_head[i] = i;
             tail[i] = i;
-            if (!is_aff[i] || !p->chain) continue;
+            if (!is_aff[i] || !p->chain || p->screen) continue;   // screening skips single calls
             const HostBlock &b = p->blocks[i];
             uint64_t h = 1469598103934665603ull;
             auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
@@ -1019,6 +1055,7 @@ static int plan_finalize(gecco_hip_plan *p) {
         d.K = (int32_t)b.K;
         d.atomic = (ns > 1 || !b.injective || overlap[i]) ? 1 : 0;
         d.next = b.next = chain_next[i];
+        d.zero_a = d.zero_b = -1;
         if (!head) continue;
         // patches of PM x PN tiles with one contraction range share their
         // A panels (PN times) and B panels (PM times); about 16 tiles each
@@ -1256,6 +1293,49 @@ static int plan_finalize(gecco_hip_plan *p) {
                 return g_err;
         }
     }
+    // norm screening: one job per distinct operand block (slot, offset,
+    // length), summed in chunks of NCHUNK elements
+    if (p->screen) {
+        constexpr int64_t NCHUNK = 1 << 16;
+        std::map<std::tuple<int32_t, int64_t, int64_t>, int32_t> jobs;
+        std::vector<DevNormTask> nt;
+        std::vector<int64_t> rows(1, 0);
+        auto job = [&](int32_t buf, int64_t off, int64_t len) -> int32_t {
+            auto key = std::make_tuple(buf, off, len);
+            auto it = jobs.find(key);
+            if (it != jobs.end()) return it->second;
+            const int32_t j = (int32_t)jobs.size();
+            jobs[key] = j;
+            for (int64_t e0 = 0; e0 < len; e0 += NCHUNK) {
+                DevNormTask t;
+                t.off = off;
+                t.e0 = e0;
+                t.e1 = std::min(len, e0 + NCHUNK);
+                t.buf = buf;
+                t.pad = 0;
+                nt.push_back(t);
+            }
+            rows.push_back((int64_t)nt.size());
+            return j;
+        };
+        for (size_t i = 0; i < p->blocks.size(); ++i) {
+            const HostBlock &b = p->blocks[i];
+            dblocks[i].zero_a = job(b.buf1, b.off1, b.len1);
+            dblocks[i].zero_b = job(b.buf2, b.off2, b.len2);
+        }
+        p->nnorm_tasks = (int64_t)nt.size();
+        p->nnorm_jobs = (int64_t)jobs.size();
+        if (p->nnorm_jobs) {
+            if (dmalloc(p, (void **)&p->d_ntasks, std::max<size_t>(1, nt.size()) * sizeof(DevNormTask), "hipMalloc(norm tasks)") ||
+                dmalloc(p, (void **)&p->d_npartial, std::max<size_t>(1, nt.size()) * sizeof(double), "hipMalloc(norm partials)") ||
+                dmalloc(p, (void **)&p->d_nrow, rows.size() * sizeof(int64_t), "hipMalloc(norm rows)") ||
+                dmalloc(p, (void **)&p->d_nzero, (size_t)p->nnorm_jobs * sizeof(int32_t), "hipMalloc(norm flags)"))
+                return g_err;
+            if ((!nt.empty() && check(hipMemcpy(p->d_ntasks, nt.data(), nt.size() * sizeof(DevNormTask), hipMemcpyHostToDevice), "hipMemcpy(norm tasks)")) ||
+                check(hipMemcpy(p->d_nrow, rows.data(), rows.size() * sizeof(int64_t), hipMemcpyHostToDevice), "hipMemcpy(norm rows)"))
+                return g_err;
+        }
+    }
     if (dmalloc(p, (void **)&p->d_blocks, std::max<size_t>(1, dblocks.size()) * sizeof(DevBlock), "hipMalloc(blocks)")) return g_err;
     if (!dblocks.empty() && check(hipMemcpy(p->d_blocks, dblocks.data(), dblocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice), "hipMemcpy(blocks)")) return g_err;
     if (dmalloc(p, (void **)&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask), "hipMalloc(tasks)")) return g_err;
@@ -1396,6 +1476,13 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (p->nitasks &&
         gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, p->d_skip, p->elem, s))
         return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
+    const int32_t *zero = nullptr;
+    if (p->screen && p->nnorm_jobs) {
+        if (gecco_launch_norms(p->d_ntasks, p->nnorm_tasks, p->d_nrow, p->nnorm_jobs, p->d_bufs,
+                               p->d_npartial, p->d_nzero, p->elem, s))
+            return fail(GECCO_HIP_EDEVICE, "norm launch failed: %s", hipGetErrorString(hipGetLastError()));
+        zero = p->d_nzero;
+    }
     const int64_t g0 = p->static_done ? p->ngtasks_static : 0;
     if (p->ngtasks > g0) {
         if (mark(3, true)) return g_err;
@@ -1408,14 +1495,14 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
-                                        p->affine_variant, p->d_skip, p->elem, s))
+                                        p->affine_variant, p->d_skip, zero, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
     if (ng) {
         if (mark(1, true)) return g_err;
         if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, p->d_skip,
-                                 p->elem, s))
+                                 zero, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
@@ -2042,6 +2129,13 @@ int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t 
         return fail(GECCO_HIP_EINVAL, "skip flags must be device memory");
     plan->d_skip = flags;
     plan->nskip = flags ? nflags : 0;
+    return 0;
+}
+
+int gecco_hip_plan_set_screen(gecco_hip_plan *plan, int64_t on) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    plan->screen = on != 0;
     return 0;
 }
 

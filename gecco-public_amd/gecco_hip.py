@@ -98,6 +98,7 @@ _lib.gecco_hip_plan_block_info.argtypes = [ctypes.c_void_p, I64, PI64, I64]
 _lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_set_constant.argtypes = [ctypes.c_void_p, ctypes.c_int32]
 _lib.gecco_hip_plan_set_chain.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_set_screen.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD, PI64]
 _lib.gecco_hip_plan_block_index.argtypes = [ctypes.c_void_p, I64, I64, ctypes.POINTER(ctypes.c_int32),
                                             I64, ctypes.POINTER(ctypes.c_int32), I64]
@@ -343,6 +344,12 @@ class Plan:
         """Slot `slot` keeps its data between executes (materialised gathers
         of it are reused); before finalize."""
         _check(_lib.gecco_hip_plan_set_constant(self._h, int(slot)))
+
+    def set_screen(self, on=True):
+        """Norm screening: skip kernel calls whose operand blocks have a
+        squared norm below 1e-28, checked on the device at every execute
+        (the reference's zero-block test); before finalize."""
+        _check(_lib.gecco_hip_plan_set_screen(self._h, 1 if on else 0))
 
     def set_chain(self, on):
         """Run strided-GEMM kernel calls that add into the same result block

@@ -301,6 +301,13 @@ int gecco_hip_plan_set_materialize(gecco_hip_plan *plan, int64_t min_dim);
  * block -- are run as one GEMM over their concatenated contraction ranges,
  * so each result tile is written once.  Before finalize only. */
 int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
+/* Norm screening (off by default), as contract/contr_op1op2_wmaps_c.f:
+ * 1652-1655,1767-1770: at every execute, a kernel call whose Op1 or Op2
+ * block (the whole Ms/IRREP distribution block of the operand) has a
+ * squared norm below 1e-28 is skipped.  The norms are computed on the
+ * device by a launch before the contractions; screened plans run every
+ * kernel call as its own GEMM (no chains).  Before finalize only. */
+int gecco_hip_plan_set_screen(gecco_hip_plan *plan, int64_t on);
 /* Instance groups, for many independent problems (e.g. CC instances of
  * uneven size) recorded into one plan: every kernel call, list
  * initialisation and symmetrisation recorded after set_group(g) belongs to

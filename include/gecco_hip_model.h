@@ -165,7 +165,9 @@ int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *list, int32_
  *                        memory (host buffers are staged)
  *   ierr                 0, or a GECCO_HIP_E* code (gecco_hip_last_error
  *                        has the message; the shim turns it into quit).
- * Every distinct contraction keeps its lists and plan for the life of the
+ * Kernel calls whose Op1 or Op2 block has a squared norm below 1e-28 are
+ * skipped, as the reference does (gecco_hip_plan_set_screen).  Every
+ * distinct contraction keeps its lists and plan for the life of the
  * process.  The kernels run on the null stream; a host result is copied back
  * before the call returns. */
 void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const int64_t *iocc_op1,

@@ -723,3 +723,45 @@ def test_collect_and_scatter_entries_inside_route_3(name, lenblock, lenscr):
         use_collect_scatter(None, None)
     assert gecco_hip._lib.gecco_hip_last_error(None, 0) == 0
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize('name', ['ladder', 'ring', 't1t1', 'fock_pp'])
+def test_norm_screening_matches_oracle(name):
+    """Plan.set_screen (contract/contr_op1op2_wmaps_c.f:1652-1655,1767-1770):
+    with one Op1 distribution block zeroed and one Op2 block scaled to 1e-16
+    (squared norm below 1e-28), the screened plan equals the oracle's
+    screened route 3 -- also where the skipped calls were the only
+    contributions, which stay exactly zero -- and the unscreened plan the
+    unscreened oracle."""
+    torch = _torch()
+    import gecco_hip
+    print(f'screening {name}', flush=True)
+    orbs = orbitals(4, 6, 2)
+    c, a, b, r = CASES[name](orbs)
+    x1, x2 = spin_symmetric(a, 51), spin_symmetric(b, 52)
+    blocks = c.blocks(False)
+    x1[blocks[0]['off1']:blocks[0]['off1'] + blocks[0]['len1']] = 0.0
+    x2[blocks[-1]['off2']:blocks[-1]['off2'] + blocks[-1]['len2']] *= 1e-16
+    from models import round_robin
+    o = gecco_hip.Orbs(round_robin(4, 2), round_robin(6, 2, 1), 2)
+    A, B, R = (gecco_hip.MEList(o, m.occ_c, m.occ_a) for m in (a, b, r))
+    dev = torch.device('cuda:0')
+    d1, d2 = torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev)
+    out = {}
+    for screen in (True, False):
+        ref = np.zeros(r.length)
+        c.run(x1, x2, ref, route=3, use_tr=False, screen=screen)
+        p = gecco_hip.Plan()
+        p.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, False)
+        p.set_screen(screen)
+        p.finalize()
+        dr = torch.zeros(r.length, dtype=torch.float64, device=dev)
+        p.execute([d1, d2, dr])
+        torch.cuda.synchronize()
+        got = dr.cpu().numpy()
+        assert _relerr(got, ref) <= TOL, screen
+        out[screen] = (got, ref)
+    (g1, r1), (g0, r0) = out[True], out[False]
+    only = (r1 == 0) & (r0 != 0)          # reached only by screened calls
+    assert only.any() or name == 'ring'
+    assert np.all(g1[only] == 0.0) and np.all(g0[only] != 0.0)
