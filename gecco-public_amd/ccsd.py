@@ -149,15 +149,20 @@ class CCSD:
     memory (formula.lifetimes / formula.pool_layout: command_new_intermediate
     and command_del_intermediate); the refined solve, whose linearised
     formula reads every intermediate's final value, then refuses to run.
+    spin_project=True projects the residual lists onto their singlet part
+    after their last write (gecco_hip_plan_add_spin_project: the reference's
+    spin_prj_list for a target with s2 = 0, contract/frm_sched2.f:146-150).
     """
 
-    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True, pool=False):
+    def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True, pool=False,
+                 spin_project=False):
         import torch
         import gecco_hip
         self.torch = torch
         self.ham = ham
         self.dev = torch.device(device)
         self.use_tr = use_tr
+        self.spin_project = spin_project
         self.formula = FORMULA if formula is None else formula
         lists = self.formula['lists']
         for n in ('T1', 'T2', 'R1', 'R2', 'E'):
@@ -310,6 +315,10 @@ class CCSD:
         if self.use_tr:
             for res in level['sym']:
                 p.add_sym_ab(self.L[res], base + self.slot[res])
+        if self.spin_project:
+            for res in level['sym']:
+                if res in self.rnames:
+                    p.add_spin_project(self.L[res], base + self.slot[res])
 
     def solve_mixed(self, tol=1e-8, switch=1e-6, maxit=100, ndiis=8, mode='graph', verbose=False,
                     stall=6, bulk=None):

@@ -1308,6 +1308,42 @@ __global__ __launch_bounds__(256) void norm_flags(const double *__restrict__ par
 }
 }  // namespace gecco
 
+namespace gecco {
+// Singlet spin projection: thread g replaces the elements of group g by
+// P_g times them (P_g row-major n x n in mats at gmat[g]; n <= 32).
+template <typename T>
+__global__ __launch_bounds__(256) void spin_project(T *__restrict__ x, const int64_t *__restrict__ gptr,
+                                                    const int64_t *__restrict__ elem,
+                                                    const int64_t *__restrict__ gmat,
+                                                    const double *__restrict__ mats, int64_t ngroups) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int64_t e0 = gptr[g];
+    const int n = (int)(gptr[g + 1] - e0);
+    const double *__restrict__ P = mats + gmat[g];
+    double v[32];
+    for (int i = 0; i < n && i < 32; ++i) v[i] = (double)x[elem[e0 + i]];
+    for (int i = 0; i < n && i < 32; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < n && j < 32; ++j) s += P[i * n + j] * v[j];
+        x[elem[e0 + i]] = (T)s;
+    }
+}
+}  // namespace gecco
+
+extern "C" int gecco_launch_spin_project(void *x, const int64_t *gptr, const int64_t *elem, const int64_t *gmat,
+                                         const double *mats, int64_t ngroups, int elem_size, hipStream_t stream) {
+    if (ngroups <= 0) return 0;
+    const unsigned grid = (unsigned)((ngroups + 255) / 256);
+    if (elem_size == 4)
+        hipLaunchKernelGGL(gecco::spin_project<float>, dim3(grid), dim3(256), 0, stream, (float *)x, gptr, elem,
+                           gmat, mats, ngroups);
+    else
+        hipLaunchKernelGGL(gecco::spin_project<double>, dim3(grid), dim3(256), 0, stream, (double *)x, gptr, elem,
+                           gmat, mats, ngroups);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
 extern "C" int gecco_launch_norms(const gecco::DevNormTask *tasks, int64_t ntasks, const int64_t *rows,
                                   int64_t njobs, void *const *bufs, double *partial, int32_t *zero, int elem,
                                   hipStream_t stream) {

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstring>
 #include <map>
@@ -41,6 +42,8 @@ int plan_add_block_internal(gecco_hip_plan *plan, const gecco_hip_block_args *ar
 int plan_add_sym_pairs(gecco_hip_plan *plan, int32_t buf, std::vector<int64_t> &&i1,
                        std::vector<int64_t> &&i2, std::vector<int8_t> &&f, double prefac);
 int set_error(int code, const char *msg);
+int plan_add_spin_groups(gecco_hip_plan *plan, int32_t buf, int64_t off, std::vector<int64_t> &&gptr,
+                         std::vector<int64_t> &&elem, std::vector<int64_t> &&gmat, std::vector<double> &&mats);
 int plan_add_trace_rows(gecco_hip_plan *plan, int32_t buf_src, int64_t off_src, int32_t buf_dst,
                         int64_t off_dst, double fact, std::vector<int64_t> &&dst,
                         std::vector<int64_t> &&src, std::vector<int8_t> &&sg);
@@ -1562,6 +1565,177 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
     if (nterms) *nterms = (int64_t)dst.size();
     return plan_add_trace_rows(plan, buf_ori, off_ori, buf_reo, off_reo, fact, std::move(dst),
                                std::move(src), std::move(sg));
+}
+
+// Singlet spin projection of a list (contract/spin_prj_list.f,
+// contract/spin_prj_blk.f with s2 = 0): the elements of a particle-conserving
+// one-vertex list fall into groups with the same spatial orbitals on each
+// side and different spins.  A group's element operators tau_e (a+ of the C
+// spin orbitals, a of the A ones in reverse order) are orthonormal, and the
+// singlet (rank-0 spin tensor) operators of the group are those X with
+// [S+, X] = 0 (S+ = sum_p a+_p,alpha a_p,beta; total Ms is zero already).
+// The commutator maps tau_e to operators with one beta creator turned alpha
+// (+1) or one alpha annihilator turned beta (-1), each brought to a sorted
+// order with its permutation sign; the projector onto the null space of that
+// map (eigenvectors of A^T A with eigenvalue 0, cyclic Jacobi) replaces the
+// reference's genealogical spin couplings -- it is the same orthogonal
+// projector onto the singlet subspace.  Distinct projectors are stored once.
+static int sort_parity(std::vector<int> &v) {
+    int sg = 1;
+    for (size_t i = 1; i < v.size(); ++i)
+        for (size_t j = i; j > 0 && v[j - 1] > v[j]; --j) {
+            std::swap(v[j - 1], v[j]);
+            sg = -sg;
+        }
+    return sg;
+}
+
+static void jacobi_eigen(std::vector<double> a, int n, std::vector<double> &vec, std::vector<double> &val) {
+    vec.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) vec[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+                if (i != j) off += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+            }
+        if (off <= 1e-30 * (tot + 1e-300)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[(size_t)p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+                const double th = 0.5 * (aqq - app) / apq;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {     // a <- J^T a J, columns then rows
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - sn * akq;
+                    a[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - sn * aqk;
+                    a[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = vec[(size_t)k * n + p], vkq = vec[(size_t)k * n + q];
+                    vec[(size_t)k * n + p] = c * vkp - sn * vkq;
+                    vec[(size_t)k * n + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    val.resize((size_t)n);
+    for (int i = 0; i < n; ++i) val[(size_t)i] = a[(size_t)i * n + i];
+}
+
+int gecco_hip_plan_add_spin_project(gecco_hip_plan *plan, gecco_hip_list *L, int32_t buf, int64_t off) {
+    if (!plan || !L) return set_error(GECCO_HIP_EINVAL, "add_spin_project: null argument");
+    if (L->tra) return set_error(GECCO_HIP_EINVAL, "add_spin_project: give the stored list, not its adjoint view");
+    if (L->njoined != 1) return set_error(GECCO_HIP_EINVAL, "add_spin_project: one-vertex lists only");
+    int ncel = 0, nael = 0;
+    for (int o : L->oc) ncel += o;
+    for (int o : L->oa) nael += o;
+    if (ncel != nael) return set_error(GECCO_HIP_EINVAL, "add_spin_project: only particle-conserving lists");
+    const int nel = ncel + nael;
+    if (nel > 6) return set_error(GECCO_HIP_EINVAL, "add_spin_project: at most three creators and annihilators");
+    std::vector<int> space;
+    for (size_t k = 0; k < L->oc.size(); ++k) space.insert(space.end(), (size_t)L->oc[k], L->hc[k]);
+    for (size_t k = 0; k < L->oa.size(); ++k) space.insert(space.end(), (size_t)L->oa[k], L->ha[k]);
+    for (int sp : space)
+        if (sp != 1 && sp != 2) return set_error(GECCO_HIP_EINVAL, "add_spin_project: hole and particle spaces only");
+    const int nh2 = 2 * (int)L->orbs->irr[1].size();
+    std::vector<int16_t> so((size_t)std::max<int64_t>(1, L->length * nel));
+    if (nel > 0 && gecco_hip_list_spinorbs(L, so.data(), (int64_t)so.size())) return 1;
+    auto mode = [&](int64_t e, int q) {
+        const int m = so[(size_t)e * nel + q];
+        return space[(size_t)q] == 2 ? nh2 + m : m;
+    };
+    // groups: same spatial orbitals (as multisets) on each side
+    std::map<std::vector<int>, std::vector<int64_t>> groups;
+    for (int64_t e = 0; e < L->length; ++e) {
+        std::vector<int> c, a;
+        for (int q = 0; q < ncel; ++q) c.push_back(mode(e, q) >> 1);
+        for (int q = ncel; q < nel; ++q) a.push_back(mode(e, q) >> 1);
+        std::sort(c.begin(), c.end());
+        std::sort(a.begin(), a.end());
+        c.push_back(-1);
+        c.insert(c.end(), a.begin(), a.end());
+        groups[c].push_back(e);
+    }
+    std::vector<int64_t> gptr(1, 0), elem, gmat;
+    std::vector<double> mats;
+    std::map<std::vector<int64_t>, int64_t> known;
+    for (auto &kv : groups) {
+        const std::vector<int64_t> &es = kv.second;
+        const int n = (int)es.size();
+        if (n > 32) return set_error(GECCO_HIP_EINVAL, "add_spin_project: group larger than 32 elements");
+        std::map<std::vector<int>, int> rows;
+        std::vector<std::vector<std::pair<int, double>>> col((size_t)n);
+        auto term = [&](int j, std::vector<int> c, std::vector<int> a, double coef) {
+            std::vector<int> sc = c, sa = a;
+            const int pc = sort_parity(sc), pa = sort_parity(sa);
+            for (size_t i = 1; i < sc.size(); ++i)
+                if (sc[i] == sc[i - 1]) return;            // a+ a+ of one spin orbital
+            for (size_t i = 1; i < sa.size(); ++i)
+                if (sa[i] == sa[i - 1]) return;
+            sc.push_back(-1);
+            sc.insert(sc.end(), sa.begin(), sa.end());
+            const int r = rows.emplace(sc, (int)rows.size()).first->second;
+            col[(size_t)j].push_back({r, coef * pc * pa});
+        };
+        for (int j = 0; j < n; ++j) {
+            std::vector<int> c, a;
+            for (int q = 0; q < ncel; ++q) c.push_back(mode(es[(size_t)j], q));
+            for (int q = ncel; q < nel; ++q) a.push_back(mode(es[(size_t)j], q));
+            for (int i = 0; i < ncel; ++i)
+                if (c[(size_t)i] & 1) {          // beta creator -> alpha
+                    std::vector<int> c2 = c;
+                    c2[(size_t)i] -= 1;
+                    term(j, c2, a, 1.0);
+                }
+            for (int i = 0; i < nael; ++i)
+                if (!(a[(size_t)i] & 1)) {       // alpha annihilator -> beta, sign -1
+                    std::vector<int> a2 = a;
+                    a2[(size_t)i] += 1;
+                    term(j, c, a2, -1.0);
+                }
+        }
+        // M = A^T A, null space projector
+        const int nr = (int)rows.size();
+        std::vector<double> A((size_t)nr * n, 0.0), M((size_t)n * n, 0.0), vec, val;
+        for (int j = 0; j < n; ++j)
+            for (auto &rc : col[(size_t)j]) A[(size_t)rc.first * n + j] += rc.second;
+        for (int r = 0; r < nr; ++r)
+            for (int j = 0; j < n; ++j)
+                for (int k = 0; k < n; ++k) M[(size_t)j * n + k] += A[(size_t)r * n + j] * A[(size_t)r * n + k];
+        jacobi_eigen(M, n, vec, val);
+        double vmax = 0.0;
+        for (double v : val) vmax = std::max(vmax, std::fabs(v));
+        std::vector<double> P((size_t)n * n, 0.0);
+        for (int t = 0; t < n; ++t) {
+            if (std::fabs(val[(size_t)t]) > 1e-9 * (1.0 + vmax)) continue;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k)
+                    P[(size_t)i * n + k] += vec[(size_t)i * n + t] * vec[(size_t)k * n + t];
+        }
+        std::vector<int64_t> key(1, n);
+        for (double x : P) key.push_back((int64_t)std::llround(x * 1e12));
+        auto it = known.find(key);
+        int64_t at;
+        if (it != known.end()) {
+            at = it->second;
+        } else {
+            at = (int64_t)mats.size();
+            mats.insert(mats.end(), P.begin(), P.end());
+            known[key] = at;
+        }
+        elem.insert(elem.end(), es.begin(), es.end());
+        gptr.push_back((int64_t)elem.size());
+        gmat.push_back(at);
+    }
+    return plan_add_spin_groups(plan, buf, off, std::move(gptr), std::move(elem), std::move(gmat), std::move(mats));
 }
 
 int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *L, int32_t buf, int64_t off,

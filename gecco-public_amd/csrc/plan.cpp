@@ -43,6 +43,8 @@ extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const int32_t *skip, const int32_t *zero, int elem,
                                            hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
+extern "C" int gecco_launch_spin_project(void *x, const int64_t *gptr, const int64_t *elem, const int64_t *gmat,
+                                         const double *mats, int64_t ngroups, int elem_size, hipStream_t stream);
 extern "C" int gecco_launch_norms(const gecco::DevNormTask *tasks, int64_t ntasks, const int64_t *rows,
                                   int64_t njobs, void *const *bufs, double *partial, int32_t *zero, int elem,
                                   hipStream_t stream);
@@ -457,9 +459,24 @@ struct TraceStep {
     int8_t *d_sgn = nullptr;
 };
 
+// Singlet spin projection of a list (contract/spin_prj_blk.f): the list's
+// elements fall into groups with the same spatial orbitals and different
+// spins; every group is replaced by P_g x_g, P_g the projector onto the
+// singlet operators of the group (one of a table of distinct projectors).
+struct SpinStep {
+    int32_t buf = 0, group = 0;
+    int64_t off = 0;
+    std::vector<int64_t> gptr, elem, gmat;   // CSR groups; offset of P_g in mats
+    std::vector<double> mats;                // n x n row-major projectors
+    int64_t ngroups = 0;
+    int64_t *d_gptr = nullptr, *d_elem = nullptr, *d_gmat = nullptr;
+    double *d_mats = nullptr;
+};
+
 struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
     std::vector<TraceStep> traces;                // after the contractions
+    std::vector<SpinStep> spins;                  // after the spin-flip projections
     std::vector<gecco::DevInit> inits;            // list initialisation, first launch
     std::vector<int32_t> init_group;              // instance group of each init
     gecco::DevInit *d_inits = nullptr;
@@ -682,6 +699,14 @@ static void plan_free_device(gecco_hip_plan *p) {
     p->nnorm_tasks = p->nnorm_jobs = 0;
     if (p->d_index) dfree(p, p->d_index);
     if (p->d_bufs) dfree(p, p->d_bufs);
+    for (auto &sp : p->spins) {
+        if (sp.d_gptr) dfree(p, sp.d_gptr);
+        if (sp.d_elem) dfree(p, sp.d_elem);
+        if (sp.d_gmat) dfree(p, sp.d_gmat);
+        if (sp.d_mats) dfree(p, sp.d_mats);
+        sp.d_gptr = sp.d_elem = sp.d_gmat = nullptr;
+        sp.d_mats = nullptr;
+    }
     for (auto &tr : p->traces) {
         if (tr.d_rowptr) dfree(p, tr.d_rowptr);
         if (tr.d_rowdst) dfree(p, tr.d_rowdst);
@@ -872,6 +897,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     for (auto &y : p->syms) nb_max = std::max<int64_t>(nb_max, y.buf + 1);
     for (auto &j : p->inits) nb_max = std::max<int64_t>(nb_max, std::max(j.dst, j.src) + 1);
     for (auto &tr : p->traces) nb_max = std::max<int64_t>(nb_max, std::max(tr.buf_src, tr.buf_dst) + 1);
+    for (auto &sp : p->spins) nb_max = std::max<int64_t>(nb_max, sp.buf + 1);
     // Blocks whose operands A and B are strided views (affine index maps)
     // run on the strided-GEMM kernel with the tile variant p->affine_variant;
     // its epilogue scatters through any result map.  A block with gathered
@@ -1343,6 +1369,24 @@ static int plan_finalize(gecco_hip_plan *p) {
     p->nbufs_max = nb_max;
     if (dmalloc(p, (void **)&p->d_bufs, (nb_max + 1) * sizeof(void *), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
+    for (auto &sp : p->spins) {
+        if (!sp.ngroups) continue;
+        if (dmalloc(p, (void **)&sp.d_gptr, sp.gptr.size() * 8, "hipMalloc(spin)") ||
+            dmalloc(p, (void **)&sp.d_elem, std::max<size_t>(1, sp.elem.size()) * 8, "hipMalloc(spin)") ||
+            dmalloc(p, (void **)&sp.d_gmat, sp.gmat.size() * 8, "hipMalloc(spin)") ||
+            dmalloc(p, (void **)&sp.d_mats, std::max<size_t>(1, sp.mats.size()) * 8, "hipMalloc(spin)") ||
+            check(hipMemcpy(sp.d_gptr, sp.gptr.data(), sp.gptr.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(spin)") ||
+            check(hipMemcpy(sp.d_elem, sp.elem.data(), sp.elem.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(spin)") ||
+            check(hipMemcpy(sp.d_gmat, sp.gmat.data(), sp.gmat.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(spin)") ||
+            check(hipMemcpy(sp.d_mats, sp.mats.data(), sp.mats.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(spin)"))
+            return g_err;
+        if (!p->keep_host) {
+            std::vector<int64_t>().swap(sp.gptr);
+            std::vector<int64_t>().swap(sp.elem);
+            std::vector<int64_t>().swap(sp.gmat);
+            std::vector<double>().swap(sp.mats);
+        }
+    }
     for (auto &tr : p->traces) {
         if (!tr.nrows) continue;
         if (dmalloc(p, (void **)&tr.d_rowptr, (tr.nrows + 1) * 8, "hipMalloc(trace)") ||
@@ -1450,9 +1494,9 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (nbufs < p->nbufs_max) return fail(GECCO_HIP_EINVAL, "plan uses %lld buffer slots, %lld given", (long long)p->nbufs_max, (long long)nbufs);
     for (int64_t i = 0; i < p->nbufs_max; ++i)
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
-    if (p->d_skip && !p->traces.empty())
-        return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace steps");
-    if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty()) return ok();
+    if (p->d_skip && (!p->traces.empty() || !p->spins.empty()))
+        return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace or spin-projection steps");
+    if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty() && p->spins.empty()) return ok();
     std::vector<void *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
     if (want != p->last_bufs) {
@@ -1519,6 +1563,10 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
         if (mark(2, false)) return g_err;
     }
+    for (auto &sp : p->spins)
+        if (sp.ngroups && gecco_launch_spin_project((char *)bufs[sp.buf] + sp.off * p->elem, sp.d_gptr, sp.d_elem,
+                                                    sp.d_gmat, sp.d_mats, sp.ngroups, p->elem, s))
+            return fail(GECCO_HIP_EDEVICE, "spin projection launch failed");
     return ok();
 }
 
@@ -1768,6 +1816,23 @@ int plan_add_trace_rows(gecco_hip_plan *plan, int32_t buf_src, int64_t off_src, 
     tr.nrows = (int64_t)tr.rowdst.size();
     tr.nnz = (int64_t)tr.col.size();
     plan->traces.push_back(std::move(tr));
+    return ok();
+}
+
+int plan_add_spin_groups(gecco_hip_plan *plan, int32_t buf, int64_t off, std::vector<int64_t> &&gptr,
+                         std::vector<int64_t> &&elem, std::vector<int64_t> &&gmat, std::vector<double> &&mats) {
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    if (buf < 0 || off < 0) return fail(GECCO_HIP_EINVAL, "negative buffer slot or offset");
+    SpinStep sp;
+    sp.buf = buf;
+    sp.off = off;
+    sp.group = plan->cur_group;
+    sp.ngroups = gptr.empty() ? 0 : (int64_t)gptr.size() - 1;
+    sp.gptr = std::move(gptr);
+    sp.elem = std::move(elem);
+    sp.gmat = std::move(gmat);
+    sp.mats = std::move(mats);
+    plan->spins.push_back(std::move(sp));
     return ok();
 }
 

@@ -490,6 +490,8 @@ _lib.gecco_hip_plan_add_init.argtypes = [VP, ctypes.c_int32, I64, I64, ctypes.c_
 _lib.gecco_hip_plan_add_init.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_sum.argtypes = _lib.gecco_hip_plan_add_init.argtypes
 _lib.gecco_hip_plan_add_sum.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_spin_project.argtypes = [VP, VP, ctypes.c_int32, I64]
+_lib.gecco_hip_plan_add_spin_project.restype = ctypes.c_int
 _lib.gecco_hip_diis_create.argtypes = [I64, I64, ctypes.POINTER(VP)]
 _lib.gecco_hip_diis_destroy.argtypes = [VP]
 _lib.gecco_hip_diis_reset.argtypes = [VP]
@@ -726,6 +728,15 @@ def _plan_add_sum(self, buf_dst, length, buf_src, factor=1.0, off_dst=0, off_src
 
 
 Plan.add_sum = _plan_add_sum
+
+
+def _plan_add_spin_project(self, lst, buf, off=0):
+    """Singlet spin projection of list `lst` in slot `buf` after the plan's
+    contractions (gecco_hip_plan_add_spin_project)."""
+    _check(_lib.gecco_hip_plan_add_spin_project(self._h, lst._h, int(buf), int(off)))
+
+
+Plan.add_spin_project = _plan_add_spin_project
 
 
 class DIIS:

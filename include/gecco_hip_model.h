@@ -138,6 +138,15 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
 int gecco_hip_plan_add_sym_ab(gecco_hip_plan *plan, gecco_hip_list *list, int32_t buf,
                               int64_t off, double prefac, double relfac);
 
+/* Singlet spin projection of a particle-conserving one-vertex list after the
+ * plan's contractions and spin-flip projections (contract/spin_prj_list.f,
+ * contract/spin_prj_blk.f with s2 = 0, as contract/frm_sched2.f:146-150
+ * applies it to a result with me_res%s2 >= 0): every group of elements with
+ * the same spatial orbitals and different spins is replaced by its
+ * projection onto the singlet operators (the null space of [S+, .]), at most
+ * three creators and three annihilators, hole and particle spaces. */
+int gecco_hip_plan_add_spin_project(gecco_hip_plan *plan, gecco_hip_list *list, int32_t buf, int64_t off);
+
 /* One binary contraction  Res (+)= xfac * Op1 . Op2  with the semantics of
  * contr_op1op2 (contract/contr_op1op2.f:2-17, called from
  * contract/fs_contr_drv.f:273-288), its arguments flattened from GeCCo's

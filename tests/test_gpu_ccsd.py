@@ -316,3 +316,17 @@ def test_ccsdt_pooled_intermediates():
     assert abs(ham.e_ref + res['energy'] - ref) < 1e-10
     with pytest.raises(ValueError):
         c.build_linear()
+
+
+def test_spin_projected_residuals_give_the_same_energy():
+    """CCSD with the residual lists projected onto their singlet part
+    (spin_project=True, the reference's spin_prj_list) converges to the
+    energy of the spin-flip-symmetrised solve: closed-shell residuals are
+    singlets."""
+    _torch()
+    import ccsd
+    ham = synthetic(4, 10, 30, nsym=2, seed=21, sigma=0.1)
+    print('spin-projected residuals 4/10/2', flush=True)
+    e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
+    r = ccsd.CCSD(ham, spin_project=True).solve(tol=1e-10, maxit=80)
+    assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11

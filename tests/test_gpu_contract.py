@@ -765,3 +765,38 @@ def test_norm_screening_matches_oracle(name):
     only = (r1 == 0) & (r0 != 0)          # reached only by screened calls
     assert only.any() or name == 'ring'
     assert np.all(g1[only] == 0.0) and np.all(g0[only] != 0.0)
+
+
+@pytest.mark.parametrize('name', ['t1', 't2', 'fae', 'wmbej', 't3'])
+@pytest.mark.parametrize('nsym', [1, 2])
+def test_spin_projection_matches_oracle(name, nsym):
+    """gecco_hip_plan_add_spin_project (the singlet projector built from the
+    combinatorics of [S+, tau_e] and a Jacobi null space) against the
+    oracle's Fock-space projector (dense operators, SVD) on random lists;
+    projecting twice changes nothing."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    from oracle.spin_project import SpinSpace
+    from test_oracle_spin_project import LISTS
+    print(f'spin projection {name} nsym={nsym}', flush=True)
+    c, a = LISTS[name]
+    orbs = orbitals(2, 2, nsym)
+    from oracle.melist import MEList as OList
+    Lo = OList(orbs, c, a)
+    Pm = SpinSpace(2, 2).projector(Lo)
+    o = gecco_hip.Orbs(round_robin(2, nsym), round_robin(2, nsym, 1), nsym)
+    L = gecco_hip.MEList(o, c, a)
+    assert L.length == Lo.length
+    x = np.random.default_rng(7).standard_normal(L.length)
+    p = gecco_hip.Plan()
+    p.add_spin_project(L, 0)
+    p.finalize()
+    d = torch.from_numpy(x.copy()).to('cuda:0')
+    p.execute([d])
+    torch.cuda.synchronize()
+    once = d.cpu().numpy()
+    assert np.abs(once - Pm @ x).max() <= 1e-12 * max(1.0, np.abs(x).max())
+    p.execute([d])
+    torch.cuda.synchronize()
+    assert np.abs(d.cpu().numpy() - once).max() <= 1e-12
