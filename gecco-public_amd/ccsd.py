@@ -113,6 +113,28 @@ _TERM_LEVELS = [
 TERMS = [t for level in _TERM_LEVELS for t in level]
 
 
+def record_contraction(p, c, lists, slot, use_tr):
+    """Record a CONTR item into plan p: lists maps names to gecco_hip.MEList,
+    slot names to buffer slots.  An item with vertex data (lists of several
+    vertices) goes through gecco_hip_plan_add_contraction_n."""
+    a, b, t = c['op1'], c['op2'], c['target']
+    buf = (slot(a), slot(b), slot(t))
+    v = c.get('vertices')
+    if v is None:
+        return p.add_contraction(lists[a], lists[b], lists[t], c['cnt_c'], c['cnt_a'], c['factor'],
+                                 use_tr, buf=buf)
+    return p.add_contraction_n(lists[a], lists[b], lists[t], v['ex1'], v['ex2'], v['cnt'], *v['merge'],
+                               fact=c['factor'], use_tr=use_tr, buf=buf)
+
+
+def record_reorder(p, r, lists, slot):
+    """Record an ADD_REO item (REORDER items expand into INIT + ADD_REO)
+    into plan p: it runs after the plan's contractions."""
+    s, t = r['source'], r['target']
+    p.add_reorder(lists[s], lists[t], r['from'], r['to'], r['shift_c'], r['shift_a'], r['factor'],
+                  r['sign'], buf=(slot(s), slot(t)))
+
+
 def _formula_items():
     items = []
     for n in LISTS:
@@ -306,12 +328,11 @@ class CCSD:
                 raise ValueError(f'ADD {src} into {n}: lists of different layout')
             p.add_sum(base + self.slot[n], self.L[n].length, base + self.slot[src], factor=it['factor'])
         for c in level['contr']:
-            res, a, b = c['target'], c['op1'], c['op2']
-            nb, fl = p.add_contraction(self.L[a], self.L[b], self.L[res], c['cnt_c'], c['cnt_a'],
-                                       c['factor'], self.use_tr,
-                                       buf=(base + self.slot[a], base + self.slot[b], base + self.slot[res]))
+            nb, fl = record_contraction(p, c, self.L, lambda n: base + self.slot[n], self.use_tr)
             self.flops += fl
             self.nblocks += nb
+        for r in level.get('reo', []):
+            record_reorder(p, r, self.L, lambda n: base + self.slot[n])
         if self.use_tr:
             for res in level['sym']:
                 p.add_sym_ab(self.L[res], base + self.slot[res])
@@ -466,11 +487,12 @@ class CCSD:
             for it in lv['add']:
                 p.add_sum(slot[it['target']], lst(it['target']).length, slot[it['source']],
                           factor=it['factor'])
+            lists = {n: lst(n) for n in slot}
             for c in lv['contr']:
-                a, b, t = c['op1'], c['op2'], c['target']
-                nb, fl = p.add_contraction(lst(a), lst(b), lst(t), c['cnt_c'], c['cnt_a'], c['factor'],
-                                           self.use_tr, buf=(slot[a], slot[b], slot[t]))
+                nb, fl = record_contraction(p, c, lists, slot.__getitem__, self.use_tr)
                 flops += fl
+            for r in lv.get('reo', []):
+                record_reorder(p, r, lists, slot.__getitem__)
             if self.use_tr:
                 for x in lv['sym']:
                     p.add_sym_ab(lst(x), slot[x])

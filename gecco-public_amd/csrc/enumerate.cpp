@@ -970,22 +970,37 @@ static int reorder_triples(const gecco_hip_list *Lo, const gecco_hip_list *Lr, i
     }
     chk = i0;
     if ((int)chk.size() < Lr->njoined) chk.resize(Lr->njoined, std::array<int, 10>{});
+    if (to >= (int)chk.size()) return set_error(GECCO_HIP_EINVAL, "add_reorder: to-vertex out of range");
     for (int h = 1; h <= 4; ++h) { chk[to][h] += kc5[h]; chk[to][5 + h] += ka5[h]; }
-    if ((int)chk.size() != Lr->njoined) return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex counts differ");
-    for (int v = 0; v < Lr->njoined; ++v)
-        for (int x = 0; x < 10; ++x)
-            if (chk[v][x] != Lr->vocc[v][x])
-                return set_error(GECCO_HIP_EINVAL, "add_reorder: REO is not I0 with K on the to-vertex");
+    // REO's vertices: those of I0 with K on vertex `to` (numbered as ORI's);
+    // when REO has fewer, the vertices left empty are dropped (an empty
+    // vertex holds no component, so the layout is unchanged)
+    std::vector<int> vmap(chk.size(), -1);
+    int nkeep = 0;
+    for (size_t v = 0; v < chk.size(); ++v) {
+        bool empty = true;
+        for (int x : chk[v]) empty = empty && x == 0;
+        if ((int)chk.size() == Lr->njoined || !empty) vmap[v] = nkeep++;
+    }
+    if (nkeep != Lr->njoined) return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex counts differ");
+    for (size_t v = 0; v < chk.size(); ++v)
+        if (vmap[v] >= 0)
+            for (int x = 0; x < 10; ++x)
+                if (chk[v][x] != Lr->vocc[vmap[v]][x])
+                    return set_error(GECCO_HIP_EINVAL, "add_reorder: REO is not I0 with K on the to-vertex");
     std::vector<int> i0c_o, i0c_h, i0c_v, i0a_o, i0a_h, i0a_v;
     condense_n(i0, i0c_o, i0c_h, i0c_v, i0a_o, i0a_h, i0a_v);
+    std::vector<int> i0c_vr(i0c_v), i0a_vr(i0a_v);   // I0's vertices in REO's numbering
+    for (int &v : i0c_vr) v = vmap[v];
+    for (int &v : i0a_vr) v = vmap[v];
     Side kc = side_of(kc5), ka = side_of(ka5);
     Side i0c, i0a;
     i0c.occ = i0c_o; i0c.hpvx = i0c_h;
     i0a.occ = i0a_o; i0a.hpvx = i0a_h;
     const auto mi_oc = reo_map_info(Lo->hc, Lo->vc, i0c_h, i0c_v, kc.hpvx, frm, false);
     const auto mi_oa = reo_map_info(Lo->ha, Lo->va, i0a_h, i0a_v, ka.hpvx, frm, true);
-    const auto mi_rc = reo_map_info(Lr->hc, Lr->vc, i0c_h, i0c_v, kc.hpvx, to, false);
-    const auto mi_ra = reo_map_info(Lr->ha, Lr->va, i0a_h, i0a_v, ka.hpvx, to, true);
+    const auto mi_rc = reo_map_info(Lr->hc, Lr->vc, i0c_h, i0c_vr, kc.hpvx, vmap[to], false);
+    const auto mi_ra = reo_map_info(Lr->ha, Lr->va, i0a_h, i0a_vr, ka.hpvx, vmap[to], true);
     const auto so_c = map_sources(mi_oc, Lo->hc.size()), so_a = map_sources(mi_oa, Lo->ha.size());
     const auto sr_c = map_sources(mi_rc, Lr->hc.size()), sr_a = map_sources(mi_ra, Lr->ha.size());
     int kc_max = 0, ka_max = 0, i0c_max = 0, i0a_max = 0;
@@ -1517,6 +1532,179 @@ int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *o
     return rc;
 }
 
+namespace gecco {
+
+typedef std::vector<std::array<int, 10>> VOcc;   // per vertex: C[h] at h, A[h] at 5 + h
+typedef std::vector<std::pair<std::vector<int>, std::vector<int>>> Merge;
+
+static VOcc vocc_of(const int64_t *iocc, int64_t nj) {
+    VOcc v((size_t)nj, std::array<int, 10>{});
+    for (int64_t j = 0; j < nj; ++j)
+        for (int h = 1; h <= 4; ++h) {
+            v[j][h] = (int)iocc[8 * j + h - 1];
+            v[j][5 + h] = (int)iocc[8 * j + 4 + h - 1];
+        }
+    return v;
+}
+
+// contract/condense_merge_map.f short form -> per target vertex (source-1
+// vertices, source-2 vertices), 1-based; false when malformed
+static bool merge_of(const int64_t *mm, size_t ntgt, int n1, int n2, Merge &out) {
+    out.clear();
+    size_t k = 0;
+    for (size_t t = 0; t < ntgt; ++t) {
+        std::pair<std::vector<int>, std::vector<int>> p;
+        for (int q = 0; q < 2; ++q) {
+            const int64_t n = mm[k++];
+            if (n < 0 || n > 16) return false;
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t v = mm[k++];
+                if (v < 1 || v > (q ? n2 : n1)) return false;
+                (q ? p.second : p.first).push_back((int)v);
+            }
+        }
+        out.push_back(std::move(p));
+    }
+    return true;
+}
+
+// contract/set_idx_seq.f (sep_ca) and contract/set_mapping_info.f:84-170
+// for one side (0: C, 1: A) of a target operator whose vertices take the
+// source vertices of merge; dag swaps a source's C and A sides (CNT as seen
+// from OP2)
+static std::vector<int64_t> mapping_info(const VOcc &tgt, const Merge &merge, const VOcc &s1,
+                                         bool dag1, const VOcc &s2, bool dag2, int side) {
+    const int c1 = dag1 ? 1 - side : side, c2 = dag2 ? 1 - side : side;
+    auto seq = [](const VOcc &v, int s) {
+        std::map<std::pair<int, int>, int64_t> q;
+        int64_t k = 0;
+        for (int h : HPVXBLKSEQ)
+            for (size_t j = 0; j < v.size(); ++j)
+                if (v[j][5 * s + h] > 0) q[{h, (int)j + 1}] = ++k;
+        return q;
+    };
+    const auto q1 = seq(s1, c1), q2 = seq(s2, c2);
+    std::vector<int64_t> mi;
+    for (int h : HPVXBLKSEQ)
+        for (size_t t = 0; t < tgt.size(); ++t) {
+            if (tgt[t][5 * side + h] == 0) continue;
+            std::vector<int64_t> i1, i2;
+            for (int v : merge[t].first)
+                if (s1[v - 1][5 * c1 + h] > 0) i1.push_back(q1.at({h, v}));
+            for (int v : merge[t].second)
+                if (s2[v - 1][5 * c2 + h] > 0) i2.push_back(q2.at({h, v}));
+            mi.push_back((int64_t)i1.size());
+            mi.insert(mi.end(), i1.begin(), i1.end());
+            mi.push_back((int64_t)i2.size());
+            mi.insert(mi.end(), i2.begin(), i2.end());
+        }
+    return mi;
+}
+
+static void sides_of(const VOcc &v, Side &c, Side &a) {
+    std::vector<int> oc, hc, vc, oa, ha, va;
+    condense_n(v, oc, hc, vc, oa, ha, va);
+    c.occ = oc; c.hpvx = hc;
+    a.occ = oa; a.hpvx = ha;
+}
+
+// every target component takes at most one component of each source list
+static bool single_sources(const std::vector<int64_t> &mi, size_t ntgt) {
+    size_t k = 0;
+    for (size_t t = 0; t < ntgt; ++t)
+        for (int q = 0; q < 2; ++q) {
+            const int64_t n = mi[k];
+            if (n > 1) return false;
+            k += 1 + (size_t)n;
+        }
+    return true;
+}
+
+}  // namespace gecco
+
+int gecco_hip_plan_add_contraction_n(gecco_hip_plan *plan, const gecco_hip_list *op1,
+                                     const gecco_hip_list *op2, const gecco_hip_list *res,
+                                     int64_t njoined_cnt, const int64_t *iocc_ex1,
+                                     const int64_t *iocc_ex2, const int64_t *iocc_cnt,
+                                     const int64_t *merge_op1, const int64_t *merge_op2,
+                                     const int64_t *merge_op1op2, const int64_t *merge_op2op1,
+                                     double fact, int64_t use_tr, int32_t buf1, int64_t off1,
+                                     int32_t buf2, int64_t off2, int32_t buf12, int64_t off12,
+                                     int64_t *nblocks, double *flops) {
+    if (!plan || !op1 || !op2 || !res || !iocc_ex1 || !iocc_ex2 || (njoined_cnt > 0 && !iocc_cnt) ||
+        !merge_op1 || !merge_op2 || !merge_op1op2 || !merge_op2op1)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction_n: null argument");
+    if (op1->orbs != op2->orbs || op1->orbs != res->orbs)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction_n: lists of different orbital sets");
+    if (op1->tra || op2->tra || res->tra)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction_n: stored lists only, no adjoint views");
+    if (njoined_cnt < 0 || njoined_cnt > 16)
+        return set_error(GECCO_HIP_EINVAL, "add_contraction_n: bad number of contraction vertices");
+    const VOcc ex1 = vocc_of(iocc_ex1, op1->njoined), ex2 = vocc_of(iocc_ex2, op2->njoined);
+    const VOcc cnt = njoined_cnt ? vocc_of(iocc_cnt, njoined_cnt) : VOcc{};
+    for (const VOcc *v : {&ex1, &ex2, &cnt})
+        for (const auto &a : *v)
+            for (int x : a)
+                if (x < 0) return set_error(GECCO_HIP_EINVAL, "add_contraction_n: negative occupation");
+    Merge m1, m2, m12, m21;
+    const int nc = (int)njoined_cnt, n1 = op1->njoined, n2 = op2->njoined;
+    if (!merge_of(merge_op1, (size_t)n1, nc, n1, m1) || !merge_of(merge_op2, (size_t)n2, nc, n2, m2) ||
+        !merge_of(merge_op1op2, (size_t)res->njoined, n1, n2, m12) ||
+        !merge_of(merge_op2op1, (size_t)res->njoined, n2, n1, m21))
+        return set_error(GECCO_HIP_EINVAL, "add_contraction_n: malformed merge map");
+    // every operand vertex is the sum of its merged parts (OP2 sees CNT
+    // daggered; the result's creators follow merge_op1op2, its annihilators
+    // merge_op2op1)
+    auto sums_to = [](const std::array<int, 10> &tgt, const std::pair<std::vector<int>, std::vector<int>> &p,
+                      const VOcc &s1, bool dag1, const VOcc &s2, int side) {
+        for (int h = 1; h <= 4; ++h) {
+            int n = 0;
+            for (int v : p.first) n += s1[v - 1][5 * (dag1 ? 1 - side : side) + h];
+            for (int v : p.second) n += s2[v - 1][5 * side + h];
+            if (n != tgt[5 * side + h]) return false;
+        }
+        return true;
+    };
+    for (int v = 0; v < n1; ++v)
+        for (int s = 0; s < 2; ++s)
+            if (!sums_to(op1->vocc[v], m1[v], cnt, false, ex1, s))
+                return set_error(GECCO_HIP_EINVAL, "add_contraction_n: op1 vertex != CNT + EX1");
+    for (int v = 0; v < n2; ++v)
+        for (int s = 0; s < 2; ++s)
+            if (!sums_to(op2->vocc[v], m2[v], cnt, true, ex2, s))
+                return set_error(GECCO_HIP_EINVAL, "add_contraction_n: op2 vertex != CNT^+ + EX2");
+    for (int v = 0; v < res->njoined; ++v)
+        if (!sums_to(res->vocc[v], m12[v], ex1, false, ex2, 0) || !sums_to(res->vocc[v], m21[v], ex2, false, ex1, 1))
+            return set_error(GECCO_HIP_EINVAL, "add_contraction_n: result vertex != EX1 + EX2");
+    ContrCtx c;
+    c.plan = plan;
+    c.o = op1->orbs;
+    c.L1 = op1; c.L2 = op2; c.L12 = res;
+    sides_of(ex1, c.ex1c, c.ex1a);
+    sides_of(ex2, c.ex2c, c.ex2a);
+    sides_of(cnt, c.cntc, c.cnta);
+    c.mi12c = mapping_info(res->vocc, m12, ex1, false, ex2, false, 0);
+    c.mi12a = mapping_info(res->vocc, m21, ex2, false, ex1, false, 1);
+    c.mi1c = mapping_info(op1->vocc, m1, cnt, false, ex1, false, 0);
+    c.mi1a = mapping_info(op1->vocc, m1, cnt, false, ex1, false, 1);
+    c.mi2c = mapping_info(op2->vocc, m2, cnt, true, ex2, false, 0);
+    c.mi2a = mapping_info(op2->vocc, m2, cnt, true, ex2, false, 1);
+    // route 3 carries one string per target component and source list
+    // (contract/collect_block_impr.f with ireo from contract/set_strmapdim_c2.f)
+    if (!single_sources(c.mi12c, res->hc.size()) || !single_sources(c.mi12a, res->ha.size()) ||
+        !single_sources(c.mi1c, op1->hc.size()) || !single_sources(c.mi1a, op1->ha.size()) ||
+        !single_sources(c.mi2c, op2->hc.size()) || !single_sources(c.mi2a, op2->ha.size()))
+        return set_error(GECCO_HIP_EINVAL,
+                         "add_contraction_n: multi map needed (two components of one source merge into one component)");
+    c.fact = fact;
+    c.buf1 = buf1; c.buf2 = buf2; c.buf12 = buf12;
+    c.off1 = off1; c.off2 = off2; c.off12 = off12;
+    int rc = enumerate(c, use_tr != 0);
+    if (nblocks) *nblocks = c.nblocks;
+    if (flops) *flops = c.flops;
+    return rc;
+}
+
 int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
                              const gecco_hip_list *trop, const int64_t *cnt, double fact,
                              int32_t buf_op, int64_t off_op, int32_t buf_trop, int64_t off_trop,
@@ -1549,7 +1737,8 @@ int gecco_hip_plan_add_reorder(gecco_hip_plan *plan, const gecco_hip_list *ori,
         return set_error(GECCO_HIP_EINVAL, "add_reorder: null argument");
     if (ori->orbs != reo->orbs) return set_error(GECCO_HIP_EINVAL, "add_reorder: lists of different orbital sets");
     if (ori->tra || reo->tra) return set_error(GECCO_HIP_EINVAL, "add_reorder: stored lists only, no adjoint views");
-    if (from_vtx < 0 || from_vtx >= ori->njoined || to_vtx < 0 || to_vtx >= reo->njoined)
+    if (from_vtx < 0 || from_vtx >= ori->njoined || to_vtx < 0 ||
+        to_vtx >= std::max(ori->njoined, reo->njoined))
         return set_error(GECCO_HIP_EINVAL, "add_reorder: vertex out of range");
     if (sign_reo != 1 && sign_reo != -1) return set_error(GECCO_HIP_EINVAL, "add_reorder: sign must be +1 or -1");
     int kc[5] = {0}, ka[5] = {0};

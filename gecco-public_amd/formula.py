@@ -25,6 +25,14 @@ commands:
                                                 (S with L's occupation)
   {"cmd": "BC", ... as CONTR}                   command_bc (10): L = f A . B,
                                                 i.e. INIT L = 0 then the CONTR
+  {"cmd": "REORDER", "target": L, "source": S,  command_reorder (6),
+   "from": v, "to": w, "shift_c": {...},        contract/fs_reo_drv.f: L = f *
+   "shift_a": {...}, "factor": f, "sign": s}    s * (S with the lines shift_c /
+                                                shift_a moved from its vertex v
+                                                to vertex w of L; vertices
+                                                0-based; reo_op_wmaps_c)
+  {"cmd": "ADD_REO", ... as REORDER}            command_add_reo (13): L += the
+                                                same
   {"cmd": "NEW", "target": L}                   command_new_intermediate (2) and
   {"cmd": "DELETE", "target": L}                command_del_intermediate (3)
                                                 (contract/fs_newintm_drv.f,
@@ -34,7 +42,13 @@ commands:
 
 Occupations are dicts over the spaces {"H": n, "P": n}.  A formula file is
 JSON: {"lists": {name: {"C": occ, "A": occ}}, "constant": [names],
-"items": [...]}.  Every list is declared with the formula; NEW and DELETE
+"items": [...]}.  A list of several vertices (an intermediate with joined
+vertices) gives "C" and "A" as lists of occupations, one per vertex; a
+CONTR or BC item between such lists carries "vertices": {"ex1": [[C, A],
+...], "ex2": [...], "cnt": [...], "merge": [merge_op1, merge_op2,
+merge_op1op2, merge_op2op1]} -- contr_op1op2's iocc_ex1, iocc_ex2, iocc_cnt
+and short-form merge maps (contract/contr_op1op2.f:2-17,
+contract/get_bc_info3.f:271-330).  Every list is declared with the formula; NEW and DELETE
 (or, without them, the items themselves) bound the levels in which an
 intermediate holds data (`lifetimes`), so that a driver may let intermediates
 whose lifetimes do not meet share device memory (`pool_layout`).
@@ -61,17 +75,45 @@ _NAME = {H: 'H', P: 'P'}
 
 
 def _occ_in(o):
+    if isinstance(o, list):
+        return [_occ_in(x) for x in o]
     return {_SPACE[k]: int(v) for k, v in o.items() if int(v)}
 
 
 def _occ_out(o):
+    if isinstance(o, list):
+        return [_occ_out(x) for x in o]
     return {_NAME[k]: int(v) for k, v in o.items() if int(v)}
+
+
+def _vertices_io(v, conv):
+    return {'ex1': [[conv(c), conv(a)] for c, a in v['ex1']],
+            'ex2': [[conv(c), conv(a)] for c, a in v['ex2']],
+            'cnt': [[conv(c), conv(a)] for c, a in v['cnt']],
+            'merge': [[int(x) for x in m] for m in v['merge']]}
 
 
 def contr(target, op1, op2, cnt_c, cnt_a, factor):
     """A CONTR item (occupations keyed by the engine's space codes)."""
     return {'cmd': 'CONTR', 'target': target, 'op1': op1, 'op2': op2,
             'cnt_c': dict(cnt_c), 'cnt_a': dict(cnt_a), 'factor': float(factor)}
+
+
+def contr_n(target, op1, op2, ex1, ex2, cnt, merge, factor):
+    """A CONTR item between lists of several vertices: ex1 / ex2 / cnt are
+    per-vertex (occ_c, occ_a) of EX1, EX2 and the contraction lines, merge
+    the four short-form merge maps (gecco_hip_plan_add_contraction_n)."""
+    it = contr(target, op1, op2, {}, {}, factor)
+    it['vertices'] = _vertices_io({'ex1': ex1, 'ex2': ex2, 'cnt': cnt, 'merge': merge}, dict)
+    return it
+
+
+def reorder(target, source, frm, to, shift_c, shift_a, factor=1.0, sign=1, add=False):
+    """A REORDER item (target = factor * sign * the reordered source; with
+    add=True an ADD_REO item, target += ...)."""
+    return {'cmd': 'ADD_REO' if add else 'REORDER', 'target': target, 'source': source,
+            'from': int(frm), 'to': int(to), 'shift_c': dict(shift_c), 'shift_a': dict(shift_a),
+            'factor': float(factor), 'sign': int(sign)}
 
 
 def init(target, source=None, factor=1.0):
@@ -99,16 +141,20 @@ def delete(target):
     return {'cmd': 'DELETE', 'target': target}
 
 
-_COMMANDS = ('INIT', 'CONTR', 'SYMMETRISE', 'ADD', 'BC', 'NEW', 'DELETE')
+_COMMANDS = ('INIT', 'CONTR', 'SYMMETRISE', 'ADD', 'BC', 'NEW', 'DELETE', 'REORDER', 'ADD_REO')
 
 
 def expand(items):
-    """The items with every BC written as INIT (zero) then CONTR."""
+    """The items with every BC written as INIT (zero) then CONTR, and every
+    REORDER as INIT (zero) then ADD_REO."""
     out = []
     for it in items:
         if it['cmd'] == 'BC':
             out.append(init(it['target']))
             out.append(dict(it, cmd='CONTR'))
+        elif it['cmd'] == 'REORDER':
+            out.append(init(it['target']))
+            out.append(dict(it, cmd='ADD_REO'))
         else:
             out.append(it)
     return out
@@ -123,6 +169,11 @@ def dump(formula, path):
         if it['cmd'] in ('CONTR', 'BC'):
             it['cnt_c'] = _occ_out(it['cnt_c'])
             it['cnt_a'] = _occ_out(it['cnt_a'])
+            if 'vertices' in it:
+                it['vertices'] = _vertices_io(it['vertices'], _occ_out)
+        elif it['cmd'] in ('REORDER', 'ADD_REO'):
+            it['shift_c'] = _occ_out(it['shift_c'])
+            it['shift_a'] = _occ_out(it['shift_a'])
         out['items'].append(it)
     with open(path, 'w') as f:
         json.dump(out, f, indent=1)
@@ -138,8 +189,15 @@ def load(path):
         if it['cmd'] not in _COMMANDS:
             raise ValueError(f'unknown formula command {it["cmd"]!r}')
         if it['cmd'] in ('CONTR', 'BC'):
+            v = it.get('vertices')
             it = dict(contr(it['target'], it['op1'], it['op2'], _occ_in(it['cnt_c']), _occ_in(it['cnt_a']),
                             it['factor']), cmd=it['cmd'])
+            if v is not None:
+                it['vertices'] = _vertices_io(v, _occ_in)
+        elif it['cmd'] in ('REORDER', 'ADD_REO'):
+            it = reorder(it['target'], it['source'], it['from'], it['to'], _occ_in(it['shift_c']),
+                         _occ_in(it['shift_a']), it.get('factor', 1.0), it.get('sign', 1),
+                         add=it['cmd'] == 'ADD_REO')
         elif it['cmd'] == 'ADD':
             it = add(it['target'], it['source'], it.get('factor', 1.0))
         items.append(it)
@@ -152,10 +210,11 @@ def load(path):
 
 def schedule(items):
     """Levels of a formula: a list of dicts {"init": [INIT items],
-    "add": [ADD items], "contr": [CONTR items], "sym": [list names]} in
-    execution order (the INITs and ADDs of a level run in one launch before
-    its contractions, the projections after them; BC items count as INIT
-    then CONTR, NEW and DELETE do no work).
+    "add": [ADD items], "contr": [CONTR items], "reo": [ADD_REO items],
+    "sym": [list names]} in execution order (the INITs and ADDs of a level
+    run in one launch before its contractions, its reorders after them and
+    the projections last; BC items count as INIT then CONTR, REORDER items
+    as INIT then ADD_REO, NEW and DELETE do no work).
 
     A contraction reads its operands as they are at the start of its level,
     so with items taken in list order:
@@ -166,6 +225,7 @@ def schedule(items):
       after an INIT      the target's last INIT is in the same or an earlier
                          level.
     Contributions to one list commute and need no order among themselves.
+    An ADD_REO is placed like a contraction with the one operand `source`.
     An INIT follows the earlier reads and writes of its target and the
     writes of its source.  An ADD reads its source like a contraction and
     runs in the launch of the level's INITs, so it follows its target's
@@ -185,7 +245,7 @@ def _place(items):
     where it is first needed (lifetimes)."""
     items = expand(items)
     last_write, last_read, last_init = {}, {}, {}     # list -> level
-    inits, contrs, adds = {}, {}, {}
+    inits, contrs, adds, reos = {}, {}, {}, {}
     where = [None] * len(items)
     pending = {}          # list -> (item index, earliest level) of a deferred INIT
 
@@ -232,29 +292,34 @@ def _place(items):
             last_read[src] = max(last_read.get(src, -1), lv)
             where[k] = lv
             continue
-        flush(it['op1'])
-        flush(it['op2'])
-        lv = max(last_write.get(it['op1'], -1) + 1, last_write.get(it['op2'], -1) + 1,
-                 last_read.get(t, -1) + 1, last_init.get(t, 0))
+        # CONTR (operands op1, op2) and ADD_REO (operand source): a level's
+        # reorders run after its contractions, and like them read lists
+        # that no item of their level writes
+        ops = (it['source'],) if it['cmd'] == 'ADD_REO' else (it['op1'], it['op2'])
+        for op in ops:
+            flush(op)
+        lv = max([last_write.get(op, -1) + 1 for op in ops] +
+                 [last_read.get(t, -1) + 1, last_init.get(t, 0)])
         if t in pending:
             put_init(t, max(pending[t][1], lv))
             lv = max(lv, last_init[t])
-        contrs.setdefault(lv, []).append(it)
+        (reos if it['cmd'] == 'ADD_REO' else contrs).setdefault(lv, []).append(it)
         last_write[t] = max(last_write.get(t, -1), lv)
-        for op in (it['op1'], it['op2']):
+        for op in ops:
             last_read[op] = max(last_read.get(op, -1), lv)
         where[k] = lv
     for t in list(pending):
         flush(t)
-    nlev = max(list(contrs) + list(inits) + list(adds) + [-1]) + 1
-    levels = [{'init': inits.get(l, []), 'add': adds.get(l, []), 'contr': contrs.get(l, []), 'sym': []}
-              for l in range(nlev)]
-    writes = [{c['target'] for c in lv['contr'] + lv['add']} for lv in levels]
+    nlev = max(list(contrs) + list(inits) + list(adds) + list(reos) + [-1]) + 1
+    levels = [{'init': inits.get(l, []), 'add': adds.get(l, []), 'contr': contrs.get(l, []),
+               'reo': reos.get(l, []), 'sym': []} for l in range(nlev)]
+    writes = [{c['target'] for c in lv['contr'] + lv['add'] + lv['reo']} for lv in levels]
     reads = [{c['op1'] for c in lv['contr']} | {c['op2'] for c in lv['contr']} |
-             {a['source'] for a in lv['add']} | {i['source'] for i in lv['init'] if 'source' in i}
+             {a['source'] for a in lv['add'] + lv['reo']} |
+             {i['source'] for i in lv['init'] if 'source' in i}
              for lv in levels]
     for li, lv in enumerate(levels):
-        for res in dict.fromkeys(c['target'] for c in lv['contr'] + lv['add']):
+        for res in dict.fromkeys(c['target'] for c in lv['contr'] + lv['add'] + lv['reo']):
             nxt = next((l for l in range(li + 1, nlev) if res in writes[l]), None)
             last = nlev - 1 if nxt is None else nxt
             if nxt is None or any(res in reads[l] for l in range(li + 1, last + 1)):
@@ -330,6 +395,7 @@ def linearize(formula, varying):
       INIT X = 0           ->  dX = 0
       INIT X = f * S       ->  dX = f * dS          (0 when S is fixed)
       ADD X += f * S       ->  dX += f * dS         (nothing when S is fixed)
+      ADD_REO X += f R(S)  ->  dX += f R(dS)        (R: the same reorder)
       CONTR X += f A . B   ->  dX += f dA . B + f A . dB
     (terms of fixed operands dropped), so that with the base lists holding
     the formula's values at t, the delta lists hold the directional
@@ -354,16 +420,16 @@ def linearize(formula, varying):
     for it in fitems:                  # lists written from varying ones
         if it['cmd'] == 'CONTR' and (it['op1'] in varies or it['op2'] in varies):
             varies.add(it['target'])
-        elif it['cmd'] in ('INIT', 'ADD') and it.get('source') in varies:
+        elif it['cmd'] in ('INIT', 'ADD', 'ADD_REO') and it.get('source') in varies:
             varies.add(it['target'])
     items = []
     for it in fitems:
         t = it['target']
         if t not in varies or t in varying:
             continue
-        if it['cmd'] == 'ADD':
+        if it['cmd'] in ('ADD', 'ADD_REO'):
             if it['source'] in varies:
-                items.append(add(delta_name(t), delta_name(it['source']), it['factor']))
+                items.append(dict(it, target=delta_name(t), source=delta_name(it['source'])))
         elif it['cmd'] == 'INIT':
             if it.get('source') in varies:
                 items.append(init(delta_name(t), delta_name(it['source']), it['factor']))
@@ -371,9 +437,7 @@ def linearize(formula, varying):
                 items.append(init(delta_name(t)))
         elif it['cmd'] == 'CONTR':
             if it['op1'] in varies:
-                items.append(contr(delta_name(t), delta_name(it['op1']), it['op2'], it['cnt_c'],
-                                   it['cnt_a'], it['factor']))
+                items.append(dict(it, target=delta_name(t), op1=delta_name(it['op1'])))
             if it['op2'] in varies:
-                items.append(contr(delta_name(t), it['op1'], delta_name(it['op2']), it['cnt_c'],
-                                   it['cnt_a'], it['factor']))
+                items.append(dict(it, target=delta_name(t), op2=delta_name(it['op2'])))
     return items, sorted(delta_name(n) for n in varies)

@@ -470,6 +470,11 @@ _lib.gecco_hip_plan_add_contraction.argtypes = [
     VP, VP, VP, VP, PI64, PI64, ctypes.c_double, I64, ctypes.c_int32, I64, ctypes.c_int32, I64,
     ctypes.c_int32, I64, PI64, PD]
 _lib.gecco_hip_plan_add_contraction.restype = ctypes.c_int
+_lib.gecco_hip_plan_add_contraction_n.argtypes = [
+    VP, VP, VP, VP, I64, PI64, PI64, PI64, PI64, PI64, PI64, PI64, ctypes.c_double, I64,
+    ctypes.c_int32, I64, ctypes.c_int32, I64, ctypes.c_int32, I64, ctypes.POINTER(I64),
+    ctypes.POINTER(ctypes.c_double)]
+_lib.gecco_hip_plan_add_contraction_n.restype = ctypes.c_int
 _lib.gecco_hip_plan_add_sym_ab.argtypes = [VP, VP, ctypes.c_int32, I64, ctypes.c_double,
                                            ctypes.c_double]
 _lib.gecco_hip_plan_add_sym_ab.restype = ctypes.c_int
@@ -672,6 +677,31 @@ def _plan_add_contraction(self, op1, op2, res, cnt_c, cnt_a, fact=1.0, use_tr=Tr
     return nb.value, fl.value
 
 
+def _iocc_n(vocc):
+    """Per-vertex (occ_c, occ_a) dicts -> iocc(4, 2, nvtx), C column first."""
+    if not vocc:
+        return np.zeros(8, dtype=np.int64)
+    return np.concatenate([np.concatenate([_occ4(c), _occ4(a)]) for c, a in vocc]).astype(np.int64)
+
+
+def _plan_add_contraction_n(self, op1, op2, res, ex1, ex2, cnt, merge1, merge2, merge12,
+                            merge21, fact=1.0, use_tr=True, buf=(0, 1, 2), off=(0, 0, 0)):
+    """Record all kernel calls of Res += fact*Op1.Op2 for operators with
+    several vertices (gecco_hip_plan_add_contraction_n): ex1 / ex2 / cnt are
+    per-vertex (occ_c, occ_a) dicts, merge* the short-form merge maps."""
+    e1, e2, cn = _iocc_n(ex1), _iocc_n(ex2), _iocc_n(cnt)
+    mm = [np.asarray(list(m) or [0], dtype=np.int64) for m in (merge1, merge2, merge12, merge21)]
+    nb = I64()
+    fl = ctypes.c_double()
+    _check(_lib.gecco_hip_plan_add_contraction_n(
+        self._h, op1._h, op2._h, res._h, len(cnt), e1.ctypes.data_as(PI64),
+        e2.ctypes.data_as(PI64), cn.ctypes.data_as(PI64), *[m.ctypes.data_as(PI64) for m in mm],
+        float(fact), int(bool(use_tr)), buf[0], off[0], buf[1], off[1], buf[2], off[2],
+        ctypes.byref(nb), ctypes.byref(fl)))
+    self.nblocks += nb.value
+    return nb.value, fl.value
+
+
 def _plan_add_sym_ab(self, lst, buf, off=0, prefac=0.5, relfac=None):
     """Spin-flip symmetrisation of a result list (sym_ab_blk)."""
     if relfac is None:
@@ -712,6 +742,7 @@ def _plan_add_reorder(self, ori, reo, frm, to, shift_c, shift_a, fact=1.0, sign_
 
 
 Plan.add_contraction = _plan_add_contraction
+Plan.add_contraction_n = _plan_add_contraction_n
 Plan.add_reorder = _plan_add_reorder
 Plan.add_trace = _plan_add_trace
 Plan.add_sym_ab = _plan_add_sym_ab

@@ -10,10 +10,11 @@
  * contr_op1op2 (contract/contr_op1op2.f:2-184, reached from
  * contract/fs_contr_drv.f:273).
  *
- * Scope: one-vertex operators (njoined = 1) over hole (H, hpvx 1) and
- * particle (P, hpvx 2) spaces with one orbital subspace each, Ms(total) 0,
- * totally symmetric (gamt 1), as in closed-shell CC; operands and results
- * may be adjoint views (gecco_hip_list_adjoint).
+ * Scope: operators over hole (H, hpvx 1) and particle (P, hpvx 2) spaces
+ * with one orbital subspace each, Ms(total) 0, totally symmetric (gamt 1),
+ * as in closed-shell CC; one-vertex operands and results may be adjoint
+ * views (gecco_hip_list_adjoint); operators with several vertices go
+ * through gecco_hip_plan_add_contraction_n and gecco_hip_plan_add_reorder.
  */
 #ifndef GECCO_HIP_MODEL_H
 #define GECCO_HIP_MODEL_H
@@ -54,8 +55,9 @@ int gecco_hip_list_create(gecco_hip_orbs *orbs, const int64_t *occ_c, const int6
  * vertices): occ[v*8 + 0..3] = C occupation of vertex v (H, P, V, X),
  * occ[v*8 + 4..7] = its A occupation.  Components are condensed as
  * operators/condense_occ.f does (X,P,V,H outer, vertex inner); layout as
- * gecco_hip_list_create.  Contractions and traces take one-vertex lists;
- * reorders (gecco_hip_plan_add_reorder) take any. */
+ * gecco_hip_list_create.  Contractions take them through
+ * gecco_hip_plan_add_contraction_n, reorders (gecco_hip_plan_add_reorder)
+ * directly; traces and spin projections take one-vertex lists. */
 int gecco_hip_list_create_n(gecco_hip_orbs *orbs, int64_t njoined, const int64_t *occ,
                             int64_t absym, gecco_hip_list **out);
 /* Adjoint view X^+ of a one-vertex list X, for a contraction operand or
@@ -103,6 +105,31 @@ int gecco_hip_plan_add_contraction(gecco_hip_plan *plan, const gecco_hip_list *o
                                    int64_t use_tr, int32_t buf1, int64_t off1, int32_t buf2,
                                    int64_t off2, int32_t buf12, int64_t off12, int64_t *nblocks,
                                    double *flops);
+/* The same for operators with several vertices (lists from
+ * gecco_hip_list_create_n), with contr_op1op2's description of the
+ * contraction (contract/contr_op1op2.f:2-17, the arguments iocc_ex1,
+ * iocc_ex2, iocc_cnt, merge_op1, merge_op2, merge_op1op2, merge_op2op1,
+ * njoined_cnt): iocc_ex1 (4, 2, op1 vertices), iocc_ex2 (4, 2, op2
+ * vertices), iocc_cnt (4, 2, njoined_cnt) with the C column before the A
+ * column (CNT C meets op1's C side), and the short-form merge maps of
+ * contract/get_bc_info3.f:271-330 (condense_merge_map: per target vertex a
+ * count and the 1-based source-1 vertices, then the same for source 2):
+ * op1 <- (CNT, EX1), op2 <- (CNT^+, EX2), res C <- (EX1, EX2), res A <-
+ * (EX2, EX1).  res is the list the kernel calls write (me_op1op2tmp); a
+ * target grouped differently is reached by gecco_hip_plan_add_reorder of
+ * res afterwards, as contract/contr_op1op2_wmaps_c.f:1962-1995 reorders
+ * each result block.  Stored lists only (no adjoint views); a component
+ * that would merge two components of one source list is refused (route 3
+ * keeps one string per target component and source). */
+int gecco_hip_plan_add_contraction_n(gecco_hip_plan *plan, const gecco_hip_list *op1,
+                                     const gecco_hip_list *op2, const gecco_hip_list *res,
+                                     int64_t njoined_cnt, const int64_t *iocc_ex1,
+                                     const int64_t *iocc_ex2, const int64_t *iocc_cnt,
+                                     const int64_t *merge_op1, const int64_t *merge_op2,
+                                     const int64_t *merge_op1op2, const int64_t *merge_op2op1,
+                                     double fact, int64_t use_tr, int32_t buf1, int64_t off1,
+                                     int32_t buf2, int64_t off2, int32_t buf12, int64_t off12,
+                                     int64_t *nblocks, double *flops);
 /* Partial trace (self-contraction) of a list, run after the plan's
  * contractions:  OPtr(IC, IA) += fact * sum_J s * OP(J.IC, J.IA), with the
  * same string J (occupation cnt[4] per space type H, P, V, X) removed from
@@ -122,6 +149,10 @@ int gecco_hip_plan_add_trace(gecco_hip_plan *plan, const gecco_hip_list *op,
  * contract/reo_op_wmaps_c.f:2-575 with its block kernel
  * contract/reo_blk_wmaps_c.f:2-854 (called from contract/fs_reo_drv.f:166
  * and for reordered contraction results, contract/contr_op1op2_wmaps_c.f:1987).
+ * Vertices are numbered as ORI's; REO may have fewer vertices when some of
+ * I0-with-K are empty (they are dropped: an empty vertex holds no
+ * component, so a two-vertex list whose second vertex empties has the
+ * one-vertex list's layout).
  * sign_reo (+1/-1) is the sign the formula layer assigns to moving K past
  * the operators of intervening vertices (reorder_info%sign_reo).  Runs after
  * the plan's contractions, with the partial traces.  Returns the number of

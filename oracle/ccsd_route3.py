@@ -13,8 +13,10 @@ table (gecco-public_amd/ccsd.py, the role GeCCo's formula file plays for
 contract/frm_sched2.f:117-365); each contraction's kernel calls from
 oracle.contract.Contraction (contract/contr_op1op2_wmaps_c.f:964-2070);
 the spin-flip projection of each written list after its level
-(contract/sym_ab_blk.f); the diagonal preconditioner (contract/dia4op_ev.f:
-300-365) and DIIS with 8 vectors (solve/optc_diis_extr.f:43-108).
+(contract/sym_ab_blk.f); lists of several vertices and REORDER items
+(oracle.contract.Contraction.joined, oracle/reorder.py); the diagonal
+preconditioner (contract/dia4op_ev.f:300-365) and DIIS with 8 vectors
+(solve/optc_diis_extr.f:43-108).
 
 Parity status: pinned -- its energy equals the dense textbook CCSD
 (oracle/ccsd_dense.py) on small systems (tests/test_oracle_ccsd_route3.py)."""
@@ -23,6 +25,7 @@ import time
 import numpy as np
 
 from .contract import Contraction, sym_ab_pairs, apply_sym_pairs
+from .reorder import Reorder
 from .melist import Orbitals, MEList
 from .strings import strings_of, to_spinorbs
 
@@ -114,15 +117,24 @@ class Route3CCSD:
         for lv in schedule:
             cs = []
             for it in lv['contr']:
-                c = Contraction(self.L[it['op1']], self.L[it['op2']], self.L[it['target']], it['cnt_c'],
-                                it['cnt_a'], it['factor'])
+                v = it.get('vertices')
+                if v is None:
+                    c = Contraction(self.L[it['op1']], self.L[it['op2']], self.L[it['target']],
+                                    it['cnt_c'], it['cnt_a'], it['factor'])
+                else:       # lists of several vertices
+                    c = Contraction.joined(self.L[it['op1']], self.L[it['op2']], self.L[it['target']],
+                                           v['ex1'], v['ex2'], v['cnt'], *v['merge'], fact=it['factor'])
                 blocks = c.blocks(True)
                 self.flops += sum(2.0 * np.prod(b['lstr_ex1']) * np.prod(b['lstr_ex2']) *
                                   np.prod(b['lstr_cnt']) for b in blocks)
                 self.calls += len(blocks)
                 cs.append((it['target'], it['op1'], it['op2'], c, blocks))
-            self.levels.append((lv['init'], lv.get('add', []), cs, lv['sym']))
-        self.pairs = {n: sym_ab_pairs(self.L[n], 1.0) for _, _, _, sym in self.levels for n in sym}
+            reos = [(it['target'], it['source'],
+                     Reorder(self.L[it['source']], self.L[it['target']], it['from'], it['to'],
+                             it['shift_c'], it['shift_a'], it['factor'], it['sign']))
+                    for it in lv.get('reo', [])]
+            self.levels.append((lv['init'], lv.get('add', []), cs, reos, lv['sym']))
+        self.pairs = {n: sym_ab_pairs(self.L[n], 1.0) for *_, sym in self.levels for n in sym}
         for n in self.tnames:
             self.X[n] = np.zeros(self.L[n].length)
 
@@ -131,7 +143,7 @@ class Route3CCSD:
         ...: returns the residual Omega = R - D t (the function g of the
         solver) over all ranks."""
         X, L = self.X, self.L
-        for inits, adds, cs, sym in self.levels:
+        for inits, adds, cs, reos, sym in self.levels:
             for it in inits:
                 n = it['target']
                 X[n] = it['factor'] * X[it['source']] if 'source' in it else np.zeros(L[n].length)
@@ -139,6 +151,8 @@ class Route3CCSD:
                 X[it['target']] = X[it['target']] + it['factor'] * X[it['source']]
             for res, a, b, c, blocks in cs:
                 c.run(X[a], X[b], X[res], route=3, use_tr=False, screen=False, blocks=blocks)
+            for res, src, ro in reos:       # REORDER items (contract/fs_reo_drv.f)
+                X[res] = ro.run(X[src], X[res])
             for n in sym:
                 apply_sym_pairs(X[n], self.pairs[n], 0.5)
         t = np.concatenate([X[n] for n in self.tnames])

@@ -29,7 +29,7 @@ import os
 import numpy as np
 
 from .strings import multd2h, strmap, flipmap, strings_of, to_spinorbs, std_spsign_msdis
-from .melist import condense, msgamdists, idxms_of, msa2idxms4op, set_op_ldim_c, HPVXSEQ
+from .melist import condense, condense_n, msgamdists, idxms_of, msa2idxms4op, set_op_ldim_c, HPVXSEQ
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -132,6 +132,52 @@ def _map_info(tgt_h, s1_h, s2_h):
     return mi
 
 
+def _merge_lists(mmap, ntgt):
+    """contract/condense_merge_map.f short form -> per target vertex the
+    (source-1 vertices, source-2 vertices), 1-based."""
+    out, k = [], 0
+    for _ in range(ntgt):
+        n = mmap[k]; v1 = list(mmap[k + 1:k + 1 + n]); k += 1 + n
+        n = mmap[k]; v2 = list(mmap[k + 1:k + 1 + n]); k += 1 + n
+        out.append((v1, v2))
+    return out
+
+
+def _idx_seq(vocc, side):
+    """contract/set_idx_seq.f (sep_ca): condensed index (1-based) of every
+    (space, vertex) of one side, X,P,V,H outer and vertex inner."""
+    seq, k = {}, 0
+    for h in HPVXSEQ:
+        for v, occ in enumerate(vocc):
+            if occ[side].get(h, 0) > 0:
+                k += 1
+                seq[(h, v + 1)] = k
+    return seq
+
+
+def _set_mapping_info(tgt, merge, src1, dag1, src2, dag2, side):
+    """contract/set_mapping_info.f:84-170 for one side (0: C, 1: A).  tgt,
+    src1, src2: per vertex (occ_c, occ_a) dicts; merge: short-form merge map
+    of the target vertices; dagger swaps a source's C and A sides."""
+    s1, s2 = (1 - side if dag1 else side), (1 - side if dag2 else side)
+    seq1, seq2 = _idx_seq(src1, s1), _idx_seq(src2, s2)
+    lists = _merge_lists(merge, len(tgt))
+    mi = []
+    for h in HPVXSEQ:
+        for tv, occ in enumerate(tgt):
+            if occ[side].get(h, 0) == 0:
+                continue
+            v1, v2 = lists[tv]
+            i1 = [seq1[(h, v)] for v in v1 if src1[v - 1][s1].get(h, 0) > 0]
+            i2 = [seq2[(h, v)] for v in v2 if src2[v - 1][s2].get(h, 0) > 0]
+            mi += [len(i1)] + i1 + [len(i2)] + i2
+    return mi
+
+
+def _vocc(mel):
+    return [(dict(c), dict(a)) for c, a in zip(mel.vocc_c, mel.vocc_a)]
+
+
 class Contraction:
     """Binary contraction  Res += fact * Op1 . Op2  over CNT.
 
@@ -172,6 +218,88 @@ class Contraction:
         self.mi_1a = _map_info(op1.ha, self.cnta_h, self.ex1a_h)
         self.mi_2c = _map_info(op2.hc, self.cnta_h, self.ex2c_h)
         self.mi_2a = _map_info(op2.ha, self.cntc_h, self.ex2a_h)
+
+    @classmethod
+    def joined(cls, op1, op2, res, ex1, ex2, cnt, merge1, merge2, merge12, merge21, fact=1.0):
+        """Contraction of operators with several vertices (njoined >= 1), as
+        contr_op1op2 receives it (contract/contr_op1op2.f:2-17): ex1 / ex2 /
+        cnt are per-vertex (occ_c, occ_a) dicts of EX1 (one per op1 vertex),
+        EX2 (one per op2 vertex) and the contraction lines CNT (one per
+        contracted vertex pair); merge1 / merge2 / merge12 / merge21 are the
+        short-form merge maps of contract/get_bc_info3.f:271-330
+        (condense_merge_map): per op1 vertex its (CNT vertices, EX1
+        vertices), per op2 vertex its (CNT vertices, EX2 vertices), per result
+        vertex its (EX1 vertices, EX2 vertices), and that map with the two
+        lists swapped (the A-side order).  The condensed description follows
+        contract/condense_bc_info.f:242-365 and the map_info arrays
+        contract/set_mapping_info.f (OP2 takes CNT daggered).  res is the
+        list the kernel calls write (op1op2tmp); a different target vertex
+        grouping is reached by a reorder of res afterwards
+        (contract/contr_op1op2_wmaps_c.f:1962-1995).
+
+        A target component may take at most one component from each source
+        list: the route-3 collect and scatter steps carry one string per
+        target component and source (contract/collect_block_impr.f:129-194,
+        ireo from contract/set_strmapdim_c2.f), so anything else is
+        refused."""
+        self = cls.__new__(cls)
+        if any(getattr(x, 'transposed', False) for x in (op1, op2, res)):
+            raise ValueError('Contraction.joined: stored lists only')
+        self.op1, self.op2, self.res = op1, op2, res
+        self.orbs = op1.orbs
+        self.fact = fact
+        ex1 = [(dict(c), dict(a)) for c, a in ex1]
+        ex2 = [(dict(c), dict(a)) for c, a in ex2]
+        cnt = [(dict(c), dict(a)) for c, a in cnt]
+        if len(ex1) != op1.njoined or len(ex2) != op2.njoined:
+            raise ValueError('Contraction.joined: EX1/EX2 need one occupation per operand vertex')
+        o1, o2, o12 = _vocc(op1), _vocc(op2), _vocc(res)
+        dag = lambda x: (x[1], x[0])  # noqa: E731
+
+        def check(tgt, mmap, s1, s2, what, sides=(0, 1)):
+            for tv, (v1, v2) in enumerate(_merge_lists(mmap, len(tgt))):
+                for side in sides:
+                    tot = {}
+                    for v in v1:
+                        for h, n in s1[v - 1][side].items():
+                            tot[h] = tot.get(h, 0) + n
+                    for v in v2:
+                        for h, n in s2[v - 1][side].items():
+                            tot[h] = tot.get(h, 0) + n
+                    if {h: n for h, n in tot.items() if n} != \
+                       {h: n for h, n in tgt[tv][side].items() if n}:
+                        raise ValueError(f'Contraction.joined: {what} vertex {tv + 1} does not '
+                                         'equal the sum of its merged parts')
+        check(o1, merge1, cnt, ex1, 'op1')
+        check(o2, merge2, [dag(x) for x in cnt], ex2, 'op2')
+        check(o12, merge12, ex1, ex2, 'result', sides=(0,))
+        check(o12, merge21, ex2, ex1, 'result', sides=(1,))
+        self.cnt_vocc, self.ex1_vocc, self.ex2_vocc = cnt, ex1, ex2
+        self.merge = (list(merge1), list(merge2), list(merge12), list(merge21))
+
+        def cond(v):
+            oc, hc, _, oa, ha, _ = condense_n([c for c, _ in v], [a for _, a in v])
+            return oc, hc, oa, ha
+        self.ex1c_o, self.ex1c_h, self.ex1a_o, self.ex1a_h = cond(ex1)
+        self.ex2c_o, self.ex2c_h, self.ex2a_o, self.ex2a_h = cond(ex2)
+        self.cntc_o, self.cntc_h, self.cnta_o, self.cnta_h = cond(cnt)
+        self.mi_12c = _set_mapping_info(o12, merge12, ex1, False, ex2, False, 0)
+        self.mi_12a = _set_mapping_info(o12, merge21, ex2, False, ex1, False, 1)
+        self.mi_1c = _set_mapping_info(o1, merge1, cnt, False, ex1, False, 0)
+        self.mi_1a = _set_mapping_info(o1, merge1, cnt, False, ex1, False, 1)
+        self.mi_2c = _set_mapping_info(o2, merge2, cnt, True, ex2, False, 0)
+        self.mi_2a = _set_mapping_info(o2, merge2, cnt, True, ex2, False, 1)
+        for mi, n in ((self.mi_12c, len(res.hc)), (self.mi_12a, len(res.ha)),
+                      (self.mi_1c, len(op1.hc)), (self.mi_1a, len(op1.ha)),
+                      (self.mi_2c, len(op2.hc)), (self.mi_2a, len(op2.ha))):
+            k = 0
+            for _ in range(n):
+                n1 = mi[k]; k += 1 + n1
+                n2 = mi[k]; k += 1 + n2
+                if n1 > 1 or n2 > 1:
+                    raise ValueError('Contraction.joined: multi map needed (two components of '
+                                     'one source merge into one target component)')
+        return self
 
     # ---------------------------------------------------------------
     def _lens(self, occs, hs, idxms, gams):

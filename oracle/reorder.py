@@ -95,22 +95,31 @@ class Reorder:
             i0a[frm][h] = i0a[frm].get(h, 0) - n
         if any(v < 0 for x in i0c + i0a for v in x.values()):
             raise ValueError('reorder: K is not part of the from-vertex')
-        chk_c = [dict(x) for x in i0c]
-        chk_a = [dict(x) for x in i0a]
+        nv = max(len(i0c), reo.njoined)
+        chk_c = [dict(x) for x in i0c] + [{} for _ in range(nv - len(i0c))]
+        chk_a = [dict(x) for x in i0a] + [{} for _ in range(nv - len(i0a))]
         for h, n in shift_c.items():
             chk_c[to][h] = chk_c[to].get(h, 0) + n
         for h, n in shift_a.items():
             chk_a[to][h] = chk_a[to].get(h, 0) + n
         norm = lambda L: [{h: n for h, n in x.items() if n} for x in L]
-        if norm(chk_c) != norm(reo.vocc_c) or norm(chk_a) != norm(reo.vocc_a):
+        # REO's vertices are I0's with K on `to` (ORI's numbering); a REO
+        # with fewer vertices drops those left empty (no component, same
+        # layout)
+        keep = [v for v in range(nv) if nv == reo.njoined or norm([chk_c[v], chk_a[v]]) != [{}, {}]]
+        vmap = {v: k for k, v in enumerate(keep)}
+        if len(keep) != reo.njoined or norm([chk_c[v] for v in keep]) != norm(reo.vocc_c) or \
+           norm([chk_a[v] for v in keep]) != norm(reo.vocc_a):
             raise ValueError('reorder: REO is not I0 with K on the to-vertex')
         (self.i0c_o, self.i0c_h, self.i0c_v,
          self.i0a_o, self.i0a_h, self.i0a_v) = condense_n(i0c, i0a)
         self.kc_o, self.kc_h, self.ka_o, self.ka_h = condense(shift_c, shift_a)
         self.ori_c = _sources(ori.hc, ori.vc, self.i0c_h, self.i0c_v, self.kc_h, frm, False)
         self.ori_a = _sources(ori.ha, ori.va, self.i0a_h, self.i0a_v, self.ka_h, frm, True)
-        self.reo_c = _sources(reo.hc, reo.vc, self.i0c_h, self.i0c_v, self.kc_h, to, False)
-        self.reo_a = _sources(reo.ha, reo.va, self.i0a_h, self.i0a_v, self.ka_h, to, True)
+        rv_c = [vmap[v] for v in self.i0c_v]
+        rv_a = [vmap[v] for v in self.i0a_v]
+        self.reo_c = _sources(reo.hc, reo.vc, self.i0c_h, rv_c, self.kc_h, vmap.get(to, -1), False)
+        self.reo_a = _sources(reo.ha, reo.va, self.i0a_h, rv_a, self.ka_h, vmap.get(to, -1), True)
 
     def _lens(self, occs, hs, ms, gm):
         return [self.orbs.lenstr(h, o, idxms_of(o, m), g) for o, h, m, g in zip(occs, hs, ms, gm)]

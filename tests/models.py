@@ -238,3 +238,107 @@ def ccsd_with_add_bc_new_delete():
             if last_use.get(x) == k:
                 items.append(F.delete(x))
     return {'lists': dict(ccsd.LISTS), 'constant': list(ccsd.INTEGRALS), 'items': items}
+
+
+# Contractions of operators with several vertices (Contraction.joined): per
+# vertex (occ_c, occ_a) of EX1, EX2 and CNT, and the short-form merge maps
+# (per target vertex: count and list of source-1 vertices, then of source-2
+# vertices; contract/condense_merge_map.f).  Each returns the contraction
+# and the lists (op1, op2, result).
+def mv_t1t1_joined(orbs):
+    """X(ai;bj) = T1(a,i) T1(b,j) as a two-vertex list, no contraction: result
+    vertex 1 takes EX1, vertex 2 EX2."""
+    T1 = MEList(orbs, {P: 1}, {H: 1})
+    X = MEList(orbs, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    c = Contraction.joined(T1, T1, X, ex1=[({P: 1}, {H: 1})], ex2=[({P: 1}, {H: 1})], cnt=[],
+                           merge1=[0, 1, 1], merge2=[0, 1, 1],
+                           merge12=[1, 1, 0, 0, 1, 1], merge21=[0, 1, 1, 1, 1, 0])
+    return c, T1, T1, X
+
+
+def mv_vertex_fock(orbs):
+    """X'(ai;bj) = sum_k X(ai;bk) F(k,j): the second vertex of a two-vertex
+    list contracts with a one-body hole operator, EX2 joins that vertex."""
+    X = MEList(orbs, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    F = MEList(orbs, {H: 1}, {H: 1})
+    Xp = MEList(orbs, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    c = Contraction.joined(X, F, Xp, ex1=[({P: 1}, {H: 1}), ({P: 1}, {})], ex2=[({}, {H: 1})],
+                           cnt=[({}, {H: 1})],
+                           merge1=[0, 1, 1, 1, 1, 1, 2], merge2=[1, 1, 1, 1],
+                           merge12=[1, 1, 0, 1, 2, 1, 1], merge21=[0, 1, 1, 1, 1, 1, 2])
+    return c, X, F, Xp
+
+
+def mv_pairwise_scalar(orbs):
+    """E = sum X(ai;bj) Y(ia;jb): both vertices of X contract with the
+    matching vertex of Y (two CNT vertices), scalar result."""
+    X = MEList(orbs, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    Y = MEList(orbs, [{H: 1}, {H: 1}], [{P: 1}, {P: 1}])
+    E = MEList(orbs, {}, {})
+    c = Contraction.joined(X, Y, E, ex1=[({}, {}), ({}, {})], ex2=[({}, {}), ({}, {})],
+                           cnt=[({P: 1}, {H: 1}), ({P: 1}, {H: 1})],
+                           merge1=[1, 1, 1, 1, 1, 2, 1, 2], merge2=[1, 1, 1, 1, 1, 2, 1, 2],
+                           merge12=[2, 1, 2, 2, 1, 2], merge21=[2, 1, 2, 2, 1, 2])
+    return c, X, Y, E
+
+
+def mv_ring_to_one_vertex(orbs):
+    """R(ak;il) += sum_{c,m} T2(ac,im) Y[(m;c),(k;l)]: T2 contracts a
+    particle and a hole with the first vertex of the two-vertex Y; Y's second
+    vertex joins EX1 in the one-vertex result (C = P from T2 + H from Y,
+    A = H from Y merged with H from T2)."""
+    T2 = MEList(orbs, {P: 2}, {H: 2})
+    Y = MEList(orbs, [{H: 1}, {H: 1}], [{P: 1}, {H: 1}])
+    R = MEList(orbs, {P: 1, H: 1}, {H: 2})
+    c = Contraction.joined(T2, Y, R, ex1=[({P: 1}, {H: 1})], ex2=[({}, {}), ({H: 1}, {H: 1})],
+                           cnt=[({P: 1}, {H: 1})],
+                           merge1=[1, 1, 1, 1], merge2=[1, 1, 1, 1, 0, 1, 2],
+                           merge12=[1, 1, 2, 1, 2], merge21=[2, 1, 2, 1, 1])
+    return c, T2, Y, R
+
+
+def mv_two_vertex_result_merge(orbs):
+    """Z[(ab;ck),(d;j)]-type: the first vertex of X(ai;dj) contracts its
+    hole with the creator hole of W(kb;ck); W's remaining lines merge into
+    result vertex 1 (creators P from X and from W in one component), X's
+    second vertex stays result vertex 2."""
+    X = MEList(orbs, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    W = MEList(orbs, {P: 1, H: 1}, {P: 1, H: 1})
+    Z = MEList(orbs, [{P: 2}, {P: 1}], [{P: 1, H: 1}, {H: 1}])
+    c = Contraction.joined(X, W, Z, ex1=[({P: 1}, {}), ({P: 1}, {H: 1})],
+                           ex2=[({P: 1}, {P: 1, H: 1})], cnt=[({}, {H: 1})],
+                           merge1=[1, 1, 1, 1, 0, 1, 2], merge2=[1, 1, 1, 1],
+                           merge12=[1, 1, 1, 1, 1, 2, 0], merge21=[1, 1, 1, 1, 0, 1, 2])
+    return c, X, W, Z
+
+
+JOINED_CASES = {'t1t1_joined': mv_t1t1_joined, 'vertex_fock': mv_vertex_fock,
+                'pairwise_scalar': mv_pairwise_scalar, 'ring_one_vertex': mv_ring_to_one_vertex,
+                'result_merge': mv_two_vertex_result_merge}
+
+
+def ccsd_with_joined_tau():
+    """The CCSD formula with the T1 x T1 parts of tau and taut formed once as
+    the two-vertex intermediate X11(ai;bj) = T1(a,i) T1(b,j) (a BC between
+    lists of several vertices) and added to tau and taut by ADD_REO items
+    that move X11's second vertex onto its first (contract/fs_reo_drv.f):
+    the reordered product is the one-vertex T1 T1 product, so the residuals
+    equal the hand formula's."""
+    import ccsd
+    import formula as F
+    lists = dict(ccsd.LISTS)
+    lists['X11'] = ([{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    items, done = [], False
+    for it in ccsd.FORMULA['items']:
+        if it['cmd'] == 'CONTR' and it['op1'] == 'T1' and it['op2'] == 'T1' and not it['cnt_c'] \
+                and not it['cnt_a']:
+            if not done:
+                items.append(F.init('X11'))
+                items.append(F.contr_n('X11', 'T1', 'T1', [({P: 1}, {H: 1})], [({P: 1}, {H: 1})], [],
+                                       [[0, 1, 1], [0, 1, 1], [1, 1, 0, 0, 1, 1], [0, 1, 1, 1, 1, 0]],
+                                       1.0))
+                done = True
+            items.append(F.reorder(it['target'], 'X11', 1, 0, {P: 1}, {H: 1}, it['factor'], 1, add=True))
+        else:
+            items.append(it)
+    return {'lists': lists, 'constant': list(ccsd.INTEGRALS), 'items': items}

@@ -74,6 +74,8 @@ def _run_sequential(items, lists):
             x[it['target']] = x[it['target']] + it['factor'] * x[it['source']]
         elif it['cmd'] == 'CONTR':
             x[it['target']] = x[it['target']] + it['factor'] * x[it['op1']] * x[it['op2']]
+        elif it['cmd'] == 'ADD_REO':        # a reorder as the identity map
+            x[it['target']] = x[it['target']] + it['factor'] * it['sign'] * x[it['source']]
     return x
 
 
@@ -119,6 +121,14 @@ def _run_levels(levels, lists, offsets=None, pool=0):
             add[c['target']] = add.get(c['target'], 0.0) + c['factor'] * x[c['op1']] * x[c['op2']]
         for k, v in add.items():
             x[k] = x[k] + v
+        # the level's reorders, after its contractions: they read lists no
+        # item of the level writes
+        rr = {r['source'] for r in lv.get('reo', [])}
+        rw = {r['target'] for r in lv.get('reo', [])}
+        assert not rr & (rw | writes | writes0), (rr, lv)
+        assert not rw & (reads | reads0), (rw, lv)
+        for r in lv.get('reo', []):
+            x[r['target']] = x[r['target']] + r['factor'] * r['sign'] * x[r['source']]
     return x
 
 
@@ -157,6 +167,11 @@ def _random_items(rng, names):
             src = names[int(rng.integers(0, len(names)))]
             if src != t:
                 items.append(F.add(t, src, float(rng.normal())))
+        elif u < 0.4:                       # REORDER / ADD_REO
+            src = names[int(rng.integers(0, len(names)))]
+            if src != t:
+                items.append(F.reorder(t, src, 0, 0, {}, {}, float(rng.normal()),
+                                       int(rng.choice([-1, 1])), add=rng.random() < 0.6))
         else:
             o1, o2 = (names[int(i)] for i in rng.integers(0, len(names), 2))
             if t in (o1, o2):
@@ -271,10 +286,12 @@ def test_linearised_formula_is_the_derivative():
         items = [F.init(n) for n in ('X', 'Y', 'Z')]
         for _ in range(int(rng.integers(2, 10))):
             t = names[3 + int(rng.integers(0, 3))]
-            if rng.random() < 0.25:           # ADD items: dX += f dS
+            u = rng.random()
+            if u < 0.35:        # ADD and ADD_REO items: dX += f dS (f R(dS))
                 src = names[int(rng.integers(0, 6))]
                 if src != t:
-                    items.append(F.add(t, src, float(rng.normal())))
+                    items.append(F.add(t, src, float(rng.normal())) if u < 0.2 else
+                                 F.reorder(t, src, 0, 0, {}, {}, float(rng.normal()), -1, add=True))
                 continue
             o1, o2 = (names[int(i)] for i in rng.integers(0, 6, 2))
             if t in (o1, o2):
@@ -305,3 +322,25 @@ def test_add_bc_new_delete_formula_gives_the_ccsd_energy():
     spans = F.lifetimes(f, keep=['T1', 'T2', 'R1', 'R2', 'E'])
     assert set(spans) == {n for n in ccsd.LISTS if n not in ccsd.INTEGRALS and
                           n not in ('T1', 'T2', 'R1', 'R2', 'E')}
+
+
+def test_joined_intermediate_and_reorder_formula_gives_the_ccsd_energy(tmp_path):
+    """A CCSD formula whose T1 x T1 products go through the two-vertex
+    intermediate X11 and ADD_REO items (tests/models.py) gives the hand
+    formula's energy on the oracle's route 3; the formula file round-trips
+    its vertex data and reorder items."""
+    from models import ccsd_with_joined_tau
+    from oracle.ccsd_route3 import Route3CCSD
+    from synth import synthetic
+    f = ccsd_with_joined_tau()
+    assert {'ADD_REO'} <= {it['cmd'] for it in f['items']}
+    assert any('vertices' in it for it in f['items'])
+    F.dump(f, tmp_path / 'j.json')
+    g = F.load(tmp_path / 'j.json')
+    assert F.schedule(g['items']) == F.schedule(f['items'])
+    lv = F.schedule(f['items'])
+    assert any(l['reo'] for l in lv)
+    ham = synthetic(3, 6, 18, nsym=2, seed=4, sigma=0.1)
+    e0, _, _ = Route3CCSD(ham, ccsd.FORMULA, ccsd.SCHEDULE).solve(tol=1e-11)
+    e1, _, _ = Route3CCSD(ham, g, F.schedule(g['items'])).solve(tol=1e-11)
+    assert abs(e0 - e1) < 1e-12

@@ -330,3 +330,25 @@ def test_spin_projected_residuals_give_the_same_energy():
     e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
     r = ccsd.CCSD(ham, spin_project=True).solve(tol=1e-10, maxit=80)
     assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11
+
+
+def test_joined_intermediate_formula_on_the_engine():
+    """A CCSD formula whose T1 x T1 products are formed as the two-vertex
+    intermediate X11 (gecco_hip_plan_add_contraction_n) and reach tau and
+    taut through ADD_REO items (gecco_hip_plan_add_reorder after the level's
+    contractions) gives the hand formula's energy on the GPU; so do its
+    float32 refined solve (linearised reorders) and graph replays."""
+    _torch()
+    import ccsd
+    from models import ccsd_with_joined_tau
+    ham = synthetic(4, 12, 36, nsym=2, seed=17, sigma=0.1)
+    print('joined-tau formula 4/12/2', flush=True)
+    e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
+    c = ccsd.CCSD(ham, formula=ccsd_with_joined_tau())
+    r = c.solve(tol=1e-10, maxit=80)
+    print('joined', r['iterations'], r['energy'] - e0, flush=True)
+    assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11
+    c2 = ccsd.CCSD(ham, formula=ccsd_with_joined_tau())
+    rr = c2.solve_refined(tol=1e-10, switch=1e-6)
+    print('joined refined', rr['f32_iterations'], rr['linear_iterations'], rr['f64_evaluations'], flush=True)
+    assert rr['converged'] and rr['f64_evaluations'] <= 2 and abs(rr['energy'] - e0) < 1e-9

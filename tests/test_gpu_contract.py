@@ -800,3 +800,75 @@ def test_spin_projection_matches_oracle(name, nsym):
     p.execute([d])
     torch.cuda.synchronize()
     assert np.abs(d.cpu().numpy() - once).max() <= 1e-12
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 6, 10)])
+@pytest.mark.parametrize('name', ['t1t1_joined', 'vertex_fock', 'pairwise_scalar', 'ring_one_vertex',
+                                  'result_merge'])
+@pytest.mark.parametrize('use_tr', [True, False])
+def test_joined_contraction_matches_oracle(nsym, nocc, nvir, name, use_tr):
+    """Contractions of operators with several vertices
+    (gecco_hip_plan_add_contraction_n) on the GPU against the oracle's
+    multi-vertex contr_op1op2 restatement, accumulating onto existing data."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin, JOINED_CASES
+    print(f'joined {name} nsym={nsym} nocc={nocc} nvir={nvir} use_tr={use_tr}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = JOINED_CASES[name](orbs)
+    x1, x2, x0 = spin_symmetric(a, 64), spin_symmetric(b, 65), spin_symmetric(r, 66)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=use_tr, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    A, B, R = (gecco_hip.MEList(o, m.vocc_c, m.vocc_a) for m in (a, b, r))
+    plan = gecco_hip.Plan()
+    m1, m2, m12, m21 = c.merge
+    nb, fl = plan.add_contraction_n(A, B, R, c.ex1_vocc, c.ex2_vocc, c.cnt_vocc, m1, m2, m12, m21,
+                                    1.0, use_tr)
+    assert nb == len(c.blocks(use_tr))
+    if use_tr:
+        plan.add_sym_ab(R, 2)
+    plan.finalize()
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x).to(dev) for x in (x1, x2, x0)]
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (4, 6, 10)])
+def test_joined_product_reordered_in_one_plan(nsym, nocc, nvir):
+    """The reorder after a contraction in one plan: T1 x T1 into the
+    two-vertex X, then X's second vertex moved onto its first (the plan runs
+    reorders after its contractions) -- equal to the one-vertex product
+    R(ab,ij) += T1(a,i) T1(b,j) of the oracle."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin, t1t1
+    print(f'joined+reorder nsym={nsym} nocc={nocc} nvir={nvir}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    c1, T1o, _, Ro = t1t1(orbs)
+    x = spin_symmetric(T1o, 67)
+    ref = np.zeros(Ro.length)
+    c1.run(x, x, ref, route=3, use_tr=False, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    P, H = 2, 1
+    T1 = gecco_hip.MEList(o, {P: 1}, {H: 1})
+    X = gecco_hip.MEList(o, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    Xp = gecco_hip.MEList(o, [{P: 2}, {}], [{H: 2}, {}])
+    assert Xp.length == Ro.length
+    plan = gecco_hip.Plan()
+    plan.add_init(2, X.length)
+    plan.add_contraction_n(T1, T1, X, [({P: 1}, {H: 1})], [({P: 1}, {H: 1})], [], [0, 1, 1],
+                           [0, 1, 1], [1, 1, 0, 0, 1, 1], [0, 1, 1, 1, 1, 0], 1.0, False,
+                           buf=(0, 0, 2))
+    plan.add_reorder(X, Xp, 1, 0, {P: 1}, {H: 1}, buf=(2, 3))
+    plan.finalize()
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x).to(dev), torch.zeros(1, dtype=torch.float64, device=dev),
+            torch.zeros(X.length, dtype=torch.float64, device=dev),
+            torch.zeros(Xp.length, dtype=torch.float64, device=dev)]
+    plan.execute(bufs)
+    plan.execute(bufs)      # the initialisation restarts X; the reorder accumulates
+    torch.cuda.synchronize()
+    assert _relerr(bufs[3].cpu().numpy(), 2 * ref) <= TOL

@@ -105,3 +105,58 @@ def test_transposed_operands_emulate_the_oracle(nsym, nocc, nvir, name, use_tr):
     if use_tr:
         sym_ab(stored[2], bufs[2], 0.5, float(stored[2].absym))
     assert np.abs(bufs[2] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _joined_plan(gecco_hip, o, c, lists, use_tr):
+    A, B, R = (gecco_hip.MEList(o, m.vocc_c, m.vocc_a) for m in lists)
+    p = gecco_hip.Plan()
+    m1, m2, m12, m21 = c.merge
+    nb, fl = p.add_contraction_n(A, B, R, c.ex1_vocc, c.ex2_vocc, c.cnt_vocc, m1, m2, m12, m21,
+                                 1.0, use_tr)
+    return p, nb, fl
+
+
+@pytest.mark.parametrize('nsym,nocc,nvir', [(1, 2, 3), (2, 3, 4), (4, 4, 5)])
+@pytest.mark.parametrize('name', ['t1t1_joined', 'vertex_fock', 'pairwise_scalar', 'ring_one_vertex',
+                                  'result_merge'])
+@pytest.mark.parametrize('use_tr', [True, False])
+def test_joined_contractions_emulate_the_oracle(nsym, nocc, nvir, name, use_tr):
+    """Operators with several vertices (gecco_hip_plan_add_contraction_n):
+    the product records the oracle's kernel calls (count and FLOP), and its
+    resolved addresses reproduce the oracle's route 3."""
+    import gecco_hip
+    from models import JOINED_CASES
+    oo = Orbitals(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    c, a, b, r = JOINED_CASES[name](oo)
+    x1, x2, x0 = spin_symmetric(a, 91), spin_symmetric(b, 92), spin_symmetric(r, 93)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=use_tr, screen=False)
+    p, nb, fl = _joined_plan(gecco_hip, o, c, (a, b, r), use_tr)
+    assert nb == len(c.blocks(use_tr)) and fl == c.flops(use_tr)
+    bufs = [x1.copy(), x2.copy(), x0.copy()]
+    emulate(p, nb, bufs)
+    if use_tr:
+        sym_ab(r, bufs[2], 0.5, float(r.absym))
+    assert np.abs(bufs[2] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_joined_contraction_refuses_a_multi_map():
+    """A vertex contracting one space with two vertices of the other
+    operator needs two CNT components merged into one operand component:
+    refused with the reference's 'multi map needed'."""
+    import gecco_hip
+    o = gecco_hip.Orbs(round_robin(2, 1), round_robin(3, 1, 1), 1)
+    H, P = 1, 2
+    V = gecco_hip.MEList(o, {H: 2}, {P: 2})
+    X = gecco_hip.MEList(o, [{P: 1}, {P: 1}], [{H: 1}, {H: 1}])
+    E = gecco_hip.MEList(o, {}, {})
+    p = gecco_hip.Plan()
+    with pytest.raises(gecco_hip.GeccoHipError, match='multi map'):
+        p.add_contraction_n(V, X, E, [({}, {})], [({}, {}), ({}, {})],
+                            [({H: 1}, {P: 1}), ({H: 1}, {P: 1})], [2, 1, 2, 1, 1],
+                            [1, 1, 1, 1, 1, 2, 1, 1], [1, 1, 2, 1, 2], [2, 1, 2, 1, 1])
+    with pytest.raises(gecco_hip.GeccoHipError, match='vertex'):
+        p.add_contraction_n(V, X, E, [({}, {})], [({}, {}), ({}, {})],
+                            [({H: 1}, {P: 1})], [1, 1, 1, 1],
+                            [1, 1, 1, 1, 0, 1, 2], [1, 1, 2, 1, 2], [2, 1, 2, 1, 1])
