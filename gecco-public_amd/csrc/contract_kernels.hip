@@ -340,6 +340,7 @@ struct AffView {
     uint32_t step;    // bytes per K step
     int kk0, kstep;   // k of element r relative to the step start: kk0 + kstep*r
     int xi0, xstep;   // m/n (relative to the tile) of element r: xi0 + xstep*r
+    int so, ss;       // LDS index of element r in a tile with rows of X + 1: so + ss*r
     bool ff;          // the operand's fast_first: which of the two layouts above
 
     // operand block at blk (element (k, x) at o.f0 + o.s0 + o.f1*k + o.s1*x,
@@ -354,6 +355,8 @@ struct AffView {
             kk0 = tid / X; kstep = NT / X;
             xi0 = tid % X; xstep = 0;
         }
+        so = kk0 * (X + 1) + xi0;
+        ss = kstep * (X + 1) + xstep;
         const int64_t extent = (o.f1 * (int64_t)(K - 1) + o.s1 * (int64_t)(Xn - 1) + 1) * ES;
         rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)(blk + o.f0 + o.s0), (short)0, (int)extent,
                                                  BUF_FLAGS);
@@ -392,8 +395,13 @@ struct AffView {
     }
     template <int LD>
     __device__ __forceinline__ void store(T *lds, const T in[L]) const {
+        if constexpr (LD == X + 1) {   // the index precomputed at init
 #pragma unroll
-        for (int r = 0; r < L; ++r) lds[(kk0 + kstep * r) * LD + xi0 + xstep * r] = in[r];
+            for (int r = 0; r < L; ++r) lds[so + ss * r] = in[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < L; ++r) lds[(kk0 + kstep * r) * LD + xi0 + xstep * r] = in[r];
+        }
     }
 };
 
@@ -943,19 +951,31 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
             const T *A = lds + cur * SA;
             const T *B = lds + 2 * SA + cur * SB;
             if (wave_live) {
+                // the LDS reads of k quad kq + 1 are issued before the MFMAs
+                // of quad kq, so they return while those MFMAs run
+                T a[2][TM], bb[2][TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) a[0][i] = A[lrow * LDA + wm * RM + i * 16 + lcol];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) bb[0][j] = B[lrow * LDB + wn * RN + j * 16 + lcol];
 #pragma unroll
                 for (int kq = 0; kq < TBK / 4; ++kq) {
-                    const int kr = kq * 4 + lrow;
-                    T a[TM], bb[TN];
+                    const int c = kq & 1;
+                    if (kq + 1 < TBK / 4) {
+                        const int kr = (kq + 1) * 4 + lrow;
 #pragma unroll
-                    for (int i = 0; i < TM; ++i) a[i] = A[kr * LDA + wm * RM + i * 16 + lcol];
+                        for (int i = 0; i < TM; ++i) a[c ^ 1][i] = A[kr * LDA + wm * RM + i * 16 + lcol];
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) bb[j] = B[kr * LDB + wn * RN + j * 16 + lcol];
+                        for (int j = 0; j < TN; ++j) bb[c ^ 1][j] = B[kr * LDB + wn * RN + j * 16 + lcol];
+                    }
+                    // keep the compiler from sinking those reads below the
+                    // MFMAs (it would reuse the operand registers)
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int i = 0; i < TM; ++i)
 #pragma unroll
                         for (int j = 0; j < TN; ++j)
-                            acc[i][j] = mfma16(a[i], bb[j], acc[i][j]);
+                            acc[i][j] = mfma16(a[c][i], bb[c][j], acc[i][j]);
                 }
             }
             if (more) {
@@ -971,7 +991,6 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
         const DevOperand &oc = b.C;
         T *pc = bufs[b.buf_c] + b.base_c;
         const T xfac = (T)b.xfac;
-        const bool caff = !oc.table && oc.first_affine && oc.second_affine;
         // The tile goes through LDS as a row-major [row][col] array (row
         // stride TBN + 1 against bank conflicts), one half of the rows --
         // those of the waves whose rows lie in half h -- at a time; then each thread adds
@@ -983,6 +1002,25 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
         // old values.  Colliding elements of a non-injective map (the two
         // orderings of an antisymmetrised string pair) add atomically.
         constexpr int HR = TBM / 2, LDE = TBN + 1;
+        // Thread tid adds column col = tid % TBN of rows r0 + RSTEP*k of each
+        // half.  Its column's half of a separable result map is read once;
+        // the row halves (or table entries) of all its RPT elements are
+        // loaded before the first of its atomic adds, so their latencies
+        // overlap instead of adding up element by element.
+        constexpr int RSTEP = NT / TBN, RPT = HR / RSTEP;
+        static_assert(NT % TBN == 0 && HR % RSTEP == 0, "epilogue thread layout");
+        const int col = tid % TBN, r0 = tid / TBN;
+        const int n = n0 + col;
+        bool cok = n < N, cneg = false;
+        int64_t cn = 0;
+        if (cok && !oc.table) {
+            if (oc.second_affine) {
+                cn = oc.s0 + oc.s1 * (int64_t)n;
+            } else {
+                const int32_t t = ldg(oc.second + n);
+                cok = t != 0; cneg = t < 0; cn = (int64_t)(t < 0 ? -t : t) - 1;
+            }
+        }
         for (int h = 0; h < 2; ++h) {
             if (h) __syncthreads();   // the previous half has been read
             if ((wm * RM) / HR == h) {
@@ -995,17 +1033,33 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
                             lds[(wm * RM - h * HR + i * 16 + mrow<T>(r, lrow)) * LDE + wn * RN + j * 16 + lcol] =
                                 acc[i][j][r];
             }
-            __syncthreads();   // (h = 0: the K loop ended with a barrier)
             const int mh = m0 + h * HR;
-            for (int e = tid; e < HR * TBN; e += NT) {
-                const int row = e / TBN, col = e - row * TBN;
-                const int m = mh + row, n = n0 + col;
-                if (m >= M || n >= N) continue;
-                bool neg;
-                const int64_t a = result_offset(oc, caff, m, n, neg);
-                if (a < 0) continue;
-                const T x = xfac * lds[row * LDE + col];
-                atomic_add_g(pc + a, neg ? -x : x);
+            int64_t adr[RPT];
+            bool ng[RPT];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int m = mh + r0 + RSTEP * k;
+                adr[k] = -1;
+                ng[k] = cneg;
+                if (!cok || m >= M) continue;
+                if (oc.table) {
+                    const int32_t t = ldg(oc.first + ((int64_t)m + oc.ld * (int64_t)n));
+                    ng[k] = t < 0;
+                    adr[k] = t != 0 ? (int64_t)(t < 0 ? -t : t) - 1 : -1;
+                } else if (oc.first_affine) {
+                    adr[k] = oc.f0 + oc.f1 * (int64_t)m + cn;
+                } else {
+                    const int32_t t = ldg(oc.first + m);
+                    ng[k] = cneg != (t < 0);
+                    adr[k] = t != 0 ? (int64_t)(t < 0 ? -t : t) - 1 + cn : -1;
+                }
+            }
+            __syncthreads();   // (h = 0: the K loop ended with a barrier)
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (adr[k] < 0) continue;
+                const T x = xfac * lds[(r0 + RSTEP * k) * LDE + col];
+                atomic_add_g(pc + adr[k], ng[k] ? -x : x);
             }
         }
     }
