@@ -443,12 +443,72 @@ __device__ __forceinline__ int64_t result_offset(const DevOperand &oc, bool caff
 // flight during the current step's MFMAs; no LDS traffic and no barrier per
 // step.  The four partial tiles are summed in LDS (1024 elements) at the end.
 // A step covers 16 * SKINNY_U consecutive k: 4 * SKINNY_U per wave.
-constexpr int SKINNY_U = 4;
+constexpr int SKINNY_U = 8;
+
+// A 1 x 1 result (a full contraction, e.g. the energy): a dot product along
+// the task's range, every thread on its own k (DOT_U loads in flight per
+// operand), then summed over the workgroup.  The MFMA layout would keep 60
+// of every 64 lanes idle here.
+constexpr int DOT_U = 8;
+static_assert(NT == 256, "dot_task sums four waves");
+
+template <typename T>
+__device__ __forceinline__ void dot_task(const DevTask &task, const DevBlock *__restrict__ blocks,
+                                         T *const *__restrict__ bufs, T *lds, int tid) {
+    const DevBlock &b = blocks[task.blk];
+    T s = (T)0;
+    int blk = task.blk, kb = 0, kl = task.k0;
+    for (;;) {
+        const DevBlock &cb = blocks[blk];
+        const int kh = min(cb.K, task.k1 - kb);
+        const DevOperand &oa = cb.A, &ob = cb.B;
+        const T *__restrict__ qa = bufs[cb.buf_a] + cb.base_a + oa.f0 + oa.s0;
+        const T *__restrict__ qb = bufs[cb.buf_b] + cb.base_b + ob.f0 + ob.s0;
+        const int64_t fa = oa.f1, fb = ob.f1;
+        for (int k0 = kl + tid; k0 < kh; k0 += NT * DOT_U) {
+            T a[DOT_U], bv[DOT_U];
+#pragma unroll
+            for (int u = 0; u < DOT_U; ++u) {
+                const int k = k0 + u * NT;
+                a[u] = k < kh ? ldg(qa + fa * (int64_t)k) : (T)0;
+                bv[u] = k < kh ? ldg(qb + fb * (int64_t)k) : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < DOT_U; ++u) s = fma(a[u], bv[u], s);
+        }
+        kb += cb.K;
+        blk = cb.next;
+        if (blk < 0 || kb >= task.k1) break;
+        kl = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __syncthreads();   // LDS of the previous task is no longer read
+    if ((tid & 63) == 0) lds[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        const T sum = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+        const DevOperand &oc = b.C;
+        const bool caff = !oc.table && oc.first_affine && oc.second_affine;
+        bool neg;
+        const int64_t a = result_offset(oc, caff, task.m0, task.n0, neg);
+        if (a >= 0) {
+            T *pc = bufs[b.buf_c] + b.base_c;
+            const T x = (T)b.xfac * sum;
+            const T v = neg ? -x : x;
+            if (b.atomic) atomic_add_g(pc + a, v);
+            else stg(pc + a, ldg(pc + a) + v);
+        }
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock *__restrict__ blocks,
                                             T *const *__restrict__ bufs, T *lds, int tid) {
     typedef typename Acc<T>::type AccT;
+    if (blocks[task.blk].M == 1 && blocks[task.blk].N == 1) {
+        dot_task(task, blocks, bufs, lds, tid);
+        return;
+    }
     const int lane = tid & 63, wave = tid >> 6;
     const int km = lane >> 4, xl = lane & 15;
     const DevBlock &b = blocks[task.blk];
@@ -624,29 +684,35 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
     }
     // results: into the LDS stage (span > 0), else non-returning atomic adds
     // (with a single writer the L2 performs the same IEEE addition as a
-    // read-modify-write)
+    // read-modify-write).  The result-map entries of all the thread's rows
+    // are loaded before the first add, so their latencies overlap.
+    int64_t adr[8];
+    bool ng8[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int m = m0 + ml + ng * i;
+        adr[i] = -1;
+        ng8[i] = cneg;
         if (!(i < rpt && cok && m < M)) continue;
-        int64_t a;
-        bool neg = cneg;
         if (oc.table) {
             const int32_t v = ldg(oc.first + ((int64_t)n + oc.ld * (int64_t)m));
-            neg = v < 0;
-            a = (int64_t)(v < 0 ? -v : v) - 1;
+            ng8[i] = v < 0;
+            adr[i] = (int64_t)(v < 0 ? -v : v) - 1;
         } else if (oc.first_affine) {
-            a = oc.f0 + oc.f1 * (int64_t)m + cadd;
+            adr[i] = oc.f0 + oc.f1 * (int64_t)m + cadd;
         } else {
             const int32_t v = ldg(oc.first + m);
-            neg = neg != (v < 0);
-            a = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
+            ng8[i] = cneg != (v < 0);
+            adr[i] = v == 0 ? -1 : (int64_t)(v < 0 ? -v : v) - 1 + cadd;
         }
-        if (a < 0) continue;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (adr[i] < 0) continue;
         const T x = (T)b.xfac * acc[i];
-        const T v = neg ? -x : x;
-        if (span > 0) atomicAdd(Cs + (a - task.lo), v);
-        else atomic_add_g(pc + a, v);
+        const T v = ng8[i] ? -x : x;
+        if (span > 0) atomicAdd(Cs + (adr[i] - task.lo), v);
+        else atomic_add_g(pc + adr[i], v);
     }
     if (span > 0) {
         __syncthreads();
