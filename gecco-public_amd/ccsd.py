@@ -74,9 +74,7 @@ _TERM_LEVELS = [
      ('Fme', 'Hoovv', 'T1', {H: 1}, {P: 1}, 1.0),
      ('Z', 'Hoovv', 'T1', {}, {P: 1}, -1.0),
      ('Fae', 'T1', 'Hovvv', {P: 1}, {H: 1}, -1.0),
-     ('F2ae', 'T1', 'Hovvv', {P: 1}, {H: 1}, -1.0),
      ('Fmi', 'T1', 'Hooov', {P: 1}, {H: 1}, 1.0),
-     ('F2mi', 'T1', 'Hooov', {P: 1}, {H: 1}, 1.0),
      ('Wmnij', 'Hooov', 'T1', {}, {P: 1}, 1.0),
      ('Wmbej', 'Hovvv', 'T1', {}, {P: 1}, 1.0),
      ('Wmbej', 'T1', 'Hooov', {}, {H: 1}, 1.0),
@@ -88,9 +86,7 @@ _TERM_LEVELS = [
      ('R2', 'Hvvvo', 'T1', {}, {P: 1}, 1.0),
      ('R2', 'T1', 'Hovoo', {}, {H: 1}, 1.0)],
     [('Fae', 'taut', 'Hoovv', {P: 1}, {H: 2}, -1.0),
-     ('F2ae', 'taut', 'Hoovv', {P: 1}, {H: 2}, -1.0),
      ('Fmi', 'Hoovv', 'taut', {H: 1}, {P: 2}, 1.0),
-     ('F2mi', 'Hoovv', 'taut', {H: 1}, {P: 2}, 1.0),
      ('F2ae', 'T1', 'Fme', {}, {H: 1}, -0.5),
      ('F2mi', 'Fme', 'T1', {}, {P: 1}, 0.5),
      ('Wmnij', 'Hoovv', 'tau', {}, {P: 2}, 1.0),
@@ -99,7 +95,9 @@ _TERM_LEVELS = [
      ('Wmbej', 'T1', 'Z', {}, {H: 1}, 1.0),
      ('R1', 'T2', 'Fme', {P: 1}, {H: 1}, 1.0),
      ('E', 'Hoovv', 'tau', {H: 2}, {P: 2}, 1.0)],
-    [('R1', 'Fae', 'T1', {}, {P: 1}, 1.0),
+    [('ADD', 'F2ae', 'Fae', 1.0),
+     ('ADD', 'F2mi', 'Fmi', 1.0),
+     ('R1', 'Fae', 'T1', {}, {P: 1}, 1.0),
      ('R1', 'T1', 'Fmi', {}, {H: 1}, -1.0),
      ('R2', 'F2ae', 'T2', {}, {P: 1}, 1.0),
      ('R2', 'T2', 'F2mi', {}, {H: 1}, -1.0),
@@ -110,7 +108,11 @@ _TERM_LEVELS = [
 
 
 
-TERMS = [t for level in _TERM_LEVELS for t in level]
+TERMS = [t for level in _TERM_LEVELS for t in level if t[0] != 'ADD']
+# ADD items (command_add_intm): F2ae = Fae + (-1/2 t_m^a F_me) and F2mi =
+# Fmi + 1/2 t_i^e F_me share Fae's and Fmi's T1 and tau~ terms, so those
+# are contracted once (into Fae, Fmi) and added to F2ae, F2mi in the
+# initialisation launch of the level that reads them.
 
 
 def record_contraction(p, c, lists, slot, use_tr):
@@ -143,7 +145,8 @@ def _formula_items():
             items.append(_formula.init(n, src, f))
         elif n not in INTEGRALS and n not in ('T1', 'T2'):
             items.append(_formula.init(n))
-    items += [_formula.contr(*t) for t in TERMS]
+    items += [_formula.add(*t[1:]) if t[0] == 'ADD' else _formula.contr(*t)
+              for level in _TERM_LEVELS for t in level]
     return items
 
 

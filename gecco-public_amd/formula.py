@@ -216,9 +216,11 @@ def schedule(items):
     the projections last; BC items count as INIT then CONTR, REORDER items
     as INIT then ADD_REO, NEW and DELETE do no work).
 
-    A contraction reads its operands as they are at the start of its level,
-    so with items taken in list order:
-      read after write   an operand's earlier writes are in earlier levels;
+    A contraction reads its operands as they are after its level's
+    initialisation launch, so with items taken in list order:
+      read after write   an operand's earlier writes by contractions (and
+                         reorders) are in earlier levels, its INITs and
+                         ADDs in the same or earlier levels;
       write after read   the target's earlier readers are in earlier
                          levels (the writer adds into the target while the
                          readers of its own level run);
@@ -245,6 +247,7 @@ def _place(items):
     where it is first needed (lifetimes)."""
     items = expand(items)
     last_write, last_read, last_init = {}, {}, {}     # list -> level
+    last_late = {}        # list -> level of its last write by a contraction or reorder
     inits, contrs, adds, reos = {}, {}, {}, {}
     where = [None] * len(items)
     pending = {}          # list -> (item index, earliest level) of a deferred INIT
@@ -298,13 +301,17 @@ def _place(items):
         ops = (it['source'],) if it['cmd'] == 'ADD_REO' else (it['op1'], it['op2'])
         for op in ops:
             flush(op)
-        lv = max([last_write.get(op, -1) + 1 for op in ops] +
+        # an operand's writes by contractions and reorders lie in earlier
+        # levels; its INIT and ADD writes may share the level (the
+        # initialisation launch runs before the contractions)
+        lv = max([max(last_late.get(op, -1) + 1, last_write.get(op, -1)) for op in ops] +
                  [last_read.get(t, -1) + 1, last_init.get(t, 0)])
         if t in pending:
             put_init(t, max(pending[t][1], lv))
             lv = max(lv, last_init[t])
         (reos if it['cmd'] == 'ADD_REO' else contrs).setdefault(lv, []).append(it)
         last_write[t] = max(last_write.get(t, -1), lv)
+        last_late[t] = max(last_late.get(t, -1), lv)
         for op in ops:
             last_read[op] = max(last_read.get(op, -1), lv)
         where[k] = lv

@@ -38,7 +38,9 @@ def evaluate_with_oracle(L, X, use_tr=True):
             X[n] = f * X[src]
         elif n not in ccsd.INTEGRALS and n not in ('T1', 'T2'):
             X[n] = np.zeros(L[n].length)
-    for level in ccsd.LEVELS:
+    for lv, level in zip(ccsd.SCHEDULE, ccsd.LEVELS):
+        for it in lv['add']:            # ADD items run before the contractions
+            X[it['target']] = X[it['target']] + it['factor'] * X[it['source']]
         written = []
         for res, a, b, cc, ca, fact in level:
             Contraction(L[a], L[b], L[res], cc, ca, fact).run(X[a], X[b], X[res], use_tr=use_tr)
