@@ -30,6 +30,24 @@ module gecco_hip_f
       integer(c_int64_t), intent(out) :: ierr
     end subroutine
 
+    ! the same for operators with several vertices: merge maps, EX/CNT
+    ! occupations and the reorder after the kernel calls (nreo 0 or 1)
+    subroutine gecco_hip_contr_op1op2_n(xfac, update, njoined_op1, iocc_op1, njoined_op2, &
+         iocc_op2, njoined_op1op2, iocc_op1op2, iocc_op1op2tmp, njoined_cnt, iocc_ex1, &
+         iocc_ex2, iocc_cnt, merge_op1, merge_op2, merge_op1op2, merge_op2op1, nreo, &
+         from_vtx, to_vtx, iocc_shift, sign_reo, use_tr, nsym, norb_h, igam_h, norb_p, &
+         igam_p, xop1, xop2, xop1op2, ierr) bind(C, name='gecco_hip_contr_op1op2_n_')
+      import :: c_double, c_int64_t
+      real(c_double), intent(in) :: xfac, xop1(*), xop2(*)
+      real(c_double), intent(inout) :: xop1op2(*)
+      integer(c_int64_t), intent(in) :: update, njoined_op1, iocc_op1(4, 2, *), njoined_op2, &
+           iocc_op2(4, 2, *), njoined_op1op2, iocc_op1op2(4, 2, *), iocc_op1op2tmp(4, 2, *), &
+           njoined_cnt, iocc_ex1(4, 2, *), iocc_ex2(4, 2, *), iocc_cnt(4, 2, *), &
+           merge_op1(*), merge_op2(*), merge_op1op2(*), merge_op2op1(*), nreo, from_vtx, &
+           to_vtx, iocc_shift(4, 2), sign_reo, use_tr, nsym, norb_h, igam_h(*), norb_p, igam_p(*)
+      integer(c_int64_t), intent(out) :: ierr
+    end subroutine
+
     ! reorder of a whole list with reo_op_wmaps_c's semantics, flattened
     ! (include/gecco_hip_model.h); vertices numbered from 1
     subroutine gecco_hip_reo_op(fac, update, sign_reo, njoined_ori, iocc_ori, &

@@ -215,6 +215,37 @@ def contr_op1op2(xfac, op1, op2, op1op2, cnt, xop1, xop2, xop1op2, irreps, updat
         raise GeccoHipError(f'gecco_hip_contr_op1op2_: error {err.value}: {last_error()[1]}')
 
 
+def contr_op1op2_n(xfac, op1, op2, op1op2, ex1, ex2, cnt, merge, xop1, xop2, xop1op2, irreps,
+                   update=True, use_tr=True, op1op2tmp=None, reo=None):
+    """A contraction of operators with several vertices through
+    gecco_hip_contr_op1op2_n_: op1/op2/op1op2 per-vertex (C, A) occupation
+    lists (lists of dicts), ex1/ex2/cnt per-vertex (C, A) dict pairs, merge
+    the four short-form merge maps; with reo = (from_vtx, to_vtx, (shift C,
+    shift A), sign_reo) (vertices from 0 here) the kernel calls write
+    op1op2tmp, which is reordered into op1op2."""
+    nsym, ih, ip = irreps
+    ih = np.ascontiguousarray(ih, dtype=np.int64)
+    ip = np.ascontiguousarray(ip, dtype=np.int64)
+    vo = lambda o: np.concatenate([_iocc(c, a) for c, a in zip(*o)])
+    tmp = op1op2 if op1op2tmp is None else op1op2tmp
+    arrs = [vo(op1), vo(op2), vo(op1op2), vo(tmp), _iocc_n(ex1), _iocc_n(ex2), _iocc_n(cnt)]
+    mm = [np.asarray(list(m) or [0], dtype=np.int64) for m in merge]
+    frm, to, shift, sgn = reo if reo is not None else (0, 0, ({}, {}), 1)
+    sh = _iocc(*shift)
+    f = _lib.gecco_hip_contr_op1op2_n_
+    f.restype = None
+    ref = lambda v: ctypes.byref(I64(int(v)))
+    err = I64()
+    P = lambda a: a.ctypes.data_as(PI64)
+    f(ctypes.byref(ctypes.c_double(float(xfac))), ref(bool(update)), ref(len(op1[0])), P(arrs[0]),
+      ref(len(op2[0])), P(arrs[1]), ref(len(op1op2[0])), P(arrs[2]), P(arrs[3]), ref(len(cnt)),
+      P(arrs[4]), P(arrs[5]), P(arrs[6]), *[P(m) for m in mm], ref(reo is not None), ref(frm + 1),
+      ref(to + 1), P(sh), ref(sgn), ref(bool(use_tr)), ref(nsym), ref(len(ih)), P(ih), ref(len(ip)),
+      P(ip), _ptr(xop1), _ptr(xop2), _ptr(xop1op2), ctypes.byref(err))
+    if err.value:
+        raise GeccoHipError(f'gecco_hip_contr_op1op2_n_: error {err.value}: {last_error()[1]}')
+
+
 def reo_op(fac, ori, reo, frm, to, shift, xori, xreo, irreps, update=True, sign_reo=1):
     """Reorder of a whole list with reo_op_wmaps_c's semantics through
     gecco_hip_reo_op_: ori/reo = (C occupations, A occupations) per vertex

@@ -218,6 +218,33 @@ void gecco_hip_contr_op1op2_(const double *xfac, const int64_t *update, const in
                              const int64_t *igam_p, const double *xop1, const double *xop2,
                              double *xop1op2, int64_t *ierr);
 
+/* The same entry for operators with several vertices, with the rest of
+ * contr_op1op2's argument list (contract/contr_op1op2.f:2-17): njoined_op1,
+ * njoined_op2, njoined_op1op2, njoined_cnt; iocc_op1 (4, 2, njoined_op1)
+ * and the like, vertex-wise; iocc_ex1, iocc_ex2, iocc_cnt and the four
+ * short-form merge maps merge_op1, merge_op2, merge_op1op2, merge_op2op1
+ * (as gecco_hip_plan_add_contraction_n takes them); iocc_op1op2tmp, the
+ * occupation the kernel calls write (me_op1op2tmp), and the reorder after
+ * the kernel calls that reo_info describes (contract/contr_op1op2_wmaps_c.f:
+ * 1962-1995): nreo = 0 (op1op2tmp is op1op2) or 1 -- the shift string
+ * iocc_shift(4, 2) moves from vertex from_vtx of op1op2tmp to vertex
+ * to_vtx of op1op2 (numbered from 1) with the sign sign_reo; the result of
+ * the kernel calls then lives in a device list of the call's own and is
+ * reordered into the target.  Stored lists only (no dagger flags); the
+ * other arguments as gecco_hip_contr_op1op2_. */
+void gecco_hip_contr_op1op2_n_(const double *xfac, const int64_t *update, const int64_t *njoined_op1,
+                               const int64_t *iocc_op1, const int64_t *njoined_op2, const int64_t *iocc_op2,
+                               const int64_t *njoined_op1op2, const int64_t *iocc_op1op2,
+                               const int64_t *iocc_op1op2tmp, const int64_t *njoined_cnt,
+                               const int64_t *iocc_ex1, const int64_t *iocc_ex2, const int64_t *iocc_cnt,
+                               const int64_t *merge_op1, const int64_t *merge_op2,
+                               const int64_t *merge_op1op2, const int64_t *merge_op2op1, const int64_t *nreo,
+                               const int64_t *from_vtx, const int64_t *to_vtx, const int64_t *iocc_shift,
+                               const int64_t *sign_reo, const int64_t *use_tr, const int64_t *nsym,
+                               const int64_t *norb_h, const int64_t *igam_h, const int64_t *norb_p,
+                               const int64_t *igam_p, const double *xop1, const double *xop2,
+                               double *xop1op2, int64_t *ierr);
+
 /* Reorder of a whole list with the semantics of reo_op_wmaps_c
  * (contract/reo_op_wmaps_c.f:2-9, called from contract/fs_reo_drv.f:166),
  * its derived-type arguments flattened as for gecco_hip_contr_op1op2_:

@@ -872,3 +872,53 @@ def test_joined_product_reordered_in_one_plan(nsym, nocc, nvir):
     plan.execute(bufs)      # the initialisation restarts X; the reorder accumulates
     torch.cuda.synchronize()
     assert _relerr(bufs[3].cpu().numpy(), 2 * ref) <= TOL
+
+
+@pytest.mark.parametrize('name', ['t1t1_joined', 'vertex_fock', 'pairwise_scalar', 'ring_one_vertex',
+                                  'result_merge'])
+def test_flattened_joined_entry_matches_oracle(name):
+    """gecco_hip_contr_op1op2_n_ (contr_op1op2's multi-vertex argument list,
+    flattened) against the oracle, with host and device buffers, update on
+    and off, and with the reorder after the kernel calls (reo_info): T1 x T1
+    into the two-vertex op1op2tmp, reordered into the one-vertex R."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin, JOINED_CASES, t1t1
+    nocc, nvir, nsym = 4, 6, 2
+    print(f'flattened joined {name}', flush=True)
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = JOINED_CASES[name](orbs)
+    irreps = (nsym, round_robin(nocc, nsym), round_robin(nvir, nsym, 1))
+    x1, x2, x0 = spin_symmetric(a, 71), spin_symmetric(b, 72), spin_symmetric(r, 73)
+    occ = lambda m: (m.vocc_c, m.vocc_a)
+    for use_tr in (True, False):
+        ref = x0.copy()
+        c.run(x1, x2, ref, route=3, use_tr=use_tr, screen=False)
+        y = x0.copy()
+        gecco_hip.contr_op1op2_n(1.0, occ(a), occ(b), occ(r), c.ex1_vocc, c.ex2_vocc, c.cnt_vocc, c.merge,
+                                 x1, x2, y, irreps, use_tr=use_tr)
+        assert _relerr(y, ref) <= TOL
+        y = x0.copy()
+        gecco_hip.contr_op1op2_n(1.0, occ(a), occ(b), occ(r), c.ex1_vocc, c.ex2_vocc, c.cnt_vocc, c.merge,
+                                 x1, x2, y, irreps, update=False, use_tr=use_tr)
+        assert _relerr(y, ref - x0) <= TOL
+    dev = torch.device('cuda:0')
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=True, screen=False)
+    dy = torch.from_numpy(x0).to(dev)
+    gecco_hip.contr_op1op2_n(1.0, occ(a), occ(b), occ(r), c.ex1_vocc, c.ex2_vocc, c.cnt_vocc, c.merge,
+                             torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev), dy, irreps)
+    torch.cuda.synchronize()
+    assert _relerr(dy.cpu().numpy(), ref) <= TOL
+    if name == 't1t1_joined':          # with the reorder into the one-vertex product
+        c1, T1o, _, Ro = t1t1(orbs)
+        xt = spin_symmetric(T1o, 74)
+        r0 = spin_symmetric(Ro, 75)
+        ref = r0.copy()
+        c1.run(xt, xt, ref, route=3, use_tr=False, screen=False)
+        y = r0.copy()
+        P, H = 2, 1
+        gecco_hip.contr_op1op2_n(1.0, occ(T1o), occ(T1o), ([{P: 2}, {}], [{H: 2}, {}]), c.ex1_vocc,
+                                 c.ex2_vocc, c.cnt_vocc, c.merge, xt, xt, y, irreps, use_tr=False,
+                                 op1op2tmp=occ(r), reo=(1, 0, ({P: 1}, {H: 1}), 1))
+        assert _relerr(y, ref) <= TOL
