@@ -62,7 +62,7 @@ METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
 # profiles/); for the CCSD iteration it is the average over the three
 # formula-level launches of the strided-GEMM kernel
 MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 396.5e6,
-                    ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine<64, 64, 16, double, 1, 2>'): 651.6e6}
+                    ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine<64, 64, 16, double, 1, 2, false>'): 651.6e6}
 GOLDEN = os.path.join(ROOT, 'tests', 'golden', 'ccsd_o20_v100_s4.json')
 
 
@@ -158,7 +158,7 @@ def run_ladder(args):
         'roofline': {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP64,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP64, 4),
                      'traffic': MEASURED_TRAFFIC.get(('ladder', args.nocc, args.nvir, args.nsym)),
-                     'kernel': 'grouped_contract_affine<64, 64, 16, double, 1, 2>', 'kernel_ms': round(kern_ms, 5)},
+                     'kernel': 'grouped_contract_affine<64, 64, 16, double, 1, 2, false>', 'kernel_ms': round(kern_ms, 5)},
     }
     if not args.no_cpu:
         v, dt, sample = cpu_ladder(args.nocc, args.nvir, args.nsym)
@@ -201,7 +201,7 @@ def kernel_times(plans, run, steps):
     _, dom, dom_fl = max(cand)
     dom_ms = ms[dom] / max(1, nl[dom])
     achieved = dom_fl / (ms[dom] / steps * 1e-3) / 1e12 if ms[dom] > 0 else 0.0
-    return dict(ms=ms, launches=nl, names=['grouped_contract_affine<64, 64, 16, double, 1, 2>', 'grouped_contract<double>'],
+    return dict(ms=ms, launches=nl, names=['grouped_contract_affine<64, 64, 16, double, 1, 2, false>', 'grouped_contract<double>'],
                 dom=dom, dom_ms=dom_ms, achieved=achieved, gather_bytes=gbytes)
 
 
@@ -319,7 +319,7 @@ def run_ccsd(args):
                    'kernel_calls_per_iter': solver.nblocks,
                    'gflop_per_iter': round(solver.flops / 1e9, 3)},
         'contraction_tflops': round(solver.flops * done / wall / 1e12, 3),
-        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double, 1, 2>': round(ms[0] / args.steps, 4),
+        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double, 1, 2, false>': round(ms[0] / args.steps, 4),
                                 'grouped_contract<double>': round(ms[1] / args.steps, 4),
                                 'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
                                 'gather_operands<double>': round(ms[3] / args.steps, 4)},
@@ -511,7 +511,7 @@ def run_triples(args):
                              'V(vv,vo).T2', 'V(ov,oo).T2'],
                    'kernel_calls': solver.nblocks, 'gflop_per_step': round(solver.flops / 1e9, 3),
                    't3_elements': solver.L['T3'].length},
-        'kernels_ms_per_step': {'grouped_contract_affine<64, 64, 16, double, 1, 2>': round(ms[0] / args.steps, 4),
+        'kernels_ms_per_step': {'grouped_contract_affine<64, 64, 16, double, 1, 2, false>': round(ms[0] / args.steps, 4),
                                 'grouped_contract<double>': round(ms[1] / args.steps, 4),
                                 'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
                                 'gather_operands<double>': round(ms[3] / args.steps, 4)},
@@ -713,7 +713,7 @@ def run_batched(args):
                    'gflop_per_iter': round(b.flops / 1e9, 3)},
         'instance_iters_per_s': round(args.instances * args.steps / wall, 2),
         'contraction_tflops': round(b.flops * args.steps / wall / 1e12, 3),
-        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double, 1, 2>': round(ms[0] / args.steps, 4),
+        'kernels_ms_per_iter': {'grouped_contract_affine<64, 64, 16, double, 1, 2, false>': round(ms[0] / args.steps, 4),
                                 'grouped_contract<double>': round(ms[1] / args.steps, 4),
                                 'sym_ab_batched<double>': round(ms[2] / args.steps, 4),
                                 'gather_operands<double>': round(ms[3] / args.steps, 4)},

@@ -177,13 +177,16 @@ class CCSD:
     spin_project=True projects the residual lists onto their singlet part
     after their last write (gecco_hip_plan_add_spin_project: the reference's
     spin_prj_list for a target with s2 = 0, contract/frm_sched2.f:146-150).
+    deterministic=True builds every plan with fixed-order summation
+    (gecco_hip_plan_set_deterministic): solves are bit-reproducible.
     """
 
     def __init__(self, ham, device='cuda:0', use_tr=True, formula=None, plans=True, pool=False,
-                 spin_project=False):
+                 spin_project=False, deterministic=False):
         import torch
         import gecco_hip
         self.torch = torch
+        self.deterministic = deterministic
         self.ham = ham
         self.dev = torch.device(device)
         self.use_tr = use_tr
@@ -315,6 +318,8 @@ class CCSD:
         """Record one scheduled level of this instance into plan p, its
         lists in buffer slots base + list index: constant slots, list
         initialisations, contractions and spin-flip projections."""
+        if getattr(self, 'deterministic', False):   # fixed-order sums, bit-reproducible
+            p.set_deterministic(True)
         for n in self.constant:       # fixed during the solve
             p.set_constant(base + self.slot[n])
         # list initialisations of the level: from a source or zero
@@ -479,6 +484,8 @@ class CCSD:
         for lv in _formula.schedule(items):
             p = gecco_hip.Plan()
             p.set_precision(bulk)
+            if self.deterministic:
+                p.set_deterministic(True)
             for n in self.constant:
                 p.set_constant(slot[n])
             for it in lv['init']:

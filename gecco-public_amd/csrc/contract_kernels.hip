@@ -454,7 +454,7 @@ static_assert(NT == 256, "dot_task sums four waves");
 
 template <typename T>
 __device__ __forceinline__ void dot_task(const DevTask &task, const DevBlock *__restrict__ blocks,
-                                         T *const *__restrict__ bufs, T *lds, int tid) {
+                                         T *const *__restrict__ bufs, T *lds, int tid, T *raw) {
     const DevBlock &b = blocks[task.blk];
     T s = (T)0;
     int blk = task.blk, kb = 0, kl = task.k0;
@@ -485,7 +485,9 @@ __device__ __forceinline__ void dot_task(const DevTask &task, const DevBlock *__
     __syncthreads();   // LDS of the previous task is no longer read
     if ((tid & 63) == 0) lds[tid >> 6] = s;
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0 && raw) {
+        stg(raw + task.lo, (T)b.xfac * ((lds[0] + lds[1]) + (lds[2] + lds[3])));
+    } else if (tid == 0) {
         const T sum = (lds[0] + lds[1]) + (lds[2] + lds[3]);
         const DevOperand &oc = b.C;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
@@ -503,10 +505,10 @@ __device__ __forceinline__ void dot_task(const DevTask &task, const DevBlock *__
 
 template <typename T>
 __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock *__restrict__ blocks,
-                                            T *const *__restrict__ bufs, T *lds, int tid) {
+                                            T *const *__restrict__ bufs, T *lds, int tid, T *raw) {
     typedef typename Acc<T>::type AccT;
     if (blocks[task.blk].M == 1 && blocks[task.blk].N == 1) {
-        dot_task(task, blocks, bufs, lds, tid);
+        dot_task(task, blocks, bufs, lds, tid, raw);
         return;
     }
     const int lane = tid & 63, wave = tid >> 6;
@@ -564,7 +566,9 @@ __device__ __forceinline__ void skinny_task(const DevTask &task, const DevBlock 
     // m = m0 + mrow(r, lane / 16), n = n0 + lane % 16
     const T sum = lds[tid] + lds[256 + tid] + lds[512 + tid] + lds[768 + tid];
     const int mm = task.m0 + mrow<T>(tid >> 6, lane >> 4), nn = task.n0 + (lane & 15);
-    if (mm < M && nn < N) {
+    if (raw) {        // fixed-order mode: the unsigned tile into the task's range
+        if (mm < M && nn < N) stg(raw + task.lo + (mm - task.m0) * 16 + (nn - task.n0), (T)b.xfac * sum);
+    } else if (mm < M && nn < N) {
         const DevOperand &oc = b.C;
         const bool caff = !oc.table && oc.first_affine && oc.second_affine;
         bool neg;
@@ -595,7 +599,7 @@ constexpr int DIRECT_STAGE = 4096;       // LDS elements for a staged result ran
 
 template <typename T>
 __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock *__restrict__ blocks,
-                                            T *const *__restrict__ bufs, T *lds, int tid) {
+                                            T *const *__restrict__ bufs, T *lds, int tid, T *raw) {
     T *const As = lds;
     T *const Cs = lds + DIRECT_A;
     const DevBlock &b = blocks[task.blk];
@@ -682,6 +686,15 @@ __device__ __forceinline__ void direct_task(const DevTask &task, const DevBlock 
             }
         }
     }
+    if (raw) {        // fixed-order mode: rows of dn columns in the task's range
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int m = m0 + ml + ng * i;
+            if (i < rpt && active && n < N && m < M)
+                stg(raw + task.lo + (int64_t)(m - m0) * dn + nl, (T)b.xfac * acc[i]);
+        }
+        return;
+    }
     // results: into the LDS stage (span > 0), else non-returning atomic adds
     // (with a single writer the L2 performs the same IEEE addition as a
     // read-modify-write).  The result-map entries of all the thread's rows
@@ -762,10 +775,14 @@ constexpr int AFFINE_WAVES = TBM * TBN >= 128 * 128 ? 2
                            : !(TBM == 64 && TBN == 64 && TBK == 16 && PF == 1) ? 1
                            : sizeof(T) == 4 ? 6 : 4;
 
-template <int TBM, int TBN, int TBK, typename T, int PF = 1, int WM = 2>
+// RAW: fixed-order mode (tiles into raw instead of atomic adds), its own
+// instantiation so that the other kernels keep their registers.
+template <int TBM, int TBN, int TBK, typename T, int PF = 1, int WM = 2, bool RAW = false>
 __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void grouped_contract_affine(
     const DevBlock *__restrict__ blocks, const DevTask *__restrict__ tasks, int ntasks,
-    T *const *__restrict__ bufs, const int32_t *__restrict__ skip, const int32_t *__restrict__ zero) {
+    T *const *__restrict__ bufs, const int32_t *__restrict__ skip, const int32_t *__restrict__ zero,
+    T *__restrict__ raw_arg) {
+    T *const raw = RAW ? raw_arg : nullptr;
     typedef typename Acc<T>::type AccT;
     constexpr int LDA = TBM + 1, LDB = TBN + 1;
     constexpr int WN = 4 / WM, RM = TBM / WM, RN = TBN / WN;   // rows and columns per wave
@@ -789,11 +806,11 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
         const DevTask task = tasks[t];
         if (skipped(skip, task.grp) || screened(zero, blocks[task.blk])) continue;
         if (task.pad == 1) {   // tile of at most 16 x 16
-            skinny_task(task, blocks, bufs, lds, tid);
+            skinny_task(task, blocks, bufs, lds, tid, raw);
             continue;
         }
         if (task.pad > 1) {    // short contraction on the vector ALU
-            direct_task(task, blocks, bufs, lds, tid);
+            direct_task(task, blocks, bufs, lds, tid, raw);
             continue;
         }
         const DevBlock &b = blocks[task.blk];
@@ -1077,6 +1094,32 @@ __global__ __launch_bounds__(NT, (AFFINE_WAVES<TBM, TBN, TBK, T, PF, WM>)) void 
         static_assert(NT % TBN == 0 && HR % RSTEP == 0, "epilogue thread layout");
         const int col = tid % TBN, r0 = tid / TBN;
         const int n = n0 + col;
+        if (raw) {
+            // fixed-order mode: the tile goes unsigned into the task's own
+            // range of raw (row-major, TBN columns); a later pass sums the
+            // tiles of every result element in a fixed order
+            for (int h = 0; h < 2; ++h) {
+                if (h) __syncthreads();
+                if ((wm * RM) / HR == h) {
+#pragma unroll
+                    for (int i = 0; i < TM; ++i)
+#pragma unroll
+                        for (int j = 0; j < TN; ++j)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                lds[(wm * RM - h * HR + i * 16 + mrow<T>(r, lrow)) * LDE + wn * RN + j * 16 + lcol] =
+                                    acc[i][j][r];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const int row = r0 + RSTEP * k, m = m0 + h * HR + row;
+                    if (m < M && n < N)
+                        stg(raw + task.lo + (int64_t)(h * HR + row) * TBN + col, xfac * lds[row * LDE + col]);
+                }
+            }
+            continue;
+        }
         bool cok = n < N, cneg = false;
         int64_t cn = 0;
         if (cok && !oc.table) {
@@ -1352,24 +1395,35 @@ extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk) {
 template <typename T>
 static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *tasks, int ntasks,
                          T *const *bufs, int grid, int variant, const int32_t *skip,
-                         const int32_t *zero, hipStream_t stream) {
+                         const int32_t *zero, T *raw, hipStream_t stream) {
     using namespace gecco;
+    if (raw && variant != 1 && variant != 9) return 1;   // fixed-order mode: tile variants 1 and 9
     switch (variant) {
-    case 0: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 1: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 2: hipLaunchKernelGGL((grouped_contract_affine<64, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 0: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
+    case 1:
+        if (raw)
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 2, true>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw);
+        else
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw);
+        break;
+    case 2: hipLaunchKernelGGL((grouped_contract_affine<64, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
+    case 3: hipLaunchKernelGGL((grouped_contract_affine<128, 64, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
+    case 4: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 16, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
+    case 5: hipLaunchKernelGGL((grouped_contract_affine<128, 128, 8, T>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
     // two K steps of loads in flight
-    case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
-    case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 6: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
+    case 7: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 32, T, 2>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw); break;
     // 64 x 64 x 16 with the waves stacked along M
-    case 9: hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero); break;
+    case 9:
+        if (raw)
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4, true>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw);
+        else
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 1, 4>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw);
+        break;
     // split-bfloat16 GEMM tiles (float lists only)
     case 8:
         if constexpr (sizeof(T) == 4) {
-            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 3>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero);
+            hipLaunchKernelGGL((grouped_contract_affine<64, 64, 16, T, 3>), dim3(grid), dim3(NT), 0, stream, blocks, tasks, ntasks, bufs, skip, zero, raw);
             break;
         } else {
             return 1;
@@ -1382,11 +1436,12 @@ static int launch_affine(const gecco::DevBlock *blocks, const gecco::DevTask *ta
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            void *const *bufs, int grid, int variant,
-                                           const int32_t *skip, const int32_t *zero, int elem,
+                                           const int32_t *skip, const int32_t *zero, void *raw, int elem,
                                            hipStream_t stream) {
     if (ntasks <= 0) return 0;
-    if (elem == 4) return launch_affine(blocks, tasks, ntasks, tbl<float>(bufs), grid, variant, skip, zero, stream);
-    return launch_affine(blocks, tasks, ntasks, tbl<double>(bufs), grid, variant, skip, zero, stream);
+    if (elem == 4)
+        return launch_affine(blocks, tasks, ntasks, tbl<float>(bufs), grid, variant, skip, zero, (float *)raw, stream);
+    return launch_affine(blocks, tasks, ntasks, tbl<double>(bufs), grid, variant, skip, zero, (double *)raw, stream);
 }
 
 

@@ -40,7 +40,7 @@ extern "C" int gecco_launch_grouped(const gecco::DevBlock *blocks, const gecco::
 extern "C" int gecco_launch_grouped_affine(const gecco::DevBlock *blocks,
                                            const gecco::DevTask *tasks, int ntasks,
                                            void *const *bufs, int grid, int variant,
-                                           const int32_t *skip, const int32_t *zero, int elem,
+                                           const int32_t *skip, const int32_t *zero, void *raw, int elem,
                                            hipStream_t stream);
 extern "C" int gecco_affine_variant_shape(int v, int *bm, int *bn, int *bk);
 extern "C" int gecco_launch_spin_project(void *x, const int64_t *gptr, const int64_t *elem, const int64_t *gmat,
@@ -476,6 +476,14 @@ struct SpinStep {
 struct gecco_hip_plan {
     std::vector<gecco::HostBlock> blocks;
     std::vector<TraceStep> traces;                // after the contractions
+    // fixed-order mode (gecco_hip_plan_set_deterministic): the strided-GEMM
+    // launch writes every task's unsigned tile into d_raw, and the CSR sums
+    // of dtraces (one per result slot, contributions in task order) add them
+    // into the result lists
+    bool deterministic = false;
+    void *d_raw = nullptr;
+    int64_t raw_elems = 0;
+    std::vector<TraceStep> dtraces;
     std::vector<SpinStep> spins;                  // after the spin-flip projections
     std::vector<gecco::DevInit> inits;            // list initialisation, first launch
     std::vector<int32_t> init_group;              // instance group of each init
@@ -715,6 +723,16 @@ static void plan_free_device(gecco_hip_plan *p) {
         tr.d_rowptr = tr.d_rowdst = tr.d_col = nullptr;
         tr.d_sgn = nullptr;
     }
+    for (auto &tr : p->dtraces) {
+        if (tr.d_rowptr) dfree(p, tr.d_rowptr);
+        if (tr.d_rowdst) dfree(p, tr.d_rowdst);
+        if (tr.d_col) dfree(p, tr.d_col);
+        if (tr.d_sgn) dfree(p, tr.d_sgn);
+    }
+    p->dtraces.clear();
+    if (p->d_raw) dfree(p, p->d_raw);
+    p->d_raw = nullptr;
+    p->raw_elems = 0;
     if (p->d_inits) dfree(p, p->d_inits);
     if (p->d_itasks) dfree(p, p->d_itasks);
     p->d_inits = nullptr;
@@ -1180,6 +1198,87 @@ static int plan_finalize(gecco_hip_plan *p) {
         std::copy(ord.begin(), ord.end(), tasks.begin());
     }
     for (auto &t : tasks) t.pad2 = 0;
+    if (p->deterministic) {
+        // every task of the strided-GEMM launch gets its own range of d_raw
+        // (tile rows x tile columns, row-major); each result element's
+        // contributions are summed in task order by one CSR row
+        if (ncls[3] > 0)
+            return fail(GECCO_HIP_EINVAL, "fixed-order plans need every kernel call on the strided-GEMM launch "
+                                          "(materialised operands)");
+        struct Contrib { int64_t dst, src; int8_t sgn; };
+        std::map<int32_t, std::vector<Contrib>> per_slot;
+        int64_t off = 0;
+        for (size_t t = 0; t < (size_t)ncls[0]; ++t) {
+            DevTask &task = tasks[t];
+            const size_t i = (size_t)task.blk;
+            const HostBlock &b = p->blocks[i];
+            const int64_t tw = tile_n(i), th = tile_m(i);
+            task.span = 0;
+            task.owned = 0;
+            task.lo = off;
+            const Resolved &c = b.C.r;
+            std::vector<Contrib> &v = per_slot[b.buf12];
+            const int64_t m1 = std::min<int64_t>(b.M, task.m0 + th), n1 = std::min<int64_t>(b.N, task.n0 + tw);
+            for (int64_t m = task.m0; m < m1; ++m)
+                for (int64_t n = task.n0; n < n1; ++n) {
+                    int64_t a;
+                    int sg = 1;
+                    if (c.table) {
+                        const int32_t e = c.first[m + c.F * n];
+                        if (!e) continue;
+                        sg = e < 0 ? -1 : 1;
+                        a = absv(e) - 1;
+                    } else {
+                        const int32_t e1 = c.first[m], e2 = c.second[n];
+                        if (!e1 || !e2) continue;
+                        sg = (e1 < 0) != (e2 < 0) ? -1 : 1;
+                        a = absv(e1) + absv(e2) - 2;
+                    }
+                    v.push_back({b.off12 + a, off + (m - task.m0) * tw + (n - task.n0), (int8_t)sg});
+                }
+            off += th * tw;
+        }
+        p->raw_elems = off;
+        for (auto &kv : per_slot) {
+            std::vector<Contrib> &v = kv.second;
+            std::stable_sort(v.begin(), v.end(), [](const Contrib &x, const Contrib &y) { return x.dst < y.dst; });
+            TraceStep tr;
+            tr.buf_src = -1;
+            tr.buf_dst = kv.first;
+            tr.rowptr.push_back(0);
+            for (size_t k = 0; k < v.size(); ++k) {
+                if (k == 0 || v[k].dst != v[k - 1].dst) {
+                    if (k) tr.rowptr.push_back((int64_t)k);
+                    tr.rowdst.push_back(v[k].dst);
+                }
+                tr.col.push_back(v[k].src);
+                tr.sgn.push_back(v[k].sgn);
+            }
+            tr.rowptr.push_back((int64_t)v.size());
+            tr.nrows = (int64_t)tr.rowdst.size();
+            tr.nnz = (int64_t)v.size();
+            if (!tr.nrows) continue;
+            std::vector<Contrib>().swap(v);
+            p->dtraces.push_back(std::move(tr));
+        }
+        if (dmalloc(p, &p->d_raw, (size_t)std::max<int64_t>(1, off) * p->elem, "hipMalloc(raw)")) return g_err;
+        for (auto &tr : p->dtraces)
+            if (dmalloc(p, (void **)&tr.d_rowptr, (tr.nrows + 1) * 8, "hipMalloc(fixed-order sums)") ||
+                dmalloc(p, (void **)&tr.d_rowdst, tr.nrows * 8, "hipMalloc(fixed-order sums)") ||
+                dmalloc(p, (void **)&tr.d_col, tr.nnz * 8, "hipMalloc(fixed-order sums)") ||
+                dmalloc(p, (void **)&tr.d_sgn, tr.nnz, "hipMalloc(fixed-order sums)") ||
+                check(hipMemcpy(tr.d_rowptr, tr.rowptr.data(), (tr.nrows + 1) * 8, hipMemcpyHostToDevice), "hipMemcpy(fixed-order sums)") ||
+                check(hipMemcpy(tr.d_rowdst, tr.rowdst.data(), tr.nrows * 8, hipMemcpyHostToDevice), "hipMemcpy(fixed-order sums)") ||
+                check(hipMemcpy(tr.d_col, tr.col.data(), tr.nnz * 8, hipMemcpyHostToDevice), "hipMemcpy(fixed-order sums)") ||
+                check(hipMemcpy(tr.d_sgn, tr.sgn.data(), tr.nnz, hipMemcpyHostToDevice), "hipMemcpy(fixed-order sums)"))
+                return g_err;
+        for (auto &tr : p->dtraces) {
+            std::vector<int64_t>().swap(tr.rowptr);
+            std::vector<int64_t>().swap(tr.rowdst);
+            std::vector<int64_t>().swap(tr.col);
+            std::vector<int8_t>().swap(tr.sgn);
+        }
+    }
     p->flops_affine = p->flops_general = p->flops_direct = 0;
     for (size_t i = 0; i < p->blocks.size(); ++i) {
         const double f = 2.0 * (double)p->blocks[i].M * (double)p->blocks[i].N * (double)p->blocks[i].K;
@@ -1367,7 +1466,7 @@ static int plan_finalize(gecco_hip_plan *p) {
     if (dmalloc(p, (void **)&p->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(DevTask), "hipMalloc(tasks)")) return g_err;
     if (!tasks.empty() && check(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice), "hipMemcpy(tasks)")) return g_err;
     p->nbufs_max = nb_max;
-    if (dmalloc(p, (void **)&p->d_bufs, (nb_max + 1) * sizeof(void *), "hipMalloc(bufs)")) return g_err;
+    if (dmalloc(p, (void **)&p->d_bufs, (nb_max + 2) * sizeof(void *), "hipMalloc(bufs)")) return g_err;
     p->ntasks = (int64_t)tasks.size();
     for (auto &sp : p->spins) {
         if (!sp.ngroups) continue;
@@ -1405,16 +1504,30 @@ static int plan_finalize(gecco_hip_plan *p) {
             std::vector<int8_t>().swap(tr.sgn);
         }
     }
-    if (!p->inits.empty()) {
+    // list initialisations, and in fixed-order mode the zeroing of d_raw
+    // (slot nbufs_max + 1): tasks that are skipped or screened leave their
+    // range at zero
+    std::vector<DevInit> inits_all(p->inits);
+    std::vector<int32_t> igroup_all(p->init_group);
+    if (p->deterministic && p->raw_elems) {
+        DevInit z;
+        std::memset(&z, 0, sizeof z);
+        z.dst = (int32_t)nb_max + 1;
+        z.src = -1;
+        z.len = p->raw_elems;
+        inits_all.push_back(z);
+        igroup_all.push_back(-1);
+    }
+    if (!inits_all.empty()) {
         std::vector<DevInitTask> it;
         const int64_t chunk = 2048;   // elements per workgroup (8 per thread)
-        for (size_t j = 0; j < p->inits.size(); ++j)
-            for (int64_t e0 = 0; e0 < p->inits[j].len; e0 += chunk)
-                it.push_back({(int32_t)j, p->init_group[j], e0, std::min(p->inits[j].len, e0 + chunk)});
+        for (size_t j = 0; j < inits_all.size(); ++j)
+            for (int64_t e0 = 0; e0 < inits_all[j].len; e0 += chunk)
+                it.push_back({(int32_t)j, igroup_all[j], e0, std::min(inits_all[j].len, e0 + chunk)});
         p->nitasks = (int64_t)it.size();
         if (p->nitasks &&
-            (dmalloc(p, (void **)&p->d_inits, p->inits.size() * sizeof(DevInit), "hipMalloc(inits)") ||
-             check(hipMemcpy(p->d_inits, p->inits.data(), p->inits.size() * sizeof(DevInit), hipMemcpyHostToDevice), "hipMemcpy(inits)") ||
+            (dmalloc(p, (void **)&p->d_inits, inits_all.size() * sizeof(DevInit), "hipMalloc(inits)") ||
+             check(hipMemcpy(p->d_inits, inits_all.data(), inits_all.size() * sizeof(DevInit), hipMemcpyHostToDevice), "hipMemcpy(inits)") ||
              dmalloc(p, (void **)&p->d_itasks, it.size() * sizeof(DevInitTask), "hipMalloc(init tasks)") ||
              check(hipMemcpy(p->d_itasks, it.data(), it.size() * sizeof(DevInitTask), hipMemcpyHostToDevice), "hipMemcpy(init tasks)")))
             return g_err;
@@ -1498,7 +1611,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace or spin-projection steps");
     if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty() && p->spins.empty()) return ok();
     std::vector<void *> want(bufs, bufs + p->nbufs_max);
-    if (p->d_scratch) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
+    if (p->d_scratch || p->d_raw) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
+    if (p->d_raw) want.push_back(p->d_raw);                         // slot nbufs_max + 1: fixed-order tiles
     if (want != p->last_bufs) {
         // slot table changes only when the caller passes different buffers;
         // the copy is ordered on the same stream before the launch
@@ -1539,7 +1653,7 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     if (na) {
         if (mark(0, true)) return g_err;
         if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
-                                        p->affine_variant, p->d_skip, zero, p->elem, s))
+                                        p->affine_variant, p->d_skip, zero, p->d_raw, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
@@ -1550,6 +1664,11 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
     }
+    // fixed-order mode: each result element's tile contributions, in task order
+    for (auto &tr : p->dtraces)
+        if (tr.nrows && gecco_launch_trace(p->d_raw, (char *)bufs[tr.buf_dst], tr.d_rowptr, tr.d_rowdst, tr.d_col,
+                                           tr.d_sgn, tr.nrows, 1.0, p->elem, s))
+            return fail(GECCO_HIP_EDEVICE, "fixed-order sum launch failed");
     for (auto &tr : p->traces)
         if (tr.nrows && gecco_launch_trace((const char *)bufs[tr.buf_src] + tr.off_src * p->elem,
                                            (char *)bufs[tr.buf_dst] + tr.off_dst * p->elem,
@@ -2201,6 +2320,13 @@ int gecco_hip_plan_set_screen(gecco_hip_plan *plan, int64_t on) {
     if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
     if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
     plan->screen = on != 0;
+    return 0;
+}
+
+int gecco_hip_plan_set_deterministic(gecco_hip_plan *plan, int64_t on) {
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (plan->finalized) return fail(GECCO_HIP_ESTATE, "plan already finalized");
+    plan->deterministic = on != 0;
     return 0;
 }
 

@@ -99,6 +99,7 @@ _lib.gecco_hip_plan_set_materialize.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_set_constant.argtypes = [ctypes.c_void_p, ctypes.c_int32]
 _lib.gecco_hip_plan_set_chain.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_set_screen.argtypes = [ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_set_deterministic.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_workspace.argtypes = [ctypes.c_void_p, PI64, PI64, PD, PI64]
 _lib.gecco_hip_plan_block_index.argtypes = [ctypes.c_void_p, I64, I64, ctypes.POINTER(ctypes.c_int32),
                                             I64, ctypes.POINTER(ctypes.c_int32), I64]
@@ -381,6 +382,11 @@ class Plan:
         squared norm below 1e-28, checked on the device at every execute
         (the reference's zero-block test); before finalize."""
         _check(_lib.gecco_hip_plan_set_screen(self._h, 1 if on else 0))
+
+    def set_deterministic(self, on=True):
+        """Fixed-order summation: bit-reproducible results
+        (gecco_hip_plan_set_deterministic); before finalize."""
+        _check(_lib.gecco_hip_plan_set_deterministic(self._h, int(bool(on))))
 
     def set_chain(self, on):
         """Run strided-GEMM kernel calls that add into the same result block

@@ -308,6 +308,16 @@ int gecco_hip_plan_set_chain(gecco_hip_plan *plan, int64_t on);
  * device by a launch before the contractions; screened plans run every
  * kernel call as its own GEMM (no chains).  Before finalize only. */
 int gecco_hip_plan_set_screen(gecco_hip_plan *plan, int64_t on);
+/* Fixed-order summation (off by default; no counterpart in the reference,
+ * whose serial loop adds the kernel calls in a fixed order): the results of
+ * a plan are bit-reproducible from execute to execute.  Every task of the
+ * strided-GEMM launch writes its unsigned tile into plan-owned device memory
+ * instead of adding atomically, and one CSR pass per result list then adds
+ * each element's contributions -- every kernel call and K chunk that
+ * reaches it -- in a fixed (task) order.  Costs one write and one read of
+ * every tile element; all kernel calls must run on the strided-GEMM launch
+ * (materialised operands, the default).  Before finalize only. */
+int gecco_hip_plan_set_deterministic(gecco_hip_plan *plan, int64_t on);
 /* Instance groups, for many independent problems (e.g. CC instances of
  * uneven size) recorded into one plan: every kernel call, list
  * initialisation and symmetrisation recorded after set_group(g) belongs to

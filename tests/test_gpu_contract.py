@@ -922,3 +922,60 @@ def test_flattened_joined_entry_matches_oracle(name):
                                  c.ex2_vocc, c.cnt_vocc, c.merge, xt, xt, y, irreps, use_tr=False,
                                  op1op2tmp=occ(r), reo=(1, 0, ({P: 1}, {H: 1}), 1))
         assert _relerr(y, ref) <= TOL
+
+
+@pytest.mark.parametrize('name', list(CASES) + ['t3_ring', 't3_pp_ladder'])
+@pytest.mark.parametrize('variant', [1, 9])
+def test_fixed_order_plans_are_bit_reproducible(name, variant, monkeypatch):
+    """gecco_hip_plan_set_deterministic: the plan matches the oracle, and
+    two executes from the same inputs give bit-identical results (every
+    result element's contributions summed in task order), with split-K,
+    chains, merged-string (table) results and direct tasks in the launch."""
+    torch = _torch()
+    import gecco_hip
+    from models import round_robin
+    monkeypatch.setenv('GECCO_HIP_AFFINE_TILE', str(variant))
+    print(f'fixed-order {name} variant {variant}', flush=True)
+    nocc, nvir, nsym = (4, 9, 2) if name.startswith('t3') else (6, 20, 2)
+    orbs = orbitals(nocc, nvir, nsym)
+    c, a, b, r = {**CASES, **T3_CASES}[name](orbs)
+    x1, x2, x0 = spin_symmetric(a, 81), spin_symmetric(b, 82), spin_symmetric(r, 83)
+    ref = x0.copy()
+    c.run(x1, x2, ref, route=3, use_tr=True, screen=False)
+    o = gecco_hip.Orbs(round_robin(nocc, nsym), round_robin(nvir, nsym, 1), nsym)
+    A, B, R = (gecco_hip.MEList(o, m.occ_c, m.occ_a) for m in (a, b, r))
+    plan = gecco_hip.Plan()
+    plan.set_deterministic(True)
+    plan.set_tuning(target_tasks=4096, min_kchunk=32)      # plenty of split-K
+    plan.add_contraction(A, B, R, c.cnt_c, c.cnt_a, 1.0, True)
+    plan.add_sym_ab(R, 2)
+    plan.finalize()
+    dev = torch.device('cuda:0')
+    outs = []
+    for rep in range(3):
+        bufs = [torch.from_numpy(x).to(dev) for x in (x1, x2, x0)]
+        plan.execute(bufs)
+        torch.cuda.synchronize()
+        outs.append(bufs[2].cpu().numpy())
+    assert _relerr(outs[0], ref) <= TOL
+    assert all(np.array_equal(outs[0], y) for y in outs[1:])
+
+
+def test_fixed_order_ccsd_solves_are_bit_identical():
+    """CCSD with every plan in fixed-order mode (CCSD(deterministic=True)):
+    two solves give bit-identical energies and amplitudes, equal to the
+    default (atomic) solve within 1e-10."""
+    torch = _torch()
+    import ccsd
+    from synth import synthetic
+    ham = synthetic(5, 16, 48, nsym=2, seed=23, sigma=0.08)
+    print('fixed-order ccsd 5/16/2', flush=True)
+    e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
+    runs = []
+    for rep in range(2):
+        c = ccsd.CCSD(ham, deterministic=True)
+        r = c.solve(tol=1e-10, maxit=80)
+        runs.append((r['energy'], r['iterations'], c.amp.cpu().numpy().copy()))
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2])
+    assert abs(runs[0][0] - e0) < 1e-10
