@@ -205,13 +205,15 @@ def kernel_times(plans, run, steps):
                 dom=dom, dom_ms=dom_ms, achieved=achieved, gather_bytes=gbytes)
 
 
-def cpu_ccsd_iteration(args):
+def cpu_ccsd_iteration(args, iters=2):
     """One whole CCSD formula iteration of the oracle's route 3 on one host
     core (oracle/cpu_baseline.py in a child process: the bundled OpenBLAS
-    dgemm limited to one thread before numpy loads), best of two."""
+    dgemm limited to one thread before numpy loads), best of `iters`; args
+    has nocc, nvir, naux, nsym, seed, sigma (attributes or keys)."""
     import subprocess
-    cmd = [sys.executable, '-m', 'oracle.cpu_baseline', str(args.nocc), str(args.nvir), str(args.naux),
-           str(args.nsym), str(args.seed), str(args.sigma), '2']
+    get = (lambda k: args[k]) if isinstance(args, dict) else (lambda k: getattr(args, k))
+    cmd = [sys.executable, '-m', 'oracle.cpu_baseline'] + \
+        [str(get(k)) for k in ('nocc', 'nvir', 'naux', 'nsym', 'seed', 'sigma')] + [str(iters)]
     out = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=900)
     if out.returncode != 0:
         raise RuntimeError('cpu baseline failed: ' + out.stderr[-2000:])
@@ -738,13 +740,20 @@ def run_batched(args):
         'setup_seconds': round(setup_s, 2),
     }
     if not args.no_cpu:
+        # the instance of median FLOP count, one whole formula iteration of
+        # the oracle's route 3 on one core; the batched rate at its GFLOP/s
         mid = sorted(range(len(hams)), key=lambda k: b.inst[k].flops)[len(hams) // 2]
-        v, dt, sample = cpu_ccsd_sample(hams[mid], b.flops)
-        out['cpu_baseline'] = {'value': round(v, 6), 'unit': 'batched iters/s', 'cores': 1,
-                               'kind': 'port',
-                               'sample': f'instance {mid} (nocc {sizes[mid][0]}, nvir {sizes[mid][1]}): '
-                                         + sample.replace('one iteration', 'one batched iteration'),
-                               'seconds': round(dt, 2), 'cpu': platform.processor() or platform.machine()}
+        r = cpu_ccsd_iteration(recipe[mid], iters=1)
+        gflops = r['gflop_per_iteration'] / r['seconds_per_iteration']
+        out['cpu_baseline'] = {
+            'value': round(gflops / (b.flops / 1e9), 6), 'unit': 'batched iters/s', 'cores': 1,
+            'kind': 'port',
+            'sample': f"instance {mid} (nocc {sizes[mid][0]}, nvir {sizes[mid][1]}): one whole CCSD "
+                      f"formula iteration ({r['kernel_calls']} kernel calls, {r['gflop_per_iteration']:.2f} "
+                      f"GFLOP) of the oracle's route 3 with {r['blas']}; the batched iteration's "
+                      f"{b.flops / 1e9:.1f} GFLOP at that rate",
+            'seconds': round(sum(r['times']), 3), 'gflops': round(gflops, 3),
+            'cpu': r['cpu'], 'nproc': r['nproc']}
     return out
 
 

@@ -11,10 +11,12 @@ k of every plan.  Each instance's asynchronous DIIS step sets its entry of a
 device flag array when it meets the tolerance (leaving the amplitudes
 unchanged; all instances' DIIS steps run as one group of three launches,
 gecco_hip.DIISGroup), and every launch skips the work of the flagged
-instances, so a
-converged instance costs nothing further while the replayed graph stays the
-same; the batch iterates until every instance has converged or maxit is
-reached.
+instances, so a converged instance costs no arithmetic while the replayed
+graph stays the same.  Once the host has seen a quarter of the remaining
+instances converge, the plans switch to live-task lists (Plan.compact: the
+tasks of the unconverged instances only) and the iteration is captured
+again, so that converged instances stop costing workgroups too.  The batch
+iterates until every instance has converged or maxit is reached.
 """
 import time
 
@@ -23,10 +25,12 @@ import ccsd
 
 class BatchedCCSD:
     def __init__(self, hams, device='cuda:0', use_tr=True, formula=None, on_instance=None,
-                 skip_converged=True):
+                 skip_converged=True, compact=True):
         """on_instance(k, instance), if given, is called after instance k's
         lists and integrals are set up.  skip_converged=False keeps
-        evaluating converged instances."""
+        evaluating converged instances; compact=False keeps launching the
+        full task lists (converged instances' workgroups only read their
+        flag)."""
         import gecco_hip
         import torch
         self.torch = torch
@@ -41,6 +45,8 @@ class BatchedCCSD:
         # converged flags, one per instance (set by the DIIS steps)
         self.done = torch.zeros(len(self.inst), dtype=torch.int32, device=device)
         self.skip_converged = skip_converged
+        self.compact = compact and skip_converged
+        self.compactions = 0
         self.plans = []
         for li in range(len(self.inst[0].schedule)):
             p = gecco_hip.Plan()
@@ -150,6 +156,10 @@ class BatchedCCSD:
                 if not keep_amplitudes:
                     c.amp.zero_()
             self._diis = [gecco_hip.DIIS(c.amp.numel(), ndiis) for c in self.inst]
+            # full task lists before the flags go back to 0
+            for ps in (self.plans, getattr(self, 'plans32', None) or []):
+                for p in ps:
+                    p.compact(False)
             self.done.zero_()
             for k, d in enumerate(self._diis):
                 d.set_flag(self.done[k:k + 1])
@@ -166,6 +176,7 @@ class BatchedCCSD:
             self._group.step_async(tol, s)
 
         n = len(self.inst)
+        live_at_compaction = n
         conv_at = [None] * n
         singular_at = [None] * n
         energy = [None] * n
@@ -216,6 +227,16 @@ class BatchedCCSD:
                     break
             if finished:
                 break
+            live = sum(x is None for x in conv_at)
+            if self.compact and live <= 0.75 * live_at_compaction:
+                # live-task lists of the unconverged instances (made on the
+                # stream after the iterations queued so far), captured anew
+                for p in (self.plans32 if precision == 'f32' else self.plans):
+                    p.compact(True, stream)
+                live_at_compaction = live
+                self.compactions += 1
+                graph = None
+                self._graph = None
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out = dict(energies=energy, iterations=[x if x is not None else it for x in conv_at],

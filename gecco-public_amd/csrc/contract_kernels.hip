@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include "contract_types.h"
 
 namespace gecco {
@@ -1591,5 +1592,102 @@ extern "C" int gecco_launch_sym_batched(const gecco::DevSymStep *steps,
     else
         hipLaunchKernelGGL(gecco::sym_ab_batched<double>, dim3(grid), dim3(256), 0, stream, steps, tasks,
                            tbl<double>(bufs), i1, i2, fsign, skip, ntasks, per);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+// ---------------------------------------------------------------- live tasks
+// Live-task lists of a plan with instance groups: the records (tasks of one
+// launch class, `words` 32-bit words each, group id at word gw) whose
+// group's skip flag is 0 -- or whose group is negative -- copied in their
+// order.  Three launches: live records per chunk of CCHUNK, one
+// workgroup's exclusive scan of the chunk counts (and the total), the
+// stable scatter (wave ballots give each live record its rank).
+namespace gecco {
+
+constexpr int CCHUNK = 1024;   // records per workgroup, 4 per thread
+
+__device__ __forceinline__ bool record_live(const uint32_t *__restrict__ src, int64_t i, int words,
+                                            int gw, const int32_t *__restrict__ skip) {
+    const int32_t g = (int32_t)src[i * words + gw];
+    return g < 0 || skip[g] == 0;
+}
+
+__global__ __launch_bounds__(256) void compact_count(const uint32_t *__restrict__ src, int64_t n,
+                                                     int words, int gw, const int32_t *__restrict__ skip,
+                                                     int32_t *__restrict__ chunk) {
+    int c = 0;
+    for (int r = 0; r < CCHUNK / 256; ++r) {
+        const int64_t i = (int64_t)blockIdx.x * CCHUNK + r * 256 + threadIdx.x;
+        c += __syncthreads_count(i < n && record_live(src, i, words, gw, skip));
+    }
+    if (threadIdx.x == 0) chunk[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void compact_scan(int32_t *__restrict__ chunk, int nchunk,
+                                                    int64_t *__restrict__ count) {
+    __shared__ int s[256];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nchunk; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        const int v = b < nchunk ? chunk[b] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int x = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+            __syncthreads();
+            s[threadIdx.x] += x;
+            __syncthreads();
+        }
+        const int incl = s[threadIdx.x];
+        if (b < nchunk) chunk[b] = carry + incl - v;   // exclusive offset of chunk b
+        __syncthreads();
+        if (threadIdx.x == 255) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = carry;
+}
+
+__global__ __launch_bounds__(256) void compact_scatter(const uint32_t *__restrict__ src,
+                                                       uint32_t *__restrict__ dst, int64_t n, int words,
+                                                       int gw, const int32_t *__restrict__ skip,
+                                                       const int32_t *__restrict__ chunk) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int base = chunk[blockIdx.x];
+    for (int r = 0; r < CCHUNK / 256; ++r) {
+        const int64_t i = (int64_t)blockIdx.x * CCHUNK + r * 256 + threadIdx.x;
+        const bool live = i < n && record_live(src, i, words, gw, skip);
+        const uint64_t m = __ballot64(live);
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (live) {
+            const int64_t j = off + __popcll(m & ((1ull << lane) - 1));
+            for (int q = 0; q < words; ++q) dst[j * words + q] = src[i * words + q];
+        }
+        const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+        base += tot;
+    }
+}
+
+}  // namespace gecco
+
+// chunk: ceil(n / 1024) int32 of device scratch; *count (device) receives
+// the number of live records.  n fits int32 ranks (the plans' task counts do).
+extern "C" int gecco_launch_compact(const void *src, void *dst, int64_t n, int rec_bytes, int grp_byte,
+                                    const int32_t *skip, int32_t *chunk, int64_t *count,
+                                    hipStream_t stream) {
+    if (n <= 0 || n > INT32_MAX || rec_bytes % 4 || grp_byte % 4 || grp_byte >= rec_bytes) return 1;
+    const int64_t nchunk = (n + gecco::CCHUNK - 1) / gecco::CCHUNK;
+    const int words = rec_bytes / 4, gw = grp_byte / 4;
+    hipLaunchKernelGGL(gecco::compact_count, dim3((unsigned)nchunk), dim3(256), 0, stream,
+                       (const uint32_t *)src, n, words, gw, skip, chunk);
+    hipLaunchKernelGGL(gecco::compact_scan, dim3(1), dim3(256), 0, stream, chunk, (int)nchunk, count);
+    hipLaunchKernelGGL(gecco::compact_scatter, dim3((unsigned)nchunk), dim3(256), 0, stream,
+                       (const uint32_t *)src, (uint32_t *)dst, n, words, gw, skip, chunk);
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }

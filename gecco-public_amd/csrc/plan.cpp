@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -60,6 +61,9 @@ extern "C" int gecco_launch_trace(const void *src, void *dst, const int64_t *row
 extern "C" int gecco_launch_init(const gecco::DevInit *jobs, const gecco::DevInitTask *tasks,
                                  int ntasks, void *const *bufs, const int32_t *skip, int elem,
                                  hipStream_t stream);
+extern "C" int gecco_launch_compact(const void *src, void *dst, int64_t n, int rec_bytes, int grp_byte,
+                                    const int32_t *skip, int32_t *chunk, int64_t *count,
+                                    hipStream_t stream);
 extern "C" int gecco_launch_gather(const gecco::DevGather *jobs, const gecco::DevGatherTask *tasks,
                                    int ntasks, void *const *bufs, void *scratch,
                                    const int32_t *skip, int elem, hipStream_t stream);
@@ -566,6 +570,18 @@ struct gecco_hip_plan {
     int elem = 8;
     const int32_t *d_skip = nullptr;
     int64_t nskip = 0;
+    // live-task lists (gecco_hip_plan_compact): copies of the task lists
+    // with only the tasks of groups whose skip flag was 0 when they were
+    // made, in their order, so that the launches need no workgroups for
+    // converged instances; the launches use them while `compacted`
+    bool compacted = false;
+    gecco::DevTask *d_ltasks = nullptr;          // strided-GEMM at [0, lt[0]), gather kernel at [na, na + lt[1])
+    gecco::DevInitTask *d_litasks = nullptr;     // [0, lt[2])
+    gecco::DevSymTask *d_lsymtasks = nullptr;    // [0, lt[3])
+    gecco::DevGatherTask *d_lgtasks = nullptr;   // constant-slot gathers at [0, lt[4]), others at [ngtasks_static, + lt[5])
+    int32_t *d_lchunk = nullptr;
+    int64_t *d_lcount = nullptr;
+    int64_t lt[6] = {0, 0, 0, 0, 0, 0};
     // device memory from a caller-given region instead of hipMalloc (the
     // allocation-free entry point): base, size, bytes handed out
     char *arena = nullptr;
@@ -752,6 +768,19 @@ static void plan_free_device(gecco_hip_plan *p) {
     p->d_symtasks = nullptr;
     p->d_sym_i1 = p->d_sym_i2 = nullptr;
     p->d_sym_f = nullptr;
+    if (p->d_ltasks) dfree(p, p->d_ltasks);
+    if (p->d_litasks) dfree(p, p->d_litasks);
+    if (p->d_lsymtasks) dfree(p, p->d_lsymtasks);
+    if (p->d_lgtasks) dfree(p, p->d_lgtasks);
+    if (p->d_lchunk) dfree(p, p->d_lchunk);
+    if (p->d_lcount) dfree(p, p->d_lcount);
+    p->d_ltasks = nullptr;
+    p->d_litasks = nullptr;
+    p->d_lsymtasks = nullptr;
+    p->d_lgtasks = nullptr;
+    p->d_lchunk = nullptr;
+    p->d_lcount = nullptr;
+    p->compacted = false;
     p->d_blocks = nullptr;
     p->d_tasks = nullptr;
     p->d_index = nullptr;
@@ -1609,6 +1638,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (!bufs[i]) return fail(GECCO_HIP_EINVAL, "buffer slot %lld is null", (long long)i);
     if (p->d_skip && (!p->traces.empty() || !p->spins.empty()))
         return fail(GECCO_HIP_EINVAL, "skip flags are not supported for plans with trace or spin-projection steps");
+    if (p->d_skip && p->deterministic)
+        return fail(GECCO_HIP_EINVAL, "skip flags are not supported for fixed-order plans");
     if (p->ntasks == 0 && p->syms.empty() && p->nitasks == 0 && p->traces.empty() && p->spins.empty()) return ok();
     std::vector<void *> want(bufs, bufs + p->nbufs_max);
     if (p->d_scratch || p->d_raw) want.push_back(p->d_scratch);   // slot nbufs_max: scratch
@@ -1623,6 +1654,14 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
     }
     const int64_t na = p->ntasks_affine;
     const int64_t ng = p->ntasks - na;
+    // the launches' task lists: the full ones, or the live ones
+    const bool cp = p->compacted;
+    const gecco::DevTask *ta = cp ? p->d_ltasks : p->d_tasks, *tg = (cp ? p->d_ltasks : p->d_tasks) + na;
+    const int64_t nla = cp ? p->lt[0] : na, nlg = cp ? p->lt[1] : ng;
+    const gecco::DevInitTask *ti = cp ? p->d_litasks : p->d_itasks;
+    const int64_t nli = cp ? p->lt[2] : p->nitasks;
+    const gecco::DevSymTask *ts = cp ? p->d_lsymtasks : p->d_symtasks;
+    const int64_t nls = cp ? p->lt[3] : p->nsymtasks;
     auto mark = [&](int cls, bool begin) -> int {
         if (!p->timing) return 0;
         hipEvent_t e;
@@ -1631,8 +1670,8 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         else p->pending.back().second.second = e;
         return 0;
     };
-    if (p->nitasks &&
-        gecco_launch_init(p->d_inits, p->d_itasks, (int)p->nitasks, p->d_bufs, p->d_skip, p->elem, s))
+    if (nli &&
+        gecco_launch_init(p->d_inits, ti, (int)nli, p->d_bufs, p->d_skip, p->elem, s))
         return fail(GECCO_HIP_EDEVICE, "init launch failed: %s", hipGetErrorString(hipGetLastError()));
     const int32_t *zero = nullptr;
     if (p->screen && p->nnorm_jobs) {
@@ -1642,7 +1681,19 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         zero = p->d_nzero;
     }
     const int64_t g0 = p->static_done ? p->ngtasks_static : 0;
-    if (p->ngtasks > g0) {
+    if (cp) {
+        // live constant-slot gathers (first execute only), then the others
+        const int64_t ns = p->static_done ? 0 : p->lt[4], nd = p->lt[5];
+        if (ns + nd) {
+            if (mark(3, true)) return g_err;
+            if ((ns && gecco_launch_gather(p->d_gjobs, p->d_lgtasks, (int)ns, p->d_bufs, p->d_scratch,
+                                           p->d_skip, p->elem, s)) ||
+                (nd && gecco_launch_gather(p->d_gjobs, p->d_lgtasks + p->ngtasks_static, (int)nd, p->d_bufs,
+                                           p->d_scratch, p->d_skip, p->elem, s)))
+                return fail(GECCO_HIP_EDEVICE, "gather launch failed: %s", hipGetErrorString(hipGetLastError()));
+            if (mark(3, false)) return g_err;
+        }
+    } else if (p->ngtasks > g0) {
         if (mark(3, true)) return g_err;
         if (gecco_launch_gather(p->d_gjobs, p->d_gtasks + g0, (int)(p->ngtasks - g0), p->d_bufs,
                                 p->d_scratch, p->d_skip, p->elem, s))
@@ -1650,16 +1701,16 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
         if (mark(3, false)) return g_err;
     }
     p->static_done = true;
-    if (na) {
+    if (nla) {
         if (mark(0, true)) return g_err;
-        if (gecco_launch_grouped_affine(p->d_blocks, p->d_tasks, (int)na, p->d_bufs, (int)na,
+        if (gecco_launch_grouped_affine(p->d_blocks, ta, (int)nla, p->d_bufs, (int)nla,
                                         p->affine_variant, p->d_skip, zero, p->d_raw, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(0, false)) return g_err;
     }
-    if (ng) {
+    if (nlg) {
         if (mark(1, true)) return g_err;
-        if (gecco_launch_grouped(p->d_blocks, p->d_tasks + na, (int)ng, p->d_bufs, (int)ng, p->d_skip,
+        if (gecco_launch_grouped(p->d_blocks, tg, (int)nlg, p->d_bufs, (int)nlg, p->d_skip,
                                  zero, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         if (mark(1, false)) return g_err;
@@ -1675,9 +1726,9 @@ static int plan_execute(gecco_hip_plan *p, double *const *bufs, int64_t nbufs, h
                                            tr.d_rowptr, tr.d_rowdst, tr.d_col, tr.d_sgn, tr.nrows,
                                            tr.fact, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "trace launch failed");
-    if (p->nsymtasks) {
+    if (nls) {
         if (mark(2, true)) return g_err;
-        if (gecco_launch_sym_batched(p->d_symsteps, p->d_symtasks, (int)p->nsymtasks, p->d_bufs,
+        if (gecco_launch_sym_batched(p->d_symsteps, ts, (int)nls, p->d_bufs,
                                      p->d_sym_i1, p->d_sym_i2, p->d_sym_f, p->d_skip, p->elem, s))
             return fail(GECCO_HIP_EDEVICE, "sym_ab launch failed");
         if (mark(2, false)) return g_err;
@@ -2313,6 +2364,74 @@ int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t 
         return fail(GECCO_HIP_EINVAL, "skip flags must be device memory");
     plan->d_skip = flags;
     plan->nskip = flags ? nflags : 0;
+    plan->compacted = false;   // live lists belong to the flags they were made from
+    return 0;
+}
+
+int gecco_hip_plan_compact(gecco_hip_plan *plan, int64_t on, void *stream) {
+    clear_error();
+    if (!plan) return fail(GECCO_HIP_EINVAL, "null plan");
+    if (!plan->finalized) return fail(GECCO_HIP_ESTATE, "compact: plan not finalized");
+    if (!on) {
+        plan->compacted = false;
+        return 0;
+    }
+    if (!plan->d_skip) return fail(GECCO_HIP_ESTATE, "compact: the plan has no skip flags");
+    if (plan->deterministic) return fail(GECCO_HIP_EINVAL, "compact: fixed-order plans keep every task");
+    if (plan->arena) return fail(GECCO_HIP_EINVAL, "compact: plan without device memory of its own");
+    gecco_hip_plan *p = plan;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t na = p->ntasks_affine;
+    struct Part { const void *src; void *dst; int64_t n; int rec, grp; };
+    if (!p->d_ltasks) {
+        const int64_t nc = [&] {
+            auto ch = [](int64_t n) { return (n + 1023) / 1024; };
+            return ch(na) + ch(p->ntasks - na) + ch(p->nitasks) + ch(p->nsymtasks) + ch(p->ngtasks_static) +
+                   ch(p->ngtasks - p->ngtasks_static);
+        }();
+        if (dmalloc(p, (void **)&p->d_ltasks, std::max<int64_t>(1, p->ntasks) * sizeof(gecco::DevTask), "hipMalloc(live tasks)") ||
+            dmalloc(p, (void **)&p->d_litasks, std::max<int64_t>(1, p->nitasks) * sizeof(gecco::DevInitTask), "hipMalloc(live tasks)") ||
+            dmalloc(p, (void **)&p->d_lsymtasks, std::max<int64_t>(1, p->nsymtasks) * sizeof(gecco::DevSymTask), "hipMalloc(live tasks)") ||
+            dmalloc(p, (void **)&p->d_lgtasks, std::max<int64_t>(1, p->ngtasks) * sizeof(gecco::DevGatherTask), "hipMalloc(live tasks)") ||
+            dmalloc(p, (void **)&p->d_lchunk, std::max<int64_t>(1, nc) * sizeof(int32_t), "hipMalloc(live tasks)") ||
+            dmalloc(p, (void **)&p->d_lcount, 6 * sizeof(int64_t), "hipMalloc(live tasks)"))
+            return g_err;
+    }
+    const Part parts[6] = {
+        {p->d_tasks, p->d_ltasks, na, (int)sizeof(gecco::DevTask), (int)offsetof(gecco::DevTask, grp)},
+        {p->d_tasks + na, p->d_ltasks + na, p->ntasks - na, (int)sizeof(gecco::DevTask), (int)offsetof(gecco::DevTask, grp)},
+        {p->d_itasks, p->d_litasks, p->nitasks, (int)sizeof(gecco::DevInitTask), (int)offsetof(gecco::DevInitTask, grp)},
+        {p->d_symtasks, p->d_lsymtasks, p->nsymtasks, (int)sizeof(gecco::DevSymTask), (int)offsetof(gecco::DevSymTask, grp)},
+        {p->d_gtasks, p->d_lgtasks, p->ngtasks_static, (int)sizeof(gecco::DevGatherTask), (int)offsetof(gecco::DevGatherTask, grp)},
+        {p->d_gtasks + p->ngtasks_static, p->d_lgtasks + p->ngtasks_static, p->ngtasks - p->ngtasks_static,
+         (int)sizeof(gecco::DevGatherTask), (int)offsetof(gecco::DevGatherTask, grp)},
+    };
+    int64_t chunk0 = 0;
+    bool launched[6] = {false, false, false, false, false, false};
+    for (int i = 0; i < 6; ++i) {
+        const Part &q = parts[i];
+        if (q.n <= 0) continue;
+        if (gecco_launch_compact(q.src, q.dst, q.n, q.rec, q.grp, p->d_skip, p->d_lchunk + chunk0,
+                                 p->d_lcount + i, s))
+            return fail(GECCO_HIP_EDEVICE, "live-task launch failed: %s", hipGetErrorString(hipGetLastError()));
+        launched[i] = true;
+        chunk0 += (q.n + 1023) / 1024;
+    }
+    int64_t cnt[6] = {0, 0, 0, 0, 0, 0};
+    if (check(hipStreamSynchronize(s), "hipStreamSynchronize") ||
+        check(hipMemcpy(cnt, p->d_lcount, sizeof cnt, hipMemcpyDeviceToHost), "hipMemcpy(live counts)"))
+        return g_err;
+    for (int i = 0; i < 6; ++i) p->lt[i] = launched[i] ? cnt[i] : 0;
+    p->compacted = true;
+    return 0;
+}
+
+int gecco_hip_plan_live_tasks(const gecco_hip_plan *plan, int64_t *counts, int64_t ncounts) {
+    if (!plan || !counts || ncounts < 1) return fail(GECCO_HIP_EINVAL, "live_tasks: null argument");
+    const int64_t na = plan->ntasks_affine;
+    const int64_t full[6] = {na, plan->ntasks - na, plan->nitasks, plan->nsymtasks, plan->ngtasks_static,
+                             plan->ngtasks - plan->ngtasks_static};
+    for (int64_t i = 0; i < std::min<int64_t>(ncounts, 6); ++i) counts[i] = plan->compacted ? plan->lt[i] : full[i];
     return 0;
 }
 

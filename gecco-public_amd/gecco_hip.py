@@ -72,7 +72,7 @@ C_SYMBOLS = ('gecco_hip_last_error', 'gecco_hip_plan_create', 'gecco_hip_plan_de
              'gecco_hip_plan_set_constant', 'gecco_hip_plan_set_chain',
              'gecco_hip_plan_set_group', 'gecco_hip_plan_set_skip', 'gecco_hip_plan_set_tuning',
              'gecco_hip_plan_keep_host', 'gecco_hip_plan_retune', 'gecco_hip_plan_set_precision',
-             'gecco_hip_task_order')
+             'gecco_hip_task_order', 'gecco_hip_plan_compact', 'gecco_hip_plan_live_tasks')
 
 _lib.gecco_hip_last_error.restype = ctypes.c_int
 _lib.gecco_hip_last_error.argtypes = [ctypes.c_char_p, I64]
@@ -89,6 +89,8 @@ _lib.gecco_hip_plan_keep_host.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_set_precision.argtypes = [ctypes.c_void_p, I64]
 _lib.gecco_hip_plan_retune.argtypes = [ctypes.c_void_p, I64, I64, I64, I64]
 _lib.gecco_hip_plan_set_skip.argtypes = [ctypes.c_void_p, ctypes.c_void_p, I64]
+_lib.gecco_hip_plan_compact.argtypes = [ctypes.c_void_p, I64, ctypes.c_void_p]
+_lib.gecco_hip_plan_live_tasks.argtypes = [ctypes.c_void_p, PI64, I64]
 _lib.gecco_hip_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), I64,
                                         ctypes.c_void_p]
 _lib.gecco_hip_plan_stats.argtypes = [ctypes.c_void_p, PD, PI64, PI64, PD, PD]
@@ -440,6 +442,20 @@ class Plan:
             raise GeccoHipError('set_skip: flags must be a contiguous int32 tensor')
         _check(_lib.gecco_hip_plan_set_skip(self._h, _ptr(flags), flags.numel()))
         self._skip = flags            # keeps the memory alive
+
+    def compact(self, on=True, stream=None):
+        """on: launch only the tasks of groups whose skip flag is 0 now (the
+        live-task lists are made on `stream`, which is then synchronized);
+        off: the full task lists again (call it before clearing a flag)."""
+        s = ctypes.c_void_p(0 if stream is None else int(stream))
+        _check(_lib.gecco_hip_plan_compact(self._h, 1 if on else 0, s))
+
+    def live_tasks(self):
+        """Task counts the launches use now: strided-GEMM, gather kernel,
+        initialisation, symmetrisation, constant-slot gathers, other gathers."""
+        out = (I64 * 6)()
+        _check(_lib.gecco_hip_plan_live_tasks(self._h, out, 6))
+        return [int(x) for x in out]
 
     def set_materialize(self, min_dim):
         """Materialise gathered operands of kernel calls with min(M, N) >=

@@ -373,6 +373,22 @@ int gecco_hip_plan_set_precision(gecco_hip_plan *plan, int64_t elem_bytes);
 int gecco_hip_plan_retune(gecco_hip_plan *plan, int64_t variant, int64_t target_tasks,
                           int64_t min_kchunk, int64_t max_kchunk);
 int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t nflags);
+/* Live-task lists of a plan with skip flags: compact(plan, 1, stream) copies,
+ * on `stream`, every task list of the plan's launches keeping only the tasks
+ * of groups whose flag is 0 now (and those of negative groups, e.g. the
+ * gathers of constant slots), in their order, then synchronizes `stream`; the
+ * launches of execute then have one workgroup per live task instead of one
+ * per task, and still read the flags.  Valid while no flag that was set goes
+ * back to 0: a caller that clears flags (a restart) calls compact(plan, 0,
+ * NULL) first, which returns to the full lists.  A graph captured before a
+ * compaction still launches the full lists (capture again to use the live
+ * ones).  Refused for fixed-order plans.  set_skip and retune drop the live
+ * lists.  live_tasks reports the task counts the launches use now (strided-
+ * GEMM, gather kernel, initialisation, symmetrisation, constant-slot
+ * gathers, other gathers; up to ncounts of them).  No counterpart in the
+ * reference, which runs one problem per process. */
+int gecco_hip_plan_compact(gecco_hip_plan *plan, int64_t on, void *stream);
+int gecco_hip_plan_live_tasks(const gecco_hip_plan *plan, int64_t *counts, int64_t ncounts);
 /* Scratch bytes held by a finalized plan, number of gather jobs, the HBM
  * bytes they move per execute (one read and one write per element), and the
  * number of tasks whose result tile is reduced in LDS and written back as

@@ -164,6 +164,7 @@ def test_batched_instances_match_separate_solves():
     assert res['batch_iterations'] == max(r['iterations'] for r in sep) + 1 or \
         res['batch_iterations'] == max(r['iterations'] for r in sep)
     assert b.done.cpu().tolist() == [1] * len(cases)    # every instance flagged converged
+    assert b.compactions >= 1                             # live-task lists were used
     # a second solve reuses the captured graph
     res2 = b.solve(tol=1e-10, maxit=80)
     assert max(abs(x - y) for x, y in zip(res2['energies'], res['energies'])) < 1e-12

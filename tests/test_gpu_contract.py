@@ -165,6 +165,94 @@ def test_instance_groups_skip_flags():
     assert _relerr(bufs[2].cpu().numpy(), refs[0]) <= TOL
 
 
+def test_live_task_lists_match_full_lists():
+    """Plan.compact: live-task lists of the groups whose skip flag is 0 give
+    the results of the full lists (the oracle's for live groups; flagged
+    groups keep their contents exactly), launch fewer tasks, still read flags
+    set after the compaction, work inside a replayed graph, and give way to
+    the full lists again before flags are cleared."""
+    torch = _torch()
+    import gecco_hip
+    orbs = orbitals(5, 8, 4)
+    dev = torch.device('cuda:0')
+    plan = gecco_hip.Plan()
+    names = ['ladder', 'ring', 't1t1', 'ladder', 'ring', 't1t1']
+    bufs, refs, prior = [], [], []
+    rng = np.random.default_rng(11)
+    for k, name in enumerate(names):
+        plan.set_group(k)
+        c, a, b, r = CASES[name](orbs)
+        x1 = spin_symmetric(a, 80 + k)
+        x2 = spin_symmetric(b, 90 + k)
+        blocks = c.blocks(True)
+        refs.append(_oracle(c, x1, x2, r.length, blocks))
+        prior.append(rng.standard_normal(r.length))
+        base = len(bufs)
+        bufs += [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+                 torch.from_numpy(prior[-1]).to(dev)]
+        plan.add_init(base + 2, r.length)
+        for bl in blocks:
+            plan.add(bl, buf=(base, base + 1, base + 2))
+    plan.finalize()
+    with pytest.raises(gecco_hip.GeccoHipError):
+        plan.compact(True)                            # no skip flags yet
+    flags = torch.tensor([0, 1, 0, 1, 1, 0], dtype=torch.int32, device=dev)
+    plan.set_skip(flags)
+    full = plan.live_tasks()
+    print('live tasks: full', full, flush=True)
+    plan.compact(True)
+    live = plan.live_tasks()
+    print('live tasks: compacted', live, flush=True)
+    assert all(x <= y for x, y in zip(live, full)) and sum(live) < sum(full)
+
+    def check(skipped):
+        torch.cuda.synchronize()
+        for k in range(len(names)):
+            got = bufs[3 * k + 2].cpu().numpy()
+            if k in skipped:
+                assert np.array_equal(got, prior[k]), k
+            else:
+                assert _relerr(got, refs[k]) <= TOL, k
+    plan.execute(bufs)
+    check({1, 3, 4})
+    # a flag set after the compaction is still read
+    flags[0] = 1
+    bufs[2].copy_(torch.from_numpy(prior[0]))
+    plan.execute(bufs)
+    check({0, 1, 3, 4})
+    # a replayed graph of the compacted plan
+    plan.compact(True)
+    assert plan.live_tasks()[0] < live[0]
+    for k in (2, 5):
+        bufs[3 * k + 2].fill_(7.0)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.execute(bufs, s.cuda_stream)             # buffer table upload outside the capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.execute(bufs, torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        for k in (2, 5):
+            bufs[3 * k + 2].fill_(float(rep))
+        torch.cuda.synchronize()
+        g.replay()
+        check({0, 1, 3, 4})
+    # the full lists again, then every flag cleared
+    plan.compact(False)
+    assert plan.live_tasks() == full
+    flags.zero_()
+    plan.execute(bufs)
+    check(set())
+    # compaction with no flag set keeps every task
+    plan.compact(True)
+    assert plan.live_tasks() == full
+    plan.execute(bufs)
+    check(set())
+
+
 @pytest.mark.parametrize('nsym,nocc,nvir', [(1, 3, 5), (2, 4, 6), (4, 6, 10)])
 @pytest.mark.parametrize('name', list(CASES))
 @pytest.mark.parametrize('use_tr', [True, False])
