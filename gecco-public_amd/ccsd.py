@@ -121,11 +121,13 @@ def record_contraction(p, c, lists, slot, use_tr):
     vertices) goes through gecco_hip_plan_add_contraction_n."""
     a, b, t = c['op1'], c['op2'], c['target']
     buf = (slot(a), slot(b), slot(t))
+    # dagger flags: the flagged lists take part as their adjoint views
+    L1, L2, Lt = [lists[n].adjoint() if f else lists[n]
+                  for n, f in zip((a, b, t), c.get('tra') or (False, False, False))]
     v = c.get('vertices')
     if v is None:
-        return p.add_contraction(lists[a], lists[b], lists[t], c['cnt_c'], c['cnt_a'], c['factor'],
-                                 use_tr, buf=buf)
-    return p.add_contraction_n(lists[a], lists[b], lists[t], v['ex1'], v['ex2'], v['cnt'], *v['merge'],
+        return p.add_contraction(L1, L2, Lt, c['cnt_c'], c['cnt_a'], c['factor'], use_tr, buf=buf)
+    return p.add_contraction_n(L1, L2, Lt, v['ex1'], v['ex2'], v['cnt'], *v['merge'],
                                fact=c['factor'], use_tr=use_tr, buf=buf)
 
 

@@ -48,7 +48,10 @@ CONTR or BC item between such lists carries "vertices": {"ex1": [[C, A],
 ...], "ex2": [...], "cnt": [...], "merge": [merge_op1, merge_op2,
 merge_op1op2, merge_op2op1]} -- contr_op1op2's iocc_ex1, iocc_ex2, iocc_cnt
 and short-form merge maps (contract/contr_op1op2.f:2-17,
-contract/get_bc_info3.f:271-330).  Every list is declared with the formula; NEW and DELETE
+contract/get_bc_info3.f:271-330).  A CONTR or BC item may carry "tra":
+[op1, op2, target], contr_op1op2's dagger flags tra_op1, tra_op2 and
+tra_op1op2: the flagged one-vertex list takes part as its adjoint (C and A
+exchanged, the same storage).  Every list is declared with the formula; NEW and DELETE
 (or, without them, the items themselves) bound the levels in which an
 intermediate holds data (`lifetimes`), so that a driver may let intermediates
 whose lifetimes do not meet share device memory (`pool_layout`).
@@ -190,10 +193,13 @@ def load(path):
             raise ValueError(f'unknown formula command {it["cmd"]!r}')
         if it['cmd'] in ('CONTR', 'BC'):
             v = it.get('vertices')
+            tra = it.get('tra')
             it = dict(contr(it['target'], it['op1'], it['op2'], _occ_in(it['cnt_c']), _occ_in(it['cnt_a']),
                             it['factor']), cmd=it['cmd'])
             if v is not None:
                 it['vertices'] = _vertices_io(v, _occ_in)
+            if tra is not None:
+                it['tra'] = [bool(x) for x in tra]
         elif it['cmd'] in ('REORDER', 'ADD_REO'):
             it = reorder(it['target'], it['source'], it['from'], it['to'], _occ_in(it['shift_c']),
                          _occ_in(it['shift_a']), it.get('factor', 1.0), it.get('sign', 1),

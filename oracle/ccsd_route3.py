@@ -26,7 +26,7 @@ import numpy as np
 
 from .contract import Contraction, sym_ab_pairs, apply_sym_pairs
 from .reorder import Reorder
-from .melist import Orbitals, MEList
+from .melist import Orbitals, MEList, Adjoint
 from .strings import strings_of, to_spinorbs
 
 H, P = 1, 2
@@ -118,12 +118,14 @@ class Route3CCSD:
             cs = []
             for it in lv['contr']:
                 v = it.get('vertices')
+                # dagger flags: adjoint views of the flagged one-vertex lists
+                L1, L2, Lt = [Adjoint(self.L[n]) if f else self.L[n] for n, f in
+                              zip((it['op1'], it['op2'], it['target']), it.get('tra') or (0, 0, 0))]
                 if v is None:
-                    c = Contraction(self.L[it['op1']], self.L[it['op2']], self.L[it['target']],
-                                    it['cnt_c'], it['cnt_a'], it['factor'])
+                    c = Contraction(L1, L2, Lt, it['cnt_c'], it['cnt_a'], it['factor'])
                 else:       # lists of several vertices
-                    c = Contraction.joined(self.L[it['op1']], self.L[it['op2']], self.L[it['target']],
-                                           v['ex1'], v['ex2'], v['cnt'], *v['merge'], fact=it['factor'])
+                    c = Contraction.joined(L1, L2, Lt, v['ex1'], v['ex2'], v['cnt'], *v['merge'],
+                                           fact=it['factor'])
                 blocks = c.blocks(True)
                 self.flops += sum(2.0 * np.prod(b['lstr_ex1']) * np.prod(b['lstr_ex2']) *
                                   np.prod(b['lstr_cnt']) for b in blocks)

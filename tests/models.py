@@ -342,3 +342,25 @@ def ccsd_with_joined_tau():
         else:
             items.append(it)
     return {'lists': lists, 'constant': list(ccsd.INTEGRALS), 'items': items}
+
+
+def ccsd_with_adjoint_integrals():
+    """The CCSD formula with every term that reads <ij||ab> (Hoovv) reading it
+    instead as the adjoint of HvvooS, a list stored as <ab||ij> (P2; H2):
+    the term's dagger flag ("tra", contr_op1op2's tra_op1 / tra_op2) set on
+    that operand.  For real integrals <ab||ij> = <ij||ab>, so the residuals
+    equal the hand formula's."""
+    import ccsd
+    lists = dict(ccsd.LISTS)
+    lists['HvvooS'] = lists['Hvvoo']
+    items = []
+    for it in ccsd.FORMULA['items']:
+        it = dict(it)
+        if it['cmd'] == 'CONTR' and 'Hoovv' in (it['op1'], it['op2']):
+            k = 0 if it['op1'] == 'Hoovv' else 1
+            it['op1' if k == 0 else 'op2'] = 'HvvooS'
+            tra = [False, False, False]
+            tra[k] = True
+            it['tra'] = tra
+        items.append(it)
+    return {'lists': lists, 'constant': list(ccsd.INTEGRALS) + ['HvvooS'], 'items': items}

@@ -333,6 +333,28 @@ def test_spin_projected_residuals_give_the_same_energy():
     assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11
 
 
+def test_formula_read_from_a_gecco_formula_file_on_the_engine():
+    """A CCSD formula written as a GeCCo binary formula list and read back
+    (formula_list: [INIT TARGET], [ADD_BC] with dagger flags on the operand
+    that reads <ij||ab> from a list stored as <ab||ij>, [CP_INTM]) runs on
+    the engine -- the flagged operands as adjoint list views -- and reaches
+    the hand formula's energy."""
+    _torch()
+    import ccsd
+    import formula_list as fl
+    from models import ccsd_with_adjoint_integrals
+    f = ccsd_with_adjoint_integrals()
+    data = fl.serialize('ccsd, adjoint integrals', fl.from_formula(f)['items'])
+    g = fl.to_formula(fl.parse(data), constant=f['constant'])
+    assert sum(1 for it in g['items'] if it.get('tra')) >= 5
+    ham = synthetic(4, 12, 36, nsym=2, seed=19, sigma=0.1)
+    print('formula-list ccsd 4/12/2', flush=True)
+    e0 = ccsd.CCSD(ham).solve(tol=1e-10, maxit=80)['energy']
+    r = ccsd.CCSD(ham, formula=g).solve(tol=1e-10, maxit=80)
+    print('formula-list', r['iterations'], r['energy'] - e0, flush=True)
+    assert r['residual'] < 1e-10 and abs(r['energy'] - e0) < 1e-11
+
+
 def test_joined_intermediate_formula_on_the_engine():
     """A CCSD formula whose T1 x T1 products are formed as the two-vertex
     intermediate X11 (gecco_hip_plan_add_contraction_n) and reach tau and
