@@ -39,6 +39,7 @@ namespace {
 constexpr int MAXSUB = 16;       // largest DIIS subspace
 constexpr int NTH = 256;
 constexpr int NGROUP = 1024;     // workgroups of the fused pass
+static_assert((MAXSUB - 1) * (MAXSUB - 1) <= NTH, "one thread per element of the shifted B");
 constexpr double DIA_THRESH = 1e-10;
 
 struct Slots {
@@ -332,43 +333,57 @@ __device__ __forceinline__ void finish_body(const double *__restrict__ part, int
             __syncthreads();
         }
     }
-    if (tid != 0) return;
-    int64_t code = code_s;
-    if (code == 0) {
-        for (int r = k - 1; r >= 0; --r) {
-            double x = rhs[r];
-            for (int j = r + 1; j < k; ++j) x -= a[r * k + j] * rhs[j];
-            rhs[r] = x / a[r * k + r];
+    const int64_t code = code_s;
+    // the next step's vector: a free slot, or the oldest one's when the
+    // subspace is full, whose row and column of B are dropped -- read here
+    // by the whole workgroup, written after the barrier below (a serial
+    // copy of B by one thread waited on each load in turn)
+    const bool full = m == maxsub;
+    const int mm = full ? m - 1 : m;
+    const int slot0 = st->order[0];
+    int o_next = 0, bi = 0, bj = 0;
+    double b_next = 0.0;
+    if (full) {
+        if (tid + 1 < m) o_next = st->order[tid + 1];
+        if (tid < mm * mm) {      // mm <= MAXSUB - 1, so mm * mm < NTH
+            bi = tid / mm;
+            bj = tid - bi * mm;
+            b_next = st->b[(bi + 1) * MAXSUB + bj + 1];
         }
     }
-    for (int j = 0; j < m; ++j) {
-        st->w[j] = code == 0 ? rhs[j] : 0.0;
-        st->wslot[j] = st->order[j];
+    if (tid == 0) {
+        if (code == 0) {
+            for (int r = k - 1; r >= 0; --r) {
+                double x = rhs[r];
+                for (int j = r + 1; j < k; ++j) x -= a[r * k + j] * rhs[j];
+                rhs[r] = x / a[r * k + r];
+            }
+        }
+        for (int j = 0; j < m; ++j) {
+            st->w[j] = code == 0 ? rhs[j] : 0.0;
+            st->wslot[j] = st->order[j];
+        }
+        st->mw = m;
+        st->skip = code != 0 ? 1 : 0;
+        StatusRec &rec = ring[st->steps % RING];
+        rec.gnorm = gn;
+        st->gn_last = gn;
+        rec.energy = energy ? energy[0] : 0.0;
+        rec.converged = code;
+        if (flag) *flag = code == 1 ? 1 : 0;
+        __threadfence_system();
+        rec.step = st->steps;
+        st->steps += 1;
     }
-    st->mw = m;
-    st->skip = code != 0 ? 1 : 0;
-    StatusRec &rec = ring[st->steps % RING];
-    rec.gnorm = gn;
-    st->gn_last = gn;
-    rec.energy = energy ? energy[0] : 0.0;
-    rec.converged = code;
-    if (flag) *flag = code == 1 ? 1 : 0;
-    __threadfence_system();
-    rec.step = st->steps;
-    st->steps += 1;
-    // the next step's vector
-    int mm = m, slot;
-    if (mm == maxsub) {
-        slot = st->order[0];
-        for (int j = 0; j + 1 < mm; ++j) st->order[j] = st->order[j + 1];
-        for (int i = 0; i + 1 < mm; ++i)
-            for (int j = 0; j + 1 < mm; ++j) st->b[i * MAXSUB + j] = st->b[(i + 1) * MAXSUB + j + 1];
-        mm -= 1;
-    } else {
-        slot = mm;
+    __syncthreads();   // every read of order and B above is done
+    if (full) {
+        if (tid + 1 < m) st->order[tid] = o_next;
+        if (tid < mm * mm) st->b[bi * MAXSUB + bj] = b_next;
     }
-    st->order[mm] = slot;
-    st->m = mm + 1;
+    if (tid == 0) {
+        st->order[mm] = full ? slot0 : mm;
+        st->m = mm + 1;
+    }
 }
 
 __global__ __launch_bounds__(NTH) void diis_finish_f64(const double *__restrict__ part, int ngroup,
