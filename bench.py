@@ -393,6 +393,9 @@ def cpu_triples_sample(ham, flops_eval):
     import numpy as np
     from oracle.melist import Orbitals, MEList
     from oracle.contract import Contraction
+    from oracle import contract
+    contract.blas_threads(1)
+    dgemm = 'the bundled OpenBLAS ILP64 dgemm on 1 thread' if contract.blas() else 'a plain-loop dgemm'
     H, P = 1, 2
     orbs = Orbitals(ham.irrep_o, ham.irrep_v, ham.nsym)
     V = MEList(orbs, {P: 1, H: 1}, {H: 2})
@@ -410,7 +413,7 @@ def cpu_triples_sample(ham, flops_eval):
     dt = time.perf_counter() - t0
     per_eval = dt * flops_eval / fl
     return 1.0 / per_eval, dt, (f'the T3 <- V(ov,oo).T2 term ({fl / 1e9:.2f} of {flops_eval / 1e9:.1f} '
-                                'GFLOP per evaluation), oracle route 3 with a plain-loop dgemm, '
+                                f'GFLOP per evaluation), oracle route 3 with {dgemm}, '
                                 'scaled to one evaluation by FLOP')
 
 
