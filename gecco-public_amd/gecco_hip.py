@@ -446,7 +446,9 @@ class Plan:
     def compact(self, on=True, stream=None):
         """on: launch only the tasks of groups whose skip flag is 0 now (the
         live-task lists are made on `stream`, which is then synchronized);
-        off: the full task lists again (call it before clearing a flag)."""
+        off: the full task lists again (call it before clearing a flag).
+        Each compaction rewrites the live lists in place: capture a graph
+        of the compacted plan again after the next call."""
         s = ctypes.c_void_p(0 if stream is None else int(stream))
         _check(_lib.gecco_hip_plan_compact(self._h, 1 if on else 0, s))
 

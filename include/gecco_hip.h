@@ -380,9 +380,11 @@ int gecco_hip_plan_set_skip(gecco_hip_plan *plan, const int32_t *flags, int64_t 
  * launches of execute then have one workgroup per live task instead of one
  * per task, and still read the flags.  Valid while no flag that was set goes
  * back to 0: a caller that clears flags (a restart) calls compact(plan, 0,
- * NULL) first, which returns to the full lists.  A graph captured before a
- * compaction still launches the full lists (capture again to use the live
- * ones).  Refused for fixed-order plans.  set_skip and retune drop the live
+ * NULL) first, which returns to the full lists.  The live lists are
+ * rewritten in place by every compaction: a graph captured while the plan
+ * was compacted must not be replayed after a later compact call (capture
+ * again); a graph captured with the full lists stays valid.  Refused for
+ * fixed-order plans.  set_skip and retune drop the live
  * lists.  live_tasks reports the task counts the launches use now (strided-
  * GEMM, gather kernel, initialisation, symmetrisation, constant-slot
  * gathers, other gathers; up to ncounts of them).  No counterpart in the
