@@ -53,6 +53,18 @@ def test_error_path_without_device_work():
         p.add(blk)
 
 
+def test_live_task_lists_refused_before_finalize():
+    """gecco_hip_plan_compact needs a finalized plan (host-side check, no
+    device work); live_tasks reports the launches' task counts."""
+    import gecco_hip
+    import pytest
+    p = gecco_hip.Plan()
+    for on in (True, False):
+        with pytest.raises(gecco_hip.GeccoHipError, match='not finalized'):
+            p.compact(on)
+    assert p.live_tasks() == [0] * 6
+
+
 def test_model_strings_match_oracle_bit_exact():
     """Product string graphs / maps / flip maps == oracle restatement."""
     import gecco_hip
