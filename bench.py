@@ -61,7 +61,7 @@ METRIC = 'CCSD iters/s & contraction TFLOP/s vs plain FP64 GPU at |dE|<=1e-8 Eh'
 # WRITE_SIZE passes scaled by a plain-copy calibration (summaries in
 # profiles/); for the CCSD iteration it is the average over the three
 # formula-level launches of the strided-GEMM kernel
-MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 396.5e6,
+MEASURED_TRAFFIC = {('ladder', 20, 100, 4): 396.5e6, ('ladder', 32, 96, 1): 5.665e9,
                     ('ccsd', 20, 100, 300, 4, 'grouped_contract_affine<64, 64, 16, double, 1, 2, false>'): 658.6e6}
 GOLDEN = os.path.join(ROOT, 'tests', 'golden', 'ccsd_o20_v100_s4.json')
 
