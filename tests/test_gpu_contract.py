@@ -165,6 +165,33 @@ def test_instance_groups_skip_flags():
     assert _relerr(bufs[2].cpu().numpy(), refs[0]) <= TOL
 
 
+def test_dense_ladder_blocks_match_oracle():
+    """The pp ladder at C1 symmetry, nocc 16, nvir 48: dense blocks of
+    2304 x 2304 x 256 (the shape class of the dense-block measurement, at a
+    size the oracle's route 3 finishes in seconds), many tiles and long
+    contraction ranges per task, against the oracle."""
+    torch = _torch()
+    import gecco_hip
+    print('dense ladder nsym=1 nocc=16 nvir=48', flush=True)
+    orbs = orbitals(16, 48, 1)
+    c, a, b, r = CASES['ladder'](orbs)
+    x1 = spin_symmetric(a, 31)
+    x2 = spin_symmetric(b, 32)
+    blocks = c.blocks(True)
+    ref = _oracle(c, x1, x2, r.length, blocks)
+    dev = torch.device('cuda:0')
+    bufs = [torch.from_numpy(x1).to(dev), torch.from_numpy(x2).to(dev),
+            torch.zeros(r.length, dtype=torch.float64, device=dev)]
+    plan = gecco_hip.Plan()
+    for bl in blocks:
+        plan.add(bl)
+    plan.finalize()
+    plan.execute(bufs)
+    torch.cuda.synchronize()
+    assert np.abs(ref).max() > 1e-3
+    assert _relerr(bufs[2].cpu().numpy(), ref) <= TOL
+
+
 def test_live_task_lists_match_full_lists():
     """Plan.compact: live-task lists of the groups whose skip flag is 0 give
     the results of the full lists (the oracle's for live groups; flagged
